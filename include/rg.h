@@ -1,0 +1,142 @@
+/*
+ * rg.h — C ABI of the MI355X-native RayGen BSDF path-tracing hot path.
+ *
+ * Plain C, no C++ types, no torch types.  Status codes are int (0 = ok,
+ * negative = RG_E_*); the text of the last error on the calling thread is
+ * returned by rg_last_error().  Caller owns host buffers; an rg_ctx owns its
+ * device memory and HIP stream.  One driving host thread per rg_ctx.
+ *
+ * Each entry point names the reference interface it replaces
+ * (jingwood/raygen-renderer, src/raygen/...).
+ */
+#ifndef RG_H
+#define RG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define RG_OK 0
+#define RG_E_INVALID -1     /* bad argument */
+#define RG_E_NODEVICE -2    /* no HIP device (or a host-only context) */
+#define RG_E_HIP -3         /* HIP runtime error */
+#define RG_E_NOMEM -4       /* host or device allocation failed */
+#define RG_E_CANCELLED -5   /* rg_cancel() observed */
+#define RG_E_UNSUPPORTED -6 /* option outside the built path */
+#define RG_E_BUDGET -7      /* request exceeds the context's memory budget */
+#define RG_E_IO -8          /* file could not be read or parsed */
+
+typedef struct rg_ctx rg_ctx;
+typedef struct rg_scene rg_scene;
+
+/* Ray, 32 B: origin, tmin, direction (need not be unit), tmax.
+ * Replaces ugm::Ray + the initial RayTriangleIntersectionInfo::t
+ * (raycommon.h:168-180; the reference's default tmax is 1e6). */
+typedef struct {
+    float o[3];
+    float tmin;
+    float d[3];
+    float tmax;
+} rg_ray;
+
+/* Closest hit, 16 B: prim = index of the triangle in the order it was handed
+ * to rg_scene_create (0xFFFFFFFF on miss); t; barycentrics u, v of
+ * RenderMeshTriangle::intersectsRay (raycommon.cpp:200-235). */
+typedef struct {
+    uint32_t prim;
+    float t;
+    float u;
+    float v;
+} rg_hit;
+
+/* Build parameters of TriangleBVH::build (bvh.cpp:22-25): bins = 16,
+ * leaf = 4; width = 2 keeps the binary layout, 4 collapses to 4-wide nodes
+ * (the traversal order and results are identical either way). */
+typedef struct {
+    uint32_t bins;
+    uint32_t leaf;
+    uint32_t width;
+} rg_bvh_params;
+
+/* 32-byte binary node, layout of BVHNode (bvh.h:22-30); used by
+ * rg_bvh_export2 so callers can inspect the tree. */
+typedef struct {
+    float bmin[3];
+    uint32_t first_or_left;
+    float bmax[3];
+    uint16_t count;
+    uint16_t axis;
+} rg_bvh2_node;
+
+/* Occlusion predicates of the reference's intersectAny call sites. */
+#define RG_PRED_ALL 0  /* every triangle occludes (microbenchmark)          */
+#define RG_PRED_AREA 1 /* transparency<0.01 && emission<=0 (rayrenderer.cpp:1956-1959) */
+#define RG_PRED_ENV 2  /* !(emission>0) && (transparency<0.01 || refraction>0.1) (:1499-1503) */
+
+/* ---- context ------------------------------------------------------------ */
+/* device >= 0: HIP device ordinal.  device == -1: host-only context (scene
+ * building and BVH export work, every device call returns RG_E_NODEVICE). */
+int rg_create(int device, rg_ctx **out);
+void rg_destroy(rg_ctx *ctx);
+const char *rg_last_error(void);
+/* Launch subsequent work on `hip_stream` (a hipStream_t; NULL = the
+ * context's own stream). */
+int rg_set_stream(rg_ctx *ctx, void *hip_stream);
+int rg_synchronize(rg_ctx *ctx);
+/* Cooperative cancel (RayRenderer::cancelRequested, rayrenderer.h:427):
+ * async-safe; observed between launches. */
+void rg_cancel(rg_ctx *ctx);
+
+/* ---- scene geometry ----------------------------------------------------- */
+/* Replaces RayRenderer::transformScene's triangle list (rayrenderer.cpp:157-363):
+ * v9 = n_tris x (v1, v2, v3) in render (camera) space.  The data is copied. */
+int rg_scene_create(rg_ctx *ctx, const float *v9, uint32_t n_tris, rg_scene **out);
+void rg_scene_destroy(rg_scene *scene);
+/* Replaces TriangleBVH::build (bvh.cpp:76-231): host binned-SAH build, then
+ * (device contexts) the 4-wide collapse and upload to HBM.  params may be
+ * NULL (16 bins, leaf 4, width 4). */
+int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params);
+/* Copies the binary tree out: nodes (capacity cap) and the primitive
+ * permutation (n_tris entries, nullable).  *n_nodes receives the count. */
+int rg_bvh_export2(const rg_scene *scene, rg_bvh2_node *nodes, uint64_t cap,
+                   uint32_t *perm, uint64_t *n_nodes);
+/* Device-side footprint of the scene (triangles + nodes), in bytes. */
+int rg_scene_device_bytes(const rg_scene *scene, uint64_t *bytes);
+/* Bounding box of the geometry: lo xyz, hi xyz. */
+int rg_scene_bounds(const rg_scene *scene, float box[6]);
+
+/* ---- traversal ---------------------------------------------------------- */
+/* Replaces TriangleBVH::intersectClosest (bvh.cpp:233-285) for a batch.
+ * d_rays/d_hits are DEVICE pointers (HBM-resident batch). */
+int rg_trace_closest(rg_ctx *ctx, const rg_scene *scene, const rg_ray *d_rays,
+                     rg_hit *d_hits, uint64_t n);
+/* Replaces TriangleBVH::intersectAny<Pred> (bvh.h:63-101) with maxT =
+ * ray.tmax; pred is one of RG_PRED_*.  d_occluded: one byte per ray. */
+int rg_trace_any(rg_ctx *ctx, const rg_scene *scene, const rg_ray *d_rays,
+                 uint8_t *d_occluded, uint64_t n, int pred);
+/* Host-buffer conveniences (copy in, trace, copy out; synchronous). */
+int rg_trace_closest_host(rg_ctx *ctx, const rg_scene *scene, const rg_ray *rays,
+                          rg_hit *hits, uint64_t n);
+int rg_trace_any_host(rg_ctx *ctx, const rg_scene *scene, const rg_ray *rays,
+                      uint8_t *occluded, uint64_t n, int pred);
+
+/* ---- workload generators (traversal microbenchmark) --------------------- */
+/* Level-`level` subdivided icosahedron (20*4^level triangles) on the unit
+ * sphere, radially displaced by 1 + amp*(2*fbm(p)-1) with 3-octave value
+ * noise (medium.cpp:125-185 basis) keyed by `seed`.  out_v9 holds
+ * 20*4^level*9 floats. */
+int rg_gen_icosphere(uint32_t level, float amp, uint32_t seed, float *out_v9);
+/* Incoherent rays on the device: origins uniform in scale x box (centred),
+ * directions uniform on the unit sphere, Philox4x32-10 keyed by `key`,
+ * counter = ray index + first.  tmin = 0, tmax = +inf. */
+int rg_gen_rays_sphere(rg_ctx *ctx, const float box[6], float scale, uint32_t key,
+                       uint64_t first, rg_ray *d_rays, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,255 @@
+// rg_api.cpp — the extern "C" surface declared in include/rg.h.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "rg_internal.h"
+
+namespace rg {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+}  // namespace rg
+
+using namespace rg;
+
+namespace {
+
+int need_device(const Ctx *c) {
+    if (!c) return set_error(RG_E_INVALID, "null context");
+    if (c->device < 0) return set_error(RG_E_NODEVICE, "host-only context has no device");
+    return RG_OK;
+}
+
+void free_device(Scene *s) {
+    dev_free(s->d_nodes4);
+    dev_free(s->d_nodes2);
+    dev_free(s->d_tris);
+    s->d_nodes4 = s->d_nodes2 = s->d_tris = nullptr;
+    s->device_bytes = 0;
+}
+
+// stream helpers live in rg_trace.hip; declared here to keep rg_internal.h
+// free of HIP types
+}  // namespace
+
+namespace rg {
+int dev_stream_create(void **s);
+void dev_stream_destroy(void *s);
+}  // namespace rg
+
+extern "C" {
+
+const char *rg_last_error(void) { return g_last_error.c_str(); }
+
+int rg_create(int device, rg_ctx **out) {
+    if (!out) return set_error(RG_E_INVALID, "out is null");
+    *out = nullptr;
+    Ctx *c = new (std::nothrow) Ctx();
+    if (!c) return set_error(RG_E_NOMEM, "context allocation failed");
+    c->device = device;
+    if (device >= 0) {
+        int rc = dev_set_device(device);
+        if (rc) { delete c; return set_error(RG_E_NODEVICE, "no HIP device " + std::to_string(device) + ": " + g_last_error); }
+        rc = dev_stream_create(&c->own_stream);
+        if (rc) { delete c; return rc; }
+        c->stream = c->own_stream;
+    }
+    *out = reinterpret_cast<rg_ctx *>(c);
+    return RG_OK;
+}
+
+void rg_destroy(rg_ctx *ctx) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return;
+    if (c->own_stream) dev_stream_destroy(c->own_stream);
+    delete c;
+}
+
+int rg_set_stream(rg_ctx *ctx, void *hip_stream) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = need_device(c)) return rc;
+    c->stream = hip_stream ? hip_stream : c->own_stream;
+    return RG_OK;
+}
+
+int rg_synchronize(rg_ctx *ctx) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = need_device(c)) return rc;
+    return dev_sync(c->stream);
+}
+
+void rg_cancel(rg_ctx *ctx) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (c) c->cancel.store(true);
+}
+
+int rg_scene_create(rg_ctx *ctx, const float *v9, uint32_t n_tris, rg_scene **out) {
+    if (!ctx || !out) return set_error(RG_E_INVALID, "null argument");
+    if (n_tris > 0 && !v9) return set_error(RG_E_INVALID, "v9 is null");
+    if (n_tris >= 0x7FFFFFFFu) return set_error(RG_E_INVALID, "too many triangles");
+    *out = nullptr;
+    Scene *s = new (std::nothrow) Scene();
+    if (!s) return set_error(RG_E_NOMEM, "scene allocation failed");
+    s->ctx = reinterpret_cast<Ctx *>(ctx);
+    s->n_tris = n_tris;
+    try {
+        s->v9.assign(v9, v9 + 9 * (size_t)n_tris);
+    } catch (...) {
+        delete s;
+        return set_error(RG_E_NOMEM, "geometry copy failed");
+    }
+    for (int k = 0; k < 3; k++) { s->box[k] = INFINITY; s->box[3 + k] = -INFINITY; }
+    for (size_t i = 0; i < (size_t)n_tris * 3; i++)
+        for (int k = 0; k < 3; k++) {
+            const float x = s->v9[3 * i + k];
+            s->box[k] = fminf(s->box[k], x);
+            s->box[3 + k] = fmaxf(s->box[3 + k], x);
+        }
+    *out = reinterpret_cast<rg_scene *>(s);
+    return RG_OK;
+}
+
+void rg_scene_destroy(rg_scene *scene) {
+    Scene *s = reinterpret_cast<Scene *>(scene);
+    if (!s) return;
+    if (s->ctx && s->ctx->device >= 0) {
+        dev_set_device(s->ctx->device);
+        free_device(s);
+    }
+    delete s;
+}
+
+int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    Scene *s = reinterpret_cast<Scene *>(scene);
+    if (!c || !s) return set_error(RG_E_INVALID, "null argument");
+    rg_bvh_params p = {16, 4, 4};
+    if (params) p = *params;
+    if (p.bins < 2 || p.bins > 256) return set_error(RG_E_INVALID, "bins must be in [2, 256]");
+    if (p.leaf < 1 || p.leaf > 0xFFFF) return set_error(RG_E_INVALID, "leaf must be in [1, 65535]");
+    if (p.width != 2 && p.width != 4) return set_error(RG_E_UNSUPPORTED, "width must be 2 or 4");
+    try {
+        build_bvh2(s->v9.data(), s->n_tris, p.bins, p.leaf, s->nodes2, s->perm);
+        collapse_bvh4(s->nodes2, s->nodes4);
+    } catch (const std::bad_alloc &) {
+        return set_error(RG_E_NOMEM, "BVH build ran out of host memory");
+    }
+    s->width = (int)p.width;
+    s->stack_need = stack_need4(s->nodes4);
+    s->built = true;
+    if (c->device < 0) return RG_OK;
+
+    if (int rc = dev_set_device(c->device)) return rc;
+    free_device(s);
+    std::vector<Tri48> tris;
+    make_tris(s->v9.data(), s->nodes2, s->perm, s->pred_mask, tris);
+    const size_t b4 = s->nodes4.size() * sizeof(Node4), bt = tris.size() * sizeof(Tri48);
+    if (int rc = dev_malloc(&s->d_nodes4, b4)) return rc;
+    if (int rc = dev_malloc(&s->d_tris, bt)) { free_device(s); return rc; }
+    if (int rc = dev_upload(s->d_nodes4, s->nodes4.data(), b4, c->stream)) { free_device(s); return rc; }
+    if (int rc = dev_upload(s->d_tris, tris.data(), bt, c->stream)) { free_device(s); return rc; }
+    s->device_bytes = b4 + bt;
+    return RG_OK;
+}
+
+int rg_bvh_export2(const rg_scene *scene, rg_bvh2_node *nodes, uint64_t cap, uint32_t *perm,
+                   uint64_t *n_nodes) {
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (!s || !n_nodes) return set_error(RG_E_INVALID, "null argument");
+    if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
+    *n_nodes = s->nodes2.size();
+    if (nodes) {
+        if (cap < s->nodes2.size()) return set_error(RG_E_INVALID, "node capacity too small");
+        std::memcpy(nodes, s->nodes2.data(), s->nodes2.size() * sizeof(Node2));
+    }
+    if (perm) std::memcpy(perm, s->perm.data(), s->perm.size() * sizeof(uint32_t));
+    return RG_OK;
+}
+
+int rg_scene_device_bytes(const rg_scene *scene, uint64_t *bytes) {
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (!s || !bytes) return set_error(RG_E_INVALID, "null argument");
+    *bytes = s->device_bytes;
+    return RG_OK;
+}
+
+int rg_scene_bounds(const rg_scene *scene, float box[6]) {
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (!s || !box) return set_error(RG_E_INVALID, "null argument");
+    std::memcpy(box, s->box, sizeof s->box);
+    return RG_OK;
+}
+
+int rg_trace_closest(rg_ctx *ctx, const rg_scene *scene, const rg_ray *d_rays, rg_hit *d_hits,
+                     uint64_t n) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (int rc = need_device(c)) return rc;
+    if (!s || (n && (!d_rays || !d_hits))) return set_error(RG_E_INVALID, "null argument");
+    if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
+    if (c->cancel.load()) return set_error(RG_E_CANCELLED, "cancelled");
+    return launch_closest(*s, d_rays, d_hits, n, c->stream);
+}
+
+int rg_trace_any(rg_ctx *ctx, const rg_scene *scene, const rg_ray *d_rays, uint8_t *d_occluded,
+                 uint64_t n, int pred) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (int rc = need_device(c)) return rc;
+    if (!s || (n && (!d_rays || !d_occluded))) return set_error(RG_E_INVALID, "null argument");
+    if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
+    if (pred < RG_PRED_ALL || pred > RG_PRED_ENV) return set_error(RG_E_INVALID, "unknown predicate");
+    if (c->cancel.load()) return set_error(RG_E_CANCELLED, "cancelled");
+    return launch_any(*s, d_rays, d_occluded, n, pred, c->stream);
+}
+
+int rg_trace_closest_host(rg_ctx *ctx, const rg_scene *scene, const rg_ray *rays, rg_hit *hits,
+                          uint64_t n) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = need_device(c)) return rc;
+    if (n == 0) return RG_OK;
+    if (!rays || !hits) return set_error(RG_E_INVALID, "null argument");
+    void *dr = nullptr, *dh = nullptr;
+    int rc = dev_malloc(&dr, n * sizeof(rg_ray));
+    if (!rc) rc = dev_malloc(&dh, n * sizeof(rg_hit));
+    if (!rc) rc = dev_upload(dr, rays, n * sizeof(rg_ray), c->stream);
+    if (!rc) rc = rg_trace_closest(ctx, scene, (const rg_ray *)dr, (rg_hit *)dh, n);
+    if (!rc) rc = dev_download(hits, dh, n * sizeof(rg_hit), c->stream);
+    dev_free(dr);
+    dev_free(dh);
+    return rc;
+}
+
+int rg_trace_any_host(rg_ctx *ctx, const rg_scene *scene, const rg_ray *rays, uint8_t *occluded,
+                      uint64_t n, int pred) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = need_device(c)) return rc;
+    if (n == 0) return RG_OK;
+    if (!rays || !occluded) return set_error(RG_E_INVALID, "null argument");
+    void *dr = nullptr, *dh = nullptr;
+    int rc = dev_malloc(&dr, n * sizeof(rg_ray));
+    if (!rc) rc = dev_malloc(&dh, n);
+    if (!rc) rc = dev_upload(dr, rays, n * sizeof(rg_ray), c->stream);
+    if (!rc) rc = rg_trace_any(ctx, scene, (const rg_ray *)dr, (uint8_t *)dh, n, pred);
+    if (!rc) rc = dev_download(occluded, dh, n, c->stream);
+    dev_free(dr);
+    dev_free(dh);
+    return rc;
+}
+
+int rg_gen_rays_sphere(rg_ctx *ctx, const float box[6], float scale, uint32_t key, uint64_t first,
+                       rg_ray *d_rays, uint64_t n) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (int rc = need_device(c)) return rc;
+    if (!box || (n && !d_rays)) return set_error(RG_E_INVALID, "null argument");
+    return launch_gen_rays(box, scale, key, first, d_rays, n, c->stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// rg_internal.h — shared host-side definitions of the rg library.
+#pragma once
+
+#include <atomic>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rg.h"
+
+namespace rg {
+
+// ---------------------------------------------------------------------------
+// Device layouts
+// ---------------------------------------------------------------------------
+
+// 4-wide node, 128 B (one L2 line on gfx950).  Child boxes are stored SoA so
+// one node fetch is 8 x 16 B vector loads.  child[i]:
+//   bit 31 clear : index of a child Node4
+//   bit 31 set   : leaf; bits 0..30 = first triangle in the permuted triangle
+//                  array (the leaf runs until a triangle flagged "last")
+//   0xFFFFFFFF   : empty slot
+// order[oct] lists the 4 slots (2 bits each, first visited in bits 0-1) in the
+// depth-first order the binary tree would visit them for a ray whose
+// direction-sign octant is oct (bit k set when 1/d_k < 0).
+struct alignas(128) Node4 {
+    float lox[4], loy[4], loz[4];
+    float hix[4], hiy[4], hiz[4];
+    uint32_t child[4];
+    uint8_t order[8];
+    uint32_t pad[2];
+};
+static_assert(sizeof(Node4) == 128, "Node4 must be 128 bytes");
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+// Triangle record, 48 B = 3 x float4, in BVH (permuted) order:
+//   a = (v1.xyz, bits(original index)), b = (v2-v1, bits(flags)), c = (v3-v1, 0)
+// flags bit0 = last triangle of its leaf; c.w holds the occluder bits
+// (bit p set when predicate RG_PRED_p counts the triangle as an occluder).  Edges are the exact binary32
+// differences RenderMeshTriangle::intersectsRay forms (raycommon.cpp:410-411).
+struct alignas(16) Tri48 {
+    float a[4];
+    float b[4];
+    float c[4];
+};
+static_assert(sizeof(Tri48) == 48, "Tri48 must be 48 bytes");
+
+// Binary node, identical to BVHNode (bvh.h:22-30).
+struct Node2 {
+    float bmin[3];
+    uint32_t first_or_left;
+    float bmax[3];
+    uint16_t count;
+    uint16_t axis;
+};
+static_assert(sizeof(Node2) == 32, "Node2 must be 32 bytes");
+
+// ---------------------------------------------------------------------------
+// Host objects
+// ---------------------------------------------------------------------------
+struct Ctx {
+    int device = -1;
+    void *stream = nullptr;        // hipStream_t in use
+    void *own_stream = nullptr;    // stream created by rg_create
+    std::atomic<bool> cancel{false};
+};
+
+struct Scene {
+    Ctx *ctx = nullptr;
+    uint32_t n_tris = 0;
+    std::vector<float> v9;            // as handed in
+    float box[6] = {0, 0, 0, 0, 0, 0};
+    // binary tree (reference layout) + permutation
+    std::vector<Node2> nodes2;
+    std::vector<uint32_t> perm;
+    // wide tree
+    std::vector<Node4> nodes4;
+    bool root_is_leaf = false;
+    uint32_t root_leaf_first = 0;
+    int width = 4;
+    uint32_t stack_need = 0;       // deepest traversal stack the wide tree can need
+    bool built = false;
+    // device copies
+    void *d_nodes4 = nullptr;
+    void *d_nodes2 = nullptr;
+    void *d_tris = nullptr;
+    void *d_tri_flags = nullptr;  // per-triangle occluder mask bits (predicates)
+    uint64_t device_bytes = 0;
+    // per-triangle predicate mask: bit p set when predicate p treats it as occluder
+    std::vector<uint8_t> pred_mask;
+};
+
+// error reporting (thread-local text)
+int set_error(int code, const std::string &msg);
+
+// rg_bvh.cpp
+void build_bvh2(const float *v9, uint32_t n, uint32_t bins, uint32_t leaf,
+                std::vector<Node2> &nodes, std::vector<uint32_t> &perm);
+void collapse_bvh4(const std::vector<Node2> &nodes2, std::vector<Node4> &out);
+void make_tris(const float *v9, const std::vector<Node2> &nodes2,
+               const std::vector<uint32_t> &perm, const std::vector<uint8_t> &pred_mask,
+               std::vector<Tri48> &out);
+uint32_t stack_need4(const std::vector<Node4> &nodes);
+
+// rg_trace.hip (device side)
+int dev_malloc(void **p, size_t bytes);
+void dev_free(void *p);
+int dev_upload(void *dst, const void *src, size_t bytes, void *stream);
+int dev_download(void *dst, const void *src, size_t bytes, void *stream);
+int dev_sync(void *stream);
+int dev_set_device(int device);
+int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n, void *stream);
+int launch_any(const Scene &s, const rg_ray *rays, uint8_t *occ, uint64_t n, int pred, void *stream);
+int launch_gen_rays(const float box[6], float scale, uint32_t key, uint64_t first,
+                    rg_ray *rays, uint64_t n, void *stream);
+
+}  // namespace rg

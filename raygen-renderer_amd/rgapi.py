@@ -1,0 +1,201 @@
+"""ctypes binding of librg.so (the C ABI declared in include/rg.h).
+
+The library is loaded from this directory; a missing or unloadable library
+raises RGError — there is no fallback path.  Device buffers are passed as
+integer addresses (e.g. ``torch.Tensor.data_ptr()``); host buffers as numpy
+arrays.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librg.so")
+
+RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("tmin", "<f4"), ("d", "<f4", (3,)), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("prim", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+NODE2_DTYPE = np.dtype([("bmin", "<f4", (3,)), ("first_or_left", "<u4"), ("bmax", "<f4", (3,)),
+                        ("count", "<u2"), ("axis", "<u2")])
+MISS = 0xFFFFFFFF
+
+PRED_ALL, PRED_AREA, PRED_ENV = 0, 1, 2
+
+
+class RGError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rg error {code}: {msg}")
+        self.code = code
+
+
+class BvhParams(ctypes.Structure):
+    _fields_ = [("bins", ctypes.c_uint32), ("leaf", ctypes.c_uint32), ("width", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load librg.so once; raise RGError when it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RGError(-1, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, u64, i32, fp = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        sig = {
+            "rg_create": (i32, [i32, pp]),
+            "rg_destroy": (None, [vp]),
+            "rg_last_error": (ctypes.c_char_p, []),
+            "rg_set_stream": (i32, [vp, vp]),
+            "rg_synchronize": (i32, [vp]),
+            "rg_cancel": (None, [vp]),
+            "rg_scene_create": (i32, [vp, vp, u32, pp]),
+            "rg_scene_destroy": (None, [vp]),
+            "rg_bvh_build": (i32, [vp, vp, ctypes.POINTER(BvhParams)]),
+            "rg_bvh_export2": (i32, [vp, vp, u64, vp, ctypes.POINTER(u64)]),
+            "rg_scene_device_bytes": (i32, [vp, ctypes.POINTER(u64)]),
+            "rg_scene_bounds": (i32, [vp, vp]),
+            "rg_trace_closest": (i32, [vp, vp, vp, vp, u64]),
+            "rg_trace_any": (i32, [vp, vp, vp, vp, u64, i32]),
+            "rg_trace_closest_host": (i32, [vp, vp, vp, vp, u64]),
+            "rg_trace_any_host": (i32, [vp, vp, vp, vp, u64, i32]),
+            "rg_gen_icosphere": (i32, [u32, fp, u32, vp]),
+            "rg_gen_rays_sphere": (i32, [vp, vp, fp, u32, u64, vp, u64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise RGError(rc, lib().rg_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """rg_ctx: device >= 0 for a HIP device, -1 for a host-only context."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().rg_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def set_stream(self, stream_ptr):
+        check(lib().rg_set_stream(self.h, ctypes.c_void_p(stream_ptr or None)))
+
+    def synchronize(self):
+        check(lib().rg_synchronize(self.h))
+
+    def cancel(self):
+        lib().rg_cancel(self.h)
+
+    def close(self):
+        if self.h:
+            lib().rg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def gen_rays_sphere(self, box, scale, key, first, d_rays, n):
+        b = np.ascontiguousarray(box, dtype=np.float32)
+        check(lib().rg_gen_rays_sphere(self.h, _ptr(b), scale, key, first, ctypes.c_void_p(d_rays), n))
+
+
+class Scene:
+    """rg_scene: triangle soup (n, 9) float32 in render space."""
+
+    def __init__(self, ctx, v9):
+        self.ctx = ctx
+        self.v9 = np.ascontiguousarray(v9, dtype=np.float32).reshape(-1, 9)
+        h = ctypes.c_void_p()
+        check(lib().rg_scene_create(ctx.h, _ptr(self.v9), len(self.v9), ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def n_tris(self):
+        return len(self.v9)
+
+    def build(self, bins=16, leaf=4, width=4):
+        p = BvhParams(bins, leaf, width)
+        check(lib().rg_bvh_build(self.ctx.h, self.h, ctypes.byref(p)))
+
+    def export2(self):
+        n = ctypes.c_uint64()
+        check(lib().rg_bvh_export2(self.h, None, 0, None, ctypes.byref(n)))
+        nodes = np.zeros(n.value, dtype=NODE2_DTYPE)
+        perm = np.zeros(self.n_tris, dtype=np.uint32)
+        check(lib().rg_bvh_export2(self.h, _ptr(nodes), n.value, _ptr(perm), ctypes.byref(n)))
+        return nodes, perm
+
+    def device_bytes(self):
+        b = ctypes.c_uint64()
+        check(lib().rg_scene_device_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
+    def bounds(self):
+        b = np.zeros(6, dtype=np.float32)
+        check(lib().rg_scene_bounds(self.h, _ptr(b)))
+        return b
+
+    def trace_closest(self, d_rays, d_hits, n):
+        check(lib().rg_trace_closest(self.ctx.h, self.h, ctypes.c_void_p(d_rays), ctypes.c_void_p(d_hits), n))
+
+    def trace_any(self, d_rays, d_occ, n, pred=PRED_ALL):
+        check(lib().rg_trace_any(self.ctx.h, self.h, ctypes.c_void_p(d_rays), ctypes.c_void_p(d_occ), n, pred))
+
+    def trace_closest_host(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        check(lib().rg_trace_closest_host(self.ctx.h, self.h, _ptr(rays), _ptr(hits), len(rays)))
+        return hits
+
+    def trace_any_host(self, rays, pred=PRED_ALL):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        occ = np.zeros(len(rays), dtype=np.uint8)
+        check(lib().rg_trace_any_host(self.ctx.h, self.h, _ptr(rays), _ptr(occ), len(rays), pred))
+        return occ
+
+    def close(self):
+        if self.h:
+            lib().rg_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gen_icosphere(level, amp=0.05, seed=0x1C05EED):
+    """Displaced icosphere, (20*4**level, 9) float32."""
+    n = 20 * 4 ** level
+    out = np.zeros((n, 9), dtype=np.float32)
+    check(lib().rg_gen_icosphere(level, amp, seed, _ptr(out)))
+    return out
+
+
+def make_rays(o, d, tmin=0.0, tmax=np.inf):
+    """Pack origins/directions (n,3) into the 32-byte ray records."""
+    o = np.asarray(o, dtype=np.float32).reshape(-1, 3)
+    d = np.asarray(d, dtype=np.float32).reshape(-1, 3)
+    r = np.zeros(len(o), dtype=RAY_DTYPE)
+    r["o"] = o
+    r["d"] = d
+    r["tmin"] = tmin
+    r["tmax"] = tmax
+    return r
