@@ -39,6 +39,13 @@ def lib():
     """Load librg.so once; raise RGError when it is not built."""
     global _lib
     if _lib is None:
+        # torch ships its own libamdhip64 (same SONAME); when torch is present
+        # it is loaded first so that librg binds to that runtime and the
+        # process holds a single HIP runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RGError(-1, f"{LIB_PATH} is not built (run __graft_entry__.build())")
         L = ctypes.CDLL(LIB_PATH)

@@ -168,6 +168,9 @@ def test_full_size_microbenchmark_subset(ctx):
     assert 0.3 < hit.mean() < 0.7
     assert (allh["prim"][hit] < len(v9)).all() and (allh["t"][hit] >= 0).all()
     assert ((allh["u"][hit] >= 0) & (allh["v"][hit] >= 0) & (allh["u"][hit] + allh["v"][hit] <= 1.0000001)).all()
-    # any-hit and closest-hit use different determinant / t thresholds
-    # (raycommon.cpp:151-235) but must agree on all but a sliver of rays
-    assert (d_occ.cpu().numpy().astype(bool) == hit).mean() > 0.9999
+    # any-hit rejects |det| < 1e-6 where closest-hit rejects |det| < 1e-8
+    # (raycommon.cpp:151-235); on these ~3e-3-edge triangles that drops some
+    # grazing hits, so the two agree on all but a small fraction of rays
+    agree = d_occ.cpu().numpy().astype(bool) == hit
+    say(f"[trace] any/closest agreement {agree.mean():.6f}")
+    assert agree.mean() > 0.999
