@@ -136,6 +136,84 @@ int rg_gen_icosphere(uint32_t level, float amp, uint32_t seed, float *out_v9);
 int rg_gen_rays_sphere(rg_ctx *ctx, const float box[6], float scale, uint32_t key,
                        uint64_t first, rg_ray *d_rays, uint64_t n);
 
+/* ---- path tracing, shading system 5 (-d 5) ------------------------------ */
+/* Material of one scene object: the Material fields (material.h:20-70) that
+ * RayBSDFShaderProvider reads.  Defaults (rg_object_default) follow
+ * material.h: color 0.8, roughness 0.5, refraction_ratio 1.45, rest 0. */
+typedef struct {
+    float color[3];
+    float glossy, roughness, transparency, refraction, refraction_ratio;
+    float metallic, anisotropy, aniso_rotation, emission;
+    int32_t interior_medium; /* index into the scene's media, -1 none */
+    uint32_t visible;
+} rg_object;
+
+/* HomogeneousMedium authoring fields (medium.h:37-130), constant emission. */
+typedef struct {
+    float sigma_a[3], sigma_s[3], sigma_e[3];
+    float g, density, pad;
+} rg_medium;
+
+/* Camera (scene.h Camera): render space has the eye at the origin looking
+ * down -Z, as after RayRenderer's view transform (rayrenderer.cpp:247). */
+typedef struct {
+    float fov_deg, exposure, depth_of_field, aperture;
+    int32_t aperture_blades;
+    float aperture_rotation_deg, view_far, pad;
+} rg_camera;
+
+/* Equirectangular environment map: w x h linear RGB floats, row 0 = top
+ * (Scene::envmap with envmapIntensity / envmapRotation). */
+typedef struct {
+    uint32_t w, h;
+    const float *rgb;
+    float intensity, rotation_deg;
+} rg_env;
+
+/* RendererSettings (rayrenderer.h:98-153) subset of the -d 5 path plus the
+ * seeding this build adds (the reference seeds from time()). */
+typedef struct {
+    uint32_t width, height;   /* -r WxH */
+    uint32_t spp;             /* -s N */
+    uint32_t max_depth;       /* MAX_TRACE_DEPTH (reference constant 32) */
+    int32_t antialias;        /* -enaa */
+    float firefly_clamp;      /* fireflyClamp, 0 disables */
+    uint32_t seed, job;       /* Philox key of the path's random draws */
+    int32_t adaptive;         /* enableAdaptiveSampling */
+    uint32_t adaptive_base;   /* adaptiveBaseSamples */
+    float adaptive_threshold; /* adaptiveThreshold (mean tile rSEM) */
+    int32_t denoise;          /* enableDenoise (A-Trous) */
+    uint32_t denoise_levels;
+    float sigma_color, sigma_normal, sigma_depth, denoise_intensity;
+    uint32_t samples_in_flight; /* paths per launch wave = pixels x this; 0 = auto */
+} rg_settings;
+
+void rg_object_default(rg_object *obj);
+void rg_settings_default(rg_settings *s);
+
+/* Per-triangle shading inputs of RayRenderer::transformObject
+ * (rayrenderer.cpp:237-363): n9 = vertex normals in render space,
+ * obj_of_tri = owning object index; objs = one material per object.  An
+ * object with emission > 0 is an area light made of its triangles. */
+int rg_scene_set_shading(rg_scene *scene, const float *n9, const uint32_t *obj_of_tri,
+                         const rg_object *objs, uint32_t n_obj);
+/* Media referenced by rg_object.interior_medium; global_medium = the
+ * scene's globalMedium (-1 none). */
+int rg_scene_set_media(rg_scene *scene, const rg_medium *media, uint32_t n, int32_t global_medium);
+/* env NULL or w == 0 removes the environment. */
+int rg_scene_set_env(rg_scene *scene, const rg_env *env);
+int rg_scene_set_camera(rg_scene *scene, const rg_camera *cam);
+
+/* Replaces RayRenderer::render() (rayrenderer.cpp:478-596) for shader 5:
+ * samples every pixel (adaptive per 32x32 tile when enabled), optionally
+ * denoises, and writes the linear HDR image (mean * exposure, >= 0; RGBA
+ * float, W*H*4) to hdr_rgba.  sum_rgb / sumsq_rgb (nullable, W*H*3) receive
+ * the per-pixel sums of the clamped sample radiance and of its square;
+ * nspp (nullable, one per 32x32 tile, row-major) the samples taken.  All
+ * output pointers are host memory.  Blocks until done. */
+int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *hdr_rgba,
+              float *sum_rgb, float *sumsq_rgb, uint32_t *nspp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -164,3 +164,136 @@ def gen_rays_sphere(box, scale, key, first, n):
         rays[i]["d"] = (r * np.cos(phi), r * np.sin(phi), z)
         rays[i]["tmax"] = np.inf
     return rays
+
+
+# --------------------------------------------------------------------------
+# path tracer
+# --------------------------------------------------------------------------
+OBJECT_DTYPE = np.dtype([("color", "<f4", (3,)), ("glossy", "<f4"), ("roughness", "<f4"),
+                         ("transparency", "<f4"), ("refraction", "<f4"), ("refraction_ratio", "<f4"),
+                         ("metallic", "<f4"), ("anisotropy", "<f4"), ("aniso_rotation", "<f4"),
+                         ("emission", "<f4"), ("interior_medium", "<i4"), ("visible", "<u4")])
+MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", (3,)), ("sigma_s", "<f4", (3,)), ("sigma_e", "<f4", (3,)),
+                         ("g", "<f4"), ("density", "<f4"), ("pad", "<f4")])
+
+
+class _Cam(ctypes.Structure):
+    _fields_ = [("fov_deg", ctypes.c_float), ("exposure", ctypes.c_float), ("depth_of_field", ctypes.c_float),
+                ("aperture", ctypes.c_float), ("aperture_blades", ctypes.c_int32),
+                ("aperture_rotation_deg", ctypes.c_float), ("view_far", ctypes.c_float), ("pad", ctypes.c_float)]
+
+
+class _Env(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("rgb", ctypes.c_void_p),
+                ("intensity", ctypes.c_float), ("rotation_deg", ctypes.c_float)]
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("v9", ctypes.c_void_p), ("n9", ctypes.c_void_p), ("obj_of_tri", ctypes.c_void_p),
+                ("ntris", ctypes.c_uint32), ("objs", ctypes.c_void_p), ("nobj", ctypes.c_uint32),
+                ("media", ctypes.c_void_p), ("nmedia", ctypes.c_uint32), ("global_medium", ctypes.c_int32),
+                ("env", _Env), ("cam", _Cam)]
+
+
+class _Settings(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+                ("max_depth", ctypes.c_uint32), ("antialias", ctypes.c_int32), ("firefly_clamp", ctypes.c_float),
+                ("seed", ctypes.c_uint32), ("job", ctypes.c_uint32), ("sample_start", ctypes.c_uint32)]
+
+
+def _desc(scene, keep):
+    arrs = {}
+    for k, dt in (("v9", np.float32), ("n9", np.float32), ("obj", np.uint32)):
+        arrs[k] = np.ascontiguousarray(scene[k], dtype=dt)
+    objs = np.ascontiguousarray(scene["objects"], dtype=OBJECT_DTYPE)
+    media = np.ascontiguousarray(scene.get("media", np.zeros(0, MEDIUM_DTYPE)), dtype=MEDIUM_DTYPE)
+    keep += [arrs, objs, media]
+    d = _Desc()
+    d.v9 = arrs["v9"].ctypes.data
+    d.n9 = arrs["n9"].ctypes.data
+    d.obj_of_tri = arrs["obj"].ctypes.data
+    d.ntris = len(arrs["obj"])
+    d.objs = objs.ctypes.data
+    d.nobj = len(objs)
+    d.media = media.ctypes.data if len(media) else None
+    d.nmedia = len(media)
+    d.global_medium = int(scene.get("global_medium", -1))
+    env = scene.get("env")
+    if env is not None:
+        rgb = np.ascontiguousarray(env["rgb"], dtype=np.float32)
+        keep.append(rgb)
+        d.env = _Env(rgb.shape[1], rgb.shape[0], rgb.ctypes.data, float(env.get("intensity", 1.0)),
+                     float(env.get("rotation_deg", 0.0)))
+    cam = scene["camera"]
+    d.cam = _Cam(cam.get("fov_deg", 75.0), cam.get("exposure", 1.0), cam.get("depth_of_field", 0.0),
+                 cam.get("aperture", 1.8), cam.get("aperture_blades", 0), cam.get("aperture_rotation_deg", 0.0),
+                 cam.get("view_far", 50.0), 0.0)
+    return d
+
+
+def render(scene, width, height, spp, max_depth=5, mode=0, threads=8, antialias=True, firefly_clamp=10.0,
+           seed=0xC0DE0000, job=0, sample_start=0, active_tile=None):
+    """Returns (sum, sumsq), each (H, W, 3) float32, over the given samples."""
+    L = lib()
+    if not hasattr(L, "_render_sig"):
+        L.or_render.restype = ctypes.c_int
+        L.or_render.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_void_p]
+        L.or_render_one.restype = None
+        L.or_render_one.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_void_p]
+        L._render_sig = True
+    keep = []
+    d = _desc(scene, keep)
+    s = _Settings(width, height, spp, max_depth, 1 if antialias else 0, firefly_clamp, seed & 0xFFFFFFFF, job,
+                  sample_start)
+    sm = np.zeros((height, width, 3), np.float32)
+    sq = np.zeros((height, width, 3), np.float32)
+    at = None if active_tile is None else np.ascontiguousarray(active_tile, dtype=np.uint8)
+    L.or_render(ctypes.byref(d), ctypes.byref(s), mode, threads, _p(at), _p(sm), _p(sq))
+    return sm, sq
+
+
+def aov(scene, width, height):
+    """Primary-hit guide buffers: (normal_rgba, albedo_rgba, depth)."""
+    L = lib()
+    L.or_aov.restype = None
+    L.or_aov.argtypes = [ctypes.c_void_p] * 5
+    keep = []
+    d = _desc(scene, keep)
+    s = _Settings(width, height, 1, 1, 1, 10.0, 0, 0, 0)
+    nrm = np.zeros((height, width, 4), np.float32)
+    alb = np.zeros((height, width, 4), np.float32)
+    dep = np.zeros((height, width), np.float32)
+    L.or_aov(ctypes.byref(d), ctypes.byref(s), _p(nrm), _p(alb), _p(dep))
+    return nrm, alb, dep
+
+
+def denoise(noisy_rgba, nrm, alb, dep, levels=5, sigma_c=0.4, sigma_n=128.0, sigma_d=0.1, intensity=1.0):
+    L = lib()
+    L.or_denoise.restype = None
+    f = ctypes.c_float
+    L.or_denoise.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int, f, f, f, f,
+                                                                                      ctypes.c_void_p]
+    h, w = noisy_rgba.shape[:2]
+    out = np.zeros((h, w, 4), np.float32)
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (noisy_rgba, nrm, alb, dep)]
+    L.or_denoise(w, h, *[_p(a) for a in arrs], levels, sigma_c, sigma_n, sigma_d, intensity, _p(out))
+    return out
+
+
+def render_adaptive(scene, width, height, spp, max_depth=5, base=4, threshold=0.02, mode=1, threads=8,
+                    antialias=True, firefly_clamp=10.0, seed=0xC0DE0000, job=0):
+    """renderAdaptive restated: returns (sum, sumsq, nspp per 32x32 tile)."""
+    L = lib()
+    L.or_render_adaptive.restype = ctypes.c_int
+    L.or_render_adaptive.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                     ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    keep = []
+    d = _desc(scene, keep)
+    s = _Settings(width, height, spp, max_depth, 1 if antialias else 0, firefly_clamp, seed & 0xFFFFFFFF, job, 0)
+    sm = np.zeros((height, width, 3), np.float32)
+    sq = np.zeros((height, width, 3), np.float32)
+    nt = np.zeros(((width + 31) // 32) * ((height + 31) // 32), np.uint32)
+    L.or_render_adaptive(ctypes.byref(d), ctypes.byref(s), mode, threads, base, threshold, _p(sm), _p(sq), _p(nt))
+    return sm, sq, nt

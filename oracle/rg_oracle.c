@@ -502,6 +502,45 @@ void or_trace_any(const or_node *nodes, int64_t nnodes, const float *v9,
     run_trav(&j, nthreads);
 }
 
+void or_closest1(const or_node *nodes, int64_t nnodes, const float *v9, const uint32_t *perm,
+                 const or_ray *r, or_hit *h) {
+    closest_one(nodes, nnodes, v9, perm, r, h, NULL, NULL);
+}
+
+/* intersectAny with the reference's shadow predicates (rayrenderer.cpp:1956-1959,
+ * 1499-1503): pred 0 = every triangle, 1 = area-light NEE, 2 = envmap NEE. */
+static int occludes(const or_object *m, uint8_t pred) {
+    if (pred == 1) return m->transparency < 0.01f && m->emission <= 0.0f;
+    if (pred == 2) return !(m->emission > 0.0f) && (m->transparency < 0.01f || m->refraction > 0.1f);
+    return 1;
+}
+
+int or_any1_pred(const or_node *nodes, int64_t nnodes, const float *v9, const uint32_t *perm,
+                 const or_ray *r, const uint32_t *obj_of_tri, const or_object *objs, uint8_t pred) {
+    if (nnodes <= 0) return 0;
+    const float inv[3] = {inv_dir(r->d[0]), inv_dir(r->d[1]), inv_dir(r->d[2])};
+    uint32_t stack[OR_STACK];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const or_node *n = &nodes[stack[--sp]];
+        if (!slab(n, r->o, inv, r->tmax)) continue;
+        if (n->count > 0) {
+            for (uint32_t i = 0; i < n->count; i++) {
+                const uint32_t p = perm[n->first_or_left + i];
+                const float *tv = v9 + 9 * (size_t)p;
+                if (or_mt_any(tv, tv + 3, tv + 6, r->o, r->d, r->tmax, NULL) &&
+                    occludes(&objs[obj_of_tri[p]], pred))
+                    return 1;
+            }
+        } else {
+            stack[sp++] = n->first_or_left + 1;
+            stack[sp++] = n->first_or_left;
+        }
+    }
+    return 0;
+}
+
 void or_brute_closest(const float *v9, uint32_t ntris, const or_ray *rays,
                       uint64_t nrays, or_hit *hits, uint8_t *ties) {
     for (uint64_t r = 0; r < nrays; r++) {

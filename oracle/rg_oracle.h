@@ -98,6 +98,86 @@ void or_brute_closest(const float *v9, uint32_t ntris, const or_ray *rays,
 void or_brute_any(const float *v9, uint32_t ntris, const or_ray *rays,
                   uint64_t nrays, uint8_t *occluded);
 
+/* single-ray helpers used by the path tracer (same arithmetic as above) */
+void or_closest1(const or_node *nodes, int64_t nnodes, const float *v9, const uint32_t *perm,
+                 const or_ray *r, or_hit *h);
+
+/* --- path tracer (rg_oracle_pt.c) ---------------------------------------- */
+/* Material of a scene object (material.h:20-70 fields used by -d 5). */
+typedef struct {
+    float color[3];
+    float glossy, roughness, transparency, refraction, refraction_ratio;
+    float metallic, anisotropy, aniso_rotation, emission;
+    int32_t interior_medium; /* index into media, -1 none */
+    uint32_t visible;
+} or_object;
+
+/* HomogeneousMedium authoring fields (medium.h:37-130, constant emission). */
+typedef struct {
+    float sigma_a[3], sigma_s[3], sigma_e[3];
+    float g, density, pad;
+} or_medium;
+
+/* Camera at the render-space origin looking down -Z (scene.h:115-135). */
+typedef struct {
+    float fov_deg, exposure, depth_of_field, aperture;
+    int32_t aperture_blades;
+    float aperture_rotation_deg, view_far, pad;
+} or_camera;
+
+/* Equirectangular environment (linear RGB float, row 0 = top). */
+typedef struct {
+    uint32_t w, h;
+    const float *rgb;
+    float intensity, rotation_deg;
+} or_env;
+
+typedef struct {
+    const float *v9;            /* render-space vertices */
+    const float *n9;            /* render-space vertex normals */
+    const uint32_t *obj_of_tri; /* object index per triangle */
+    uint32_t ntris;
+    const or_object *objs;
+    uint32_t nobj;
+    const or_medium *media;
+    uint32_t nmedia;
+    int32_t global_medium;      /* -1 none */
+    or_env env;
+    or_camera cam;
+} or_scene_desc;
+
+typedef struct {
+    uint32_t width, height, spp, max_depth;
+    int32_t antialias;
+    float firefly_clamp;
+    uint32_t seed, job;
+    uint32_t sample_start;
+} or_settings;
+
+int or_any1_pred(const or_node *nodes, int64_t nnodes, const float *v9, const uint32_t *perm,
+                 const or_ray *r, const uint32_t *obj_of_tri, const or_object *objs, uint8_t pred);
+
+/* Adds, per pixel, the sum and sum of squares over samples
+ * [sample_start, sample_start + spp) of the clamped per-sample radiance.
+ * mode 0 = recursive restatement, 1 = single-lobe twin of the GPU path.
+ * active_tile (nullable) masks 32x32 tiles. */
+int or_render(const or_scene_desc *desc, const or_settings *s, int mode, int nthreads,
+              const uint8_t *active_tile, float *sum_rgb, float *sumsq_rgb);
+void or_render_one(const or_scene_desc *desc, const or_settings *s, int mode, int x, int y,
+                   int sample, float out[3]);
+
+/* guide buffers (RGBA normal/albedo, depth) and the A-Trous denoiser */
+void or_aov(const or_scene_desc *desc, const or_settings *s, float *normal_rgba, float *albedo_rgba,
+            float *depth);
+void or_denoise(int w, int h, const float *noisy_rgba, const float *normal_rgba, const float *albedo_rgba,
+                const float *depth, int levels, float sigma_c, float sigma_n, float sigma_d, float intensity,
+                float *out_rgba);
+
+/* adaptive sampling (computeTileNoise / renderAdaptive) */
+float or_tile_noise(const float *sum_rgb, const float *sumsq_rgb, int w, int h, int tx, int ty, int n);
+int or_render_adaptive(const or_scene_desc *desc, const or_settings *s, int mode, int nthreads, uint32_t base,
+                       float threshold, float *sum_rgb, float *sumsq_rgb, uint32_t *nspp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,8 @@ void free_device(Scene *s) {
 namespace rg {
 int dev_stream_create(void **s);
 void dev_stream_destroy(void *s);
+void release_pt(Ctx *c);
+void release_pt_scene(Scene *s);
 }  // namespace rg
 
 extern "C" {
@@ -68,6 +70,10 @@ int rg_create(int device, rg_ctx **out) {
 void rg_destroy(rg_ctx *ctx) {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c) return;
+    if (c->device >= 0) {
+        dev_set_device(c->device);
+        release_pt(c);
+    }
     if (c->own_stream) dev_stream_destroy(c->own_stream);
     delete c;
 }
@@ -122,6 +128,7 @@ void rg_scene_destroy(rg_scene *scene) {
     if (s->ctx && s->ctx->device >= 0) {
         dev_set_device(s->ctx->device);
         free_device(s);
+        release_pt_scene(s);
     }
     delete s;
 }
@@ -156,6 +163,7 @@ int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params) {
     if (int rc = dev_upload(s->d_nodes4, s->nodes4.data(), b4, c->stream)) { free_device(s); return rc; }
     if (int rc = dev_upload(s->d_tris, tris.data(), bt, c->stream)) { free_device(s); return rc; }
     s->device_bytes = b4 + bt;
+    s->tris_dirty = false;
     return RG_OK;
 }
 

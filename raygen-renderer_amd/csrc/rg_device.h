@@ -1,0 +1,265 @@
+// rg_device.h — device-side building blocks shared by the traversal and
+// path-tracing kernels (gfx950).  Built with -ffp-contract=off so the box and
+// triangle tests are the exact binary32 operations of the reference
+// (src/raygen/bvh.cpp:233-285, bvh.h:63-101, raycommon.cpp:151-235).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rg_internal.h"
+
+namespace rg {
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+struct RootBox {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ float inv_dir(float d) {
+    return (fabsf(d) > 1e-20f) ? 1.0f / d : 1e30f;
+}
+
+// Slab test of bvh.cpp:259-267; returns the entry distance, and whether the
+// time-independent part (tmax >= 0, tmin <= tmax) and tmin <= tcur hold.
+__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
+                                     float ox, float oy, float oz, float ix, float iy, float iz,
+                                     float tcur, float &tmin_out) {
+    const float t1x = (lx - ox) * ix;
+    const float t2x = (hx - ox) * ix;
+    const float t1y = (ly - oy) * iy;
+    const float t2y = (hy - oy) * iy;
+    const float t1z = (lz - oz) * iz;
+    const float t2z = (hz - oz) * iz;
+    const float tmin = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+    const float tmax = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
+    tmin_out = tmin;
+    return !(tmax < 0.0f || tmin > tmax || tmin > tcur);
+}
+
+// Möller–Trumbore, closest variant (raycommon.cpp:200-235) on a precomputed
+// edge record; e1 = v2 - v1 and e2 = v3 - v1 are the same binary32 values the
+// reference forms on the fly.
+__device__ __forceinline__ bool mt_closest(const float4 a, const float4 b, const float4 c,
+                                           float ox, float oy, float oz, float dx, float dy,
+                                           float dz, float tmin, float &t, float &u, float &v) {
+    // pvec = cross(d, e2)
+    const float px = dy * c.z - dz * c.y;
+    const float py = dz * c.x - dx * c.z;
+    const float pz = dx * c.y - dy * c.x;
+    const float det = b.x * px + b.y * py + b.z * pz;
+    if (fabsf(det) < 1e-8f) return false;
+    const float inv_det = 1.0f / det;
+    const float tx = ox - a.x, ty = oy - a.y, tz = oz - a.z;
+    const float uu = (tx * px + ty * py + tz * pz) * inv_det;
+    if (uu < 0.0f || uu > 1.0f) return false;
+    // qvec = cross(tvec, e1)
+    const float qx = ty * b.z - tz * b.y;
+    const float qy = tz * b.x - tx * b.z;
+    const float qz = tx * b.y - ty * b.x;
+    const float vv = (dx * qx + dy * qy + dz * qz) * inv_det;
+    if (vv < 0.0f || uu + vv > 1.0f) return false;
+    const float tt = (c.x * qx + c.y * qy + c.z * qz) * inv_det;
+    if (tt < tmin) return false;
+    if (tt >= t) return false;
+    t = tt;
+    u = uu;
+    v = vv;
+    return true;
+}
+
+// Möller–Trumbore, any-hit variant (raycommon.cpp:151-198).
+__device__ __forceinline__ bool mt_any(const float4 a, const float4 b, const float4 c, float ox,
+                                       float oy, float oz, float dx, float dy, float dz,
+                                       float maxt) {
+    const float hx = dy * c.z - dz * c.y;
+    const float hy = dz * c.x - dx * c.z;
+    const float hz = dx * c.y - dy * c.x;
+    const float aa = b.x * hx + b.y * hy + b.z * hz;
+    if (fabsf(aa) < 1e-6f) return false;
+    const float f = 1.0f / aa;
+    const float sx = ox - a.x, sy = oy - a.y, sz = oz - a.z;
+    const float u = f * (sx * hx + sy * hy + sz * hz);
+    if (u < 0.0f || u > 1.0f) return false;
+    const float qx = sy * b.z - sz * b.y;
+    const float qy = sz * b.x - sx * b.z;
+    const float qz = sx * b.y - sy * b.x;
+    const float v = f * (dx * qx + dy * qy + dz * qz);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    const float t = f * (c.x * qx + c.y * qy + c.z * qz);
+    return t > 1e-6f && t < maxt;
+}
+
+
+// Device view of a built scene's acceleration structure.
+struct DevBvh {
+    const Node4 *nodes;
+    const float4 *tris;
+    RootBox root;
+    int has_tree;
+};
+
+// Closest-hit traversal of one ray (TriangleBVH::intersectClosest,
+// bvh.cpp:233-285) over the 4-wide tree.  Entries on the stack carry the
+// child's entry distance so the reference's "tmin > closest t" rejection is
+// applied when the entry is popped, exactly when the binary traversal would
+// have tested that node.  On return t/prim/u/v hold the closest hit (prim =
+// 0xFFFFFFFF and t unchanged on a miss).
+template <int kStack>
+__device__ __forceinline__ void trace_closest_dev(const DevBvh &bv, float ox, float oy, float oz,
+                                                  float dx, float dy, float dz, float rtmin,
+                                                  float &t, uint32_t &prim, float &bu, float &bv_) {
+    const float ix = inv_dir(dx), iy = inv_dir(dy), iz = inv_dir(dz);
+    const int oct = (ix < 0.0f ? 1 : 0) | (iy < 0.0f ? 2 : 0) | (iz < 0.0f ? 4 : 0);
+    float root_tmin;
+    if (!bv.has_tree || !slab(bv.root.lo[0], bv.root.lo[1], bv.root.lo[2], bv.root.hi[0], bv.root.hi[1],
+                              bv.root.hi[2], ox, oy, oz, ix, iy, iz, t, root_tmin))
+        return;
+    uint32_t stk_ref[kStack];
+    float stk_t[kStack];
+    int sp = 0;
+    uint32_t cur = 0;
+    for (;;) {
+        if (!(cur & kLeafBit)) {
+            const Node4 *nd = bv.nodes + cur;
+            const float4 lx = *reinterpret_cast<const float4 *>(nd->lox);
+            const float4 ly = *reinterpret_cast<const float4 *>(nd->loy);
+            const float4 lz = *reinterpret_cast<const float4 *>(nd->loz);
+            const float4 hx = *reinterpret_cast<const float4 *>(nd->hix);
+            const float4 hy = *reinterpret_cast<const float4 *>(nd->hiy);
+            const float4 hz = *reinterpret_cast<const float4 *>(nd->hiz);
+            const uint4 ch = *reinterpret_cast<const uint4 *>(nd->child);
+            const uint2 ordw = *reinterpret_cast<const uint2 *>(nd->order);
+            const uint32_t ord = (oct < 4 ? (ordw.x >> (8 * oct)) : (ordw.y >> (8 * (oct - 4)))) & 0xFFu;
+            float tm[4];
+            bool ok[4];
+            ok[0] = ch.x != kEmpty && slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, t, tm[0]);
+            ok[1] = ch.y != kEmpty && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, t, tm[1]);
+            ok[2] = ch.z != kEmpty && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, t, tm[2]);
+            ok[3] = ch.w != kEmpty && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, t, tm[3]);
+            const uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
+            bool have = false;
+            uint32_t next = 0;
+            float next_t = 0.0f;
+#pragma unroll
+            for (int j = 3; j >= 0; j--) {
+                const int s = (ord >> (2 * j)) & 3;
+                if (!ok[s]) continue;
+                if (have) {
+                    stk_ref[sp] = next;
+                    stk_t[sp] = next_t;
+                    sp++;
+                }
+                next = cref[s];
+                next_t = tm[s];
+                have = true;
+            }
+            if (have) { cur = next; continue; }
+        } else {
+            uint32_t k = cur & ~kLeafBit;
+            for (;;) {
+                const float4 a = bv.tris[3 * (size_t)k];
+                const float4 b = bv.tris[3 * (size_t)k + 1];
+                const float4 c = bv.tris[3 * (size_t)k + 2];
+                if (mt_closest(a, b, c, ox, oy, oz, dx, dy, dz, rtmin, t, bu, bv_)) prim = __float_as_uint(a.w);
+                if (__float_as_uint(b.w) & 1u) break;
+                k++;
+            }
+        }
+        bool found = false;
+        while (sp > 0) {
+            sp--;
+            if (!(stk_t[sp] > t)) { cur = stk_ref[sp]; found = true; break; }
+        }
+        if (!found) break;
+    }
+}
+
+// Any-hit traversal (TriangleBVH::intersectAny, bvh.h:63-101) with maxT =
+// tmax; a triangle occludes when its occluder bits include pred_bit.  The
+// boolean result does not depend on visit order.
+template <int kStack>
+__device__ __forceinline__ bool trace_any_dev(const DevBvh &bv, float ox, float oy, float oz, float dx,
+                                              float dy, float dz, float maxt, uint32_t pred_bit) {
+    const float ix = inv_dir(dx), iy = inv_dir(dy), iz = inv_dir(dz);
+    const int oct = (ix < 0.0f ? 1 : 0) | (iy < 0.0f ? 2 : 0) | (iz < 0.0f ? 4 : 0);
+    float tm_unused;
+    if (!bv.has_tree || !slab(bv.root.lo[0], bv.root.lo[1], bv.root.lo[2], bv.root.hi[0], bv.root.hi[1],
+                              bv.root.hi[2], ox, oy, oz, ix, iy, iz, maxt, tm_unused))
+        return false;
+    uint32_t stk[kStack];
+    int sp = 0;
+    uint32_t cur = 0;
+    for (;;) {
+        if (!(cur & kLeafBit)) {
+            const Node4 *nd = bv.nodes + cur;
+            const float4 lx = *reinterpret_cast<const float4 *>(nd->lox);
+            const float4 ly = *reinterpret_cast<const float4 *>(nd->loy);
+            const float4 lz = *reinterpret_cast<const float4 *>(nd->loz);
+            const float4 hx = *reinterpret_cast<const float4 *>(nd->hix);
+            const float4 hy = *reinterpret_cast<const float4 *>(nd->hiy);
+            const float4 hz = *reinterpret_cast<const float4 *>(nd->hiz);
+            const uint4 ch = *reinterpret_cast<const uint4 *>(nd->child);
+            const uint2 ordw = *reinterpret_cast<const uint2 *>(nd->order);
+            const uint32_t ord = (oct < 4 ? (ordw.x >> (8 * oct)) : (ordw.y >> (8 * (oct - 4)))) & 0xFFu;
+            float tm[4];
+            bool ok[4];
+            ok[0] = ch.x != kEmpty && slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, maxt, tm[0]);
+            ok[1] = ch.y != kEmpty && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, maxt, tm[1]);
+            ok[2] = ch.z != kEmpty && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, maxt, tm[2]);
+            ok[3] = ch.w != kEmpty && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, maxt, tm[3]);
+            const uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
+            bool have = false;
+            uint32_t next = 0;
+#pragma unroll
+            for (int j = 3; j >= 0; j--) {
+                const int s = (ord >> (2 * j)) & 3;
+                if (!ok[s]) continue;
+                if (have) stk[sp++] = next;
+                next = cref[s];
+                have = true;
+            }
+            if (have) { cur = next; continue; }
+        } else {
+            uint32_t k = cur & ~kLeafBit;
+            for (;;) {
+                const float4 a = bv.tris[3 * (size_t)k];
+                const float4 b = bv.tris[3 * (size_t)k + 1];
+                const float4 c = bv.tris[3 * (size_t)k + 2];
+                if ((__float_as_uint(c.w) & pred_bit) && mt_any(a, b, c, ox, oy, oz, dx, dy, dz, maxt))
+                    return true;
+                if (__float_as_uint(b.w) & 1u) break;
+                k++;
+            }
+        }
+        if (sp == 0) break;
+        cur = stk[--sp];
+    }
+    return false;
+}
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint32_t lo0 = 0xD2511F53u * c[0];
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        const uint32_t lo1 = 0xCD9E8D57u * c[2];
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        const uint32_t n0 = hi1 ^ c[1] ^ k0;
+        const uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+
+}  // namespace rg

@@ -60,11 +60,26 @@ static_assert(sizeof(Node2) == 32, "Node2 must be 32 bytes");
 // ---------------------------------------------------------------------------
 // Host objects
 // ---------------------------------------------------------------------------
+// Device buffers of the wavefront path tracer, grown on demand and reused
+// across rg_render calls on one context.
+struct PTBuffers {
+    uint64_t cap = 0;       // path slots
+    uint64_t npix_cap = 0;  // pixels
+    uint64_t ntile_cap = 0; // 32x32 tiles
+    void *ray = nullptr, *hit = nullptr, *beta = nullptr, *tp = nullptr, *L = nullptr, *meta = nullptr;
+    void *sray = nullptr, *scontrib = nullptr, *nshadow = nullptr, *q0 = nullptr, *q1 = nullptr;
+    void *count = nullptr, *sample_out = nullptr;
+    void *sum = nullptr, *sumsq = nullptr, *hdr = nullptr;
+    void *tile_mask = nullptr, *tile_nspp = nullptr, *tile_noise = nullptr, *n_active = nullptr;
+    void *aov = nullptr, *dn_a = nullptr, *dn_b = nullptr;
+};
+
 struct Ctx {
     int device = -1;
     void *stream = nullptr;        // hipStream_t in use
     void *own_stream = nullptr;    // stream created by rg_create
     std::atomic<bool> cancel{false};
+    PTBuffers pt;
 };
 
 struct Scene {
@@ -90,6 +105,26 @@ struct Scene {
     uint64_t device_bytes = 0;
     // per-triangle predicate mask: bit p set when predicate p treats it as occluder
     std::vector<uint8_t> pred_mask;
+
+    // shading inputs (rg_scene_set_*)
+    bool has_shading = false;
+    std::vector<float> n9;
+    std::vector<uint32_t> obj_of_tri;
+    std::vector<rg_object> objects;
+    std::vector<rg_medium> media;
+    int32_t global_medium = -1;
+    uint32_t env_w = 0, env_h = 0;
+    std::vector<float> env_rgb;
+    float env_intensity = 1.0f, env_rotation_deg = 0.0f;
+    rg_camera camera{75.0f, 1.0f, 0.0f, 1.8f, 0, 0.0f, 50.0f, 0.0f};
+    // device copies for the path tracer (uploaded lazily)
+    bool pt_dirty = true;
+    bool tris_dirty = false;
+    void *d_v9 = nullptr, *d_shade = nullptr, *d_objs = nullptr, *d_lights = nullptr;
+    void *d_light_tris = nullptr, *d_media = nullptr, *d_env_rgb = nullptr, *d_env_marg = nullptr;
+    void *d_env_cond = nullptr;
+    uint32_t nlights = 0;
+    float env_total = 0.0f;
 };
 
 // error reporting (thread-local text)

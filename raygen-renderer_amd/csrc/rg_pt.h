@@ -1,0 +1,99 @@
+// rg_pt.h — device-resident scene and path-state layouts of the wavefront
+// path tracer, shared by the host driver (rg_render.cpp) and the kernels
+// (rg_pt.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rg_internal.h"
+
+namespace rg {
+
+// One SceneObject's material (material.h:20-70); same layout as rg_object.
+struct DevObject {
+    float color[3];
+    float glossy, roughness, transparency, refraction, refraction_ratio;
+    float metallic, anisotropy, aniso_rotation, emission;
+    int32_t interior_medium;
+    uint32_t visible;
+};
+static_assert(sizeof(DevObject) == sizeof(rg_object), "DevObject must mirror rg_object");
+
+// Per-triangle shading record (original triangle order), 64 B:
+// vertex normals, ti.normalizedpd, object index, triCount*area of its
+// object (the light pdf's sampled area), ti.a and area
+// (raycommon.cpp:128-149; rayrenderer.cpp:2030-2036).
+struct DevTriShade {
+    float n[9];
+    float gn[3];
+    uint32_t obj;
+    float sampled_area;
+    float ti_a;
+    float area;
+};
+static_assert(sizeof(DevTriShade) == 64, "DevTriShade must be 64 bytes");
+
+// HomogeneousMedium after prepare() (medium.cpp:18-57).
+struct DevMedium {
+    float sigma_t[3], sigma_s[3], sigma_e[3];
+    float hero, g;
+    int32_t active;
+};
+
+// Area light = an emissive object's triangles (transformScene,
+// rayrenderer.cpp:333-354).
+struct DevLight {
+    uint32_t obj, first, count, pad;
+};
+
+// Everything a shade kernel reads, passed by value.
+struct PTScene {
+    const float *v9;
+    const DevTriShade *shade;
+    const DevObject *objs;
+    const DevLight *lights;
+    const uint32_t *light_tris;
+    uint32_t nlights;
+    const DevMedium *media;
+    int32_t global_medium;
+    // equirect environment
+    const float *env_rgb;
+    const float *env_marg;
+    const float *env_cond;
+    uint32_t env_w, env_h;
+    float env_total, env_intensity, env_cos, env_sin;
+    int32_t has_env;
+    // camera (initRenderThreadContext, rayrenderer.cpp:97-132)
+    float view_scale, half_w, half_h, aperture, dof, ap_rot;
+    int32_t blades;
+    // settings
+    int32_t max_depth;
+    int32_t antialias;
+    uint32_t width, height;
+    uint32_t seed, job;
+};
+
+// Structure-of-arrays path state, capacity = pixels x samples in flight.
+struct PTState {
+    rg_ray *ray;        // current ray
+    rg_hit *hit;        // its closest hit
+    float4 *beta;       // estimator weight, w = bsdfSampledPdf
+    float4 *tp;         // BSDFParam::throughput (Russian roulette)
+    float4 *L;          // radiance so far, w = alive flag
+    int4 *meta;         // passes, medium, next LDS dim, next Philox draw
+    rg_ray *sray;       // 2 NEE shadow rays per path (tmin = predicate)
+    float4 *scontrib;   // their unoccluded contributions
+    uint32_t *nshadow;
+    uint32_t *q[2];     // live-path queues
+    uint32_t *count;    // [2] queue lengths
+    float4 *sample_out; // clamped sample radiance per path
+    uint32_t sample_base;
+};
+
+int launch_generation(const Scene &scn, const PTScene &sc, const PTState &st, uint32_t S, uint32_t sample_base,
+                      const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream);
+int launch_resolve(const float *d_sum, const uint32_t *d_nspp, uint32_t width, uint32_t height, float exposure,
+                   float *d_hdr, void *stream);
+
+}  // namespace rg

@@ -1,0 +1,451 @@
+// rg_render.cpp — host driver of the wavefront path tracer: scene shading
+// inputs, device upload, the sample loop (uniform or per-tile adaptive,
+// RayRenderer::render / renderAdaptive, rayrenderer.cpp:478-596, 1162-1252)
+// and the HDR resolve.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rg_internal.h"
+#include "rg_pt.h"
+
+namespace rg {
+
+int launch_tile_noise(const float *d_sum, const float *d_sumsq, const uint32_t *d_nspp, uint32_t width,
+                      uint32_t height, uint32_t target, float threshold, uint8_t *d_mask, uint32_t *d_nactive,
+                      void *stream);
+int launch_tiles_add(uint32_t *d_nspp, const uint8_t *d_mask, uint32_t ntiles, uint32_t add, void *stream);
+int launch_fill_u32(void *p, uint32_t value, uint64_t n, void *stream);
+int launch_denoise(const rg_settings &s, uint32_t width, uint32_t height, const PTScene &sc, const Scene &scn,
+                   float *d_hdr, void *d_aov, void *d_a, void *d_b, void *stream);
+int launch_aov(const Scene &scn, const PTScene &sc, void *d_aov, void *stream);
+
+namespace {
+
+int hip_ok(hipError_t e, const char *what) {
+    if (e == hipSuccess) return RG_OK;
+    return set_error(RG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct V {
+    float x, y, z;
+};
+inline V vsub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V vcross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float vlen(V a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+void free_pt_scene(Scene *s) {
+    for (void **p : {&s->d_v9, &s->d_shade, &s->d_objs, &s->d_lights, &s->d_light_tris, &s->d_media,
+                     &s->d_env_rgb, &s->d_env_marg, &s->d_env_cond}) {
+        dev_free(*p);
+        *p = nullptr;
+    }
+}
+
+template <class T>
+int upload_vec(void **dst, const std::vector<T> &v, void *stream) {
+    dev_free(*dst);
+    *dst = nullptr;
+    if (v.empty()) return RG_OK;
+    if (int rc = dev_malloc(dst, v.size() * sizeof(T))) return rc;
+    return dev_upload(*dst, v.data(), v.size() * sizeof(T), stream);
+}
+
+// Per-triangle records, lights, media and envmap CDF; the arithmetic is the
+// reference's (precalc raycommon.cpp:128-149, prepare medium.cpp:18-57,
+// buildEnvmapCDF scene.cpp:400-463).
+int upload_pt_scene(Scene *s, void *stream) {
+    const uint32_t n = s->n_tris;
+    std::vector<DevTriShade> sh(n);
+    std::vector<uint32_t> cnt(s->objects.size(), 0);
+    for (uint32_t t = 0; t < n; t++) cnt[s->obj_of_tri[t]]++;
+    for (uint32_t t = 0; t < n; t++) {
+        const float *v = s->v9.data() + 9 * (size_t)t;
+        const V a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
+        const V pd = vcross(vsub(b, a), vsub(c, b));
+        DevTriShade &r = sh[t];
+        std::memcpy(r.n, s->n9.data() + 9 * (size_t)t, sizeof r.n);
+        const float inv = 1.0f / vlen(pd);
+        r.gn[0] = pd.x * inv;
+        r.gn[1] = pd.y * inv;
+        r.gn[2] = pd.z * inv;
+        r.obj = s->obj_of_tri[t];
+        r.ti_a = 1.0f / vlen(vcross(vsub(a, b), vsub(a, c)));
+        r.area = 0.5f * vlen(vcross(vsub(b, a), vsub(c, a)));
+        r.sampled_area = (s->objects[r.obj].emission > 0.0f) ? (float)cnt[r.obj] * r.area : 0.0f;
+    }
+    std::vector<DevLight> lights;
+    std::vector<uint32_t> light_tris;
+    for (uint32_t o = 0; o < s->objects.size(); o++) {
+        if (!(s->objects[o].emission > 0.0f) || cnt[o] == 0) continue;
+        DevLight L{o, (uint32_t)light_tris.size(), cnt[o], 0};
+        for (uint32_t t = 0; t < n; t++)
+            if (s->obj_of_tri[t] == o) light_tris.push_back(t);
+        lights.push_back(L);
+    }
+    std::vector<DevMedium> media(s->media.size());
+    for (size_t i = 0; i < s->media.size(); i++) {
+        const rg_medium &m = s->media[i];
+        DevMedium &q = media[i];
+        const float den = fmaxf(0.0f, m.density);
+        for (int k = 0; k < 3; k++) {
+            q.sigma_s[k] = m.sigma_s[k] * den;
+            q.sigma_e[k] = m.sigma_e[k] * den;
+            q.sigma_t[k] = m.sigma_a[k] * den + q.sigma_s[k];
+        }
+        q.hero = (q.sigma_t[0] + q.sigma_t[1] + q.sigma_t[2]) * (1.0f / 3.0f);
+        q.g = m.g;
+        q.active = (q.hero > 0.0f || q.sigma_e[0] != 0.0f || q.sigma_e[1] != 0.0f || q.sigma_e[2] != 0.0f) ? 1 : 0;
+    }
+    std::vector<float> marg, cond;
+    s->env_total = 0.0f;
+    if (s->env_w > 0 && s->env_h > 0) {
+        const int W = (int)s->env_w, H = (int)s->env_h;
+        cond.assign((size_t)H * (W + 1), 0.0f);
+        marg.assign((size_t)H + 1, 0.0f);
+        std::vector<float> roww(H, 0.0f);
+        for (int y = 0; y < H; y++) {
+            const float v = ((float)y + 0.5f) / (float)H;
+            const float st = sinf(3.14159265358979323846f * v);
+            float rs = 0.0f;
+            for (int x = 0; x < W; x++) {
+                const float *c = s->env_rgb.data() + 3 * ((size_t)y * W + x);
+                const float l = fmaxf(0.0f, 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2]);
+                rs += l * st;
+                cond[(size_t)y * (W + 1) + x + 1] = rs;
+            }
+            roww[y] = rs;
+        }
+        for (int y = 0; y < H; y++)
+            if (roww[y] > 0.0f) {
+                const float inv = 1.0f / roww[y];
+                for (int x = 1; x <= W; x++) cond[(size_t)y * (W + 1) + x] *= inv;
+            }
+        for (int y = 0; y < H; y++) marg[y + 1] = marg[y] + roww[y];
+        s->env_total = marg[H];
+        if (s->env_total > 0.0f) {
+            const float inv = 1.0f / s->env_total;
+            for (int y = 0; y <= H; y++) marg[y] *= inv;
+        }
+    }
+    s->nlights = (uint32_t)lights.size();
+    int rc = upload_vec(&s->d_v9, s->v9, stream);
+    if (!rc) rc = upload_vec(&s->d_shade, sh, stream);
+    if (!rc) rc = upload_vec(&s->d_objs, s->objects, stream);
+    if (!rc) rc = upload_vec(&s->d_lights, lights, stream);
+    if (!rc) rc = upload_vec(&s->d_light_tris, light_tris, stream);
+    if (!rc) rc = upload_vec(&s->d_media, media, stream);
+    if (!rc) rc = upload_vec(&s->d_env_rgb, s->env_rgb, stream);
+    if (!rc) rc = upload_vec(&s->d_env_marg, marg, stream);
+    if (!rc) rc = upload_vec(&s->d_env_cond, cond, stream);
+    if (rc) return rc;
+    s->pt_dirty = false;
+    return RG_OK;
+}
+
+int reupload_tris(Scene *s, void *stream) {
+    std::vector<Tri48> tris;
+    make_tris(s->v9.data(), s->nodes2, s->perm, s->pred_mask, tris);
+    if (tris.empty()) return RG_OK;
+    s->tris_dirty = false;
+    return dev_upload(s->d_tris, tris.data(), tris.size() * sizeof(Tri48), stream);
+}
+
+void free_buffers(PTBuffers &b) {
+    for (void **p : {&b.ray, &b.hit, &b.beta, &b.tp, &b.L, &b.meta, &b.sray, &b.scontrib, &b.nshadow, &b.q0,
+                     &b.q1, &b.count, &b.sample_out, &b.sum, &b.sumsq, &b.hdr, &b.tile_mask, &b.tile_nspp,
+                     &b.tile_noise, &b.n_active, &b.aov, &b.dn_a, &b.dn_b}) {
+        dev_free(*p);
+        *p = nullptr;
+    }
+    b.cap = b.npix_cap = b.ntile_cap = 0;
+}
+
+int ensure_buffers(PTBuffers &b, uint64_t cap, uint64_t npix, uint64_t ntiles) {
+    if (cap <= b.cap && npix <= b.npix_cap && ntiles <= b.ntile_cap) return RG_OK;
+    cap = std::max(cap, b.cap);
+    npix = std::max(npix, b.npix_cap);
+    ntiles = std::max(ntiles, b.ntile_cap);
+    free_buffers(b);
+    struct A {
+        void **p;
+        uint64_t bytes;
+    } allocs[] = {
+        {&b.ray, cap * 32}, {&b.hit, cap * 16}, {&b.beta, cap * 16}, {&b.tp, cap * 16}, {&b.L, cap * 16},
+        {&b.meta, cap * 16}, {&b.sray, cap * 64}, {&b.scontrib, cap * 32}, {&b.nshadow, cap * 4},
+        {&b.q0, cap * 4}, {&b.q1, cap * 4}, {&b.count, 64}, {&b.sample_out, cap * 16}, {&b.sum, npix * 12},
+        {&b.sumsq, npix * 12}, {&b.hdr, npix * 16}, {&b.tile_mask, ntiles}, {&b.tile_nspp, ntiles * 4},
+        {&b.tile_noise, ntiles * 4}, {&b.n_active, 64}, {&b.aov, npix * 36}, {&b.dn_a, npix * 16},
+        {&b.dn_b, npix * 16},
+    };
+    for (auto &a : allocs)
+        if (int rc = dev_malloc(a.p, a.bytes)) {
+            free_buffers(b);
+            return rc;
+        }
+    b.cap = cap;
+    b.npix_cap = npix;
+    b.ntile_cap = ntiles;
+    return RG_OK;
+}
+
+}  // namespace
+
+void release_pt(Ctx *c) { free_buffers(c->pt); }
+void release_pt_scene(Scene *s) { free_pt_scene(s); }
+
+}  // namespace rg
+
+using namespace rg;
+
+extern "C" {
+
+void rg_object_default(rg_object *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->color[0] = o->color[1] = o->color[2] = 0.8f;
+    o->roughness = 0.5f;
+    o->refraction_ratio = 1.45f;
+    o->interior_medium = -1;
+    o->visible = 1;
+}
+
+void rg_settings_default(rg_settings *s) {
+    if (!s) return;
+    std::memset(s, 0, sizeof *s);
+    s->width = 800;
+    s->height = 600;
+    s->spp = 100;
+    s->max_depth = 32;
+    s->antialias = 1;
+    s->firefly_clamp = 10.0f;
+    s->seed = 0xC0DE0000u;
+    s->adaptive_base = 4;
+    s->adaptive_threshold = 0.02f;
+    s->denoise_levels = 5;
+    s->sigma_color = 0.4f;
+    s->sigma_normal = 128.0f;
+    s->sigma_depth = 0.1f;
+    s->denoise_intensity = 1.0f;
+}
+
+int rg_scene_set_shading(rg_scene *scene, const float *n9, const uint32_t *obj_of_tri, const rg_object *objs,
+                         uint32_t n_obj) {
+    Scene *s = reinterpret_cast<Scene *>(scene);
+    if (!s || (s->n_tris && (!n9 || !obj_of_tri)) || (n_obj && !objs))
+        return set_error(RG_E_INVALID, "null argument");
+    for (uint32_t t = 0; t < s->n_tris; t++)
+        if (obj_of_tri[t] >= n_obj) return set_error(RG_E_INVALID, "object index out of range");
+    for (uint32_t o = 0; o < n_obj; o++)
+        if (objs[o].interior_medium >= 0 && (size_t)objs[o].interior_medium >= s->media.size() &&
+            objs[o].interior_medium >= 1 << 20)
+            return set_error(RG_E_INVALID, "interior medium index out of range");
+    s->n9.assign(n9, n9 + 9 * (size_t)s->n_tris);
+    s->obj_of_tri.assign(obj_of_tri, obj_of_tri + s->n_tris);
+    s->objects.assign(objs, objs + n_obj);
+    s->pred_mask.resize(s->n_tris);
+    for (uint32_t t = 0; t < s->n_tris; t++) {
+        const rg_object &m = objs[obj_of_tri[t]];
+        uint8_t mask = 1u << RG_PRED_ALL;
+        if (m.transparency < 0.01f && m.emission <= 0.0f) mask |= 1u << RG_PRED_AREA;
+        if (!(m.emission > 0.0f) && (m.transparency < 0.01f || m.refraction > 0.1f)) mask |= 1u << RG_PRED_ENV;
+        s->pred_mask[t] = mask;
+    }
+    s->has_shading = true;
+    s->pt_dirty = true;
+    s->tris_dirty = s->built;
+    return RG_OK;
+}
+
+int rg_scene_set_media(rg_scene *scene, const rg_medium *media, uint32_t n, int32_t global_medium) {
+    Scene *s = reinterpret_cast<Scene *>(scene);
+    if (!s || (n && !media)) return set_error(RG_E_INVALID, "null argument");
+    if (global_medium >= (int32_t)n) return set_error(RG_E_INVALID, "global medium index out of range");
+    s->media.assign(media, media + n);
+    s->global_medium = global_medium < 0 ? -1 : global_medium;
+    s->pt_dirty = true;
+    return RG_OK;
+}
+
+int rg_scene_set_env(rg_scene *scene, const rg_env *env) {
+    Scene *s = reinterpret_cast<Scene *>(scene);
+    if (!s) return set_error(RG_E_INVALID, "null scene");
+    if (!env || env->w == 0 || env->h == 0) {
+        s->env_w = s->env_h = 0;
+        s->env_rgb.clear();
+    } else {
+        if (!env->rgb) return set_error(RG_E_INVALID, "env rgb is null");
+        s->env_w = env->w;
+        s->env_h = env->h;
+        s->env_rgb.assign(env->rgb, env->rgb + 3 * (size_t)env->w * env->h);
+        s->env_intensity = env->intensity;
+        s->env_rotation_deg = env->rotation_deg;
+    }
+    s->pt_dirty = true;
+    return RG_OK;
+}
+
+int rg_scene_set_camera(rg_scene *scene, const rg_camera *cam) {
+    Scene *s = reinterpret_cast<Scene *>(scene);
+    if (!s || !cam) return set_error(RG_E_INVALID, "null argument");
+    s->camera = *cam;
+    return RG_OK;
+}
+
+int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *hdr_rgba, float *sum_rgb,
+              float *sumsq_rgb, uint32_t *nspp) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    Scene *s = reinterpret_cast<Scene *>(scene);
+    if (!c || !s || !settings) return set_error(RG_E_INVALID, "null argument");
+    if (c->device < 0) return set_error(RG_E_NODEVICE, "host-only context has no device");
+    const rg_settings &st = *settings;
+    if (st.width == 0 || st.height == 0 || st.width > 16384 || st.height > 16384)
+        return set_error(RG_E_INVALID, "resolution must be 1..16384 per axis");
+    if (st.max_depth < 1 || st.max_depth > 64) return set_error(RG_E_INVALID, "max_depth must be 1..64");
+    if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
+    if (!s->has_shading) return set_error(RG_E_INVALID, "rg_scene_set_shading was not called");
+    for (const rg_object &o : s->objects)
+        if (o.interior_medium >= (int32_t)s->media.size())
+            return set_error(RG_E_INVALID, "interior medium index out of range");
+    if (c->cancel.load()) return set_error(RG_E_CANCELLED, "cancelled");
+    if (int rc = dev_set_device(c->device)) return rc;
+    void *stream = c->stream;
+    if (s->tris_dirty)
+        if (int rc = reupload_tris(s, stream)) return rc;
+    if (s->pt_dirty)
+        if (int rc = upload_pt_scene(s, stream)) return rc;
+
+    const uint32_t W = st.width, H = st.height, npix = W * H;
+    const uint32_t tx = (W + 31) / 32, ty = (H + 31) / 32, ntiles = tx * ty;
+    const uint32_t spp = std::max<uint32_t>(1, st.spp);
+    uint32_t S = st.samples_in_flight;
+    if (S == 0) S = std::max<uint32_t>(1, (1u << 21) / npix);
+    S = std::min(S, spp);
+    if (st.adaptive) S = std::min(S, std::max<uint32_t>(1, st.adaptive_base));
+    if ((uint64_t)npix * S > (1ull << 31)) return set_error(RG_E_BUDGET, "too many samples in flight");
+    if (int rc = ensure_buffers(c->pt, (uint64_t)npix * S, npix, ntiles)) return rc;
+    PTBuffers &B = c->pt;
+
+    PTScene sc;
+    std::memset(&sc, 0, sizeof sc);
+    sc.v9 = (const float *)s->d_v9;
+    sc.shade = (const DevTriShade *)s->d_shade;
+    sc.objs = (const DevObject *)s->d_objs;
+    sc.lights = (const DevLight *)s->d_lights;
+    sc.light_tris = (const uint32_t *)s->d_light_tris;
+    sc.nlights = s->nlights;
+    sc.media = (const DevMedium *)s->d_media;
+    sc.global_medium = s->global_medium;
+    sc.env_rgb = (const float *)s->d_env_rgb;
+    sc.env_marg = (const float *)s->d_env_marg;
+    sc.env_cond = (const float *)s->d_env_cond;
+    sc.env_w = s->env_w;
+    sc.env_h = s->env_h;
+    sc.env_total = s->env_total;
+    sc.env_intensity = s->env_intensity;
+    {
+        const float yaw = s->env_rotation_deg * (float)(M_PI / 180.0);
+        sc.env_cos = cosf(yaw);
+        sc.env_sin = sinf(yaw);
+    }
+    sc.has_env = (s->env_w > 0 && s->env_h > 0) ? 1 : 0;
+    const rg_camera &cam = s->camera;
+    sc.half_w = (float)W * 0.5f;
+    sc.half_h = (float)H * 0.5f;
+    {
+        const float fov = fminf(fmaxf(cam.fov_deg, 1.0f), 179.0f);
+        const float th = tanf(fov * (float)(M_PI / 180.0) * 0.5f);
+        sc.view_scale = (2.0f * th) / (float)H;
+    }
+    sc.dof = cam.depth_of_field;
+    sc.aperture = cam.aperture > 0.0f ? 1.0f / cam.aperture : 0.0f;
+    sc.blades = cam.aperture_blades;
+    sc.ap_rot = cam.aperture_rotation_deg * (float)(M_PI / 180.0);
+    sc.max_depth = (int32_t)st.max_depth;
+    sc.antialias = st.antialias ? 1 : 0;
+    sc.width = W;
+    sc.height = H;
+    sc.seed = st.seed;
+    sc.job = st.job;
+
+    PTState ps;
+    ps.ray = (rg_ray *)B.ray;
+    ps.hit = (rg_hit *)B.hit;
+    ps.beta = (float4 *)B.beta;
+    ps.tp = (float4 *)B.tp;
+    ps.L = (float4 *)B.L;
+    ps.meta = (int4 *)B.meta;
+    ps.sray = (rg_ray *)B.sray;
+    ps.scontrib = (float4 *)B.scontrib;
+    ps.nshadow = (uint32_t *)B.nshadow;
+    ps.q[0] = (uint32_t *)B.q0;
+    ps.q[1] = (uint32_t *)B.q1;
+    ps.count = (uint32_t *)B.count;
+    ps.sample_out = (float4 *)B.sample_out;
+    ps.sample_base = 0;
+
+    float *d_sum = (float *)B.sum, *d_sumsq = (float *)B.sumsq;
+    uint32_t *d_nspp = (uint32_t *)B.tile_nspp;
+    uint8_t *d_mask = (uint8_t *)B.tile_mask;
+    if (int rc = launch_fill_u32(d_sum, 0u, (uint64_t)npix * 3, stream)) return rc;
+    if (int rc = launch_fill_u32(d_sumsq, 0u, (uint64_t)npix * 3, stream)) return rc;
+    if (int rc = launch_fill_u32(d_nspp, 0u, ntiles, stream)) return rc;
+
+    auto run_samples = [&](uint32_t first, uint32_t count, const uint8_t *mask) -> int {
+        uint32_t done = 0;
+        while (done < count) {
+            if (c->cancel.load()) return set_error(RG_E_CANCELLED, "cancelled");
+            const uint32_t n = std::min(S, count - done);
+            if (int rc = launch_generation(*s, sc, ps, n, first + done, mask, d_sum, d_sumsq, st.firefly_clamp,
+                                           stream))
+                return rc;
+            done += n;
+        }
+        return RG_OK;
+    };
+
+    if (!st.adaptive) {
+        if (int rc = run_samples(0, spp, nullptr)) return rc;
+        if (int rc = launch_fill_u32(d_nspp, spp, ntiles, stream)) return rc;
+    } else {
+        const uint32_t base = std::max<uint32_t>(1, st.adaptive_base);
+        const uint32_t first = std::min(base, spp);
+        if (int rc = run_samples(0, first, nullptr)) return rc;
+        if (int rc = launch_fill_u32(d_nspp, first, ntiles, stream)) return rc;
+        uint32_t spent = first;
+        uint32_t *d_nactive = (uint32_t *)B.n_active;
+        while (spent < spp) {
+            if (int rc = launch_tile_noise(d_sum, d_sumsq, d_nspp, W, H, spp, st.adaptive_threshold, d_mask,
+                                           d_nactive, stream))
+                return rc;
+            uint32_t nact = 0;
+            if (int rc = dev_download(&nact, d_nactive, 4, stream)) return rc;
+            if (nact == 0) break;
+            const uint32_t pc = std::min(base, spp - spent);
+            if (int rc = run_samples(spent, pc, d_mask)) return rc;
+            if (int rc = launch_tiles_add(d_nspp, d_mask, ntiles, pc, stream)) return rc;
+            spent += pc;
+        }
+    }
+    float *d_hdr = (float *)B.hdr;
+    if (int rc = launch_resolve(d_sum, d_nspp, W, H, cam.exposure, d_hdr, stream)) return rc;
+    if (st.denoise) {
+        if (int rc = launch_aov(*s, sc, B.aov, stream)) return rc;
+        if (int rc = launch_denoise(st, W, H, sc, *s, d_hdr, B.aov, B.dn_a, B.dn_b, stream)) return rc;
+    }
+    if (hdr_rgba)
+        if (int rc = dev_download(hdr_rgba, d_hdr, (size_t)npix * 16, stream)) return rc;
+    if (sum_rgb)
+        if (int rc = dev_download(sum_rgb, d_sum, (size_t)npix * 12, stream)) return rc;
+    if (sumsq_rgb)
+        if (int rc = dev_download(sumsq_rgb, d_sumsq, (size_t)npix * 12, stream)) return rc;
+    if (nspp)
+        if (int rc = dev_download(nspp, d_nspp, (size_t)ntiles * 4, stream)) return rc;
+    return hip_ok(hipStreamSynchronize((hipStream_t)stream), "render sync");
+}
+
+}  // extern "C"

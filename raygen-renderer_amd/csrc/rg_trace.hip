@@ -10,6 +10,7 @@
 
 #include <cstdint>
 
+#include "rg_device.h"
 #include "rg_internal.h"
 
 namespace rg {
@@ -70,288 +71,31 @@ int dev_sync(void *stream) {
     return hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
 }
 
-// ---------------------------------------------------------------------------
-// Device helpers
-// ---------------------------------------------------------------------------
-struct RootBox {
-    float lo[3], hi[3];
-};
-
-__device__ __forceinline__ float inv_dir(float d) {
-    return (fabsf(d) > 1e-20f) ? 1.0f / d : 1e30f;
-}
-
-// Slab test of bvh.cpp:259-267; returns the entry distance, and whether the
-// time-independent part (tmax >= 0, tmin <= tmax) and tmin <= tcur hold.
-__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
-                                     float ox, float oy, float oz, float ix, float iy, float iz,
-                                     float tcur, float &tmin_out) {
-    const float t1x = (lx - ox) * ix;
-    const float t2x = (hx - ox) * ix;
-    const float t1y = (ly - oy) * iy;
-    const float t2y = (hy - oy) * iy;
-    const float t1z = (lz - oz) * iz;
-    const float t2z = (hz - oz) * iz;
-    const float tmin = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
-    const float tmax = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
-    tmin_out = tmin;
-    return !(tmax < 0.0f || tmin > tmax || tmin > tcur);
-}
-
-// Möller–Trumbore, closest variant (raycommon.cpp:200-235) on a precomputed
-// edge record; e1 = v2 - v1 and e2 = v3 - v1 are the same binary32 values the
-// reference forms on the fly.
-__device__ __forceinline__ bool mt_closest(const float4 a, const float4 b, const float4 c,
-                                           float ox, float oy, float oz, float dx, float dy,
-                                           float dz, float tmin, float &t, float &u, float &v) {
-    // pvec = cross(d, e2)
-    const float px = dy * c.z - dz * c.y;
-    const float py = dz * c.x - dx * c.z;
-    const float pz = dx * c.y - dy * c.x;
-    const float det = b.x * px + b.y * py + b.z * pz;
-    if (fabsf(det) < 1e-8f) return false;
-    const float inv_det = 1.0f / det;
-    const float tx = ox - a.x, ty = oy - a.y, tz = oz - a.z;
-    const float uu = (tx * px + ty * py + tz * pz) * inv_det;
-    if (uu < 0.0f || uu > 1.0f) return false;
-    // qvec = cross(tvec, e1)
-    const float qx = ty * b.z - tz * b.y;
-    const float qy = tz * b.x - tx * b.z;
-    const float qz = tx * b.y - ty * b.x;
-    const float vv = (dx * qx + dy * qy + dz * qz) * inv_det;
-    if (vv < 0.0f || uu + vv > 1.0f) return false;
-    const float tt = (c.x * qx + c.y * qy + c.z * qz) * inv_det;
-    if (tt < tmin) return false;
-    if (tt >= t) return false;
-    t = tt;
-    u = uu;
-    v = vv;
-    return true;
-}
-
-// Möller–Trumbore, any-hit variant (raycommon.cpp:151-198).
-__device__ __forceinline__ bool mt_any(const float4 a, const float4 b, const float4 c, float ox,
-                                       float oy, float oz, float dx, float dy, float dz,
-                                       float maxt) {
-    const float hx = dy * c.z - dz * c.y;
-    const float hy = dz * c.x - dx * c.z;
-    const float hz = dx * c.y - dy * c.x;
-    const float aa = b.x * hx + b.y * hy + b.z * hz;
-    if (fabsf(aa) < 1e-6f) return false;
-    const float f = 1.0f / aa;
-    const float sx = ox - a.x, sy = oy - a.y, sz = oz - a.z;
-    const float u = f * (sx * hx + sy * hy + sz * hz);
-    if (u < 0.0f || u > 1.0f) return false;
-    const float qx = sy * b.z - sz * b.y;
-    const float qy = sz * b.x - sx * b.z;
-    const float qz = sx * b.y - sy * b.x;
-    const float v = f * (dx * qx + dy * qy + dz * qz);
-    if (v < 0.0f || u + v > 1.0f) return false;
-    const float t = f * (c.x * qx + c.y * qy + c.z * qz);
-    return t > 1e-6f && t < maxt;
-}
-
 constexpr int kBlock = 128;
 
-// One thread per ray.  Entries on the stack carry the child's entry distance
-// so the "tmin > closest t" rejection of the reference is applied when the
-// entry is popped, exactly when the binary traversal would have tested it.
+// One thread per ray.
 template <int kStack>
-__global__ __launch_bounds__(kBlock) void k_closest(const Node4 *__restrict__ nodes,
-                                                    const float4 *__restrict__ tris,
-                                                    const rg_ray *__restrict__ rays,
-                                                    rg_hit *__restrict__ hits, uint64_t n,
-                                                    RootBox root, int has_tree) {
+__global__ __launch_bounds__(kBlock) void k_closest(DevBvh bv, const rg_ray *__restrict__ rays,
+                                                    rg_hit *__restrict__ hits, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float4 r0 = reinterpret_cast<const float4 *>(rays)[2 * i];
     const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
-    const float ox = r0.x, oy = r0.y, oz = r0.z, rtmin = r0.w;
-    const float dx = r1.x, dy = r1.y, dz = r1.z;
-    float t = r1.w;
+    float t = r1.w, bu = 0.0f, bvv = 0.0f;
     uint32_t prim = 0xFFFFFFFFu;
-    float bu = 0.0f, bv = 0.0f;
-
-    const float ix = inv_dir(dx), iy = inv_dir(dy), iz = inv_dir(dz);
-    const int oct = (ix < 0.0f ? 1 : 0) | (iy < 0.0f ? 2 : 0) | (iz < 0.0f ? 4 : 0);
-
-    float root_tmin;
-    if (has_tree && slab(root.lo[0], root.lo[1], root.lo[2], root.hi[0], root.hi[1], root.hi[2],
-                         ox, oy, oz, ix, iy, iz, t, root_tmin)) {
-        uint32_t stk_ref[kStack];
-        float stk_t[kStack];
-        int sp = 0;
-        uint32_t cur = 0;
-        for (;;) {
-            if (!(cur & kLeafBit)) {
-                const Node4 *nd = nodes + cur;
-                const float4 lx = *reinterpret_cast<const float4 *>(nd->lox);
-                const float4 ly = *reinterpret_cast<const float4 *>(nd->loy);
-                const float4 lz = *reinterpret_cast<const float4 *>(nd->loz);
-                const float4 hx = *reinterpret_cast<const float4 *>(nd->hix);
-                const float4 hy = *reinterpret_cast<const float4 *>(nd->hiy);
-                const float4 hz = *reinterpret_cast<const float4 *>(nd->hiz);
-                const uint4 ch = *reinterpret_cast<const uint4 *>(nd->child);
-                const uint2 ordw = *reinterpret_cast<const uint2 *>(nd->order);
-                const uint32_t ord =
-                    (oct < 4 ? (ordw.x >> (8 * oct)) : (ordw.y >> (8 * (oct - 4)))) & 0xFFu;
-                float tm[4];
-                bool ok[4];
-                ok[0] = ch.x != kEmpty &&
-                        slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, t, tm[0]);
-                ok[1] = ch.y != kEmpty &&
-                        slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, t, tm[1]);
-                ok[2] = ch.z != kEmpty &&
-                        slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, t, tm[2]);
-                ok[3] = ch.w != kEmpty &&
-                        slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, t, tm[3]);
-                const uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
-                // push in reverse visit order; the first one becomes `cur`
-                bool have = false;
-                uint32_t next = 0;
-                float next_t = 0.0f;
-#pragma unroll
-                for (int j = 3; j >= 0; j--) {
-                    const int s = (ord >> (2 * j)) & 3;
-                    if (!ok[s]) continue;
-                    if (have) {
-                        stk_ref[sp] = next;
-                        stk_t[sp] = next_t;
-                        sp++;
-                    }
-                    next = cref[s];
-                    next_t = tm[s];
-                    have = true;
-                }
-                if (have) { cur = next; continue; }
-            } else {
-                uint32_t k = cur & ~kLeafBit;
-                for (;;) {
-                    const float4 a = tris[3 * (size_t)k];
-                    const float4 b = tris[3 * (size_t)k + 1];
-                    const float4 c = tris[3 * (size_t)k + 2];
-                    if (mt_closest(a, b, c, ox, oy, oz, dx, dy, dz, rtmin, t, bu, bv))
-                        prim = __float_as_uint(a.w);
-                    if (__float_as_uint(b.w) & 1u) break;
-                    k++;
-                }
-            }
-            bool found = false;
-            while (sp > 0) {
-                sp--;
-                if (!(stk_t[sp] > t)) { cur = stk_ref[sp]; found = true; break; }
-            }
-            if (!found) break;
-        }
-    }
-    reinterpret_cast<float4 *>(hits)[i] = make_float4(__uint_as_float(prim), t, bu, bv);
+    trace_closest_dev<kStack>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w, t, prim, bu, bvv);
+    reinterpret_cast<float4 *>(hits)[i] = make_float4(__uint_as_float(prim), t, bu, bvv);
 }
 
-// Any-hit traversal (bvh.h:63-101).  The boolean result does not depend on
-// visit order, so children are visited in the same octant order as above.
 template <int kStack>
-__global__ __launch_bounds__(kBlock) void k_any(const Node4 *__restrict__ nodes,
-                                                const float4 *__restrict__ tris,
-                                                const rg_ray *__restrict__ rays,
-                                                uint8_t *__restrict__ occ, uint64_t n,
-                                                RootBox root, int has_tree, uint32_t pred_bit) {
+__global__ __launch_bounds__(kBlock) void k_any(DevBvh bv, const rg_ray *__restrict__ rays,
+                                                uint8_t *__restrict__ occ, uint64_t n, uint32_t pred_bit) {
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float4 r0 = reinterpret_cast<const float4 *>(rays)[2 * i];
     const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
-    const float ox = r0.x, oy = r0.y, oz = r0.z;
-    const float dx = r1.x, dy = r1.y, dz = r1.z;
-    const float maxt = r1.w;
-    const float ix = inv_dir(dx), iy = inv_dir(dy), iz = inv_dir(dz);
-    const int oct = (ix < 0.0f ? 1 : 0) | (iy < 0.0f ? 2 : 0) | (iz < 0.0f ? 4 : 0);
-    uint8_t hit = 0;
-    float tm_unused;
-    if (has_tree && slab(root.lo[0], root.lo[1], root.lo[2], root.hi[0], root.hi[1], root.hi[2],
-                         ox, oy, oz, ix, iy, iz, maxt, tm_unused)) {
-        uint32_t stk[kStack];
-        int sp = 0;
-        uint32_t cur = 0;
-        for (;;) {
-            if (!(cur & kLeafBit)) {
-                const Node4 *nd = nodes + cur;
-                const float4 lx = *reinterpret_cast<const float4 *>(nd->lox);
-                const float4 ly = *reinterpret_cast<const float4 *>(nd->loy);
-                const float4 lz = *reinterpret_cast<const float4 *>(nd->loz);
-                const float4 hx = *reinterpret_cast<const float4 *>(nd->hix);
-                const float4 hy = *reinterpret_cast<const float4 *>(nd->hiy);
-                const float4 hz = *reinterpret_cast<const float4 *>(nd->hiz);
-                const uint4 ch = *reinterpret_cast<const uint4 *>(nd->child);
-                const uint2 ordw = *reinterpret_cast<const uint2 *>(nd->order);
-                const uint32_t ord =
-                    (oct < 4 ? (ordw.x >> (8 * oct)) : (ordw.y >> (8 * (oct - 4)))) & 0xFFu;
-                float tm[4];
-                bool ok[4];
-                ok[0] = ch.x != kEmpty &&
-                        slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, maxt, tm[0]);
-                ok[1] = ch.y != kEmpty &&
-                        slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, maxt, tm[1]);
-                ok[2] = ch.z != kEmpty &&
-                        slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, maxt, tm[2]);
-                ok[3] = ch.w != kEmpty &&
-                        slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, maxt, tm[3]);
-                const uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
-                bool have = false;
-                uint32_t next = 0;
-#pragma unroll
-                for (int j = 3; j >= 0; j--) {
-                    const int s = (ord >> (2 * j)) & 3;
-                    if (!ok[s]) continue;
-                    if (have) stk[sp++] = next;
-                    next = cref[s];
-                    have = true;
-                }
-                if (have) { cur = next; continue; }
-            } else {
-                uint32_t k = cur & ~kLeafBit;
-                for (;;) {
-                    const float4 a = tris[3 * (size_t)k];
-                    const float4 b = tris[3 * (size_t)k + 1];
-                    const float4 c = tris[3 * (size_t)k + 2];
-                    if ((__float_as_uint(c.w) & pred_bit) &&
-                        mt_any(a, b, c, ox, oy, oz, dx, dy, dz, maxt)) {
-                        hit = 1;
-                        break;
-                    }
-                    if (__float_as_uint(b.w) & 1u) break;
-                    k++;
-                }
-                if (hit) break;
-            }
-            if (sp == 0) break;
-            cur = stk[--sp];
-        }
-    }
-    occ[i] = hit;
+    occ[i] = trace_any_dev<kStack>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, pred_bit) ? 1 : 0;
 }
-
-// ---------------------------------------------------------------------------
-// Ray generation for the traversal microbenchmark: Philox4x32-10
-// (Salmon et al., SC'11), counter = (index lo, index hi, block, 0).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        const uint32_t lo0 = 0xD2511F53u * c[0];
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
-        const uint32_t lo1 = 0xCD9E8D57u * c[2];
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
-        const uint32_t n0 = hi1 ^ c[1] ^ k0;
-        const uint32_t n2 = hi0 ^ c[3] ^ k1;
-        c[0] = n0;
-        c[1] = lo1;
-        c[2] = n2;
-        c[3] = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-}
-
-__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
 
 __global__ __launch_bounds__(256) void k_gen_rays(float cx, float cy, float cz, float ex, float ey,
                                                   float ez, uint32_t key, uint64_t first,
@@ -379,26 +123,27 @@ __global__ __launch_bounds__(256) void k_gen_rays(float cx, float cy, float cz, 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-static RootBox root_of(const Scene &s) {
-    RootBox r;
+DevBvh dev_bvh(const Scene &s) {
+    DevBvh b;
+    b.nodes = (const Node4 *)s.d_nodes4;
+    b.tris = (const float4 *)s.d_tris;
     if (!s.nodes2.empty()) {
-        for (int k = 0; k < 3; k++) { r.lo[k] = s.nodes2[0].bmin[k]; r.hi[k] = s.nodes2[0].bmax[k]; }
+        for (int k = 0; k < 3; k++) { b.root.lo[k] = s.nodes2[0].bmin[k]; b.root.hi[k] = s.nodes2[0].bmax[k]; }
     } else {
-        for (int k = 0; k < 3; k++) { r.lo[k] = 0.0f; r.hi[k] = 0.0f; }
+        for (int k = 0; k < 3; k++) { b.root.lo[k] = 0.0f; b.root.hi[k] = 0.0f; }
     }
-    return r;
+    b.has_tree = s.nodes4.empty() ? 0 : 1;
+    return b;
 }
 
 int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n, void *stream) {
     if (n == 0) return RG_OK;
     const uint64_t blocks = (n + kBlock - 1) / kBlock;
-    const int has = s.nodes4.empty() ? 0 : 1;
+    const DevBvh bv = dev_bvh(s);
     if (s.stack_need <= 64)
-        hipLaunchKernelGGL(k_closest<64>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream,
-                           (const Node4 *)s.d_nodes4, (const float4 *)s.d_tris, rays, hits, n, root_of(s), has);
+        hipLaunchKernelGGL(k_closest<64>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream, bv, rays, hits, n);
     else if (s.stack_need <= 256)
-        hipLaunchKernelGGL(k_closest<256>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream,
-                           (const Node4 *)s.d_nodes4, (const float4 *)s.d_tris, rays, hits, n, root_of(s), has);
+        hipLaunchKernelGGL(k_closest<256>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream, bv, rays, hits, n);
     else
         return set_error(RG_E_UNSUPPORTED, "BVH too deep for the traversal stack");
     return hip_check(hipGetLastError(), "k_closest launch");
@@ -407,14 +152,12 @@ int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n,
 int launch_any(const Scene &s, const rg_ray *rays, uint8_t *occ, uint64_t n, int pred, void *stream) {
     if (n == 0) return RG_OK;
     const uint64_t blocks = (n + kBlock - 1) / kBlock;
-    const int has = s.nodes4.empty() ? 0 : 1;
+    const DevBvh bv = dev_bvh(s);
     const uint32_t bit = 1u << pred;
     if (s.stack_need <= 64)
-        hipLaunchKernelGGL(k_any<64>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream,
-                           (const Node4 *)s.d_nodes4, (const float4 *)s.d_tris, rays, occ, n, root_of(s), has, bit);
+        hipLaunchKernelGGL(k_any<64>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream, bv, rays, occ, n, bit);
     else if (s.stack_need <= 256)
-        hipLaunchKernelGGL(k_any<256>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream,
-                           (const Node4 *)s.d_nodes4, (const float4 *)s.d_tris, rays, occ, n, root_of(s), has, bit);
+        hipLaunchKernelGGL(k_any<256>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream, bv, rays, occ, n, bit);
     else
         return set_error(RG_E_UNSUPPORTED, "BVH too deep for the traversal stack");
     return hip_check(hipGetLastError(), "k_any launch");

@@ -70,6 +70,13 @@ def lib():
             "rg_trace_any_host": (i32, [vp, vp, vp, vp, u64, i32]),
             "rg_gen_icosphere": (i32, [u32, fp, u32, vp]),
             "rg_gen_rays_sphere": (i32, [vp, vp, fp, u32, u64, vp, u64]),
+            "rg_object_default": (None, [vp]),
+            "rg_settings_default": (None, [vp]),
+            "rg_scene_set_shading": (i32, [vp, vp, vp, vp, u32]),
+            "rg_scene_set_media": (i32, [vp, vp, u32, i32]),
+            "rg_scene_set_env": (i32, [vp, vp]),
+            "rg_scene_set_camera": (i32, [vp, vp]),
+            "rg_render": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -206,3 +213,81 @@ def make_rays(o, d, tmin=0.0, tmax=np.inf):
     r["tmin"] = tmin
     r["tmax"] = tmax
     return r
+
+
+# --------------------------------------------------------------------------
+# path tracing
+# --------------------------------------------------------------------------
+class Camera(ctypes.Structure):
+    _fields_ = [("fov_deg", ctypes.c_float), ("exposure", ctypes.c_float), ("depth_of_field", ctypes.c_float),
+                ("aperture", ctypes.c_float), ("aperture_blades", ctypes.c_int32),
+                ("aperture_rotation_deg", ctypes.c_float), ("view_far", ctypes.c_float), ("pad", ctypes.c_float)]
+
+
+class Env(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("rgb", ctypes.c_void_p),
+                ("intensity", ctypes.c_float), ("rotation_deg", ctypes.c_float)]
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+                ("max_depth", ctypes.c_uint32), ("antialias", ctypes.c_int32), ("firefly_clamp", ctypes.c_float),
+                ("seed", ctypes.c_uint32), ("job", ctypes.c_uint32), ("adaptive", ctypes.c_int32),
+                ("adaptive_base", ctypes.c_uint32), ("adaptive_threshold", ctypes.c_float),
+                ("denoise", ctypes.c_int32), ("denoise_levels", ctypes.c_uint32), ("sigma_color", ctypes.c_float),
+                ("sigma_normal", ctypes.c_float), ("sigma_depth", ctypes.c_float),
+                ("denoise_intensity", ctypes.c_float), ("samples_in_flight", ctypes.c_uint32)]
+
+
+def default_settings(**kw):
+    s = Settings()
+    lib().rg_settings_default(ctypes.byref(s))
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise KeyError(k)
+        setattr(s, k, v)
+    return s
+
+
+def make_camera(d):
+    return Camera(d.get("fov_deg", 75.0), d.get("exposure", 1.0), d.get("depth_of_field", 0.0),
+                  d.get("aperture", 1.8), d.get("aperture_blades", 0), d.get("aperture_rotation_deg", 0.0),
+                  d.get("view_far", 50.0), 0.0)
+
+
+class RenderScene(Scene):
+    """A scene dict (see scenes.py) made resident on the device: geometry,
+    BVH, shading, media, environment and camera."""
+
+    def __init__(self, ctx, sc, width=4):
+        super().__init__(ctx, sc["v9"])
+        self.src = sc
+        self._keep = []
+        n9 = np.ascontiguousarray(sc["n9"], np.float32)
+        obj = np.ascontiguousarray(sc["obj"], np.uint32)
+        objs = np.ascontiguousarray(sc["objects"])
+        assert objs.dtype.itemsize == 56
+        media = np.ascontiguousarray(sc.get("media", np.zeros(0, objs.dtype)))
+        self._keep += [n9, obj, objs, media]
+        check(lib().rg_scene_set_media(self.h, _ptr(media) if len(media) else None, len(media),
+                                       int(sc.get("global_medium", -1))))
+        check(lib().rg_scene_set_shading(self.h, _ptr(n9), _ptr(obj), _ptr(objs), len(objs)))
+        env = sc.get("env")
+        if env is not None:
+            rgb = np.ascontiguousarray(env["rgb"], np.float32)
+            self._keep.append(rgb)
+            e = Env(rgb.shape[1], rgb.shape[0], rgb.ctypes.data, float(env.get("intensity", 1.0)),
+                    float(env.get("rotation_deg", 0.0)))
+            check(lib().rg_scene_set_env(self.h, ctypes.byref(e)))
+        cam = make_camera(sc["camera"])
+        check(lib().rg_scene_set_camera(self.h, ctypes.byref(cam)))
+        self.build(width=width)
+
+    def render(self, settings):
+        W, H = settings.width, settings.height
+        hdr = np.zeros((H, W, 4), np.float32)
+        sm = np.zeros((H, W, 3), np.float32)
+        sq = np.zeros((H, W, 3), np.float32)
+        nt = np.zeros(((W + 31) // 32) * ((H + 31) // 32), np.uint32)
+        check(lib().rg_render(self.ctx.h, self.h, ctypes.byref(settings), _ptr(hdr), _ptr(sm), _ptr(sq), _ptr(nt)))
+        return {"hdr": hdr, "sum": sm, "sumsq": sq, "nspp": nt}

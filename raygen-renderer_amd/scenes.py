@@ -1,0 +1,175 @@
+"""Scene generators for the path-tracing workloads, in render (camera) space:
+the camera sits at the origin looking down -Z, as RayRenderer::transformScene
+leaves the world after its view transform (rayrenderer.cpp:157-363).
+
+A scene is a dict of flat arrays (the shape the C ABI takes):
+  v9 (n,9) vertices, n9 (n,9) vertex normals, obj (n,) object index per
+  triangle, objects (structured OBJECT_DTYPE: one Material per SceneObject,
+  material.h:20-70), media (MEDIUM_DTYPE), global_medium, env, camera.
+"""
+import numpy as np
+
+OBJECT_DTYPE = np.dtype([("color", "<f4", (3,)), ("glossy", "<f4"), ("roughness", "<f4"),
+                         ("transparency", "<f4"), ("refraction", "<f4"), ("refraction_ratio", "<f4"),
+                         ("metallic", "<f4"), ("anisotropy", "<f4"), ("aniso_rotation", "<f4"),
+                         ("emission", "<f4"), ("interior_medium", "<i4"), ("visible", "<u4")])
+MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", (3,)), ("sigma_s", "<f4", (3,)), ("sigma_e", "<f4", (3,)),
+                         ("g", "<f4"), ("density", "<f4"), ("pad", "<f4")])
+
+
+def material(color=(0.8, 0.8, 0.8), glossy=0.0, roughness=0.5, transparency=0.0, refraction=0.0,
+             refraction_ratio=1.45, metallic=0.0, anisotropy=0.0, aniso_rotation=0.0, emission=0.0,
+             interior_medium=-1, visible=1):
+    """One object's material with the reference defaults (material.h:26-50)."""
+    m = np.zeros(1, OBJECT_DTYPE)[0]
+    m["color"] = color
+    m["glossy"] = glossy
+    m["roughness"] = roughness
+    m["transparency"] = transparency
+    m["refraction"] = refraction
+    m["refraction_ratio"] = refraction_ratio
+    m["metallic"] = metallic
+    m["anisotropy"] = anisotropy
+    m["aniso_rotation"] = aniso_rotation
+    m["emission"] = emission
+    m["interior_medium"] = interior_medium
+    m["visible"] = visible
+    return m
+
+
+class Builder:
+    def __init__(self):
+        self.v, self.n, self.o, self.objects = [], [], [], []
+
+    def add_object(self, mat):
+        self.objects.append(mat)
+        return len(self.objects) - 1
+
+    def quad(self, obj, p0, p1, p2, p3, normal=None):
+        """Two triangles (p0,p1,p2), (p0,p2,p3); flat normal from the winding
+        unless given."""
+        p0, p1, p2, p3 = (np.asarray(p, np.float32) for p in (p0, p1, p2, p3))
+        if normal is None:
+            normal = np.cross(p1 - p0, p2 - p0)
+            normal = normal / np.linalg.norm(normal)
+        nn = np.tile(np.asarray(normal, np.float32), 3)
+        for tri in ((p0, p1, p2), (p0, p2, p3)):
+            self.v.append(np.concatenate(tri))
+            self.n.append(nn)
+            self.o.append(obj)
+
+    def box(self, obj, lo, hi):
+        """Axis-aligned box, 12 triangles, outward flat normals (polygons.cpp:61-130 layout)."""
+        x0, y0, z0 = lo
+        x1, y1, z1 = hi
+        self.quad(obj, (x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1))  # +z
+        self.quad(obj, (x1, y0, z0), (x0, y0, z0), (x0, y1, z0), (x1, y1, z0))  # -z
+        self.quad(obj, (x1, y0, z1), (x1, y0, z0), (x1, y1, z0), (x1, y1, z1))  # +x
+        self.quad(obj, (x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0))  # -x
+        self.quad(obj, (x0, y1, z1), (x1, y1, z1), (x1, y1, z0), (x0, y1, z0))  # +y
+        self.quad(obj, (x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1))  # -y
+
+    def sphere(self, obj, center, radius, stacks, slices):
+        """SphereMesh (polygons.cpp:132-180): (stacks+1)(slices+1) vertices,
+        phi = i/stacks * pi, theta = j/slices * 2pi, triangles (cur, next,
+        cur+1) and (cur+1, next, next+1); smooth normals; 2*stacks*slices
+        triangles (pole rows are degenerate, as in the reference)."""
+        c = np.asarray(center, np.float32)
+        i = np.arange(stacks + 1, dtype=np.float32)[:, None]
+        j = np.arange(slices + 1, dtype=np.float32)[None, :]
+        phi = ((i / np.float32(stacks)).astype(np.float64) * np.pi).astype(np.float32)
+        theta = ((j / np.float32(slices) * np.float32(2.0)).astype(np.float64) * np.pi).astype(np.float32)
+        x = np.sin(phi) * np.cos(theta)
+        y = np.cos(phi) * np.ones_like(theta)
+        z = np.sin(phi) * np.sin(theta)
+        nrm = np.stack([x, y, z], axis=-1).reshape(-1, 3).astype(np.float32)
+        pos = (nrm * np.float32(radius) + c).astype(np.float32)
+        ii, jj = np.meshgrid(np.arange(stacks), np.arange(slices), indexing="ij")
+        cur = (ii * (slices + 1) + jj).ravel()
+        nxt = ((ii + 1) * (slices + 1) + jj).ravel()
+        idx = np.stack([np.stack([cur, nxt, cur + 1], 1), np.stack([cur + 1, nxt, nxt + 1], 1)], 1).reshape(-1, 3)
+        self.v.append(pos[idx].reshape(-1, 9))
+        self.n.append(nrm[idx].reshape(-1, 9))
+        self.o.append(np.full(len(idx), obj, np.uint32))
+
+    def finish(self, camera, media=None, global_medium=-1, env=None):
+        v = [np.atleast_2d(x) for x in self.v]
+        n = [np.atleast_2d(x) for x in self.n]
+        o = [np.atleast_1d(np.asarray(x, np.uint32)) for x in self.o]
+        scene = {
+            "v9": np.ascontiguousarray(np.concatenate(v), np.float32),
+            "n9": np.ascontiguousarray(np.concatenate(n), np.float32),
+            "obj": np.ascontiguousarray(np.concatenate(o), np.uint32),
+            "objects": np.array(self.objects, dtype=OBJECT_DTYPE),
+            "media": np.array(media if media is not None else [], dtype=MEDIUM_DTYPE),
+            "global_medium": global_medium,
+            "camera": camera,
+        }
+        if env is not None:
+            scene["env"] = env
+        return scene
+
+
+def cube_room():
+    """Plain cube room: a 2x2x2 box open toward the camera (5 planes, 10
+    triangles; floor/ceiling/back 0.75 grey, left red, right green), a tall
+    and a short axis-aligned inner box (24 triangles, albedo 0.7) and one
+    0.5x0.5 emissive ceiling quad (radiance 15) facing -Y: 36 triangles."""
+    b = Builder()
+    z0, z1 = -4.6, -2.6
+    grey = material(color=(0.75, 0.75, 0.75))
+    floor = b.add_object(grey)
+    b.quad(floor, (-1, -1, z1), (1, -1, z1), (1, -1, z0), (-1, -1, z0), normal=(0, 1, 0))
+    ceil = b.add_object(grey)
+    b.quad(ceil, (-1, 1, z0), (1, 1, z0), (1, 1, z1), (-1, 1, z1), normal=(0, -1, 0))
+    back = b.add_object(grey)
+    b.quad(back, (-1, -1, z0), (1, -1, z0), (1, 1, z0), (-1, 1, z0), normal=(0, 0, 1))
+    left = b.add_object(material(color=(0.75, 0.25, 0.25)))
+    b.quad(left, (-1, -1, z1), (-1, -1, z0), (-1, 1, z0), (-1, 1, z1), normal=(1, 0, 0))
+    right = b.add_object(material(color=(0.25, 0.75, 0.25)))
+    b.quad(right, (1, -1, z0), (1, -1, z1), (1, 1, z1), (1, 1, z0), normal=(-1, 0, 0))
+    tall = b.add_object(material(color=(0.7, 0.7, 0.7)))
+    b.box(tall, (-0.65, -1.0, -4.2), (-0.1, 0.2, -3.65))
+    short = b.add_object(material(color=(0.7, 0.7, 0.7)))
+    b.box(short, (0.1, -1.0, -3.5), (0.65, -0.4, -2.95))
+    light = b.add_object(material(color=(1.0, 1.0, 1.0), emission=15.0))
+    b.quad(light, (-0.25, 0.999, -3.85), (0.25, 0.999, -3.85), (0.25, 0.999, -3.35), (-0.25, 0.999, -3.35),
+           normal=(0, -1, 0))
+    return b.finish(camera={"fov_deg": 45.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0})
+
+
+def constant_env(value=0.3, w=64, h=32):
+    """A uniform equirectangular environment (the "constant environment")."""
+    return {"rgb": np.full((h, w, 3), value, np.float32), "intensity": 1.0, "rotation_deg": 0.0}
+
+
+def bsdf_frame(grid=5, stacks=200, slices=200, seed=0xB5DF0002, env_value=0.3):
+    """Full BSDF frame: grid x grid tessellated spheres (2*stacks*slices
+    triangles each; 5x5 at 200x200 = 2,000,000), seeded GGX materials with
+    roughness U[0.05, 1], glossy U[0, 1] and refraction 0 or 0.9 at IOR 1.5,
+    one area-light quad, a constant environment and one homogeneous medium box
+    (sigma_a 0.1, sigma_s 0.4, in a transparency-1 cube)."""
+    rng = np.random.default_rng(seed)
+    b = Builder()
+    span = 4.0
+    r = 0.42 * span / max(grid, 1)
+    for gy in range(grid):
+        for gx in range(grid):
+            cx = (gx - (grid - 1) / 2) * span / max(grid - 1, 1) if grid > 1 else 0.0
+            cy = (gy - (grid - 1) / 2) * span / max(grid - 1, 1) if grid > 1 else 0.0
+            mat = material(color=tuple(rng.uniform(0.3, 0.9, 3)), roughness=float(rng.uniform(0.05, 1.0)),
+                           glossy=float(rng.uniform(0.0, 1.0)), refraction=float(0.9 if rng.random() < 0.5 else 0.0),
+                           refraction_ratio=1.5)
+            o = b.add_object(mat)
+            b.sphere(o, (cx, cy, -6.0), r, stacks, slices)
+    light = b.add_object(material(color=(1.0, 1.0, 1.0), emission=8.0))
+    b.quad(light, (-1.0, 3.2, -7.0), (1.0, 3.2, -7.0), (1.0, 3.2, -5.0), (-1.0, 3.2, -5.0), normal=(0, -1, 0))
+    box = b.add_object(material(color=(1.0, 1.0, 1.0), transparency=1.0, interior_medium=0))
+    b.box(box, (-0.8, -1.0, -4.6), (0.8, 0.6, -3.0))
+    med = np.zeros(1, MEDIUM_DTYPE)
+    med[0]["sigma_a"] = (0.1, 0.1, 0.1)
+    med[0]["sigma_s"] = (0.4, 0.4, 0.4)
+    med[0]["density"] = 1.0
+    return b.finish(camera={"fov_deg": 45.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0,
+                            "view_far": 50.0},
+                    media=med, env=constant_env(env_value))
