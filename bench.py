@@ -19,6 +19,7 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "raygen-renderer_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 PEAK_HBM_GBS = 8000.0
 N_RAYS = 1 << 24
@@ -61,6 +62,93 @@ def cpu_baseline(v9, rays_np, seconds, threads):
     }, pops / done, tests / done
 
 
+def bench_cube_room(ctx, stream, steps, cpu_seconds, do_cpu):
+    """Plain cube room (36 triangles), 256x256, 16 spp, depth 5, NEE+MIS, AA,
+    no denoise: frame time, samples/s and rays/s of the whole render call."""
+    import numpy as np
+    import torch
+
+    import rgapi
+    import scenes
+
+    sc = scenes.cube_room()
+    rs = rgapi.RenderScene(ctx, sc)
+    st = rgapi.default_settings(width=256, height=256, spp=16, max_depth=5, antialias=1, seed=0xC0DE0000)
+    out = rs.render(st)  # warm-up (buffers, upload)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = rs.render(st)
+    wall = (time.perf_counter() - t0) / steps
+    res = {"workload": "cube room 36 tris, 256x256, 16 spp, depth 5, NEE+MIS, AA, no denoise",
+           "ms_per_frame": round(wall * 1e3, 3),
+           "samples_per_s": round(out["samples"] / wall, 1),
+           "mrays_per_s": round((out["rays"] + out["shadow_rays"]) / wall / 1e6, 2),
+           "rays_per_frame": out["rays"], "shadow_rays_per_frame": out["shadow_rays"],
+           "mean_radiance": [round(float(x), 5) for x in out["hdr"][..., :3].mean((0, 1))]}
+    if do_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = max(1, min(16, os.cpu_count() or 1))
+        spp, t, done = 1, 0.0, 0
+        while t < cpu_seconds and done < 16:
+            t0 = time.perf_counter()
+            oracle.render(sc, 256, 256, spp, 5, mode=0, threads=threads, sample_start=done)
+            t += time.perf_counter() - t0
+            done += spp
+        res["cpu_baseline"] = {"value": round(256 * 256 * done / t, 1), "unit": "samples/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{done} spp of the 256x256 frame, recursive restatement of the "
+                                         f"reference shader (all Mix lobes), {threads} threads"}
+    return res
+
+
+def bench_bsdf_frame(ctx, do_cpu, cpu_seconds):
+    """Full BSDF frame: 5x5 spheres x 80k triangles, GGX/refraction, area
+    light + constant environment, homogeneous medium box, 1920x1080, 64 spp,
+    depth 8, adaptive (threshold 1e-3), 5-level A-Trous."""
+    import numpy as np
+    import torch
+
+    import rgapi
+    import scenes
+
+    t0 = time.perf_counter()
+    sc = scenes.bsdf_frame(grid=5, stacks=200, slices=200)
+    t1 = time.perf_counter()
+    rs = rgapi.RenderScene(ctx, sc)
+    t2 = time.perf_counter()
+    st = rgapi.default_settings(width=1920, height=1080, spp=64, max_depth=8, adaptive=1, adaptive_base=4,
+                                adaptive_threshold=1e-3, denoise=1, denoise_levels=5, seed=0xB5DF0002)
+    warm = rgapi.default_settings(width=1920, height=1080, spp=1, max_depth=8)
+    rs.render(warm)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    out = rs.render(st)
+    wall = time.perf_counter() - t3
+    res = {"workload": "BSDF frame: 25 spheres x 80k tris (2,000,014 tris), GGX/refraction/medium/env, "
+                       "1920x1080, 64 spp, depth 8, adaptive 1e-3, 5-level A-Trous",
+           "tris": int(len(sc["v9"])),
+           "frame_s": round(wall, 4),
+           "samples": out["samples"],
+           "samples_per_s": round(out["samples"] / wall, 1),
+           "mrays_per_s": round((out["rays"] + out["shadow_rays"]) / wall / 1e6, 2),
+           "mean_spp": round(float(out["samples"]) / (1920 * 1080), 2),
+           "scene_gen_s": round(t1 - t0, 2), "bvh_build_upload_s": round(t2 - t1, 2)}
+    if do_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = max(1, min(16, os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        oracle.render(sc, 192, 108, 2, 8, mode=0, threads=threads)
+        t = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(192 * 108 * 2 / t, 1), "unit": "samples/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"192x108 at 2 spp of the same scene (includes the oracle's own BVH "
+                                         f"build), recursive restatement, {threads} threads"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,10 +156,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-render", action="store_true", help="traversal microbenchmark only")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; the path would shard rays/tiles "
-              "across GPUs with a replicated BVH (see DESIGN.md), which is not built.", file=sys.stderr)
+        print("bench.py: this build runs on one GPU only (multi-GPU sharding of rays / tiles / jobs "
+              "with a replicated BVH is designed but not built).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -151,6 +242,19 @@ def main():
                 "tri_tests_per_ray": round(tests, 3), "kernel": "k_closest",
                 "kernel_avg_ms": round(avg_ms, 4)}
 
+    extra = {}
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        extra["copy_calibration_bytes"] = 1 << 30
+    if not args.skip_render:
+        extra["cube_room"] = bench_cube_room(ctx, stream, max(3, args.steps // 2), 8.0, not args.no_cpu_baseline)
+        extra["bsdf_frame"] = bench_bsdf_frame(ctx, not args.no_cpu_baseline, 8.0)
+
     out = {
         "metric": "closest-hit traversal throughput, 1.31M-tri displaced icosphere, 16M incoherent rays",
         "value": round(mrays, 2),
@@ -176,6 +280,7 @@ def main():
         "mesh_gen_s": round(gen_s, 3),
         "scene_device_mb": round(sc.device_bytes() / 2**20, 1),
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

@@ -214,6 +214,10 @@ int rg_scene_set_camera(rg_scene *scene, const rg_camera *cam);
 int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *hdr_rgba,
               float *sum_rgb, float *sumsq_rgb, uint32_t *nspp);
 
+/* Counters of the last rg_render on this context: [0] extension (closest-hit)
+ * rays, [1] NEE shadow (any-hit) rays, [2] path samples. */
+int rg_render_stats(rg_ctx *ctx, uint64_t out[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,7 @@ struct PTBuffers {
     void *sum = nullptr, *sumsq = nullptr, *hdr = nullptr;
     void *tile_mask = nullptr, *tile_nspp = nullptr, *tile_noise = nullptr, *n_active = nullptr;
     void *aov = nullptr, *dn_a = nullptr, *dn_b = nullptr;
+    void *stats = nullptr;
 };
 
 struct Ctx {
@@ -80,6 +81,7 @@ struct Ctx {
     void *own_stream = nullptr;    // stream created by rg_create
     std::atomic<bool> cancel{false};
     PTBuffers pt;
+    uint64_t last_stats[3] = {0, 0, 0};
 };
 
 struct Scene {

@@ -88,6 +88,7 @@ struct PTState {
     uint32_t *q[2];     // live-path queues
     uint32_t *count;    // [2] queue lengths
     float4 *sample_out; // clamped sample radiance per path
+    unsigned long long *stats; // [0] extension rays, [1] shadow rays, [2] path samples
     uint32_t sample_base;
 };
 

@@ -992,12 +992,17 @@ __global__ __launch_bounds__(kPtBlock) void k_camera(PTScene sc, PTState st, uin
         }
     }
     wave_append(active, i, st.q[0], st.count + 0);
+    const uint64_t m = __ballot(active ? 1 : 0);
+    if (m && __lane_id() == (uint32_t)__ffsll((unsigned long long)m) - 1)
+        atomicAdd(st.stats + 2, (unsigned long long)__popcll(m));
 }
 
 template <int kStack>
 __global__ __launch_bounds__(kPtBlock) void k_extend(DevBvh bv, PTState st, int qi) {
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    if (i >= st.count[qi]) return;
+    const uint32_t count = st.count[qi];
+    if (i == 0 && count) atomicAdd(st.stats + 0, (unsigned long long)count);
+    if (i >= count) return;
     const uint32_t p = st.q[qi][i];
     const rg_ray r = st.ray[p];
     float t = r.tmax, bu = 0.0f, bvv = 0.0f;
@@ -1049,10 +1054,11 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(DevBvh bv, PTState st, int 
     const uint32_t count = st.count[qi];
     bool alive = false;
     uint32_t p = 0;
+    uint32_t ns = 0;
     if (i < count) {
         p = st.q[qi][i];
         float4 L = st.L[p];
-        const uint32_t ns = st.nshadow[p];
+        ns = st.nshadow[p];
         for (uint32_t k = 0; k < ns; k++) {
             const rg_ray r = st.sray[2 * p + k];
             const uint32_t pred_bit = 1u << (int)r.tmin;
@@ -1077,6 +1083,10 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(DevBvh bv, PTState st, int 
         }
     }
     wave_append(alive, p, st.q[qi ^ 1], st.count + (qi ^ 1));
+    // wave-reduced shadow-ray count
+    uint32_t tot = ns;
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_down(tot, off);
+    if (__lane_id() == 0 && tot) atomicAdd(st.stats + 1, (unsigned long long)tot);
 }
 
 // Paths still alive after the last bounce (cannot happen: every event

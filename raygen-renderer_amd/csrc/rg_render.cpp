@@ -158,7 +158,7 @@ int reupload_tris(Scene *s, void *stream) {
 void free_buffers(PTBuffers &b) {
     for (void **p : {&b.ray, &b.hit, &b.beta, &b.tp, &b.L, &b.meta, &b.sray, &b.scontrib, &b.nshadow, &b.q0,
                      &b.q1, &b.count, &b.sample_out, &b.sum, &b.sumsq, &b.hdr, &b.tile_mask, &b.tile_nspp,
-                     &b.tile_noise, &b.n_active, &b.aov, &b.dn_a, &b.dn_b}) {
+                     &b.tile_noise, &b.n_active, &b.aov, &b.dn_a, &b.dn_b, &b.stats}) {
         dev_free(*p);
         *p = nullptr;
     }
@@ -180,7 +180,7 @@ int ensure_buffers(PTBuffers &b, uint64_t cap, uint64_t npix, uint64_t ntiles) {
         {&b.q0, cap * 4}, {&b.q1, cap * 4}, {&b.count, 64}, {&b.sample_out, cap * 16}, {&b.sum, npix * 12},
         {&b.sumsq, npix * 12}, {&b.hdr, npix * 16}, {&b.tile_mask, ntiles}, {&b.tile_nspp, ntiles * 4},
         {&b.tile_noise, ntiles * 4}, {&b.n_active, 64}, {&b.aov, npix * 36}, {&b.dn_a, npix * 16},
-        {&b.dn_b, npix * 16},
+        {&b.dn_b, npix * 16}, {&b.stats, 64},
     };
     for (auto &a : allocs)
         if (int rc = dev_malloc(a.p, a.bytes)) {
@@ -386,6 +386,7 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     ps.q[1] = (uint32_t *)B.q1;
     ps.count = (uint32_t *)B.count;
     ps.sample_out = (float4 *)B.sample_out;
+    ps.stats = (unsigned long long *)B.stats;
     ps.sample_base = 0;
 
     float *d_sum = (float *)B.sum, *d_sumsq = (float *)B.sumsq;
@@ -394,6 +395,7 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     if (int rc = launch_fill_u32(d_sum, 0u, (uint64_t)npix * 3, stream)) return rc;
     if (int rc = launch_fill_u32(d_sumsq, 0u, (uint64_t)npix * 3, stream)) return rc;
     if (int rc = launch_fill_u32(d_nspp, 0u, ntiles, stream)) return rc;
+    if (int rc = launch_fill_u32(B.stats, 0u, 6, stream)) return rc;
 
     auto run_samples = [&](uint32_t first, uint32_t count, const uint8_t *mask) -> int {
         uint32_t done = 0;
@@ -445,7 +447,15 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
         if (int rc = dev_download(sumsq_rgb, d_sumsq, (size_t)npix * 12, stream)) return rc;
     if (nspp)
         if (int rc = dev_download(nspp, d_nspp, (size_t)ntiles * 4, stream)) return rc;
+    if (int rc = dev_download(c->last_stats, B.stats, sizeof c->last_stats, stream)) return rc;
     return hip_ok(hipStreamSynchronize((hipStream_t)stream), "render sync");
+}
+
+int rg_render_stats(rg_ctx *ctx, uint64_t out[3]) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c || !out) return set_error(RG_E_INVALID, "null argument");
+    for (int k = 0; k < 3; k++) out[k] = c->last_stats[k];
+    return RG_OK;
 }
 
 }  // extern "C"

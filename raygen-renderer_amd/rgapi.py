@@ -77,6 +77,7 @@ def lib():
             "rg_scene_set_env": (i32, [vp, vp]),
             "rg_scene_set_camera": (i32, [vp, vp]),
             "rg_render": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+            "rg_render_stats": (i32, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -290,4 +291,7 @@ class RenderScene(Scene):
         sq = np.zeros((H, W, 3), np.float32)
         nt = np.zeros(((W + 31) // 32) * ((H + 31) // 32), np.uint32)
         check(lib().rg_render(self.ctx.h, self.h, ctypes.byref(settings), _ptr(hdr), _ptr(sm), _ptr(sq), _ptr(nt)))
-        return {"hdr": hdr, "sum": sm, "sumsq": sq, "nspp": nt}
+        stats = np.zeros(3, np.uint64)
+        check(lib().rg_render_stats(self.ctx.h, _ptr(stats)))
+        return {"hdr": hdr, "sum": sm, "sumsq": sq, "nspp": nt,
+                "rays": int(stats[0]), "shadow_rays": int(stats[1]), "samples": int(stats[2])}
