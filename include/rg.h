@@ -214,6 +214,31 @@ int rg_scene_set_camera(rg_scene *scene, const rg_camera *cam);
 int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *hdr_rgba,
               float *sum_rgb, float *sumsq_rgb, uint32_t *nspp);
 
+/* ---- scene ingest ------------------------------------------------------- */
+/* Replaces RendererSceneLoader::load + RayRenderer::transformScene
+ * (sceneloader.cpp:847-882, 505-783; rayrenderer.cpp:157-363): reads the
+ * reference's scene JSON (comments and trailing commas allowed) with its
+ * objects, location/angle/scale transforms, plane/cube/sphere primitives,
+ * binary .mesh files, inline and `_materials` materials, object and global
+ * `medium`, `mainCamera` and an .hdr `envmap`, and returns a scene in render
+ * space with shading, media, environment and camera set (BVH not built).
+ * Parts outside this path (OBJ, .toba, textures, cubemaps, non-homogeneous
+ * media) are skipped with a warning (rg_scene_warnings). */
+int rg_scene_load_json(rg_ctx *ctx, const char *path, rg_scene **out);
+/* Newline-separated warnings of the load into buf (capacity cap); returns the
+ * full length. */
+int rg_scene_warnings(const rg_scene *scene, char *buf, uint64_t cap);
+int rg_scene_get_camera(const rg_scene *scene, rg_camera *cam);
+/* Copies the flattened scene out: n_tris and n_obj first (any array
+ * argument may be NULL), then v9/n9 (n_tris*9 floats), obj_of_tri, objs. */
+int rg_scene_export(const rg_scene *scene, float *v9, float *n9, uint32_t *obj_of_tri, rg_object *objs,
+                    uint32_t *n_tris, uint32_t *n_obj);
+/* Copies the media table and environment out: n_media, global_medium and env
+ * (env->rgb is left NULL) first, then media (n_media records) and env_rgb
+ * (env->width * env->height * 3 floats) where non-NULL. */
+int rg_scene_export_media(const rg_scene *scene, rg_medium *media, uint32_t *n_media, int32_t *global_medium,
+                          rg_env *env, float *env_rgb);
+
 /* Counters of the last rg_render on this context: [0] extension (closest-hit)
  * rays, [1] NEE shadow (any-hit) rays, [2] path samples. */
 int rg_render_stats(rg_ctx *ctx, uint64_t out[3]);

@@ -127,6 +127,9 @@ struct Scene {
     void *d_env_cond = nullptr;
     uint32_t nlights = 0;
     float env_total = 0.0f;
+    // loader diagnostics
+    std::vector<std::string> warnings;
+    bool has_camera = false;
 };
 
 // error reporting (thread-local text)

@@ -1,0 +1,761 @@
+// rg_scene_json.cpp — host-side scene ingest behind rg_scene_load_json:
+// the reference's scene JSON (SceneJsonLoader, sceneloader.cpp:101-165,
+// 202-334, 505-783, 847-882), its primitive meshes (polygons.cpp:13-180),
+// the binary .mesh format (meshloader.cpp:25-137), Radiance .hdr textures for
+// the environment (texture.cpp:74-151) and the flattening of the object tree
+// into render-space triangles (RayRenderer::transformScene / transformObject,
+// rayrenderer.cpp:157-363; SceneTransformStack, renderer.cpp:24-73).
+//
+// Transform convention: a point is scaled, rotated (Euler angles in degrees,
+// about X, then Y, then Z) and translated by its object, then by each
+// parent; the camera's inverse (translate by -location, rotate by -angle)
+// maps world to render space, where the eye sits at the origin looking down
+// -Z.  The reference delegates these to its ugm Matrix4, which is not
+// vendored, so the Euler order is the one documented here.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "rg_internal.h"
+#include "rg_json.h"
+
+namespace rg {
+
+// ---------------------------------------------------------------------------
+// affine transforms (double precision, rounded to binary32 on output)
+// ---------------------------------------------------------------------------
+struct Affine {
+    double m[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double t[3] = {0, 0, 0};
+
+    // this ∘ o : apply o first, then this
+    Affine then_after(const Affine &o) const {
+        Affine r;
+        for (int i = 0; i < 3; i++) {
+            for (int j = 0; j < 3; j++) r.m[i][j] = m[i][0] * o.m[0][j] + m[i][1] * o.m[1][j] + m[i][2] * o.m[2][j];
+            r.t[i] = m[i][0] * o.t[0] + m[i][1] * o.t[1] + m[i][2] * o.t[2] + t[i];
+        }
+        return r;
+    }
+    void apply(const float p[3], double out[3]) const {
+        for (int i = 0; i < 3; i++) out[i] = m[i][0] * p[0] + m[i][1] * p[1] + m[i][2] * p[2] + t[i];
+    }
+    // normal transform: inverse transpose of the linear part
+    bool normal_matrix(double n[3][3]) const {
+        const double a = m[0][0], b = m[0][1], c = m[0][2], d = m[1][0], e = m[1][1], f = m[1][2], g = m[2][0],
+                     h = m[2][1], k = m[2][2];
+        const double det = a * (e * k - f * h) - b * (d * k - f * g) + c * (d * h - e * g);
+        if (det == 0.0) return false;
+        const double inv[3][3] = {{(e * k - f * h) / det, (c * h - b * k) / det, (b * f - c * e) / det},
+                                  {(f * g - d * k) / det, (a * k - c * g) / det, (c * d - a * f) / det},
+                                  {(d * h - e * g) / det, (b * g - a * h) / det, (a * e - b * d) / det}};
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) n[i][j] = inv[j][i];
+        return true;
+    }
+};
+
+static Affine translation(const double v[3]) {
+    Affine a;
+    for (int i = 0; i < 3; i++) a.t[i] = v[i];
+    return a;
+}
+static Affine scaling(const double v[3]) {
+    Affine a;
+    for (int i = 0; i < 3; i++) a.m[i][i] = v[i];
+    return a;
+}
+static Affine rot_axis(int axis, double deg) {
+    const double r = deg * M_PI / 180.0, c = std::cos(r), s = std::sin(r);
+    Affine a;
+    const int i = (axis + 1) % 3, j = (axis + 2) % 3;
+    a.m[i][i] = c;
+    a.m[i][j] = -s;
+    a.m[j][i] = s;
+    a.m[j][j] = c;
+    return a;
+}
+static Affine euler(const double deg[3]) {
+    // X first, then Y, then Z
+    return rot_axis(2, deg[2]).then_after(rot_axis(1, deg[1]).then_after(rot_axis(0, deg[0])));
+}
+
+// ---------------------------------------------------------------------------
+// meshes
+// ---------------------------------------------------------------------------
+struct MeshData {
+    std::vector<float> v;   // triangle soup: 9 floats per triangle
+    std::vector<float> n;   // 9 floats per triangle
+};
+
+static void push_tri(MeshData &m, const float *a, const float *b, const float *c, const float *na, const float *nb,
+                     const float *nc) {
+    m.v.insert(m.v.end(), a, a + 3);
+    m.v.insert(m.v.end(), b, b + 3);
+    m.v.insert(m.v.end(), c, c + 3);
+    m.n.insert(m.n.end(), na, na + 3);
+    m.n.insert(m.n.end(), nb, nb + 3);
+    m.n.insert(m.n.end(), nc, nc + 3);
+}
+
+// PlaneMesh (polygons.cpp:13-59): unit plane in XZ, normal +Y
+static MeshData plane_mesh() {
+    MeshData m;
+    const float v[6][3] = {{-0.5f, 0, -0.5f}, {-0.5f, 0, 0.5f}, {0.5f, 0, -0.5f},
+                           {-0.5f, 0, 0.5f},  {0.5f, 0, 0.5f},  {0.5f, 0, -0.5f}};
+    const float n[3] = {0, 1, 0};
+    push_tri(m, v[0], v[1], v[2], n, n, n);
+    push_tri(m, v[3], v[4], v[5], n, n, n);
+    return m;
+}
+
+// CubeMesh (polygons.cpp:61-130): unit cube, flat face normals
+static MeshData cube_mesh() {
+    MeshData m;
+    const float p[8][3] = {{-0.5f, -0.5f, -0.5f}, {0.5f, -0.5f, -0.5f}, {0.5f, 0.5f, -0.5f}, {-0.5f, 0.5f, -0.5f},
+                           {-0.5f, -0.5f, 0.5f},  {0.5f, -0.5f, 0.5f},  {0.5f, 0.5f, 0.5f},  {-0.5f, 0.5f, 0.5f}};
+    const int f[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 4, 7, 3}, {1, 5, 6, 2}, {3, 2, 6, 7}, {0, 1, 5, 4}};
+    const float nn[6][3] = {{0, 0, -1}, {0, 0, 1}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
+    for (int i = 0; i < 6; i++) {
+        push_tri(m, p[f[i][0]], p[f[i][1]], p[f[i][2]], nn[i], nn[i], nn[i]);
+        push_tri(m, p[f[i][0]], p[f[i][2]], p[f[i][3]], nn[i], nn[i], nn[i]);
+    }
+    return m;
+}
+
+// SphereMesh (polygons.cpp:132-180)
+static MeshData sphere_mesh(float radius, int stacks, int slices) {
+    MeshData m;
+    std::vector<float> pos, nrm;
+    for (int i = 0; i <= stacks; ++i) {
+        const float v = (float)i / stacks;
+        const float phi = (float)(v * M_PI);
+        for (int j = 0; j <= slices; ++j) {
+            const float u = (float)j / slices;
+            const float theta = (float)(u * 2.0f * M_PI);
+            const float x = sinf(phi) * cosf(theta), y = cosf(phi), z = sinf(phi) * sinf(theta);
+            pos.insert(pos.end(), {x * radius, y * radius, z * radius});
+            nrm.insert(nrm.end(), {x, y, z});
+        }
+    }
+    for (int i = 0; i < stacks; ++i)
+        for (int j = 0; j < slices; ++j) {
+            const int cur = i * (slices + 1) + j, next = (i + 1) * (slices + 1) + j;
+            const int tri[2][3] = {{cur, next, cur + 1}, {cur + 1, next, next + 1}};
+            for (auto &t : tri)
+                push_tri(m, &pos[3 * t[0]], &pos[3 * t[1]], &pos[3 * t[2]], &nrm[3 * t[0]], &nrm[3 * t[1]],
+                         &nrm[3 * t[2]]);
+        }
+    return m;
+}
+
+static bool read_file(const std::string &path, std::string &out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    out = ss.str();
+    return true;
+}
+
+// Binary .mesh (meshloader.cpp:25-137): header {tag 'mesh', ver, flags,
+// length}; meta; then vertices, normals, texcoords, tangents, colors,
+// indexes.  Non-indexed meshes list 3 vertices per triangle.
+static bool load_mesh_file(const std::string &path, MeshData &m, std::string &err) {
+    std::string buf;
+    if (!read_file(path, buf)) {
+        err = "cannot read " + path;
+        return false;
+    }
+    const uint8_t *p = (const uint8_t *)buf.data();
+    const size_t sz = buf.size();
+    auto rd = [&](size_t off, void *dst, size_t n) -> bool {
+        if (off + n > sz) return false;
+        std::memcpy(dst, p + off, n);
+        return true;
+    };
+    uint32_t tag = 0, length = 0;
+    uint16_t ver = 0, flags = 0;
+    uint32_t vcount = 0, uvcount = 0, icount = 0;
+    bool has_n = false, has_t = false, has_tb = false, has_c = false;
+    size_t off = 0;
+    if (!rd(0, &tag, 4)) {
+        err = path + ": truncated";
+        return false;
+    }
+    if (tag == 0x6873656du) {
+        rd(4, &ver, 2);
+        rd(6, &flags, 2);
+        rd(8, &length, 4);
+        uint32_t meta[5] = {0, 0, 0, 0, 0};
+        if (!rd(12, meta, sizeof meta)) {
+            err = path + ": truncated header";
+            return false;
+        }
+        if (ver < 0x0102) {
+            // vertexCount, normalCount, uvCount, texcoordCount, indexCount
+            vcount = meta[0];
+            uvcount = meta[2];
+            icount = meta[4];
+            has_n = meta[1] > 0;
+            has_t = meta[3] > 0;
+            has_tb = (flags & 0x2) != 0;
+        } else {
+            vcount = meta[0];
+            uvcount = meta[1];
+            icount = meta[2];
+            has_n = (flags & 0x2) != 0;
+            has_t = (flags & 0x4) != 0;
+            has_tb = (flags & 0x10) != 0;
+            has_c = ver >= 0x0103 && (flags & 0x20) != 0;
+        }
+        off = length;
+    } else {
+        uint32_t old[3];
+        if (!rd(0, old, sizeof old)) {
+            err = path + ": truncated";
+            return false;
+        }
+        vcount = old[0];
+        uvcount = 1;
+        icount = 0;
+        off = sizeof old;
+    }
+    if (!has_t) uvcount = 0;
+    std::vector<float> verts(3 * (size_t)vcount), norms;
+    if (!rd(off, verts.data(), verts.size() * 4)) {
+        err = path + ": truncated vertices";
+        return false;
+    }
+    off += verts.size() * 4;
+    if (has_n) {
+        norms.resize(3 * (size_t)vcount);
+        if (!rd(off, norms.data(), norms.size() * 4)) {
+            err = path + ": truncated normals";
+            return false;
+        }
+        off += norms.size() * 4;
+    }
+    if (has_t && uvcount > 0) off += 8 * (size_t)vcount * uvcount;
+    if (has_tb) off += 24 * (size_t)vcount;
+    if (has_c) off += 12 * (size_t)vcount;
+    std::vector<uint32_t> idx(icount);
+    if (icount && !rd(off, idx.data(), idx.size() * 4)) {
+        err = path + ": truncated indexes";
+        return false;
+    }
+    const size_t ntri = icount > 0 ? icount / 3 : vcount / 3;
+    const float zero[3] = {0, 0, 0};
+    for (size_t k = 0; k < ntri; k++) {
+        uint32_t i[3];
+        for (int j = 0; j < 3; j++) i[j] = icount > 0 ? idx[3 * k + j] : (uint32_t)(3 * k + j);
+        for (int j = 0; j < 3; j++)
+            if (i[j] >= vcount) {
+                err = path + ": index out of range";
+                return false;
+            }
+        push_tri(m, &verts[3 * i[0]], &verts[3 * i[1]], &verts[3 * i[2]], has_n ? &norms[3 * i[0]] : zero,
+                 has_n ? &norms[3 * i[1]] : zero, has_n ? &norms[3 * i[2]] : zero);
+    }
+    return true;
+}
+
+// Radiance RGBE (.hdr) reader: flat and new-style run-length scanlines.
+bool read_rgbe(const std::string &path, uint32_t &w, uint32_t &h, std::vector<float> &rgb, std::string &err) {
+    std::string buf;
+    if (!read_file(path, buf)) {
+        err = "cannot read " + path;
+        return false;
+    }
+    size_t pos = 0;
+    auto line = [&]() {
+        const size_t e = buf.find('\n', pos);
+        std::string l = buf.substr(pos, e == std::string::npos ? std::string::npos : e - pos);
+        pos = e == std::string::npos ? buf.size() : e + 1;
+        return l;
+    };
+    std::string l = line();
+    if (l.rfind("#?", 0) != 0) {
+        err = path + ": not a Radiance HDR file";
+        return false;
+    }
+    for (;;) {
+        if (pos >= buf.size()) {
+            err = path + ": missing resolution line";
+            return false;
+        }
+        l = line();
+        if (l.empty()) break;
+    }
+    l = line();
+    int H = 0, W = 0;
+    char ys[3] = {0}, xs[3] = {0};
+    if (std::sscanf(l.c_str(), "%2s %d %2s %d", ys, &H, xs, &W) != 4 || std::strcmp(ys, "-Y") || std::strcmp(xs, "+X") ||
+        W <= 0 || H <= 0) {
+        err = path + ": unsupported resolution line '" + l + "'";
+        return false;
+    }
+    w = (uint32_t)W;
+    h = (uint32_t)H;
+    rgb.assign((size_t)W * H * 3, 0.0f);
+    std::vector<uint8_t> scan((size_t)W * 4);
+    const uint8_t *p = (const uint8_t *)buf.data();
+    for (int y = 0; y < H; y++) {
+        if (pos + 4 > buf.size()) {
+            err = path + ": truncated";
+            return false;
+        }
+        if (W >= 8 && W < 32768 && p[pos] == 2 && p[pos + 1] == 2 && ((p[pos + 2] << 8) | p[pos + 3]) == W) {
+            pos += 4;
+            for (int c = 0; c < 4; c++) {
+                int x = 0;
+                while (x < W) {
+                    if (pos >= buf.size()) {
+                        err = path + ": truncated scanline";
+                        return false;
+                    }
+                    int cnt = p[pos++];
+                    if (cnt > 128) {
+                        cnt -= 128;
+                        if (x + cnt > W || pos >= buf.size()) {
+                            err = path + ": bad run";
+                            return false;
+                        }
+                        const uint8_t v = p[pos++];
+                        for (int k = 0; k < cnt; k++) scan[4 * (x++) + c] = v;
+                    } else {
+                        if (cnt == 0 || x + cnt > W || pos + cnt > buf.size()) {
+                            err = path + ": bad literal";
+                            return false;
+                        }
+                        for (int k = 0; k < cnt; k++) scan[4 * (x++) + c] = p[pos++];
+                    }
+                }
+            }
+        } else {
+            if (pos + 4 * (size_t)W > buf.size()) {
+                err = path + ": truncated flat scanline";
+                return false;
+            }
+            std::memcpy(scan.data(), p + pos, 4 * (size_t)W);
+            pos += 4 * (size_t)W;
+        }
+        for (int x = 0; x < W; x++) {
+            const uint8_t *q = &scan[4 * x];
+            float f = 0.0f;
+            if (q[3]) f = std::ldexp(1.0f, (int)q[3] - (128 + 8));
+            float *o = &rgb[3 * ((size_t)y * W + x)];
+            o[0] = q[0] * f;
+            o[1] = q[1] * f;
+            o[2] = q[2] * f;
+        }
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// the loader
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Loader {
+    std::string base;
+    std::vector<float> v9, n9;
+    std::vector<uint32_t> obj_of_tri;
+    std::vector<rg_object> objects;
+    std::vector<rg_medium> media;
+    int32_t global_medium = -1;
+    std::vector<std::vector<std::pair<std::string, rg_object>>> scopes;  // _materials per level
+    bool has_camera = false;
+    rg_camera camera{75.0f, 1.0f, 0.0f, 1.8f, 0, 0.0f, 50.0f, 0.0f};
+    Affine view;  // world -> render
+    std::string env_path;
+    float env_intensity = 1.0f, env_rotation = 0.0f;
+    std::vector<std::string> warnings;
+
+    std::string resolve(const std::string &p) const {
+        if (!p.empty() && p[0] == '/') return p;
+        return base + p;
+    }
+
+    static bool vec3_of(const json::Value *v, double out[3]) {
+        if (!v) return false;
+        if (v->is_array() && v->arr.size() >= 3) {
+            for (int k = 0; k < 3; k++) out[k] = v->arr[k].is_number() ? v->arr[k].num : 0.0;
+            return true;
+        }
+        if (v->is_object()) {
+            const char *names[3] = {"x", "y", "z"};
+            for (int k = 0; k < 3; k++) {
+                const json::Value *c = v->get(names[k]);
+                out[k] = c && c->is_number() ? c->num : 0.0;
+            }
+            return true;
+        }
+        return false;
+    }
+    static void num(const json::Value &o, const char *key, float &dst) {
+        const json::Value *v = o.get(key);
+        if (v && v->is_number()) dst = (float)v->num;
+    }
+
+    // colour string: "#rrggbb", "#rrggbbaa" (and the reference's 3/4-digit forms)
+    static bool parse_color(const std::string &s, float out[3]) {
+        std::string h = s;
+        if (!h.empty() && h[0] == '#') h = h.substr(1);
+        if (h.size() != 3 && h.size() != 4 && h.size() != 6 && h.size() != 8) return false;
+        unsigned long v = std::strtoul(h.c_str(), nullptr, 16);
+        if (h.size() == 3 || h.size() == 4) v *= 2;  // sceneloader.cpp:181-186
+        if (h.size() == 8 || h.size() == 4) v >>= 8;
+        out[0] = (float)((v >> 16) & 0xFF) / 255.0f;
+        out[1] = (float)((v >> 8) & 0xFF) / 255.0f;
+        out[2] = (float)(v & 0xFF) / 255.0f;
+        return true;
+    }
+
+    // SceneJsonLoader::readMaterial (sceneloader.cpp:101-165)
+    void read_material(rg_object &m, const json::Value &o) {
+        num(o, "emission", m.emission);
+        num(o, "glossy", m.glossy);
+        num(o, "roughness", m.roughness);
+        num(o, "transparency", m.transparency);
+        num(o, "refraction", m.refraction);
+        num(o, "refractionRatio", m.refraction_ratio);
+        num(o, "metallic", m.metallic);
+        num(o, "anisotropy", m.anisotropy);
+        num(o, "anisoRotation", m.aniso_rotation);
+        const json::Value *c = o.get("color");
+        if (c && c->is_string()) parse_color(c->str, m.color);
+        else if (c && c->is_array() && c->arr.size() >= 3)
+            for (int k = 0; k < 3; k++)
+                if (c->arr[k].is_number()) m.color[k] = (float)c->arr[k].num;
+        const json::Value *t = o.get("tex");
+        if (t && t->is_string() && !t->str.empty()) warnings.push_back("texture ignored: " + t->str);
+    }
+
+    const rg_object *find_material(const std::string &name) const {
+        for (auto s = scopes.rbegin(); s != scopes.rend(); ++s)
+            for (auto it = s->rbegin(); it != s->rend(); ++it)
+                if (it->first == name) return &it->second;
+        return nullptr;
+    }
+
+    // SceneJsonLoader::readMedium (sceneloader.cpp:202-334), homogeneous part
+    int read_medium(const json::Value &o) {
+        rg_medium m;
+        std::memset(&m, 0, sizeof m);
+        m.density = 1.0f;
+        double v[3];
+        if (vec3_of(o.get("sigma_a"), v)) for (int k = 0; k < 3; k++) m.sigma_a[k] = (float)v[k];
+        if (vec3_of(o.get("sigma_s"), v)) for (int k = 0; k < 3; k++) m.sigma_s[k] = (float)v[k];
+        if (vec3_of(o.get("emission"), v)) for (int k = 0; k < 3; k++) m.sigma_e[k] = (float)v[k];
+        num(o, "g", m.g);
+        num(o, "density", m.density);
+        for (const char *k : {"emissionMode", "densityField", "heatHaze"})
+            if (o.get(k)) warnings.push_back(std::string("medium field outside the homogeneous path ignored: ") + k);
+        // an inactive medium is dropped (sceneloader.cpp:326-332)
+        const float d = fmaxf(0.0f, m.density);
+        const float hero = ((m.sigma_a[0] + m.sigma_s[0]) * d + (m.sigma_a[1] + m.sigma_s[1]) * d +
+                            (m.sigma_a[2] + m.sigma_s[2]) * d) * (1.0f / 3.0f);
+        const bool emits = m.sigma_e[0] * d != 0.0f || m.sigma_e[1] * d != 0.0f || m.sigma_e[2] * d != 0.0f;
+        if (!(hero > 0.0f) && !emits) return -1;
+        media.push_back(m);
+        return (int)media.size() - 1;
+    }
+
+    bool is_camera_like(const std::string &key) const { return key == "mainCamera"; }
+
+    void read_camera(const json::Value &o) {
+        has_camera = true;
+        num(o, "fieldOfView", camera.fov_deg);
+        num(o, "depthOfField", camera.depth_of_field);
+        num(o, "aperture", camera.aperture);
+        float blades = (float)camera.aperture_blades;
+        num(o, "apertureBlades", blades);
+        camera.aperture_blades = (int32_t)blades;
+        num(o, "apertureRotation", camera.aperture_rotation_deg);
+        num(o, "exposure", camera.exposure);
+        num(o, "viewFar", camera.view_far);
+        if (o.get("focusOn")) warnings.push_back("camera focusOn ignored");
+        double loc[3] = {0, 0, 0}, ang[3] = {0, 0, 0}, sc[3] = {1, 1, 1};
+        vec3_of(o.get("location"), loc);
+        vec3_of(o.get("angle"), ang);
+        vec3_of(o.get("scale"), sc);
+        // Renderer::applyCameraTransform (renderer.cpp:24-28): rotate(-angle),
+        // translate(-location), scale(1/scale)
+        const double nl[3] = {-loc[0], -loc[1], -loc[2]};
+        const double na[3] = {-ang[0], -ang[1], -ang[2]};
+        const double is[3] = {1.0 / sc[0], 1.0 / sc[1], 1.0 / sc[2]};
+        // inverse rotation of an X-then-Y-then-Z Euler rotation: Z, then Y, then X
+        const Affine inv_rot = rot_axis(0, na[0]).then_after(rot_axis(1, na[1]).then_after(rot_axis(2, na[2])));
+        view = inv_rot.then_after(translation(nl).then_after(scaling(is)));
+    }
+
+    std::string err;
+
+    // mesh property of an object (sceneloader.cpp:618-641, 390-403)
+    bool read_meshes(const json::Value &val, std::vector<MeshData> &out) {
+        auto from_path = [&](const std::string &rel) -> bool {
+            const std::string p = resolve(rel);
+            const size_t dot = p.rfind('.');
+            const std::string ext = dot == std::string::npos ? "" : p.substr(dot);
+            if (ext == ".obj" || ext == ".OBJ") {
+                warnings.push_back("OBJ import is outside this path, skipped: " + rel);
+                return true;
+            }
+            MeshData m;
+            if (!load_mesh_file(p, m, err)) return false;
+            out.push_back(std::move(m));
+            return true;
+        };
+        if (val.is_string()) return from_path(val.str);
+        if (val.is_array()) {
+            for (const json::Value &x : val.arr)
+                if (x.is_string() && !from_path(x.str)) return false;
+            return true;
+        }
+        if (val.is_object()) {
+            const json::Value *t = val.get("type");
+            const std::string type = t && t->is_string() ? t->str : "";
+            if (type == "plane") out.push_back(plane_mesh());
+            else if (type == "cube") out.push_back(cube_mesh());
+            else if (type == "sphere") {
+                float r = 1, st = 16, sl = 32;
+                num(val, "radius", r);
+                num(val, "stacks", st);
+                num(val, "slices", sl);
+                out.push_back(sphere_mesh(r, (int)st, (int)sl));
+            } else if (!type.empty()) {
+                warnings.push_back("primitive '" + type + "' is outside this path, skipped");
+            }
+        }
+        return true;
+    }
+
+    // readSceneObject + transformObject, depth first in file order
+    bool object(const json::Value &o, const Affine &parent, bool is_root) {
+        scopes.emplace_back();
+        if (const json::Value *mats = o.get("_materials"))
+            if (mats->is_object())
+                for (auto &p : mats->obj)
+                    if (p.second.is_object()) {
+                        rg_object m;
+                        rg_object_default(&m);
+                        read_material(m, p.second);
+                        scopes.back().emplace_back(p.first, m);
+                    }
+        double loc[3] = {0, 0, 0}, ang[3] = {0, 0, 0}, sc[3] = {1, 1, 1};
+        vec3_of(o.get("location"), loc);
+        vec3_of(o.get("angle"), ang);
+        vec3_of(o.get("scale"), sc);
+        const Affine model = parent.then_after(translation(loc).then_after(euler(ang).then_after(scaling(sc))));
+        rg_object mat;
+        rg_object_default(&mat);
+        int32_t interior = -1;
+        std::vector<MeshData> meshes;
+        std::vector<const json::Value *> children;
+        std::vector<std::string> child_names;
+        bool visible = true;
+        for (auto &p : o.obj) {
+            const std::string &key = p.first;
+            const json::Value &val = p.second;
+            if (key == "_materials" || key == "_bundles" || key == "location" || key == "angle" || key == "scale") {
+            } else if (key == "_bundle") {
+                warnings.push_back(".toba bundles are outside this path, skipped");
+            } else if (key == "mesh") {
+                if (!read_meshes(val, meshes)) return false;
+            } else if (key == "mat") {
+                if (val.is_string()) {
+                    const rg_object *m = find_material(val.str);
+                    if (m) mat = *m;
+                    else warnings.push_back("unknown material '" + val.str + "'");
+                } else if (val.is_object()) {
+                    read_material(mat, val);
+                }
+            } else if (key == "visible" && val.kind == json::Value::Bool) {
+                visible = val.b;
+            } else if (key == "medium" && val.is_object()) {
+                const int m = read_medium(val);
+                if (is_root) global_medium = m;
+                else interior = m;
+            } else if (key == "envmap") {
+                if (val.is_string()) env_path = resolve(val.str);
+                else if (val.is_object()) {
+                    const json::Value *t = val.get("texture");
+                    if (t && t->is_string() && !t->str.empty()) env_path = resolve(t->str);
+                    if (val.get("cubemap")) warnings.push_back("cubemap environments are outside this path");
+                    num(val, "intensity", env_intensity);
+                    num(val, "rotation", env_rotation);
+                }
+            } else if (is_camera_like(key) && val.is_object()) {
+                read_camera(val);
+            } else if (val.is_object()) {
+                children.push_back(&val);
+                child_names.push_back(key);
+            }
+        }
+        if (!visible) {
+            scopes.pop_back();
+            return true;
+        }
+        mat.interior_medium = interior;
+        if (!meshes.empty()) {
+            const uint32_t oid = (uint32_t)objects.size();
+            objects.push_back(mat);
+            pending.push_back({model, oid, std::move(meshes)});
+        }
+        for (const json::Value *c : children)
+            if (!object(*c, model, false)) return false;
+        scopes.pop_back();
+        return true;
+    }
+
+    struct Pending {
+        Affine model;
+        uint32_t obj;
+        std::vector<MeshData> meshes;
+    };
+    std::vector<Pending> pending;
+
+    // transformObject (rayrenderer.cpp:237-363): render-space vertices and
+    // normalised normals, once the camera is known
+    void flatten() {
+        for (Pending &p : pending) {
+            const Affine vm = view.then_after(p.model);
+            double nm[3][3];
+            const bool ok = vm.normal_matrix(nm);
+            for (MeshData &m : p.meshes) {
+                const size_t nt = m.v.size() / 9;
+                for (size_t t = 0; t < nt; t++) {
+                    for (int k = 0; k < 3; k++) {
+                        double q[3];
+                        vm.apply(&m.v[9 * t + 3 * k], q);
+                        for (int c = 0; c < 3; c++) v9.push_back((float)q[c]);
+                        const float *nn = &m.n[9 * t + 3 * k];
+                        double r[3] = {nn[0], nn[1], nn[2]};
+                        if (ok)
+                            for (int c = 0; c < 3; c++) r[c] = nm[c][0] * nn[0] + nm[c][1] * nn[1] + nm[c][2] * nn[2];
+                        const double l = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                        for (int c = 0; c < 3; c++) n9.push_back(l > 0 ? (float)(r[c] / l) : 0.0f);
+                    }
+                    obj_of_tri.push_back(p.obj);
+                }
+            }
+        }
+    }
+};
+
+}  // namespace
+
+}  // namespace rg
+
+using namespace rg;
+
+extern "C" int rg_scene_load_json(rg_ctx *ctx, const char *path, rg_scene **out) {
+    if (!ctx || !path || !out) return set_error(RG_E_INVALID, "null argument");
+    *out = nullptr;
+    std::string text;
+    if (!read_file(path, text)) return set_error(RG_E_IO, std::string("cannot read ") + path);
+    if (text.empty()) return set_error(RG_E_IO, "scene file is empty");
+    Loader L;
+    const std::string p(path);
+    const size_t slash = p.rfind('/');
+    L.base = slash == std::string::npos ? "" : p.substr(0, slash + 1);
+    try {
+        json::Parser parser(text);
+        const json::Value root = parser.parse();
+        if (!root.is_object()) return set_error(RG_E_IO, "scene root is not an object");
+        Affine ident;
+        if (!L.object(root, ident, true)) return set_error(RG_E_IO, L.err);
+    } catch (const std::exception &e) {
+        return set_error(RG_E_IO, e.what());
+    }
+    L.flatten();
+    rg_scene *sc = nullptr;
+    if (int rc = rg_scene_create(ctx, L.v9.data(), (uint32_t)L.obj_of_tri.size(), &sc)) return rc;
+    int rc = rg_scene_set_media(sc, L.media.data(), (uint32_t)L.media.size(), L.global_medium);
+    if (!rc) rc = rg_scene_set_shading(sc, L.n9.data(), L.obj_of_tri.data(), L.objects.data(), (uint32_t)L.objects.size());
+    if (!rc) rc = rg_scene_set_camera(sc, &L.camera);
+    if (!rc && !L.env_path.empty()) {
+        const size_t dot = L.env_path.rfind('.');
+        const std::string ext = dot == std::string::npos ? "" : L.env_path.substr(dot);
+        if (ext == ".hdr" || ext == ".HDR") {
+            uint32_t w = 0, h = 0;
+            std::vector<float> rgb;
+            std::string err;
+            if (read_rgbe(L.env_path, w, h, rgb, err)) {
+                rg_env env{w, h, rgb.data(), L.env_intensity, L.env_rotation};
+                rc = rg_scene_set_env(sc, &env);
+            } else {
+                L.warnings.push_back("environment not loaded: " + err);
+            }
+        } else {
+            L.warnings.push_back("only .hdr environments are read on this path: " + L.env_path);
+        }
+    }
+    if (rc) {
+        rg_scene_destroy(sc);
+        return rc;
+    }
+    Scene *s = reinterpret_cast<Scene *>(sc);
+    s->warnings = L.warnings;
+    s->has_camera = L.has_camera;
+    *out = sc;
+    return RG_OK;
+}
+
+extern "C" int rg_scene_warnings(const rg_scene *scene, char *buf, uint64_t cap) {
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (!s || (!buf && cap)) return set_error(RG_E_INVALID, "null argument");
+    std::string all;
+    for (const std::string &w : s->warnings) all += w + "\n";
+    if (cap) {
+        const size_t n = std::min<size_t>(all.size(), (size_t)cap - 1);
+        std::memcpy(buf, all.data(), n);
+        buf[n] = 0;
+    }
+    return (int)std::min<size_t>(all.size(), 0x7FFFFFFF);
+}
+
+extern "C" int rg_scene_get_camera(const rg_scene *scene, rg_camera *cam) {
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (!s || !cam) return set_error(RG_E_INVALID, "null argument");
+    *cam = s->camera;
+    return RG_OK;
+}
+
+extern "C" int rg_scene_export(const rg_scene *scene, float *v9, float *n9, uint32_t *obj_of_tri, rg_object *objs,
+                               uint32_t *n_tris, uint32_t *n_obj) {
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (!s) return set_error(RG_E_INVALID, "null scene");
+    if (n_tris) *n_tris = s->n_tris;
+    if (n_obj) *n_obj = (uint32_t)s->objects.size();
+    if (v9 && !s->v9.empty()) std::memcpy(v9, s->v9.data(), s->v9.size() * sizeof(float));
+    if (n9 && !s->n9.empty()) std::memcpy(n9, s->n9.data(), s->n9.size() * sizeof(float));
+    if (obj_of_tri && !s->obj_of_tri.empty())
+        std::memcpy(obj_of_tri, s->obj_of_tri.data(), s->obj_of_tri.size() * sizeof(uint32_t));
+    if (objs && !s->objects.empty()) std::memcpy(objs, s->objects.data(), s->objects.size() * sizeof(rg_object));
+    return RG_OK;
+}
+
+extern "C" int rg_scene_export_media(const rg_scene *scene, rg_medium *media, uint32_t *n_media,
+                                     int32_t *global_medium, rg_env *env, float *env_rgb) {
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (!s) return set_error(RG_E_INVALID, "null scene");
+    if (n_media) *n_media = (uint32_t)s->media.size();
+    if (global_medium) *global_medium = s->global_medium;
+    if (env) {
+        env->w = s->env_w;
+        env->h = s->env_h;
+        env->rgb = nullptr;
+        env->intensity = s->env_intensity;
+        env->rotation_deg = s->env_rotation_deg;
+    }
+    if (media && !s->media.empty()) std::memcpy(media, s->media.data(), s->media.size() * sizeof(rg_medium));
+    if (env_rgb && !s->env_rgb.empty()) std::memcpy(env_rgb, s->env_rgb.data(), s->env_rgb.size() * sizeof(float));
+    return RG_OK;
+}

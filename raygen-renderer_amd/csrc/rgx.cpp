@@ -1,0 +1,186 @@
+// rgx — command-line front-end with the reference's render flags
+// (src/main.cpp:162-490): `rgx render <scene.json> [options]`.
+//
+//   -o FILE            output; .hdr = linear HDR (RGBE), otherwise a
+//                      tonemapped binary PPM (Reinhard on luminance + 1/2.2
+//                      gamma, RayRenderer::applyTonemapGamma)
+//   -r W,H | WxH | N   resolution (N = N x N)
+//   -s N               samples per pixel
+//   -c N               host threads (accepted; the GPU path does not use it)
+//   -d N               shading system; only 5 (BSDF) is on this path
+//   -enaa B  -endn B  -enad B   antialias / denoise / adaptive (yes|true|1|on)
+//   -dni F  --adaptive-base N  --adaptive-threshold F
+//   --depth N          trace depth (the reference's MAX_TRACE_DEPTH, 32)
+//   --seed N           Philox key of the sample stream
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rg.h"
+
+static bool parse_bool(const char *s) {
+    return !strcasecmp(s, "yes") || !strcasecmp(s, "true") || !strcmp(s, "1") || !strcasecmp(s, "on");
+}
+
+// Radiance RGBE writer (flat scanlines)
+static bool write_hdr(const char *path, const float *rgba, int w, int h) {
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return false;
+    std::fprintf(f, "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n", h, w);
+    std::vector<unsigned char> row((size_t)w * 4);
+    for (int y = 0; y < h; y++) {
+        for (int x = 0; x < w; x++) {
+            const float *c = rgba + 4 * ((size_t)y * w + x);
+            const float v = std::fmax(c[0], std::fmax(c[1], c[2]));
+            unsigned char *o = &row[4 * (size_t)x];
+            if (v < 1e-32f) {
+                o[0] = o[1] = o[2] = o[3] = 0;
+            } else {
+                int e;
+                const float m = std::frexp(v, &e) * 256.0f / v;
+                o[0] = (unsigned char)(c[0] * m);
+                o[1] = (unsigned char)(c[1] * m);
+                o[2] = (unsigned char)(c[2] * m);
+                o[3] = (unsigned char)(e + 128);
+            }
+        }
+        std::fwrite(row.data(), 1, row.size(), f);
+    }
+    return std::fclose(f) == 0;
+}
+
+// applyTonemapGamma (rayrenderer.cpp:2610-2647) into an 8-bit PPM
+static bool write_ppm(const char *path, const float *rgba, int w, int h) {
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return false;
+    std::fprintf(f, "P6\n%d %d\n255\n", w, h);
+    const float inv_gamma = 1.0f / 2.2f;
+    for (size_t i = 0; i < (size_t)w * h; i++) {
+        const float *c = rgba + 4 * i;
+        const float L = 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2];
+        float m[3] = {0, 0, 0};
+        if (L > 1e-6f) {
+            const float s = (L / (L + 1.0f)) / L;
+            for (int k = 0; k < 3; k++) m[k] = c[k] * s;
+            const float peak = std::fmax(std::fmax(m[0], m[1]), m[2]);
+            if (peak > 1.0f)
+                for (int k = 0; k < 3; k++) m[k] /= peak;
+        }
+        unsigned char px[3];
+        for (int k = 0; k < 3; k++) {
+            float v = std::pow(std::fmax(m[k], 0.0f), inv_gamma);
+            if (v > 1.0f) v = 1.0f;
+            px[k] = (unsigned char)(v * 255.0f + 0.5f);
+        }
+        std::fwrite(px, 1, 3, f);
+    }
+    return std::fclose(f) == 0;
+}
+
+int main(int argc, char **argv) {
+    if (argc < 2) {
+        std::printf("usage: rgx render <scene.json> [-o out.hdr|out.ppm] [-r WxH] [-s N] [-d 5] [-enaa B] ...\n");
+        return 0;
+    }
+    rg_settings st;
+    rg_settings_default(&st);
+    int shader = 5;
+    std::string cmd, scene, out = "out.hdr";
+    for (int i = 1; i < argc; i++) {
+        const char *a = argv[i];
+        auto next = [&]() -> const char * {
+            if (i + 1 >= argc) {
+                std::fprintf(stderr, "missing value for %s\n", a);
+                std::exit(1);
+            }
+            return argv[++i];
+        };
+        if (a[0] == '-') {
+            if (!strcmp(a, "-o")) out = next();
+            else if (!strcmp(a, "-r") || !strcmp(a, "--resolution")) {
+                const char *v = next();
+                int w = (int)st.width, h = (int)st.height;
+                int n = std::sscanf(v, "%d,%d", &w, &h);
+                if (n < 2) n = std::sscanf(v, "%dx%d", &w, &h);
+                if (n == 1) h = w;
+                if (n >= 1) {
+                    st.width = (uint32_t)w;
+                    st.height = (uint32_t)h;
+                }
+            } else if (!strcmp(a, "-s") || !strcmp(a, "--samples")) st.spp = (uint32_t)std::atoi(next());
+            else if (!strcmp(a, "-c") || !strcmp(a, "--threads") || !strcmp(a, "--cores")) (void)next();
+            else if (!strcmp(a, "-d") || !strcmp(a, "--shader")) shader = std::atoi(next());
+            else if (!strcmp(a, "-enaa") || !strcmp(a, "--enable-antialias")) st.antialias = parse_bool(next());
+            else if (!strcmp(a, "-endn") || !strcmp(a, "--enable-denoise")) st.denoise = parse_bool(next());
+            else if (!strcmp(a, "-dni") || !strcmp(a, "--denoise-intensity")) st.denoise_intensity = (float)std::atof(next());
+            else if (!strcmp(a, "-enad") || !strcmp(a, "--enable-adaptive")) st.adaptive = parse_bool(next());
+            else if (!strcmp(a, "--adaptive-base")) st.adaptive_base = (uint32_t)std::atoi(next());
+            else if (!strcmp(a, "--adaptive-threshold")) st.adaptive_threshold = (float)std::atof(next());
+            else if (!strcmp(a, "--depth")) st.max_depth = (uint32_t)std::atoi(next());
+            else if (!strcmp(a, "--seed")) st.seed = (uint32_t)std::strtoul(next(), nullptr, 0);
+            else if (!strcmp(a, "-h") || !strcmp(a, "--help")) {
+                std::printf("rgx render <scene.json> -o out.hdr -r 800,600 -s 100 -d 5 -enaa true\n");
+                return 0;
+            } else {
+                std::printf("unknown argument: %s\n", a);
+                return 1;
+            }
+        } else if (cmd.empty()) {
+            cmd = a;
+        } else if (scene.empty()) {
+            scene = a;
+        } else {
+            std::printf("unknown argument: %s\n", a);
+            return 1;
+        }
+    }
+    if (cmd != "render") {
+        std::fprintf(stderr, "only the render command is on this path\n");
+        return 1;
+    }
+    if (shader != 5) {
+        std::fprintf(stderr, "shading system %d is outside this path (only -d 5)\n", shader);
+        return 1;
+    }
+    rg_ctx *ctx = nullptr;
+    if (rg_create(0, &ctx) != RG_OK) {
+        std::fprintf(stderr, "rg_create: %s\n", rg_last_error());
+        return 1;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    rg_scene *sc = nullptr;
+    if (rg_scene_load_json(ctx, scene.c_str(), &sc) != RG_OK) {
+        std::fprintf(stderr, "load %s: %s\n", scene.c_str(), rg_last_error());
+        return 1;
+    }
+    char warn[4096];
+    if (rg_scene_warnings(sc, warn, sizeof warn) > 0) std::fprintf(stderr, "%s", warn);
+    if (rg_bvh_build(ctx, sc, nullptr) != RG_OK) {
+        std::fprintf(stderr, "bvh: %s\n", rg_last_error());
+        return 1;
+    }
+    std::printf("resolution: %u x %u, samples: %u, shader: %d, antialias: %s, denoise: %s, adaptive: %s\n", st.width,
+                st.height, st.spp, shader, st.antialias ? "yes" : "no", st.denoise ? "yes" : "no",
+                st.adaptive ? "yes" : "no");
+    std::vector<float> hdr((size_t)st.width * st.height * 4);
+    if (rg_render(ctx, sc, &st, hdr.data(), nullptr, nullptr, nullptr) != RG_OK) {
+        std::fprintf(stderr, "render: %s\n", rg_last_error());
+        return 1;
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const bool is_hdr = out.size() >= 4 && (out.compare(out.size() - 4, 4, ".hdr") == 0);
+    const bool ok = is_hdr ? write_hdr(out.c_str(), hdr.data(), (int)st.width, (int)st.height)
+                           : write_ppm(out.c_str(), hdr.data(), (int)st.width, (int)st.height);
+    if (!ok) {
+        std::fprintf(stderr, "cannot write %s\n", out.c_str());
+        return 1;
+    }
+    std::printf("done. (%.3f s) -> %s\n", secs, out.c_str());
+    rg_scene_destroy(sc);
+    rg_destroy(ctx);
+    return 0;
+}

@@ -78,6 +78,11 @@ def lib():
             "rg_scene_set_camera": (i32, [vp, vp]),
             "rg_render": (i32, [vp, vp, vp, vp, vp, vp, vp]),
             "rg_render_stats": (i32, [vp, vp]),
+            "rg_scene_load_json": (i32, [vp, ctypes.c_char_p, pp]),
+            "rg_scene_warnings": (i32, [vp, ctypes.c_char_p, u64]),
+            "rg_scene_get_camera": (i32, [vp, vp]),
+            "rg_scene_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+            "rg_scene_export_media": (i32, [vp, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -184,6 +189,19 @@ class Scene:
         check(lib().rg_trace_any_host(self.ctx.h, self.h, _ptr(rays), _ptr(occ), len(rays), pred))
         return occ
 
+    def render(self, settings):
+        """rg_render of this scene (shading set and BVH built) with host outputs."""
+        W, H = settings.width, settings.height
+        hdr = np.zeros((H, W, 4), np.float32)
+        sm = np.zeros((H, W, 3), np.float32)
+        sq = np.zeros((H, W, 3), np.float32)
+        nt = np.zeros(((W + 31) // 32) * ((H + 31) // 32), np.uint32)
+        check(lib().rg_render(self.ctx.h, self.h, ctypes.byref(settings), _ptr(hdr), _ptr(sm), _ptr(sq), _ptr(nt)))
+        stats = np.zeros(3, np.uint64)
+        check(lib().rg_render_stats(self.ctx.h, _ptr(stats)))
+        return {"hdr": hdr, "sum": sm, "sumsq": sq, "nspp": nt,
+                "rays": int(stats[0]), "shadow_rays": int(stats[1]), "samples": int(stats[2])}
+
     def close(self):
         if self.h:
             lib().rg_scene_destroy(self.h)
@@ -284,14 +302,35 @@ class RenderScene(Scene):
         check(lib().rg_scene_set_camera(self.h, ctypes.byref(cam)))
         self.build(width=width)
 
-    def render(self, settings):
-        W, H = settings.width, settings.height
-        hdr = np.zeros((H, W, 4), np.float32)
-        sm = np.zeros((H, W, 3), np.float32)
-        sq = np.zeros((H, W, 3), np.float32)
-        nt = np.zeros(((W + 31) // 32) * ((H + 31) // 32), np.uint32)
-        check(lib().rg_render(self.ctx.h, self.h, ctypes.byref(settings), _ptr(hdr), _ptr(sm), _ptr(sq), _ptr(nt)))
-        stats = np.zeros(3, np.uint64)
-        check(lib().rg_render_stats(self.ctx.h, _ptr(stats)))
-        return {"hdr": hdr, "sum": sm, "sumsq": sq, "nspp": nt,
-                "rays": int(stats[0]), "shadow_rays": int(stats[1]), "samples": int(stats[2])}
+def load_json(ctx, path):
+    """rg_scene_load_json: returns (rg_scene handle wrapper, scene dict with
+    the flattened arrays, camera and warnings)."""
+    h = ctypes.c_void_p()
+    check(lib().rg_scene_load_json(ctx.h, path.encode(), ctypes.byref(h)))
+    nt, no = ctypes.c_uint32(), ctypes.c_uint32()
+    check(lib().rg_scene_export(h, None, None, None, None, ctypes.byref(nt), ctypes.byref(no)))
+    from scenes import OBJECT_DTYPE
+    v9 = np.zeros((nt.value, 9), np.float32)
+    n9 = np.zeros((nt.value, 9), np.float32)
+    obj = np.zeros(nt.value, np.uint32)
+    objs = np.zeros(no.value, OBJECT_DTYPE)
+    check(lib().rg_scene_export(h, _ptr(v9), _ptr(n9), _ptr(obj), _ptr(objs), ctypes.byref(nt), ctypes.byref(no)))
+    cam = Camera()
+    check(lib().rg_scene_get_camera(h, ctypes.byref(cam)))
+    buf = ctypes.create_string_buffer(8192)
+    lib().rg_scene_warnings(h, buf, len(buf))
+    nm, gm, env = ctypes.c_uint32(), ctypes.c_int32(), Env()
+    check(lib().rg_scene_export_media(h, None, ctypes.byref(nm), ctypes.byref(gm), ctypes.byref(env), None))
+    from scenes import MEDIUM_DTYPE
+    media = np.zeros(nm.value, MEDIUM_DTYPE)
+    env_rgb = np.zeros((env.h, env.w, 3), np.float32)
+    check(lib().rg_scene_export_media(h, _ptr(media), ctypes.byref(nm), ctypes.byref(gm), None, _ptr(env_rgb)))
+    sc = Scene.__new__(Scene)
+    sc.ctx, sc.h = ctx, h
+    sc.v9 = v9
+    cam_d = {f: getattr(cam, f) for f, _ in Camera._fields_ if f != "pad"}
+    d = {"v9": v9, "n9": n9, "obj": obj, "objects": objs, "camera": cam_d, "media": media,
+         "global_medium": gm.value, "warnings": [w for w in buf.value.decode().split("\n") if w]}
+    if env.w and env.h:
+        d["env"] = {"rgb": env_rgb, "intensity": env.intensity, "rotation_deg": env.rotation_deg}
+    return sc, d

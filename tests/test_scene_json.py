@@ -1,0 +1,188 @@
+"""Scene ingest (rg_scene_load_json) on a host-only context: the reference's
+JSON schema, primitive meshes, transforms, materials, media, camera and the
+binary .mesh format.  CPU only."""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import rgapi
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def loaded():
+    ctx = rgapi.Context(-1)
+    sc, d = rgapi.load_json(ctx, os.path.join(GOLD, "room_scene.json"))
+    return ctx, sc, d
+
+
+def tris_of(d, k):
+    return d["v9"][d["obj"] == k].reshape(-1, 3, 3)
+
+
+def test_object_order_and_counts(loaded):
+    _, _, d = loaded
+    # light, floor, left, back (2 each), box 12, ball 2*8*12, mesh 528; hidden skipped
+    sizes = [int((d["obj"] == k).sum()) for k in range(len(d["objects"]))]
+    assert sizes == [2, 2, 2, 2, 12, 192, 528]
+
+
+def test_mesh_file_matches_independent_parse(loaded):
+    _, _, d = loaded
+    raw = open(os.path.join(GOLD, "sphere_v0102.mesh"), "rb").read()
+    tag, ver, flags, length = struct.unpack("<IHHI", raw[:12])
+    vcount = struct.unpack("<I", raw[12:16])[0]
+    assert (tag, ver, flags, length, vcount) == (0x6873656D, 0x102, 0xA, 80, 1584)
+    verts = np.frombuffer(raw, np.float32, 3 * vcount, length).reshape(-1, 3)
+    want = verts * 0.5 + np.array([0, -2.5, 1.5 - 5.0], np.float32)  # camera at z = 5
+    got = tris_of(d, 6).reshape(-1, 3)
+    np.testing.assert_allclose(got, want, atol=1e-6)
+
+
+def test_transforms_and_camera_view(loaded):
+    _, _, d = loaded
+    light = tris_of(d, 0).reshape(-1, 3)
+    # plane scaled 1.5 in x/z, flipped by 180 deg about X, at y = 2.95, camera at z = 5
+    assert np.allclose(light[:, 1], 2.95, atol=1e-6)
+    assert np.isclose(light[:, 0].min(), -0.75) and np.isclose(light[:, 0].max(), 0.75)
+    assert np.isclose(light[:, 2].min(), -0.75 - 5.0) and np.isclose(light[:, 2].max(), 0.75 - 5.0)
+    n = d["n9"][d["obj"] == 0].reshape(-1, 3)
+    assert np.allclose(n, [0, -1, 0], atol=1e-6)          # faces down after the flip
+    left = d["n9"][d["obj"] == 2].reshape(-1, 3)
+    assert np.allclose(left, [1, 0, 0], atol=1e-6)        # -90 deg about Z turns +Y into +X
+    back = d["n9"][d["obj"] == 3].reshape(-1, 3)
+    assert np.allclose(back, [0, 0, 1], atol=1e-6)        # +90 deg about X turns +Y into +Z
+    assert np.isclose(d["camera"]["fov_deg"], 60) and np.isclose(d["camera"]["exposure"], 1.5)
+
+
+def test_rotated_cube_normals_are_unit_and_rotated(loaded):
+    _, _, d = loaded
+    n = d["n9"][d["obj"] == 4].reshape(-1, 3)
+    assert np.allclose(np.linalg.norm(n, axis=1), 1, atol=1e-6)
+    c, s = np.cos(np.radians(40)), np.sin(np.radians(40))
+    # the +X face normal after 40 deg about Y (x' = x c + z s, z' = -x s + z c)
+    assert np.isclose(n, [c, 0, -s], atol=1e-5).all(axis=1).any()
+
+
+def test_materials_and_media(loaded):
+    _, _, d = loaded
+    objs = d["objects"]
+    assert np.allclose(objs[0]["color"], [1, 1, 240 / 255]) and objs[0]["emission"] == 10
+    assert np.allclose(objs[1]["color"], 204 / 255) and objs[1]["roughness"] == 0  # "wall"
+    assert objs[2]["roughness"] == np.float32(0.5)                                 # default
+    assert objs[4]["transparency"] == 1 and objs[4]["interior_medium"] == 0
+    assert objs[5]["refraction"] == np.float32(0.9) and objs[5]["refraction_ratio"] == np.float32(1.5)
+    assert all(o["interior_medium"] == -1 for i, o in enumerate(objs) if i != 4)
+
+
+def test_bad_files_report_io_errors():
+    ctx = rgapi.Context(-1)
+    with pytest.raises(rgapi.RGError) as e:
+        rgapi.load_json(ctx, "/nonexistent/scene.json")
+    assert e.value.code == -8
+    bad = os.path.join(GOLD, "..", "..", "_bad_scene.json")
+    with open(bad, "w") as f:
+        f.write("{ \"a\": [1, 2 }")
+    try:
+        with pytest.raises(rgapi.RGError) as e:
+            rgapi.load_json(ctx, bad)
+        assert e.value.code == -8 and "line" in str(e.value)
+    finally:
+        os.remove(bad)
+
+
+def test_object_medium_exported(loaded):
+    _, _, d = loaded
+    assert d["global_medium"] == -1 and len(d["media"]) == 1
+    m = d["media"][0]
+    assert np.allclose(m["sigma_a"], 0.1) and np.allclose(m["sigma_s"], 0.4) and m["density"] == 1
+
+
+def _rgbe(c):
+    v = max(c)
+    if v < 1e-32:
+        return [0, 0, 0, 0]
+    m, e = np.frexp(v)
+    s = m * 256.0 / v
+    return [int(c[0] * s), int(c[1] * s), int(c[2] * s), int(e + 128)]
+
+
+def _write_hdr(path, img, rle):
+    h, w, _ = img.shape
+    with open(path, "wb") as f:
+        f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {h} +X {w}\n".encode())
+        for y in range(h):
+            px = np.array([_rgbe(img[y, x]) for x in range(w)], np.uint8)
+            if not rle:
+                f.write(px.tobytes())
+                continue
+            f.write(bytes([2, 2, w >> 8, w & 255]))
+            for c in range(4):
+                ch = px[:, c]
+                x = 0
+                while x < w:                      # runs of equal bytes, else literals
+                    r = 1
+                    while x + r < w and r < 127 and ch[x + r] == ch[x]:
+                        r += 1
+                    if r >= 3:
+                        f.write(bytes([128 + r, ch[x]]))
+                        x += r
+                    else:
+                        n = min(w - x, 128)
+                        f.write(bytes([n]) + ch[x:x + n].tobytes())
+                        x += n
+
+
+@pytest.mark.parametrize("rle", [False, True])
+def test_envmap_hdr_and_root_medium(tmp_path, rle):
+    rng = np.random.default_rng(5)
+    img = rng.uniform(0, 4, (6, 16, 3)).astype(np.float32)
+    img[2, :8] = 1.5                              # a run for the RLE encoder
+    _write_hdr(tmp_path / "sky.hdr", img, rle)
+    (tmp_path / "s.json").write_text("""{
+      "envmap": { "texture": "sky.hdr", "intensity": 2.5, "rotation": 30 },
+      "medium": { "sigma_s": [0.01, 0.02, 0.03], "g": 0.3 },
+      "quad": { "mesh": { "type": "plane" } } }""")
+    ctx = rgapi.Context(-1)
+    _, d = rgapi.load_json(ctx, str(tmp_path / "s.json"))
+    env = d["env"]
+    assert env["rgb"].shape == img.shape and env["intensity"] == 2.5 and env["rotation_deg"] == 30
+    # RGBE keeps 8 mantissa bits of the largest channel: decode = m * 2^(e-136)
+    want = np.zeros_like(img)
+    for y in range(6):
+        for x in range(16):
+            r, g, b, e = _rgbe(img[y, x])
+            want[y, x] = np.array([r, g, b], np.float32) * np.float32(2.0 ** (e - 136))
+    np.testing.assert_array_equal(env["rgb"], want)
+    assert d["global_medium"] == 0 and np.isclose(d["media"][0]["g"], 0.3)
+    assert np.allclose(d["media"][0]["sigma_s"], [0.01, 0.02, 0.03])
+
+
+def test_inactive_medium_dropped_and_skips_warn(tmp_path):
+    (tmp_path / "s.json").write_text("""{
+      "a": { "mesh": { "type": "cone" }, "medium": { "sigma_a": [0, 0, 0] } },
+      "b": { "mesh": ["x.obj"], "mat": "nosuch" },
+      "c": { "mesh": { "type": "cube" }, "mat": { "color": "#f80" } } }""")
+    ctx = rgapi.Context(-1)
+    _, d = rgapi.load_json(ctx, str(tmp_path / "s.json"))
+    assert len(d["media"]) == 0 and len(d["objects"]) == 1
+    w = "\n".join(d["warnings"])
+    assert "cone" in w and "x.obj" in w and "nosuch" in w
+    # 3-digit hex is read as one number times 2 (sceneloader.cpp:184-186):
+    # 0xf80 * 2 = 0x001F00
+    np.testing.assert_array_equal(d["objects"][0]["color"], np.float32([0, 31 / 255, 0]))
+
+
+def test_rgx_cli_argument_errors():
+    import subprocess
+    exe = os.path.join(rgapi.HERE, "rgx")
+    if not os.path.exists(exe):
+        pytest.skip("rgx not built")
+    r = subprocess.run([exe, "render", os.path.join(GOLD, "room_scene.json"), "-d", "3"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1 and "only -d 5" in r.stderr
+    r = subprocess.run([exe, "render", "x.json", "--bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "unknown argument" in r.stdout
