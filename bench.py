@@ -149,6 +149,145 @@ def bench_bsdf_frame(ctx, do_cpu, cpu_seconds):
     return res
 
 
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def draw_jobs(n, n_scenes, seed):
+    """n render jobs drawn from one seed: scene uniform over the resident
+    scenes, width and height each log-uniform in [128, 1024], spp uniform in
+    [4, 256], depth uniform in [4, 8], a distinct Philox key per job."""
+    import math
+
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        scene = int(rng.integers(0, n_scenes))
+        w = int(round(math.exp(rng.uniform(math.log(128), math.log(1024)))))
+        h = int(round(math.exp(rng.uniform(math.log(128), math.log(1024)))))
+        spp = int(rng.integers(4, 257))
+        depth = int(rng.integers(4, 9))
+        jobs.append((scene, dict(width=w, height=h, spp=spp, max_depth=depth, seed=(seed + i) & 0xFFFFFFFF, job=i)))
+    return jobs
+
+
+def bench_batch(ctx, resident, n_jobs=64, seed=0xC3C30000):
+    """64 jobs (cube room or BSDF frame; uneven resolution, spp, depth) as one
+    rg_render_batch over a path pool the autotuner sized, against the same
+    jobs one at a time through rg_render."""
+    import numpy as np
+
+    import rgapi
+
+    spec = draw_jobs(n_jobs, len(resident), seed)
+    jobs = [rgapi.make_job(s, **kw) for s, kw in spec]
+    samples = sum(kw["width"] * kw["height"] * kw["spp"] for _, kw in spec)
+    # warm both paths on a small job
+    resident[0].render(rgapi.default_settings(width=64, height=64, spp=2), hdr_only=True)
+    t0 = time.perf_counter()
+    probe = [rgapi.make_job(i, width=512, height=384, spp=8, max_depth=6) for i in range(len(resident))]
+    tune = rgapi.autotune(ctx, resident, probe)
+    tune_s = time.perf_counter() - t0
+    log("autotune", tune.as_dict(), f"{tune_s:.1f}s")
+    t0 = time.perf_counter()
+    outs, stats = rgapi.render_batch(ctx, resident, jobs, tune=tune, want_sum=False)
+    batch_s = time.perf_counter() - t0
+    log(f"batch {batch_s:.2f}s")
+    single_s, rays, same = 0.0, 0, 0
+    for i, (s, kw) in enumerate(spec):
+        st = rgapi.default_settings(**kw)
+        t0 = time.perf_counter()
+        o = resident[s].render(st, hdr_only=True)
+        single_s += time.perf_counter() - t0
+        rays += o["rays"] + o["shadow_rays"]
+        same += int(np.array_equal(o["hdr"], outs[i]["hdr"]))
+    log(f"one at a time {single_s:.2f}s")
+    brays = int(stats[0]) + int(stats[1])
+    return {"workload": f"{n_jobs} jobs (cube room 36 tris / BSDF frame 2,000,014 tris), W,H log-U[128,1024], "
+                        f"spp U[4,256], depth U[4,8], seed {seed:#x}",
+            "jobs": n_jobs, "samples": samples,
+            "batched_s": round(batch_s, 3), "one_at_a_time_s": round(single_s, 3),
+            "speedup": round(single_s / batch_s, 3),
+            "batched_samples_per_s": round(samples / batch_s, 1),
+            "one_at_a_time_samples_per_s": round(samples / single_s, 1),
+            "batched_mrays_per_s": round(brays / batch_s / 1e6, 2),
+            "one_at_a_time_mrays_per_s": round(rays / single_s / 1e6, 2),
+            "rays_equal": brays == rays, "images_bit_equal": f"{same}/{n_jobs}",
+            "tune": tune.as_dict(), "autotune_s": round(tune_s, 2)}
+
+
+def bench_serve(ctx, resident, n_req, rates, seed=0xC4C40000, budget=32 << 30, n_plain=24):
+    """Poisson request stream over four resident scenes: served by rg_server
+    (continuous batching, admission by framebuffer footprint) at several
+    arrival rates, with p50/p99 latency, against the plain path one request
+    at a time."""
+    import numpy as np
+
+    import rgapi
+
+    spec = draw_jobs(n_req, len(resident), seed)
+    jobs = [rgapi.make_job(s, **kw) for s, kw in spec]
+    sam = [kw["width"] * kw["height"] * kw["spp"] for _, kw in spec]
+    # plain path, one at a time, on the first n_plain requests
+    plain_s, plain_hdr = 0.0, {}
+    for i in range(min(n_plain, n_req)):
+        s, kw = spec[i]
+        t0 = time.perf_counter()
+        o = resident[s].render(rgapi.default_settings(**kw), hdr_only=True)
+        plain_s += time.perf_counter() - t0
+        plain_hdr[i] = o["hdr"]
+    plain_sps = sum(sam[:len(plain_hdr)]) / plain_s
+    lam_plain = plain_sps / float(np.mean(sam))
+    log(f"plain {plain_sps / 1e6:.1f} Msamples/s -> {lam_plain:.2f} req/s")
+    res = {"workload": f"{n_req} requests per rate, drawn like the batch jobs over 4 resident scenes "
+                       f"(cube room, two BSDF frames, GGX icosphere 1,310,736 tris + medium box), "
+                       f"Poisson arrivals, HBM budget {budget >> 30} GiB",
+           "plain_samples_per_s": round(plain_sps, 1), "plain_requests": len(plain_hdr),
+           "rates": []}
+    rng = np.random.default_rng(seed ^ 0x5A5A)
+    for f in rates:
+        lam = f * lam_plain
+        gaps = rng.exponential(1.0 / lam, n_req)
+        srv = rgapi.Server(ctx, resident, hbm_budget=budget)
+        tickets = []
+        t_start = time.perf_counter()
+        t_next = t_start
+        for i, j in enumerate(jobs):
+            t_next += gaps[i]
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit(j))
+        lat, first, last, rmse = [], None, None, []
+        for i, t in enumerate(tickets):
+            info, out = srv.wait(t, timeout=600.0)
+            if info is None or info.status != 0:
+                raise RuntimeError(f"request {i} failed")
+            lat.append(info.done_s - info.submit_s)
+            first = info.submit_s if first is None else min(first, info.submit_s)
+            last = info.done_s if last is None else max(last, info.done_s)
+            if i in plain_hdr:
+                a, b = out["hdr"][..., :3], plain_hdr[i][..., :3]
+                rmse.append(float(np.sqrt(np.mean((a - b) ** 2)) / max(float(b.mean()), 1e-12)))
+        st = srv.stats()
+        tn = srv.tune().as_dict()
+        srv.close()
+        span = last - first
+        lat = np.array(lat)
+        r = {"rate_factor": f, "arrival_rate_req_s": round(lam, 3),
+             "served_samples_per_s": round(sum(sam) / span, 1),
+             "vs_plain": round(sum(sam) / span / plain_sps, 3),
+             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+             "makespan_s": round(span, 3), "max_rel_rmse_vs_plain": max(rmse) if rmse else None,
+             "graph_replays": st["graphs"], "tune": tn}
+        log("rate", r)
+        res["rates"].append(r)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +296,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-render", action="store_true", help="traversal microbenchmark only")
+    ap.add_argument("--serve-requests", type=int, default=60, help="requests per arrival rate")
+    ap.add_argument("--skip-many", action="store_true", help="skip the batched and served legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -252,8 +393,22 @@ def main():
         torch.cuda.synchronize()
         extra["copy_calibration_bytes"] = 1 << 30
     if not args.skip_render:
+        log("cube room")
         extra["cube_room"] = bench_cube_room(ctx, stream, max(3, args.steps // 2), 8.0, not args.no_cpu_baseline)
+        log("bsdf frame")
         extra["bsdf_frame"] = bench_bsdf_frame(ctx, not args.no_cpu_baseline, 8.0)
+    if not args.skip_render and not args.skip_many:
+        import scenes
+
+        log("resident scenes")
+        room = rgapi.RenderScene(ctx, scenes.cube_room())
+        frame = rgapi.RenderScene(ctx, scenes.bsdf_frame(grid=5, stacks=200, slices=200))
+        log("batch")
+        extra["batch"] = bench_batch(ctx, [room, frame])
+        frame2 = rgapi.RenderScene(ctx, scenes.bsdf_frame(grid=5, stacks=200, slices=200, seed=0xB5DF0003))
+        ico = rgapi.RenderScene(ctx, scenes.ico_ggx())
+        log("serve")
+        extra["serve"] = bench_serve(ctx, [room, frame, frame2, ico], args.serve_requests, (0.5, 1.0, 2.0))
 
     out = {
         "metric": "closest-hit traversal throughput, 1.31M-tri displaced icosphere, 16M incoherent rays",

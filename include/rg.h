@@ -29,6 +29,7 @@ extern "C" {
 #define RG_E_UNSUPPORTED -6 /* option outside the built path */
 #define RG_E_BUDGET -7      /* request exceeds the context's memory budget */
 #define RG_E_IO -8          /* file could not be read or parsed */
+#define RG_TIMEOUT 1        /* rg_server_wait: not finished within the timeout */
 
 typedef struct rg_ctx rg_ctx;
 typedef struct rg_scene rg_scene;
@@ -242,6 +243,72 @@ int rg_scene_export_media(const rg_scene *scene, rg_medium *media, uint32_t *n_m
 /* Counters of the last rg_render on this context: [0] extension (closest-hit)
  * rays, [1] NEE shadow (any-hit) rays, [2] path samples. */
 int rg_render_stats(rg_ctx *ctx, uint64_t out[3]);
+
+/* ---- many jobs on one GPU ------------------------------------------------
+ * The reference renders one job per RayRenderer::render call
+ * (rayrenderer.cpp:478-596, driven by main.cpp:162-490).  The entry points
+ * below run many such jobs together: one pool of path slots is refilled
+ * every iteration from the samples of all admitted jobs, the iterations are
+ * replayed as a captured hipGraph, and each job's image is what rg_render
+ * returns for the same scene and settings (same per-sample random streams,
+ * same per-pixel summation order).  Jobs render a fixed sample count:
+ * settings.adaptive and settings.denoise must be 0. */
+typedef struct {
+    uint32_t scene;        /* index into the scene list of the batch / server */
+    uint32_t pad;
+    rg_settings settings;  /* width, height, spp, max_depth, antialias,
+                              firefly_clamp, seed, job, samples_in_flight */
+} rg_job;
+
+/* Engine knobs; 0 picks the default.  rg_autotune measures candidates. */
+typedef struct {
+    uint32_t pool_paths;       /* path slots in flight (default 2^21) */
+    uint32_t stack;            /* traversal stack entries 32/64/128/256
+                                  (default: smallest that fits every BVH) */
+    uint32_t tile;             /* pixel tile edge of the issue order (default 8) */
+    uint32_t blocks_per_cu;    /* persistent any-hit blocks per compute unit (default 8) */
+    uint32_t iters_per_graph;  /* iterations per hipGraph replay, even (default 4) */
+    uint32_t pad;
+} rg_tune;
+
+/* Renders n_jobs jobs over the scenes (BVH built, shading set) and blocks
+ * until all are done.  hdr_out[i] (W*H*4 floats) and sum_out[i] (W*H*3) are
+ * host buffers (the arrays and their entries may be NULL).  stats (nullable):
+ * [0] extension rays, [1] shadow rays, [2] samples, [3] iterations. */
+int rg_render_batch(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const rg_job *jobs, uint32_t n_jobs,
+                    const rg_tune *tune, float *const *hdr_out, float *const *sum_out, uint64_t stats[4]);
+
+/* Times every candidate of a small grid of pool / stack / tile / blocks
+ * settings on the probe jobs (NULL: one 512x384, 8 spp, depth 6 job per
+ * scene) and returns the fastest. */
+int rg_autotune(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const rg_job *probe, uint32_t n_probe,
+                rg_tune *out);
+
+/* Request server: keeps the scenes resident, admits requests first come,
+ * first served as a job slot and hbm_budget bytes of framebuffer space
+ * (per-sample buffer + sums + image) allow, and renders every admitted
+ * request together with the others on a thread of its own. */
+typedef struct rg_server rg_server;
+typedef struct {
+    int32_t status;     /* RG_OK or the failure */
+    uint32_t pad;
+    uint64_t samples;   /* W*H*spp */
+    double submit_s, admit_s, done_s;  /* library steady clock, seconds */
+} rg_request_info;
+
+int rg_server_create(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, uint64_t hbm_budget,
+                     const rg_tune *tune, rg_server **out);
+/* Queues one request; hdr_out / sum_out (nullable host buffers) are written
+ * before rg_server_wait reports it done. */
+int rg_server_submit(rg_server *srv, const rg_job *job, float *hdr_out, float *sum_out, uint64_t *ticket);
+/* Waits up to timeout_s (< 0: forever) for a request; RG_TIMEOUT if still
+ * running.  A finished ticket is forgotten after it is reported. */
+int rg_server_wait(rg_server *srv, uint64_t ticket, double timeout_s, rg_request_info *info);
+/* [0] requests done, [1] samples done, [2] extension rays, [3] shadow rays,
+ * [4] iterations, [5] framebuffer bytes in use, [6] budget, [7] graph replays */
+int rg_server_stats(rg_server *srv, uint64_t out[8]);
+int rg_server_get_tune(rg_server *srv, rg_tune *out);
+int rg_server_destroy(rg_server *srv);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,7 @@ void rg_destroy(rg_ctx *ctx) {
         release_pt(c);
     }
     if (c->own_stream) dev_stream_destroy(c->own_stream);
+    if (c->device >= 0) dev_clear_error();
     delete c;
 }
 
@@ -103,7 +104,7 @@ int rg_scene_create(rg_ctx *ctx, const float *v9, uint32_t n_tris, rg_scene **ou
     *out = nullptr;
     Scene *s = new (std::nothrow) Scene();
     if (!s) return set_error(RG_E_NOMEM, "scene allocation failed");
-    s->ctx = reinterpret_cast<Ctx *>(ctx);
+    s->device = reinterpret_cast<Ctx *>(ctx)->device;
     s->n_tris = n_tris;
     try {
         s->v9.assign(v9, v9 + 9 * (size_t)n_tris);
@@ -125,10 +126,13 @@ int rg_scene_create(rg_ctx *ctx, const float *v9, uint32_t n_tris, rg_scene **ou
 void rg_scene_destroy(rg_scene *scene) {
     Scene *s = reinterpret_cast<Scene *>(scene);
     if (!s) return;
-    if (s->ctx && s->ctx->device >= 0) {
-        dev_set_device(s->ctx->device);
+    // the scene may outlive the context it was created on: it keeps its own
+    // device id
+    if (s->device >= 0) {
+        dev_set_device(s->device);
         free_device(s);
         release_pt_scene(s);
+        dev_clear_error();
     }
     delete s;
 }
@@ -155,6 +159,7 @@ int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params) {
 
     if (int rc = dev_set_device(c->device)) return rc;
     free_device(s);
+    s->device = c->device;
     std::vector<Tri48> tris;
     make_tris(s->v9.data(), s->nodes2, s->perm, s->pred_mask, tris);
     const size_t b4 = s->nodes4.size() * sizeof(Node4), bt = tris.size() * sizeof(Tri48);

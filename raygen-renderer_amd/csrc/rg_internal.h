@@ -67,7 +67,8 @@ struct PTBuffers {
     uint64_t npix_cap = 0;  // pixels
     uint64_t ntile_cap = 0; // 32x32 tiles
     void *ray = nullptr, *hit = nullptr, *beta = nullptr, *tp = nullptr, *L = nullptr, *meta = nullptr;
-    void *sray = nullptr, *scontrib = nullptr, *nshadow = nullptr, *q0 = nullptr, *q1 = nullptr;
+    void *sray = nullptr, *ssrc = nullptr, *socc = nullptr, *scontrib = nullptr, *nshadow = nullptr;
+    void *q0 = nullptr, *q1 = nullptr;
     void *count = nullptr, *sample_out = nullptr;
     void *sum = nullptr, *sumsq = nullptr, *hdr = nullptr;
     void *tile_mask = nullptr, *tile_nspp = nullptr, *tile_noise = nullptr, *n_active = nullptr;
@@ -85,7 +86,7 @@ struct Ctx {
 };
 
 struct Scene {
-    Ctx *ctx = nullptr;
+    int device = -1;                  // device holding the scene's buffers (-1: none)
     uint32_t n_tris = 0;
     std::vector<float> v9;            // as handed in
     float box[6] = {0, 0, 0, 0, 0, 0};
@@ -151,6 +152,7 @@ int dev_upload(void *dst, const void *src, size_t bytes, void *stream);
 int dev_download(void *dst, const void *src, size_t bytes, void *stream);
 int dev_sync(void *stream);
 int dev_set_device(int device);
+void dev_clear_error();
 int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n, void *stream);
 int launch_any(const Scene &s, const rg_ray *rays, uint8_t *occ, uint64_t n, int pred, void *stream);
 int launch_gen_rays(const float box[6], float scale, uint32_t key, uint64_t first,

@@ -82,15 +82,22 @@ struct PTState {
     float4 *tp;         // BSDFParam::throughput (Russian roulette)
     float4 *L;          // radiance so far, w = alive flag
     int4 *meta;         // passes, medium, next LDS dim, next Philox draw
-    rg_ray *sray;       // 2 NEE shadow rays per path (tmin = predicate)
-    float4 *scontrib;   // their unoccluded contributions
-    uint32_t *nshadow;
+    rg_ray *sray;       // NEE shadow rays per (path, k), k < 2 (tmin = predicate)
+    uint32_t *ssrc;     // queue of the (path, k) pairs with a shadow ray: 2 * path + k
+    uint8_t *socc;      // occluded flag per (path, k)
+    float4 *scontrib;   // unoccluded contribution per (path, k)
+    uint32_t *nshadow;  // shadow rays per path this event
     uint32_t *q[2];     // live-path queues
-    uint32_t *count;    // [2] queue lengths
+    uint32_t *count;    // [0], [1] path-queue lengths, [2] shadow-ray queue length
     float4 *sample_out; // clamped sample radiance per path
     unsigned long long *stats; // [0] extension rays, [1] shadow rays, [2] path samples
     uint32_t sample_base;
 };
+
+// Kernel-side scene view of one job: scene buffers, camera and settings.
+PTScene make_ptscene(const Scene &scn, const rg_settings &st);
+// Validates shading inputs and uploads what changed since the last upload.
+int prepare_scene(Scene *s, void *stream);
 
 int launch_generation(const Scene &scn, const PTScene &sc, const PTState &st, uint32_t S, uint32_t sample_base,
                       const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream);

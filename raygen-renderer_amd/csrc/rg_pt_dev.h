@@ -1,0 +1,988 @@
+// rg_pt_dev.h — device functions of the BSDF path tracer (shading system 5)
+// shared by the single-job kernels (rg_pt.hip) and the batched streaming
+// kernels (rg_stream.hip): vector helpers, random numbers, scene access,
+// environment, media, GGX, NEE sampling, one path event (step) and the
+// camera ray.  Binary32 arithmetic in explicit order, no contraction.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "rg_device.h"
+#include "rg_internal.h"
+#include "rg_pt.h"
+
+namespace rg {
+
+#define PI_F 3.14159265358979323846f
+#define SURFACE_THICKNESS 0.0001f
+#define MAX_RAY_DISTANCE 1.0e6f
+#define RAY_MAX_DISTANCE 100.0f
+#define MIN_RR_DEPTH 3
+#define RR_MIN_PROB 0.05f
+#define RR_MAX_PROB 0.95f
+
+// ---------------------------------------------------------------------------
+// vector helpers: explicit component order, no contraction
+// ---------------------------------------------------------------------------
+struct v3 {
+    float x, y, z;
+};
+__device__ __forceinline__ v3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 mul(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ v3 scl(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ v3 cross(v3 a, v3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float len(v3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ v3 nrm(v3 a) {
+    const float inv = 1.0f / len(a);
+    return scl(a, inv);
+}
+__device__ __forceinline__ v3 ld3(const float *p) { return mk(p[0], p[1], p[2]); }
+__device__ __forceinline__ float maxc(v3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__device__ __forceinline__ bool is_zero(v3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+__device__ __forceinline__ v3 reflect(v3 d, v3 n) { return sub(d, scl(n, dot(d, n) * 2.0f)); }
+__device__ __forceinline__ v3 f4v(float4 a) { return mk(a.x, a.y, a.z); }
+__device__ __forceinline__ float4 v4(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// ---------------------------------------------------------------------------
+// random numbers: Halton + per-pixel Cranley-Patterson (raycommon.cpp:21-92)
+// for the reference's LDS call sites, Philox4x32-10 for the rest
+// ---------------------------------------------------------------------------
+struct Rng {
+    int x, y, sample, dim;
+    uint32_t pix, key0, key1, k;
+    uint32_t blk[4];
+    uint32_t blk_id;
+};
+
+__device__ __forceinline__ float prng(Rng &r) {
+    const uint32_t b = r.k >> 2;
+    if (b != r.blk_id) {
+        uint32_t c[4] = {r.pix, (uint32_t)r.sample, b, 0u};
+        philox(c, r.key0, r.key1);
+        r.blk[0] = c[0];
+        r.blk[1] = c[1];
+        r.blk[2] = c[2];
+        r.blk[3] = c[3];
+        r.blk_id = b;
+    }
+    const uint32_t w = r.blk[r.k & 3];
+    r.k++;
+    return (float)(w >> 8) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ uint32_t hash_pixel(int x, int y, uint32_t salt) {
+    uint32_t v = (uint32_t)x * 73856093u ^ (uint32_t)y * 19349663u ^ salt * 83492791u;
+    v ^= v >> 16;
+    v *= 0x7feb352du;
+    v ^= v >> 15;
+    v *= 0x846ca68bu;
+    v ^= v >> 16;
+    return v;
+}
+
+__device__ __forceinline__ float radical_inverse(uint32_t i, uint32_t base) {
+    const float inv_base = 1.0f / (float)base;
+    float inv_base_n = 1.0f;
+    uint64_t rev = 0;
+    while (i > 0) {
+        const uint32_t next = i / base;
+        const uint32_t digit = i - next * base;
+        rev = rev * base + digit;
+        inv_base_n *= inv_base;
+        i = next;
+    }
+    const float v = (float)rev * inv_base_n;
+    return v < 1.0f ? v : 0.9999999f;
+}
+
+__device__ __forceinline__ float lds1(Rng &r) {
+    if (r.dim >= 16) return prng(r);
+    const uint32_t primes[16] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53};
+    const uint32_t scramble = hash_pixel(r.x, r.y, 0);
+    const float v = radical_inverse((uint32_t)r.sample + 1u, primes[r.dim]);
+    const uint32_t h = hash_pixel(r.dim, 0, scramble);
+    const float offset = (float)(h >> 8) * (1.0f / (float)(1u << 24));
+    float w = v + offset;
+    if (w >= 1.0f) w -= 1.0f;
+    r.dim++;
+    return w;
+}
+
+__device__ __forceinline__ uint32_t pick(float u, uint32_t n) {
+    const uint32_t i = (uint32_t)(u * (float)n);
+    return i < n ? i : n - 1;
+}
+
+__device__ __forceinline__ Rng rng_for(const PTScene &sc, uint32_t pix, int sample, int dim, uint32_t k) {
+    Rng r;
+    r.x = (int)(pix % sc.width);
+    r.y = (int)(pix / sc.width);
+    r.sample = sample;
+    r.dim = dim;
+    r.pix = pix;
+    r.key0 = sc.seed;
+    r.key1 = sc.job;
+    r.k = k;
+    r.blk_id = 0xFFFFFFFFu;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// scene access
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const DevObject &obj_of(const PTScene &sc, uint32_t t) {
+    return sc.objs[sc.shade[t].obj];
+}
+__device__ __forceinline__ v3 tri_v(const PTScene &sc, uint32_t t, int k) { return ld3(sc.v9 + 9 * (size_t)t + 3 * k); }
+__device__ __forceinline__ v3 tri_n(const PTScene &sc, uint32_t t, int k) { return ld3(sc.shade[t].n + 3 * k); }
+
+// geomNormal (bsdf.cpp:27-30)
+__device__ __forceinline__ v3 geom_normal(const PTScene &sc, uint32_t t, v3 shading_n) {
+    const v3 g = ld3(sc.shade[t].gn);
+    return dot(g, shading_n) >= 0.0f ? g : neg(g);
+}
+
+// SurfaceRay (raycommon.h:82-85): offset origin
+__device__ __forceinline__ v3 surface_origin(v3 hit, v3 dir, v3 n) {
+    const float side = (dot(dir, n) >= 0.0f) ? 1.0f : -1.0f;
+    return add(hit, scl(n, side * SURFACE_THICKNESS));
+}
+
+// calcVertexInterpolation(info) (rayrenderer.cpp:2354-2362)
+__device__ __forceinline__ v3 interp_normal(const PTScene &sc, uint32_t t, float u, float v) {
+    const float w = 1.0f - u - v;
+    const v3 n = add(add(scl(tri_n(sc, t, 0), w), scl(tri_n(sc, t, 1), u)), scl(tri_n(sc, t, 2), v));
+    return nrm(n);
+}
+
+// calcVertexInterpolation(rt, hit) (rayrenderer.cpp:2337-2352)
+__device__ __forceinline__ v3 interp_normal_at(const PTScene &sc, uint32_t t, v3 p) {
+    const v3 f1 = sub(tri_v(sc, t, 0), p), f2 = sub(tri_v(sc, t, 1), p), f3 = sub(tri_v(sc, t, 2), p);
+    const float ta = sc.shade[t].ti_a;
+    const float a1 = len(cross(f2, f3)) * ta;
+    const float a2 = len(cross(f3, f1)) * ta;
+    const float a3 = len(cross(f1, f2)) * ta;
+    return add(add(scl(tri_n(sc, t, 0), a1), scl(tri_n(sc, t, 1), a2)), scl(tri_n(sc, t, 2), a3));
+}
+
+// ---------------------------------------------------------------------------
+// environment (rayrenderer.cpp:1299-1573)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ v3 env_texel(const PTScene &sc, int x, int y) {
+    return ld3(sc.env_rgb + 3 * ((size_t)y * sc.env_w + (size_t)x));
+}
+__device__ __forceinline__ float lum(v3 c) { return fmaxf(0.0f, 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z); }
+
+__device__ v3 sample_environment(const PTScene &sc, v3 dir) {
+    if (!sc.has_env) return mk(0, 0, 0);
+    const v3 d = nrm(dir);
+    const float cy = sc.env_cos, sy = sc.env_sin;
+    const float rx = d.x * cy - d.z * sy;
+    const float rz = d.x * sy + d.z * cy;
+    const float ry = d.y;
+    const float u = 0.5f + atan2f(rz, rx) * (float)(0.5 / M_PI);
+    const float v = 0.5f - asinf(fminf(fmaxf(ry, -1.0f), 1.0f)) * (float)(1.0 / M_PI);
+    const int W = (int)sc.env_w, H = (int)sc.env_h;
+    if (!(u == u) || !(v == v)) return mk(0, 0, 0);
+    float fu = u - floorf(u), fv = v - floorf(v);
+    if (fu < 0.0f || fu >= 1.0f) fu = 0.0f;
+    if (fv < 0.0f || fv >= 1.0f) fv = 0.0f;
+    int x = (int)(fu * (float)W), y = (int)(fv * (float)H);
+    if (x >= W) x = W - 1;
+    if (y >= H) y = H - 1;
+    return scl(env_texel(sc, x, y), sc.env_intensity);
+}
+
+__device__ __forceinline__ int search_cdf(const float *cdf, int n, float u) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid + 1] <= u) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n ? lo : n - 1;
+}
+
+__device__ void sample_env_dir(const PTScene &sc, float u0, float u1, v3 &out, float &pdf) {
+    out = mk(0, 1, 0);
+    pdf = 0.0f;
+    const int W = (int)sc.env_w, H = (int)sc.env_h;
+    if (!sc.has_env || sc.env_total <= 0.0f) return;
+    const int y = search_cdf(sc.env_marg, H, u0);
+    const float *row = sc.env_cond + (size_t)y * (W + 1);
+    const int x = search_cdf(row, W, u1);
+    const float yp = sc.env_marg[y], ys = sc.env_marg[y + 1] - yp;
+    const float xp = row[x], xs = row[x + 1] - xp;
+    const float yf = (ys > 0.0f) ? (u0 - yp) / ys : 0.5f;
+    const float xf = (xs > 0.0f) ? (u1 - xp) / xs : 0.5f;
+    const float u = ((float)x + xf) / (float)W;
+    const float v = ((float)y + yf) / (float)H;
+    const float phi = (u - 0.5f) * 2.0f * PI_F;
+    const float theta = (0.5f - v) * PI_F;
+    const float ct = cosf(theta), st = sinf(theta);
+    const float dx = ct * cosf(phi), dy = st, dz = ct * sinf(phi);
+    const float cy = sc.env_cos, sy = sc.env_sin;
+    const float wx = dx * cy + dz * sy;
+    const float wz = -dx * sy + dz * cy;
+    out = nrm(mk(wx, dy, wz));
+    const float l = lum(env_texel(sc, x, y));
+    const float sin_t = sinf(PI_F * v);
+    if (sin_t <= 0.0f) {
+        pdf = 0.0f;
+        return;
+    }
+    const float pdf_uv = (l * sin_t) / sc.env_total;
+    pdf = pdf_uv * (float)(W * H) / (2.0f * (float)(M_PI * M_PI) * sin_t);
+}
+
+__device__ float env_dir_pdf(const PTScene &sc, v3 dir) {
+    if (!sc.has_env || sc.env_total <= 0.0f) return 0.0f;
+    const v3 d = nrm(dir);
+    const float cy = sc.env_cos, sy = sc.env_sin;
+    const float rx = d.x * cy - d.z * sy;
+    const float rz = d.x * sy + d.z * cy;
+    const int W = (int)sc.env_w, H = (int)sc.env_h;
+    const float u = 0.5f + atan2f(rz, rx) * (float)(0.5 / M_PI);
+    const float v = 0.5f - asinf(fminf(fmaxf(d.y, -1.0f), 1.0f)) * (float)(1.0 / M_PI);
+    int x = (int)(u * (float)W);
+    if (x < 0) x = 0;
+    if (x >= W) x = W - 1;
+    int y = (int)(v * (float)H);
+    if (y < 0) y = 0;
+    if (y >= H) y = H - 1;
+    const float l = lum(env_texel(sc, x, y));
+    const float sin_t = sinf(PI_F * v);
+    if (sin_t <= 0.0f) return 0.0f;
+    const float pdf_uv = (l * sin_t) / sc.env_total;
+    return pdf_uv * (float)(W * H) / (2.0f * (float)(M_PI * M_PI) * sin_t);
+}
+
+__device__ __forceinline__ float env_miss_weight(const PTScene &sc, v3 dir, float bsdf_pdf) {
+    if (bsdf_pdf > 0.0f) {
+        const float ep = env_dir_pdf(sc, dir);
+        const float b2 = bsdf_pdf * bsdf_pdf, e2 = ep * ep;
+        return (b2 + e2 > 0.0f) ? b2 / (b2 + e2) : 1.0f;
+    }
+    return 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// homogeneous media (medium.cpp:18-123, 475-525)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ v3 transmittance(const DevMedium &m, float d) {
+    if (d <= 0.0f) return mk(1, 1, 1);
+    return mk(expf(-m.sigma_t[0] * d), expf(-m.sigma_t[1] * d), expf(-m.sigma_t[2] * d));
+}
+__device__ __forceinline__ float ew_channel(float st, float d) {
+    if (st < 1e-6f) return d;
+    return (1.0f - expf(-st * d)) / st;
+}
+__device__ __forceinline__ v3 emission_along(const DevMedium &m, float d) {
+    if (d <= 0.0f) return mk(0, 0, 0);
+    return mk(m.sigma_e[0] * ew_channel(m.sigma_t[0], d), m.sigma_e[1] * ew_channel(m.sigma_t[1], d),
+              m.sigma_e[2] * ew_channel(m.sigma_t[2], d));
+}
+__device__ __forceinline__ float free_flight(const DevMedium &m, float u) {
+    if (m.hero <= 0.0f) return 1e30f;
+    const float xi = fminf(0.9999999f, fmaxf(0.0f, u));
+    return -logf(1.0f - xi) / m.hero;
+}
+__device__ __forceinline__ float phase_pdf(const DevMedium &m, v3 wo, v3 wi) {
+    const float c = dot(wo, wi);
+    const float g = m.g;
+    const float den = 1.0f + g * g - 2.0f * g * c;
+    return (1.0f - g * g) * (float)(1.0 / (4.0 * M_PI)) / (den * sqrtf(fmaxf(1e-8f, den)));
+}
+__device__ v3 sample_phase(const DevMedium &m, v3 wo, float u1, float u2) {
+    float ct;
+    if (fabsf(m.g) < 1e-3f) ct = 1.0f - 2.0f * u1;
+    else {
+        const float g = m.g;
+        const float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
+        ct = (1.0f + g * g - sq * sq) / (2.0f * g);
+    }
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    const float phi = 2.0f * PI_F * u2;
+    v3 t1;
+    if (fabsf(wo.x) > fabsf(wo.y)) {
+        const float il = 1.0f / sqrtf(wo.x * wo.x + wo.z * wo.z);
+        t1 = mk(-wo.z * il, 0.0f, wo.x * il);
+    } else {
+        const float il = 1.0f / sqrtf(wo.y * wo.y + wo.z * wo.z);
+        t1 = mk(0.0f, wo.z * il, -wo.y * il);
+    }
+    const v3 t2 = cross(wo, t1);
+    return nrm(add(add(scl(t1, st * cosf(phi)), scl(t2, st * sinf(phi))), scl(wo, ct)));
+}
+
+// ---------------------------------------------------------------------------
+// sampling helpers
+// ---------------------------------------------------------------------------
+// cosineWeightedDirection (raycommon.cpp:94-115)
+__device__ v3 cosine_dir(v3 n, float r1, float r2) {
+    const float theta = acosf(sqrtf(1.0f - r1));
+    const float phi = 2.0f * PI_F * r2;
+    const float x = sinf(theta) * cosf(phi);
+    const float y = sinf(theta) * sinf(phi);
+    const float z = cosf(theta);
+    v3 tg;
+    if (fabsf(n.x) > fabsf(n.y)) tg = nrm(cross(mk(0, 1, 0), n));
+    else tg = nrm(cross(mk(1, 0, 0), n));
+    const v3 bt = cross(n, tg);
+    return nrm(add(add(scl(tg, x), scl(bt, y)), scl(n, z)));
+}
+
+// ldsPointInTriangle (raycommon.cpp:117-122)
+__device__ __forceinline__ v3 point_in_tri(const PTScene &sc, uint32_t t, float u, float v) {
+    if (u + v > 1.0f) {
+        u = 1.0f - u;
+        v = 1.0f - v;
+    }
+    const v3 a = tri_v(sc, t, 0);
+    return add(add(a, scl(sub(tri_v(sc, t, 1), a), u)), scl(sub(tri_v(sc, t, 2), a), v));
+}
+
+// randomRayInHemisphere (ugm, not vendored): uniform hemisphere about n
+__device__ v3 hemisphere_dir(v3 n, float u1, float u2) {
+    const float z = u1;
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float phi = 2.0f * PI_F * u2;
+    v3 tg;
+    if (fabsf(n.x) > fabsf(n.y)) tg = nrm(cross(mk(0, 1, 0), n));
+    else tg = nrm(cross(mk(1, 0, 0), n));
+    const v3 bt = cross(n, tg);
+    return add(add(scl(tg, r * cosf(phi)), scl(bt, r * sinf(phi))), scl(n, z));
+}
+
+__device__ __forceinline__ float fresnel_schlick(float c, float ior) {
+    float r0 = (1.0f - ior) / (1.0f + ior);
+    r0 = r0 * r0;
+    const float m = 1.0f - c;
+    return r0 + (1.0f - r0) * (m * m) * (m * m) * m;
+}
+
+// refract (bsdf.cpp:39-52)
+__device__ v3 refract_dir(v3 d, v3 n, float r) {
+    const v3 nl = dot(d, n) < 0.0f ? n : neg(n);
+    const bool into = dot(nl, n) > 0.0f;
+    if (into) r = 1.0f / r;
+    const float c = dot(d, nl);
+    const float t = 1.0f - r * r * (1.0f - c * c);
+    if (t < 0.0f) return reflect(d, n);
+    return nrm(sub(scl(d, r), scl(n, (into ? 1.0f : -1.0f) * (c * r + sqrtf(t)))));
+}
+
+struct Ggx {
+    v3 t, b, n;
+    float ax, ay;
+    v3 V;
+    float g1v;
+    v3 F0;
+};
+
+__device__ __forceinline__ float smith_g1(v3 w, float ax, float ay) {
+    if (w.z <= 0.0f) return 0.0f;
+    const float ax2 = (ax * w.x) * (ax * w.x);
+    const float ay2 = (ay * w.y) * (ay * w.y);
+    const float lambda = 0.5f * (sqrtf(1.0f + (ax2 + ay2) / (w.z * w.z)) - 1.0f);
+    return 1.0f / (1.0f + lambda);
+}
+// Heitz 2018 VNDF sampling (bsdf.cpp:139-161)
+__device__ v3 sample_vndf(v3 Ve, float ax, float ay, float u1, float u2) {
+    const v3 Vh = nrm(mk(ax * Ve.x, ay * Ve.y, Ve.z));
+    const float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+    const v3 T1 = (lensq > 0.0f) ? scl(mk(-Vh.y, Vh.x, 0.0f), 1.0f / sqrtf(lensq)) : mk(1, 0, 0);
+    const v3 T2 = cross(Vh, T1);
+    const float r = sqrtf(u1);
+    const float phi = 2.0f * PI_F * u2;
+    const float t1 = r * cosf(phi);
+    float t2 = r * sinf(phi);
+    const float s = 0.5f * (1.0f + Vh.z);
+    t2 = (1.0f - s) * sqrtf(fmaxf(0.0f, 1.0f - t1 * t1)) + s * t2;
+    const v3 Nh = add(add(scl(T1, t1), scl(T2, t2)), scl(Vh, sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2))));
+    return nrm(mk(ax * Nh.x, ay * Nh.y, fmaxf(0.0f, Nh.z)));
+}
+__device__ __forceinline__ v3 schlick_f(v3 F0, float vdoth) {
+    const float a1 = fmaxf(0.0f, 1.0f - vdoth);
+    const float a5 = a1 * a1 * a1 * a1 * a1;
+    return add(F0, scl(sub(mk(1, 1, 1), F0), a5));
+}
+__device__ __forceinline__ float ggx_d(v3 H, float ax, float ay) {
+    const float hx = H.x / ax, hy = H.y / ay;
+    const float den = hx * hx + hy * hy + H.z * H.z;
+    return 1.0f / (PI_F * ax * ay * den * den);
+}
+// evalGlossy (bsdf.cpp:258-280)
+__device__ bool eval_glossy(const Ggx &G, v3 dw, v3 &fcos, float &pdf) {
+    const v3 Ll = mk(dot(dw, G.t), dot(dw, G.b), dot(dw, G.n));
+    if (Ll.z <= 0.0f) return false;
+    v3 Hl = add(G.V, Ll);
+    const float h2 = Hl.x * Hl.x + Hl.y * Hl.y + Hl.z * Hl.z;
+    if (h2 <= 0.0f) return false;
+    Hl = scl(Hl, 1.0f / sqrtf(h2));
+    if (Hl.z <= 0.0f) return false;
+    const float vdh = fmaxf(0.0f, dot(G.V, Hl));
+    if (vdh <= 0.0f) return false;
+    const float D = ggx_d(Hl, G.ax, G.ay);
+    const float g1l = smith_g1(Ll, G.ax, G.ay);
+    const v3 F = schlick_f(G.F0, vdh);
+    fcos = scl(F, D * G.g1v * g1l / (4.0f * G.V.z));
+    pdf = G.g1v * D / (4.0f * G.V.z);
+    return true;
+}
+__device__ bool ggx_setup(const DevObject &m, v3 n, v3 in_dir, Ggx &G) {
+    const v3 alb = ld3(m.color);
+    G.F0 = (m.metallic >= 1.0f) ? alb : add(scl(alb, m.metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m.metallic));
+    const v3 up = (fabsf(n.y) > 0.99f) ? mk(1, 0, 0) : mk(0, 1, 0);
+    G.t = nrm(cross(up, n));
+    G.b = cross(n, G.t);
+    if (fabsf(m.aniso_rotation) > 0.0f) {
+        const float a = m.aniso_rotation * (float)(M_PI / 180.0);
+        const float ca = cosf(a), sa = sinf(a);
+        const v3 t2 = add(scl(G.t, ca), scl(G.b, sa));
+        const v3 b2 = sub(scl(G.b, ca), scl(G.t, sa));
+        G.t = t2;
+        G.b = b2;
+    }
+    G.n = n;
+    const float an = fmaxf(-0.99f, fminf(0.99f, m.anisotropy));
+    const float r2 = m.roughness * m.roughness;
+    G.ax = fmaxf(1e-3f, r2 * (1.0f - an));
+    G.ay = fmaxf(1e-3f, r2 * (1.0f + an));
+    const v3 V = neg(in_dir);
+    G.V = mk(dot(V, G.t), dot(V, G.b), dot(V, n));
+    if (G.V.z <= 0.0f) return false;
+    G.g1v = smith_g1(G.V, G.ax, G.ay);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// shadow-ray emission: NEE samples are drawn in the shade kernel and tested
+// in the shadow kernel; a contribution is added only when unoccluded
+// ---------------------------------------------------------------------------
+struct Shadows {
+    int n;
+    float4 o[2], d[2];   // d.w = tmax, o.w = predicate
+    float4 c[2];
+};
+
+__device__ __forceinline__ void push_shadow(Shadows &S, v3 o, v3 d, float tmax, int pred, v3 contrib) {
+    S.o[S.n] = v4(o, (float)pred);
+    S.d[S.n] = v4(d, tmax);
+    S.c[S.n] = v4(contrib, 0.0f);
+    S.n++;
+}
+
+struct AreaSample {
+    bool ok;
+    v3 dir, Le, o, lr;
+    float pdf, co, cl, r2, sa;
+};
+
+// sampleAreaLightForNEE (rayrenderer.cpp:1976-2036) up to the shadow test
+__device__ AreaSample sample_area(const PTScene &sc, Rng &R, v3 hit, v3 n) {
+    AreaSample s;
+    s.ok = false;
+    if (sc.nlights == 0) return s;
+    const uint32_t li = pick(prng(R), sc.nlights);
+    const DevLight L = sc.lights[li];
+    const uint32_t tri = sc.light_tris[L.first + pick(prng(R), L.count)];
+    const float u = lds1(R), v = lds1(R);
+    const v3 p = point_in_tri(sc, tri, u, v);
+    const v3 lr = sub(p, hit);
+    const v3 dir = nrm(lr);
+    const float co = dot(dir, n);
+    if (co <= 0.0f) return s;
+    const v3 ln = interp_normal_at(sc, tri, p);
+    const float cl = dot(neg(dir), ln);
+    if (cl <= 0.0f) return s;
+    const DevObject &lo = sc.objs[L.obj];
+    const float sa = (float)L.count * sc.shade[tri].area;
+    const float r2 = lr.x * lr.x + lr.y * lr.y + lr.z * lr.z;
+    s.ok = true;
+    s.dir = dir;
+    s.pdf = r2 / (cl * sa);
+    s.Le = scl(ld3(lo.color), lo.emission);
+    s.co = co;
+    s.cl = cl;
+    s.r2 = r2;
+    s.sa = sa;
+    s.lr = lr;
+    s.o = surface_origin(hit, lr, n);
+    return s;
+}
+
+// traceAreaLight's value (rayrenderer.cpp:1916-1974): Lambert NEE incl. 1/pi and MIS
+__device__ __forceinline__ v3 area_value(const AreaSample &s) {
+    const float pl = s.r2 / (s.cl * s.sa);
+    const float pb = s.co / PI_F;
+    const float pl2 = pl * pl, pb2 = pb * pb;
+    const float w = pl2 / (pl2 + pb2);
+    return scl(scl(s.Le, s.co * s.cl * s.sa / (PI_F * s.r2)), w);
+}
+
+struct EnvSample {
+    bool ok;
+    v3 dir, Le, o;
+    float pdf, co;
+};
+
+// sampleEnvmapForNEE (rayrenderer.cpp:2038-2065) up to the shadow test
+__device__ EnvSample sample_env_nee(const PTScene &sc, Rng &R, v3 hit, v3 n) {
+    EnvSample s;
+    s.ok = false;
+    if (!sc.has_env || sc.env_total <= 0.0f) return s;
+    const float u0 = prng(R), u1 = prng(R);
+    v3 dir;
+    float pdf;
+    sample_env_dir(sc, u0, u1, dir, pdf);
+    if (pdf <= 0.0f) return s;
+    const float co = dot(dir, n);
+    if (co <= 0.0f) return s;
+    s.ok = true;
+    s.dir = dir;
+    s.pdf = pdf;
+    s.Le = sample_environment(sc, dir);
+    s.co = co;
+    s.o = surface_origin(hit, dir, n);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// path state access
+// ---------------------------------------------------------------------------
+struct Path {
+    v3 o, d, beta, tp, L;
+    float bsdf_pdf;
+    int passes, medium;
+    bool alive;
+};
+
+__device__ __forceinline__ int crossed_medium(const PTScene &sc, const DevObject &m, bool entering, int cur) {
+    if (entering) return m.interior_medium >= 0 ? m.interior_medium : cur;
+    return sc.global_medium;
+}
+
+// TransparencyShader as one path step (bsdf.cpp:513-546)
+__device__ void step_transparency(const PTScene &sc, Path &p, uint32_t prim, v3 hit, v3 n, float beta_scale,
+                                  bool fresh_param) {
+    const DevObject &m = obj_of(sc, prim);
+    p.beta = scl(p.beta, beta_scale);
+    p.tp = scl(p.tp, m.transparency);
+    p.medium = crossed_medium(sc, m, dot(p.d, n) < 0.0f, p.medium);
+    const v3 gN = geom_normal(sc, prim, n);
+    p.o = surface_origin(hit, p.d, gN);
+    if (fresh_param) p.bsdf_pdf = 0.0f;
+}
+
+// MixShader (bsdf.cpp:579-610) with one lobe drawn in proportion to its weight
+__device__ void step_mix(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n) {
+    const DevObject &m = obj_of(sc, prim);
+    const float wd = 1.0f - m.glossy - m.refraction;
+    const float w0 = wd > 0.00001f ? wd : 0.0f;
+    const float w1 = m.glossy > 0.00001f ? m.glossy : 0.0f;
+    const float w2 = m.refraction > 0.00001f ? m.refraction : 0.0f;
+    const float W = w0 + w1 + w2;
+    if (!(W > 0.0f)) {
+        p.alive = false;
+        return;
+    }
+    const float u = prng(R) * W;
+    int lobe = (u < w0) ? 0 : (u < w0 + w1 ? 1 : 2);
+    if (lobe == 2 && w2 == 0.0f) lobe = w1 > 0.0f ? 1 : 0;
+    if (lobe == 1 && w1 == 0.0f) lobe = 0;
+    const float wl = lobe == 0 ? w0 : (lobe == 1 ? w1 : w2);
+    p.beta = scl(p.beta, W);
+    p.tp = scl(p.tp, wl);
+    if (lobe == 0) {
+        // DiffuseShader (bsdf.cpp:54-108)
+        const v3 alb = ld3(m.color);
+        const float r1 = lds1(R), r2 = lds1(R);
+        const v3 dir = cosine_dir(n, r1, r2);
+        const float pdf = fmaxf(0.0f, dot(dir, n)) * (float)(1.0 / M_PI);
+        const v3 ba = mul(p.beta, alb);
+        const AreaSample as = sample_area(sc, R, hit, n);
+        if (as.ok) push_shadow(S, as.o, as.lr, 0.99999f, RG_PRED_AREA, mul(ba, area_value(as)));
+        const EnvSample es = sample_env_nee(sc, R, hit, n);
+        if (es.ok) {
+            float w = 1.0f;
+            if (pdf > 0.0f) {
+                const float e2 = es.pdf * es.pdf, b2 = pdf * pdf;
+                w = e2 / (e2 + b2);
+            }
+            push_shadow(S, es.o, es.dir, 1e30f, RG_PRED_ENV, mul(ba, scl(es.Le, es.co * w / (PI_F * es.pdf))));
+        }
+        p.beta = ba;
+        p.tp = mul(p.tp, alb);
+        p.bsdf_pdf = pdf;
+        p.o = surface_origin(hit, dir, geom_normal(sc, prim, n));
+        p.d = dir;
+        return;
+    }
+    if (lobe == 1) {
+        // GlossyShader (bsdf.cpp:173-374)
+        const v3 alb = ld3(m.color);
+        if (m.roughness < 1e-3f) {
+            const v3 F0 = (m.metallic >= 1.0f) ? alb
+                                               : add(scl(alb, m.metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m.metallic));
+            const v3 gN = geom_normal(sc, prim, n);
+            v3 r = reflect(p.d, n);
+            if (dot(r, gN) <= 0.0f) r = reflect(p.d, gN);
+            const float ci = fmaxf(0.0f, -dot(p.d, n));
+            const v3 F = schlick_f(F0, ci);
+            p.beta = mul(p.beta, F);
+            p.tp = mul(p.tp, F);
+            p.bsdf_pdf = 0.0f;
+            p.o = surface_origin(hit, r, gN);
+            p.d = r;
+            return;
+        }
+        Ggx G;
+        if (!ggx_setup(m, n, p.d, G)) {
+            p.alive = false;
+            return;
+        }
+        {
+            const AreaSample as = sample_area(sc, R, hit, n);
+            v3 fc;
+            float pb;
+            if (as.ok && eval_glossy(G, as.dir, fc, pb)) {
+                const float pl2 = as.pdf * as.pdf, pb2 = pb * pb;
+                push_shadow(S, as.o, as.lr, 0.99999f, RG_PRED_AREA,
+                            mul(p.beta, scl(mul(as.Le, fc), (pl2 / (pl2 + pb2)) / as.pdf)));
+            }
+        }
+        {
+            const EnvSample es = sample_env_nee(sc, R, hit, n);
+            v3 fc;
+            float pb;
+            if (es.ok && eval_glossy(G, es.dir, fc, pb)) {
+                const float pe2 = es.pdf * es.pdf, pb2 = pb * pb;
+                push_shadow(S, es.o, es.dir, 1e30f, RG_PRED_ENV,
+                            mul(p.beta, scl(mul(es.Le, fc), (pe2 / (pe2 + pb2)) / es.pdf)));
+            }
+        }
+        const float u1 = prng(R), u2 = prng(R);
+        const v3 H = sample_vndf(G.V, G.ax, G.ay, u1, u2);
+        const v3 Ll = sub(scl(H, 2.0f * dot(G.V, H)), G.V);
+        if (Ll.z <= 0.0f) {
+            p.alive = false;
+            return;
+        }
+        const v3 L = add(add(scl(G.t, Ll.x), scl(G.b, Ll.y)), scl(n, Ll.z));
+        const v3 gN = geom_normal(sc, prim, n);
+        if (dot(L, gN) <= 0.0f) {
+            p.alive = false;
+            return;
+        }
+        const float vdh = fmaxf(0.0f, dot(G.V, H));
+        const v3 F = schlick_f(G.F0, vdh);
+        const v3 wgt = scl(F, smith_g1(Ll, G.ax, G.ay));
+        p.beta = mul(p.beta, wgt);
+        p.tp = mul(p.tp, wgt);
+        p.bsdf_pdf = G.g1v * ggx_d(H, G.ax, G.ay) / (4.0f * G.V.z);
+        p.o = surface_origin(hit, L, gN);
+        p.d = L;
+        return;
+    }
+    // RefractionShader (bsdf.cpp:376-483), no chromatic dispersion
+    const float ct = fabsf(dot(p.d, n));
+    const float ior = m.refraction_ratio;
+    const float fr = fresnel_schlick(ct, ior);
+    const bool refl = prng(R) < fr;
+    v3 dir = refl ? reflect(p.d, n) : refract_dir(p.d, n, ior);
+    if (m.roughness > 0.0f) {
+        const float u1 = prng(R), u2 = prng(R);
+        dir = nrm(add(dir, scl(hemisphere_dir(n, u1, u2), m.roughness)));
+    }
+    const v3 gN = geom_normal(sc, prim, n);
+    if (refl && dot(dir, gN) <= 0.0f) dir = reflect(p.d, gN);
+    const v3 col = ld3(m.color);
+    p.beta = mul(p.beta, col);
+    p.tp = mul(p.tp, col);
+    p.bsdf_pdf = 0.0f;
+    const float di = dot(p.d, n), dout = dot(dir, n);
+    if (di * dout > 0.0f) p.medium = crossed_medium(sc, m, di < 0.0f, p.medium);
+    p.o = surface_origin(hit, dir, gN);
+    p.d = dir;
+}
+
+// RayBSDFShaderProvider::shade (rayrenderer.cpp:2776-2896) as one path event
+__device__ void step_surface(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n) {
+    const DevObject &m = obj_of(sc, prim);
+    const bool eye = p.passes < 0;
+    if (m.emission > 0.0f) {
+        const v3 E = scl(ld3(m.color), m.emission);
+        if (!eye && p.bsdf_pdf > 0.0f) {
+            const float cl = -dot(p.d, n);
+            if (cl > 0.0f) {
+                const v3 dd = sub(hit, p.o);
+                const float r2 = dd.x * dd.x + dd.y * dd.y + dd.z * dd.z;
+                const float pl = r2 / (cl * sc.shade[prim].sampled_area);
+                const float pb = p.bsdf_pdf;
+                const float pb2 = pb * pb, pl2 = pl * pl;
+                p.L = add(p.L, mul(p.beta, scl(E, pb2 / (pb2 + pl2))));
+            }
+        } else {
+            p.L = add(p.L, mul(p.beta, E));
+        }
+        p.alive = false;
+        return;
+    }
+    if (dot(p.d, n) > 0.0f) {
+        if (m.transparency > 0.001f) {
+            if (!eye && p.passes + 1 <= sc.max_depth) {
+                p.passes = p.passes + 1;
+                step_transparency(sc, p, prim, hit, n, m.transparency, true);
+                return;
+            }
+            p.alive = false;
+            return;
+        } else if (m.refraction < 0.001f && m.glossy > 0.001f) {
+            p.alive = false;
+            return;
+        }
+    }
+    if (!eye) {
+        if (p.passes + 1 >= sc.max_depth) {
+            if (1.0f - m.glossy - m.refraction > 0.00001f) {
+                const AreaSample as = sample_area(sc, R, hit, n);
+                if (as.ok) push_shadow(S, as.o, as.lr, 0.99999f, RG_PRED_AREA,
+                                       mul(mul(p.beta, ld3(m.color)), area_value(as)));
+            }
+            p.alive = false;
+            return;
+        }
+        p.passes = p.passes + 1;
+        if (p.passes >= MIN_RR_DEPTH) {
+            float q = maxc(p.tp);
+            q = fminf(RR_MAX_PROB, fmaxf(RR_MIN_PROB, q));
+            if (prng(R) >= q) {
+                p.alive = false;
+                return;
+            }
+            p.beta = scl(p.beta, 1.0f / q);
+        }
+        if (m.transparency > 0.001f) {
+            step_transparency(sc, p, prim, hit, n, m.transparency, false);
+            return;
+        }
+        step_mix(sc, R, p, S, prim, hit, n);
+        return;
+    }
+    p.passes = 0;
+    if (m.transparency > 0.01f) {
+        if (prng(R) < m.transparency) {
+            step_transparency(sc, p, prim, hit, n, 1.0f, true);
+            return;
+        }
+    }
+    step_mix(sc, R, p, S, prim, hit, n);
+}
+
+// RayRenderer::tracePath (rayrenderer.cpp:1596-1864) as one path event
+__device__ void step(const PTScene &sc, Rng &R, Path &p, Shadows &S, const rg_hit &h) {
+    const bool eye = p.passes < 0;
+    const bool hit = h.prim != 0xFFFFFFFFu;
+    const v3 pos = add(p.o, scl(p.d, h.t));
+    const DevMedium *med = (p.medium >= 0 && sc.media[p.medium].active) ? &sc.media[p.medium] : nullptr;
+    if (!med) {
+        if (hit && (!eye || obj_of(sc, h.prim).visible)) {
+            step_surface(sc, R, p, S, h.prim, pos, interp_normal(sc, h.prim, h.u, h.v));
+            return;
+        }
+        v3 env = sample_environment(sc, p.d);
+        if (eye) {
+            if (is_zero(env)) env = mk(1.0f * 0.2f, 0.95f * 0.2f, 0.9f * 0.2f);
+        } else {
+            env = scl(env, env_miss_weight(sc, p.d, p.bsdf_pdf));
+        }
+        p.L = add(p.L, mul(p.beta, env));
+        p.alive = false;
+        return;
+    }
+    const float maxT = hit ? h.t : RAY_MAX_DISTANCE;
+    const float tf = free_flight(*med, prng(R));
+    const bool scattered = tf < maxT;
+    p.L = add(p.L, mul(p.beta, emission_along(*med, scattered ? tf : maxT)));
+    if (scattered) {
+        v3 sw = mk(0, 0, 0);
+        {
+            const float ph = med->hero * expf(-med->hero * tf);
+            const v3 Tr = transmittance(*med, tf);
+            if (ph > 0.0f)
+                sw = mk(med->sigma_s[0] * Tr.x / ph, med->sigma_s[1] * Tr.y / ph, med->sigma_s[2] * Tr.z / ph);
+        }
+        if (is_zero(sw)) {
+            p.alive = false;
+            return;
+        }
+        const v3 spos = add(p.o, scl(p.d, tf));
+        const int passes = eye ? 1 : p.passes + 1;
+        if (passes > sc.max_depth) {
+            p.alive = false;
+            return;
+        }
+        const v3 bsw = mul(p.beta, sw);
+        const v3 proxy = neg(p.d);
+        {
+            const AreaSample as = sample_area(sc, R, spos, proxy);
+            if (as.ok && as.pdf > 0.0f)
+                push_shadow(S, as.o, as.lr, 0.99999f, RG_PRED_AREA,
+                            mul(bsw, scl(as.Le, phase_pdf(*med, neg(p.d), as.dir) * (1.0f / as.pdf))));
+        }
+        {
+            const EnvSample es = sample_env_nee(sc, R, spos, proxy);
+            if (es.ok && es.pdf > 0.0f)
+                push_shadow(S, es.o, es.dir, 1e30f, RG_PRED_ENV,
+                            mul(bsw, scl(es.Le, phase_pdf(*med, neg(p.d), es.dir) * (1.0f / es.pdf))));
+        }
+        const float u1 = prng(R), u2 = prng(R);
+        const v3 nd = sample_phase(*med, p.d, u1, u2);
+        float rrw = 1.0f;
+        if (passes >= MIN_RR_DEPTH) {
+            float q = maxc(mul(p.tp, sw));
+            q = fminf(RR_MAX_PROB, fmaxf(RR_MIN_PROB, q));
+            if (prng(R) >= q) {
+                p.alive = false;
+                return;
+            }
+            rrw = 1.0f / q;
+        }
+        p.beta = scl(bsw, rrw);
+        p.passes = passes;
+        p.bsdf_pdf = 0.0f;
+        p.o = add(spos, scl(nrm(nd), SURFACE_THICKNESS));
+        p.d = nd;
+        return;
+    }
+    v3 sw = mk(1, 1, 1);
+    {
+        const float surv = expf(-med->hero * maxT);
+        if (surv > 0.0f) {
+            const v3 Tr = transmittance(*med, maxT);
+            sw = mk(Tr.x / surv, Tr.y / surv, Tr.z / surv);
+        }
+    }
+    p.beta = mul(p.beta, sw);
+    if (hit) {
+        v3 n = interp_normal(sc, h.prim, h.u, h.v);
+        const v3 g = ld3(sc.shade[h.prim].gn);
+        const v3 gN = dot(g, p.d) <= 0.0f ? g : neg(g);
+        if (dot(p.d, n) > 0.0f && dot(p.d, gN) < 0.0f) n = gN;
+        step_surface(sc, R, p, S, h.prim, pos, n);
+        return;
+    }
+    v3 env = sample_environment(sc, p.d);
+    if (!eye) env = scl(env, env_miss_weight(sc, p.d, p.bsdf_pdf));
+    p.L = add(p.L, mul(p.beta, env));
+    p.alive = false;
+}
+
+
+// Primary ray through pixel (x, y) (rayrenderer.cpp:860-934): antialias
+// jitter, then thin-lens depth of field with a disc or polygonal aperture.
+__device__ __forceinline__ void camera_ray(const PTScene &sc, Rng &R, uint32_t x, uint32_t y, v3 &o, v3 &d) {
+    const float dx = ((float)x + 0.5f - sc.half_w) * sc.view_scale;
+    const float dy = -((float)y + 0.5f - sc.half_h) * sc.view_scale;
+    const float jx = lds1(R), jy = lds1(R);
+    const float jxo = sc.antialias ? (jx - 0.5f) : 0.0f;
+    const float jyo = sc.antialias ? (jy - 0.5f) : 0.0f;
+    if (sc.dof >= 0.001f && sc.aperture > 0.0f) {
+        const float px = dx + jxo * sc.view_scale;
+        const float py = dy - jyo * sc.view_scale;
+        const v3 focal = mk(px * sc.dof, py * sc.dof, -sc.dof);
+        const float du = lds1(R), dv = lds1(R);
+        float ox, oy;
+        if (sc.blades >= 3) {
+            const int N = sc.blades;
+            const float u0 = du * (float)N;
+            const int wedge = (int)fminf((float)(N - 1), floorf(u0));
+            float s2 = u0 - (float)wedge, t2 = dv;
+            if (s2 + t2 > 1.0f) {
+                s2 = 1.0f - s2;
+                t2 = 1.0f - t2;
+            }
+            const float stp = 2.0f * PI_F / (float)N;
+            const float a0 = sc.ap_rot + stp * (float)wedge;
+            const float a1 = a0 + stp;
+            ox = (cosf(a0) * s2 + cosf(a1) * t2) * sc.aperture;
+            oy = (sinf(a0) * s2 + sinf(a1) * t2) * sc.aperture;
+        } else {
+            const float a = 2.0f * du - 1.0f, b = 2.0f * dv - 1.0f;
+            float r, phi;
+            if (a == 0.0f && b == 0.0f) {
+                r = 0.0f;
+                phi = 0.0f;
+            } else if (a * a > b * b) {
+                r = a;
+                phi = (float)(M_PI / 4.0) * (b / a);
+            } else {
+                r = b;
+                phi = (float)(M_PI / 2.0) - (float)(M_PI / 4.0) * (a / b);
+            }
+            ox = r * cosf(phi) * sc.aperture;
+            oy = r * sinf(phi) * sc.aperture;
+        }
+        o = mk(ox, oy, 0.0f);
+        d = nrm(sub(focal, o));
+    } else {
+        o = mk(0, 0, 0);
+        d = nrm(mk(dx + jxo * sc.view_scale, dy - jyo * sc.view_scale, -1.0f));
+    }
+}
+
+constexpr int kPtBlock = 256;
+
+// Position in a queue for every active lane with pred set, in lane order
+// (0xFFFFFFFF for the others): one atomic per wave.
+__device__ __forceinline__ uint32_t wave_slot(bool pred, uint32_t *counter) {
+    const uint64_t m = __ballot(pred ? 1 : 0);
+    if (m == 0) return 0xFFFFFFFFu;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader);
+    return pred ? base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : 0xFFFFFFFFu;
+}
+
+// Position in a queue for every thread of the block with pred set, in
+// thread order (0xFFFFFFFF for the others): one atomic per block.  Every
+// thread of the block must call it; `lds` holds kPtBlock / 64 + 1 words.
+__device__ __forceinline__ uint32_t block_slot(bool pred, uint32_t *counter, uint32_t *lds) {
+    constexpr uint32_t kWaves = kPtBlock / 64;
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(pred ? 1 : 0);
+    if (lane == 0) lds[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < kWaves; w++) {
+            const uint32_t c = lds[w];
+            lds[w] = tot;
+            tot += c;
+        }
+        lds[kWaves] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t pos = lds[kWaves] + lds[wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads();  // the words are reused by the next call
+    return pred ? pos : 0xFFFFFFFFu;
+}
+
+// Appends `value` for every lane with pred set: one atomic per wave.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t *queue, uint32_t *counter) {
+    const uint32_t pos = wave_slot(pred, counter);
+    if (pred) queue[pos] = value;
+}
+
+}  // namespace rg

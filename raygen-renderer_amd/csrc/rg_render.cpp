@@ -156,7 +156,8 @@ int reupload_tris(Scene *s, void *stream) {
 }
 
 void free_buffers(PTBuffers &b) {
-    for (void **p : {&b.ray, &b.hit, &b.beta, &b.tp, &b.L, &b.meta, &b.sray, &b.scontrib, &b.nshadow, &b.q0,
+    for (void **p : {&b.ray, &b.hit, &b.beta, &b.tp, &b.L, &b.meta, &b.sray, &b.ssrc, &b.socc, &b.scontrib,
+                     &b.nshadow, &b.q0,
                      &b.q1, &b.count, &b.sample_out, &b.sum, &b.sumsq, &b.hdr, &b.tile_mask, &b.tile_nspp,
                      &b.tile_noise, &b.n_active, &b.aov, &b.dn_a, &b.dn_b, &b.stats}) {
         dev_free(*p);
@@ -176,7 +177,8 @@ int ensure_buffers(PTBuffers &b, uint64_t cap, uint64_t npix, uint64_t ntiles) {
         uint64_t bytes;
     } allocs[] = {
         {&b.ray, cap * 32}, {&b.hit, cap * 16}, {&b.beta, cap * 16}, {&b.tp, cap * 16}, {&b.L, cap * 16},
-        {&b.meta, cap * 16}, {&b.sray, cap * 64}, {&b.scontrib, cap * 32}, {&b.nshadow, cap * 4},
+        {&b.meta, cap * 16}, {&b.sray, cap * 64}, {&b.ssrc, cap * 8}, {&b.socc, cap * 2},
+        {&b.scontrib, cap * 32}, {&b.nshadow, cap * 4},
         {&b.q0, cap * 4}, {&b.q1, cap * 4}, {&b.count, 64}, {&b.sample_out, cap * 16}, {&b.sum, npix * 12},
         {&b.sumsq, npix * 12}, {&b.hdr, npix * 16}, {&b.tile_mask, ntiles}, {&b.tile_nspp, ntiles * 4},
         {&b.tile_noise, ntiles * 4}, {&b.n_active, 64}, {&b.aov, npix * 36}, {&b.dn_a, npix * 16},
@@ -194,6 +196,66 @@ int ensure_buffers(PTBuffers &b, uint64_t cap, uint64_t npix, uint64_t ntiles) {
 }
 
 }  // namespace
+
+PTScene make_ptscene(const Scene &scn, const rg_settings &st) {
+    const Scene *s = &scn;
+    const uint32_t W = st.width, H = st.height;
+    PTScene sc;
+    std::memset(&sc, 0, sizeof sc);
+    sc.v9 = (const float *)s->d_v9;
+    sc.shade = (const DevTriShade *)s->d_shade;
+    sc.objs = (const DevObject *)s->d_objs;
+    sc.lights = (const DevLight *)s->d_lights;
+    sc.light_tris = (const uint32_t *)s->d_light_tris;
+    sc.nlights = s->nlights;
+    sc.media = (const DevMedium *)s->d_media;
+    sc.global_medium = s->global_medium;
+    sc.env_rgb = (const float *)s->d_env_rgb;
+    sc.env_marg = (const float *)s->d_env_marg;
+    sc.env_cond = (const float *)s->d_env_cond;
+    sc.env_w = s->env_w;
+    sc.env_h = s->env_h;
+    sc.env_total = s->env_total;
+    sc.env_intensity = s->env_intensity;
+    {
+        const float yaw = s->env_rotation_deg * (float)(M_PI / 180.0);
+        sc.env_cos = cosf(yaw);
+        sc.env_sin = sinf(yaw);
+    }
+    sc.has_env = (s->env_w > 0 && s->env_h > 0) ? 1 : 0;
+    const rg_camera &cam = s->camera;
+    sc.half_w = (float)W * 0.5f;
+    sc.half_h = (float)H * 0.5f;
+    {
+        const float fov = fminf(fmaxf(cam.fov_deg, 1.0f), 179.0f);
+        const float th = tanf(fov * (float)(M_PI / 180.0) * 0.5f);
+        sc.view_scale = (2.0f * th) / (float)H;
+    }
+    sc.dof = cam.depth_of_field;
+    sc.aperture = cam.aperture > 0.0f ? 1.0f / cam.aperture : 0.0f;
+    sc.blades = cam.aperture_blades;
+    sc.ap_rot = cam.aperture_rotation_deg * (float)(M_PI / 180.0);
+    sc.max_depth = (int32_t)st.max_depth;
+    sc.antialias = st.antialias ? 1 : 0;
+    sc.width = W;
+    sc.height = H;
+    sc.seed = st.seed;
+    sc.job = st.job;
+    return sc;
+}
+
+int prepare_scene(Scene *s, void *stream) {
+    if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
+    if (!s->has_shading) return set_error(RG_E_INVALID, "rg_scene_set_shading was not called");
+    for (const rg_object &o : s->objects)
+        if (o.interior_medium >= (int32_t)s->media.size())
+            return set_error(RG_E_INVALID, "interior medium index out of range");
+    if (s->tris_dirty)
+        if (int rc = reupload_tris(s, stream)) return rc;
+    if (s->pt_dirty)
+        if (int rc = upload_pt_scene(s, stream)) return rc;
+    return RG_OK;
+}
 
 void release_pt(Ctx *c) { free_buffers(c->pt); }
 void release_pt_scene(Scene *s) { free_pt_scene(s); }
@@ -306,18 +368,10 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     if (st.width == 0 || st.height == 0 || st.width > 16384 || st.height > 16384)
         return set_error(RG_E_INVALID, "resolution must be 1..16384 per axis");
     if (st.max_depth < 1 || st.max_depth > 64) return set_error(RG_E_INVALID, "max_depth must be 1..64");
-    if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
-    if (!s->has_shading) return set_error(RG_E_INVALID, "rg_scene_set_shading was not called");
-    for (const rg_object &o : s->objects)
-        if (o.interior_medium >= (int32_t)s->media.size())
-            return set_error(RG_E_INVALID, "interior medium index out of range");
     if (c->cancel.load()) return set_error(RG_E_CANCELLED, "cancelled");
     if (int rc = dev_set_device(c->device)) return rc;
     void *stream = c->stream;
-    if (s->tris_dirty)
-        if (int rc = reupload_tris(s, stream)) return rc;
-    if (s->pt_dirty)
-        if (int rc = upload_pt_scene(s, stream)) return rc;
+    if (int rc = prepare_scene(s, stream)) return rc;
 
     const uint32_t W = st.width, H = st.height, npix = W * H;
     const uint32_t tx = (W + 31) / 32, ty = (H + 31) / 32, ntiles = tx * ty;
@@ -330,47 +384,8 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     if (int rc = ensure_buffers(c->pt, (uint64_t)npix * S, npix, ntiles)) return rc;
     PTBuffers &B = c->pt;
 
-    PTScene sc;
-    std::memset(&sc, 0, sizeof sc);
-    sc.v9 = (const float *)s->d_v9;
-    sc.shade = (const DevTriShade *)s->d_shade;
-    sc.objs = (const DevObject *)s->d_objs;
-    sc.lights = (const DevLight *)s->d_lights;
-    sc.light_tris = (const uint32_t *)s->d_light_tris;
-    sc.nlights = s->nlights;
-    sc.media = (const DevMedium *)s->d_media;
-    sc.global_medium = s->global_medium;
-    sc.env_rgb = (const float *)s->d_env_rgb;
-    sc.env_marg = (const float *)s->d_env_marg;
-    sc.env_cond = (const float *)s->d_env_cond;
-    sc.env_w = s->env_w;
-    sc.env_h = s->env_h;
-    sc.env_total = s->env_total;
-    sc.env_intensity = s->env_intensity;
-    {
-        const float yaw = s->env_rotation_deg * (float)(M_PI / 180.0);
-        sc.env_cos = cosf(yaw);
-        sc.env_sin = sinf(yaw);
-    }
-    sc.has_env = (s->env_w > 0 && s->env_h > 0) ? 1 : 0;
+    const PTScene sc = make_ptscene(*s, st);
     const rg_camera &cam = s->camera;
-    sc.half_w = (float)W * 0.5f;
-    sc.half_h = (float)H * 0.5f;
-    {
-        const float fov = fminf(fmaxf(cam.fov_deg, 1.0f), 179.0f);
-        const float th = tanf(fov * (float)(M_PI / 180.0) * 0.5f);
-        sc.view_scale = (2.0f * th) / (float)H;
-    }
-    sc.dof = cam.depth_of_field;
-    sc.aperture = cam.aperture > 0.0f ? 1.0f / cam.aperture : 0.0f;
-    sc.blades = cam.aperture_blades;
-    sc.ap_rot = cam.aperture_rotation_deg * (float)(M_PI / 180.0);
-    sc.max_depth = (int32_t)st.max_depth;
-    sc.antialias = st.antialias ? 1 : 0;
-    sc.width = W;
-    sc.height = H;
-    sc.seed = st.seed;
-    sc.job = st.job;
 
     PTState ps;
     ps.ray = (rg_ray *)B.ray;
@@ -380,6 +395,8 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     ps.L = (float4 *)B.L;
     ps.meta = (int4 *)B.meta;
     ps.sray = (rg_ray *)B.sray;
+    ps.ssrc = (uint32_t *)B.ssrc;
+    ps.socc = (uint8_t *)B.socc;
     ps.scontrib = (float4 *)B.scontrib;
     ps.nshadow = (uint32_t *)B.nshadow;
     ps.q[0] = (uint32_t *)B.q0;

@@ -25,6 +25,10 @@ static int hip_check(hipError_t e, const char *what) {
 
 int dev_set_device(int device) { return hip_check(hipSetDevice(device), "hipSetDevice"); }
 
+// Drops a pending error of a call whose failure was already handled, so a
+// later launch check does not report it.
+void dev_clear_error() { (void)hipGetLastError(); }
+
 int dev_malloc(void **p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) return RG_OK;

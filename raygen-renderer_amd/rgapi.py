@@ -83,6 +83,14 @@ def lib():
             "rg_scene_get_camera": (i32, [vp, vp]),
             "rg_scene_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
             "rg_scene_export_media": (i32, [vp, vp, vp, vp, vp, vp]),
+            "rg_render_batch": (i32, [vp, vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp, vp, vp, vp]),
+            "rg_autotune": (i32, [vp, vp, ctypes.c_uint32, vp, ctypes.c_uint32, vp]),
+            "rg_server_create": (i32, [vp, vp, ctypes.c_uint32, u64, vp, pp]),
+            "rg_server_submit": (i32, [vp, vp, vp, vp, vp]),
+            "rg_server_wait": (i32, [vp, u64, ctypes.c_double, vp]),
+            "rg_server_stats": (i32, [vp, vp]),
+            "rg_server_get_tune": (i32, [vp, vp]),
+            "rg_server_destroy": (i32, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -98,7 +106,7 @@ def check(rc):
 
 
 def _ptr(a):
-    return a.ctypes.data_as(ctypes.c_void_p)
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
 class Context:
@@ -189,18 +197,24 @@ class Scene:
         check(lib().rg_trace_any_host(self.ctx.h, self.h, _ptr(rays), _ptr(occ), len(rays), pred))
         return occ
 
-    def render(self, settings):
-        """rg_render of this scene (shading set and BVH built) with host outputs."""
+    def render(self, settings, hdr_only=False):
+        """rg_render of this scene (shading set and BVH built) with host
+        outputs; hdr_only skips the sums and tile sample counts."""
         W, H = settings.width, settings.height
         hdr = np.zeros((H, W, 4), np.float32)
-        sm = np.zeros((H, W, 3), np.float32)
-        sq = np.zeros((H, W, 3), np.float32)
-        nt = np.zeros(((W + 31) // 32) * ((H + 31) // 32), np.uint32)
+        if hdr_only:
+            sm = sq = nt = None
+        else:
+            sm = np.zeros((H, W, 3), np.float32)
+            sq = np.zeros((H, W, 3), np.float32)
+            nt = np.zeros(((W + 31) // 32) * ((H + 31) // 32), np.uint32)
         check(lib().rg_render(self.ctx.h, self.h, ctypes.byref(settings), _ptr(hdr), _ptr(sm), _ptr(sq), _ptr(nt)))
         stats = np.zeros(3, np.uint64)
         check(lib().rg_render_stats(self.ctx.h, _ptr(stats)))
-        return {"hdr": hdr, "sum": sm, "sumsq": sq, "nspp": nt,
-                "rays": int(stats[0]), "shadow_rays": int(stats[1]), "samples": int(stats[2])}
+        out = {"hdr": hdr, "rays": int(stats[0]), "shadow_rays": int(stats[1]), "samples": int(stats[2])}
+        if not hdr_only:
+            out.update({"sum": sm, "sumsq": sq, "nspp": nt})
+        return out
 
     def close(self):
         if self.h:
@@ -334,3 +348,132 @@ def load_json(ctx, path):
     if env.w and env.h:
         d["env"] = {"rgb": env_rgb, "intensity": env.intensity, "rotation_deg": env.rotation_deg}
     return sc, d
+
+
+# ---------------------------------------------------------------------------
+# many jobs: rg_render_batch, rg_autotune, rg_server_*
+# ---------------------------------------------------------------------------
+TIMEOUT = 1
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("settings", Settings)]
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [("pool_paths", ctypes.c_uint32), ("stack", ctypes.c_uint32), ("tile", ctypes.c_uint32),
+                ("blocks_per_cu", ctypes.c_uint32), ("iters_per_graph", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_ if f != "pad"}
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("pad", ctypes.c_uint32), ("samples", ctypes.c_uint64),
+                ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+def make_job(scene, **kw):
+    j = Job()
+    j.scene = scene
+    j.settings = default_settings(**kw)
+    return j
+
+
+def make_tune(**kw):
+    t = Tune()
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
+def _scene_array(scenes):
+    arr = (ctypes.c_void_p * len(scenes))(*[s.h for s in scenes])
+    return arr
+
+
+def render_batch(ctx, scenes, jobs, tune=None, want_sum=True):
+    """rg_render_batch: returns (list of {"hdr", "sum"}, stats[4])."""
+    jarr = (Job * len(jobs))(*jobs)
+    outs = []
+    hp = (ctypes.c_void_p * len(jobs))()
+    sp = (ctypes.c_void_p * len(jobs))()
+    for i, j in enumerate(jobs):
+        W, H = j.settings.width, j.settings.height
+        o = {"hdr": np.zeros((H, W, 4), np.float32)}
+        hp[i] = o["hdr"].ctypes.data
+        if want_sum:
+            o["sum"] = np.zeros((H, W, 3), np.float32)
+            sp[i] = o["sum"].ctypes.data
+        outs.append(o)
+    stats = np.zeros(4, np.uint64)
+    sarr = _scene_array(scenes)
+    check(lib().rg_render_batch(ctx.h, sarr, len(scenes), jarr, len(jobs), ctypes.byref(tune) if tune else None,
+                                hp, sp if want_sum else None, _ptr(stats)))
+    return outs, stats
+
+
+def autotune(ctx, scenes, probe=None):
+    t = Tune()
+    sarr = _scene_array(scenes)
+    parr = (Job * len(probe))(*probe) if probe else None
+    check(lib().rg_autotune(ctx.h, sarr, len(scenes), parr, len(probe) if probe else 0, ctypes.byref(t)))
+    return t
+
+
+class Server:
+    """rg_server_*: scenes stay resident; submit() queues a request and
+    returns its ticket, wait() returns (info, outputs)."""
+
+    def __init__(self, ctx, scenes, hbm_budget=0, tune=None):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        h = ctypes.c_void_p()
+        check(lib().rg_server_create(ctx.h, _scene_array(self.scenes), len(self.scenes), hbm_budget,
+                                     ctypes.byref(tune) if tune else None, ctypes.byref(h)))
+        self.h = h
+        self.outs = {}
+
+    def submit(self, job, want_hdr=True, want_sum=False):
+        W, H = job.settings.width, job.settings.height
+        o = {}
+        if want_hdr:
+            o["hdr"] = np.zeros((H, W, 4), np.float32)
+        if want_sum:
+            o["sum"] = np.zeros((H, W, 3), np.float32)
+        t = ctypes.c_uint64()
+        check(lib().rg_server_submit(self.h, ctypes.byref(job), _ptr(o["hdr"]) if want_hdr else None,
+                                     _ptr(o["sum"]) if want_sum else None, ctypes.byref(t)))
+        self.outs[t.value] = o
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        info = RequestInfo()
+        rc = lib().rg_server_wait(self.h, ticket, timeout, ctypes.byref(info))
+        if rc == TIMEOUT:
+            return None, None
+        check(rc)
+        return info, self.outs.pop(ticket)
+
+    def stats(self):
+        out = np.zeros(8, np.uint64)
+        check(lib().rg_server_stats(self.h, _ptr(out)))
+        keys = ["requests", "samples", "rays", "shadow_rays", "iterations", "fb_bytes", "budget", "graphs"]
+        return dict(zip(keys, (int(x) for x in out)))
+
+    def tune(self):
+        t = Tune()
+        check(lib().rg_server_get_tune(self.h, ctypes.byref(t)))
+        return t
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -173,3 +173,35 @@ def bsdf_frame(grid=5, stacks=200, slices=200, seed=0xB5DF0002, env_value=0.3):
     return b.finish(camera={"fov_deg": 45.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0,
                             "view_far": 50.0},
                     media=med, env=constant_env(env_value))
+
+
+def ico_ggx(level=8, amp=0.05, seed=0x1C05EED, env_value=0.3):
+    """The traversal benchmark's displaced icosphere (20 * 4^level triangles,
+    level 8 = 1,310,720) shaded with GGX (glossy 1, roughness 0.3, flat face
+    normals), beside the BSDF frame's medium box (sigma_a 0.1, sigma_s 0.4 in
+    a transparency-1 cube), lit by one area-light quad and a constant
+    environment."""
+    import rgapi
+
+    v = rgapi.gen_icosphere(level, amp, seed).reshape(-1, 3, 3).astype(np.float32)
+    v = (v * np.float32(1.3) + np.array([0.9, 0.0, -5.5], np.float32)).astype(np.float32)
+    fn = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    fn = (fn / np.linalg.norm(fn, axis=1, keepdims=True)).astype(np.float32)
+    b = Builder()
+    ball = b.add_object(material(color=(0.9, 0.75, 0.35), glossy=1.0, roughness=0.3))
+    b.v.append(v.reshape(-1, 9))
+    b.n.append(np.repeat(fn, 3, axis=0).reshape(-1, 9))
+    b.o.append(np.full(len(v), ball, np.uint32))
+    floor = b.add_object(material(color=(0.6, 0.6, 0.6)))
+    b.quad(floor, (-6, -1.5, -1), (6, -1.5, -1), (6, -1.5, -12), (-6, -1.5, -12), normal=(0, 1, 0))
+    light = b.add_object(material(color=(1.0, 1.0, 1.0), emission=8.0))
+    b.quad(light, (-1.0, 3.2, -6.5), (1.0, 3.2, -6.5), (1.0, 3.2, -4.5), (-1.0, 3.2, -4.5), normal=(0, -1, 0))
+    box = b.add_object(material(color=(1.0, 1.0, 1.0), transparency=1.0, interior_medium=0))
+    b.box(box, (-2.4, -1.49, -5.6), (-0.8, 0.1, -4.0))
+    med = np.zeros(1, MEDIUM_DTYPE)
+    med[0]["sigma_a"] = (0.1, 0.1, 0.1)
+    med[0]["sigma_s"] = (0.4, 0.4, 0.4)
+    med[0]["density"] = 1.0
+    return b.finish(camera={"fov_deg": 45.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0,
+                            "view_far": 50.0},
+                    media=med, env=constant_env(env_value))

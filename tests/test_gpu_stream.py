@@ -1,0 +1,148 @@
+"""Batched and served rendering (rg_render_batch, rg_server_*, rg_autotune)
+on the GPU.  Every job's image must equal what the single-job path
+(rg_render) returns for the same scene and settings: the engine draws the
+same random numbers per sample and sums each pixel's samples in the same
+order and grouping, so the comparison is exact."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import rgapi
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return rgapi.Context(0)
+
+
+@pytest.fixture(scope="module")
+def resident(ctx):
+    room = rgapi.RenderScene(ctx, scenes.cube_room())
+    frame = rgapi.RenderScene(ctx, scenes.bsdf_frame(grid=3, stacks=16, slices=24))
+    return [room, frame]
+
+
+# uneven resolutions, sample counts, depths, seeds; one job with an explicit
+# samples-in-flight count so the per-pixel grouping is not trivial
+JOBS = [
+    (0, dict(width=64, height=48, spp=8, max_depth=5)),
+    (1, dict(width=96, height=64, spp=5, max_depth=8)),
+    (0, dict(width=33, height=17, spp=3, max_depth=4, seed=0x1234)),
+    (1, dict(width=128, height=96, spp=16, max_depth=6, samples_in_flight=3)),
+    (0, dict(width=200, height=150, spp=1, max_depth=5, job=7)),
+    (1, dict(width=17, height=200, spp=7, max_depth=4, firefly_clamp=0.0)),
+]
+
+
+def make_jobs():
+    return [rgapi.make_job(s, **kw) for s, kw in JOBS]
+
+
+@pytest.fixture(scope="module")
+def single(resident):
+    """rg_render of every job, one at a time."""
+    outs = []
+    for s, kw in JOBS:
+        outs.append(resident[s].render(rgapi.default_settings(**kw)))
+    return outs
+
+
+def assert_same(got, want, what):
+    assert np.array_equal(got["hdr"], want["hdr"]), (what, np.abs(got["hdr"] - want["hdr"]).max())
+    if "sum" in got:
+        assert np.array_equal(got["sum"], want["sum"]), (what, np.abs(got["sum"] - want["sum"]).max())
+
+
+def test_batch_matches_single_job_path(ctx, resident, single):
+    say("[stream] batch of 6 uneven jobs vs rg_render")
+    outs, stats = rgapi.render_batch(ctx, resident, make_jobs())
+    for i, (o, w) in enumerate(zip(outs, single)):
+        assert_same(o, w, f"job {i}")
+    want_samples = sum(kw["width"] * kw["height"] * kw["spp"] for _, kw in JOBS)
+    assert int(stats[2]) == want_samples
+    assert int(stats[0]) == sum(w["rays"] for w in single)
+    assert int(stats[1]) == sum(w["shadow_rays"] for w in single)
+
+
+@pytest.mark.parametrize("tune", [
+    dict(pool_paths=4096, stack=256, tile=1, blocks_per_cu=4, iters_per_graph=2),
+    dict(pool_paths=1 << 20, tile=16, blocks_per_cu=16, iters_per_graph=8),
+])
+def test_tuning_knobs_do_not_change_images(ctx, resident, single, tune):
+    say(f"[stream] batch with tune {tune}")
+    outs, _ = rgapi.render_batch(ctx, resident, make_jobs(), tune=rgapi.make_tune(**tune))
+    for i, (o, w) in enumerate(zip(outs, single)):
+        assert_same(o, w, f"job {i} {tune}")
+
+
+def test_server_alone_equals_mixed(ctx, resident, single):
+    """A request renders the same image alone and among others that arrive
+    while it is on the GPU, with admission limited by a small budget."""
+    jobs = make_jobs()
+    sums = [kw["width"] * kw["height"] * (kw["spp"] * 12 + 40) for _, kw in JOBS]
+    budget = max(sums) * 2 + (1 << 20)      # room for about two jobs at a time
+    srv = rgapi.Server(ctx, resident, hbm_budget=budget)
+    try:
+        say("[stream] server: request 3 alone")
+        t = srv.submit(jobs[3], want_sum=True)
+        info, alone = srv.wait(t, timeout=60.0)
+        assert info is not None and info.status == 0
+        assert_same(alone, single[3], "alone")
+        say("[stream] server: all six, staggered")
+        tickets = []
+        for i, j in enumerate(jobs):
+            tickets.append(srv.submit(j, want_sum=True))
+            time.sleep(0.002 * i)
+        for i, t in enumerate(tickets):
+            info, out = srv.wait(t, timeout=120.0)
+            assert info is not None, f"request {i} timed out"
+            assert info.status == 0
+            assert info.submit_s <= info.admit_s <= info.done_s
+            assert info.samples == JOBS[i][1]["width"] * JOBS[i][1]["height"] * JOBS[i][1]["spp"]
+            assert_same(out, single[i], f"mixed {i}")
+        st = srv.stats()
+        assert st["requests"] == 7 and st["fb_bytes"] == 0 and st["budget"] == budget
+    finally:
+        srv.close()
+
+
+def test_server_rejects_bad_requests(ctx, resident):
+    srv = rgapi.Server(ctx, resident, hbm_budget=1 << 20)
+    try:
+        with pytest.raises(rgapi.RGError) as e:
+            srv.submit(rgapi.make_job(0, width=1024, height=1024, spp=64))
+        assert e.value.code == -7                      # RG_E_BUDGET
+        with pytest.raises(rgapi.RGError) as e:
+            srv.submit(rgapi.make_job(0, width=16, height=16, spp=4, adaptive=1))
+        assert e.value.code == -6                      # RG_E_UNSUPPORTED
+        with pytest.raises(rgapi.RGError) as e:
+            srv.submit(rgapi.make_job(5, width=16, height=16, spp=4))
+        assert e.value.code == -1                      # RG_E_INVALID
+        t = srv.submit(rgapi.make_job(0, width=16, height=16, spp=4))
+        info, out = srv.wait(t, timeout=60.0)
+        assert info.status == 0 and out["hdr"].shape == (16, 16, 4)
+    finally:
+        srv.close()
+
+
+def test_autotune_returns_a_working_setting(ctx, resident, single):
+    probe = [rgapi.make_job(0, width=128, height=96, spp=4, max_depth=5),
+             rgapi.make_job(1, width=128, height=96, spp=4, max_depth=5)]
+    say("[stream] autotune on two small probe jobs")
+    t = rgapi.autotune(ctx, resident, probe)
+    d = t.as_dict()
+    say(f"[stream] tuned: {d}")
+    assert d["pool_paths"] in (1 << 20, 1 << 21, 1 << 22) and d["stack"] in (32, 64, 128, 256)
+    outs, _ = rgapi.render_batch(ctx, resident, make_jobs()[:2], tune=t)
+    for i in range(2):
+        assert_same(outs[i], single[i], f"tuned job {i}")
