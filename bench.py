@@ -297,7 +297,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-render", action="store_true", help="traversal microbenchmark only")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests per arrival rate")
-    ap.add_argument("--skip-many", action="store_true", help="skip the batched and served legs")
+    ap.add_argument("--legs", default="cube,frame,batch,serve",
+                    help="comma-separated workloads run after the traversal headline: cube, frame, batch, serve")
+    ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -392,23 +394,27 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         extra["copy_calibration_bytes"] = 1 << 30
-    if not args.skip_render:
+    legs = set() if args.skip_render else {x for x in args.legs.split(",") if x}
+    if "cube" in legs:
         log("cube room")
         extra["cube_room"] = bench_cube_room(ctx, stream, max(3, args.steps // 2), 8.0, not args.no_cpu_baseline)
+    if "frame" in legs:
         log("bsdf frame")
         extra["bsdf_frame"] = bench_bsdf_frame(ctx, not args.no_cpu_baseline, 8.0)
-    if not args.skip_render and not args.skip_many:
+    if legs & {"batch", "serve"}:
         import scenes
 
         log("resident scenes")
         room = rgapi.RenderScene(ctx, scenes.cube_room())
         frame = rgapi.RenderScene(ctx, scenes.bsdf_frame(grid=5, stacks=200, slices=200))
-        log("batch")
-        extra["batch"] = bench_batch(ctx, [room, frame])
-        frame2 = rgapi.RenderScene(ctx, scenes.bsdf_frame(grid=5, stacks=200, slices=200, seed=0xB5DF0003))
-        ico = rgapi.RenderScene(ctx, scenes.ico_ggx())
-        log("serve")
-        extra["serve"] = bench_serve(ctx, [room, frame, frame2, ico], args.serve_requests, (0.5, 1.0, 2.0))
+        if "batch" in legs:
+            log("batch")
+            extra["batch"] = bench_batch(ctx, [room, frame], n_jobs=args.batch_jobs)
+        if "serve" in legs:
+            frame2 = rgapi.RenderScene(ctx, scenes.bsdf_frame(grid=5, stacks=200, slices=200, seed=0xB5DF0003))
+            ico = rgapi.RenderScene(ctx, scenes.ico_ggx())
+            log("serve")
+            extra["serve"] = bench_serve(ctx, [room, frame, frame2, ico], args.serve_requests, (0.5, 1.0, 2.0))
 
     out = {
         "metric": "closest-hit traversal throughput, 1.31M-tri displaced icosphere, 16M incoherent rays",

@@ -435,6 +435,13 @@ struct Loader {
                 if (c->arr[k].is_number()) m.color[k] = (float)c->arr[k].num;
         const json::Value *t = o.get("tex");
         if (t && t->is_string() && !t->str.empty()) warnings.push_back("texture ignored: " + t->str);
+        // read by the reference but outside the -d 5 path kept here
+        float cd = 0.0f, sr = 0.0f;
+        num(o, "chromaDispersion", cd);
+        if (cd > 0.0f) warnings.push_back("chromatic dispersion ignored");
+        num(o, "spotRange", sr);
+        if (sr > 0.0f) warnings.push_back("spot range ignored");
+        if (o.get("normalmap")) warnings.push_back("normal map ignored");
     }
 
     const rg_object *find_material(const std::string &name) const {

@@ -32,7 +32,7 @@ def test_object_order_and_counts(loaded):
 
 def test_mesh_file_matches_independent_parse(loaded):
     _, _, d = loaded
-    raw = open(os.path.join(GOLD, "sphere_v0102.mesh"), "rb").read()
+    raw = open(os.path.join(GOLD, "ref_scenes", "sphereArray", "Sphere.mesh"), "rb").read()
     tag, ver, flags, length = struct.unpack("<IHHI", raw[:12])
     vcount = struct.unpack("<I", raw[12:16])[0]
     assert (tag, ver, flags, length, vcount) == (0x6873656D, 0x102, 0xA, 80, 1584)
@@ -186,3 +186,104 @@ def test_rgx_cli_argument_errors():
     assert r.returncode == 1 and "only -d 5" in r.stderr
     r = subprocess.run([exe, "render", "x.json", "--bogus"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "unknown argument" in r.stdout
+
+
+REF = os.path.join(GOLD, "ref_scenes")
+
+
+def test_reference_cube_room_scene():
+    """The reference's own cubeRoom.json: six planes and two rotated cubes
+    (36 triangles), camera at z = 5.6 with a 70 degree field of view."""
+    ctx = rgapi.Context(-1)
+    _, d = rgapi.load_json(ctx, os.path.join(REF, "cubeRoom", "cubeRoom.json"))
+    assert d["warnings"] == []
+    assert len(d["obj"]) == 36 and len(d["objects"]) == 8
+    assert np.isclose(d["camera"]["fov_deg"], 70)
+    floor = tris_of(d, 2).reshape(-1, 3)
+    assert np.allclose(floor[:, 1], -3) and np.allclose(sorted(set(np.round(floor[:, 2], 4))), [-8.6, -2.6])
+    objs = d["objects"]
+    assert objs[2]["glossy"] == np.float32(0.7) and objs[2]["metallic"] == 1 and objs[2]["roughness"] == 0.5
+    np.testing.assert_allclose(objs[3]["color"], [1, 0x55 / 255, 0x55 / 255])  # left wall "#ff5555"
+    assert objs[0]["emission"] == 10
+    # top light: 180 degrees about X faces down
+    assert np.allclose(d["n9"][d["obj"] == 0].reshape(-1, 3), [0, -1, 0], atol=1e-6)
+
+
+def test_reference_sphere_array_scene():
+    """sphereArray.json: five Sphere.mesh instances, a camera tilted by -14
+    degrees about X with depth of field, and a textured floor (texture
+    reported, not loaded)."""
+    ctx = rgapi.Context(-1)
+    _, d = rgapi.load_json(ctx, os.path.join(REF, "sphereArray", "sphereArray.json"))
+    assert len(d["obj"]) == 2 + 2 + 5 * 528
+    assert any("checkerboard" in w for w in d["warnings"])
+    cam = d["camera"]
+    assert np.isclose(cam["depth_of_field"], 6.2) and np.isclose(cam["aperture"], 5.6)
+    # render space = R_x(+14 deg) (world - location): the floor plane's corners
+    a = np.radians(14.0)
+    R = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    corners = np.array([[sx * 6, -0.5, sz * 6] for sx in (-1, 1) for sz in (-1, 1)]) - [0, 2.2, 6.0]
+    want = corners @ R.T
+    got = tris_of(d, 1).reshape(-1, 3)
+    for w in want:
+        assert np.min(np.linalg.norm(got - w, axis=1)) < 1e-4
+    obj3 = d["objects"][4]
+    assert np.isclose(obj3["refraction"], 0.8) and np.isclose(obj3["roughness"], 0.01)
+
+
+def test_reference_medium_and_aniso_scenes():
+    ctx = rgapi.Context(-1)
+    _, d = rgapi.load_json(ctx, os.path.join(REF, "mediumTest", "waterCube.json"))
+    m = d["media"][0]
+    assert np.allclose(m["sigma_a"], [1.5, 0.4, 0.2]) and m["density"] == 2
+    cube = [o for o in d["objects"] if o["interior_medium"] == 0]
+    assert len(cube) == 1 and cube[0]["refraction"] == 1 and np.isclose(cube[0]["refraction_ratio"], 1.33)
+    assert any("sky_sun.hdr" in w for w in d["warnings"])
+    _, d = rgapi.load_json(ctx, os.path.join(REF, "mediumTest", "cloudHomo.json"))
+    assert np.isclose(d["media"][0]["g"], 0.6) and d["objects"][1]["transparency"] == 1
+    _, d = rgapi.load_json(ctx, os.path.join(REF, "anisoTest", "anisoTest.json"))
+    assert np.isclose(d["objects"]["anisotropy"].max(), 0.95)
+
+
+def _rgbe_decode(path):
+    """Independent Radiance RLE decoder (new-style scanlines)."""
+    raw = open(path, "rb").read()
+    pos = raw.index(b"\n\n") + 2
+    eol = raw.index(b"\n", pos)
+    _, h, _, w = raw[pos:eol].split()
+    h, w = int(h), int(w)
+    pos = eol + 1
+    out = np.zeros((h, w, 4), np.uint8)
+    for y in range(h):
+        assert raw[pos] == 2 and raw[pos + 1] == 2
+        pos += 4
+        for c in range(4):
+            x = 0
+            while x < w:
+                n = raw[pos]
+                pos += 1
+                if n > 128:
+                    out[y, x:x + n - 128, c] = raw[pos]
+                    pos += 1
+                    x += n - 128
+                else:
+                    out[y, x:x + n, c] = np.frombuffer(raw[pos:pos + n], np.uint8)
+                    pos += n
+                    x += n
+    e = out[..., 3].astype(np.int32)
+    f = np.where(e > 0, np.ldexp(np.float32(1.0), e - 136), 0).astype(np.float32)
+    return out[..., :3].astype(np.float32) * f[..., None]
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/resources/scenes"), reason="reference checkout absent")
+def test_reference_scene_in_place_with_environment():
+    """Loaded where it lies in the reference checkout, waterCube.json finds
+    its run-length-encoded sky_sun.hdr; the decoded map equals an
+    independent decoder's."""
+    ctx = rgapi.Context(-1)
+    _, d = rgapi.load_json(ctx, "/root/reference/resources/scenes/mediumTest/waterCube.json")
+    assert d["warnings"] == []
+    env = d["env"]
+    assert env["rgb"].shape == (512, 1024, 3) and np.isclose(env["intensity"], 1.5)
+    want = _rgbe_decode("/root/reference/resources/textures/sky_sun.hdr")
+    np.testing.assert_array_equal(env["rgb"], want)
