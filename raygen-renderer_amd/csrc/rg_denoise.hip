@@ -30,6 +30,7 @@ __device__ __forceinline__ float lum3(float r, float g, float b) { return 0.2126
 template <int kStack>
 __global__ __launch_bounds__(256) void k_aov(DevBvh bv, PTScene sc, float view_far, float4 *__restrict__ nrm,
                                               float4 *__restrict__ alb, float *__restrict__ dep) {
+    RG_LDS_COLUMN(lc, 256);
     const uint32_t pix = blockIdx.x * 256 + threadIdx.x;
     const uint32_t npix = sc.width * sc.height;
     if (pix >= npix) return;
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(256) void k_aov(DevBvh bv, PTScene sc, float view_f
     const float ddx = dx * il, ddy = dy * il, ddz = -1.0f * il;
     float t = 1.0e6f, u = 0.0f, v = 0.0f;
     uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack>(bv, 0.0f, 0.0f, 0.0f, ddx, ddy, ddz, 0.0f, t, prim, u, v);
+    trace_closest_dev<kStack>(bv, 0.0f, 0.0f, 0.0f, ddx, ddy, ddz, 0.0f, t, prim, u, v, lc);
     if (prim != 0xFFFFFFFFu) {
         const DevTriShade &s = sc.shade[prim];
         const float w = 1.0f - u - v;

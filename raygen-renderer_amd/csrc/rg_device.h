@@ -101,24 +101,49 @@ struct DevBvh {
     int has_tree;
 };
 
+// The top of each lane's traversal stack lives in LDS, column-major across
+// the block (entry e of thread x at [e * kStride + x]: consecutive lanes hit
+// consecutive banks); entries deeper than kLdsStack fall back to private
+// (scratch) memory.  Keeping the hot part out of scratch removes most of the
+// traversal's HBM writes.
+#ifndef RG_LDS_STACK
+#define RG_LDS_STACK 8
+#endif
+constexpr int kLdsStack = RG_LDS_STACK;
+constexpr int kLdsWords = kLdsStack > 0 ? kLdsStack : 1;
+
+template <int kStride>
+struct LdsColumn {
+    uint32_t *ref;  // this lane's entry 0; entry e at ref[e * kStride]
+    float *t;       // entry distances (closest hit), or more refs (any hit)
+};
+
+// Declares the block's LDS stack area and this lane's column.
+#define RG_LDS_COLUMN(name, kBlk)                                  \
+    __shared__ uint32_t name##_ref[kLdsWords * (kBlk)];            \
+    __shared__ float name##_t[kLdsWords * (kBlk)];                 \
+    const LdsColumn<(kBlk)> name{name##_ref + threadIdx.x, name##_t + threadIdx.x}
+
 // Closest-hit traversal of one ray (TriangleBVH::intersectClosest,
 // bvh.cpp:233-285) over the 4-wide tree.  Entries on the stack carry the
 // child's entry distance so the reference's "tmin > closest t" rejection is
 // applied when the entry is popped, exactly when the binary traversal would
 // have tested that node.  On return t/prim/u/v hold the closest hit (prim =
 // 0xFFFFFFFF and t unchanged on a miss).
-template <int kStack>
+template <int kStack, int kStride>
 __device__ __forceinline__ void trace_closest_dev(const DevBvh &bv, float ox, float oy, float oz,
                                                   float dx, float dy, float dz, float rtmin,
-                                                  float &t, uint32_t &prim, float &bu, float &bv_) {
+                                                  float &t, uint32_t &prim, float &bu, float &bv_,
+                                                  const LdsColumn<kStride> col) {
     const float ix = inv_dir(dx), iy = inv_dir(dy), iz = inv_dir(dz);
     const int oct = (ix < 0.0f ? 1 : 0) | (iy < 0.0f ? 2 : 0) | (iz < 0.0f ? 4 : 0);
     float root_tmin;
     if (!bv.has_tree || !slab(bv.root.lo[0], bv.root.lo[1], bv.root.lo[2], bv.root.hi[0], bv.root.hi[1],
                               bv.root.hi[2], ox, oy, oz, ix, iy, iz, t, root_tmin))
         return;
-    uint32_t stk_ref[kStack];
-    float stk_t[kStack];
+    constexpr int kSpill = kStack > kLdsStack ? kStack - kLdsStack : 1;
+    uint32_t stk_ref[kSpill];
+    float stk_t[kSpill];
     int sp = 0;
     uint32_t cur = 0;
     for (;;) {
@@ -148,8 +173,13 @@ __device__ __forceinline__ void trace_closest_dev(const DevBvh &bv, float ox, fl
                 const int s = (ord >> (2 * j)) & 3;
                 if (!ok[s]) continue;
                 if (have) {
-                    stk_ref[sp] = next;
-                    stk_t[sp] = next_t;
+                    if (sp < kLdsStack) {
+                        col.ref[sp * kStride] = next;
+                        col.t[sp * kStride] = next_t;
+                    } else {
+                        stk_ref[sp - kLdsStack] = next;
+                        stk_t[sp - kLdsStack] = next_t;
+                    }
                     sp++;
                 }
                 next = cref[s];
@@ -171,7 +201,13 @@ __device__ __forceinline__ void trace_closest_dev(const DevBvh &bv, float ox, fl
         bool found = false;
         while (sp > 0) {
             sp--;
-            if (!(stk_t[sp] > t)) { cur = stk_ref[sp]; found = true; break; }
+            const bool in_lds = sp < kLdsStack;
+            const float et = in_lds ? col.t[sp * kStride] : stk_t[sp - kLdsStack];
+            if (!(et > t)) {
+                cur = in_lds ? col.ref[sp * kStride] : stk_ref[sp - kLdsStack];
+                found = true;
+                break;
+            }
         }
         if (!found) break;
     }
@@ -180,16 +216,21 @@ __device__ __forceinline__ void trace_closest_dev(const DevBvh &bv, float ox, fl
 // Any-hit traversal (TriangleBVH::intersectAny, bvh.h:63-101) with maxT =
 // tmax; a triangle occludes when its occluder bits include pred_bit.  The
 // boolean result does not depend on visit order.
-template <int kStack>
+template <int kStack, int kStride>
 __device__ __forceinline__ bool trace_any_dev(const DevBvh &bv, float ox, float oy, float oz, float dx,
-                                              float dy, float dz, float maxt, uint32_t pred_bit) {
+                                              float dy, float dz, float maxt, uint32_t pred_bit,
+                                              const LdsColumn<kStride> col) {
+    // entries are refs only: the column's distance words hold more of them
+    constexpr int kLds = 2 * kLdsStack;
+    uint32_t *const lt = reinterpret_cast<uint32_t *>(col.t);
     const float ix = inv_dir(dx), iy = inv_dir(dy), iz = inv_dir(dz);
     const int oct = (ix < 0.0f ? 1 : 0) | (iy < 0.0f ? 2 : 0) | (iz < 0.0f ? 4 : 0);
     float tm_unused;
     if (!bv.has_tree || !slab(bv.root.lo[0], bv.root.lo[1], bv.root.lo[2], bv.root.hi[0], bv.root.hi[1],
                               bv.root.hi[2], ox, oy, oz, ix, iy, iz, maxt, tm_unused))
         return false;
-    uint32_t stk[kStack];
+    constexpr int kSpill = kStack > kLds ? kStack - kLds : 1;
+    uint32_t stk[kSpill];
     int sp = 0;
     uint32_t cur = 0;
     for (;;) {
@@ -217,7 +258,12 @@ __device__ __forceinline__ bool trace_any_dev(const DevBvh &bv, float ox, float 
             for (int j = 3; j >= 0; j--) {
                 const int s = (ord >> (2 * j)) & 3;
                 if (!ok[s]) continue;
-                if (have) stk[sp++] = next;
+                if (have) {
+                    if (sp < kLdsStack) col.ref[sp * kStride] = next;
+                    else if (sp < kLds) lt[(sp - kLdsStack) * kStride] = next;
+                    else stk[sp - kLds] = next;
+                    sp++;
+                }
                 next = cref[s];
                 have = true;
             }
@@ -235,7 +281,8 @@ __device__ __forceinline__ bool trace_any_dev(const DevBvh &bv, float ox, float 
             }
         }
         if (sp == 0) break;
-        cur = stk[--sp];
+        sp--;
+        cur = sp < kLdsStack ? col.ref[sp * kStride] : (sp < kLds ? lt[(sp - kLdsStack) * kStride] : stk[sp - kLds]);
     }
     return false;
 }

@@ -77,6 +77,7 @@ __global__ __launch_bounds__(kPtBlock) void k_camera(PTScene sc, PTState st, uin
 
 template <int kStack>
 __global__ __launch_bounds__(kPtBlock) void k_extend(DevBvh bv, PTState st, int qi) {
+    RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
     const uint32_t count = st.count[qi];
     if (i == 0 && count) atomicAdd(st.stats + 0, (unsigned long long)count);
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(DevBvh bv, PTState st, int 
     const rg_ray r = st.ray[p];
     float t = r.tmax, bu = 0.0f, bvv = 0.0f;
     uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmin, t, prim, bu, bvv);
+    trace_closest_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmin, t, prim, bu, bvv, lc);
     st.hit[p] = {prim, t, bu, bvv};
 }
 
@@ -142,6 +143,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
 // bvh.cpp:287-339, with the ray's occluder predicate).
 template <int kStack>
 __global__ __launch_bounds__(kPtBlock) void k_anyhit(DevBvh bv, PTState st) {
+    RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
     const uint32_t n = st.count[2];
     if (i == 0 && n) atomicAdd(st.stats + 1, (unsigned long long)n);
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(kPtBlock) void k_anyhit(DevBvh bv, PTState st) {
     const rg_ray r = st.sray[src];
     const uint32_t pred_bit = 1u << (int)r.tmin;
     st.socc[src] =
-        trace_any_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmax, pred_bit) ? 1 : 0;
+        trace_any_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmax, pred_bit, lc) ? 1 : 0;
 }
 
 // Adds the unoccluded NEE contributions in event order, closes finished

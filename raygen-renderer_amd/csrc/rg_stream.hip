@@ -180,6 +180,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sregen(StreamPool sp, const DevJob
 
 template <int kStack>
 __global__ __launch_bounds__(kPtBlock) void k_sextend(StreamPool sp, const DevJobSlot *__restrict__ jobs) {
+    RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
     if (i >= sp.cap) return;
     const int job = sp.st.sid[i].x;
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sextend(StreamPool sp, const DevJo
     const rg_ray r = sp.st.ray[i];
     float t = r.tmax, bu = 0.0f, bvv = 0.0f;
     uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmin, t, prim, bu, bvv);
+    trace_closest_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmin, t, prim, bu, bvv, lc);
     sp.hit[i] = {prim, t, bu, bvv};
 }
 
@@ -249,6 +250,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sshade(StreamPool sp, const DevJob
 // pulling 64 rays at a time.
 template <int kStack>
 __global__ __launch_bounds__(kPtBlock) void k_sanyhit(StreamPool sp, const DevJobSlot *__restrict__ jobs, int qi) {
+    RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t n = sp.ctr->nsh[qi];
     uint32_t *fetch = &sp.ctr->fetch[qi];
     const uint32_t lane = __lane_id();
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sanyhit(StreamPool sp, const DevJo
             const rg_ray r = sp.sray[src];
             const uint32_t pred_bit = 1u << (int)r.tmin;
             sp.socc[src] =
-                trace_any_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmax, pred_bit) ? 1 : 0;
+                trace_any_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmax, pred_bit, lc) ? 1 : 0;
         }
     }
 }

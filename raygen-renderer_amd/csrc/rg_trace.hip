@@ -81,24 +81,26 @@ constexpr int kBlock = 128;
 template <int kStack>
 __global__ __launch_bounds__(kBlock) void k_closest(DevBvh bv, const rg_ray *__restrict__ rays,
                                                     rg_hit *__restrict__ hits, uint64_t n) {
+    RG_LDS_COLUMN(lc, kBlock);
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float4 r0 = reinterpret_cast<const float4 *>(rays)[2 * i];
     const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
     float t = r1.w, bu = 0.0f, bvv = 0.0f;
     uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w, t, prim, bu, bvv);
+    trace_closest_dev<kStack>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w, t, prim, bu, bvv, lc);
     reinterpret_cast<float4 *>(hits)[i] = make_float4(__uint_as_float(prim), t, bu, bvv);
 }
 
 template <int kStack>
 __global__ __launch_bounds__(kBlock) void k_any(DevBvh bv, const rg_ray *__restrict__ rays,
                                                 uint8_t *__restrict__ occ, uint64_t n, uint32_t pred_bit) {
+    RG_LDS_COLUMN(lc, kBlock);
     const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float4 r0 = reinterpret_cast<const float4 *>(rays)[2 * i];
     const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
-    occ[i] = trace_any_dev<kStack>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, pred_bit) ? 1 : 0;
+    occ[i] = trace_any_dev<kStack>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, pred_bit, lc) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_gen_rays(float cx, float cy, float cz, float ex, float ey,
