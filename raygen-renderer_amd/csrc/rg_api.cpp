@@ -145,15 +145,18 @@ int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params) {
     if (params) p = *params;
     if (p.bins < 2 || p.bins > 256) return set_error(RG_E_INVALID, "bins must be in [2, 256]");
     if (p.leaf < 1 || p.leaf > 0xFFFF) return set_error(RG_E_INVALID, "leaf must be in [1, 65535]");
-    if (p.width != 2 && p.width != 4) return set_error(RG_E_UNSUPPORTED, "width must be 2 or 4");
+    if (p.width != 2 && p.width != 4 && p.width != 8) return set_error(RG_E_UNSUPPORTED, "width must be 2, 4 or 8");
     try {
         build_bvh2(s->v9.data(), s->n_tris, p.bins, p.leaf, s->nodes2, s->perm);
-        collapse_bvh4(s->nodes2, s->nodes4);
+        s->nodes4.clear();
+        s->nodes8.clear();
+        if (p.width == 8) collapse_bvh8(s->nodes2, s->nodes8);
+        else collapse_bvh4(s->nodes2, s->nodes4);
     } catch (const std::bad_alloc &) {
         return set_error(RG_E_NOMEM, "BVH build ran out of host memory");
     }
-    s->width = (int)p.width;
-    s->stack_need = stack_need4(s->nodes4);
+    s->width = p.width == 8 ? 8 : 4;
+    s->stack_need = p.width == 8 ? stack_need8(s->nodes8) : stack_need4(s->nodes4);
     s->built = true;
     if (c->device < 0) return RG_OK;
 
@@ -162,10 +165,12 @@ int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params) {
     s->device = c->device;
     std::vector<Tri48> tris;
     make_tris(s->v9.data(), s->nodes2, s->perm, s->pred_mask, tris);
-    const size_t b4 = s->nodes4.size() * sizeof(Node4), bt = tris.size() * sizeof(Tri48);
+    const void *hn = s->width == 8 ? (const void *)s->nodes8.data() : (const void *)s->nodes4.data();
+    const size_t b4 = s->width == 8 ? s->nodes8.size() * sizeof(Node8) : s->nodes4.size() * sizeof(Node4);
+    const size_t bt = tris.size() * sizeof(Tri48);
     if (int rc = dev_malloc(&s->d_nodes4, b4)) return rc;
     if (int rc = dev_malloc(&s->d_tris, bt)) { free_device(s); return rc; }
-    if (int rc = dev_upload(s->d_nodes4, s->nodes4.data(), b4, c->stream)) { free_device(s); return rc; }
+    if (int rc = dev_upload(s->d_nodes4, hn, b4, c->stream)) { free_device(s); return rc; }
     if (int rc = dev_upload(s->d_tris, tris.data(), bt, c->stream)) { free_device(s); return rc; }
     s->device_bytes = b4 + bt;
     s->tris_dirty = false;

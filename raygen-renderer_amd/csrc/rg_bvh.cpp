@@ -194,9 +194,24 @@ void build_bvh2(const float *v9, uint32_t n, uint32_t bins, uint32_t leaf,
 
 namespace {
 
+// Packs a visit order (slots ord[0..W-1], first visited first) into a node.
+inline void set_order(Node4 &nd, int oct, const int *ord) {
+    nd.order[oct] = (uint8_t)(ord[0] | (ord[1] << 2) | (ord[2] << 4) | (ord[3] << 6));
+}
+inline void set_order(Node8 &nd, int oct, const int *ord) {
+    uint32_t o = 0;
+    for (int s = 0; s < 8; s++) o |= (uint32_t)ord[s] << (3 * s);
+    nd.order[oct] = o;
+}
+
+// Folds binary levels into W-wide nodes: starting from a binary node, the
+// internal child with the largest surface area is expanded until W slots are
+// filled or only leaves remain.  Per octant, the slots are ordered the way
+// the binary tree's near-child-first rule would reach them.
+template <class NodeT, int W>
 struct Collapse {
     const std::vector<Node2> &n2;
-    std::vector<Node4> &out;
+    std::vector<NodeT> &out;
 
     static float area(const Node2 &n) {
         const float dx = n.bmax[0] - n.bmin[0];
@@ -224,10 +239,10 @@ struct Collapse {
         }
     }
 
-    void fill(uint32_t idx4, uint32_t k2) {
+    void fill(uint32_t idxw, uint32_t k2) {
         std::vector<uint32_t> slots = {n2[k2].first_or_left, n2[k2].first_or_left + 1};
         std::vector<uint32_t> expanded = {k2};
-        while (slots.size() < 4) {
+        while ((int)slots.size() < W) {
             int pick = -1;
             float best = -1.0f;
             for (int s = 0; s < (int)slots.size(); s++) {
@@ -240,10 +255,10 @@ struct Collapse {
             slots[pick] = n2[k].first_or_left;
             slots.insert(slots.begin() + pick + 1, n2[k].first_or_left + 1);
         }
-        Node4 nd;
+        NodeT nd;
         std::memset(&nd, 0, sizeof nd);
         std::vector<std::pair<int, uint32_t>> inner;  // (slot, binary node)
-        for (int s = 0; s < 4; s++) {
+        for (int s = 0; s < W; s++) {
             if (s >= (int)slots.size()) { nd.child[s] = kEmpty; continue; }
             const Node2 &c = n2[slots[s]];
             nd.lox[s] = c.bmin[0]; nd.loy[s] = c.bmin[1]; nd.loz[s] = c.bmin[2];
@@ -252,41 +267,72 @@ struct Collapse {
             else inner.push_back({s, slots[s]});
         }
         for (int oct = 0; oct < 8; oct++) {
-            int ord[4], m = 0;
+            int ord[W], m = 0;
             order(k2, oct, slots, expanded, ord, m);
-            for (int s = 0; s < 4; s++)
+            for (int s = 0; s < W; s++)
                 if (std::find(ord, ord + m, s) == ord + m) ord[m++] = s;
-            nd.order[oct] = (uint8_t)(ord[0] | (ord[1] << 2) | (ord[2] << 4) | (ord[3] << 6));
+            set_order(nd, oct, ord);
         }
         const uint32_t base = (uint32_t)out.size();
         out.resize(out.size() + inner.size());
         for (size_t i = 0; i < inner.size(); i++) nd.child[inner[i].first] = base + (uint32_t)i;
-        out[idx4] = nd;
+        out[idxw] = nd;
         for (size_t i = 0; i < inner.size(); i++) fill(base + (uint32_t)i, inner[i].second);
     }
 };
 
-}  // namespace
-
-void collapse_bvh4(const std::vector<Node2> &nodes2, std::vector<Node4> &out) {
+template <class NodeT, int W>
+void collapse_wide(const std::vector<Node2> &nodes2, std::vector<NodeT> &out) {
     out.clear();
     if (nodes2.empty()) return;
     out.resize(1);
     if (nodes2[0].count > 0) {
         // A leaf root: one leaf child whose box is the root box.
-        Node4 nd;
+        NodeT nd;
         std::memset(&nd, 0, sizeof nd);
         const Node2 &r = nodes2[0];
         nd.lox[0] = r.bmin[0]; nd.loy[0] = r.bmin[1]; nd.loz[0] = r.bmin[2];
         nd.hix[0] = r.bmax[0]; nd.hiy[0] = r.bmax[1]; nd.hiz[0] = r.bmax[2];
         nd.child[0] = kLeafBit | r.first_or_left;
-        nd.child[1] = nd.child[2] = nd.child[3] = kEmpty;
-        for (int oct = 0; oct < 8; oct++) nd.order[oct] = (uint8_t)(0 | (1 << 2) | (2 << 4) | (3 << 6));
+        for (int s = 1; s < W; s++) nd.child[s] = kEmpty;
+        int ord[W];
+        for (int s = 0; s < W; s++) ord[s] = s;
+        for (int oct = 0; oct < 8; oct++) set_order(nd, oct, ord);
         out[0] = nd;
         return;
     }
-    Collapse c{nodes2, out};
+    Collapse<NodeT, W> c{nodes2, out};
     c.fill(0, 0);
+}
+
+// Deepest sum of (children - 1) pushed along any root-to-node path, plus one.
+template <class NodeT, int W>
+uint32_t stack_need_wide(const std::vector<NodeT> &nodes) {
+    if (nodes.empty()) return 0;
+    std::vector<std::pair<uint32_t, uint32_t>> st = {{0u, 0u}};
+    uint32_t best = 0;
+    while (!st.empty()) {
+        auto [idx, acc] = st.back();
+        st.pop_back();
+        const NodeT &nd = nodes[idx];
+        uint32_t valid = 0;
+        for (int s = 0; s < W; s++) valid += nd.child[s] != kEmpty;
+        const uint32_t here = acc + (valid > 0 ? valid - 1 : 0);
+        best = std::max(best, here);
+        for (int s = 0; s < W; s++)
+            if (nd.child[s] != kEmpty && !(nd.child[s] & kLeafBit)) st.push_back({nd.child[s], here});
+    }
+    return best + 1;
+}
+
+}  // namespace
+
+void collapse_bvh4(const std::vector<Node2> &nodes2, std::vector<Node4> &out) {
+    collapse_wide<Node4, 4>(nodes2, out);
+}
+
+void collapse_bvh8(const std::vector<Node2> &nodes2, std::vector<Node8> &out) {
+    collapse_wide<Node8, 8>(nodes2, out);
 }
 
 void make_tris(const float *v9, const std::vector<Node2> &nodes2,
@@ -316,23 +362,7 @@ void make_tris(const float *v9, const std::vector<Node2> &nodes2,
     }
 }
 
-uint32_t stack_need4(const std::vector<Node4> &nodes) {
-    if (nodes.empty()) return 0;
-    // deepest sum of (children - 1) pushed along any root-to-node path
-    std::vector<std::pair<uint32_t, uint32_t>> st = {{0u, 0u}};
-    uint32_t best = 0;
-    while (!st.empty()) {
-        auto [idx, acc] = st.back();
-        st.pop_back();
-        const Node4 &nd = nodes[idx];
-        uint32_t valid = 0;
-        for (int s = 0; s < 4; s++) valid += nd.child[s] != kEmpty;
-        const uint32_t here = acc + (valid > 0 ? valid - 1 : 0);
-        best = std::max(best, here);
-        for (int s = 0; s < 4; s++)
-            if (nd.child[s] != kEmpty && !(nd.child[s] & kLeafBit)) st.push_back({nd.child[s], here});
-    }
-    return best + 1;
-}
+uint32_t stack_need4(const std::vector<Node4> &nodes) { return stack_need_wide<Node4, 4>(nodes); }
+uint32_t stack_need8(const std::vector<Node8> &nodes) { return stack_need_wide<Node8, 8>(nodes); }
 
 }  // namespace rg

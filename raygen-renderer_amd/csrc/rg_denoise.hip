@@ -27,7 +27,7 @@ __device__ __forceinline__ float lum3(float r, float g, float b) { return 0.2126
 // Guide buffers from the pixel-centre camera ray (accumulatePixelSamples
 // :819-844): normal*0.5+0.5 / material colour / 1 - clamp(sqrt(dist/far)),
 // alpha 1 on a hit; (0.5, backColor, 0) with alpha 0 on a miss.
-template <int kStack>
+template <int kStack, int kW>
 __global__ __launch_bounds__(256) void k_aov(DevBvh bv, PTScene sc, float view_far, float4 *__restrict__ nrm,
                                               float4 *__restrict__ alb, float *__restrict__ dep) {
     RG_LDS_COLUMN(lc, 256);
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_aov(DevBvh bv, PTScene sc, float view_f
     const float ddx = dx * il, ddy = dy * il, ddz = -1.0f * il;
     float t = 1.0e6f, u = 0.0f, v = 0.0f;
     uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack>(bv, 0.0f, 0.0f, 0.0f, ddx, ddy, ddz, 0.0f, t, prim, u, v, lc);
+    trace_closest_dev<kStack, 256, kW>(bv, 0.0f, 0.0f, 0.0f, ddx, ddy, ddz, 0.0f, t, prim, u, v, lc);
     if (prim != 0xFFFFFFFFu) {
         const DevTriShade &s = sc.shade[prim];
         const float w = 1.0f - u - v;
@@ -223,15 +223,12 @@ int launch_aov(const Scene &scn, const PTScene &sc, void *d_aov, void *stream) {
     float *dep = (float *)(alb + npix);
     const DevBvh bv = dev_bvh(scn);
     const dim3 grid((npix + 255) / 256);
-    if (scn.stack_need <= 64)
-        hipLaunchKernelGGL(k_aov<64>, grid, dim3(256), 0, (hipStream_t)stream, bv, sc, scn.camera.view_far, nrm, alb,
-                           dep);
-    else if (scn.stack_need <= 256)
-        hipLaunchKernelGGL(k_aov<256>, grid, dim3(256), 0, (hipStream_t)stream, bv, sc, scn.camera.view_far, nrm,
-                           alb, dep);
-    else
-        return set_error(RG_E_UNSUPPORTED, "BVH too deep for the traversal stack");
-    return chk(hipGetLastError(), "aov launch");
+    return with_traversal(scn.stack_need, scn.width, [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL((k_aov<T::kStack, T::kW>), grid, dim3(256), 0, (hipStream_t)stream, bv, sc,
+                           scn.camera.view_far, nrm, alb, dep);
+        return chk(hipGetLastError(), "aov launch");
+    });
 }
 
 int launch_denoise(const rg_settings &st, uint32_t width, uint32_t height, const PTScene &, const Scene &,

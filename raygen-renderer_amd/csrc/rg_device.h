@@ -95,10 +95,71 @@ __device__ __forceinline__ bool mt_any(const float4 a, const float4 b, const flo
 
 // Device view of a built scene's acceleration structure.
 struct DevBvh {
-    const Node4 *nodes;
+    const void *nodes;  // Node4 or Node8 array (the kernel's width template)
     const float4 *tris;
     RootBox root;
     int has_tree;
+    int width;          // 4 or 8
+};
+
+// One wide node's child tests: for each slot whether the ray enters the
+// child box before tmax (and the entry distance), the child refs and the
+// visit order for the ray's octant (kOrdBits per slot, first in the low bits).
+template <int W>
+struct WideTest;
+
+template <>
+struct WideTest<4> {
+    static constexpr int kOrdBits = 2;
+    __device__ __forceinline__ static void run(const void *nodes, uint32_t cur, int oct, float ox, float oy,
+                                               float oz, float ix, float iy, float iz, float tmax, bool *ok,
+                                               float *tm, uint32_t *cref, uint32_t &ord) {
+        const Node4 *nd = reinterpret_cast<const Node4 *>(nodes) + cur;
+        const float4 lx = *reinterpret_cast<const float4 *>(nd->lox);
+        const float4 ly = *reinterpret_cast<const float4 *>(nd->loy);
+        const float4 lz = *reinterpret_cast<const float4 *>(nd->loz);
+        const float4 hx = *reinterpret_cast<const float4 *>(nd->hix);
+        const float4 hy = *reinterpret_cast<const float4 *>(nd->hiy);
+        const float4 hz = *reinterpret_cast<const float4 *>(nd->hiz);
+        const uint4 ch = *reinterpret_cast<const uint4 *>(nd->child);
+        const uint2 ordw = *reinterpret_cast<const uint2 *>(nd->order);
+        ord = (oct < 4 ? (ordw.x >> (8 * oct)) : (ordw.y >> (8 * (oct - 4)))) & 0xFFu;
+        cref[0] = ch.x; cref[1] = ch.y; cref[2] = ch.z; cref[3] = ch.w;
+        ok[0] = ch.x != kEmpty && slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, tmax, tm[0]);
+        ok[1] = ch.y != kEmpty && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, tmax, tm[1]);
+        ok[2] = ch.z != kEmpty && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, tmax, tm[2]);
+        ok[3] = ch.w != kEmpty && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, tmax, tm[3]);
+    }
+};
+
+template <>
+struct WideTest<8> {
+    static constexpr int kOrdBits = 3;
+    __device__ __forceinline__ static void run(const void *nodes, uint32_t cur, int oct, float ox, float oy,
+                                               float oz, float ix, float iy, float iz, float tmax, bool *ok,
+                                               float *tm, uint32_t *cref, uint32_t &ord) {
+        const Node8 *nd = reinterpret_cast<const Node8 *>(nodes) + cur;
+        ord = nd->order[oct];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const float4 lx = *reinterpret_cast<const float4 *>(nd->lox + 4 * h);
+            const float4 ly = *reinterpret_cast<const float4 *>(nd->loy + 4 * h);
+            const float4 lz = *reinterpret_cast<const float4 *>(nd->loz + 4 * h);
+            const float4 hx = *reinterpret_cast<const float4 *>(nd->hix + 4 * h);
+            const float4 hy = *reinterpret_cast<const float4 *>(nd->hiy + 4 * h);
+            const float4 hz = *reinterpret_cast<const float4 *>(nd->hiz + 4 * h);
+            const uint4 ch = *reinterpret_cast<const uint4 *>(nd->child + 4 * h);
+            const int b = 4 * h;
+            cref[b] = ch.x; cref[b + 1] = ch.y; cref[b + 2] = ch.z; cref[b + 3] = ch.w;
+            ok[b] = ch.x != kEmpty && slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, tmax, tm[b]);
+            ok[b + 1] =
+                ch.y != kEmpty && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, tmax, tm[b + 1]);
+            ok[b + 2] =
+                ch.z != kEmpty && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, tmax, tm[b + 2]);
+            ok[b + 3] =
+                ch.w != kEmpty && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, tmax, tm[b + 3]);
+        }
+    }
 };
 
 // The top of each lane's traversal stack lives in LDS, column-major across
@@ -125,16 +186,15 @@ struct LdsColumn {
     const LdsColumn<(kBlk)> name{name##_ref + threadIdx.x, name##_t + threadIdx.x}
 
 // Closest-hit traversal of one ray (TriangleBVH::intersectClosest,
-// bvh.cpp:233-285) over the 4-wide tree.  Entries on the stack carry the
+// bvh.cpp:233-285) over the W-wide tree.  Entries on the stack carry the
 // child's entry distance so the reference's "tmin > closest t" rejection is
 // applied when the entry is popped, exactly when the binary traversal would
 // have tested that node.  On return t/prim/u/v hold the closest hit (prim =
 // 0xFFFFFFFF and t unchanged on a miss).
-template <int kStack, int kStride>
-__device__ __forceinline__ void trace_closest_dev(const DevBvh &bv, float ox, float oy, float oz,
-                                                  float dx, float dy, float dz, float rtmin,
-                                                  float &t, uint32_t &prim, float &bu, float &bv_,
-                                                  const LdsColumn<kStride> col) {
+template <int kStack, int kStride, int W>
+__device__ __forceinline__ void trace_closest_dev(const DevBvh &bv, float ox, float oy, float oz, float dx,
+                                                float dy, float dz, float rtmin, float &t, uint32_t &prim,
+                                                float &bu, float &bv_, const LdsColumn<kStride> col) {
     const float ix = inv_dir(dx), iy = inv_dir(dy), iz = inv_dir(dz);
     const int oct = (ix < 0.0f ? 1 : 0) | (iy < 0.0f ? 2 : 0) | (iz < 0.0f ? 4 : 0);
     float root_tmin;
@@ -142,35 +202,23 @@ __device__ __forceinline__ void trace_closest_dev(const DevBvh &bv, float ox, fl
                               bv.root.hi[2], ox, oy, oz, ix, iy, iz, t, root_tmin))
         return;
     constexpr int kSpill = kStack > kLdsStack ? kStack - kLdsStack : 1;
+    constexpr int kOrdBits = WideTest<W>::kOrdBits;
     uint32_t stk_ref[kSpill];
     float stk_t[kSpill];
     int sp = 0;
     uint32_t cur = 0;
     for (;;) {
         if (!(cur & kLeafBit)) {
-            const Node4 *nd = bv.nodes + cur;
-            const float4 lx = *reinterpret_cast<const float4 *>(nd->lox);
-            const float4 ly = *reinterpret_cast<const float4 *>(nd->loy);
-            const float4 lz = *reinterpret_cast<const float4 *>(nd->loz);
-            const float4 hx = *reinterpret_cast<const float4 *>(nd->hix);
-            const float4 hy = *reinterpret_cast<const float4 *>(nd->hiy);
-            const float4 hz = *reinterpret_cast<const float4 *>(nd->hiz);
-            const uint4 ch = *reinterpret_cast<const uint4 *>(nd->child);
-            const uint2 ordw = *reinterpret_cast<const uint2 *>(nd->order);
-            const uint32_t ord = (oct < 4 ? (ordw.x >> (8 * oct)) : (ordw.y >> (8 * (oct - 4)))) & 0xFFu;
-            float tm[4];
-            bool ok[4];
-            ok[0] = ch.x != kEmpty && slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, t, tm[0]);
-            ok[1] = ch.y != kEmpty && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, t, tm[1]);
-            ok[2] = ch.z != kEmpty && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, t, tm[2]);
-            ok[3] = ch.w != kEmpty && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, t, tm[3]);
-            const uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
+            bool ok[W];
+            float tm[W];
+            uint32_t cref[W], ord;
+            WideTest<W>::run(bv.nodes, cur, oct, ox, oy, oz, ix, iy, iz, t, ok, tm, cref, ord);
             bool have = false;
             uint32_t next = 0;
             float next_t = 0.0f;
 #pragma unroll
-            for (int j = 3; j >= 0; j--) {
-                const int s = (ord >> (2 * j)) & 3;
+            for (int j = W - 1; j >= 0; j--) {
+                const int s = (ord >> (kOrdBits * j)) & (W - 1);
                 if (!ok[s]) continue;
                 if (have) {
                     if (sp < kLdsStack) {
@@ -216,12 +264,13 @@ __device__ __forceinline__ void trace_closest_dev(const DevBvh &bv, float ox, fl
 // Any-hit traversal (TriangleBVH::intersectAny, bvh.h:63-101) with maxT =
 // tmax; a triangle occludes when its occluder bits include pred_bit.  The
 // boolean result does not depend on visit order.
-template <int kStack, int kStride>
-__device__ __forceinline__ bool trace_any_dev(const DevBvh &bv, float ox, float oy, float oz, float dx,
-                                              float dy, float dz, float maxt, uint32_t pred_bit,
-                                              const LdsColumn<kStride> col) {
+template <int kStack, int kStride, int W>
+__device__ __forceinline__ bool trace_any_dev(const DevBvh &bv, float ox, float oy, float oz, float dx, float dy,
+                                            float dz, float maxt, uint32_t pred_bit,
+                                            const LdsColumn<kStride> col) {
     // entries are refs only: the column's distance words hold more of them
     constexpr int kLds = 2 * kLdsStack;
+    constexpr int kOrdBits = WideTest<W>::kOrdBits;
     uint32_t *const lt = reinterpret_cast<uint32_t *>(col.t);
     const float ix = inv_dir(dx), iy = inv_dir(dy), iz = inv_dir(dz);
     const int oct = (ix < 0.0f ? 1 : 0) | (iy < 0.0f ? 2 : 0) | (iz < 0.0f ? 4 : 0);
@@ -235,28 +284,15 @@ __device__ __forceinline__ bool trace_any_dev(const DevBvh &bv, float ox, float 
     uint32_t cur = 0;
     for (;;) {
         if (!(cur & kLeafBit)) {
-            const Node4 *nd = bv.nodes + cur;
-            const float4 lx = *reinterpret_cast<const float4 *>(nd->lox);
-            const float4 ly = *reinterpret_cast<const float4 *>(nd->loy);
-            const float4 lz = *reinterpret_cast<const float4 *>(nd->loz);
-            const float4 hx = *reinterpret_cast<const float4 *>(nd->hix);
-            const float4 hy = *reinterpret_cast<const float4 *>(nd->hiy);
-            const float4 hz = *reinterpret_cast<const float4 *>(nd->hiz);
-            const uint4 ch = *reinterpret_cast<const uint4 *>(nd->child);
-            const uint2 ordw = *reinterpret_cast<const uint2 *>(nd->order);
-            const uint32_t ord = (oct < 4 ? (ordw.x >> (8 * oct)) : (ordw.y >> (8 * (oct - 4)))) & 0xFFu;
-            float tm[4];
-            bool ok[4];
-            ok[0] = ch.x != kEmpty && slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, maxt, tm[0]);
-            ok[1] = ch.y != kEmpty && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, maxt, tm[1]);
-            ok[2] = ch.z != kEmpty && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, maxt, tm[2]);
-            ok[3] = ch.w != kEmpty && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, maxt, tm[3]);
-            const uint32_t cref[4] = {ch.x, ch.y, ch.z, ch.w};
+            bool ok[W];
+            float tm[W];
+            uint32_t cref[W], ord;
+            WideTest<W>::run(bv.nodes, cur, oct, ox, oy, oz, ix, iy, iz, maxt, ok, tm, cref, ord);
             bool have = false;
             uint32_t next = 0;
 #pragma unroll
-            for (int j = 3; j >= 0; j--) {
-                const int s = (ord >> (2 * j)) & 3;
+            for (int j = W - 1; j >= 0; j--) {
+                const int s = (ord >> (kOrdBits * j)) & (W - 1);
                 if (!ok[s]) continue;
                 if (have) {
                     if (sp < kLdsStack) col.ref[sp * kStride] = next;
@@ -285,6 +321,22 @@ __device__ __forceinline__ bool trace_any_dev(const DevBvh &bv, float ox, float 
         cur = sp < kLdsStack ? col.ref[sp * kStride] : (sp < kLds ? lt[(sp - kLdsStack) * kStride] : stk[sp - kLds]);
     }
     return false;
+}
+
+// Host-side dispatch to the traversal kernels instantiated for a stack size
+// (64 or 256 entries, the smallest that covers the tree's stack need) and a
+// node width (4 or 8): calls f(TravTag<stack, width>{}).
+template <int S, int W>
+struct TravTag {
+    static constexpr int kStack = S;
+    static constexpr int kW = W;
+};
+
+template <class F>
+int with_traversal(uint32_t stack_need, int width, F &&f) {
+    if (stack_need <= 64) return width == 8 ? f(TravTag<64, 8>{}) : f(TravTag<64, 4>{});
+    if (stack_need <= 256) return width == 8 ? f(TravTag<256, 8>{}) : f(TravTag<256, 4>{});
+    return set_error(RG_E_UNSUPPORTED, "BVH too deep for the traversal stack");
 }
 
 // Philox4x32-10 (Salmon et al., SC'11).

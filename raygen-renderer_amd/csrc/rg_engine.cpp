@@ -146,14 +146,18 @@ class Engine {
         if (int rc = dev_set_device(c->device)) return rc;
         if (int rc = hip_ok(hipStreamCreate(&stream), "stream create")) return rc;
         uint32_t need = 0;
+        int width = 0;
         for (uint32_t i = 0; i < n; i++) {
             Scene *s = reinterpret_cast<Scene *>(sc[i]);
             if (!s) return set_error(RG_E_INVALID, "null scene");
             if (int rc = prepare_scene(s, stream)) return rc;
             need = std::max(need, s->stack_need);
+            if (width && s->width != width)
+                return set_error(RG_E_UNSUPPORTED, "all scenes of one engine need the same BVH width");
+            width = s->width;
             scenes.push_back(s);
         }
-        tune = {1u << 21, 0, 8, 8, 4};
+        tune = {1u << 21, 0, 8, 8, 4, (uint32_t)width};
         if (t) {
             if (t->pool_paths) tune.pool = t->pool_paths;
             if (t->stack) tune.stack = t->stack;

@@ -32,6 +32,17 @@ struct alignas(128) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 must be 128 bytes");
 
+// 8-wide node, 256 B (two L2 lines): the same SoA boxes for 8 children, 8
+// child refs (same encoding as Node4), and per octant a 24-bit visit order
+// (3 bits per slot, first visited in bits 0-2).
+struct alignas(128) Node8 {
+    float lox[8], loy[8], loz[8];
+    float hix[8], hiy[8], hiz[8];
+    uint32_t child[8];
+    uint32_t order[8];
+};
+static_assert(sizeof(Node8) == 256, "Node8 must be 256 bytes");
+
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
@@ -94,14 +105,15 @@ struct Scene {
     std::vector<Node2> nodes2;
     std::vector<uint32_t> perm;
     // wide tree
-    std::vector<Node4> nodes4;
+    std::vector<Node4> nodes4;      // width 2 or 4
+    std::vector<Node8> nodes8;      // width 8
     bool root_is_leaf = false;
     uint32_t root_leaf_first = 0;
     int width = 4;
     uint32_t stack_need = 0;       // deepest traversal stack the wide tree can need
     bool built = false;
     // device copies
-    void *d_nodes4 = nullptr;
+    void *d_nodes4 = nullptr;        // the wide nodes on the device (Node4 or Node8 by width)
     void *d_nodes2 = nullptr;
     void *d_tris = nullptr;
     void *d_tri_flags = nullptr;  // per-triangle occluder mask bits (predicates)
@@ -140,10 +152,12 @@ int set_error(int code, const std::string &msg);
 void build_bvh2(const float *v9, uint32_t n, uint32_t bins, uint32_t leaf,
                 std::vector<Node2> &nodes, std::vector<uint32_t> &perm);
 void collapse_bvh4(const std::vector<Node2> &nodes2, std::vector<Node4> &out);
+void collapse_bvh8(const std::vector<Node2> &nodes2, std::vector<Node8> &out);
 void make_tris(const float *v9, const std::vector<Node2> &nodes2,
                const std::vector<uint32_t> &perm, const std::vector<uint8_t> &pred_mask,
                std::vector<Tri48> &out);
 uint32_t stack_need4(const std::vector<Node4> &nodes);
+uint32_t stack_need8(const std::vector<Node8> &nodes);
 
 // rg_trace.hip (device side)
 int dev_malloc(void **p, size_t bytes);

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kPtBlock) void k_camera(PTScene sc, PTState st, uin
         atomicAdd(st.stats + 2, (unsigned long long)__popcll(m));
 }
 
-template <int kStack>
+template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_extend(DevBvh bv, PTState st, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(DevBvh bv, PTState st, int 
     const rg_ray r = st.ray[p];
     float t = r.tmax, bu = 0.0f, bvv = 0.0f;
     uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmin, t, prim, bu, bvv, lc);
+    trace_closest_dev<kStack, kPtBlock, kW>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmin, t, prim, bu, bvv, lc);
     st.hit[p] = {prim, t, bu, bvv};
 }
 
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
 
 // Any-hit test of every queued shadow ray (TriangleBVH::intersectAny,
 // bvh.cpp:287-339, with the ray's occluder predicate).
-template <int kStack>
+template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_anyhit(DevBvh bv, PTState st) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kPtBlock) void k_anyhit(DevBvh bv, PTState st) {
     const rg_ray r = st.sray[src];
     const uint32_t pred_bit = 1u << (int)r.tmin;
     st.socc[src] =
-        trace_any_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmax, pred_bit, lc) ? 1 : 0;
+        trace_any_dev<kStack, kPtBlock, kW>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmax, pred_bit, lc) ? 1 : 0;
 }
 
 // Adds the unoccluded NEE contributions in event order, closes finished
@@ -327,7 +327,7 @@ static int chk(hipError_t e, const char *what) {
 
 DevBvh dev_bvh(const Scene &s);
 
-template <int kStack>
+template <int kStack, int kW>
 static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint32_t S, uint32_t sample_base,
                            const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max,
                            hipStream_t stream) {
@@ -341,10 +341,10 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
     int qi = 0;
     const int bounces = sc.max_depth + 2;
     for (int b = 0; b < bounces; b++) {
-        hipLaunchKernelGGL(k_extend<kStack>, dim3(blocks), dim3(kPtBlock), 0, stream, bv, st, qi);
+        hipLaunchKernelGGL((k_extend<kStack, kW>), dim3(blocks), dim3(kPtBlock), 0, stream, bv, st, qi);
         hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, stream, st.count + 2, st.count + (qi ^ 1));
         hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, qi);
-        hipLaunchKernelGGL(k_anyhit<kStack>, dim3(2 * blocks), dim3(kPtBlock), 0, stream, bv, st);
+        hipLaunchKernelGGL((k_anyhit<kStack, kW>), dim3(2 * blocks), dim3(kPtBlock), 0, stream, bv, st);
         hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(kPtBlock), 0, stream, st, qi, clamp_max);
         qi ^= 1;
     }
@@ -356,13 +356,11 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
 
 int launch_generation(const Scene &scn, const PTScene &sc, const PTState &st, uint32_t S, uint32_t sample_base,
                       const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream) {
-    if (scn.stack_need <= 64)
-        return generation_impl<64>(scn, sc, st, S, sample_base, d_tile_mask, d_sum, d_sumsq, clamp_max,
-                                   (hipStream_t)stream);
-    if (scn.stack_need <= 256)
-        return generation_impl<256>(scn, sc, st, S, sample_base, d_tile_mask, d_sum, d_sumsq, clamp_max,
-                                    (hipStream_t)stream);
-    return set_error(RG_E_UNSUPPORTED, "BVH too deep for the traversal stack");
+    return with_traversal(scn.stack_need, scn.width, [&](auto tag) {
+        using T = decltype(tag);
+        return generation_impl<T::kStack, T::kW>(scn, sc, st, S, sample_base, d_tile_mask, d_sum, d_sumsq,
+                                                  clamp_max, (hipStream_t)stream);
+    });
 }
 
 int launch_resolve(const float *d_sum, const uint32_t *d_nspp, uint32_t width, uint32_t height, float exposure,

@@ -30,6 +30,7 @@ struct StreamTune {
     uint32_t tile;           // pixel tile edge of the issue order (1 = scanline)
     uint32_t blocks_per_cu;  // persistent any-hit blocks per compute unit
     uint32_t iters;          // iterations per captured graph (even)
+    uint32_t width;          // node width of every resident scene's BVH (4 or 8)
 };
 
 // Device-side counters; iteration parity qi reads slot [qi] and writes [qi^1].

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sregen(StreamPool sp, const DevJob
     }
 }
 
-template <int kStack>
+template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_sextend(StreamPool sp, const DevJobSlot *__restrict__ jobs) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sextend(StreamPool sp, const DevJo
     const rg_ray r = sp.st.ray[i];
     float t = r.tmax, bu = 0.0f, bvv = 0.0f;
     uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmin, t, prim, bu, bvv, lc);
+    trace_closest_dev<kStack, kPtBlock, kW>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmin, t, prim, bu, bvv, lc);
     sp.hit[i] = {prim, t, bu, bvv};
 }
 
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sshade(StreamPool sp, const DevJob
 
 // Any-hit test of every queued NEE ray, one ray per lane, persistent waves
 // pulling 64 rays at a time.
-template <int kStack>
+template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_sanyhit(StreamPool sp, const DevJobSlot *__restrict__ jobs, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t n = sp.ctr->nsh[qi];
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sanyhit(StreamPool sp, const DevJo
             const rg_ray r = sp.sray[src];
             const uint32_t pred_bit = 1u << (int)r.tmin;
             sp.socc[src] =
-                trace_any_dev<kStack>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmax, pred_bit, lc) ? 1 : 0;
+                trace_any_dev<kStack, kPtBlock, kW>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmax, pred_bit, lc) ? 1 : 0;
         }
     }
 }
@@ -386,16 +386,16 @@ int stream_launch_reset(const StreamPool &sp, void *stream) {
     return chk_s(hipGetLastError(), "stream reset launch");
 }
 
-template <int kStack>
+template <int kStack, int kW>
 static void iteration_impl(const StreamPool &sp, const DevJobSlot *jobs, int qi, const StreamTune &tune, int n_cu,
                            hipStream_t s) {
     const uint32_t flat = (sp.cap + kPtBlock - 1) / kPtBlock;
     const uint32_t regen_blocks = std::min<uint32_t>(flat, (uint32_t)n_cu * 4);
     const uint32_t persistent = (uint32_t)n_cu * tune.blocks_per_cu;
     hipLaunchKernelGGL(k_sregen, dim3(regen_blocks), dim3(kPtBlock), 0, s, sp, jobs, qi);
-    hipLaunchKernelGGL(k_sextend<kStack>, dim3(flat), dim3(kPtBlock), 0, s, sp, jobs);
+    hipLaunchKernelGGL((k_sextend<kStack, kW>), dim3(flat), dim3(kPtBlock), 0, s, sp, jobs);
     hipLaunchKernelGGL(k_sshade, dim3(flat), dim3(kPtBlock), 0, s, sp, jobs, qi);
-    hipLaunchKernelGGL(k_sanyhit<kStack>, dim3(persistent), dim3(kPtBlock), 0, s, sp, jobs, qi);
+    hipLaunchKernelGGL((k_sanyhit<kStack, kW>), dim3(persistent), dim3(kPtBlock), 0, s, sp, jobs, qi);
     hipLaunchKernelGGL(k_sresolve, dim3(flat), dim3(kPtBlock), 0, s, sp, jobs, qi);
 }
 
@@ -403,11 +403,12 @@ int stream_launch_iteration(const StreamPool &sp, const void *d_jobs, int qi, co
                             void *stream) {
     const DevJobSlot *jobs = (const DevJobSlot *)d_jobs;
     hipStream_t s = (hipStream_t)stream;
+    const bool w8 = tune.width == 8;
     switch (tune.stack) {
-        case 32: iteration_impl<32>(sp, jobs, qi, tune, n_cu, s); break;
-        case 64: iteration_impl<64>(sp, jobs, qi, tune, n_cu, s); break;
-        case 128: iteration_impl<128>(sp, jobs, qi, tune, n_cu, s); break;
-        case 256: iteration_impl<256>(sp, jobs, qi, tune, n_cu, s); break;
+        case 32: w8 ? iteration_impl<32, 8>(sp, jobs, qi, tune, n_cu, s) : iteration_impl<32, 4>(sp, jobs, qi, tune, n_cu, s); break;
+        case 64: w8 ? iteration_impl<64, 8>(sp, jobs, qi, tune, n_cu, s) : iteration_impl<64, 4>(sp, jobs, qi, tune, n_cu, s); break;
+        case 128: w8 ? iteration_impl<128, 8>(sp, jobs, qi, tune, n_cu, s) : iteration_impl<128, 4>(sp, jobs, qi, tune, n_cu, s); break;
+        case 256: w8 ? iteration_impl<256, 8>(sp, jobs, qi, tune, n_cu, s) : iteration_impl<256, 4>(sp, jobs, qi, tune, n_cu, s); break;
         default: return set_error(RG_E_INVALID, "stack must be 32, 64, 128 or 256");
     }
     return chk_s(hipGetLastError(), "stream iteration launch");

@@ -78,7 +78,7 @@ int dev_sync(void *stream) {
 constexpr int kBlock = 128;
 
 // One thread per ray.
-template <int kStack>
+template <int kStack, int kW>
 __global__ __launch_bounds__(kBlock) void k_closest(DevBvh bv, const rg_ray *__restrict__ rays,
                                                     rg_hit *__restrict__ hits, uint64_t n) {
     RG_LDS_COLUMN(lc, kBlock);
@@ -88,11 +88,11 @@ __global__ __launch_bounds__(kBlock) void k_closest(DevBvh bv, const rg_ray *__r
     const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
     float t = r1.w, bu = 0.0f, bvv = 0.0f;
     uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w, t, prim, bu, bvv, lc);
+    trace_closest_dev<kStack, kBlock, kW>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w, t, prim, bu, bvv, lc);
     reinterpret_cast<float4 *>(hits)[i] = make_float4(__uint_as_float(prim), t, bu, bvv);
 }
 
-template <int kStack>
+template <int kStack, int kW>
 __global__ __launch_bounds__(kBlock) void k_any(DevBvh bv, const rg_ray *__restrict__ rays,
                                                 uint8_t *__restrict__ occ, uint64_t n, uint32_t pred_bit) {
     RG_LDS_COLUMN(lc, kBlock);
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_any(DevBvh bv, const rg_ray *__restr
     if (i >= n) return;
     const float4 r0 = reinterpret_cast<const float4 *>(rays)[2 * i];
     const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
-    occ[i] = trace_any_dev<kStack>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, pred_bit, lc) ? 1 : 0;
+    occ[i] = trace_any_dev<kStack, kBlock, kW>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, pred_bit, lc) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void k_gen_rays(float cx, float cy, float cz, float ex, float ey,
@@ -131,14 +131,15 @@ __global__ __launch_bounds__(256) void k_gen_rays(float cx, float cy, float cz, 
 // ---------------------------------------------------------------------------
 DevBvh dev_bvh(const Scene &s) {
     DevBvh b;
-    b.nodes = (const Node4 *)s.d_nodes4;
+    b.nodes = s.d_nodes4;
+    b.width = s.width;
     b.tris = (const float4 *)s.d_tris;
     if (!s.nodes2.empty()) {
         for (int k = 0; k < 3; k++) { b.root.lo[k] = s.nodes2[0].bmin[k]; b.root.hi[k] = s.nodes2[0].bmax[k]; }
     } else {
         for (int k = 0; k < 3; k++) { b.root.lo[k] = 0.0f; b.root.hi[k] = 0.0f; }
     }
-    b.has_tree = s.nodes4.empty() ? 0 : 1;
+    b.has_tree = (s.nodes4.empty() && s.nodes8.empty()) ? 0 : 1;
     return b;
 }
 
@@ -146,13 +147,12 @@ int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n,
     if (n == 0) return RG_OK;
     const uint64_t blocks = (n + kBlock - 1) / kBlock;
     const DevBvh bv = dev_bvh(s);
-    if (s.stack_need <= 64)
-        hipLaunchKernelGGL(k_closest<64>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream, bv, rays, hits, n);
-    else if (s.stack_need <= 256)
-        hipLaunchKernelGGL(k_closest<256>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream, bv, rays, hits, n);
-    else
-        return set_error(RG_E_UNSUPPORTED, "BVH too deep for the traversal stack");
-    return hip_check(hipGetLastError(), "k_closest launch");
+    return with_traversal(s.stack_need, s.width, [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL((k_closest<T::kStack, T::kW>), dim3((uint32_t)blocks), dim3(kBlock), 0,
+                           (hipStream_t)stream, bv, rays, hits, n);
+        return hip_check(hipGetLastError(), "k_closest launch");
+    });
 }
 
 int launch_any(const Scene &s, const rg_ray *rays, uint8_t *occ, uint64_t n, int pred, void *stream) {
@@ -160,13 +160,12 @@ int launch_any(const Scene &s, const rg_ray *rays, uint8_t *occ, uint64_t n, int
     const uint64_t blocks = (n + kBlock - 1) / kBlock;
     const DevBvh bv = dev_bvh(s);
     const uint32_t bit = 1u << pred;
-    if (s.stack_need <= 64)
-        hipLaunchKernelGGL(k_any<64>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream, bv, rays, occ, n, bit);
-    else if (s.stack_need <= 256)
-        hipLaunchKernelGGL(k_any<256>, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream, bv, rays, occ, n, bit);
-    else
-        return set_error(RG_E_UNSUPPORTED, "BVH too deep for the traversal stack");
-    return hip_check(hipGetLastError(), "k_any launch");
+    return with_traversal(s.stack_need, s.width, [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL((k_any<T::kStack, T::kW>), dim3((uint32_t)blocks), dim3(kBlock), 0,
+                           (hipStream_t)stream, bv, rays, occ, n, bit);
+        return hip_check(hipGetLastError(), "k_any launch");
+    });
 }
 
 int launch_gen_rays(const float box[6], float scale, uint32_t key, uint64_t first, rg_ray *rays,

@@ -180,3 +180,14 @@ def test_reference_scene_files_match_single_lobe_restatement(ctx, name, spp, dep
     assert np.median(r) < 1e-5
     assert abs(out["sum"].mean() / want.mean() - 1) < 5e-3
     assert want.mean() > 1e-3
+
+
+def test_eight_wide_tree_renders_identically(ctx):
+    """The 8-wide tree visits the binary tree's leaves in the same order as
+    the 4-wide one, so the whole path tracer's output is bit-identical."""
+    sc = scenes.bsdf_frame(grid=3, stacks=16, slices=24)
+    st = rgapi.default_settings(width=48, height=32, spp=8, max_depth=6)
+    say("[pt] BSDF frame, 4-wide vs 8-wide BVH")
+    a = rgapi.RenderScene(ctx, sc, width=4).render(st)
+    b = rgapi.RenderScene(ctx, sc, width=8).render(st)
+    assert np.array_equal(a["sum"], b["sum"]) and a["rays"] == b["rays"]

@@ -146,3 +146,17 @@ def test_autotune_returns_a_working_setting(ctx, resident, single):
     outs, _ = rgapi.render_batch(ctx, resident, make_jobs()[:2], tune=t)
     for i in range(2):
         assert_same(outs[i], single[i], f"tuned job {i}")
+
+
+def test_batch_on_eight_wide_trees(ctx, single):
+    """The batched engine on scenes built with 8-wide nodes gives the same
+    images; scenes of different widths cannot share an engine."""
+    r8 = [rgapi.RenderScene(ctx, scenes.cube_room(), width=8),
+          rgapi.RenderScene(ctx, scenes.bsdf_frame(grid=3, stacks=16, slices=24), width=8)]
+    outs, _ = rgapi.render_batch(ctx, r8, make_jobs()[:3])
+    for i in range(3):
+        assert_same(outs[i], single[i], f"8-wide job {i}")
+    r4 = rgapi.RenderScene(ctx, scenes.cube_room(), width=4)
+    with pytest.raises(rgapi.RGError) as e:
+        rgapi.render_batch(ctx, [r4, r8[1]], make_jobs()[:1])
+    assert e.value.code == -6

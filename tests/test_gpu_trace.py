@@ -62,10 +62,10 @@ def random_rays(n, seed, box=1.5, tmax=np.inf):
     return r
 
 
-def compare(ctx, v9, rays, label):
-    say(f"[trace] {label}: tris={len(v9)} rays={len(rays)}")
+def compare(ctx, v9, rays, label, width=4):
+    say(f"[trace] {label}: tris={len(v9)} rays={len(rays)} width={width}")
     sc = rgapi.Scene(ctx, v9)
-    sc.build()
+    sc.build(width=width)
     nodes, perm = oracle.bvh_build(v9)
     want = oracle.trace_closest(nodes, perm, v9, rays, threads=8)
     got = sc.trace_closest_host(rays)
@@ -85,8 +85,9 @@ def compare(ctx, v9, rays, label):
     ("leafroot3", lambda: soup(3, 4), 2000),
     ("coincident9", lambda: np.repeat(soup(1, 5), 9, axis=0), 2000),
 ])
-def test_closest_and_any_bit_exact(ctx, label, maker, nrays):
-    compare(ctx, maker(), random_rays(nrays, hash(label) & 0xFFFF), label)
+@pytest.mark.parametrize("width", [4, 8])
+def test_closest_and_any_bit_exact(ctx, label, maker, nrays, width):
+    compare(ctx, maker(), random_rays(nrays, hash(label) & 0xFFFF), label, width)
 
 
 def test_cube_room_ties_bit_exact(ctx):
@@ -105,6 +106,7 @@ def test_cube_room_ties_bit_exact(ctx):
     _, tie = oracle.trace_closest(nodes, perm, v9, r, ties=True)
     say(f"[trace] cube room: exact-t ties flagged on {int(tie.sum())} rays")
     compare(ctx, v9, r, "cube_room")
+    compare(ctx, v9, r, "cube_room", width=8)
 
 
 def test_tmax_and_tmin_respected(ctx):
