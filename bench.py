@@ -187,7 +187,12 @@ def bench_batch(ctx, resident, n_jobs=64, seed=0xC3C30000):
     # warm both paths on a small job
     resident[0].render(rgapi.default_settings(width=64, height=64, spp=2), hdr_only=True)
     t0 = time.perf_counter()
-    probe = [rgapi.make_job(i, width=512, height=384, spp=8, max_depth=6) for i in range(len(resident))]
+    # probe jobs from the same distribution (another seed), capped at 64 spp so
+    # every candidate is timed on ~10^8 samples in well under a second
+    probe = []
+    for s, kw in draw_jobs(8, len(resident), seed ^ 0x9E3779B9):
+        kw["spp"] = min(kw["spp"], 64)
+        probe.append(rgapi.make_job(s, **kw))
     tune = rgapi.autotune(ctx, resident, probe)
     tune_s = time.perf_counter() - t0
     log("autotune", tune.as_dict(), f"{tune_s:.1f}s")
@@ -382,7 +387,7 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
                 "bytes_per_ray": round(b_ray, 2), "node_pops_per_ray": round(pops, 3),
-                "tri_tests_per_ray": round(tests, 3), "kernel": "k_closest",
+                "tri_tests_per_ray": round(tests, 3), "kernel": "k_trace_ps<64, 4, closest>",
                 "kernel_avg_ms": round(avg_ms, 4)}
 
     extra = {}

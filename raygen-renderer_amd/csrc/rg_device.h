@@ -323,6 +323,177 @@ __device__ __forceinline__ bool trace_any_dev(const DevBvh &bv, float ox, float 
     return false;
 }
 
+// One ray of a persistent traversal lane.
+struct PsRay {
+    float ox, oy, oz, dx, dy, dz, tmin, tmax;
+    uint32_t pred_bit;
+};
+
+constexpr uint32_t kNoNode = 0xFFFFFFFEu;
+
+// Persistent traversal with leaf postponement and lane refill (after Aila
+// & Laine, HPG 2009), shared by every kernel that traces a batch of rays.
+// Each wave owns a contiguous chunk of the n ray indices; a lane whose ray is
+// done takes the next index of the chunk, so a wave does not idle behind its
+// longest ray.  Inner nodes are traversed until every lane of the wave holds
+// a leaf to intersect (the first leaf a lane reaches is postponed and the
+// lane keeps descending with its current t), then all lanes intersect their
+// leaves together.  Descending with a stale, larger t only visits nodes the
+// plain traversal culls (their triangles cannot pass the strict
+// t < closest test), and leaves are intersected in the same order, so the
+// hits are those of trace_closest_dev, exact-t ties included.
+//
+// Src provides: bool load(i, bv, PsRay &) (false: nothing to trace at i; it
+// may replace bv with the ray's own tree), store_closest(i, prim, t, u, v),
+// store_any(i, occluded).  kLaneBvh: the tree differs per ray.
+template <int kStack, int kW, bool kAny, int kStride, class Src, bool kLaneBvh = false>
+__device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src, uint64_t n,
+                                              const LdsColumn<kStride> lc) {
+    constexpr int kSpill = kStack > kLdsStack ? kStack - kLdsStack : 1;
+    constexpr int kOrdBits = WideTest<kW>::kOrdBits;
+    uint32_t stk_ref[kSpill];
+    float stk_t[kSpill];
+    const uint32_t lane = __lane_id();
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t chunk = (n + waves - 1) / waves;
+    uint64_t cursor = wave * chunk;
+    const uint64_t chunk_end = cursor + chunk < n ? cursor + chunk : n;
+    DevBvh bv = bv0;
+    bool active = false;
+    uint64_t ri = 0;
+    PsRay r{};
+    float ix = 0, iy = 0, iz = 0, t = 0, bu = 0, bvv = 0;
+    int oct = 0, sp = 0;
+    uint32_t prim = 0xFFFFFFFFu, cur = kNoNode, leaf = kNoNode;
+    bool hit_any = false;
+
+    auto pop = [&]() -> uint32_t {
+        while (sp > 0) {
+            sp--;
+            const bool in_lds = sp < kLdsStack;
+            if (kAny) return in_lds ? lc.ref[sp * kStride] : stk_ref[sp - kLdsStack];
+            const float et = in_lds ? lc.t[sp * kStride] : stk_t[sp - kLdsStack];
+            if (!(et > t)) return in_lds ? lc.ref[sp * kStride] : stk_ref[sp - kLdsStack];
+        }
+        return kNoNode;
+    };
+    auto finish = [&]() {
+        if (kAny) src.store_any(ri, hit_any);
+        else src.store_closest(ri, prim, t, bu, bvv);
+        active = false;
+    };
+
+    for (;;) {
+        // refill idle lanes from the wave's chunk
+        const uint64_t idle = __ballot(!active);
+        if (idle && cursor < chunk_end) {
+            const uint64_t idx = cursor + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
+            if (!active && idx < chunk_end) {
+                ri = idx;
+                if (kLaneBvh) bv = bv0;
+                if (src.load(idx, bv, r)) {
+                    t = r.tmax;
+                    ix = inv_dir(r.dx); iy = inv_dir(r.dy); iz = inv_dir(r.dz);
+                    oct = (ix < 0.0f ? 1 : 0) | (iy < 0.0f ? 2 : 0) | (iz < 0.0f ? 4 : 0);
+                    prim = 0xFFFFFFFFu; bu = 0.0f; bvv = 0.0f; hit_any = false;
+                    sp = 0; leaf = kNoNode;
+                    float tm;
+                    active = true;
+                    if (bv.has_tree && slab(bv.root.lo[0], bv.root.lo[1], bv.root.lo[2], bv.root.hi[0],
+                                            bv.root.hi[1], bv.root.hi[2], r.ox, r.oy, r.oz, ix, iy, iz, t, tm))
+                        cur = 0;
+                    else
+                        finish();
+                }
+            }
+            cursor += (uint64_t)__popcll(idle);
+        }
+        if (__ballot(active) == 0) {
+            if (cursor >= chunk_end) break;
+            continue;
+        }
+        // inner phase
+        for (;;) {
+            if (active && leaf == kNoNode && cur != kNoNode && (cur & kLeafBit)) {
+                leaf = cur;
+                cur = pop();
+            }
+            const bool want = active && cur != kNoNode && !(cur & kLeafBit);
+            const bool ready = !active || leaf != kNoNode || cur == kNoNode;
+            if (!__any(want) || __all(ready)) break;
+            if (want) {
+                bool ok[kW];
+                float tm[kW];
+                uint32_t cref[kW], ord;
+                WideTest<kW>::run(bv.nodes, cur, oct, r.ox, r.oy, r.oz, ix, iy, iz, t, ok, tm, cref, ord);
+                bool have = false;
+                uint32_t next = 0;
+                float next_t = 0.0f;
+#pragma unroll
+                for (int j = kW - 1; j >= 0; j--) {
+                    const int s = (ord >> (kOrdBits * j)) & (kW - 1);
+                    if (!ok[s]) continue;
+                    if (have) {
+                        if (sp < kLdsStack) {
+                            lc.ref[sp * kStride] = next;
+                            if (!kAny) lc.t[sp * kStride] = next_t;
+                        } else {
+                            stk_ref[sp - kLdsStack] = next;
+                            if (!kAny) stk_t[sp - kLdsStack] = next_t;
+                        }
+                        sp++;
+                    }
+                    next = cref[s];
+                    next_t = tm[s];
+                    have = true;
+                }
+                cur = have ? next : pop();
+            }
+        }
+        // leaf phase
+        if (active && leaf != kNoNode) {
+            uint32_t k = leaf & ~kLeafBit;
+            for (;;) {
+                const float4 a = bv.tris[3 * (size_t)k];
+                const float4 b = bv.tris[3 * (size_t)k + 1];
+                const float4 c = bv.tris[3 * (size_t)k + 2];
+                if (kAny) {
+                    if ((__float_as_uint(c.w) & r.pred_bit) &&
+                        mt_any(a, b, c, r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, t)) {
+                        hit_any = true;
+                        break;
+                    }
+                } else if (mt_closest(a, b, c, r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, r.tmin, t, bu, bvv)) {
+                    prim = __float_as_uint(a.w);
+                }
+                if (__float_as_uint(b.w) & 1u) break;
+                k++;
+            }
+            leaf = kNoNode;
+            if (kAny && hit_any) cur = kNoNode;
+        }
+        if (active && leaf == kNoNode && cur == kNoNode) finish();
+    }
+}
+
+// Blocks of `kernel` the whole device holds at once (occupancy x compute
+// units), looked up once per kernel: the grid of a persistent launch.
+template <auto kernel>
+uint32_t resident_blocks(int block) {
+    static int cached = 0;  // one per kernel instantiation
+    if (!cached) {
+        int dev = 0, n_cu = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu <= 0)
+            n_cu = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu <= 0)
+            per_cu = 1;
+        cached = n_cu * per_cu;
+    }
+    return (uint32_t)cached;
+}
+
 // Host-side dispatch to the traversal kernels instantiated for a stack size
 // (64 or 256 entries, the smallest that covers the tree's stack need) and a
 // node width (4 or 8): calls f(TravTag<stack, width>{}).

@@ -23,6 +23,7 @@
 // the last-ulp differences of transcendental functions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "rg_pt_dev.h"
@@ -75,19 +76,28 @@ __global__ __launch_bounds__(kPtBlock) void k_camera(PTScene sc, PTState st, uin
         atomicAdd(st.stats + 2, (unsigned long long)__popcll(m));
 }
 
+// Closest hit of every live path's ray (queue qi), persistent waves.
+struct ExtendSrc {
+    const rg_ray *ray;
+    const uint32_t *q;
+    rg_hit *hit;
+    __device__ __forceinline__ bool load(uint64_t i, const DevBvh &, PsRay &r) const {
+        const rg_ray x = ray[q[i]];
+        r = {x.o[0], x.o[1], x.o[2], x.d[0], x.d[1], x.d[2], x.tmin, x.tmax, 0u};
+        return true;
+    }
+    __device__ __forceinline__ void store_closest(uint64_t i, uint32_t prim, float t, float u, float v) const {
+        hit[q[i]] = {prim, t, u, v};
+    }
+    __device__ __forceinline__ void store_any(uint64_t, bool) const {}
+};
+
 template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_extend(DevBvh bv, PTState st, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
-    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
     const uint32_t count = st.count[qi];
-    if (i == 0 && count) atomicAdd(st.stats + 0, (unsigned long long)count);
-    if (i >= count) return;
-    const uint32_t p = st.q[qi][i];
-    const rg_ray r = st.ray[p];
-    float t = r.tmax, bu = 0.0f, bvv = 0.0f;
-    uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack, kPtBlock, kW>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmin, t, prim, bu, bvv, lc);
-    st.hit[p] = {prim, t, bu, bvv};
+    if (blockIdx.x == 0 && threadIdx.x == 0 && count) atomicAdd(st.stats + 0, (unsigned long long)count);
+    trace_ps_loop<kStack, kW, false, kPtBlock>(bv, ExtendSrc{st.ray, st.q[qi], st.hit}, count, lc);
 }
 
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(PTScene sc, PTState st,
@@ -140,19 +150,26 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
 }
 
 // Any-hit test of every queued shadow ray (TriangleBVH::intersectAny,
-// bvh.cpp:287-339, with the ray's occluder predicate).
+// bvh.cpp:287-339, with the ray's occluder predicate), persistent waves.
+struct ShadowSrc {
+    const rg_ray *sray;
+    const uint32_t *ssrc;
+    uint8_t *socc;
+    __device__ __forceinline__ bool load(uint64_t i, const DevBvh &, PsRay &r) const {
+        const rg_ray x = sray[ssrc[i]];
+        r = {x.o[0], x.o[1], x.o[2], x.d[0], x.d[1], x.d[2], 0.0f, x.tmax, 1u << (int)x.tmin};
+        return true;
+    }
+    __device__ __forceinline__ void store_closest(uint64_t, uint32_t, float, float, float) const {}
+    __device__ __forceinline__ void store_any(uint64_t i, bool o) const { socc[ssrc[i]] = o ? 1 : 0; }
+};
+
 template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_anyhit(DevBvh bv, PTState st) {
     RG_LDS_COLUMN(lc, kPtBlock);
-    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
     const uint32_t n = st.count[2];
-    if (i == 0 && n) atomicAdd(st.stats + 1, (unsigned long long)n);
-    if (i >= n) return;
-    const uint32_t src = st.ssrc[i];
-    const rg_ray r = st.sray[src];
-    const uint32_t pred_bit = 1u << (int)r.tmin;
-    st.socc[src] =
-        trace_any_dev<kStack, kPtBlock, kW>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmax, pred_bit, lc) ? 1 : 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(st.stats + 1, (unsigned long long)n);
+    trace_ps_loop<kStack, kW, true, kPtBlock>(bv, ShadowSrc{st.sray, st.ssrc, st.socc}, n, lc);
 }
 
 // Adds the unoccluded NEE contributions in event order, closes finished
@@ -335,16 +352,20 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
     const uint32_t cap = npix * S;
     const uint32_t blocks = (cap + kPtBlock - 1) / kPtBlock;
     const DevBvh bv = dev_bvh(scn);
+    // persistent traversal grids: what the device holds at once, at most one
+    // thread per possible ray
+    const uint32_t ext_blocks = std::min<uint32_t>(blocks, resident_blocks<k_extend<kStack, kW>>(kPtBlock));
+    const uint32_t any_blocks = std::min<uint32_t>(2 * blocks, resident_blocks<k_anyhit<kStack, kW>>(kPtBlock));
     st.sample_base = sample_base;
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, st.count + 0, d_tile_mask ? 0u : cap);
     hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, S, sample_base, d_tile_mask);
     int qi = 0;
     const int bounces = sc.max_depth + 2;
     for (int b = 0; b < bounces; b++) {
-        hipLaunchKernelGGL((k_extend<kStack, kW>), dim3(blocks), dim3(kPtBlock), 0, stream, bv, st, qi);
+        hipLaunchKernelGGL((k_extend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, stream, bv, st, qi);
         hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, stream, st.count + 2, st.count + (qi ^ 1));
         hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, qi);
-        hipLaunchKernelGGL((k_anyhit<kStack, kW>), dim3(2 * blocks), dim3(kPtBlock), 0, stream, bv, st);
+        hipLaunchKernelGGL((k_anyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, stream, bv, st);
         hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(kPtBlock), 0, stream, st, qi, clamp_max);
         qi ^= 1;
     }

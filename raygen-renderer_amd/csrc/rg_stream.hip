@@ -5,7 +5,8 @@
 // kernels touch element i in thread i.
 //   regen   puts the next samples of the issue sequence (camera rays,
 //           rayrenderer.cpp:860-934) into positions freed last iteration
-//   extend  closest-hit traversal of every live position
+//   extend  closest-hit traversal of every live position (persistent waves,
+//           leaf postponement and lane refill; each ray on its job's tree)
 //   shade   one path event per live position (the same step() as the
 //           single-job path, with the job's scene record loaded
 //           wave-uniform); the event's 0-2 NEE rays are queued with
@@ -178,19 +179,32 @@ __global__ __launch_bounds__(kPtBlock) void k_sregen(StreamPool sp, const DevJob
     }
 }
 
+// Closest hit of every live position; each ray traverses its own job's tree.
+struct PoolExtendSrc {
+    const int4 *sid;
+    const rg_ray *ray;
+    rg_hit *hit;
+    const DevJobSlot *jobs;
+    __device__ __forceinline__ bool load(uint64_t i, DevBvh &bv, PsRay &r) const {
+        const int job = sid[i].x;
+        if (job < 0) return false;
+        bv = jobs[job].j.bv;
+        const rg_ray x = ray[i];
+        r = {x.o[0], x.o[1], x.o[2], x.d[0], x.d[1], x.d[2], x.tmin, x.tmax, 0u};
+        return true;
+    }
+    __device__ __forceinline__ void store_closest(uint64_t i, uint32_t prim, float t, float u, float v) const {
+        hit[i] = {prim, t, u, v};
+    }
+    __device__ __forceinline__ void store_any(uint64_t, bool) const {}
+};
+
 template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_sextend(StreamPool sp, const DevJobSlot *__restrict__ jobs) {
     RG_LDS_COLUMN(lc, kPtBlock);
-    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    if (i >= sp.cap) return;
-    const int job = sp.st.sid[i].x;
-    if (job < 0) return;
-    const DevBvh bv = jobs[job].j.bv;
-    const rg_ray r = sp.st.ray[i];
-    float t = r.tmax, bu = 0.0f, bvv = 0.0f;
-    uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack, kPtBlock, kW>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmin, t, prim, bu, bvv, lc);
-    sp.hit[i] = {prim, t, bu, bvv};
+    const DevBvh none{};
+    trace_ps_loop<kStack, kW, false, kPtBlock, PoolExtendSrc, true>(none, PoolExtendSrc{sp.st.sid, sp.st.ray, sp.hit, jobs},
+                                                                  sp.cap, lc);
 }
 
 // One path event per live position, state updated in place; the event's
@@ -246,31 +260,33 @@ __global__ __launch_bounds__(kPtBlock) void k_sshade(StreamPool sp, const DevJob
     }
 }
 
-// Any-hit test of every queued NEE ray, one ray per lane, persistent waves
-// pulling 64 rays at a time.
+// Any-hit test of every queued NEE ray, persistent waves; each ray traverses
+// its own job's tree.
+struct PoolShadowSrc {
+    const uint32_t *ssrc;
+    const int4 *sid;
+    const rg_ray *sray;
+    uint8_t *socc;
+    const DevJobSlot *jobs;
+    __device__ __forceinline__ bool load(uint64_t i, DevBvh &bv, PsRay &r) const {
+        const uint32_t src = ssrc[i];
+        bv = jobs[sid[src >> 1].x].j.bv;
+        const rg_ray x = sray[src];
+        r = {x.o[0], x.o[1], x.o[2], x.d[0], x.d[1], x.d[2], 0.0f, x.tmax, 1u << (int)x.tmin};
+        return true;
+    }
+    __device__ __forceinline__ void store_closest(uint64_t, uint32_t, float, float, float) const {}
+    __device__ __forceinline__ void store_any(uint64_t i, bool o) const { socc[ssrc[i]] = o ? 1 : 0; }
+};
+
 template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_sanyhit(StreamPool sp, const DevJobSlot *__restrict__ jobs, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t n = sp.ctr->nsh[qi];
-    uint32_t *fetch = &sp.ctr->fetch[qi];
-    const uint32_t lane = __lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&sp.ctr->stats[1], (unsigned long long)n);
-    for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(fetch, 64u);
-        base = __shfl(base, 0);
-        if (base >= n) break;
-        const uint32_t i = base + lane;
-        if (i < n) {
-            const uint32_t src = sp.ssrc[i];
-            const uint32_t job = (uint32_t)sp.st.sid[src >> 1].x;
-            const DevBvh bv = jobs[job].j.bv;
-            const rg_ray r = sp.sray[src];
-            const uint32_t pred_bit = 1u << (int)r.tmin;
-            sp.socc[src] =
-                trace_any_dev<kStack, kPtBlock, kW>(bv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], r.tmax, pred_bit, lc) ? 1 : 0;
-        }
-    }
+    const DevBvh none{};
+    trace_ps_loop<kStack, kW, true, kPtBlock, PoolShadowSrc, true>(
+        none, PoolShadowSrc{sp.ssrc, sp.st.sid, sp.sray, sp.socc, jobs}, n, lc);
 }
 
 // Adds the unoccluded NEE contributions in event order.  A path that ended
@@ -391,11 +407,18 @@ static void iteration_impl(const StreamPool &sp, const DevJobSlot *jobs, int qi,
                            hipStream_t s) {
     const uint32_t flat = (sp.cap + kPtBlock - 1) / kPtBlock;
     const uint32_t regen_blocks = std::min<uint32_t>(flat, (uint32_t)n_cu * 4);
-    const uint32_t persistent = (uint32_t)n_cu * tune.blocks_per_cu;
+    // persistent traversal grids: what the device holds at once, or fewer
+    // when blocks_per_cu caps it
+    uint32_t ext_blocks = std::min<uint32_t>(flat, resident_blocks<k_sextend<kStack, kW>>(kPtBlock));
+    uint32_t any_blocks = std::min<uint32_t>(2 * flat, resident_blocks<k_sanyhit<kStack, kW>>(kPtBlock));
+    if (tune.blocks_per_cu) {
+        ext_blocks = std::min<uint32_t>(ext_blocks, (uint32_t)n_cu * tune.blocks_per_cu);
+        any_blocks = std::min<uint32_t>(any_blocks, (uint32_t)n_cu * tune.blocks_per_cu);
+    }
     hipLaunchKernelGGL(k_sregen, dim3(regen_blocks), dim3(kPtBlock), 0, s, sp, jobs, qi);
-    hipLaunchKernelGGL((k_sextend<kStack, kW>), dim3(flat), dim3(kPtBlock), 0, s, sp, jobs);
+    hipLaunchKernelGGL((k_sextend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, s, sp, jobs);
     hipLaunchKernelGGL(k_sshade, dim3(flat), dim3(kPtBlock), 0, s, sp, jobs, qi);
-    hipLaunchKernelGGL((k_sanyhit<kStack, kW>), dim3(persistent), dim3(kPtBlock), 0, s, sp, jobs, qi);
+    hipLaunchKernelGGL((k_sanyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, s, sp, jobs, qi);
     hipLaunchKernelGGL(k_sresolve, dim3(flat), dim3(kPtBlock), 0, s, sp, jobs, qi);
 }
 

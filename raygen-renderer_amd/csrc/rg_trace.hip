@@ -8,6 +8,8 @@
 // restatement.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstdint>
 
 #include "rg_device.h"
@@ -77,30 +79,30 @@ int dev_sync(void *stream) {
 
 constexpr int kBlock = 128;
 
-// One thread per ray.
-template <int kStack, int kW>
-__global__ __launch_bounds__(kBlock) void k_closest(DevBvh bv, const rg_ray *__restrict__ rays,
-                                                    rg_hit *__restrict__ hits, uint64_t n) {
-    RG_LDS_COLUMN(lc, kBlock);
-    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const float4 r0 = reinterpret_cast<const float4 *>(rays)[2 * i];
-    const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
-    float t = r1.w, bu = 0.0f, bvv = 0.0f;
-    uint32_t prim = 0xFFFFFFFFu;
-    trace_closest_dev<kStack, kBlock, kW>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w, t, prim, bu, bvv, lc);
-    reinterpret_cast<float4 *>(hits)[i] = make_float4(__uint_as_float(prim), t, bu, bvv);
-}
+// Closest / any hit of a ray array (rg_trace_closest / rg_trace_any).
+struct RayArraySrc {
+    const rg_ray *rays;
+    rg_hit *hits;
+    uint8_t *occ;
+    uint32_t pred_bit;
+    __device__ __forceinline__ bool load(uint64_t i, const DevBvh &, PsRay &r) const {
+        const float4 r0 = reinterpret_cast<const float4 *>(rays)[2 * i];
+        const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
+        r.ox = r0.x; r.oy = r0.y; r.oz = r0.z; r.tmin = r0.w;
+        r.dx = r1.x; r.dy = r1.y; r.dz = r1.z; r.tmax = r1.w;
+        r.pred_bit = pred_bit;
+        return true;
+    }
+    __device__ __forceinline__ void store_closest(uint64_t i, uint32_t prim, float t, float u, float v) const {
+        reinterpret_cast<float4 *>(hits)[i] = make_float4(__uint_as_float(prim), t, u, v);
+    }
+    __device__ __forceinline__ void store_any(uint64_t i, bool o) const { occ[i] = o ? 1 : 0; }
+};
 
-template <int kStack, int kW>
-__global__ __launch_bounds__(kBlock) void k_any(DevBvh bv, const rg_ray *__restrict__ rays,
-                                                uint8_t *__restrict__ occ, uint64_t n, uint32_t pred_bit) {
+template <int kStack, int kW, bool kAny>
+__global__ __launch_bounds__(kBlock) void k_trace_ps(DevBvh bv, RayArraySrc src, uint64_t n) {
     RG_LDS_COLUMN(lc, kBlock);
-    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const float4 r0 = reinterpret_cast<const float4 *>(rays)[2 * i];
-    const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
-    occ[i] = trace_any_dev<kStack, kBlock, kW>(bv, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r1.w, pred_bit, lc) ? 1 : 0;
+    trace_ps_loop<kStack, kW, kAny, kBlock>(bv, src, n, lc);
 }
 
 __global__ __launch_bounds__(256) void k_gen_rays(float cx, float cy, float cz, float ex, float ey,
@@ -143,29 +145,36 @@ DevBvh dev_bvh(const Scene &s) {
     return b;
 }
 
-int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n, void *stream) {
-    if (n == 0) return RG_OK;
-    const uint64_t blocks = (n + kBlock - 1) / kBlock;
+// Persistent launch: as many blocks as are resident at once, each wave
+// taking an equal chunk of the rays.
+template <int kStack, int kW, bool kAny>
+static int launch_ps(const DevBvh &bv, const rg_ray *rays, rg_hit *hits, uint8_t *occ, uint64_t n, uint32_t bit,
+                     hipStream_t stream) {
+    const uint32_t resident = resident_blocks<k_trace_ps<kStack, kW, kAny>>(kBlock);
+    const uint64_t blocks = std::min<uint64_t>((uint64_t)resident, (n + kBlock - 1) / kBlock);
+    const RayArraySrc src{rays, hits, occ, bit};
+    hipLaunchKernelGGL((k_trace_ps<kStack, kW, kAny>), dim3((uint32_t)blocks), dim3(kBlock), 0, stream, bv, src, n);
+    return hip_check(hipGetLastError(), "traversal launch");
+}
+
+static int trace_ps(const Scene &s, const rg_ray *rays, rg_hit *hits, uint8_t *occ, uint64_t n, uint32_t bit,
+                    void *stream) {
     const DevBvh bv = dev_bvh(s);
     return with_traversal(s.stack_need, s.width, [&](auto tag) {
         using T = decltype(tag);
-        hipLaunchKernelGGL((k_closest<T::kStack, T::kW>), dim3((uint32_t)blocks), dim3(kBlock), 0,
-                           (hipStream_t)stream, bv, rays, hits, n);
-        return hip_check(hipGetLastError(), "k_closest launch");
+        if (occ) return launch_ps<T::kStack, T::kW, true>(bv, rays, hits, occ, n, bit, (hipStream_t)stream);
+        return launch_ps<T::kStack, T::kW, false>(bv, rays, hits, occ, n, bit, (hipStream_t)stream);
     });
+}
+
+int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n, void *stream) {
+    if (n == 0) return RG_OK;
+    return trace_ps(s, rays, hits, nullptr, n, 0, stream);
 }
 
 int launch_any(const Scene &s, const rg_ray *rays, uint8_t *occ, uint64_t n, int pred, void *stream) {
     if (n == 0) return RG_OK;
-    const uint64_t blocks = (n + kBlock - 1) / kBlock;
-    const DevBvh bv = dev_bvh(s);
-    const uint32_t bit = 1u << pred;
-    return with_traversal(s.stack_need, s.width, [&](auto tag) {
-        using T = decltype(tag);
-        hipLaunchKernelGGL((k_any<T::kStack, T::kW>), dim3((uint32_t)blocks), dim3(kBlock), 0,
-                           (hipStream_t)stream, bv, rays, occ, n, bit);
-        return hip_check(hipGetLastError(), "k_any launch");
-    });
+    return trace_ps(s, rays, nullptr, occ, n, 1u << pred, stream);
 }
 
 int launch_gen_rays(const float box[6], float scale, uint32_t key, uint64_t first, rg_ray *rays,
