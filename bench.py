@@ -246,16 +246,23 @@ def bench_serve(ctx, resident, n_req, rates, seed=0xC4C40000, budget=32 << 30, n
     plain_sps = sum(sam[:len(plain_hdr)]) / plain_s
     lam_plain = plain_sps / float(np.mean(sam))
     log(f"plain {plain_sps / 1e6:.1f} Msamples/s -> {lam_plain:.2f} req/s")
+    probe = []
+    for s, kw in draw_jobs(8, len(resident), seed ^ 0x9E3779B9):
+        kw["spp"] = min(kw["spp"], 64)
+        probe.append(rgapi.make_job(s, **kw))
+    t0 = time.perf_counter()
+    tune = rgapi.autotune(ctx, resident, probe)
+    log("autotune", tune.as_dict(), f"{time.perf_counter() - t0:.1f}s")
     res = {"workload": f"{n_req} requests per rate, drawn like the batch jobs over 4 resident scenes "
                        f"(cube room, two BSDF frames, GGX icosphere 1,310,736 tris + medium box), "
                        f"Poisson arrivals, HBM budget {budget >> 30} GiB",
            "plain_samples_per_s": round(plain_sps, 1), "plain_requests": len(plain_hdr),
-           "rates": []}
+           "tune": tune.as_dict(), "rates": []}
     rng = np.random.default_rng(seed ^ 0x5A5A)
     for f in rates:
         lam = f * lam_plain
         gaps = rng.exponential(1.0 / lam, n_req)
-        srv = rgapi.Server(ctx, resident, hbm_budget=budget)
+        srv = rgapi.Server(ctx, resident, hbm_budget=budget, tune=tune)
         tickets = []
         t_start = time.perf_counter()
         t_next = t_start
@@ -379,13 +386,14 @@ def main():
     if pops is not None:
         b_ray = 32 + 16 + 32 * pops + 36 * tests
         achieved = b_ray * n / (avg_ms / 1e3) / 1e9
-        traffic = None
+        traffic = l2 = None
         tf = os.path.join(ROOT, "profiles", "c1_closest_traffic.json")
         if os.path.exists(tf):
             with open(tf) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+                tj = json.load(f)
+            traffic, l2 = tj.get("hbm_bytes_per_launch"), tj.get("l2_hit_rate")
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
+                "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic, "l2_hit_rate": l2,
                 "bytes_per_ray": round(b_ray, 2), "node_pops_per_ray": round(pops, 3),
                 "tri_tests_per_ray": round(tests, 3), "kernel": "k_trace_ps<64, 4, closest>",
                 "kernel_avg_ms": round(avg_ms, 4)}
