@@ -171,6 +171,7 @@ struct WideTest<8> {
 #define RG_LDS_STACK 8
 #endif
 constexpr int kLdsStack = RG_LDS_STACK;
+
 constexpr int kLdsWords = kLdsStack > 0 ? kLdsStack : 1;
 
 template <int kStride>
