@@ -266,7 +266,8 @@ typedef struct {
     uint32_t stack;            /* traversal stack entries 32/64/128/256
                                   (default: smallest that fits every BVH) */
     uint32_t tile;             /* pixel tile edge of the issue order (default 8) */
-    uint32_t blocks_per_cu;    /* persistent any-hit blocks per compute unit (default 8) */
+    uint32_t blocks_per_cu;    /* cap on persistent traversal blocks per compute unit
+                                  (default 8) */
     uint32_t iters_per_graph;  /* iterations per hipGraph replay, even (default 4) */
     uint32_t pad;
 } rg_tune;

@@ -28,7 +28,7 @@ struct StreamTune {
     uint32_t pool;           // path slots in flight (queue size)
     uint32_t stack;          // traversal stack entries: 32, 64, 128 or 256
     uint32_t tile;           // pixel tile edge of the issue order (1 = scanline)
-    uint32_t blocks_per_cu;  // persistent any-hit blocks per compute unit
+    uint32_t blocks_per_cu;  // cap on persistent traversal blocks per compute unit
     uint32_t iters;          // iterations per captured graph (even)
     uint32_t width;          // node width of every resident scene's BVH (4 or 8)
 };
