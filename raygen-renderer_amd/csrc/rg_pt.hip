@@ -135,10 +135,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         st.tp[p] = v4(P.tp, 0.0f);
         st.L[p] = v4(P.L, P.alive ? 1.0f : 0.0f);
         st.meta[p] = make_int4(P.passes, P.medium, R.dim, (int)R.k);
-        for (int k = 0; k < S.n; k++) {
-            st.sray[2 * p + k] = {{S.o[k].x, S.o[k].y, S.o[k].z}, S.o[k].w, {S.d[k].x, S.d[k].y, S.d[k].z}, S.d[k].w};
-            st.scontrib[2 * p + k] = S.c[k];
-        }
+        store_shadows(S, p, st.sray, st.scontrib);
         ns = (uint32_t)S.n;
         st.nshadow[p] = ns;
     }

@@ -72,7 +72,9 @@ __device__ __forceinline__ float prng(Rng &r) {
         r.blk[3] = c[3];
         r.blk_id = b;
     }
-    const uint32_t w = r.blk[r.k & 3];
+    // constant indices only, so the block stays in registers
+    const uint32_t j = r.k & 3;
+    const uint32_t w = j == 0 ? r.blk[0] : (j == 1 ? r.blk[1] : (j == 2 ? r.blk[2] : r.blk[3]));
     r.k++;
     return (float)(w >> 8) * (1.0f / 16777216.0f);
 }
@@ -102,11 +104,32 @@ __device__ __forceinline__ float radical_inverse(uint32_t i, uint32_t base) {
     return v < 1.0f ? v : 0.9999999f;
 }
 
+// The first 16 primes, the Halton bases of dimensions 0-15, selected without
+// a private-memory table.
+__device__ __forceinline__ uint32_t halton_base(int dim) {
+    uint32_t b = 53u;
+    b = dim == 14 ? 47u : b;
+    b = dim == 13 ? 43u : b;
+    b = dim == 12 ? 41u : b;
+    b = dim == 11 ? 37u : b;
+    b = dim == 10 ? 31u : b;
+    b = dim == 9 ? 29u : b;
+    b = dim == 8 ? 23u : b;
+    b = dim == 7 ? 19u : b;
+    b = dim == 6 ? 17u : b;
+    b = dim == 5 ? 13u : b;
+    b = dim == 4 ? 11u : b;
+    b = dim == 3 ? 7u : b;
+    b = dim == 2 ? 5u : b;
+    b = dim == 1 ? 3u : b;
+    b = dim == 0 ? 2u : b;
+    return b;
+}
+
 __device__ __forceinline__ float lds1(Rng &r) {
     if (r.dim >= 16) return prng(r);
-    const uint32_t primes[16] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53};
     const uint32_t scramble = hash_pixel(r.x, r.y, 0);
-    const float v = radical_inverse((uint32_t)r.sample + 1u, primes[r.dim]);
+    const float v = radical_inverse((uint32_t)r.sample + 1u, halton_base(r.dim));
     const uint32_t h = hash_pixel(r.dim, 0, scramble);
     const float offset = (float)(h >> 8) * (1.0f / (float)(1u << 24));
     float w = v + offset;
@@ -180,7 +203,7 @@ __device__ __forceinline__ v3 env_texel(const PTScene &sc, int x, int y) {
 }
 __device__ __forceinline__ float lum(v3 c) { return fmaxf(0.0f, 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z); }
 
-__device__ v3 sample_environment(const PTScene &sc, v3 dir) {
+__device__ __forceinline__ v3 sample_environment(const PTScene &sc, v3 dir) {
     if (!sc.has_env) return mk(0, 0, 0);
     const v3 d = nrm(dir);
     const float cy = sc.env_cos, sy = sc.env_sin;
@@ -210,7 +233,7 @@ __device__ __forceinline__ int search_cdf(const float *cdf, int n, float u) {
     return lo < n ? lo : n - 1;
 }
 
-__device__ void sample_env_dir(const PTScene &sc, float u0, float u1, v3 &out, float &pdf) {
+__device__ __forceinline__ void sample_env_dir(const PTScene &sc, float u0, float u1, v3 &out, float &pdf) {
     out = mk(0, 1, 0);
     pdf = 0.0f;
     const int W = (int)sc.env_w, H = (int)sc.env_h;
@@ -242,7 +265,7 @@ __device__ void sample_env_dir(const PTScene &sc, float u0, float u1, v3 &out, f
     pdf = pdf_uv * (float)(W * H) / (2.0f * (float)(M_PI * M_PI) * sin_t);
 }
 
-__device__ float env_dir_pdf(const PTScene &sc, v3 dir) {
+__device__ __forceinline__ float env_dir_pdf(const PTScene &sc, v3 dir) {
     if (!sc.has_env || sc.env_total <= 0.0f) return 0.0f;
     const v3 d = nrm(dir);
     const float cy = sc.env_cos, sy = sc.env_sin;
@@ -300,7 +323,7 @@ __device__ __forceinline__ float phase_pdf(const DevMedium &m, v3 wo, v3 wi) {
     const float den = 1.0f + g * g - 2.0f * g * c;
     return (1.0f - g * g) * (float)(1.0 / (4.0 * M_PI)) / (den * sqrtf(fmaxf(1e-8f, den)));
 }
-__device__ v3 sample_phase(const DevMedium &m, v3 wo, float u1, float u2) {
+__device__ __forceinline__ v3 sample_phase(const DevMedium &m, v3 wo, float u1, float u2) {
     float ct;
     if (fabsf(m.g) < 1e-3f) ct = 1.0f - 2.0f * u1;
     else {
@@ -326,7 +349,7 @@ __device__ v3 sample_phase(const DevMedium &m, v3 wo, float u1, float u2) {
 // sampling helpers
 // ---------------------------------------------------------------------------
 // cosineWeightedDirection (raycommon.cpp:94-115)
-__device__ v3 cosine_dir(v3 n, float r1, float r2) {
+__device__ __forceinline__ v3 cosine_dir(v3 n, float r1, float r2) {
     const float theta = acosf(sqrtf(1.0f - r1));
     const float phi = 2.0f * PI_F * r2;
     const float x = sinf(theta) * cosf(phi);
@@ -350,7 +373,7 @@ __device__ __forceinline__ v3 point_in_tri(const PTScene &sc, uint32_t t, float 
 }
 
 // randomRayInHemisphere (ugm, not vendored): uniform hemisphere about n
-__device__ v3 hemisphere_dir(v3 n, float u1, float u2) {
+__device__ __forceinline__ v3 hemisphere_dir(v3 n, float u1, float u2) {
     const float z = u1;
     const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
     const float phi = 2.0f * PI_F * u2;
@@ -369,7 +392,7 @@ __device__ __forceinline__ float fresnel_schlick(float c, float ior) {
 }
 
 // refract (bsdf.cpp:39-52)
-__device__ v3 refract_dir(v3 d, v3 n, float r) {
+__device__ __forceinline__ v3 refract_dir(v3 d, v3 n, float r) {
     const v3 nl = dot(d, n) < 0.0f ? n : neg(n);
     const bool into = dot(nl, n) > 0.0f;
     if (into) r = 1.0f / r;
@@ -395,7 +418,7 @@ __device__ __forceinline__ float smith_g1(v3 w, float ax, float ay) {
     return 1.0f / (1.0f + lambda);
 }
 // Heitz 2018 VNDF sampling (bsdf.cpp:139-161)
-__device__ v3 sample_vndf(v3 Ve, float ax, float ay, float u1, float u2) {
+__device__ __forceinline__ v3 sample_vndf(v3 Ve, float ax, float ay, float u1, float u2) {
     const v3 Vh = nrm(mk(ax * Ve.x, ay * Ve.y, Ve.z));
     const float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
     const v3 T1 = (lensq > 0.0f) ? scl(mk(-Vh.y, Vh.x, 0.0f), 1.0f / sqrtf(lensq)) : mk(1, 0, 0);
@@ -420,7 +443,7 @@ __device__ __forceinline__ float ggx_d(v3 H, float ax, float ay) {
     return 1.0f / (PI_F * ax * ay * den * den);
 }
 // evalGlossy (bsdf.cpp:258-280)
-__device__ bool eval_glossy(const Ggx &G, v3 dw, v3 &fcos, float &pdf) {
+__device__ __forceinline__ bool eval_glossy(const Ggx &G, v3 dw, v3 &fcos, float &pdf) {
     const v3 Ll = mk(dot(dw, G.t), dot(dw, G.b), dot(dw, G.n));
     if (Ll.z <= 0.0f) return false;
     v3 Hl = add(G.V, Ll);
@@ -437,7 +460,7 @@ __device__ bool eval_glossy(const Ggx &G, v3 dw, v3 &fcos, float &pdf) {
     pdf = G.g1v * D / (4.0f * G.V.z);
     return true;
 }
-__device__ bool ggx_setup(const DevObject &m, v3 n, v3 in_dir, Ggx &G) {
+__device__ __forceinline__ bool ggx_setup(const DevObject &m, v3 n, v3 in_dir, Ggx &G) {
     const v3 alb = ld3(m.color);
     G.F0 = (m.metallic >= 1.0f) ? alb : add(scl(alb, m.metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m.metallic));
     const v3 up = (fabsf(n.y) > 0.99f) ? mk(1, 0, 0) : mk(0, 1, 0);
@@ -467,17 +490,38 @@ __device__ bool ggx_setup(const DevObject &m, v3 n, v3 in_dir, Ggx &G) {
 // shadow-ray emission: NEE samples are drawn in the shade kernel and tested
 // in the shadow kernel; a contribution is added only when unoccluded
 // ---------------------------------------------------------------------------
+// Up to two NEE rays per event, in two named slots (no dynamically indexed
+// private array, which would live in scratch memory).
 struct Shadows {
     int n;
-    float4 o[2], d[2];   // d.w = tmax, o.w = predicate
-    float4 c[2];
+    float4 o0, d0, c0;   // d.w = tmax, o.w = predicate
+    float4 o1, d1, c1;
 };
 
 __device__ __forceinline__ void push_shadow(Shadows &S, v3 o, v3 d, float tmax, int pred, v3 contrib) {
-    S.o[S.n] = v4(o, (float)pred);
-    S.d[S.n] = v4(d, tmax);
-    S.c[S.n] = v4(contrib, 0.0f);
+    const float4 o4 = v4(o, (float)pred), d4 = v4(d, tmax), c4 = v4(contrib, 0.0f);
+    if (S.n == 0) {
+        S.o0 = o4;
+        S.d0 = d4;
+        S.c0 = c4;
+    } else {
+        S.o1 = o4;
+        S.d1 = d4;
+        S.c1 = c4;
+    }
     S.n++;
+}
+
+// Stores the event's NEE rays and contributions at slots 2·p and 2·p + 1.
+__device__ __forceinline__ void store_shadows(const Shadows &S, uint32_t p, rg_ray *sray, float4 *scontrib) {
+    if (S.n > 0) {
+        sray[2 * p] = {{S.o0.x, S.o0.y, S.o0.z}, S.o0.w, {S.d0.x, S.d0.y, S.d0.z}, S.d0.w};
+        scontrib[2 * p] = S.c0;
+    }
+    if (S.n > 1) {
+        sray[2 * p + 1] = {{S.o1.x, S.o1.y, S.o1.z}, S.o1.w, {S.d1.x, S.d1.y, S.d1.z}, S.d1.w};
+        scontrib[2 * p + 1] = S.c1;
+    }
 }
 
 struct AreaSample {
@@ -487,7 +531,7 @@ struct AreaSample {
 };
 
 // sampleAreaLightForNEE (rayrenderer.cpp:1976-2036) up to the shadow test
-__device__ AreaSample sample_area(const PTScene &sc, Rng &R, v3 hit, v3 n) {
+__device__ __forceinline__ AreaSample sample_area(const PTScene &sc, Rng &R, v3 hit, v3 n) {
     AreaSample s;
     s.ok = false;
     if (sc.nlights == 0) return s;
@@ -535,7 +579,7 @@ struct EnvSample {
 };
 
 // sampleEnvmapForNEE (rayrenderer.cpp:2038-2065) up to the shadow test
-__device__ EnvSample sample_env_nee(const PTScene &sc, Rng &R, v3 hit, v3 n) {
+__device__ __forceinline__ EnvSample sample_env_nee(const PTScene &sc, Rng &R, v3 hit, v3 n) {
     EnvSample s;
     s.ok = false;
     if (!sc.has_env || sc.env_total <= 0.0f) return s;
@@ -571,7 +615,7 @@ __device__ __forceinline__ int crossed_medium(const PTScene &sc, const DevObject
 }
 
 // TransparencyShader as one path step (bsdf.cpp:513-546)
-__device__ void step_transparency(const PTScene &sc, Path &p, uint32_t prim, v3 hit, v3 n, float beta_scale,
+__device__ __forceinline__ void step_transparency(const PTScene &sc, Path &p, uint32_t prim, v3 hit, v3 n, float beta_scale,
                                   bool fresh_param) {
     const DevObject &m = obj_of(sc, prim);
     p.beta = scl(p.beta, beta_scale);
@@ -583,7 +627,7 @@ __device__ void step_transparency(const PTScene &sc, Path &p, uint32_t prim, v3 
 }
 
 // MixShader (bsdf.cpp:579-610) with one lobe drawn in proportion to its weight
-__device__ void step_mix(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n) {
+__device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n) {
     const DevObject &m = obj_of(sc, prim);
     const float wd = 1.0f - m.glossy - m.refraction;
     const float w0 = wd > 0.00001f ? wd : 0.0f;
@@ -715,7 +759,7 @@ __device__ void step_mix(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_
 }
 
 // RayBSDFShaderProvider::shade (rayrenderer.cpp:2776-2896) as one path event
-__device__ void step_surface(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n) {
+__device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n) {
     const DevObject &m = obj_of(sc, prim);
     const bool eye = p.passes < 0;
     if (m.emission > 0.0f) {
@@ -788,7 +832,7 @@ __device__ void step_surface(const PTScene &sc, Rng &R, Path &p, Shadows &S, uin
 }
 
 // RayRenderer::tracePath (rayrenderer.cpp:1596-1864) as one path event
-__device__ void step(const PTScene &sc, Rng &R, Path &p, Shadows &S, const rg_hit &h) {
+__device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows &S, const rg_hit &h) {
     const bool eye = p.passes < 0;
     const bool hit = h.prim != 0xFFFFFFFFu;
     const v3 pos = add(p.o, scl(p.d, h.t));

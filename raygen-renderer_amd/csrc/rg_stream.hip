@@ -209,7 +209,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sextend(StreamPool sp, const DevJo
 
 // One path event per live position, state updated in place; the event's
 // NEE rays are queued for the any-hit kernel.
-__global__ __launch_bounds__(kPtBlock) void k_sshade(StreamPool sp, const DevJobSlot *__restrict__ jobs, int qi) {
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_sshade(StreamPool sp, const DevJobSlot *__restrict__ jobs, int qi) {
     __shared__ uint32_t lds[kPtBlock / 64 + 1];
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
     const PathSoA S = sp.st;
@@ -243,11 +243,7 @@ __global__ __launch_bounds__(kPtBlock) void k_sshade(StreamPool sp, const DevJob
             S.tp[i] = v4(P.tp, 0.0f);
             S.L[i] = v4(P.L, survive ? 1.0f : 0.0f);
             S.meta[i] = make_int4(P.passes, P.medium, R.dim, (int)R.k);
-            for (int k = 0; k < Sh.n; k++) {
-                sp.sray[2 * i + k] = {{Sh.o[k].x, Sh.o[k].y, Sh.o[k].z}, Sh.o[k].w,
-                                      {Sh.d[k].x, Sh.d[k].y, Sh.d[k].z}, Sh.d[k].w};
-                sp.scontrib[2 * i + k] = Sh.c[k];
-            }
+            store_shadows(Sh, i, sp.sray, sp.scontrib);
             ns = (uint32_t)Sh.n;
         });
         sid.w += 1;
