@@ -135,6 +135,7 @@ def bench_bsdf_frame(ctx, do_cpu, cpu_seconds):
            "mrays_per_s": round((out["rays"] + out["shadow_rays"]) / wall / 1e6, 2),
            "mean_spp": round(float(out["samples"]) / (1920 * 1080), 2),
            "scene_gen_s": round(t1 - t0, 2), "bvh_build_upload_s": round(t2 - t1, 2)}
+    res["postprocess"] = bench_postprocess(ctx, out["hdr"], do_cpu)
     if do_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
@@ -146,6 +147,50 @@ def bench_bsdf_frame(ctx, do_cpu, cpu_seconds):
                                "kind": "port",
                                "sample": f"192x108 at 2 spp of the same scene (includes the oracle's own BVH "
                                          f"build), recursive restatement, {threads} threads"}
+    return res
+
+
+def bench_postprocess(ctx, hdr, do_cpu, reps=20):
+    """Output stage of the 1080p frame on device images: HDR bloom with the
+    reference's defaults + Reinhard/gamma tonemap (rg_postprocess_device),
+    timed with HIP events on the context's stream.  Algorithmic bytes: the
+    HDR image read by the glow downsample (16 B/px), read and written by the
+    composite (32 B/px) and the LDR image written (16 B/px) = 64 B/px."""
+    import numpy as np
+    import torch
+
+    import rgapi
+
+    h, w = hdr.shape[:2]
+    d_src = torch.from_numpy(np.ascontiguousarray(hdr)).to("cuda:0")
+    d_hdr = d_src.clone()
+    d_ldr = torch.empty_like(d_hdr)
+    b = rgapi.default_bloom()
+    stream = torch.cuda.current_stream()
+    rgapi.postprocess_device(ctx, w, h, d_hdr.data_ptr(), b, d_ldr.data_ptr())   # warm-up
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        d_hdr.copy_(d_src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        rgapi.postprocess_device(ctx, w, h, d_hdr.data_ptr(), b, d_ldr.data_ptr())
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    ms.sort()
+    med = ms[len(ms) // 2]
+    res = {"workload": f"bloom (threshold 1, strength 1, radius 0.03, size 0.15) + tonemap, {w}x{h}",
+           "ms": round(med, 4), "mpix_per_s": round(w * h / (med / 1e3) / 1e6, 1),
+           "algorithmic_gb_s": round(64 * w * h / (med / 1e3) / 1e9, 1)}
+    if do_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        t0 = time.perf_counter()
+        oracle.tonemap(oracle.bloom(hdr))
+        t = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(w * h / t / 1e6, 2), "unit": "Mpix/s", "cores": 1, "kind": "port",
+                               "sample": f"the same {w}x{h} frame through the oracle's bloom + tonemap, 1 thread"}
     return res
 
 

@@ -215,6 +215,33 @@ int rg_scene_set_camera(rg_scene *scene, const rg_camera *cam);
 int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *hdr_rgba,
               float *sum_rgb, float *sumsq_rgb, uint32_t *nspp);
 
+/* ---- output stage ------------------------------------------------------- */
+/* Bloom knobs of RendererSettings (rayrenderer.h:137-144; -blth -blst -blrd
+ * -blcv).  Defaults (rg_bloom_default) are the reference's. */
+typedef struct {
+    float threshold;   /* bloomThreshold: linear luma where bloom starts (1.0) */
+    float strength;    /* bloomStrength: gain on the blurred halo (1.0) */
+    float radius;      /* bloomRadius: halo sigma as a fraction of the width (0.03) */
+    float size_aspect; /* bloomSizeAspect: glow buffer size / image size (0.15) */
+    float curve;       /* bloomCurve: knee sharpness, 1 = linear (1.0) */
+    float pad;
+} rg_bloom;
+void rg_bloom_default(rg_bloom *b);
+
+/* Replaces RayRenderer::applyPostProcess + applyTonemapGamma, the stage that
+ * ends render() and reapplyPostProcess() (rayrenderer.cpp:598-697,
+ * 2614-2647, 701-712).  rg_render returns the pre-bloom HDR image; this
+ * call takes it on: when bloom is non-NULL (enableRenderingPostProcess,
+ * -enpp) the energy-based HDR bloom is added to hdr_rgba in place, then, when
+ * ldr_rgba is non-NULL, the luminance-Reinhard + 1/2.2-gamma image is
+ * written there (RGBA floats in [0, 1], alpha copied).  Host buffers of
+ * W*H*4 floats; blocks until done. */
+int rg_postprocess(rg_ctx *ctx, uint32_t width, uint32_t height, float *hdr_rgba, const rg_bloom *bloom,
+                   float *ldr_rgba);
+/* The same on DEVICE images, queued on the context's stream (asynchronous). */
+int rg_postprocess_device(rg_ctx *ctx, uint32_t width, uint32_t height, float *d_hdr_rgba, const rg_bloom *bloom,
+                          float *d_ldr_rgba);
+
 /* ---- scene ingest ------------------------------------------------------- */
 /* Replaces RendererSceneLoader::load + RayRenderer::transformScene
  * (sceneloader.cpp:847-882, 505-783; rayrenderer.cpp:157-363): reads the

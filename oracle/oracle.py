@@ -297,3 +297,27 @@ def render_adaptive(scene, width, height, spp, max_depth=5, base=4, threshold=0.
     nt = np.zeros(((width + 31) // 32) * ((height + 31) // 32), np.uint32)
     L.or_render_adaptive(ctypes.byref(d), ctypes.byref(s), mode, threads, base, threshold, _p(sm), _p(sq), _p(nt))
     return sm, sq, nt
+
+
+def bloom(hdr, threshold=1.0, strength=1.0, radius=0.03, size_aspect=0.15, curve=1.0):
+    """or_bloom on a copy of hdr (H x W x 4 float32); returns the bloomed image."""
+    L = lib()
+    f = ctypes.c_float
+    L.or_bloom.restype = None
+    L.or_bloom.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, f, f, f, f, f]
+    out = np.ascontiguousarray(hdr, dtype=np.float32).copy()
+    h, w = out.shape[:2]
+    L.or_bloom(_p(out), w, h, threshold, strength, radius, size_aspect, curve)
+    return out
+
+
+def tonemap(hdr):
+    """or_tonemap: H x W x 4 float32 HDR -> LDR in [0, 1]."""
+    L = lib()
+    L.or_tonemap.restype = None
+    L.or_tonemap.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    src = np.ascontiguousarray(hdr, dtype=np.float32)
+    h, w = src.shape[:2]
+    out = np.empty_like(src)
+    L.or_tonemap(_p(src), w, h, _p(out))
+    return out

@@ -178,6 +178,13 @@ float or_tile_noise(const float *sum_rgb, const float *sumsq_rgb, int w, int h, 
 int or_render_adaptive(const or_scene_desc *desc, const or_settings *s, int mode, int nthreads, uint32_t base,
                        float threshold, float *sum_rgb, float *sumsq_rgb, uint32_t *nspp);
 
+/* Output stage (rg_oracle_post.c): HDR bloom in place
+ * (RayRenderer::applyPostProcess, rayrenderer.cpp:598-697) and the
+ * Reinhard + gamma tonemap (applyTonemapGamma, :2614-2647); RGBA floats. */
+void or_bloom(float *hdr_rgba, int w, int h, float threshold, float strength, float radius, float size_aspect,
+              float curve);
+void or_tonemap(const float *hdr_rgba, int w, int h, float *ldr_rgba);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,7 @@ int dev_stream_create(void **s);
 void dev_stream_destroy(void *s);
 void release_pt(Ctx *c);
 void release_pt_scene(Scene *s);
+void release_post(Ctx *c);
 }  // namespace rg
 
 extern "C" {
@@ -73,6 +74,7 @@ void rg_destroy(rg_ctx *ctx) {
     if (c->device >= 0) {
         dev_set_device(c->device);
         release_pt(c);
+        release_post(c);
     }
     if (c->own_stream) dev_stream_destroy(c->own_stream);
     if (c->device >= 0) dev_clear_error();

@@ -87,12 +87,25 @@ struct PTBuffers {
     void *stats = nullptr;
 };
 
+// Device buffers of the output stage (bloom glow ping-pong, Gaussian taps,
+// staging images of the host-buffer entry point), grown on demand.
+struct PostBuffers {
+    void *glow_a = nullptr, *glow_b = nullptr, *taps = nullptr, *img_hdr = nullptr, *img_ldr = nullptr;
+    uint64_t glow_a_cap = 0, glow_b_cap = 0, taps_cap = 0, img_hdr_cap = 0, img_ldr_cap = 0;
+    // Gaussian taps now on the device and the sigma they were built for
+    // (the host copy is the source of the asynchronous upload)
+    std::vector<float> taps_host;
+    float taps_sigma = -1.0f;
+    int taps_r = 0;
+};
+
 struct Ctx {
     int device = -1;
     void *stream = nullptr;        // hipStream_t in use
     void *own_stream = nullptr;    // stream created by rg_create
     std::atomic<bool> cancel{false};
     PTBuffers pt;
+    PostBuffers post;
     uint64_t last_stats[3] = {0, 0, 0};
 };
 

@@ -1,9 +1,13 @@
 // rgx — command-line front-end with the reference's render flags
 // (src/main.cpp:162-490): `rgx render <scene.json> [options]`.
 //
-//   -o FILE            output; .hdr = linear HDR (RGBE), otherwise a
-//                      tonemapped binary PPM (Reinhard on luminance + 1/2.2
+//   -o FILE            output; .hdr = linear HDR (RGBE, after bloom when
+//                      enabled), otherwise the tonemapped image as a binary
+//                      PPM (rg_postprocess: Reinhard on luminance + 1/2.2
 //                      gamma, RayRenderer::applyTonemapGamma)
+//   -enpp B            HDR bloom (enableRenderingPostProcess)
+//   -blth F -blst F -blcv F -blrd F   bloom threshold / strength / curve /
+//                      radius (--bloom-threshold, ...), main.cpp:225-262
 //   -r W,H | WxH | N   resolution (N = N x N)
 //   -s N               samples per pixel
 //   -c N               host threads (accepted; the GPU path does not use it)
@@ -53,27 +57,15 @@ static bool write_hdr(const char *path, const float *rgba, int w, int h) {
     return std::fclose(f) == 0;
 }
 
-// applyTonemapGamma (rayrenderer.cpp:2610-2647) into an 8-bit PPM
-static bool write_ppm(const char *path, const float *rgba, int w, int h) {
+// LDR image (RGBA floats in [0, 1]) as an 8-bit binary PPM
+static bool write_ppm(const char *path, const float *ldr, int w, int h) {
     FILE *f = std::fopen(path, "wb");
     if (!f) return false;
     std::fprintf(f, "P6\n%d %d\n255\n", w, h);
-    const float inv_gamma = 1.0f / 2.2f;
     for (size_t i = 0; i < (size_t)w * h; i++) {
-        const float *c = rgba + 4 * i;
-        const float L = 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2];
-        float m[3] = {0, 0, 0};
-        if (L > 1e-6f) {
-            const float s = (L / (L + 1.0f)) / L;
-            for (int k = 0; k < 3; k++) m[k] = c[k] * s;
-            const float peak = std::fmax(std::fmax(m[0], m[1]), m[2]);
-            if (peak > 1.0f)
-                for (int k = 0; k < 3; k++) m[k] /= peak;
-        }
         unsigned char px[3];
         for (int k = 0; k < 3; k++) {
-            float v = std::pow(std::fmax(m[k], 0.0f), inv_gamma);
-            if (v > 1.0f) v = 1.0f;
+            const float v = std::fmin(std::fmax(ldr[4 * i + k], 0.0f), 1.0f);
             px[k] = (unsigned char)(v * 255.0f + 0.5f);
         }
         std::fwrite(px, 1, 3, f);
@@ -88,6 +80,9 @@ int main(int argc, char **argv) {
     }
     rg_settings st;
     rg_settings_default(&st);
+    rg_bloom bloom;
+    rg_bloom_default(&bloom);
+    bool post = false;
     int shader = 5;
     std::string cmd, scene, out = "out.hdr";
     for (int i = 1; i < argc; i++) {
@@ -120,6 +115,11 @@ int main(int argc, char **argv) {
             else if (!strcmp(a, "-enad") || !strcmp(a, "--enable-adaptive")) st.adaptive = parse_bool(next());
             else if (!strcmp(a, "--adaptive-base")) st.adaptive_base = (uint32_t)std::atoi(next());
             else if (!strcmp(a, "--adaptive-threshold")) st.adaptive_threshold = (float)std::atof(next());
+            else if (!strcmp(a, "-enpp") || !strcmp(a, "--enable-postprocess")) post = parse_bool(next());
+            else if (!strcmp(a, "-blth") || !strcmp(a, "--bloom-threshold")) bloom.threshold = (float)std::atof(next());
+            else if (!strcmp(a, "-blst") || !strcmp(a, "--bloom-strength")) bloom.strength = (float)std::atof(next());
+            else if (!strcmp(a, "-blcv") || !strcmp(a, "--bloom-curve")) bloom.curve = (float)std::atof(next());
+            else if (!strcmp(a, "-blrd") || !strcmp(a, "--bloom-radius")) bloom.radius = (float)std::atof(next());
             else if (!strcmp(a, "--depth")) st.max_depth = (uint32_t)std::atoi(next());
             else if (!strcmp(a, "--seed")) st.seed = (uint32_t)std::strtoul(next(), nullptr, 0);
             else if (!strcmp(a, "-h") || !strcmp(a, "--help")) {
@@ -171,10 +171,17 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "render: %s\n", rg_last_error());
         return 1;
     }
-    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // render()'s output stage: bloom (when enabled) and the tonemap
     const bool is_hdr = out.size() >= 4 && (out.compare(out.size() - 4, 4, ".hdr") == 0);
+    std::vector<float> ldr(is_hdr ? 0 : hdr.size());
+    if (rg_postprocess(ctx, st.width, st.height, hdr.data(), post ? &bloom : nullptr,
+                       is_hdr ? nullptr : ldr.data()) != RG_OK) {
+        std::fprintf(stderr, "post-process: %s\n", rg_last_error());
+        return 1;
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const bool ok = is_hdr ? write_hdr(out.c_str(), hdr.data(), (int)st.width, (int)st.height)
-                           : write_ppm(out.c_str(), hdr.data(), (int)st.width, (int)st.height);
+                           : write_ppm(out.c_str(), ldr.data(), (int)st.width, (int)st.height);
     if (!ok) {
         std::fprintf(stderr, "cannot write %s\n", out.c_str());
         return 1;

@@ -91,6 +91,9 @@ def lib():
             "rg_server_stats": (i32, [vp, vp]),
             "rg_server_get_tune": (i32, [vp, vp]),
             "rg_server_destroy": (i32, [vp]),
+            "rg_bloom_default": (None, [vp]),
+            "rg_postprocess": (i32, [vp, u32, u32, vp, vp, vp]),
+            "rg_postprocess_device": (i32, [vp, u32, u32, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -477,3 +480,36 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Bloom(ctypes.Structure):
+    """rg_bloom: RendererSettings' bloom knobs (rayrenderer.h:137-144)."""
+    _fields_ = [("threshold", ctypes.c_float), ("strength", ctypes.c_float), ("radius", ctypes.c_float),
+                ("size_aspect", ctypes.c_float), ("curve", ctypes.c_float), ("pad", ctypes.c_float)]
+
+
+def default_bloom(**kw):
+    b = Bloom()
+    lib().rg_bloom_default(ctypes.byref(b))
+    for k, v in kw.items():
+        setattr(b, k, v)
+    return b
+
+
+def postprocess(ctx, hdr, bloom=None, want_ldr=True):
+    """Output stage on host images (RayRenderer::applyPostProcess +
+    applyTonemapGamma): returns (bloomed HDR, LDR or None); hdr (H x W x 4
+    float32) is not modified."""
+    out = np.ascontiguousarray(hdr, dtype=np.float32).copy()
+    h, w = out.shape[:2]
+    ldr = np.empty_like(out) if want_ldr else None
+    check(lib().rg_postprocess(ctx.h, w, h, _ptr(out), ctypes.byref(bloom) if bloom is not None else None,
+                               _ptr(ldr)))
+    return out, ldr
+
+
+def postprocess_device(ctx, width, height, d_hdr, bloom=None, d_ldr=None):
+    """Output stage on device images (integer addresses), queued on the
+    context's stream."""
+    check(lib().rg_postprocess_device(ctx.h, width, height, d_hdr,
+                                      ctypes.byref(bloom) if bloom is not None else None, d_ldr))
