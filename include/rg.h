@@ -247,11 +247,13 @@ int rg_postprocess_device(rg_ctx *ctx, uint32_t width, uint32_t height, float *d
  * (sceneloader.cpp:847-882, 505-783; rayrenderer.cpp:157-363): reads the
  * reference's scene JSON (comments and trailing commas allowed) with its
  * objects, location/angle/scale transforms, plane/cube/sphere primitives,
- * binary .mesh files, inline and `_materials` materials, object and global
- * `medium`, `mainCamera` and an .hdr `envmap`, and returns a scene in render
- * space with shading, media, environment and camera set (BVH not built).
- * Parts outside this path (OBJ, .toba, textures, cubemaps, non-homogeneous
- * media) are skipped with a warning (rg_scene_warnings). */
+ * binary .mesh files, Wavefront .obj files with their .mtl libraries (one
+ * child object per OBJ object, ObjFileReader, objreader.cpp:93-590;
+ * sceneloader.cpp:405-478), inline and `_materials` materials, object and
+ * global `medium`, `mainCamera` and an .hdr `envmap`, and returns a scene in
+ * render space with shading, media, environment and camera set (BVH not
+ * built).  Parts outside this path (FBX, .toba, textures, cubemaps,
+ * non-homogeneous media) are skipped with a warning (rg_scene_warnings). */
 int rg_scene_load_json(rg_ctx *ctx, const char *path, rg_scene **out);
 /* Newline-separated warnings of the load into buf (capacity cap); returns the
  * full length. */

@@ -12,6 +12,7 @@
 // maps world to render space, where the eye sits at the origin looking down
 // -Z.  The reference delegates these to its ugm Matrix4, which is not
 // vendored, so the Euler order is the one documented here.
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -264,6 +265,224 @@ static bool load_mesh_file(const std::string &path, MeshData &m, std::string &er
     return true;
 }
 
+// Face normals of a mesh read without normals, as Mesh::calcNormals
+// computes them (mesh.cpp:531-550): normalize(cross(v1 - v3, v1 - v2)).
+static void face_normals(MeshData &m) {
+    const size_t nt = m.v.size() / 9;
+    m.n.assign(m.v.size(), 0.0f);
+    for (size_t t = 0; t < nt; t++) {
+        const float *v = &m.v[9 * t];
+        const float a[3] = {v[0] - v[6], v[1] - v[7], v[2] - v[8]};
+        const float b[3] = {v[0] - v[3], v[1] - v[4], v[2] - v[5]};
+        const float c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        const float l = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        if (!(l > 0.0f)) continue;
+        for (int k = 0; k < 3; k++)
+            for (int j = 0; j < 3; j++) m.n[9 * t + 3 * k + j] = c[j] / l;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront OBJ: ObjFileReader::read (objreader.cpp:93-289), readSurfaceLine
+// (:291-400), finalizeObject (:402-432), readMaterialLibrary (:490-590)
+// ---------------------------------------------------------------------------
+struct ObjMtl {
+    std::string name, texture;
+    float kd[3] = {0, 0, 0};
+    float transparency = 0.0f;
+};
+
+struct ObjPart {
+    std::string name, mat_name;
+    int mtl = -1;  // index into the library, -1 when usemtl named none of it
+    MeshData mesh;
+};
+
+// tag followed by a space (ObjFileReader::isLine)
+static bool is_tag(const std::string &line, const char *tag) {
+    const size_t n = std::strlen(tag);
+    return line.size() > n && line.compare(0, n, tag) == 0 && line[n] == ' ';
+}
+
+static void read_mtl(const std::string &path, std::vector<ObjMtl> &out, std::vector<std::string> &warnings) {
+    std::ifstream f(path);
+    if (!f) {
+        warnings.push_back("material library not read: " + path);
+        return;
+    }
+    ObjMtl mat;
+    bool has = false;
+    std::string line;
+    while (std::getline(f, line)) {
+        while (!line.empty() && (line.back() == '\r' || line.back() == '\n')) line.pop_back();
+        if (is_tag(line, "newmtl")) {
+            if (has) out.push_back(mat);
+            // ObjMaterial::reset clears name, texture and colours but keeps
+            // the transparency of the previous entry
+            const float keep = mat.transparency;
+            mat = ObjMtl();
+            mat.transparency = keep;
+            mat.name = line.substr(7);
+            has = true;
+        } else if (is_tag(line, "Kd")) {
+            float c[3];
+            if (std::sscanf(line.c_str() + 3, "%f %f %f", &c[0], &c[1], &c[2]) >= 1)
+                for (int k = 0; k < 3; k++) mat.kd[k] = c[k];
+        } else if (is_tag(line, "Tf")) {
+            float c[3];
+            if (std::sscanf(line.c_str() + 3, "%f %f %f", &c[0], &c[1], &c[2]) >= 1)
+                mat.transparency = 1.0f - (c[0] + c[1] + c[2]) / 3.0f;
+        } else if (is_tag(line, "map_Kd")) {
+            mat.texture = line.substr(7);
+        }
+    }
+    if (has) out.push_back(mat);
+}
+
+// Reads `path` into its objects (fan-triangulated faces, 1-based indices,
+// negative ones counted from the end as the OBJ format defines them).
+// Objects whose faces mix texcoord or normal layouts are dropped, as the
+// reference drops its error objects.
+static bool load_obj(const std::string &path, std::vector<ObjPart> &parts, std::vector<ObjMtl> &mtls,
+                     std::vector<std::string> &warnings, std::string &err) {
+    std::ifstream f(path);
+    if (!f) {
+        err = "cannot read " + path;
+        return false;
+    }
+    const size_t slash = path.rfind('/');
+    const std::string dir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
+    std::vector<float> vs, ns;
+    std::vector<ObjPart> done;
+    ObjPart cur;
+    bool cur_n = false, cur_t = false, first_face = true, cur_error = false, head = true, mm = false;
+    int line_no = 0;
+    auto finalize = [&]() {
+        if (cur_error) warnings.push_back(path + ": object '" + cur.name + "' has inconsistent faces, skipped");
+        else if (!cur.mesh.v.empty()) {
+            if (!cur_n) face_normals(cur.mesh);
+            done.push_back(std::move(cur));
+        }
+        cur = ObjPart();
+        cur_n = cur_t = cur_error = false;
+        first_face = true;
+    };
+    auto index = [](long v, size_t n) -> long { return v > 0 ? v - 1 : (v < 0 ? (long)n + v : -1); };
+    std::string line;
+    while (std::getline(f, line)) {
+        line_no++;
+        while (!line.empty() && (line.back() == '\r' || line.back() == '\n')) line.pop_back();
+        if (head && !line.empty() && line[0] == '#') {
+            if (line.find("3ds Max") != std::string::npos ||
+                line.find("uses millimeters as units") != std::string::npos) {
+                mm = true;  // millimetre exports are scaled to metres
+                head = false;
+            }
+            continue;
+        }
+        head = false;
+        if (is_tag(line, "v")) {
+            float x = 0, y = 0, z = 0;
+            std::sscanf(line.c_str() + 2, "%f %f %f", &x, &y, &z);
+            if (mm) {
+                x *= 0.01f;
+                y *= 0.01f;
+                z *= 0.01f;
+            }
+            vs.insert(vs.end(), {x, y, z});
+        } else if (is_tag(line, "vn")) {
+            float x = 0, y = 0, z = 0;
+            std::sscanf(line.c_str() + 3, "%f %f %f", &x, &y, &z);
+            ns.insert(ns.end(), {x, y, z});
+        } else if (is_tag(line, "f")) {
+            if (cur_error) continue;
+            std::vector<long> vi, ni;
+            bool has_t = false, has_n = false, ok = true;
+            const char *p = line.c_str() + 2;
+            while (*p && ok) {
+                while (*p == ' ' || *p == '\t') p++;
+                if (!*p) break;
+                char *e = nullptr;
+                const long v = std::strtol(p, &e, 10);
+                if (e == p) { ok = false; break; }
+                vi.push_back(index(v, vs.size() / 3));
+                p = e;
+                long t = 0, n = 0;
+                bool gt = false, gn = false;
+                if (*p == '/') {
+                    p++;
+                    if (*p != '/' && *p && *p != ' ') {
+                        t = std::strtol(p, &e, 10);
+                        if (e == p) { ok = false; break; }
+                        gt = true;
+                        p = e;
+                    }
+                    if (*p == '/') {
+                        p++;
+                        if (*p && *p != ' ') {
+                            n = std::strtol(p, &e, 10);
+                            if (e == p) { ok = false; break; }
+                            gn = true;
+                            p = e;
+                        }
+                    }
+                }
+                (void)t;
+                // every face of an object carries the same attribute set
+                if (gt ? (!cur_t && !first_face) : cur_t) ok = false;
+                if (gn ? (!cur_n && !first_face) : cur_n) ok = false;
+                if (gt) cur_t = has_t = true;
+                if (gn) {
+                    cur_n = has_n = true;
+                    ni.push_back(index(n, ns.size() / 3));
+                }
+            }
+            if (ok && vi.size() < 3) ok = false;
+            for (long v : vi)
+                if (v < 0 || (size_t)v >= vs.size() / 3) ok = false;
+            for (long n : ni)
+                if (n < 0 || (size_t)n >= ns.size() / 3) ok = false;
+            if (has_n && ni.size() != vi.size()) ok = false;
+            (void)has_t;
+            if (!ok) {
+                warnings.push_back(path + ": invalid face at line " + std::to_string(line_no));
+                cur_error = true;
+                continue;
+            }
+            const float zero[3] = {0, 0, 0};
+            for (size_t i = 1; i + 1 < vi.size(); i++) {
+                const float *a = &vs[3 * vi[0]], *b = &vs[3 * vi[i]], *c = &vs[3 * vi[i + 1]];
+                push_tri(cur.mesh, a, b, c, has_n ? &ns[3 * ni[0]] : zero, has_n ? &ns[3 * ni[i]] : zero,
+                         has_n ? &ns[3 * ni[i + 1]] : zero);
+            }
+            first_face = false;
+        } else if (is_tag(line, "o")) {
+            finalize();
+            cur.name = line.substr(2);
+        } else if (is_tag(line, "g")) {
+            if (line.substr(2) == "default") continue;
+            finalize();
+            std::istringstream ss(line.substr(2));
+            std::string g, name;
+            while (ss >> g) name += (name.empty() ? "" : "_") + g;
+            cur.name = name;
+        } else if (is_tag(line, "mtllib")) {
+            read_mtl(dir + line.substr(7), mtls, warnings);
+        } else if (is_tag(line, "usemtl")) {
+            cur.mat_name = line.substr(7);
+            cur.mtl = -1;
+            for (size_t k = 0; k < mtls.size(); k++)
+                if (mtls[k].name == cur.mat_name) {
+                    cur.mtl = (int)k;
+                    break;
+                }
+        }
+    }
+    finalize();
+    parts = std::move(done);
+    return true;
+}
+
 // Radiance RGBE (.hdr) reader: flat and new-style run-length scanlines.
 bool read_rgbe(const std::string &path, uint32_t &w, uint32_t &h, std::vector<float> &rgb, std::string &err) {
     std::string buf;
@@ -504,14 +723,54 @@ struct Loader {
 
     std::string err;
 
+    // A child of an object: a JSON object, or an object of an OBJ file with
+    // its resolved material (SceneJsonLoader::readObjAsSceneObjects,
+    // sceneloader.cpp:424-478), in the order the reference adds them.
+    struct Child {
+        const json::Value *json = nullptr;
+        rg_object mat;
+        MeshData mesh;
+    };
+
+    // .obj as child objects; `parent_mat` is the loading object's material
+    // at the point its "mesh" key is read
+    bool read_obj(const std::string &path, const rg_object &parent_mat, std::vector<Child> &out) {
+        std::vector<ObjPart> parts;
+        std::vector<ObjMtl> mtls;
+        if (!load_obj(path, parts, mtls, warnings, err)) return false;
+        for (ObjPart &p : parts) {
+            Child c;
+            // _materials override by name > .mtl defaults > the parent's mat
+            const rg_object *over = p.mat_name.empty() ? nullptr : find_material(p.mat_name);
+            if (over) {
+                c.mat = *over;
+            } else if (p.mtl >= 0) {
+                const ObjMtl &m = mtls[(size_t)p.mtl];
+                rg_object_default(&c.mat);
+                for (int k = 0; k < 3; k++) c.mat.color[k] = m.kd[k];
+                c.mat.transparency = m.transparency;
+                if (!m.texture.empty()) warnings.push_back("texture ignored: " + m.texture);
+            } else {
+                c.mat = parent_mat;
+                c.mat.interior_medium = -1;
+            }
+            c.mesh = std::move(p.mesh);
+            out.push_back(std::move(c));
+        }
+        return true;
+    }
+
     // mesh property of an object (sceneloader.cpp:618-641, 390-403)
-    bool read_meshes(const json::Value &val, std::vector<MeshData> &out) {
+    bool read_meshes(const json::Value &val, std::vector<MeshData> &out, const rg_object &cur_mat,
+                     std::vector<Child> &kids) {
         auto from_path = [&](const std::string &rel) -> bool {
             const std::string p = resolve(rel);
             const size_t dot = p.rfind('.');
-            const std::string ext = dot == std::string::npos ? "" : p.substr(dot);
-            if (ext == ".obj" || ext == ".OBJ") {
-                warnings.push_back("OBJ import is outside this path, skipped: " + rel);
+            std::string ext = dot == std::string::npos ? "" : p.substr(dot);
+            for (char &ch : ext) ch = (char)std::tolower((unsigned char)ch);
+            if (ext == ".obj") return read_obj(p, cur_mat, kids);
+            if (ext == ".fbx") {
+                warnings.push_back("FBX import is outside this path, skipped: " + rel);
                 return true;
             }
             MeshData m;
@@ -564,8 +823,7 @@ struct Loader {
         rg_object_default(&mat);
         int32_t interior = -1;
         std::vector<MeshData> meshes;
-        std::vector<const json::Value *> children;
-        std::vector<std::string> child_names;
+        std::vector<Child> children;
         bool visible = true;
         for (auto &p : o.obj) {
             const std::string &key = p.first;
@@ -574,7 +832,7 @@ struct Loader {
             } else if (key == "_bundle") {
                 warnings.push_back(".toba bundles are outside this path, skipped");
             } else if (key == "mesh") {
-                if (!read_meshes(val, meshes)) return false;
+                if (!read_meshes(val, meshes, mat, children)) return false;
             } else if (key == "mat") {
                 if (val.is_string()) {
                     const rg_object *m = find_material(val.str);
@@ -601,8 +859,9 @@ struct Loader {
             } else if (is_camera_like(key) && val.is_object()) {
                 read_camera(val);
             } else if (val.is_object()) {
-                children.push_back(&val);
-                child_names.push_back(key);
+                Child c;
+                c.json = &val;
+                children.push_back(std::move(c));
             }
         }
         if (!visible) {
@@ -615,8 +874,18 @@ struct Loader {
             objects.push_back(mat);
             pending.push_back({model, oid, std::move(meshes)});
         }
-        for (const json::Value *c : children)
-            if (!object(*c, model, false)) return false;
+        for (Child &c : children) {
+            if (c.json) {
+                if (!object(*c.json, model, false)) return false;
+            } else {
+                // an OBJ object: a child with the identity transform
+                const uint32_t oid = (uint32_t)objects.size();
+                objects.push_back(c.mat);
+                std::vector<MeshData> one;
+                one.push_back(std::move(c.mesh));
+                pending.push_back({model, oid, std::move(one)});
+            }
+        }
         scopes.pop_back();
         return true;
     }

@@ -164,13 +164,13 @@ def test_envmap_hdr_and_root_medium(tmp_path, rle):
 def test_inactive_medium_dropped_and_skips_warn(tmp_path):
     (tmp_path / "s.json").write_text("""{
       "a": { "mesh": { "type": "cone" }, "medium": { "sigma_a": [0, 0, 0] } },
-      "b": { "mesh": ["x.obj"], "mat": "nosuch" },
+      "b": { "mesh": ["x.fbx"], "mat": "nosuch" },
       "c": { "mesh": { "type": "cube" }, "mat": { "color": "#f80" } } }""")
     ctx = rgapi.Context(-1)
     _, d = rgapi.load_json(ctx, str(tmp_path / "s.json"))
     assert len(d["media"]) == 0 and len(d["objects"]) == 1
     w = "\n".join(d["warnings"])
-    assert "cone" in w and "x.obj" in w and "nosuch" in w
+    assert "cone" in w and "x.fbx" in w and "nosuch" in w
     # 3-digit hex is read as one number times 2 (sceneloader.cpp:184-186):
     # 0xf80 * 2 = 0x001F00
     np.testing.assert_array_equal(d["objects"][0]["color"], np.float32([0, 31 / 255, 0]))
