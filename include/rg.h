@@ -189,6 +189,17 @@ typedef struct {
     uint32_t samples_in_flight; /* paths per launch wave = pixels x this; 0 = auto */
 } rg_settings;
 
+/* One material texture (Texture, texture.h:19-50): w x h texels of RGB
+ * floats as stored in the image (LDR: byte / 255, HDR: linear radiance), row
+ * 0 = top.  srgb != 0 marks an LDR image authored in sRGB: its texels are
+ * converted to linear as Texture::sample does (texture.cpp:199-235). */
+typedef struct {
+    uint32_t w, h;
+    const float *rgb;
+    int32_t srgb;
+    uint32_t pad;
+} rg_texture;
+
 void rg_object_default(rg_object *obj);
 void rg_settings_default(rg_settings *s);
 
@@ -198,6 +209,14 @@ void rg_settings_default(rg_settings *s);
  * object with emission > 0 is an area light made of its triangles. */
 int rg_scene_set_shading(rg_scene *scene, const float *n9, const uint32_t *obj_of_tri,
                          const rg_object *objs, uint32_t n_obj);
+/* Albedo textures (Material::texture + texTiling, material.h:54-60, sampled
+ * by the diffuse and glossy lobes and the depth-cap light term, bsdf.cpp:67-73,
+ * 183-188; rayrenderer.cpp:2843-2850): tex_of_obj[n_obj] names each object's
+ * texture (-1 none), tiling_of_obj[n_obj * 2] its texTiling, uv6[n_tris * 6]
+ * every triangle's first UV set (RenderMeshTriangle uv1..uv3).  Call after
+ * rg_scene_set_shading; n_tex == 0 removes all textures. */
+int rg_scene_set_textures(rg_scene *scene, const rg_texture *tex, uint32_t n_tex, const int32_t *tex_of_obj,
+                          const float *tiling_of_obj, const float *uv6);
 /* Media referenced by rg_object.interior_medium; global_medium = the
  * scene's globalMedium (-1 none). */
 int rg_scene_set_media(rg_scene *scene, const rg_medium *media, uint32_t n, int32_t global_medium);
@@ -268,6 +287,13 @@ int rg_scene_export(const rg_scene *scene, float *v9, float *n9, uint32_t *obj_o
  * (env->width * env->height * 3 floats) where non-NULL. */
 int rg_scene_export_media(const rg_scene *scene, rg_medium *media, uint32_t *n_media, int32_t *global_medium,
                           rg_env *env, float *env_rgb);
+
+/* Copies the textures out: n_tex first, then, where non-NULL, tex (w, h,
+ * srgb of each; rgb left NULL), texels (the textures' RGB floats as handed
+ * in, concatenated), tex_of_obj (n_obj), tiling_of_obj (n_obj * 2) and uv6
+ * (n_tris * 6; zeros when the scene has no textures). */
+int rg_scene_export_textures(const rg_scene *scene, uint32_t *n_tex, rg_texture *tex, float *texels,
+                             int32_t *tex_of_obj, float *tiling_of_obj, float *uv6);
 
 /* Counters of the last rg_render on this context: [0] extension (closest-hit)
  * rays, [1] NEE shadow (any-hit) rays, [2] path samples. */

@@ -188,11 +188,17 @@ class _Env(ctypes.Structure):
                 ("intensity", ctypes.c_float), ("rotation_deg", ctypes.c_float)]
 
 
+class _Tex(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("rgb", ctypes.c_void_p), ("srgb", ctypes.c_int32),
+                ("pad", ctypes.c_uint32)]
+
+
 class _Desc(ctypes.Structure):
     _fields_ = [("v9", ctypes.c_void_p), ("n9", ctypes.c_void_p), ("obj_of_tri", ctypes.c_void_p),
                 ("ntris", ctypes.c_uint32), ("objs", ctypes.c_void_p), ("nobj", ctypes.c_uint32),
                 ("media", ctypes.c_void_p), ("nmedia", ctypes.c_uint32), ("global_medium", ctypes.c_int32),
-                ("env", _Env), ("cam", _Cam)]
+                ("env", _Env), ("cam", _Cam), ("tex", ctypes.c_void_p), ("ntex", ctypes.c_uint32),
+                ("tex_of_obj", ctypes.c_void_p), ("tiling", ctypes.c_void_p), ("uv6", ctypes.c_void_p)]
 
 
 class _Settings(ctypes.Structure):
@@ -224,6 +230,20 @@ def _desc(scene, keep):
         keep.append(rgb)
         d.env = _Env(rgb.shape[1], rgb.shape[0], rgb.ctypes.data, float(env.get("intensity", 1.0)),
                      float(env.get("rotation_deg", 0.0)))
+    texs = scene.get("textures")
+    if texs:
+        rgbs = [np.ascontiguousarray(t["rgb"], np.float32) for t in texs]
+        tab = (_Tex * len(texs))(*[_Tex(r.shape[1], r.shape[0], r.ctypes.data, 1 if t.get("srgb", True) else 0, 0)
+                                   for r, t in zip(rgbs, texs)])
+        too = np.ascontiguousarray(scene["tex_of_obj"], np.int32)
+        til = np.ascontiguousarray(scene["tiling"], np.float32)
+        uv6 = np.ascontiguousarray(scene["uv6"], np.float32)
+        keep += [rgbs, tab, too, til, uv6]
+        d.tex = ctypes.addressof(tab)
+        d.ntex = len(texs)
+        d.tex_of_obj = too.ctypes.data
+        d.tiling = til.ctypes.data
+        d.uv6 = uv6.ctypes.data
     cam = scene["camera"]
     d.cam = _Cam(cam.get("fov_deg", 75.0), cam.get("exposure", 1.0), cam.get("depth_of_field", 0.0),
                  cam.get("aperture", 1.8), cam.get("aperture_blades", 0), cam.get("aperture_rotation_deg", 0.0),

@@ -132,6 +132,15 @@ typedef struct {
     float intensity, rotation_deg;
 } or_env;
 
+/* Material texture (texture.h:19-50): RGB floats as stored (LDR byte/255),
+ * row 0 = top; srgb marks an LDR sRGB image. */
+typedef struct {
+    uint32_t w, h;
+    const float *rgb;
+    int32_t srgb;
+    uint32_t pad;
+} or_texture;
+
 typedef struct {
     const float *v9;            /* render-space vertices */
     const float *n9;            /* render-space vertex normals */
@@ -144,6 +153,13 @@ typedef struct {
     int32_t global_medium;      /* -1 none */
     or_env env;
     or_camera cam;
+    /* albedo textures (nullable): texture table, texture per object (-1
+     * none), texTiling per object (2 floats), first UV set per triangle (6) */
+    const or_texture *tex;
+    uint32_t ntex;
+    const int32_t *tex_of_obj;
+    const float *tiling;
+    const float *uv6;
 } or_scene_desc;
 
 typedef struct {

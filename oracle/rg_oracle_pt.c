@@ -102,6 +102,8 @@ typedef struct {
     float view_scale, half_w, half_h, aperture, dof, ap_rot;
     int blades;
     int max_depth;
+    /* linear RGB texels of each texture (sRGB decoded once) */
+    float **tex_lin;
 } prep_t;
 
 /* ------------------------------------------------------------------------ */
@@ -195,6 +197,38 @@ static v3 interp_normal(const prep_t *P, const isect_t *h) {
     v3 n = add(add(scl(tri_n(P, h->prim, 0), w), scl(tri_n(P, h->prim, 1), h->u)),
                scl(tri_n(P, h->prim, 2), h->v));
     return nrm(n);
+}
+
+/* Texture::sample (texture.cpp:209-235) of the hit object's albedo texture
+ * at the barycentric UV (calcVertexInterpolation, rayrenderer.cpp:2354-2362)
+ * times texTiling; (1, 1, 1) when the object has no texture. */
+static v3 albedo_tex(const prep_t *P, const isect_t *h) {
+    const or_scene_desc *d = P->d;
+    if (!d->ntex || !d->tex_of_obj) return mk(1.0f, 1.0f, 1.0f);
+    const uint32_t o = d->obj_of_tri[h->prim];
+    const int32_t t = d->tex_of_obj[o];
+    if (t < 0) return mk(1.0f, 1.0f, 1.0f);
+    const float *q = d->uv6 + 6 * (size_t)h->prim;
+    const float w = 1.0f - h->u - h->v;
+    const float su = (q[0] * w + q[2] * h->u + q[4] * h->v) * d->tiling[2 * o];
+    const float sv = (q[1] * w + q[3] * h->u + q[5] * h->v) * d->tiling[2 * o + 1];
+    if (!(su == su) || !(sv == sv)) return mk(0.0f, 0.0f, 0.0f);
+    float fu = su - floorf(su), fv = sv - floorf(sv);
+    if (fu < 0.0f || fu >= 1.0f) fu = 0.0f;
+    if (fv < 0.0f || fv >= 1.0f) fv = 0.0f;
+    const int W = (int)d->tex[t].w, H = (int)d->tex[t].h;
+    int x = (int)(fu * (float)W), y = (int)(fv * (float)H);
+    if (x >= W) x = W - 1;
+    if (y >= H) y = H - 1;
+    if (x < 0) x = 0;
+    if (y < 0) y = 0;
+    return ld3(P->tex_lin[t] + 3 * ((size_t)y * W + x));
+}
+
+/* srgbToLinear (texture.cpp:203-206) */
+static float srgb_to_linear(float c) {
+    if (c <= 0.04045f) return c * (1.0f / 12.92f);
+    return powf((c + 0.055f) * (1.0f / 1.055f), 2.4f);
 }
 
 /* calcVertexInterpolation(rt, hit) (rayrenderer.cpp:2337-2352): area weights */
@@ -461,8 +495,7 @@ static int eval_glossy(const ggx_t *G, v3 dw, v3 *fcos, float *pdf) {
     return 1;
 }
 /* returns 0 when the view is below the shading plane */
-static int ggx_setup(const or_object *m, v3 n, v3 in_dir, ggx_t *G) {
-    const v3 alb = ld3(m->color);
+static int ggx_setup(const or_object *m, v3 alb, v3 n, v3 in_dir, ggx_t *G) {
     G->F0 = (m->metallic >= 1.0f) ? alb
             : add(scl(alb, m->metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m->metallic));
     tangent_frame(n, &G->t, &G->b);
@@ -601,7 +634,7 @@ static v3 diffuse_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3
     const float r1 = lds1(R), r2 = lds1(R);
     const v3 dir = cosine_dir(n, r1, r2);
     const v3 o = surface_origin(h->pos, dir, geom_normal(P, h->prim, n));
-    const v3 alb = ld3(m->color);
+    const v3 alb = mul(ld3(m->color), albedo_tex(P, h));
     const v3 st = pm->tp;
     const float sp = pm->bsdf_pdf;
     pm->tp = mul(pm->tp, alb);
@@ -618,7 +651,7 @@ static v3 diffuse_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3
 /* GlossyShader (bsdf.cpp:173-374) */
 static v3 glossy_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_d, v3 n, param_t *pm) {
     const or_object *m = obj_of(P, h->prim);
-    const v3 alb = ld3(m->color);
+    const v3 alb = mul(ld3(m->color), albedo_tex(P, h));
     const v3 F0 = (m->metallic >= 1.0f) ? alb
                   : add(scl(alb, m->metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m->metallic));
     if (m->roughness < 1e-3f) {
@@ -637,7 +670,7 @@ static v3 glossy_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_d, v3 
         return mul(inc, F);
     }
     ggx_t G;
-    if (!ggx_setup(m, n, in_d, &G)) return mk(0, 0, 0);
+    if (!ggx_setup(m, alb, n, in_d, &G)) return mk(0, 0, 0);
     v3 direct = mk(0, 0, 0);
     {
         float co, cl, r2, sa;
@@ -791,7 +824,7 @@ static v3 shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3 in_d, v
     if (sp) {
         if (sp->passes + 1 >= P->max_depth) {
             if (1.0f - m->glossy - m->refraction > 0.00001f)
-                return mul(trace_light(P, R, h->pos, n), ld3(m->color));
+                return mul(trace_light(P, R, h->pos, n), mul(ld3(m->color), albedo_tex(P, h)));
             return mk(0, 0, 0);
         }
         pm.passes = sp->passes + 1;
@@ -1046,7 +1079,7 @@ static void twin_mix(const prep_t *P, rng_t *R, path_t *p, const isect_t *h, v3 
     p->tp = scl(p->tp, wl);
     const v3 hit = h->pos;
     if (lobe == 0) {
-        const v3 alb = ld3(m->color);
+        const v3 alb = mul(ld3(m->color), albedo_tex(P, h));
         const float r1 = lds1(R), r2 = lds1(R);
         const v3 dir = cosine_dir(n, r1, r2);
         const float pdf = fmaxf(0.0f, dot(dir, n)) * (float)(1.0 / M_PI);
@@ -1061,7 +1094,7 @@ static void twin_mix(const prep_t *P, rng_t *R, path_t *p, const isect_t *h, v3 
         return;
     }
     if (lobe == 1) {
-        const v3 alb = ld3(m->color);
+        const v3 alb = mul(ld3(m->color), albedo_tex(P, h));
         if (m->roughness < 1e-3f) {
             const v3 F0 = (m->metallic >= 1.0f) ? alb
                           : add(scl(alb, m->metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m->metallic));
@@ -1078,7 +1111,7 @@ static void twin_mix(const prep_t *P, rng_t *R, path_t *p, const isect_t *h, v3 
             return;
         }
         ggx_t G;
-        if (!ggx_setup(m, n, p->d, &G)) { p->alive = 0; return; }
+        if (!ggx_setup(m, alb, n, p->d, &G)) { p->alive = 0; return; }
         {
             float co, cl, r2, sa;
             nee_t s = sample_area(P, R, hit, n, &co, &cl, &r2, &sa);
@@ -1177,7 +1210,7 @@ static void twin_surface(const prep_t *P, rng_t *R, path_t *p, const isect_t *h,
     if (!eye) {
         if (p->passes + 1 >= P->max_depth) {
             if (1.0f - m->glossy - m->refraction > 0.00001f)
-                add_L(p, mul(mul(p->beta, ld3(m->color)), trace_light(P, R, h->pos, n)));
+                add_L(p, mul(mul(p->beta, mul(ld3(m->color), albedo_tex(P, h))), trace_light(P, R, h->pos, n)));
             p->alive = 0;
             return;
         }
@@ -1297,6 +1330,15 @@ static v3 render_sample(const prep_t *P, const or_settings *S, int mode, int x, 
 static int prepare(prep_t *P, const or_scene_desc *d, const or_settings *S) {
     memset(P, 0, sizeof *P);
     P->d = d;
+    if (d->ntex) {
+        P->tex_lin = (float **)calloc(d->ntex, sizeof(float *));
+        for (uint32_t k = 0; k < d->ntex; k++) {
+            const size_t m = 3 * (size_t)d->tex[k].w * d->tex[k].h;
+            P->tex_lin[k] = (float *)malloc(m * sizeof(float));
+            for (size_t i = 0; i < m; i++)
+                P->tex_lin[k][i] = d->tex[k].srgb ? srgb_to_linear(d->tex[k].rgb[i]) : d->tex[k].rgb[i];
+        }
+    }
     const uint32_t n = d->ntris;
     P->nodes = (or_node *)calloc(n ? 2 * (size_t)n : 1, sizeof(or_node));
     P->perm = (uint32_t *)calloc(n ? n : 1, sizeof(uint32_t));
@@ -1398,6 +1440,10 @@ static void release(prep_t *P) {
     free(P->nodes); free(P->perm); free(P->gn); free(P->ti_a); free(P->area); free(P->sampled_area);
     free(P->light_obj); free(P->light_first); free(P->light_count); free(P->light_tris);
     free(P->media); free(P->env_cond); free(P->env_marg);
+    if (P->tex_lin) {
+        for (uint32_t k = 0; k < P->d->ntex; k++) free(P->tex_lin[k]);
+        free(P->tex_lin);
+    }
 }
 
 typedef struct {

@@ -153,6 +153,16 @@ struct Scene {
     void *d_env_cond = nullptr;
     uint32_t nlights = 0;
     float env_total = 0.0f;
+    // albedo textures (rg_scene_set_textures): the texels as handed in, their
+    // linear device form, per-object binding and per-triangle UVs
+    std::vector<rg_texture> tex;          // rgb pointers unused after the copy
+    std::vector<float> tex_rgb;           // RGB as handed in, concatenated
+    std::vector<float> tex_linear;        // RGBA linear, concatenated (device layout)
+    std::vector<uint32_t> tex_offset;     // first texel of each texture
+    std::vector<int32_t> tex_of_obj;
+    std::vector<float> tex_tiling;        // 2 per object
+    std::vector<float> uv6;               // 6 per triangle
+    void *d_texels = nullptr, *d_tex_desc = nullptr, *d_otex = nullptr, *d_uv6 = nullptr;
     // loader diagnostics
     std::vector<std::string> warnings;
     bool has_camera = false;

@@ -64,6 +64,14 @@ struct PTScene {
     uint32_t env_w, env_h;
     float env_total, env_intensity, env_cos, env_sin;
     int32_t has_env;
+    // albedo textures: per object (texture id or -1, texTiling u, v as
+    // float bits), per texture (w, h, first texel), linear RGBA texels,
+    // per-triangle UVs
+    const int4 *otex;
+    const uint4 *tex_desc;
+    const float4 *texels;
+    const float *uv6;
+    int32_t has_tex;
     // camera (initRenderThreadContext, rayrenderer.cpp:97-132)
     float view_scale, half_w, half_h, aperture, dof, ap_rot;
     int32_t blades;

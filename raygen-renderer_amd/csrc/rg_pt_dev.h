@@ -195,6 +195,30 @@ __device__ __forceinline__ v3 interp_normal_at(const PTScene &sc, uint32_t t, v3
     return add(add(scl(tri_n(sc, t, 0), a1), scl(tri_n(sc, t, 1), a2)), scl(tri_n(sc, t, 2), a3));
 }
 
+// Texture::sample (texture.cpp:209-235) of the object's albedo texture at the
+// hit's interpolated UV (calcVertexInterpolation, rayrenderer.cpp:2354-2362)
+// times texTiling: UV folded into [0, 1), nearest texel, linear RGB (sRGB
+// textures are converted at upload).  (1, 1, 1) for an untextured object.
+__device__ __forceinline__ v3 albedo_tex(const PTScene &sc, uint32_t prim, float u, float v) {
+    if (!sc.has_tex) return mk(1.0f, 1.0f, 1.0f);
+    const int4 ot = sc.otex[sc.shade[prim].obj];
+    if (ot.x < 0) return mk(1.0f, 1.0f, 1.0f);
+    const float *q = sc.uv6 + 6 * (size_t)prim;
+    const float w = 1.0f - u - v;
+    const float su = (q[0] * w + q[2] * u + q[4] * v) * __int_as_float(ot.y);
+    const float sv = (q[1] * w + q[3] * u + q[5] * v) * __int_as_float(ot.z);
+    if (!(su == su) || !(sv == sv)) return mk(0.0f, 0.0f, 0.0f);
+    float fu = su - floorf(su), fv = sv - floorf(sv);
+    if (fu < 0.0f || fu >= 1.0f) fu = 0.0f;
+    if (fv < 0.0f || fv >= 1.0f) fv = 0.0f;
+    const uint4 d = sc.tex_desc[ot.x];
+    const int W = (int)d.x, H = (int)d.y;
+    int x = (int)(fu * (float)W), y = (int)(fv * (float)H);
+    x = x >= W ? W - 1 : (x < 0 ? 0 : x);
+    y = y >= H ? H - 1 : (y < 0 ? 0 : y);
+    return f4v(sc.texels[(size_t)d.z + (size_t)y * W + x]);
+}
+
 // ---------------------------------------------------------------------------
 // environment (rayrenderer.cpp:1299-1573)
 // ---------------------------------------------------------------------------
@@ -460,8 +484,7 @@ __device__ __forceinline__ bool eval_glossy(const Ggx &G, v3 dw, v3 &fcos, float
     pdf = G.g1v * D / (4.0f * G.V.z);
     return true;
 }
-__device__ __forceinline__ bool ggx_setup(const DevObject &m, v3 n, v3 in_dir, Ggx &G) {
-    const v3 alb = ld3(m.color);
+__device__ __forceinline__ bool ggx_setup(const DevObject &m, v3 alb, v3 n, v3 in_dir, Ggx &G) {
     G.F0 = (m.metallic >= 1.0f) ? alb : add(scl(alb, m.metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m.metallic));
     const v3 up = (fabsf(n.y) > 0.99f) ? mk(1, 0, 0) : mk(0, 1, 0);
     G.t = nrm(cross(up, n));
@@ -626,8 +649,10 @@ __device__ __forceinline__ void step_transparency(const PTScene &sc, Path &p, ui
     if (fresh_param) p.bsdf_pdf = 0.0f;
 }
 
-// MixShader (bsdf.cpp:579-610) with one lobe drawn in proportion to its weight
-__device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n) {
+// MixShader (bsdf.cpp:579-610) with one lobe drawn in proportion to its
+// weight; tex = the albedo texture sample at the hit (1 when untextured)
+__device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n,
+                                         v3 tex) {
     const DevObject &m = obj_of(sc, prim);
     const float wd = 1.0f - m.glossy - m.refraction;
     const float w0 = wd > 0.00001f ? wd : 0.0f;
@@ -646,8 +671,8 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
     p.beta = scl(p.beta, W);
     p.tp = scl(p.tp, wl);
     if (lobe == 0) {
-        // DiffuseShader (bsdf.cpp:54-108)
-        const v3 alb = ld3(m.color);
+        // DiffuseShader (bsdf.cpp:54-108): albedo = colour x texture
+        const v3 alb = mul(ld3(m.color), tex);
         const float r1 = lds1(R), r2 = lds1(R);
         const v3 dir = cosine_dir(n, r1, r2);
         const float pdf = fmaxf(0.0f, dot(dir, n)) * (float)(1.0 / M_PI);
@@ -671,8 +696,8 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
         return;
     }
     if (lobe == 1) {
-        // GlossyShader (bsdf.cpp:173-374)
-        const v3 alb = ld3(m.color);
+        // GlossyShader (bsdf.cpp:173-374): albedo = colour x texture drives F0
+        const v3 alb = mul(ld3(m.color), tex);
         if (m.roughness < 1e-3f) {
             const v3 F0 = (m.metallic >= 1.0f) ? alb
                                                : add(scl(alb, m.metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m.metallic));
@@ -689,7 +714,7 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
             return;
         }
         Ggx G;
-        if (!ggx_setup(m, n, p.d, G)) {
+        if (!ggx_setup(m, alb, n, p.d, G)) {
             p.alive = false;
             return;
         }
@@ -759,7 +784,8 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
 }
 
 // RayBSDFShaderProvider::shade (rayrenderer.cpp:2776-2896) as one path event
-__device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n) {
+__device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n,
+                                             v3 tex) {
     const DevObject &m = obj_of(sc, prim);
     const bool eye = p.passes < 0;
     if (m.emission > 0.0f) {
@@ -799,7 +825,7 @@ __device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p,
             if (1.0f - m.glossy - m.refraction > 0.00001f) {
                 const AreaSample as = sample_area(sc, R, hit, n);
                 if (as.ok) push_shadow(S, as.o, as.lr, 0.99999f, RG_PRED_AREA,
-                                       mul(mul(p.beta, ld3(m.color)), area_value(as)));
+                                       mul(mul(p.beta, mul(ld3(m.color), tex)), area_value(as)));
             }
             p.alive = false;
             return;
@@ -818,7 +844,7 @@ __device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p,
             step_transparency(sc, p, prim, hit, n, m.transparency, false);
             return;
         }
-        step_mix(sc, R, p, S, prim, hit, n);
+        step_mix(sc, R, p, S, prim, hit, n, tex);
         return;
     }
     p.passes = 0;
@@ -828,7 +854,7 @@ __device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p,
             return;
         }
     }
-    step_mix(sc, R, p, S, prim, hit, n);
+    step_mix(sc, R, p, S, prim, hit, n, tex);
 }
 
 // RayRenderer::tracePath (rayrenderer.cpp:1596-1864) as one path event
@@ -839,7 +865,8 @@ __device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows
     const DevMedium *med = (p.medium >= 0 && sc.media[p.medium].active) ? &sc.media[p.medium] : nullptr;
     if (!med) {
         if (hit && (!eye || obj_of(sc, h.prim).visible)) {
-            step_surface(sc, R, p, S, h.prim, pos, interp_normal(sc, h.prim, h.u, h.v));
+            step_surface(sc, R, p, S, h.prim, pos, interp_normal(sc, h.prim, h.u, h.v),
+                         albedo_tex(sc, h.prim, h.u, h.v));
             return;
         }
         v3 env = sample_environment(sc, p.d);
@@ -921,7 +948,7 @@ __device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows
         const v3 g = ld3(sc.shade[h.prim].gn);
         const v3 gN = dot(g, p.d) <= 0.0f ? g : neg(g);
         if (dot(p.d, n) > 0.0f && dot(p.d, gN) < 0.0f) n = gN;
-        step_surface(sc, R, p, S, h.prim, pos, n);
+        step_surface(sc, R, p, S, h.prim, pos, n, albedo_tex(sc, h.prim, h.u, h.v));
         return;
     }
     v3 env = sample_environment(sc, p.d);

@@ -40,7 +40,8 @@ inline float vlen(V a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
 
 void free_pt_scene(Scene *s) {
     for (void **p : {&s->d_v9, &s->d_shade, &s->d_objs, &s->d_lights, &s->d_light_tris, &s->d_media,
-                     &s->d_env_rgb, &s->d_env_marg, &s->d_env_cond}) {
+                     &s->d_env_rgb, &s->d_env_marg, &s->d_env_cond, &s->d_texels, &s->d_tex_desc, &s->d_otex,
+                     &s->d_uv6}) {
         dev_free(*p);
         *p = nullptr;
     }
@@ -133,7 +134,31 @@ int upload_pt_scene(Scene *s, void *stream) {
         }
     }
     s->nlights = (uint32_t)lights.size();
+    // texture tables (kept empty when no object has a texture)
+    std::vector<int32_t> otex;
+    std::vector<uint32_t> tex_desc;
+    if (!s->tex.empty()) {
+        otex.resize(4 * s->objects.size());
+        for (size_t o = 0; o < s->objects.size(); o++) {
+            const int32_t t = o < s->tex_of_obj.size() ? s->tex_of_obj[o] : -1;
+            float tu = 1.0f, tv = 1.0f;
+            if (2 * o + 1 < s->tex_tiling.size()) {
+                tu = s->tex_tiling[2 * o];
+                tv = s->tex_tiling[2 * o + 1];
+            }
+            otex[4 * o] = t;
+            std::memcpy(&otex[4 * o + 1], &tu, 4);
+            std::memcpy(&otex[4 * o + 2], &tv, 4);
+            otex[4 * o + 3] = 0;
+        }
+        for (size_t k = 0; k < s->tex.size(); k++)
+            tex_desc.insert(tex_desc.end(), {s->tex[k].w, s->tex[k].h, s->tex_offset[k], 0u});
+    }
     int rc = upload_vec(&s->d_v9, s->v9, stream);
+    if (!rc) rc = upload_vec(&s->d_otex, otex, stream);
+    if (!rc) rc = upload_vec(&s->d_tex_desc, tex_desc, stream);
+    if (!rc) rc = upload_vec(&s->d_texels, s->tex.empty() ? std::vector<float>() : s->tex_linear, stream);
+    if (!rc) rc = upload_vec(&s->d_uv6, s->tex.empty() ? std::vector<float>() : s->uv6, stream);
     if (!rc) rc = upload_vec(&s->d_shade, sh, stream);
     if (!rc) rc = upload_vec(&s->d_objs, s->objects, stream);
     if (!rc) rc = upload_vec(&s->d_lights, lights, stream);
@@ -223,6 +248,11 @@ PTScene make_ptscene(const Scene &scn, const rg_settings &st) {
         sc.env_sin = sinf(yaw);
     }
     sc.has_env = (s->env_w > 0 && s->env_h > 0) ? 1 : 0;
+    sc.has_tex = s->tex.empty() ? 0 : 1;
+    sc.otex = (const int4 *)s->d_otex;
+    sc.tex_desc = (const uint4 *)s->d_tex_desc;
+    sc.texels = (const float4 *)s->d_texels;
+    sc.uv6 = (const float *)s->d_uv6;
     const rg_camera &cam = s->camera;
     sc.half_w = (float)W * 0.5f;
     sc.half_h = (float)H * 0.5f;
@@ -320,6 +350,87 @@ int rg_scene_set_shading(rg_scene *scene, const float *n9, const uint32_t *obj_o
     s->has_shading = true;
     s->pt_dirty = true;
     s->tris_dirty = s->built;
+    return RG_OK;
+}
+
+// sRGB -> linear companding of Texture::sample (texture.cpp:203-206)
+static float srgb_to_linear(float c) {
+    if (c <= 0.04045f) return c * (1.0f / 12.92f);
+    return powf((c + 0.055f) * (1.0f / 1.055f), 2.4f);
+}
+
+int rg_scene_set_textures(rg_scene *scene, const rg_texture *tex, uint32_t n_tex, const int32_t *tex_of_obj,
+                          const float *tiling_of_obj, const float *uv6) {
+    Scene *s = reinterpret_cast<Scene *>(scene);
+    if (!s) return set_error(RG_E_INVALID, "null scene");
+    if (n_tex == 0) {
+        s->tex.clear();
+        s->tex_rgb.clear();
+        s->tex_linear.clear();
+        s->tex_offset.clear();
+        s->tex_of_obj.clear();
+        s->tex_tiling.clear();
+        s->uv6.clear();
+        s->pt_dirty = true;
+        return RG_OK;
+    }
+    if (!s->has_shading) return set_error(RG_E_INVALID, "rg_scene_set_shading must come first");
+    const size_t n_obj = s->objects.size();
+    if (!tex || !tex_of_obj || !tiling_of_obj || (s->n_tris && !uv6)) return set_error(RG_E_INVALID, "null argument");
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < n_tex; k++) {
+        if (tex[k].w == 0 || tex[k].h == 0 || tex[k].w > 65536 || tex[k].h > 65536 || !tex[k].rgb)
+            return set_error(RG_E_INVALID, "texture " + std::to_string(k) + " is empty or too large");
+        total += (uint64_t)tex[k].w * tex[k].h;
+    }
+    if (total > (1ull << 30)) return set_error(RG_E_INVALID, "textures exceed 2^30 texels");
+    for (size_t o = 0; o < n_obj; o++)
+        if (tex_of_obj[o] < -1 || tex_of_obj[o] >= (int32_t)n_tex)
+            return set_error(RG_E_INVALID, "texture index out of range");
+    s->tex.assign(tex, tex + n_tex);
+    s->tex_rgb.clear();
+    s->tex_linear.clear();
+    s->tex_offset.clear();
+    s->tex_rgb.reserve(3 * total);
+    s->tex_linear.reserve(4 * total);
+    for (uint32_t k = 0; k < n_tex; k++) {
+        const rg_texture &t = tex[k];
+        s->tex[k].rgb = nullptr;
+        s->tex_offset.push_back((uint32_t)(s->tex_linear.size() / 4));
+        const size_t m = (size_t)t.w * t.h;
+        s->tex_rgb.insert(s->tex_rgb.end(), t.rgb, t.rgb + 3 * m);
+        for (size_t i = 0; i < m; i++) {
+            for (int c = 0; c < 3; c++) {
+                const float v = t.rgb[3 * i + c];
+                s->tex_linear.push_back(t.srgb ? srgb_to_linear(v) : v);
+            }
+            s->tex_linear.push_back(1.0f);
+        }
+    }
+    s->tex_of_obj.assign(tex_of_obj, tex_of_obj + n_obj);
+    s->tex_tiling.assign(tiling_of_obj, tiling_of_obj + 2 * n_obj);
+    s->uv6.assign(uv6, uv6 + 6 * (size_t)s->n_tris);
+    s->pt_dirty = true;
+    return RG_OK;
+}
+
+int rg_scene_export_textures(const rg_scene *scene, uint32_t *n_tex, rg_texture *tex, float *texels,
+                             int32_t *tex_of_obj, float *tiling_of_obj, float *uv6) {
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (!s || !n_tex) return set_error(RG_E_INVALID, "null argument");
+    *n_tex = (uint32_t)s->tex.size();
+    if (tex)
+        for (size_t k = 0; k < s->tex.size(); k++) tex[k] = s->tex[k];
+    if (texels && !s->tex_rgb.empty()) std::memcpy(texels, s->tex_rgb.data(), s->tex_rgb.size() * 4);
+    const size_t n_obj = s->objects.size();
+    if (tex_of_obj)
+        for (size_t o = 0; o < n_obj; o++) tex_of_obj[o] = o < s->tex_of_obj.size() ? s->tex_of_obj[o] : -1;
+    if (tiling_of_obj)
+        for (size_t o = 0; o < 2 * n_obj; o++) tiling_of_obj[o] = o < s->tex_tiling.size() ? s->tex_tiling[o] : 1.0f;
+    if (uv6) {
+        if (s->uv6.size() == 6 * (size_t)s->n_tris) std::memcpy(uv6, s->uv6.data(), s->uv6.size() * 4);
+        else std::memset(uv6, 0, 6 * (size_t)s->n_tris * 4);
+    }
     return RG_OK;
 }
 

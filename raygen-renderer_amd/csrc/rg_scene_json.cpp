@@ -91,16 +91,20 @@ static Affine euler(const double deg[3]) {
 struct MeshData {
     std::vector<float> v;   // triangle soup: 9 floats per triangle
     std::vector<float> n;   // 9 floats per triangle
+    std::vector<float> uv;  // first UV set: 6 floats per triangle (zeros without UVs)
 };
 
+static const float kNoUv[2] = {0.0f, 0.0f};
+
 static void push_tri(MeshData &m, const float *a, const float *b, const float *c, const float *na, const float *nb,
-                     const float *nc) {
+                     const float *nc, const float *ta = kNoUv, const float *tb = kNoUv, const float *tc = kNoUv) {
     m.v.insert(m.v.end(), a, a + 3);
     m.v.insert(m.v.end(), b, b + 3);
     m.v.insert(m.v.end(), c, c + 3);
     m.n.insert(m.n.end(), na, na + 3);
     m.n.insert(m.n.end(), nb, nb + 3);
     m.n.insert(m.n.end(), nc, nc + 3);
+    m.uv.insert(m.uv.end(), {ta[0], ta[1], tb[0], tb[1], tc[0], tc[1]});
 }
 
 // PlaneMesh (polygons.cpp:13-59): unit plane in XZ, normal +Y
@@ -108,9 +112,10 @@ static MeshData plane_mesh() {
     MeshData m;
     const float v[6][3] = {{-0.5f, 0, -0.5f}, {-0.5f, 0, 0.5f}, {0.5f, 0, -0.5f},
                            {-0.5f, 0, 0.5f},  {0.5f, 0, 0.5f},  {0.5f, 0, -0.5f}};
+    const float t[6][2] = {{0, 0}, {0, 1}, {1, 0}, {0, 1}, {1, 1}, {1, 0}};
     const float n[3] = {0, 1, 0};
-    push_tri(m, v[0], v[1], v[2], n, n, n);
-    push_tri(m, v[3], v[4], v[5], n, n, n);
+    push_tri(m, v[0], v[1], v[2], n, n, n, t[0], t[1], t[2]);
+    push_tri(m, v[3], v[4], v[5], n, n, n, t[3], t[4], t[5]);
     return m;
 }
 
@@ -121,14 +126,17 @@ static MeshData cube_mesh() {
                            {-0.5f, -0.5f, 0.5f},  {0.5f, -0.5f, 0.5f},  {0.5f, 0.5f, 0.5f},  {-0.5f, 0.5f, 0.5f}};
     const int f[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 4, 7, 3}, {1, 5, 6, 2}, {3, 2, 6, 7}, {0, 1, 5, 4}};
     const float nn[6][3] = {{0, 0, -1}, {0, 0, 1}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, -1, 0}};
+    const float t[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
     for (int i = 0; i < 6; i++) {
-        push_tri(m, p[f[i][0]], p[f[i][1]], p[f[i][2]], nn[i], nn[i], nn[i]);
-        push_tri(m, p[f[i][0]], p[f[i][2]], p[f[i][3]], nn[i], nn[i], nn[i]);
+        push_tri(m, p[f[i][0]], p[f[i][1]], p[f[i][2]], nn[i], nn[i], nn[i], t[0], t[1], t[2]);
+        push_tri(m, p[f[i][0]], p[f[i][2]], p[f[i][3]], nn[i], nn[i], nn[i], t[0], t[2], t[3]);
     }
     return m;
 }
 
-// SphereMesh (polygons.cpp:132-180)
+// SphereMesh (polygons.cpp:132-180).  Its texcoords are never read: the
+// constructor leaves uvCount at 0, so transformObject keeps zero UVs
+// (rayrenderer.cpp:270-272; mesh.h:92).
 static MeshData sphere_mesh(float radius, int stacks, int slices) {
     MeshData m;
     std::vector<float> pos, nrm;
@@ -241,7 +249,15 @@ static bool load_mesh_file(const std::string &path, MeshData &m, std::string &er
         }
         off += norms.size() * 4;
     }
-    if (has_t && uvcount > 0) off += 8 * (size_t)vcount * uvcount;
+    std::vector<float> uvs;
+    if (has_t && uvcount > 0) {
+        uvs.resize(2 * (size_t)vcount);  // the first of uvcount sets
+        if (!rd(off, uvs.data(), uvs.size() * 4)) {
+            err = path + ": truncated texcoords";
+            return false;
+        }
+        off += 8 * (size_t)vcount * uvcount;
+    }
     if (has_tb) off += 24 * (size_t)vcount;
     if (has_c) off += 12 * (size_t)vcount;
     std::vector<uint32_t> idx(icount);
@@ -260,7 +276,9 @@ static bool load_mesh_file(const std::string &path, MeshData &m, std::string &er
                 return false;
             }
         push_tri(m, &verts[3 * i[0]], &verts[3 * i[1]], &verts[3 * i[2]], has_n ? &norms[3 * i[0]] : zero,
-                 has_n ? &norms[3 * i[1]] : zero, has_n ? &norms[3 * i[2]] : zero);
+                 has_n ? &norms[3 * i[1]] : zero, has_n ? &norms[3 * i[2]] : zero,
+                 uvs.empty() ? kNoUv : &uvs[2 * i[0]], uvs.empty() ? kNoUv : &uvs[2 * i[1]],
+                 uvs.empty() ? kNoUv : &uvs[2 * i[2]]);
     }
     return true;
 }
@@ -352,7 +370,7 @@ static bool load_obj(const std::string &path, std::vector<ObjPart> &parts, std::
     }
     const size_t slash = path.rfind('/');
     const std::string dir = slash == std::string::npos ? "" : path.substr(0, slash + 1);
-    std::vector<float> vs, ns;
+    std::vector<float> vs, ns, ts;
     std::vector<ObjPart> done;
     ObjPart cur;
     bool cur_n = false, cur_t = false, first_face = true, cur_error = false, head = true, mm = false;
@@ -394,9 +412,13 @@ static bool load_obj(const std::string &path, std::vector<ObjPart> &parts, std::
             float x = 0, y = 0, z = 0;
             std::sscanf(line.c_str() + 3, "%f %f %f", &x, &y, &z);
             ns.insert(ns.end(), {x, y, z});
+        } else if (is_tag(line, "vt")) {
+            float u = 0, v = 0;
+            std::sscanf(line.c_str() + 3, "%f %f", &u, &v);
+            ts.insert(ts.end(), {u, 1.0f - v});  // OBJ's V origin is the image bottom
         } else if (is_tag(line, "f")) {
             if (cur_error) continue;
-            std::vector<long> vi, ni;
+            std::vector<long> vi, ni, ti;
             bool has_t = false, has_n = false, ok = true;
             const char *p = line.c_str() + 2;
             while (*p && ok) {
@@ -427,7 +449,7 @@ static bool load_obj(const std::string &path, std::vector<ObjPart> &parts, std::
                         }
                     }
                 }
-                (void)t;
+                if (gt) ti.push_back(index(t, ts.size() / 2));
                 // every face of an object carries the same attribute set
                 if (gt ? (!cur_t && !first_face) : cur_t) ok = false;
                 if (gn ? (!cur_n && !first_face) : cur_n) ok = false;
@@ -443,7 +465,9 @@ static bool load_obj(const std::string &path, std::vector<ObjPart> &parts, std::
             for (long n : ni)
                 if (n < 0 || (size_t)n >= ns.size() / 3) ok = false;
             if (has_n && ni.size() != vi.size()) ok = false;
-            (void)has_t;
+            for (long t : ti)
+                if (t < 0 || (size_t)t >= ts.size() / 2) ok = false;
+            if (has_t && ti.size() != vi.size()) ok = false;
             if (!ok) {
                 warnings.push_back(path + ": invalid face at line " + std::to_string(line_no));
                 cur_error = true;
@@ -453,7 +477,8 @@ static bool load_obj(const std::string &path, std::vector<ObjPart> &parts, std::
             for (size_t i = 1; i + 1 < vi.size(); i++) {
                 const float *a = &vs[3 * vi[0]], *b = &vs[3 * vi[i]], *c = &vs[3 * vi[i + 1]];
                 push_tri(cur.mesh, a, b, c, has_n ? &ns[3 * ni[0]] : zero, has_n ? &ns[3 * ni[i]] : zero,
-                         has_n ? &ns[3 * ni[i + 1]] : zero);
+                         has_n ? &ns[3 * ni[i + 1]] : zero, has_t ? &ts[2 * ti[0]] : kNoUv,
+                         has_t ? &ts[2 * ti[i]] : kNoUv, has_t ? &ts[2 * ti[i + 1]] : kNoUv);
             }
             first_face = false;
         } else if (is_tag(line, "o")) {
@@ -480,6 +505,66 @@ static bool load_obj(const std::string &path, std::vector<ObjPart> &parts, std::
     }
     finalize();
     parts = std::move(done);
+    return true;
+}
+
+bool decode_jpeg(const uint8_t *data, size_t size, uint32_t &w, uint32_t &h, std::vector<uint8_t> &rgb,
+                 std::string &err);
+bool read_rgbe(const std::string &path, uint32_t &w, uint32_t &h, std::vector<float> &rgb, std::string &err);
+
+// Binary PPM (P6, maxval <= 255).
+static bool read_ppm(const std::string &buf, uint32_t &w, uint32_t &h, std::vector<uint8_t> &rgb) {
+    std::istringstream ss(buf);
+    std::string magic;
+    int W = 0, H = 0, mx = 0;
+    ss >> magic;
+    auto skip = [&]() {
+        while (ss >> std::ws && ss.peek() == '#') {
+            std::string c;
+            std::getline(ss, c);
+        }
+    };
+    skip(); ss >> W; skip(); ss >> H; skip(); ss >> mx;
+    if (magic != "P6" || W <= 0 || H <= 0 || mx <= 0 || mx > 255) return false;
+    ss.get();
+    const size_t at = (size_t)ss.tellg();
+    if (at + 3 * (size_t)W * H > buf.size()) return false;
+    rgb.assign(buf.begin() + (long)at, buf.begin() + (long)(at + 3 * (size_t)W * H));
+    w = (uint32_t)W;
+    h = (uint32_t)H;
+    return true;
+}
+
+// A material texture as Texture::loadFromFile reads it (texture.cpp:177-195):
+// Radiance .hdr as linear floats, other images as sRGB bytes / 255.  JPEG
+// (our decoder) and binary PPM are read; other formats are reported.
+static bool load_texture(const std::string &path, uint32_t &w, uint32_t &h, std::vector<float> &rgb, bool &srgb,
+                         std::string &err) {
+    const size_t dot = path.rfind('.');
+    std::string ext = dot == std::string::npos ? "" : path.substr(dot);
+    for (char &ch : ext) ch = (char)std::tolower((unsigned char)ch);
+    if (ext == ".hdr") {
+        srgb = false;
+        return read_rgbe(path, w, h, rgb, err);
+    }
+    std::string buf;
+    if (!read_file(path, buf)) {
+        err = "cannot read " + path;
+        return false;
+    }
+    std::vector<uint8_t> px;
+    bool ok = false;
+    if (ext == ".jpg" || ext == ".jpeg") ok = decode_jpeg((const uint8_t *)buf.data(), buf.size(), w, h, px, err);
+    else if (ext == ".ppm") {
+        ok = read_ppm(buf, w, h, px);
+        if (!ok) err = path + ": not a binary PPM";
+    } else {
+        err = "texture format not read on this path: " + path;
+    }
+    if (!ok) return false;
+    srgb = true;
+    rgb.resize(px.size());
+    for (size_t i = 0; i < px.size(); i++) rgb[i] = (float)px[i] / 255.0f;
     return true;
 }
 
@@ -581,14 +666,25 @@ bool read_rgbe(const std::string &path, uint32_t &w, uint32_t &h, std::vector<fl
 // ---------------------------------------------------------------------------
 namespace {
 
+// A scene object's Material as far as this path reads it: the shading
+// fields, the albedo texture path and texTiling (material.h:20-70).
+struct Mat {
+    rg_object o;
+    std::string tex;
+    float tiling[2] = {1.0f, 1.0f};
+    Mat() { rg_object_default(&o); }
+};
+
 struct Loader {
     std::string base;
-    std::vector<float> v9, n9;
+    std::vector<float> v9, n9, uv6;
     std::vector<uint32_t> obj_of_tri;
     std::vector<rg_object> objects;
+    std::vector<std::string> obj_tex;      // texture path per object ("" none)
+    std::vector<float> obj_tiling;         // 2 per object
     std::vector<rg_medium> media;
     int32_t global_medium = -1;
-    std::vector<std::vector<std::pair<std::string, rg_object>>> scopes;  // _materials per level
+    std::vector<std::vector<std::pair<std::string, Mat>>> scopes;  // _materials per level
     bool has_camera = false;
     rg_camera camera{75.0f, 1.0f, 0.0f, 1.8f, 0, 0.0f, 50.0f, 0.0f};
     Affine view;  // world -> render
@@ -637,7 +733,8 @@ struct Loader {
     }
 
     // SceneJsonLoader::readMaterial (sceneloader.cpp:101-165)
-    void read_material(rg_object &m, const json::Value &o) {
+    void read_material(Mat &mt, const json::Value &o) {
+        rg_object &m = mt.o;
         num(o, "emission", m.emission);
         num(o, "glossy", m.glossy);
         num(o, "roughness", m.roughness);
@@ -653,7 +750,16 @@ struct Loader {
             for (int k = 0; k < 3; k++)
                 if (c->arr[k].is_number()) m.color[k] = (float)c->arr[k].num;
         const json::Value *t = o.get("tex");
-        if (t && t->is_string() && !t->str.empty()) warnings.push_back("texture ignored: " + t->str);
+        if (t && t->is_string() && !t->str.empty()) mt.tex = resolve(t->str);
+        if (const json::Value *tt = o.get("texTiling")) {
+            if (tt->is_array() && tt->arr.size() >= 2) {
+                for (int k = 0; k < 2; k++)
+                    if (tt->arr[k].is_number()) mt.tiling[k] = (float)tt->arr[k].num;
+            } else if (tt->is_object()) {
+                num(*tt, "x", mt.tiling[0]);
+                num(*tt, "y", mt.tiling[1]);
+            }
+        }
         // read by the reference but outside the -d 5 path kept here
         float cd = 0.0f, sr = 0.0f;
         num(o, "chromaDispersion", cd);
@@ -663,7 +769,7 @@ struct Loader {
         if (o.get("normalmap")) warnings.push_back("normal map ignored");
     }
 
-    const rg_object *find_material(const std::string &name) const {
+    const Mat *find_material(const std::string &name) const {
         for (auto s = scopes.rbegin(); s != scopes.rend(); ++s)
             for (auto it = s->rbegin(); it != s->rend(); ++it)
                 if (it->first == name) return &it->second;
@@ -728,31 +834,35 @@ struct Loader {
     // sceneloader.cpp:424-478), in the order the reference adds them.
     struct Child {
         const json::Value *json = nullptr;
-        rg_object mat;
+        Mat mat;
         MeshData mesh;
     };
 
     // .obj as child objects; `parent_mat` is the loading object's material
     // at the point its "mesh" key is read
-    bool read_obj(const std::string &path, const rg_object &parent_mat, std::vector<Child> &out) {
+    bool read_obj(const std::string &path, const Mat &parent_mat, std::vector<Child> &out) {
         std::vector<ObjPart> parts;
         std::vector<ObjMtl> mtls;
         if (!load_obj(path, parts, mtls, warnings, err)) return false;
         for (ObjPart &p : parts) {
             Child c;
             // _materials override by name > .mtl defaults > the parent's mat
-            const rg_object *over = p.mat_name.empty() ? nullptr : find_material(p.mat_name);
+            const Mat *over = p.mat_name.empty() ? nullptr : find_material(p.mat_name);
             if (over) {
                 c.mat = *over;
             } else if (p.mtl >= 0) {
+                // applyObjMaterialDefaults (sceneloader.cpp:405-422)
                 const ObjMtl &m = mtls[(size_t)p.mtl];
-                rg_object_default(&c.mat);
-                for (int k = 0; k < 3; k++) c.mat.color[k] = m.kd[k];
-                c.mat.transparency = m.transparency;
-                if (!m.texture.empty()) warnings.push_back("texture ignored: " + m.texture);
+                c.mat = Mat();
+                for (int k = 0; k < 3; k++) c.mat.o.color[k] = m.kd[k];
+                c.mat.o.transparency = m.transparency;
+                if (!m.texture.empty()) {
+                    const size_t sl = path.rfind('/');
+                    c.mat.tex = (sl == std::string::npos ? std::string() : path.substr(0, sl + 1)) + m.texture;
+                }
             } else {
                 c.mat = parent_mat;
-                c.mat.interior_medium = -1;
+                c.mat.o.interior_medium = -1;
             }
             c.mesh = std::move(p.mesh);
             out.push_back(std::move(c));
@@ -761,7 +871,7 @@ struct Loader {
     }
 
     // mesh property of an object (sceneloader.cpp:618-641, 390-403)
-    bool read_meshes(const json::Value &val, std::vector<MeshData> &out, const rg_object &cur_mat,
+    bool read_meshes(const json::Value &val, std::vector<MeshData> &out, const Mat &cur_mat,
                      std::vector<Child> &kids) {
         auto from_path = [&](const std::string &rel) -> bool {
             const std::string p = resolve(rel);
@@ -809,8 +919,7 @@ struct Loader {
             if (mats->is_object())
                 for (auto &p : mats->obj)
                     if (p.second.is_object()) {
-                        rg_object m;
-                        rg_object_default(&m);
+                        Mat m;
                         read_material(m, p.second);
                         scopes.back().emplace_back(p.first, m);
                     }
@@ -819,8 +928,7 @@ struct Loader {
         vec3_of(o.get("angle"), ang);
         vec3_of(o.get("scale"), sc);
         const Affine model = parent.then_after(translation(loc).then_after(euler(ang).then_after(scaling(sc))));
-        rg_object mat;
-        rg_object_default(&mat);
+        Mat mat;
         int32_t interior = -1;
         std::vector<MeshData> meshes;
         std::vector<Child> children;
@@ -835,7 +943,7 @@ struct Loader {
                 if (!read_meshes(val, meshes, mat, children)) return false;
             } else if (key == "mat") {
                 if (val.is_string()) {
-                    const rg_object *m = find_material(val.str);
+                    const Mat *m = find_material(val.str);
                     if (m) mat = *m;
                     else warnings.push_back("unknown material '" + val.str + "'");
                 } else if (val.is_object()) {
@@ -868,10 +976,9 @@ struct Loader {
             scopes.pop_back();
             return true;
         }
-        mat.interior_medium = interior;
+        mat.o.interior_medium = interior;
         if (!meshes.empty()) {
-            const uint32_t oid = (uint32_t)objects.size();
-            objects.push_back(mat);
+            const uint32_t oid = add_object(mat);
             pending.push_back({model, oid, std::move(meshes)});
         }
         for (Child &c : children) {
@@ -879,8 +986,7 @@ struct Loader {
                 if (!object(*c.json, model, false)) return false;
             } else {
                 // an OBJ object: a child with the identity transform
-                const uint32_t oid = (uint32_t)objects.size();
-                objects.push_back(c.mat);
+                const uint32_t oid = add_object(c.mat);
                 std::vector<MeshData> one;
                 one.push_back(std::move(c.mesh));
                 pending.push_back({model, oid, std::move(one)});
@@ -888,6 +994,13 @@ struct Loader {
         }
         scopes.pop_back();
         return true;
+    }
+
+    uint32_t add_object(const Mat &m) {
+        objects.push_back(m.o);
+        obj_tex.push_back(m.tex);
+        obj_tiling.insert(obj_tiling.end(), {m.tiling[0], m.tiling[1]});
+        return (uint32_t)objects.size() - 1;
     }
 
     struct Pending {
@@ -919,6 +1032,7 @@ struct Loader {
                         for (int c = 0; c < 3; c++) n9.push_back(l > 0 ? (float)(r[c] / l) : 0.0f);
                     }
                     obj_of_tri.push_back(p.obj);
+                    for (int k = 0; k < 6; k++) uv6.push_back(t < m.uv.size() / 6 ? m.uv[6 * t + k] : 0.0f);
                 }
             }
         }
@@ -972,6 +1086,48 @@ extern "C" int rg_scene_load_json(rg_ctx *ctx, const char *path, rg_scene **out)
         } else {
             L.warnings.push_back("only .hdr environments are read on this path: " + L.env_path);
         }
+    }
+    if (!rc) {
+        // material textures, one per distinct path; a texture that cannot be
+        // read is dropped and its objects keep their flat colour, as
+        // SceneResourcePool::getTexture drops a failed decode (scene.cpp:656-664)
+        std::vector<std::string> paths;
+        std::vector<std::vector<float>> texels;
+        std::vector<rg_texture> tex;
+        std::vector<int32_t> tex_of_obj(L.objects.size(), -1);
+        for (size_t o = 0; o < L.objects.size(); o++) {
+            const std::string &tp = L.obj_tex[o];
+            if (tp.empty()) continue;
+            size_t k = 0;
+            while (k < paths.size() && paths[k] != tp) k++;
+            if (k == paths.size()) {
+                paths.push_back(tp);
+                texels.emplace_back();
+                uint32_t w = 0, h = 0;
+                bool srgb = true;
+                std::string err;
+                if (load_texture(tp, w, h, texels.back(), srgb, err)) {
+                    tex.push_back(rg_texture{w, h, nullptr, srgb ? 1 : 0, 0});
+                } else {
+                    L.warnings.push_back("texture not loaded: " + err);
+                    tex.push_back(rg_texture{0, 0, nullptr, 0, 0});
+                }
+            }
+            if (tex[k].w) tex_of_obj[o] = (int32_t)k;
+        }
+        // compact away the textures that failed
+        std::vector<rg_texture> used;
+        std::vector<int32_t> remap(tex.size(), -1);
+        for (size_t k = 0; k < tex.size(); k++)
+            if (tex[k].w) {
+                remap[k] = (int32_t)used.size();
+                used.push_back(tex[k]);
+                used.back().rgb = texels[k].data();
+            }
+        for (int32_t &t : tex_of_obj) t = t >= 0 ? remap[(size_t)t] : -1;
+        if (!used.empty())
+            rc = rg_scene_set_textures(sc, used.data(), (uint32_t)used.size(), tex_of_obj.data(),
+                                       L.obj_tiling.data(), L.uv6.data());
     }
     if (rc) {
         rg_scene_destroy(sc);

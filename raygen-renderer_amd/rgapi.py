@@ -92,6 +92,8 @@ def lib():
             "rg_server_get_tune": (i32, [vp, vp]),
             "rg_server_destroy": (i32, [vp]),
             "rg_bloom_default": (None, [vp]),
+            "rg_scene_set_textures": (i32, [vp, vp, u32, vp, vp, vp]),
+            "rg_scene_export_textures": (i32, [vp, vp, vp, vp, vp, vp, vp]),
             "rg_postprocess": (i32, [vp, u32, u32, vp, vp, vp]),
             "rg_postprocess_device": (i32, [vp, u32, u32, vp, vp, vp]),
         }
@@ -291,6 +293,12 @@ def make_camera(d):
                   d.get("view_far", 50.0), 0.0)
 
 
+class Texture(ctypes.Structure):
+    """rg_texture: RGB floats as stored (LDR byte / 255), srgb flag."""
+    _fields_ = [("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("rgb", ctypes.c_void_p), ("srgb", ctypes.c_int32),
+                ("pad", ctypes.c_uint32)]
+
+
 class RenderScene(Scene):
     """A scene dict (see scenes.py) made resident on the device: geometry,
     BVH, shading, media, environment and camera."""
@@ -315,6 +323,16 @@ class RenderScene(Scene):
             e = Env(rgb.shape[1], rgb.shape[0], rgb.ctypes.data, float(env.get("intensity", 1.0)),
                     float(env.get("rotation_deg", 0.0)))
             check(lib().rg_scene_set_env(self.h, ctypes.byref(e)))
+        texs = sc.get("textures")
+        if texs:
+            rgbs = [np.ascontiguousarray(t["rgb"], np.float32) for t in texs]
+            tab = (Texture * len(texs))(*[Texture(r.shape[1], r.shape[0], r.ctypes.data,
+                                                  1 if t.get("srgb", True) else 0, 0) for r, t in zip(rgbs, texs)])
+            too = np.ascontiguousarray(sc["tex_of_obj"], np.int32)
+            til = np.ascontiguousarray(sc["tiling"], np.float32)
+            uv6 = np.ascontiguousarray(sc["uv6"], np.float32)
+            self._keep += [rgbs, tab, too, til, uv6]
+            check(lib().rg_scene_set_textures(self.h, tab, len(texs), _ptr(too), _ptr(til), _ptr(uv6)))
         cam = make_camera(sc["camera"])
         check(lib().rg_scene_set_camera(self.h, ctypes.byref(cam)))
         self.build(width=width)
@@ -350,6 +368,23 @@ def load_json(ctx, path):
          "global_medium": gm.value, "warnings": [w for w in buf.value.decode().split("\n") if w]}
     if env.w and env.h:
         d["env"] = {"rgb": env_rgb, "intensity": env.intensity, "rotation_deg": env.rotation_deg}
+    ntex = ctypes.c_uint32()
+    check(lib().rg_scene_export_textures(h, ctypes.byref(ntex), None, None, None, None, None))
+    if ntex.value:
+        tab = (Texture * ntex.value)()
+        check(lib().rg_scene_export_textures(h, ctypes.byref(ntex), tab, None, None, None, None))
+        texels = np.zeros(sum(3 * t.w * t.h for t in tab), np.float32)
+        too = np.zeros(no.value, np.int32)
+        til = np.zeros((no.value, 2), np.float32)
+        uv6 = np.zeros((nt.value, 6), np.float32)
+        check(lib().rg_scene_export_textures(h, ctypes.byref(ntex), tab, _ptr(texels), _ptr(too), _ptr(til),
+                                             _ptr(uv6)))
+        texs, at = [], 0
+        for t in tab:
+            m = 3 * t.w * t.h
+            texs.append({"rgb": texels[at:at + m].reshape(t.h, t.w, 3), "srgb": bool(t.srgb)})
+            at += m
+        d.update(textures=texs, tex_of_obj=too, tiling=til, uv6=uv6)
     return sc, d
 
 

@@ -39,24 +39,37 @@ def material(color=(0.8, 0.8, 0.8), glossy=0.0, roughness=0.5, transparency=0.0,
 
 class Builder:
     def __init__(self):
-        self.v, self.n, self.o, self.objects = [], [], [], []
+        self.v, self.n, self.o, self.uv, self.objects = [], [], [], [], []
+        self.textures, self.tex_of_obj, self.tiling = [], [], []
 
-    def add_object(self, mat):
+    def add_object(self, mat, texture=-1, tiling=(1.0, 1.0)):
+        """One SceneObject; texture = index into add_texture's table (-1
+        none), tiling = its texTiling."""
         self.objects.append(mat)
+        self.tex_of_obj.append(texture)
+        self.tiling.append(tiling)
         return len(self.objects) - 1
 
-    def quad(self, obj, p0, p1, p2, p3, normal=None):
+    def add_texture(self, rgb, srgb=True):
+        """An albedo texture: (H, W, 3) floats as stored (LDR byte / 255),
+        row 0 at the top."""
+        self.textures.append({"rgb": np.ascontiguousarray(rgb, np.float32), "srgb": srgb})
+        return len(self.textures) - 1
+
+    def quad(self, obj, p0, p1, p2, p3, normal=None, uv=((0, 0), (0, 0), (0, 0), (0, 0))):
         """Two triangles (p0,p1,p2), (p0,p2,p3); flat normal from the winding
-        unless given."""
+        unless given; uv = the corners' texture coordinates."""
         p0, p1, p2, p3 = (np.asarray(p, np.float32) for p in (p0, p1, p2, p3))
         if normal is None:
             normal = np.cross(p1 - p0, p2 - p0)
             normal = normal / np.linalg.norm(normal)
         nn = np.tile(np.asarray(normal, np.float32), 3)
-        for tri in ((p0, p1, p2), (p0, p2, p3)):
+        t = [np.asarray(x, np.float32) for x in uv]
+        for tri, tt in (((p0, p1, p2), (t[0], t[1], t[2])), ((p0, p2, p3), (t[0], t[2], t[3]))):
             self.v.append(np.concatenate(tri))
             self.n.append(nn)
             self.o.append(obj)
+            self.uv.append(np.concatenate(tt))
 
     def box(self, obj, lo, hi):
         """Axis-aligned box, 12 triangles, outward flat normals (polygons.cpp:61-130 layout)."""
@@ -91,6 +104,7 @@ class Builder:
         self.v.append(pos[idx].reshape(-1, 9))
         self.n.append(nrm[idx].reshape(-1, 9))
         self.o.append(np.full(len(idx), obj, np.uint32))
+        self.uv.append(np.zeros((len(idx), 6), np.float32))  # uvCount 0: no UVs (mesh.h:92)
 
     def finish(self, camera, media=None, global_medium=-1, env=None):
         v = [np.atleast_2d(x) for x in self.v]
@@ -107,7 +121,58 @@ class Builder:
         }
         if env is not None:
             scene["env"] = env
+        if self.textures:
+            scene["textures"] = self.textures
+            scene["tex_of_obj"] = np.asarray(self.tex_of_obj, np.int32)
+            scene["tiling"] = np.asarray(self.tiling, np.float32).reshape(-1, 2)
+            scene["uv6"] = np.ascontiguousarray(np.concatenate([np.atleast_2d(x) for x in self.uv]), np.float32)
         return scene
+
+
+def checker(n, cells, a, b):
+    """n x n texture of cells x cells squares alternating colours a and b."""
+    y, x = np.mgrid[0:n, 0:n]
+    m = ((x * cells // n + y * cells // n) % 2)[..., None]
+    return np.where(m == 0, np.asarray(a, np.float32), np.asarray(b, np.float32)).astype(np.float32)
+
+
+def textured_room():
+    """The cube room with albedo textures: a checkerboard floor (sRGB, tiled
+    3 x 3), a colour-gradient back wall (sRGB, 64 x 32 texels) and a glossy
+    metallic tall box whose wood-like stripes tint its Fresnel F0 (linear
+    texture); the box faces carry no UVs, so they sample texel (0, 0), like a
+    mesh without a UV set."""
+    b = Builder()
+    z0, z1 = -4.6, -2.6
+    grey = material(color=(0.75, 0.75, 0.75))
+    t_floor = b.add_texture(checker(64, 8, (0.9, 0.9, 0.85), (0.15, 0.2, 0.3)))
+    yy, xx = np.mgrid[0:32, 0:64].astype(np.float32)
+    grad = np.stack([xx / 63.0, yy / 31.0, 1.0 - xx / 63.0 * 0.5], -1)
+    t_back = b.add_texture(np.floor(grad * 255.0) / 255.0)
+    stripes = np.tile(np.stack([0.9 - 0.3 * (np.arange(16) % 2), 0.6 + 0.0 * np.arange(16),
+                                0.3 + 0.2 * (np.arange(16) % 3 == 0)], -1)[None], (4, 1, 1))
+    t_box = b.add_texture(stripes, srgb=False)
+    floor = b.add_object(grey, texture=t_floor, tiling=(3.0, 3.0))
+    b.quad(floor, (-1, -1, z1), (1, -1, z1), (1, -1, z0), (-1, -1, z0), normal=(0, 1, 0),
+           uv=((0, 1), (1, 1), (1, 0), (0, 0)))
+    ceil = b.add_object(grey)
+    b.quad(ceil, (-1, 1, z0), (1, 1, z0), (1, 1, z1), (-1, 1, z1), normal=(0, -1, 0))
+    back = b.add_object(grey, texture=t_back, tiling=(2.0, 1.0))
+    b.quad(back, (-1, -1, z0), (1, -1, z0), (1, 1, z0), (-1, 1, z0), normal=(0, 0, 1),
+           uv=((0, 1), (1, 1), (1, 0), (0, 0)))
+    left = b.add_object(material(color=(0.75, 0.25, 0.25)))
+    b.quad(left, (-1, -1, z1), (-1, -1, z0), (-1, 1, z0), (-1, 1, z1), normal=(1, 0, 0))
+    right = b.add_object(material(color=(0.25, 0.75, 0.25)))
+    b.quad(right, (1, -1, z0), (1, -1, z1), (1, 1, z1), (1, 1, z0), normal=(-1, 0, 0))
+    tall = b.add_object(material(color=(0.95, 0.9, 0.8), glossy=0.8, roughness=0.3, metallic=1.0),
+                        texture=t_box)
+    b.box(tall, (-0.65, -1.0, -4.2), (-0.1, 0.2, -3.65))
+    short = b.add_object(material(color=(0.7, 0.7, 0.7)))
+    b.box(short, (0.1, -1.0, -3.5), (0.65, -0.4, -2.95))
+    light = b.add_object(material(color=(1.0, 1.0, 1.0), emission=15.0))
+    b.quad(light, (-0.25, 0.999, -3.85), (0.25, 0.999, -3.85), (0.25, 0.999, -3.35), (-0.25, 0.999, -3.35),
+           normal=(0, -1, 0))
+    return b.finish(camera={"fov_deg": 45.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0})
 
 
 def cube_room():

@@ -160,8 +160,9 @@ def test_loaded_json_scene_matches_single_lobe_restatement(ctx):
                                             ("sphereArray/sphereArray.json", 16, 6),
                                             ("mediumTest/waterCube.json", 16, 8)])
 def test_reference_scene_files_match_single_lobe_restatement(ctx, name, spp, depth):
-    """The reference's own sample scenes (loaded by rg_scene_load_json,
-    textures and environment maps absent) rendered on the GPU against the
+    """The reference's own sample scenes (loaded by rg_scene_load_json, with
+    sphereArray's checkerboard floor texture; environment maps absent)
+    rendered on the GPU against the
     oracle twin on the loader's flattened arrays.  cubeRoom's big cube sits
     on the floor with coplanar faces, where last-ulp differences of
     transcendental functions pick the other face; so the bar is per-pixel
@@ -191,3 +192,50 @@ def test_eight_wide_tree_renders_identically(ctx):
     a = rgapi.RenderScene(ctx, sc, width=4).render(st)
     b = rgapi.RenderScene(ctx, sc, width=8).render(st)
     assert np.array_equal(a["sum"], b["sum"]) and a["rays"] == b["rays"]
+
+
+def test_textured_room_matches_single_lobe_restatement(ctx):
+    """Albedo textures (diffuse floor and back wall in sRGB with tiling, a
+    glossy metallic box whose F0 is tinted by a linear texture) on the GPU
+    against the oracle twin: the texel lookups are integer/float-exact, so
+    the bar is the untextured cube room's."""
+    sc = scenes.textured_room()
+    rs = rgapi.RenderScene(ctx, sc)
+    W, H, spp = 64, 48, 32
+    say("[pt] textured room 64x48x32 vs oracle twin")
+    out = rs.render(rgapi.default_settings(width=W, height=H, spp=spp, max_depth=5))
+    want, _ = oracle.render(sc, W, H, spp, 5, mode=1, threads=8)
+    r = rel_diff(out["sum"], want)
+    assert np.mean(r > 1e-4) < 0.002, np.mean(r > 1e-4)
+    assert abs(out["sum"].mean() / want.mean() - 1) < 1e-4
+    plain = rgapi.RenderScene(ctx, scenes.cube_room()).render(
+        rgapi.default_settings(width=W, height=H, spp=spp, max_depth=5))
+    assert not np.allclose(out["sum"], plain["sum"], rtol=0.05)     # the textures change the image
+
+
+def test_texture_furnace_on_the_gpu(ctx):
+    """The oracle's texture furnace pin (a diffuse plane under a uniform sky
+    reads colour x texel per quadrant) rendered on the GPU."""
+    b = scenes.Builder()
+    tex = np.array([[[0.9, 0.2, 0.1], [0.1, 0.8, 0.3]], [[0.4, 0.4, 0.9], [0.7, 0.05, 0.6]]], np.float32)
+    t = b.add_texture(tex, srgb=True)
+    o = b.add_object(scenes.material(color=(0.8, 0.8, 0.8)), texture=t)
+    b.quad(o, (-1, 1, -3), (-1, -1, -3), (1, -1, -3), (1, 1, -3), normal=(0, 0, 1),
+           uv=((0, 0), (0, 1), (1, 1), (1, 0)))
+    sc = b.finish(camera={"fov_deg": 30.0, "aperture": 1.8}, env=scenes.constant_env(1.0, 64, 32))
+    rs = rgapi.RenderScene(ctx, sc)
+    st = rgapi.default_settings(width=40, height=40, spp=256, max_depth=8, antialias=0, firefly_clamp=0.0)
+    out = rs.render(st)
+    want, _ = oracle.render(sc, 40, 40, 256, 8, mode=1, threads=8, antialias=False, firefly_clamp=0.0)
+    np.testing.assert_allclose(out["sum"], want, rtol=1e-4, atol=1e-5)
+    lin = np.where(tex <= 0.04045, tex / 12.92, ((tex + 0.055) / 1.055) ** 2.4)
+    q = out["hdr"][8:14, 8:14, :3].reshape(-1, 3).mean(0)
+    np.testing.assert_allclose(q, 0.8 * lin[0, 0], rtol=0.02)
+
+
+def test_textured_batch_equals_single_job(ctx):
+    rs = rgapi.RenderScene(ctx, scenes.textured_room())
+    st = dict(width=48, height=40, spp=6, max_depth=5)
+    single = rs.render(rgapi.default_settings(**st))
+    outs, _ = rgapi.render_batch(ctx, [rs], [rgapi.make_job(0, **st)])
+    assert np.array_equal(outs[0]["hdr"], single["hdr"])

@@ -243,3 +243,62 @@ def test_adaptive_driver_concentrates_samples():
     assert nt.min() >= 4 and nt.max() <= 64
     assert len(np.unique(nt)) > 1          # some tiles stopped early
     assert all(v % 4 == 0 for v in nt)     # passes of adaptiveBaseSamples
+
+
+def textured_plane(tex, srgb, tiling=(1.0, 1.0)):
+    """A diffuse plane facing the camera (z = -3, x, y in [-1, 1]) under a
+    uniform environment, UV (0, 0) at its top-left corner and (1, 1) at the
+    bottom-right, carrying texture `tex`."""
+    b = scenes.Builder()
+    t = b.add_texture(tex, srgb=srgb)
+    o = b.add_object(scenes.material(color=(0.8, 0.8, 0.8)), texture=t, tiling=tiling)
+    b.quad(o, (-1, 1, -3), (-1, -1, -3), (1, -1, -3), (1, 1, -3), normal=(0, 0, 1),
+           uv=((0, 0), (0, 1), (1, 1), (1, 0)))
+    return b.finish(camera={"fov_deg": 30.0, "aperture": 1.8}, env=scenes.constant_env(1.0, 64, 32))
+
+
+def srgb_to_linear_f64(c):
+    c = np.asarray(c, np.float64)
+    return np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("srgb", [False, True])
+def test_texture_furnace_quadrants(mode, srgb):
+    """Under a uniform environment of radiance 1 a diffuse surface reflects
+    exactly its albedo (the furnace above), so every pixel of the plane reads
+    colour x texel: the 2x2 texture's top-left texel on the top-left
+    quadrant, and so on (nearest texel, V down, sRGB decoded)."""
+    tex = np.array([[[0.9, 0.2, 0.1], [0.1, 0.8, 0.3]],
+                    [[0.4, 0.4, 0.9], [0.7, 0.05, 0.6]]], np.float32)
+    sc = textured_plane(tex, srgb)
+    W = H = 40
+    n = 512
+    s, _ = oracle.render(sc, W, H, n, 8, mode=mode, threads=THREADS, antialias=False, firefly_clamp=0.0)
+    img = s / n
+    lin = srgb_to_linear_f64(tex) if srgb else tex.astype(np.float64)
+    # the plane spans the middle of the view (30 degree fov at distance 3)
+    for (r0, c0), (ty, tx) in (((8, 8), (0, 0)), ((8, 26), (0, 1)), ((26, 8), (1, 0)), ((26, 26), (1, 1))):
+        q = img[r0:r0 + 6, c0:c0 + 6].reshape(-1, 3).mean(0)
+        np.testing.assert_allclose(q, 0.8 * lin[ty, tx], rtol=0.02)
+
+
+def test_texture_tiling_repeats_and_wraps():
+    """texTiling (2, 2) repeats the texture twice across the plane; a UV
+    of exactly 1 wraps to texel 0 (the fold into [0, 1))."""
+    tex = np.array([[[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]]], np.float32)   # 1 x 2: red | green
+    sc = textured_plane(tex, False, tiling=(2.0, 1.0))
+    W = H = 48
+    s, _ = oracle.render(sc, W, H, 64, 8, mode=1, threads=THREADS, antialias=False, firefly_clamp=0.0)
+    row = (s / 64)[24]
+    # the plane fills the view: pixel centre c looks at x = (c + 0.5 - 24) *
+    # 2 tan(15 deg) * 3 / 48, u = (x + 1) / 2, texel = floor(frac(2u) * 2)
+    half = 3.0 * np.tan(np.radians(15.0))
+    x = (np.arange(W) + 0.5 - W / 2) * (2 * half / W)
+    su = 2.0 * (x + 1.0) / 2.0
+    f = su - np.floor(su)
+    want = np.floor(f * 2).astype(int)
+    away = np.abs(f * 2 - np.round(f * 2)) > 0.1          # not at a band edge
+    got = np.argmax(row, axis=1)
+    assert away.sum() > 30 and np.array_equal(got[away], want[away])
+    assert set(want[away]) == {0, 1} and (np.diff(want[away]) != 0).sum() >= 3   # four bands

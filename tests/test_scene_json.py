@@ -211,12 +211,13 @@ def test_reference_cube_room_scene():
 
 def test_reference_sphere_array_scene():
     """sphereArray.json: five Sphere.mesh instances, a camera tilted by -14
-    degrees about X with depth of field, and a textured floor (texture
-    reported, not loaded)."""
+    degrees about X with depth of field, and a floor textured with the
+    reference's checkerboard.jpg (copied beside the scene as data)."""
     ctx = rgapi.Context(-1)
     _, d = rgapi.load_json(ctx, os.path.join(REF, "sphereArray", "sphereArray.json"))
     assert len(d["obj"]) == 2 + 2 + 5 * 528
-    assert any("checkerboard" in w for w in d["warnings"])
+    assert not any("checkerboard" in w for w in d["warnings"])
+    assert len(d["textures"]) == 1 and (d["tex_of_obj"] >= 0).sum() == 1
     cam = d["camera"]
     assert np.isclose(cam["depth_of_field"], 6.2) and np.isclose(cam["aperture"], 5.6)
     # render space = R_x(+14 deg) (world - location): the floor plane's corners
