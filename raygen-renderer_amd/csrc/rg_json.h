@@ -2,6 +2,7 @@
 // reference's scene files use (// and /* */ comments, trailing commas) and
 // keeps object properties in file order.
 #pragma once
+#include <cctype>
 #include <cstdlib>
 #include <map>
 #include <memory>
@@ -118,6 +119,20 @@ class Parser {
         return out;
     }
 
+    // An object key: a string, or (as in the reference's JavaScript-style
+    // scene files, e.g. `{ stacks: 16 }`) a bare identifier.
+    std::string key() {
+        ws();
+        if (i_ < s_.size() && s_[i_] == '"') return string_lit();
+        const size_t b = i_;
+        auto ident = [](char c, bool first) {
+            return std::isalpha((unsigned char)c) || c == '_' || c == '$' || (!first && std::isdigit((unsigned char)c));
+        };
+        while (i_ < s_.size() && ident(s_[i_], i_ == b)) i_++;
+        if (i_ == b) fail("expected string");
+        return s_.substr(b, i_ - b);
+    }
+
     Value value() {
         ws();
         if (i_ >= s_.size()) fail("unexpected end");
@@ -128,7 +143,7 @@ class Parser {
             v.kind = Value::Object;
             for (;;) {
                 if (eat('}')) break;
-                std::string k = string_lit();
+                std::string k = key();
                 if (!eat(':')) fail("expected ':'");
                 v.obj.emplace_back(std::move(k), value());
                 if (eat(',')) continue;

@@ -158,11 +158,15 @@ def test_loaded_json_scene_matches_single_lobe_restatement(ctx):
 
 @pytest.mark.parametrize("name,spp,depth", [("cubeRoom/cubeRoom.json", 32, 6),
                                             ("sphereArray/sphereArray.json", 16, 6),
-                                            ("mediumTest/waterCube.json", 16, 8)])
+                                            ("mediumTest/waterCube.json", 16, 8),
+                                            ("bokehTest/bokehTest.json", 16, 6),
+                                            ("iblTest/iblTest.json", 16, 6),
+                                            ("suzanne/suzanne.json", 16, 8)])
 def test_reference_scene_files_match_single_lobe_restatement(ctx, name, spp, depth):
     """The reference's own sample scenes (loaded by rg_scene_load_json, with
-    sphereArray's checkerboard floor texture; environment maps absent)
-    rendered on the GPU against the
+    the checkerboard and wood floor textures; environment maps absent, so
+    iblTest and suzanne see only the background colour and their own
+    surfaces) rendered on the GPU against the
     oracle twin on the loader's flattened arrays.  cubeRoom's big cube sits
     on the floor with coplanar faces, where last-ulp differences of
     transcendental functions pick the other face; so the bar is per-pixel

@@ -246,6 +246,50 @@ def test_reference_medium_and_aniso_scenes():
     assert np.isclose(d["objects"]["anisotropy"].max(), 0.95)
 
 
+def test_reference_bokeh_and_ibl_scenes():
+    """bokehTest.json: an 8-blade iris rotated 10 degrees, depth of field
+    4.5, twelve sphere emitters (SphereMesh defaults 16 x 32: 1024 triangles
+    each); iblTest.json: a checkerboard floor tiled 12 x 12 (the reference's
+    texture, decoded) and a dispersive glass ball (dispersion reported)."""
+    ctx = rgapi.Context(-1)
+    _, d = rgapi.load_json(ctx, os.path.join(REF, "bokehTest", "bokehTest.json"))
+    cam = d["camera"]
+    assert cam["aperture_blades"] == 8 and np.isclose(cam["aperture_rotation_deg"], 10)
+    assert np.isclose(cam["depth_of_field"], 4.5) and np.isclose(cam["aperture"], 2.8)
+    sizes = [int((d["obj"] == k).sum()) for k in range(len(d["objects"]))]
+    assert sizes.count(1024) == 13 and sizes[:2] == [2, 2]
+    assert (d["objects"]["emission"] > 0).sum() >= 13
+    _, d = rgapi.load_json(ctx, os.path.join(REF, "iblTest", "iblTest.json"))
+    floor = int(np.nonzero(d["tex_of_obj"] >= 0)[0][0])
+    np.testing.assert_array_equal(d["tiling"][floor], [12, 12])
+    assert d["textures"][0]["rgb"].shape == (256, 256, 3)
+    assert any("dispersion" in w for w in d["warnings"])
+
+
+def test_reference_suzanne_scene_and_unquoted_keys():
+    """suzanne.json: Suzanne.mesh (checked against an independent struct
+    parse) on a floor textured with the reference's wood.jpg (1024 x 1024,
+    4:2:0 with restart intervals, decoded by our JPEG decoder); test/bokeh.json
+    uses bare identifiers as keys (`stacks: 16`), as the reference's
+    JavaScript-style scene files may."""
+    ctx = rgapi.Context(-1)
+    _, d = rgapi.load_json(ctx, os.path.join(REF, "suzanne", "suzanne.json"))
+    raw = open(os.path.join(REF, "suzanne", "Suzanne.mesh"), "rb").read()
+    tag, ver, flags, length = struct.unpack("<IHHI", raw[:12])
+    vcount, uvcount, icount = struct.unpack("<III", raw[12:24])
+    assert tag == 0x6873656D
+    ntri = icount // 3 if icount else vcount // 3
+    sizes = [int((d["obj"] == k).sum()) for k in range(len(d["objects"]))]
+    assert sizes == [2, ntri]
+    wood = d["textures"][d["tex_of_obj"][0]]["rgb"]
+    assert wood.shape == (1024, 1024, 3)
+    mean = wood.reshape(-1, 3).mean(0)
+    assert mean[0] > mean[1] > mean[2]            # a warm brown wood
+    assert wood.std() > 0.02                        # with grain, not flat
+    _, d = rgapi.load_json(ctx, os.path.join(REF, "test", "bokeh.json"))
+    sizes = [int((d["obj"] == k).sum()) for k in range(len(d["objects"]))]
+    assert sizes[0] == 2 * 16 * 32
+
 def _rgbe_decode(path):
     """Independent Radiance RLE decoder (new-style scanlines)."""
     raw = open(path, "rb").read()
