@@ -8,15 +8,13 @@
 #include <cstdint>
 
 #include "rg_internal.h"
+#include "rg_pt.h"
 
 namespace rg {
 
 // ---------------------------------------------------------------------------
 // Device helpers
 // ---------------------------------------------------------------------------
-struct RootBox {
-    float lo[3], hi[3];
-};
 
 __device__ __forceinline__ float inv_dir(float d) {
     return (fabsf(d) > 1e-20f) ? 1.0f / d : 1e30f;
@@ -92,15 +90,6 @@ __device__ __forceinline__ bool mt_any(const float4 a, const float4 b, const flo
     return t > 1e-6f && t < maxt;
 }
 
-
-// Device view of a built scene's acceleration structure.
-struct DevBvh {
-    const void *nodes;  // Node4 or Node8 array (the kernel's width template)
-    const float4 *tris;
-    RootBox root;
-    int has_tree;
-    int width;          // 4 or 8
-};
 
 // One wide node's child tests: for each slot whether the ray enters the
 // child box before tmax (and the entry distance), the child refs and the
@@ -339,10 +328,20 @@ constexpr uint32_t kNoNode = 0xFFFFFFFEu;
 // longest ray.  Inner nodes are traversed until every lane of the wave holds
 // a leaf to intersect (the first leaf a lane reaches is postponed and the
 // lane keeps descending with its current t), then all lanes intersect their
-// leaves together.  Descending with a stale, larger t only visits nodes the
-// plain traversal culls (their triangles cannot pass the strict
-// t < closest test), and leaves are intersected in the same order, so the
-// hits are those of trace_closest_dev, exact-t ties included.
+// leaves together.
+//
+// The hits are those of trace_closest_dev, exact-t ties included, whatever
+// rays share the wave: descending with the stale, larger t tests boxes
+// against a t that the plain traversal would already have lowered, so after
+// every leaf intersection the node the lane is about to enter is tested
+// again against the new t (its entry distance is kept beside it).  Node
+// boxes nest exactly and the slab test is monotone in the box bounds, so a
+// node's entry distance is at least its ancestors': if it passes, every
+// node on its path passes as well; the stack entries pushed meanwhile are
+// re-tested when popped.  Relying instead on "a triangle inside a culled
+// box cannot beat t" fails where the triangle test's t for a hit on a box
+// face rounds below the slab's entry distance (a hit on an edge shared by
+// triangles in two leaves).
 //
 // Src provides: bool load(i, bv, PsRay &) (false: nothing to trace at i; it
 // may replace bv with the ray's own tree), store_closest(i, prim, t, u, v),
@@ -367,6 +366,7 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src,
     float ix = 0, iy = 0, iz = 0, t = 0, bu = 0, bvv = 0;
     int oct = 0, sp = 0;
     uint32_t prim = 0xFFFFFFFFu, cur = kNoNode, leaf = kNoNode;
+    float cur_t = 0.0f;  // entry distance of cur (closest hit only)
     bool hit_any = false;
 
     auto pop = [&]() -> uint32_t {
@@ -375,7 +375,10 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src,
             const bool in_lds = sp < kLdsStack;
             if (kAny) return in_lds ? lc.ref[sp * kStride] : stk_ref[sp - kLdsStack];
             const float et = in_lds ? lc.t[sp * kStride] : stk_t[sp - kLdsStack];
-            if (!(et > t)) return in_lds ? lc.ref[sp * kStride] : stk_ref[sp - kLdsStack];
+            if (!(et > t)) {
+                cur_t = et;
+                return in_lds ? lc.ref[sp * kStride] : stk_ref[sp - kLdsStack];
+            }
         }
         return kNoNode;
     };
@@ -402,10 +405,12 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src,
                     float tm;
                     active = true;
                     if (bv.has_tree && slab(bv.root.lo[0], bv.root.lo[1], bv.root.lo[2], bv.root.hi[0],
-                                            bv.root.hi[1], bv.root.hi[2], r.ox, r.oy, r.oz, ix, iy, iz, t, tm))
+                                            bv.root.hi[1], bv.root.hi[2], r.ox, r.oy, r.oz, ix, iy, iz, t, tm)) {
                         cur = 0;
-                    else
+                        cur_t = tm;
+                    } else {
                         finish();
+                    }
                 }
             }
             cursor += (uint64_t)__popcll(idle);
@@ -449,7 +454,12 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src,
                     next_t = tm[s];
                     have = true;
                 }
-                cur = have ? next : pop();
+                if (have) {
+                    cur = next;
+                    cur_t = next_t;
+                } else {
+                    cur = pop();
+                }
             }
         }
         // leaf phase
@@ -473,6 +483,9 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src,
             }
             leaf = kNoNode;
             if (kAny && hit_any) cur = kNoNode;
+            // the node entered next was chosen against the t before this
+            // leaf: test it against the new t, as the plain traversal does
+            if (!kAny && cur != kNoNode && cur_t > t) cur = pop();
         }
         if (active && leaf == kNoNode && cur == kNoNode) finish();
     }

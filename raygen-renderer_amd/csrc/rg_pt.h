@@ -10,6 +10,19 @@
 
 namespace rg {
 
+struct RootBox {
+    float lo[3], hi[3];
+};
+
+// Device view of a built scene's acceleration structure.
+struct DevBvh {
+    const void *nodes;  // Node4 or Node8 array (the kernel's width template)
+    const float4 *tris;
+    RootBox root;
+    int has_tree;
+    int width;          // 4 or 8
+};
+
 // One SceneObject's material (material.h:20-70); same layout as rg_object.
 struct DevObject {
     float color[3];
@@ -111,5 +124,7 @@ int launch_generation(const Scene &scn, const PTScene &sc, const PTState &st, ui
                       const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream);
 int launch_resolve(const float *d_sum, const uint32_t *d_nspp, uint32_t width, uint32_t height, float exposure,
                    float *d_hdr, void *stream);
+// Fills n 32-bit words with value.
+int launch_fill_u32(void *p, uint32_t value, uint64_t n, void *stream);
 
 }  // namespace rg

@@ -109,6 +109,54 @@ def test_cube_room_ties_bit_exact(ctx):
     compare(ctx, v9, r, "cube_room", width=8)
 
 
+def lattice_plane(n=64):
+    """2n^2 triangles tiling [-1, 1]^2 at z = 0, vertices on the k/(n/2)
+    lattice (exact in binary32); neighbouring quads sit in different leaves."""
+    g = np.linspace(-1.0, 1.0, n + 1, dtype=np.float32)
+    q = []
+    for i in range(n):
+        for j in range(n):
+            p00, p10 = [g[i], g[j], 0.0], [g[i + 1], g[j], 0.0]
+            p01, p11 = [g[i], g[j + 1], 0.0], [g[i + 1], g[j + 1], 0.0]
+            if (i + j) & 1:
+                q += [p00 + p10 + p11, p00 + p11 + p01]
+            else:
+                q += [p00 + p10 + p01, p10 + p11 + p01]
+    return np.array(q, dtype=np.float32)
+
+
+def lattice_rays(n, seed, cells=64):
+    """Rays from above and below the lattice plane aimed exactly at lattice
+    vertices and edge midpoints: the triangle test's t on the two sides of
+    a shared edge and the slab entry of the neighbouring leaf's flat box
+    differ in the last bits."""
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, dtype=rgapi.RAY_DTYPE)
+    o = rng.uniform(-1.2, 1.2, (n, 3))
+    o[:, 2] = rng.uniform(0.05, 2.0, n) * np.where(rng.random(n) < 0.25, -1.0, 1.0)
+    half = cells // 2
+    tx = rng.integers(-cells, cells + 1, n) / (2.0 * half)
+    ty = rng.integers(-cells, cells + 1, n) / (2.0 * half)
+    tgt = np.stack([tx, ty, np.zeros(n)], axis=1)
+    r["o"] = o.astype(np.float32)
+    r["d"] = (tgt.astype(np.float32) - o.astype(np.float32))
+    r["tmax"] = np.inf
+    return r
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_shared_edges_across_leaves_bit_exact(ctx, width):
+    """Closest hits on edges shared by triangles of different leaves: the
+    persistent traversal tests the node it enters after a leaf against the
+    new t, as the plain traversal does, so the same triangle wins."""
+    v9 = lattice_plane()
+    rays = lattice_rays(200000, 0x1A77)
+    nodes, perm = oracle.bvh_build(v9)
+    _, tie = oracle.trace_closest(nodes, perm, v9, rays, ties=True)
+    say(f"[trace] lattice plane: exact-t ties flagged on {int(tie.sum())} rays")
+    compare(ctx, v9, rays, "lattice_plane", width=width)
+
+
 def test_tmax_and_tmin_respected(ctx):
     v9 = rgapi.gen_icosphere(3)
     rays = random_rays(20000, 11, tmax=0.5)
