@@ -245,14 +245,17 @@ def bench_batch(ctx, resident, n_jobs=64, seed=0xC3C30000):
     outs, stats = rgapi.render_batch(ctx, resident, jobs, tune=tune, want_sum=False)
     batch_s = time.perf_counter() - t0
     log(f"batch {batch_s:.2f}s")
-    single_s, rays, same = 0.0, 0, 0
+    single_s, rays, same, differ = 0.0, 0, 0, []
     for i, (s, kw) in enumerate(spec):
         st = rgapi.default_settings(**kw)
         t0 = time.perf_counter()
         o = resident[s].render(st, hdr_only=True)
         single_s += time.perf_counter() - t0
         rays += o["rays"] + o["shadow_rays"]
-        same += int(np.array_equal(o["hdr"], outs[i]["hdr"]))
+        eq = np.array_equal(o["hdr"], outs[i]["hdr"])
+        same += int(eq)
+        if not eq:
+            differ.append(i)
     log(f"one at a time {single_s:.2f}s")
     brays = int(stats[0]) + int(stats[1])
     return {"workload": f"{n_jobs} jobs (cube room 36 tris / BSDF frame 2,000,014 tris), W,H log-U[128,1024], "
@@ -264,7 +267,7 @@ def bench_batch(ctx, resident, n_jobs=64, seed=0xC3C30000):
             "one_at_a_time_samples_per_s": round(samples / single_s, 1),
             "batched_mrays_per_s": round(brays / batch_s / 1e6, 2),
             "one_at_a_time_mrays_per_s": round(rays / single_s / 1e6, 2),
-            "rays_equal": brays == rays, "images_bit_equal": f"{same}/{n_jobs}",
+            "rays_equal": brays == rays, "images_bit_equal": f"{same}/{n_jobs}", "images_differing": differ,
             "tune": tune.as_dict(), "autotune_s": round(tune_s, 2)}
 
 

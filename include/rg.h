@@ -302,12 +302,14 @@ int rg_render_stats(rg_ctx *ctx, uint64_t out[3]);
 /* ---- many jobs on one GPU ------------------------------------------------
  * The reference renders one job per RayRenderer::render call
  * (rayrenderer.cpp:478-596, driven by main.cpp:162-490).  The entry points
- * below run many such jobs together: one pool of path slots is refilled
- * every iteration from the samples of all admitted jobs, the iterations are
- * replayed as a captured hipGraph, and each job's image is what rg_render
- * returns for the same scene and settings (same per-sample random streams,
- * same per-pixel summation order).  Jobs render a fixed sample count:
- * settings.adaptive and settings.denoise must be 0. */
+ * below run many such jobs together: the admitted jobs' samples are packed
+ * into shared wavefront generations (each job contributes chunks of S
+ * samples of every pixel, S being rg_render's samples in flight for that
+ * job), the bounce loop runs over one compacted queue of all their paths,
+ * and each job's image is what rg_render returns for the same scene and
+ * settings (same per-sample random streams, same per-pixel summation
+ * order).  Jobs render a fixed sample count: settings.adaptive and
+ * settings.denoise must be 0. */
 typedef struct {
     uint32_t scene;        /* index into the scene list of the batch / server */
     uint32_t pad;
@@ -317,33 +319,36 @@ typedef struct {
 
 /* Engine knobs; 0 picks the default.  rg_autotune measures candidates. */
 typedef struct {
-    uint32_t pool_paths;       /* path slots in flight (default 2^21) */
+    uint32_t pool_paths;       /* path slots of one generation (default 2^25; a
+                                  generation always takes at least one chunk) */
     uint32_t stack;            /* traversal stack entries 32/64/128/256
                                   (default: smallest that fits every BVH) */
-    uint32_t tile;             /* pixel tile edge of the issue order (default 8) */
+    uint32_t tile;             /* unused, 0 or 1 */
     uint32_t blocks_per_cu;    /* cap on persistent traversal blocks per compute unit
                                   (default 8) */
-    uint32_t iters_per_graph;  /* iterations per hipGraph replay, even (default 4) */
+    uint32_t iters_per_graph;  /* nonzero: replay each generation's bounce loop from a
+                                  captured hipGraph (default 0: direct launches) */
     uint32_t pad;
 } rg_tune;
 
 /* Renders n_jobs jobs over the scenes (BVH built, shading set) and blocks
  * until all are done.  hdr_out[i] (W*H*4 floats) and sum_out[i] (W*H*3) are
  * host buffers (the arrays and their entries may be NULL).  stats (nullable):
- * [0] extension rays, [1] shadow rays, [2] samples, [3] iterations. */
+ * [0] extension rays, [1] shadow rays, [2] samples, [3] generations. */
 int rg_render_batch(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const rg_job *jobs, uint32_t n_jobs,
                     const rg_tune *tune, float *const *hdr_out, float *const *sum_out, uint64_t stats[4]);
 
-/* Times every candidate of a small grid of pool / stack / tile / blocks
- * settings on the probe jobs (NULL: one 512x384, 8 spp, depth 6 job per
- * scene) and returns the fastest. */
+/* Times every candidate of a small grid of generation size / stack / blocks
+ * per CU / graph settings on the probe jobs (NULL: one 512x384, 8 spp,
+ * depth 6 job per scene) and returns the fastest. */
 int rg_autotune(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const rg_job *probe, uint32_t n_probe,
                 rg_tune *out);
 
 /* Request server: keeps the scenes resident, admits requests first come,
  * first served as a job slot and hbm_budget bytes of framebuffer space
- * (per-sample buffer + sums + image) allow, and renders every admitted
- * request together with the others on a thread of its own. */
+ * (sums, sums of squares, image) allow, and renders every admitted request
+ * together with the others on a thread of its own: a request admitted while
+ * generations run joins the next generation. */
 typedef struct rg_server rg_server;
 typedef struct {
     int32_t status;     /* RG_OK or the failure */
@@ -361,7 +366,7 @@ int rg_server_submit(rg_server *srv, const rg_job *job, float *hdr_out, float *s
  * running.  A finished ticket is forgotten after it is reported. */
 int rg_server_wait(rg_server *srv, uint64_t ticket, double timeout_s, rg_request_info *info);
 /* [0] requests done, [1] samples done, [2] extension rays, [3] shadow rays,
- * [4] iterations, [5] framebuffer bytes in use, [6] budget, [7] graph replays */
+ * [4] generations, [5] framebuffer bytes in use, [6] budget, [7] graph replays */
 int rg_server_stats(rg_server *srv, uint64_t out[8]);
 int rg_server_get_tune(rg_server *srv, rg_tune *out);
 int rg_server_destroy(rg_server *srv);

@@ -1,8 +1,8 @@
-// rg_engine.cpp — host side of the streaming engine: job admission by HBM
-// footprint into one device arena, the issue table, hipGraph replay of the
-// captured iterations, completion tracking from a device status snapshot,
-// batch rendering (rg_render_batch), the request server (rg_server_*) and
-// the autotuner (rg_autotune).
+// rg_engine.cpp — host side of the batched engine: job admission by HBM
+// footprint into one device arena, packing of the admitted jobs' chunks into
+// shared generations, hipGraph replay of the bounce loop, completion
+// tracking by generation events, batch rendering (rg_render_batch), the
+// request server (rg_server_*) and the autotuner (rg_autotune).
 //
 // The reference renders one job per RayRenderer::render call
 // (rayrenderer.cpp:478-596); these entry points run many such jobs at once
@@ -97,18 +97,17 @@ struct Request {
     float *out_hdr = nullptr, *out_sum = nullptr, *out_sumsq = nullptr;
     uint64_t samples = 0;  // npix * spp
     uint32_t npix = 0;
+    uint32_t group = 1;    // samples per pixel per chunk (the single-job path's samples in flight)
+    uint32_t issued = 0;   // samples per pixel put into generations so far
     uint64_t bytes = 0;    // arena footprint
     uint64_t off = 0;
-    uint64_t begin = 0;
     uint32_t slot = 0;
-    std::atomic<int> state{0};  // 0 pending, 1 issued, 2 finishing, 3 done
+    std::atomic<int> state{0};  // 0 pending, 1 admitted, 2 all samples issued, 3 done
     int status = RG_OK;
     double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
-    hipEvent_t ev = nullptr;
     int pin = -1;
-    uint32_t bounces = 0;     // max_depth + 2: the longest a path of the job lives, in iterations
-    uint64_t issued_iter = ~0ull;  // iteration count when the cursor was first seen past the job
     float *d_hdr = nullptr, *d_sum = nullptr, *d_sumsq = nullptr;
+    DevBvh bvh{};
 };
 
 struct PinnedBuf {
@@ -117,10 +116,8 @@ struct PinnedBuf {
     bool busy = false;
 };
 
-// Footprint of one job in the arena: per-sample buffer, sums, HDR image.
-uint64_t job_bytes(uint64_t npix, uint64_t spp) {
-    return align_up(npix * spp * 12) + 2 * align_up(npix * 12) + align_up(npix * 16);
-}
+// Footprint of one job in the arena: sums, sums of squares, HDR image.
+uint64_t job_bytes(uint64_t npix, uint64_t) { return 2 * align_up(npix * 12) + align_up(npix * 16); }
 
 // The single-job path's samples in flight, which fixes its accumulation
 // grouping (rg_render).
@@ -128,9 +125,17 @@ uint32_t plain_group(const rg_settings &st) {
     const uint32_t npix = st.width * st.height;
     const uint32_t spp = std::max<uint32_t>(1, st.spp);
     uint32_t S = st.samples_in_flight;
-    if (S == 0) S = std::max<uint32_t>(1, (1u << 21) / npix);
+    if (S == 0) S = std::max<uint32_t>(1, kPathsInFlight / npix);
     return std::min(S, spp);
 }
+
+// A generation launched and not yet known to be finished: its event, the
+// jobs whose last samples it carried, and the counter snapshot it copies.
+struct Inflight {
+    hipEvent_t ev = nullptr;
+    int stage = 0;  // pinned chunk-table / counter staging buffer in use
+    std::vector<Request *> finishing;
+};
 
 }  // namespace
 
@@ -157,13 +162,12 @@ class Engine {
             width = s->width;
             scenes.push_back(s);
         }
-        tune = {1u << 21, 0, 8, 8, 4, (uint32_t)width};
+        tune = {kPathsInFlight, 0, 8, 0, (uint32_t)width};
         if (t) {
-            if (t->pool_paths) tune.pool = t->pool_paths;
+            if (t->pool_paths) tune.budget = t->pool_paths;
             if (t->stack) tune.stack = t->stack;
-            if (t->tile) tune.tile = t->tile;
             if (t->blocks_per_cu) tune.blocks_per_cu = t->blocks_per_cu;
-            if (t->iters_per_graph) tune.iters = t->iters_per_graph;
+            tune.graphs = t->iters_per_graph ? 1 : 0;
         }
         if (tune.stack == 0)
             for (uint32_t s : {32u, 64u, 128u, 256u})
@@ -173,33 +177,38 @@ class Engine {
                 }
         if (!stream_stack_ok(tune.stack, need))
             return set_error(RG_E_UNSUPPORTED, "traversal stack too small for the deepest scene BVH");
-        if (tune.pool < 1024 || tune.pool > (1u << 26)) return set_error(RG_E_INVALID, "pool_paths out of range");
-        if (tune.tile > 64 || tune.blocks_per_cu == 0 || tune.blocks_per_cu > 32)
-            return set_error(RG_E_INVALID, "tile or blocks_per_cu out of range");
-        if (tune.iters == 0 || (tune.iters & 1) || tune.iters > 64)
-            return set_error(RG_E_INVALID, "iters_per_graph must be even, 2..64");
+        if (tune.budget < 1024 || tune.budget > (1u << 26)) return set_error(RG_E_INVALID, "pool_paths out of range");
+        if (tune.blocks_per_cu == 0 || tune.blocks_per_cu > 32)
+            return set_error(RG_E_INVALID, "blocks_per_cu out of range");
         n_cu = stream_cu_count(c->device);
         if (n_cu <= 0) return set_error(RG_E_HIP, "cannot query compute units");
 
-        const uint64_t P = tune.pool;
+        // path slots: a generation holds up to `budget` paths, and always at
+        // least one whole chunk of the default grouping
+        cap = std::max(tune.budget, kPathsInFlight);
+        const uint64_t P = cap;
+        PTState &st = args.st;
+        std::memset(&st, 0, sizeof st);
         struct A {
             void **p;
             uint64_t bytes;
         } allocs[] = {
-            {(void **)&sp.st.ray, P * 32},    {(void **)&sp.st.beta, P * 16},  {(void **)&sp.st.tp, P * 16},
-            {(void **)&sp.st.L, P * 16},      {(void **)&sp.st.meta, P * 16},  {(void **)&sp.st.sid, P * 16},
-            {(void **)&sp.hit, P * 16},       {(void **)&sp.sray, P * 64},     {(void **)&sp.scontrib, P * 32},
-            {(void **)&sp.socc, P * 2},       {(void **)&sp.ssrc, P * 8},      {(void **)&sp.nshadow, P * 4},
-            {(void **)&sp.holes, P * 4},      {&d_jobs, (uint64_t)kMaxJobs * kDevJobBytes},
-            {&d_status, status_bytes()},      {(void **)&d_issue, (uint64_t)kMaxJobs * sizeof(IssueEntry)},
+            {(void **)&st.ray, P * 32},       {(void **)&st.hit, P * 16},      {(void **)&st.beta, P * 16},
+            {(void **)&st.tp, P * 16},        {(void **)&st.L, P * 16},        {(void **)&st.meta, P * 16},
+            {(void **)&st.sray, P * 64},      {(void **)&st.ssrc, P * 8},      {(void **)&st.socc, P * 2},
+            {(void **)&st.scontrib, P * 32},  {(void **)&st.nshadow, P * 4},   {(void **)&st.q[0], P * 4},
+            {(void **)&st.q[1], P * 4},       {(void **)&st.count, 64},        {(void **)&st.sample_out, P * 16},
+            {(void **)&st.stats, 64},         {(void **)&args.chunk_of, P * 4},
+            {&d_chunks, (uint64_t)kMaxChunks * sizeof(GenChunk)},
+            {&d_jobs, (uint64_t)kMaxJobs * kDevJobBytes},
         };
         for (auto &a : allocs) {
             if (int rc = dev_malloc(a.p, a.bytes)) return rc;
             dev_allocs.push_back(*a.p);
         }
-        sp.cap = tune.pool;
-        sp.ctr = (StreamCounters *)d_status;
-        sp.issue = d_issue;
+        args.chunks = (const GenChunk *)d_chunks;
+        args.cap = cap;
+        if (int rc = launch_fill_u32(st.stats, 0u, 16, stream)) return rc;
         if (budget == 0) budget = 16ull << 30;
         if (int rc = dev_malloc((void **)&d_arena, budget)) return rc;
         dev_allocs.push_back(d_arena);
@@ -207,30 +216,14 @@ class Engine {
 
         if (int rc = hip_ok(hipHostMalloc((void **)&h_jobs, (size_t)kMaxJobs * kDevJobBytes), "pinned alloc")) return rc;
         for (int k = 0; k < 2; k++) {
-            if (int rc = hip_ok(hipHostMalloc((void **)&h_status[k], status_bytes()), "pinned alloc")) return rc;
-            if (int rc = hip_ok(hipHostMalloc((void **)&h_issue[k], (size_t)kMaxJobs * sizeof(IssueEntry) + 64),
+            if (int rc = hip_ok(hipHostMalloc((void **)&h_chunks[k], (size_t)kMaxChunks * sizeof(GenChunk)),
                                 "pinned alloc"))
                 return rc;
-            if (int rc = hip_ok(hipEventCreateWithFlags(&status_ev[k], hipEventDisableTiming), "event")) return rc;
-            if (int rc = hip_ok(hipEventCreateWithFlags(&issue_ev[k], hipEventDisableTiming), "event")) return rc;
+            if (int rc = hip_ok(hipHostMalloc((void **)&h_stats[k], 64), "pinned alloc")) return rc;
+            if (int rc = hip_ok(hipEventCreateWithFlags(&stage_ev[k], hipEventDisableTiming), "event")) return rc;
+            if (int rc = hip_ok(hipEventRecord(stage_ev[k], stream), "event record")) return rc;
         }
-        if (int rc = hip_ok(hipHostMalloc((void **)&h_zero, 64), "pinned alloc")) return rc;
-        std::memset(h_zero, 0, 64);
         for (uint32_t s = kMaxJobs; s-- > 0;) free_slots.push_back(s);
-
-        if (int rc = stream_launch_reset(sp, stream)) return rc;
-        // capture the iterations once; every replay reuses the same arguments
-        hipGraph_t g = nullptr;
-        if (int rc = hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "begin capture")) return rc;
-        int lrc = RG_OK;
-        for (uint32_t it = 0; it < tune.iters && lrc == RG_OK; it++)
-            lrc = stream_launch_iteration(sp, d_jobs, (int)(it & 1), tune, n_cu, stream);
-        const hipError_t ce = hipStreamEndCapture(stream, &g);
-        if (lrc) return lrc;
-        if (int rc = hip_ok(ce, "end capture")) return rc;
-        const hipError_t ie = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (int rc = hip_ok(ie, "graph instantiate")) return rc;
         return hip_ok(hipStreamSynchronize(stream), "engine init");
     }
 
@@ -252,27 +245,30 @@ class Engine {
                     r->state = 3;
                     r->status = RG_E_CANCELLED;
                 }
-                if (r->ev) (void)hipEventDestroy(r->ev);
-                r->ev = nullptr;
             }
         }
         cv_done.notify_all();
-        if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
+        for (auto &g : graphs_by_bounces)
+            if (g.second) (void)hipGraphExecDestroy(g.second);
+        graphs_by_bounces.clear();
+        for (Inflight &f : inflight)
+            if (f.ev) (void)hipEventDestroy(f.ev);
+        inflight.clear();
+        for (hipEvent_t e : spare_events) (void)hipEventDestroy(e);
+        spare_events.clear();
         for (void *p : dev_allocs) dev_free(p);
         dev_allocs.clear();
-        for (char *p : {h_jobs, h_status[0], h_status[1], (char *)h_issue[0], (char *)h_issue[1], (char *)h_zero})
+        for (char *p : {h_jobs, (char *)h_chunks[0], (char *)h_chunks[1], (char *)h_stats[0], (char *)h_stats[1]})
             if (p) (void)hipHostFree(p);
-        h_jobs = h_status[0] = h_status[1] = nullptr;
-        h_issue[0] = h_issue[1] = nullptr;
-        h_zero = nullptr;
+        h_jobs = nullptr;
+        h_chunks[0] = h_chunks[1] = nullptr;
+        h_stats[0] = h_stats[1] = nullptr;
         for (PinnedBuf &b : pins)
             if (b.p) (void)hipHostFree(b.p);
         pins.clear();
         for (int k = 0; k < 2; k++) {
-            if (status_ev[k]) (void)hipEventDestroy(status_ev[k]);
-            if (issue_ev[k]) (void)hipEventDestroy(issue_ev[k]);
-            status_ev[k] = issue_ev[k] = nullptr;
+            if (stage_ev[k]) (void)hipEventDestroy(stage_ev[k]);
+            stage_ev[k] = nullptr;
         }
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
@@ -290,6 +286,8 @@ class Engine {
             return set_error(RG_E_UNSUPPORTED, "batched jobs render a fixed sample count without denoise");
         const uint64_t npix = (uint64_t)st.width * st.height;
         if (npix * st.spp >= (1ull << 32)) return set_error(RG_E_BUDGET, "job has 2^32 or more samples");
+        const uint32_t group = plain_group(st);
+        if (npix * group > cap) return set_error(RG_E_BUDGET, "job's samples in flight exceed the engine's path slots");
         const uint64_t bytes = job_bytes(npix, st.spp);
         if (bytes > arena.size()) return set_error(RG_E_BUDGET, "job footprint exceeds the engine's HBM budget");
         auto r = std::make_unique<Request>();
@@ -299,6 +297,7 @@ class Engine {
         r->out_sum = sum;
         r->out_sumsq = sumsq;
         r->npix = (uint32_t)npix;
+        r->group = group;
         r->samples = npix * st.spp;
         r->bytes = bytes;
         r->t_submit = now_s();
@@ -313,7 +312,7 @@ class Engine {
         return RG_OK;
     }
 
-    // One round of the loop: admit, replay, read status, finish, deliver.
+    // One round of the loop: admit, launch the next generation, deliver.
     int step() {
         if (ctx->cancel.load()) return fail_all(RG_E_CANCELLED, "cancelled");
         {
@@ -324,27 +323,14 @@ class Engine {
             }
         }
         if (int rc = admit()) return fail_all(rc, nullptr);
-        if (int rc = upload_issue()) return fail_all(rc, nullptr);
-        bool running = false;
-        for (Request *r : active) running |= r->state == 1;
-        if (running) {
-            if (int rc = hip_ok(hipGraphLaunch(graph, stream), "graph launch")) return fail_all(rc, nullptr);
-            graphs++;
-            const int s = status_cur;
-            status_cur ^= 1;
-            if (int rc = hip_ok(hipMemcpyAsync(h_status[s], d_status, status_bytes(), hipMemcpyDeviceToHost, stream),
-                                "status copy"))
-                return fail_all(rc, nullptr);
-            if (int rc = hip_ok(hipEventRecord(status_ev[s], stream), "event record")) return fail_all(rc, nullptr);
-            status_pending[s] = true;
-            if (status_pending[s ^ 1])
-                if (int rc = process_status(s ^ 1)) return fail_all(rc, nullptr);
-        } else {
-            for (int k : {status_cur, status_cur ^ 1})
-                if (status_pending[k])
-                    if (int rc = process_status(k)) return fail_all(rc, nullptr);
-        }
-        return deliver(!running);
+        // at most two generations in flight: later requests join the next
+        // generation instead of queueing behind everything already launched
+        while (inflight.size() >= 2)
+            if (int rc = retire(true)) return fail_all(rc, nullptr);
+        bool launched = false;
+        if (int rc = launch_generation(launched)) return fail_all(rc, nullptr);
+        if (int rc = retire(!launched)) return fail_all(rc, nullptr);
+        return RG_OK;
     }
 
     // True while anything is queued, admitted or finishing.
@@ -404,18 +390,16 @@ class Engine {
         out[1] = samples_done;
         out[2] = dev_stats[0];
         out[3] = dev_stats[1];
-        out[4] = dev_stats[3];
+        out[4] = n_generations;
         out[5] = arena.used();
         out[6] = arena.size();
-        out[7] = graphs;
+        out[7] = n_replays;
     }
 
     StreamTune tune{};
     uint64_t dev_stats[4] = {0, 0, 0, 0};
 
   private:
-    static size_t status_bytes() { return sizeof(StreamCounters); }
-
     int admit() {
         while (!pending.empty() && !free_slots.empty()) {
             Request *r = pending.front();
@@ -425,104 +409,167 @@ class Engine {
             r->off = off;
             r->slot = free_slots.back();
             free_slots.pop_back();
-            r->begin = seq_end;
-            seq_end += r->samples;
             char *base = d_arena + off;
             const uint64_t np = r->npix;
             JobDesc d;
             d.scene = scenes[r->scene];
             d.st = r->st;
-            d.d_samples = (float *)base;
-            d.d_sum = (float *)(base + align_up(np * r->st.spp * 12));
-            d.d_sumsq = (float *)((char *)d.d_sum + align_up(np * 12));
+            d.d_sum = (float *)base;
+            d.d_sumsq = (float *)(base + align_up(np * 12));
             d.d_hdr = (float *)((char *)d.d_sumsq + align_up(np * 12));
-            d.group = plain_group(r->st);
-            d.tile = tune.tile;
+            // the slot's pinned record is free: its last job was delivered,
+            // so the upload of that job's record has completed
             char *rec = h_jobs + (size_t)r->slot * kDevJobBytes;
             stream_job_record(d, rec);
             if (int rc = hip_ok(hipMemcpyAsync((char *)d_jobs + (size_t)r->slot * kDevJobBytes, rec, kDevJobBytes,
                                                hipMemcpyHostToDevice, stream),
                                 "job upload"))
                 return rc;
+            if (int rc = launch_fill_u32(d.d_sum, 0u, 3 * np, stream)) return rc;
+            if (int rc = launch_fill_u32(d.d_sumsq, 0u, 3 * np, stream)) return rc;
             r->d_hdr = d.d_hdr;
             r->d_sum = d.d_sum;
             r->d_sumsq = d.d_sumsq;
-            r->bounces = r->st.max_depth + 2;
+            r->bvh = stream_job_bvh(*d.scene);
             r->state = 1;
             r->t_admit = now_s();
             active.push_back(r);
-            issue_dirty = true;
         }
         return RG_OK;
     }
 
-    int upload_issue() {
-        if (!issue_dirty) return RG_OK;
-        const int b = issue_cur;
-        issue_cur ^= 1;
-        if (int rc = hip_ok(hipEventSynchronize(issue_ev[b]), "issue staging")) return rc;
-        IssueEntry *e = h_issue[b];
-        uint32_t n = 0;
-        for (Request *r : active)
-            if (r->begin + r->samples > cursor_seen) e[n++] = {r->begin, r->slot, 0};
-        uint64_t *hdr = (uint64_t *)(e + kMaxJobs);
-        hdr[0] = seq_end;
-        ((uint32_t *)(hdr + 1))[0] = n;
-        if (n)
-            if (int rc = hip_ok(hipMemcpyAsync(d_issue, e, n * sizeof(IssueEntry), hipMemcpyHostToDevice, stream),
-                                "issue upload"))
-                return rc;
-        if (int rc = hip_ok(hipMemcpyAsync(&sp.ctr->seq_end, hdr, 8, hipMemcpyHostToDevice, stream), "issue upload"))
-            return rc;
-        if (int rc = hip_ok(hipMemcpyAsync(&sp.ctr->n_issue, hdr + 1, 4, hipMemcpyHostToDevice, stream),
-                            "issue upload"))
-            return rc;
-        if (int rc = hip_ok(hipEventRecord(issue_ev[b], stream), "event record")) return rc;
-        issue_dirty = false;
-        return RG_OK;
-    }
-
-    int process_status(int k) {
-        if (int rc = hip_ok(hipEventSynchronize(status_ev[k]), "status wait")) return rc;
-        status_pending[k] = false;
-        const StreamCounters *C = (const StreamCounters *)h_status[k];
-        // after a replay (an even number of iterations) the live cursor is slot 0
-        cursor_seen = C->cursor[0];
-        const uint64_t iters = C->stats[3];
-        for (int q = 0; q < 4; q++) dev_stats[q] = C->stats[q];
+    // Packs the next chunks of the admitted jobs (first come, first served;
+    // a job's chunks in sample order) into one generation and launches it.
+    int launch_generation(bool &launched) {
+        launched = false;
+        std::vector<GenChunk> ch;
+        std::vector<Request *> last;  // jobs whose final chunk is in this generation
+        uint64_t total = 0;
+        uint32_t bounces = 0;
         for (Request *r : active) {
             if (r->state != 1) continue;
-            // all samples issued by iteration issued_iter; none lives longer
-            // than `bounces` iterations
-            if (r->issued_iter == ~0ull && cursor_seen >= r->begin + r->samples) r->issued_iter = iters;
-            if (r->issued_iter == ~0ull || iters + 1 < r->issued_iter + r->bounces) continue;
-            if (int rc = stream_launch_finish(d_jobs, r->slot, r->npix, stream)) return rc;
-            const size_t hb = (size_t)r->npix * 16, sb = (size_t)r->npix * 12;
-            const size_t need = (r->out_hdr ? hb : 0) + (r->out_sum ? sb : 0) + (r->out_sumsq ? sb : 0);
-            if (need) {
-                r->pin = get_pinned(need);
-                if (r->pin < 0) return set_error(RG_E_NOMEM, "pinned staging allocation failed");
-                char *p = (char *)pins[r->pin].p;
-                if (r->out_hdr) {
-                    if (int rc = hip_ok(hipMemcpyAsync(p, r->d_hdr, hb, hipMemcpyDeviceToHost, stream), "result copy"))
-                        return rc;
-                    p += hb;
-                }
-                if (r->out_sum) {
-                    if (int rc = hip_ok(hipMemcpyAsync(p, r->d_sum, sb, hipMemcpyDeviceToHost, stream), "result copy"))
-                        return rc;
-                    p += sb;
-                }
-                if (r->out_sumsq)
-                    if (int rc = hip_ok(hipMemcpyAsync(p, r->d_sumsq, sb, hipMemcpyDeviceToHost, stream),
-                                        "result copy"))
-                        return rc;
+            while (r->issued < r->st.spp && ch.size() < kMaxChunks) {
+                const uint32_t S = std::min(r->group, r->st.spp - r->issued);
+                const uint64_t paths = (uint64_t)r->npix * S;
+                if (!ch.empty() && total + paths > tune.budget) break;
+                if (total + paths > cap) break;
+                GenChunk c;
+                std::memset(&c, 0, sizeof c);
+                c.bv = r->bvh;
+                c.job = r->slot;
+                c.first = (uint32_t)total;
+                c.npix = r->npix;
+                c.width = r->st.width;
+                c.sample_base = r->issued;
+                c.S = S;
+                c.bounces = r->st.max_depth + 2;
+                c.clamp = r->st.firefly_clamp;
+                ch.push_back(c);
+                total += paths;
+                r->issued += S;
+                bounces = std::max(bounces, r->st.max_depth + 2);
+                chunk_jobs.push_back(r);
             }
-            if (!r->ev)
-                if (int rc = hip_ok(hipEventCreateWithFlags(&r->ev, hipEventDisableTiming), "event")) return rc;
-            if (int rc = hip_ok(hipEventRecord(r->ev, stream), "event record")) return rc;
-            r->state = 2;
+            if (r->issued >= r->st.spp) {
+                r->state = 2;
+                last.push_back(r);
+            }
+            if (ch.size() >= kMaxChunks || total >= tune.budget) break;
         }
+        if (ch.empty()) {
+            chunk_jobs.clear();
+            return RG_OK;
+        }
+        // pinned staging of this generation's chunk table and counters
+        const int b = stage_cur;
+        stage_cur ^= 1;
+        if (int rc = hip_ok(hipEventSynchronize(stage_ev[b]), "chunk staging")) return rc;
+        std::memcpy(h_chunks[b], ch.data(), ch.size() * sizeof(GenChunk));
+        if (int rc = hip_ok(hipMemcpyAsync(d_chunks, h_chunks[b], ch.size() * sizeof(GenChunk), hipMemcpyHostToDevice,
+                                           stream),
+                            "chunk upload"))
+            return rc;
+        BatchArgs a = args;
+        a.nchunks = (uint32_t)ch.size();
+        if (int rc = stream_launch_camera(a, d_jobs, (uint32_t)total, stream)) return rc;
+        if (int rc = launch_bounces(a, (uint32_t)total, bounces)) return rc;
+        for (size_t k = 0; k < ch.size(); k++)
+            if (int rc = stream_launch_accumulate(a, ch[k], chunk_jobs[k]->d_sum, chunk_jobs[k]->d_sumsq, stream))
+                return rc;
+        chunk_jobs.clear();
+        for (Request *r : last)
+            if (int rc = finish(r)) return rc;
+        if (int rc = hip_ok(hipMemcpyAsync(h_stats[b], args.st.stats, 32, hipMemcpyDeviceToHost, stream), "stats copy"))
+            return rc;
+        if (int rc = hip_ok(hipEventRecord(stage_ev[b], stream), "event record")) return rc;
+        Inflight f;
+        if (!spare_events.empty()) {
+            f.ev = spare_events.back();
+            spare_events.pop_back();
+        } else if (int rc = hip_ok(hipEventCreateWithFlags(&f.ev, hipEventDisableTiming), "event")) {
+            return rc;
+        }
+        if (int rc = hip_ok(hipEventRecord(f.ev, stream), "event record")) return rc;
+        f.stage = b;
+        f.finishing = std::move(last);
+        inflight.push_back(std::move(f));
+        n_generations++;
+        launched = true;
+        return RG_OK;
+    }
+
+    // The bounce loop, replayed from a captured graph when tune.graphs is
+    // set.  Its kernels read the queue lengths from device counters and the
+    // chunks through the per-path chunk index, so one capture serves every
+    // generation with the same bounce count and slot count rounded up to a
+    // power of two (the extra blocks find no work).
+    int launch_bounces(const BatchArgs &a, uint32_t paths, uint32_t bounces) {
+        if (!tune.graphs) return stream_launch_bounces(a, d_jobs, paths, bounces, tune, n_cu, stream);
+        uint32_t slots = 1024;
+        while (slots < paths && slots < cap) slots <<= 1;
+        slots = std::min(slots, cap);
+        const uint64_t key = ((uint64_t)bounces << 32) | slots;
+        auto it = graphs_by_bounces.find(key);
+        if (it == graphs_by_bounces.end()) {
+            hipGraph_t g = nullptr;
+            if (int rc = hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "begin capture"))
+                return rc;
+            const int lrc = stream_launch_bounces(a, d_jobs, slots, bounces, tune, n_cu, stream);
+            const hipError_t ce = hipStreamEndCapture(stream, &g);
+            if (lrc) return lrc;
+            if (int rc = hip_ok(ce, "end capture")) return rc;
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (int rc = hip_ok(ie, "graph instantiate")) return rc;
+            it = graphs_by_bounces.emplace(key, ex).first;
+        }
+        n_replays++;
+        return hip_ok(hipGraphLaunch(it->second, stream), "graph launch");
+    }
+
+    // Queues a job's HDR resolve and the copies of its results.
+    int finish(Request *r) {
+        if (int rc = stream_launch_resolve(d_jobs, r->slot, r->npix, stream)) return rc;
+        const size_t hb = (size_t)r->npix * 16, sb = (size_t)r->npix * 12;
+        const size_t need = (r->out_hdr ? hb : 0) + (r->out_sum ? sb : 0) + (r->out_sumsq ? sb : 0);
+        if (!need) return RG_OK;
+        r->pin = get_pinned(need);
+        if (r->pin < 0) return set_error(RG_E_NOMEM, "pinned staging allocation failed");
+        char *p = (char *)pins[r->pin].p;
+        if (r->out_hdr) {
+            if (int rc = hip_ok(hipMemcpyAsync(p, r->d_hdr, hb, hipMemcpyDeviceToHost, stream), "result copy"))
+                return rc;
+            p += hb;
+        }
+        if (r->out_sum) {
+            if (int rc = hip_ok(hipMemcpyAsync(p, r->d_sum, sb, hipMemcpyDeviceToHost, stream), "result copy"))
+                return rc;
+            p += sb;
+        }
+        if (r->out_sumsq)
+            if (int rc = hip_ok(hipMemcpyAsync(p, r->d_sumsq, sb, hipMemcpyDeviceToHost, stream), "result copy"))
+                return rc;
         return RG_OK;
     }
 
@@ -541,57 +588,68 @@ class Engine {
         return best;
     }
 
-    // Hands finished requests to their waiters; blocks on the oldest
-    // finishing request when nothing else is running.
-    int deliver(bool may_block) {
-        bool any = false;
-        for (size_t i = 0; i < active.size();) {
-            Request *r = active[i];
-            if (r->state != 2) {
-                i++;
-                continue;
-            }
-            hipError_t q = hipEventQuery(r->ev);
-            if (q == hipErrorNotReady && may_block && !any) q = hipEventSynchronize(r->ev);
-            if (q == hipErrorNotReady) {
-                i++;
-                continue;
-            }
-            if (int rc = hip_ok(q, "result wait")) return fail_all(rc, nullptr);
-            if (r->pin >= 0) {
-                const char *p = (const char *)pins[r->pin].p;
-                const size_t hb = (size_t)r->npix * 16, sb = (size_t)r->npix * 12;
-                if (r->out_hdr) {
-                    std::memcpy(r->out_hdr, p, hb);
-                    p += hb;
-                }
-                if (r->out_sum) {
-                    std::memcpy(r->out_sum, p, sb);
-                    p += sb;
-                }
-                if (r->out_sumsq) std::memcpy(r->out_sumsq, p, sb);
-                pins[r->pin].busy = false;
-                r->pin = -1;
-            }
-            arena.release(r->off, r->bytes);
-            free_slots.push_back(r->slot);
-            active.erase(active.begin() + (long)i);
-            any = true;
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                r->t_done = now_s();
-                r->state = 3;
-                n_done++;
-                samples_done += r->samples;
-            }
-            cv_done.notify_all();
+    // Retires finished generations in launch order and hands their finished
+    // jobs to the waiters; with `block`, waits for the oldest one.
+    int retire(bool block) {
+        while (!inflight.empty()) {
+            Inflight &f = inflight.front();
+            hipError_t q = hipEventQuery(f.ev);
+            if (q == hipErrorNotReady && block) q = hipEventSynchronize(f.ev);
+            if (q == hipErrorNotReady) return RG_OK;
+            if (int rc = hip_ok(q, "generation wait")) return rc;
+            const unsigned long long *S = (const unsigned long long *)h_stats[f.stage];
+            for (int k = 0; k < 3; k++) dev_stats[k] = S[k];
+            dev_stats[3] = n_generations;
+            for (Request *r : f.finishing) deliver(r);
+            spare_events.push_back(f.ev);
+            inflight.pop_front();
+            block = false;
         }
         return RG_OK;
+    }
+
+    void deliver(Request *r) {
+        if (r->pin >= 0) {
+            const char *p = (const char *)pins[r->pin].p;
+            const size_t hb = (size_t)r->npix * 16, sb = (size_t)r->npix * 12;
+            if (r->out_hdr) {
+                std::memcpy(r->out_hdr, p, hb);
+                p += hb;
+            }
+            if (r->out_sum) {
+                std::memcpy(r->out_sum, p, sb);
+                p += sb;
+            }
+            if (r->out_sumsq) std::memcpy(r->out_sumsq, p, sb);
+            pins[r->pin].busy = false;
+            r->pin = -1;
+        }
+        arena.release(r->off, r->bytes);
+        free_slots.push_back(r->slot);
+        active.erase(std::find(active.begin(), active.end(), r));
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            r->t_done = now_s();
+            r->state = 3;
+            n_done++;
+            samples_done += r->samples;
+        }
+        cv_done.notify_all();
     }
 
     int fail_all(int rc, const char *msg) {
         std::string text = msg ? std::string(msg) : std::string(rg_last_error());
         if (stream) (void)hipStreamSynchronize(stream);
+        for (Inflight &f : inflight) spare_events.push_back(f.ev);
+        inflight.clear();
+        chunk_jobs.clear();
+        // release before the requests are marked done: a waiter may free a
+        // request as soon as it is
+        for (Request *r : active) {
+            arena.release(r->off, r->bytes);
+            free_slots.push_back(r->slot);
+        }
+        active.clear();
         {
             std::lock_guard<std::mutex> lk(mu);
             failure = text;
@@ -606,11 +664,6 @@ class Engine {
             }
             inbox.clear();
         }
-        for (Request *r : active) {
-            arena.release(r->off, r->bytes);
-            free_slots.push_back(r->slot);
-        }
-        active.clear();
         pending.clear();
         cv_done.notify_all();
         return set_error(rc, text);
@@ -619,40 +672,37 @@ class Engine {
     Ctx *ctx = nullptr;
     std::vector<Scene *> scenes;
     int n_cu = 0;
+    uint32_t cap = 0;
     hipStream_t stream = nullptr;
-    hipGraphExec_t graph = nullptr;
-    StreamPool sp{};
+    std::map<uint64_t, hipGraphExec_t> graphs_by_bounces;
+    BatchArgs args{};
     void *d_jobs = nullptr;
-    void *d_status = nullptr;
-    IssueEntry *d_issue = nullptr;
+    void *d_chunks = nullptr;
     char *d_arena = nullptr;
     std::vector<void *> dev_allocs;
     Arena arena;
     char *h_jobs = nullptr;
-    char *h_status[2] = {nullptr, nullptr};
-    hipEvent_t status_ev[2] = {nullptr, nullptr};
-    bool status_pending[2] = {false, false};
-    int status_cur = 0;
-    IssueEntry *h_issue[2] = {nullptr, nullptr};
-    hipEvent_t issue_ev[2] = {nullptr, nullptr};
-    int issue_cur = 0;
-    uint32_t *h_zero = nullptr;
+    GenChunk *h_chunks[2] = {nullptr, nullptr};
+    unsigned long long *h_stats[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    int stage_cur = 0;
+    std::deque<Inflight> inflight;
+    std::vector<hipEvent_t> spare_events;
+    std::vector<Request *> chunk_jobs;  // job of each chunk of the generation being built
     std::vector<PinnedBuf> pins;
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request *> inbox;    // submitted, not yet seen by the loop
     std::deque<Request *> pending;  // waiting for a slot and arena space
-    std::vector<Request *> active;  // admitted, in issue order
+    std::vector<Request *> active;  // admitted, in admission order
     std::map<uint64_t, std::unique_ptr<Request>> requests;
     std::vector<uint32_t> free_slots;
-    uint64_t seq_end = 0, cursor_seen = 0, next_ticket = 1;
-    bool issue_dirty = false;
+    uint64_t next_ticket = 1;
     bool stop = false;
     std::thread worker;
     std::string failure;
-    uint64_t n_done = 0, samples_done = 0, graphs = 0;
-
+    uint64_t n_done = 0, samples_done = 0, n_generations = 0, n_replays = 0;
 };
 
 }  // namespace rg
@@ -739,7 +789,7 @@ int rg_server_stats(rg_server *srv, uint64_t out[8]) {
 int rg_server_get_tune(rg_server *srv, rg_tune *out) {
     if (!srv || !out) return set_error(RG_E_INVALID, "null argument");
     const StreamTune &t = srv->eng->tune;
-    *out = {t.pool, t.stack, t.tile, t.blocks_per_cu, t.iters, 0};
+    *out = {t.budget, t.stack, 1, t.blocks_per_cu, t.graphs ? 2u : 0u, 0};
     return RG_OK;
 }
 
@@ -795,11 +845,11 @@ int rg_autotune(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const r
     };
     // coordinate descent: one knob at a time over its candidates, keeping
     // the best setting found so far for the others
-    rg_tune best_t{1u << 21, stacks[0], 8, 8, 4, 0};
+    rg_tune best_t{kPathsInFlight, stacks[0], 1, 8, 0, 0};
     double best = 0.0;
     if (int rc = measure(best_t, best)) return rc;
     const std::vector<std::vector<uint32_t>> options = {
-        {1u << 20, 1u << 21, 1u << 22}, stacks, {1u, 8u, 16u}, {4u, 8u, 16u}, {2u, 4u, 8u}};
+        {1u << 23, 1u << 24, 1u << 25}, stacks, {1u}, {4u, 8u, 16u}, {0u, 1u}};
     for (size_t knob = 0; knob < options.size(); knob++) {
         for (uint32_t v : options[knob]) {
             rg_tune t = best_t;

@@ -43,6 +43,13 @@ struct alignas(128) Node8 {
 };
 static_assert(sizeof(Node8) == 256, "Node8 must be 256 bytes");
 
+// Default path budget of the single-job path: samples_in_flight = 0 runs
+// max(1, kPathsInFlight / pixels) samples of every pixel per launch wave
+// (capped by spp, and by the adaptive pass size).  The batched engine sums
+// each pixel's samples in groups of the same size, so both stay
+// bit-identical.
+constexpr uint32_t kPathsInFlight = 1u << 25;
+
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 

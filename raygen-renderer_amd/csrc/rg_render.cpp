@@ -19,7 +19,6 @@ int launch_tile_noise(const float *d_sum, const float *d_sumsq, const uint32_t *
                       uint32_t height, uint32_t target, float threshold, uint8_t *d_mask, uint32_t *d_nactive,
                       void *stream);
 int launch_tiles_add(uint32_t *d_nspp, const uint8_t *d_mask, uint32_t ntiles, uint32_t add, void *stream);
-int launch_fill_u32(void *p, uint32_t value, uint64_t n, void *stream);
 int launch_denoise(const rg_settings &s, uint32_t width, uint32_t height, const PTScene &sc, const Scene &scn,
                    float *d_hdr, void *d_aov, void *d_a, void *d_b, void *stream);
 int launch_aov(const Scene &scn, const PTScene &sc, void *d_aov, void *stream);
@@ -488,7 +487,7 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     const uint32_t tx = (W + 31) / 32, ty = (H + 31) / 32, ntiles = tx * ty;
     const uint32_t spp = std::max<uint32_t>(1, st.spp);
     uint32_t S = st.samples_in_flight;
-    if (S == 0) S = std::max<uint32_t>(1, (1u << 21) / npix);
+    if (S == 0) S = std::max<uint32_t>(1, kPathsInFlight / npix);
     S = std::min(S, spp);
     if (st.adaptive) S = std::min(S, std::max<uint32_t>(1, st.adaptive_base));
     if ((uint64_t)npix * S > (1ull << 31)) return set_error(RG_E_BUDGET, "too many samples in flight");

@@ -1,15 +1,16 @@
-// rg_stream.h — the streaming (batched) wavefront engine: many render jobs
-// share one pool of paths.  Every iteration puts the next samples of the
-// issue sequence (the concatenation of the admitted jobs' samples,
-// pass-major within a job) into the positions freed by finished paths, so
-// the extend / shade / shadow kernels always see a full pool whatever the
-// mix of resolutions, sample counts and depths.  Finished samples land in
-// their job's per-sample buffer; a job is resolved once all its samples are
-// in, summing them per pixel in the same order and grouping as the
-// single-job path, so a job's image does not depend on what ran beside it.
-// A job is complete max_depth + 1 iterations after the issue cursor passed
-// its last sample (no path lives longer), which the host reads from the
-// counter snapshot it takes after every graph replay.
+// rg_stream.h — the batched engine: many render jobs packed into shared
+// wavefront generations.
+//
+// A generation is a list of chunks; a chunk is one job's samples
+// [base, base + S) of every pixel, S being the single-job path's samples in
+// flight for that job (its accumulation group, capped by what is left).
+// Chunks of any number of jobs share the generation's path slots up to the
+// budget, and the bounce loop (extend, shade, shadow, gather) runs over one
+// compacted queue of all their paths, each ray on its own job's tree and
+// each event with its own job's scene record.  After the loop every chunk is
+// summed per pixel into its job's sums exactly as the single-job path sums
+// one of its generations, so a job's image does not depend on what ran
+// beside it.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -22,87 +23,66 @@ namespace rg {
 
 constexpr uint32_t kMaxJobs = 1024;    // job slots of one engine
 constexpr size_t kDevJobBytes = 512;   // device job record
+constexpr uint32_t kMaxChunks = 4096;  // chunks of one generation
 
 // Knobs picked by the autotuner (mirror of rg_tune).
 struct StreamTune {
-    uint32_t pool;           // path slots in flight (queue size)
+    uint32_t budget;         // path slots of a generation
     uint32_t stack;          // traversal stack entries: 32, 64, 128 or 256
-    uint32_t tile;           // pixel tile edge of the issue order (1 = scanline)
     uint32_t blocks_per_cu;  // cap on persistent traversal blocks per compute unit
-    uint32_t iters;          // iterations per captured graph (even)
+    uint32_t graphs;         // replay each generation's bounce loop as a captured hipGraph
     uint32_t width;          // node width of every resident scene's BVH (4 or 8)
 };
 
-// Device-side counters; iteration parity qi reads slot [qi] and writes [qi^1].
-struct StreamCounters {
-    uint64_t cursor[2];            // next sample of the issue sequence
-    uint64_t seq_end;              // end of the issue sequence (host-written)
-    uint32_t nholes[2];            // free positions in the hole list
-    uint32_t nsh[2];               // shadow-ray queue length
-    uint32_t fetch[2];             // any-hit dynamic-fetch cursor
-    uint32_t n_issue;              // issue-table entries (host-written)
-    uint32_t pad[3];
-    unsigned long long stats[4];   // extension rays, shadow rays, samples issued, iterations
-};
-static_assert(sizeof(StreamCounters) % 16 == 0, "counter block keeps 16-byte alignment");
-
-// Issue-table entry: the job whose samples start at `begin` in the issue
-// sequence.  Entries are sorted by begin.
-struct IssueEntry {
-    uint64_t begin;
-    uint32_t job;
-    uint32_t pad;
+// One chunk of a generation.  Chunks are sorted by `first`.
+struct GenChunk {
+    DevBvh bv;             // the job's tree
+    uint32_t job;          // job slot
+    uint32_t first;        // first path slot of the chunk in the generation
+    uint32_t npix, width;  // the job's image
+    uint32_t sample_base;  // first sample of every pixel in the chunk
+    uint32_t S;            // samples per pixel in the chunk
+    uint32_t bounces;      // the job's path events (max_depth + 2): its paths
+                           // still alive after that many are closed
+    float clamp;           // the job's firefly clamp (0: none)
 };
 
-// Path state, structure of arrays indexed by pool position.  A path keeps
-// its position from its camera ray to its last event, so every kernel reads
-// and writes element i in thread i; a finished path's position goes to the
-// hole list and the next iteration's regen puts a new sample there.
-struct PathSoA {
-    rg_ray *ray;
-    float4 *beta;   // estimator weight, w = bsdfSampledPdf
-    float4 *tp;     // reference throughput (Russian roulette)
-    float4 *L;      // radiance so far, w = alive after the event
-    int4 *meta;     // passes, medium, next LDS dim, next Philox draw
-    int4 *sid;      // job slot (-1: free position), pass, pixel, bounces taken
-};
-
-// Device pointers of the path pool, passed to every kernel by value.
-struct StreamPool {
-    uint32_t cap;
-    PathSoA st;
-    rg_hit *hit;
-    rg_ray *sray;       // NEE rays per (position, k), k < 2 (tmin = predicate)
-    float4 *scontrib;   // unoccluded contribution per (position, k)
-    uint8_t *socc;      // occluded flag per (position, k)
-    uint32_t *ssrc;     // queue of the (position, k) pairs with a ray: 2 * position + k
-    uint32_t *nshadow;  // shadow rays per position this event
-    uint32_t *holes;    // free positions
-    StreamCounters *ctr;
-    const IssueEntry *issue;
+// Device pointers of a generation, passed to every kernel by value: the
+// single-job path's path state, the chunk of each path slot, the chunk table.
+struct BatchArgs {
+    PTState st;
+    uint32_t *chunk_of;
+    const GenChunk *chunks;
+    uint32_t nchunks;
+    uint32_t cap;  // path slots of the buffers
 };
 
 // What the host knows of one job when it writes the device record.
 struct JobDesc {
     const Scene *scene;
     rg_settings st;
-    float *d_samples;  // spp * npix * 3 floats
     float *d_sum, *d_sumsq, *d_hdr;
-    uint32_t group;    // samples per pixel per accumulation group
-    uint32_t tile;
 };
 
 // Writes the device job record (kDevJobBytes) for `d` into `out`.
 void stream_job_record(const JobDesc &d, void *out);
+// The job's tree as a generation chunk carries it.
+DevBvh stream_job_bvh(const Scene &s);
 // Grid geometry of the persistent kernels.
 int stream_cu_count(int device);
-// Puts the pool in its initial state: every slot free, queues empty.
-int stream_launch_reset(const StreamPool &sp, void *stream);
-// One iteration (regenerate, extend, shade, shadow) at parity qi.
-int stream_launch_iteration(const StreamPool &sp, const void *d_jobs, int qi, const StreamTune &tune, int n_cu,
-                            void *stream);
-// Sums a finished job's samples per pixel and writes its HDR image.
-int stream_launch_finish(const void *d_jobs, uint32_t slot, uint32_t npix, void *stream);
+// Camera rays of every chunk's samples (queue 0 = all `total` paths).
+int stream_launch_camera(const BatchArgs &a, const void *d_jobs, uint32_t total, void *stream);
+// The bounce loop (max_bounces events, the largest of the chunks' bounce
+// counts; each path stops after its own chunk's) and the flush of paths
+// still alive after it; the per-path kernels are sized for `paths` slots
+// (at least the generation's paths, at most the buffers' slots).
+int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, uint32_t paths, uint32_t max_bounces,
+                          const StreamTune &tune, int n_cu, void *stream);
+// Per-pixel sums of one chunk into its job's sums (the single-job path's
+// accumulation of one generation).
+int stream_launch_accumulate(const BatchArgs &a, const GenChunk &c, float *d_sum, float *d_sumsq, void *stream);
+// HDR image of a finished job: mean * exposure, clamped at 0.
+int stream_launch_resolve(const void *d_jobs, uint32_t slot, uint32_t npix, void *stream);
 // Largest stack_need the traversal templates cover for a stack knob value.
 bool stream_stack_ok(uint32_t stack, uint32_t need);
 

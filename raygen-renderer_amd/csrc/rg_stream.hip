@@ -1,22 +1,23 @@
-// rg_stream.hip — kernels of the streaming (batched) wavefront engine.
+// rg_stream.hip — kernels of the batched engine: packed wavefront
+// generations of many jobs (see rg_stream.h).
 //
-// Each iteration at parity qi runs five kernels over one pool of paths
-// shared by every admitted job.  A path keeps its pool position for life;
-// kernels touch element i in thread i.
-//   regen   puts the next samples of the issue sequence (camera rays,
-//           rayrenderer.cpp:860-934) into positions freed last iteration
-//   extend  closest-hit traversal of every live position (persistent waves,
-//           leaf postponement and lane refill; each ray on its job's tree)
-//   shade   one path event per live position (the same step() as the
-//           single-job path, with the job's scene record loaded
-//           wave-uniform); the event's 0-2 NEE rays are queued with
-//           block-aggregated appends (one atomic per block)
-//   anyhit  the queued NEE rays, one per lane, persistent waves
-//   resolve adds the unoccluded contributions in event order; a finished
-//           path writes its clamped radiance to the job's sample buffer and
-//           frees its position
-// All bookkeeping lives in device counters, so a run of iterations needs no
-// host round trip and is captured once as a hipGraph.
+// Per generation:
+//   camera  one path per (chunk, sample, pixel): the chunk is found by a
+//           binary search over the sorted chunk table, the camera ray built
+//           with the job's scene record (rayrenderer.cpp:860-934); the queue
+//           is the identity over all paths
+//   then max_bounces times, over the compacted queue of live paths:
+//   extend  closest hit, each ray on its job's tree (persistent waves, leaf
+//           postponement, lane refill)
+//   shade   one path event with the job's scene record, made wave-uniform
+//           (the same step() as the single-job path); the event's 0-2 NEE
+//           rays are queued with block-aggregated appends
+//   anyhit  the queued NEE rays, one per lane, on their job's tree
+//   gather  unoccluded contributions in event order; finished paths, and
+//           paths at their job's last event, write their radiance clamped
+//           by their job's firefly clamp, live ones are appended to the
+//           next queue
+//   then each chunk is summed per pixel into its job's sums.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,14 +33,11 @@ namespace rg {
 // Device job record.
 struct DevJob {
     PTScene sc;
-    DevBvh bv;
-    float *samples;  // [spp][npix][3] clamped radiance
     float *sum;      // [npix][3]
     float *sumsq;    // [npix][3]
     float *hdr;      // [npix][4]
-    uint32_t npix, spp, group, max_bounces;
+    uint32_t npix, spp;
     float clamp, exposure;
-    uint32_t tile, pad;
 };
 struct DevJobSlot {
     DevJob j;
@@ -49,23 +47,20 @@ static_assert(sizeof(DevJobSlot) == kDevJobBytes, "job record size");
 
 DevBvh dev_bvh(const Scene &s);
 
+DevBvh stream_job_bvh(const Scene &s) { return dev_bvh(s); }
+
 void stream_job_record(const JobDesc &d, void *out) {
     DevJobSlot rec;
     std::memset(&rec, 0, sizeof rec);
     DevJob &J = rec.j;
     J.sc = make_ptscene(*d.scene, d.st);
-    J.bv = dev_bvh(*d.scene);
-    J.samples = d.d_samples;
     J.sum = d.d_sum;
     J.sumsq = d.d_sumsq;
     J.hdr = d.d_hdr;
     J.npix = d.st.width * d.st.height;
     J.spp = d.st.spp;
-    J.group = d.group;
-    J.max_bounces = d.st.max_depth + 2;
     J.clamp = d.st.firefly_clamp;
     J.exposure = d.scene->camera.exposure;
-    J.tile = d.tile;
     std::memcpy(out, &rec, sizeof rec);
 }
 
@@ -90,183 +85,140 @@ __device__ __forceinline__ void with_job(const DevJobSlot *__restrict__ jobs, ui
     }
 }
 
-// Issue order of the pixels of one pass: bands of T rows, each walked in
-// T-column chunks (partial bands and chunks at the image edge), so the 64
-// lanes of a wave cover a compact block of the image.
-__device__ __forceinline__ uint32_t tiled_pixel(uint32_t k, uint32_t W, uint32_t H, uint32_t T) {
-    if (T <= 1) return k;
-    const uint32_t band = k / (W * T);
-    const uint32_t y0 = band * T;
-    const uint32_t hb = min(T, H - y0);
-    const uint32_t idx = k - y0 * W;
-    const uint32_t c = idx / (hb * T);
-    const uint32_t x0 = c * T;
-    const uint32_t wc = min(T, W - x0);
-    const uint32_t r = idx - c * hb * T;
-    return (y0 + r / wc) * W + x0 + r % wc;
-}
-
-// Last issue-table entry whose begin <= g.
-__device__ __forceinline__ uint32_t find_issue(const IssueEntry *__restrict__ t, uint32_t n, uint64_t g) {
+// The last chunk whose first path slot is <= p.
+__device__ __forceinline__ uint32_t find_chunk(const GenChunk *__restrict__ ch, uint32_t n, uint32_t p) {
     uint32_t lo = 0, hi = n;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (t[mid].begin <= g) lo = mid;
+        if (ch[mid].first <= p) lo = mid;
         else hi = mid;
     }
     return lo;
 }
 
-__global__ void k_sreset(StreamPool sp) {
+__global__ void k_bset_u32(uint32_t *p, uint32_t v) { *p = v; }
+__global__ void k_bzero2(uint32_t *a, uint32_t *b) {
+    *a = 0;
+    *b = 0;
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_bcamera(BatchArgs a, const DevJobSlot *__restrict__ jobs,
+                                                       uint32_t total) {
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    if (i < sp.cap) {
-        sp.st.sid[i] = make_int4(-1, 0, 0, 0);
-        sp.holes[i] = sp.cap - 1 - i;
-    }
-    if (i == 0) {
-        StreamCounters *C = sp.ctr;
-        C->cursor[0] = C->cursor[1] = 0;
-        C->seq_end = 0;
-        C->nholes[0] = sp.cap;
-        C->nholes[1] = 0;
-        C->nsh[0] = C->nsh[1] = 0;
-        C->fetch[0] = C->fetch[1] = 0;
-        C->n_issue = 0;
-        for (int k = 0; k < 4; k++) C->stats[k] = 0;
-    }
+    if (i >= total) return;
+    const PTState &st = a.st;
+    const uint32_t c = find_chunk(a.chunks, a.nchunks, i);
+    const GenChunk &C = a.chunks[c];
+    const uint32_t local = i - C.first;
+    const uint32_t pix = local % C.npix;
+    const int sample = (int)(C.sample_base + local / C.npix);
+    with_job(jobs, C.job, [&](const DevJob &J) {
+        Rng R = rng_for(J.sc, pix, sample, 0, 0);
+        v3 o, d;
+        camera_ray(J.sc, R, pix % J.sc.width, pix / J.sc.width, o, d);
+        st.ray[i] = {{o.x, o.y, o.z}, 0.0f, {d.x, d.y, d.z}, MAX_RAY_DISTANCE};
+        st.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        st.tp[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        st.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        st.meta[i] = make_int4(-1, J.sc.global_medium, R.dim, (int)R.k);
+    });
+    st.q[0][i] = i;
+    a.chunk_of[i] = c;
+    if (i == 0) atomicAdd(st.stats + 2, (unsigned long long)total);
 }
 
-// Puts the next samples of the issue sequence into free positions.
-__global__ __launch_bounds__(kPtBlock) void k_sregen(StreamPool sp, const DevJobSlot *__restrict__ jobs, int qi) {
-    StreamCounters *C = sp.ctr;
-    const uint32_t h = C->nholes[qi];
-    const uint64_t cur = C->cursor[qi];
-    const uint64_t end = C->seq_end;
-    const uint32_t n_issue = C->n_issue;
-    // nothing is issued without an issue table covering [cur, end)
-    const uint64_t left = (end > cur && n_issue > 0) ? end - cur : 0;
-    const uint32_t issue = (uint32_t)(left < (uint64_t)h ? left : (uint64_t)h);
-    const uint32_t gid = blockIdx.x * kPtBlock + threadIdx.x;
-    if (gid == 0) {
-        C->cursor[qi ^ 1] = cur + issue;
-        C->nholes[qi ^ 1] = h - issue;
-        C->nsh[qi] = 0;
-        C->fetch[qi] = 0;
-        C->stats[0] += sp.cap - h + issue;
-        C->stats[2] += issue;
-        C->stats[3] += 1;
-    }
-    const PathSoA S = sp.st;
-    const uint32_t stride = gridDim.x * kPtBlock;
-    for (uint32_t k = gid; k < issue; k += stride) {
-        const uint64_t g = cur + k;
-        const IssueEntry e = sp.issue[find_issue(sp.issue, n_issue, g)];
-        const uint32_t i = sp.holes[h - 1 - k];
-        const uint32_t local = (uint32_t)(g - e.begin);
-        with_job(jobs, e.job, [&](const DevJob &J) {
-            const uint32_t pass = local / J.npix;
-            const uint32_t pix = tiled_pixel(local - pass * J.npix, J.sc.width, J.sc.height, J.tile);
-            Rng R = rng_for(J.sc, pix, (int)pass, 0, 0);
-            v3 o, d;
-            camera_ray(J.sc, R, pix % J.sc.width, pix / J.sc.width, o, d);
-            S.ray[i] = {{o.x, o.y, o.z}, 0.0f, {d.x, d.y, d.z}, MAX_RAY_DISTANCE};
-            S.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-            S.tp[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-            S.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            S.meta[i] = make_int4(-1, J.sc.global_medium, R.dim, (int)R.k);
-            S.sid[i] = make_int4((int)e.job, (int)pass, (int)pix, 0);
-        });
-    }
-}
-
-// Closest hit of every live position; each ray traverses its own job's tree.
-struct PoolExtendSrc {
-    const int4 *sid;
+// Closest hit of every live path (queue qi), each ray on its job's tree.
+struct BatchExtendSrc {
     const rg_ray *ray;
+    const uint32_t *q;
     rg_hit *hit;
-    const DevJobSlot *jobs;
+    const uint32_t *chunk_of;
+    const GenChunk *chunks;
     __device__ __forceinline__ bool load(uint64_t i, DevBvh &bv, PsRay &r) const {
-        const int job = sid[i].x;
-        if (job < 0) return false;
-        bv = jobs[job].j.bv;
-        const rg_ray x = ray[i];
+        const uint32_t p = q[i];
+        bv = chunks[chunk_of[p]].bv;
+        const rg_ray x = ray[p];
         r = {x.o[0], x.o[1], x.o[2], x.d[0], x.d[1], x.d[2], x.tmin, x.tmax, 0u};
         return true;
     }
     __device__ __forceinline__ void store_closest(uint64_t i, uint32_t prim, float t, float u, float v) const {
-        hit[i] = {prim, t, u, v};
+        hit[q[i]] = {prim, t, u, v};
     }
     __device__ __forceinline__ void store_any(uint64_t, bool) const {}
 };
 
 template <int kStack, int kW>
-__global__ __launch_bounds__(kPtBlock) void k_sextend(StreamPool sp, const DevJobSlot *__restrict__ jobs) {
+__global__ __launch_bounds__(kPtBlock) void k_bextend(BatchArgs a, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
+    const PTState &st = a.st;
+    const uint32_t count = st.count[qi];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && count) atomicAdd(st.stats + 0, (unsigned long long)count);
     const DevBvh none{};
-    trace_ps_loop<kStack, kW, false, kPtBlock, PoolExtendSrc, true>(none, PoolExtendSrc{sp.st.sid, sp.st.ray, sp.hit, jobs},
-                                                                  sp.cap, lc);
+    trace_ps_loop<kStack, kW, false, kPtBlock, BatchExtendSrc, true>(
+        none, BatchExtendSrc{st.ray, st.q[qi], st.hit, a.chunk_of, a.chunks}, count, lc);
 }
 
-// One path event per live position, state updated in place; the event's
-// NEE rays are queued for the any-hit kernel.
-__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_sshade(StreamPool sp, const DevJobSlot *__restrict__ jobs, int qi) {
+// One path event per live path with its job's scene record.
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bshade(
+    BatchArgs a, const DevJobSlot *__restrict__ jobs, int qi) {
     __shared__ uint32_t lds[kPtBlock / 64 + 1];
+    const PTState &st = a.st;
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    const PathSoA S = sp.st;
-    uint32_t ns = 0;
-    int4 sid = make_int4(-1, 0, 0, 0);
-    if (i < sp.cap) sid = S.sid[i];
-    if (sid.x >= 0) {
-        const int4 meta = S.meta[i];
-        with_job(jobs, (uint32_t)sid.x, [&](const DevJob &J) {
+    const uint32_t count = st.count[qi];
+    if (blockIdx.x * kPtBlock >= count) return;  // whole block past the end
+    uint32_t p = 0, ns = 0;
+    if (i < count) {
+        p = st.q[qi][i];
+        const GenChunk &C = a.chunks[a.chunk_of[p]];
+        const uint32_t local = p - C.first;
+        const uint32_t pix = local % C.npix;
+        const int sample = (int)(C.sample_base + local / C.npix);
+        const int4 meta = st.meta[p];
+        with_job(jobs, C.job, [&](const DevJob &J) {
             const PTScene &sc = J.sc;
-            Rng R = rng_for(sc, (uint32_t)sid.z, sid.y, meta.z, (uint32_t)meta.w);
-            const rg_ray r = S.ray[i];
-            const float4 b = S.beta[i];
+            Rng R = rng_for(sc, pix, sample, meta.z, (uint32_t)meta.w);
+            const rg_ray r = st.ray[p];
+            const float4 b = st.beta[p];
             Path P;
             P.o = mk(r.o[0], r.o[1], r.o[2]);
             P.d = mk(r.d[0], r.d[1], r.d[2]);
             P.beta = f4v(b);
             P.bsdf_pdf = b.w;
-            P.tp = f4v(S.tp[i]);
-            P.L = f4v(S.L[i]);
+            P.tp = f4v(st.tp[p]);
+            P.L = f4v(st.L[p]);
             P.passes = meta.x;
             P.medium = meta.y;
             P.alive = true;
-            Shadows Sh;
-            Sh.n = 0;
-            const rg_hit h = sp.hit[i];
-            step(sc, R, P, Sh, h);
-            const bool survive = P.alive && (uint32_t)(sid.w + 1) < J.max_bounces;
-            S.ray[i] = {{P.o.x, P.o.y, P.o.z}, 0.0f, {P.d.x, P.d.y, P.d.z}, MAX_RAY_DISTANCE};
-            S.beta[i] = v4(P.beta, P.bsdf_pdf);
-            S.tp[i] = v4(P.tp, 0.0f);
-            S.L[i] = v4(P.L, survive ? 1.0f : 0.0f);
-            S.meta[i] = make_int4(P.passes, P.medium, R.dim, (int)R.k);
-            store_shadows(Sh, i, sp.sray, sp.scontrib);
-            ns = (uint32_t)Sh.n;
+            Shadows S;
+            S.n = 0;
+            const rg_hit h = st.hit[p];
+            step(sc, R, P, S, h);
+            st.ray[p] = {{P.o.x, P.o.y, P.o.z}, 0.0f, {P.d.x, P.d.y, P.d.z}, MAX_RAY_DISTANCE};
+            st.beta[p] = v4(P.beta, P.bsdf_pdf);
+            st.tp[p] = v4(P.tp, 0.0f);
+            st.L[p] = v4(P.L, P.alive ? 1.0f : 0.0f);
+            st.meta[p] = make_int4(P.passes, P.medium, R.dim, (int)R.k);
+            store_shadows(S, p, st.sray, st.scontrib);
+            ns = (uint32_t)S.n;
         });
-        sid.w += 1;
-        S.sid[i] = sid;
-        sp.nshadow[i] = ns;
+        st.nshadow[p] = ns;
     }
     for (uint32_t k = 0; k < 2; k++) {
-        const uint32_t pos = block_slot(k < ns, &sp.ctr->nsh[qi], lds);
-        if (k < ns) sp.ssrc[pos] = 2 * i + k;
+        const uint32_t pos = block_slot(k < ns, st.count + 2, lds);
+        if (k < ns) st.ssrc[pos] = 2 * p + k;
     }
 }
 
-// Any-hit test of every queued NEE ray, persistent waves; each ray traverses
-// its own job's tree.
-struct PoolShadowSrc {
-    const uint32_t *ssrc;
-    const int4 *sid;
+// Any-hit test of every queued NEE ray on its job's tree.
+struct BatchShadowSrc {
     const rg_ray *sray;
+    const uint32_t *ssrc;
     uint8_t *socc;
-    const DevJobSlot *jobs;
+    const uint32_t *chunk_of;
+    const GenChunk *chunks;
     __device__ __forceinline__ bool load(uint64_t i, DevBvh &bv, PsRay &r) const {
         const uint32_t src = ssrc[i];
-        bv = jobs[sid[src >> 1].x].j.bv;
+        bv = chunks[chunk_of[src >> 1]].bv;
         const rg_ray x = sray[src];
         r = {x.o[0], x.o[1], x.o[2], x.d[0], x.d[1], x.d[2], 0.0f, x.tmax, 1u << (int)x.tmin};
         return true;
@@ -276,101 +228,104 @@ struct PoolShadowSrc {
 };
 
 template <int kStack, int kW>
-__global__ __launch_bounds__(kPtBlock) void k_sanyhit(StreamPool sp, const DevJobSlot *__restrict__ jobs, int qi) {
+__global__ __launch_bounds__(kPtBlock) void k_banyhit(BatchArgs a) {
     RG_LDS_COLUMN(lc, kPtBlock);
-    const uint32_t n = sp.ctr->nsh[qi];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(&sp.ctr->stats[1], (unsigned long long)n);
+    const PTState &st = a.st;
+    const uint32_t n = st.count[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(st.stats + 1, (unsigned long long)n);
     const DevBvh none{};
-    trace_ps_loop<kStack, kW, true, kPtBlock, PoolShadowSrc, true>(
-        none, PoolShadowSrc{sp.ssrc, sp.st.sid, sp.sray, sp.socc, jobs}, n, lc);
+    trace_ps_loop<kStack, kW, true, kPtBlock, BatchShadowSrc, true>(
+        none, BatchShadowSrc{st.sray, st.ssrc, st.socc, a.chunk_of, a.chunks}, n, lc);
 }
 
-// Adds the unoccluded NEE contributions in event order.  A path that ended
-// writes its clamped radiance to the job's sample buffer and frees its
-// position (one hole-list atomic per block).  Job completion needs no
-// counter: a path lives at most max_depth + 2 iterations, so a job is done
-// that many iterations after the issue cursor passed its last sample.
-__global__ __launch_bounds__(kPtBlock) void k_sresolve(StreamPool sp, const DevJobSlot *__restrict__ jobs, int qi) {
+__device__ __forceinline__ float4 clamped(float4 L, float clamp_max) {
+    float4 c = make_float4(fmaxf(L.x, 0.0f), fmaxf(L.y, 0.0f), fmaxf(L.z, 0.0f), 1.0f);
+    if (clamp_max > 0.0f) {
+        c.x = fminf(c.x, clamp_max);
+        c.y = fminf(c.y, clamp_max);
+        c.z = fminf(c.z, clamp_max);
+    }
+    return c;
+}
+
+// Adds the unoccluded NEE contributions in event order, closes finished
+// paths and paths at their job's last event (bounce + 1 = max_depth + 2)
+// with their radiance clamped by the job's firefly clamp and appends the
+// live ones to the next queue.
+__global__ __launch_bounds__(kPtBlock) void k_bgather(BatchArgs a, int qi, uint32_t bounce) {
     __shared__ uint32_t lds[kPtBlock / 64 + 1];
+    const PTState &st = a.st;
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    const PathSoA S = sp.st;
-    bool fin = false;
-    int4 sid = make_int4(-1, 0, 0, 0);
-    if (i < sp.cap) sid = S.sid[i];
-    if (sid.x >= 0) {
-        float4 L = S.L[i];
-        const uint32_t ns = sp.nshadow[i];
+    const uint32_t count = st.count[qi];
+    if (blockIdx.x * kPtBlock >= count) return;  // whole block past the end
+    bool alive = false;
+    uint32_t p = 0;
+    if (i < count) {
+        p = st.q[qi][i];
+        float4 L = st.L[p];
+        const uint32_t ns = st.nshadow[p];
         for (uint32_t k = 0; k < ns; k++) {
-            if (!sp.socc[2 * i + k]) {
-                const float4 c = sp.scontrib[2 * i + k];
+            if (!st.socc[2 * p + k]) {
+                const float4 c = st.scontrib[2 * p + k];
                 L.x = L.x + c.x;
                 L.y = L.y + c.y;
                 L.z = L.z + c.z;
             }
         }
-        if (L.w != 0.0f) {
-            S.L[i] = L;
-        } else {
-            const DevJob &J = jobs[sid.x].j;
-            fin = true;
-            float cx = fmaxf(L.x, 0.0f), cy = fmaxf(L.y, 0.0f), cz = fmaxf(L.z, 0.0f);
-            const float clamp_max = J.clamp;
-            if (clamp_max > 0.0f) {
-                cx = fminf(cx, clamp_max);
-                cy = fminf(cy, clamp_max);
-                cz = fminf(cz, clamp_max);
-            }
-            float *o = J.samples + 3 * ((size_t)(uint32_t)sid.y * J.npix + (uint32_t)sid.z);
-            o[0] = cx;
-            o[1] = cy;
-            o[2] = cz;
-            S.sid[i] = make_int4(-1, 0, 0, 0);
-        }
+        const GenChunk &C = a.chunks[a.chunk_of[p]];
+        // a path past its job's last event is closed as the single-job
+        // path's final flush closes it
+        alive = L.w != 0.0f && bounce + 1 < C.bounces;
+        if (alive) st.L[p] = L;
+        else st.sample_out[p] = clamped(L, C.clamp);
     }
-    const uint32_t pos = block_slot(fin, &sp.ctr->nholes[qi ^ 1], lds);
-    if (fin) sp.holes[pos] = i;
+    const uint32_t pos = block_slot(alive, st.count + (qi ^ 1), lds);
+    if (alive) st.q[qi ^ 1][pos] = p;
 }
 
+// Paths still alive after the last bounce are closed with their radiance.
+__global__ __launch_bounds__(kPtBlock) void k_bflush(BatchArgs a, int qi) {
+    const PTState &st = a.st;
+    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+    if (i >= st.count[qi]) return;
+    const uint32_t p = st.q[qi][i];
+    st.sample_out[p] = clamped(st.L[p], a.chunks[a.chunk_of[p]].clamp);
+}
 
-// Per-pixel sums of a finished job in pass order, in groups of `group`
-// passes (the single-job path's samples in flight), then the HDR resolve
-// (mean * exposure, clamped at 0).
-__global__ __launch_bounds__(kPtBlock) void k_sfinish(const DevJobSlot *__restrict__ jobs, uint32_t slot) {
-    const DevJob &J = jobs[slot].j;
-    const uint32_t npix = J.npix;
+// Per-pixel sums of a chunk's S samples in sample order, added to the job's
+// sums: the single-job path's accumulation of one generation.
+__global__ __launch_bounds__(kPtBlock) void k_baccumulate(const float4 *__restrict__ out, uint32_t npix, uint32_t S,
+                                                           float *__restrict__ sum, float *__restrict__ sumsq) {
     const uint32_t pix = blockIdx.x * kPtBlock + threadIdx.x;
     if (pix >= npix) return;
-    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
-    for (uint32_t p0 = 0; p0 < J.spp; p0 += J.group) {
-        const uint32_t p1 = min(J.spp, p0 + J.group);
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
-        for (uint32_t p = p0; p < p1; p++) {
-            const float *c = J.samples + 3 * ((size_t)p * npix + pix);
-            const float cx = c[0], cy = c[1], cz = c[2];
-            sx = sx + cx;
-            sy = sy + cy;
-            sz = sz + cz;
-            qx = qx + cx * cx;
-            qy = qy + cy * cy;
-            qz = qz + cz * cz;
-        }
-        s0 += sx;
-        s1 += sy;
-        s2 += sz;
-        q0 += qx;
-        q1 += qy;
-        q2 += qz;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
+    for (uint32_t s = 0; s < S; s++) {
+        const float4 c = out[(size_t)s * npix + pix];
+        sx = sx + c.x;
+        sy = sy + c.y;
+        sz = sz + c.z;
+        qx = qx + c.x * c.x;
+        qy = qy + c.y * c.y;
+        qz = qz + c.z * c.z;
     }
-    J.sum[3 * pix] = s0;
-    J.sum[3 * pix + 1] = s1;
-    J.sum[3 * pix + 2] = s2;
-    J.sumsq[3 * pix] = q0;
-    J.sumsq[3 * pix + 1] = q1;
-    J.sumsq[3 * pix + 2] = q2;
-    const float inv = 1.0f / (float)J.spp;
-    J.hdr[4 * pix] = fmaxf(s0 * inv * J.exposure, 0.0f);
-    J.hdr[4 * pix + 1] = fmaxf(s1 * inv * J.exposure, 0.0f);
-    J.hdr[4 * pix + 2] = fmaxf(s2 * inv * J.exposure, 0.0f);
+    sum[3 * pix] += sx;
+    sum[3 * pix + 1] += sy;
+    sum[3 * pix + 2] += sz;
+    sumsq[3 * pix] += qx;
+    sumsq[3 * pix + 1] += qy;
+    sumsq[3 * pix + 2] += qz;
+}
+
+// HDR output of a finished job (rayrenderer.cpp:990-1009): mean * exposure,
+// clamped at 0.
+__global__ __launch_bounds__(kPtBlock) void k_bresolve(const DevJobSlot *__restrict__ jobs, uint32_t slot) {
+    const DevJob &J = jobs[slot].j;
+    const uint32_t pix = blockIdx.x * kPtBlock + threadIdx.x;
+    if (pix >= J.npix) return;
+    const float inv = J.spp > 0 ? 1.0f / (float)J.spp : 0.0f;
+    J.hdr[4 * pix] = fmaxf(J.sum[3 * pix] * inv * J.exposure, 0.0f);
+    J.hdr[4 * pix + 1] = fmaxf(J.sum[3 * pix + 1] * inv * J.exposure, 0.0f);
+    J.hdr[4 * pix + 2] = fmaxf(J.sum[3 * pix + 2] * inv * J.exposure, 0.0f);
     J.hdr[4 * pix + 3] = 1.0f;
 }
 
@@ -392,51 +347,64 @@ bool stream_stack_ok(uint32_t stack, uint32_t need) {
     return (stack == 32 || stack == 64 || stack == 128 || stack == 256) && need <= stack;
 }
 
-int stream_launch_reset(const StreamPool &sp, void *stream) {
-    const uint32_t blocks = (sp.cap + kPtBlock - 1) / kPtBlock;
-    hipLaunchKernelGGL(k_sreset, dim3(blocks ? blocks : 1), dim3(kPtBlock), 0, (hipStream_t)stream, sp);
-    return chk_s(hipGetLastError(), "stream reset launch");
+int stream_launch_camera(const BatchArgs &a, const void *d_jobs, uint32_t total, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_bset_u32, dim3(1), dim3(1), 0, s, a.st.count + 0, total);
+    if (total)
+        hipLaunchKernelGGL(k_bcamera, dim3((total + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, s, a,
+                           (const DevJobSlot *)d_jobs, total);
+    return chk_s(hipGetLastError(), "batch camera launch");
 }
 
 template <int kStack, int kW>
-static void iteration_impl(const StreamPool &sp, const DevJobSlot *jobs, int qi, const StreamTune &tune, int n_cu,
-                           hipStream_t s) {
-    const uint32_t flat = (sp.cap + kPtBlock - 1) / kPtBlock;
-    const uint32_t regen_blocks = std::min<uint32_t>(flat, (uint32_t)n_cu * 4);
+static void bounces_impl(const BatchArgs &a, const DevJobSlot *jobs, uint32_t paths, uint32_t max_bounces,
+                         const StreamTune &tune, int n_cu, hipStream_t s) {
+    const uint32_t flat = (std::min(paths, a.cap) + kPtBlock - 1) / kPtBlock;
     // persistent traversal grids: what the device holds at once, or fewer
     // when blocks_per_cu caps it
-    uint32_t ext_blocks = std::min<uint32_t>(flat, resident_blocks<k_sextend<kStack, kW>>(kPtBlock));
-    uint32_t any_blocks = std::min<uint32_t>(2 * flat, resident_blocks<k_sanyhit<kStack, kW>>(kPtBlock));
+    uint32_t ext_blocks = std::min<uint32_t>(flat, resident_blocks<k_bextend<kStack, kW>>(kPtBlock));
+    uint32_t any_blocks = std::min<uint32_t>(2 * flat, resident_blocks<k_banyhit<kStack, kW>>(kPtBlock));
     if (tune.blocks_per_cu) {
         ext_blocks = std::min<uint32_t>(ext_blocks, (uint32_t)n_cu * tune.blocks_per_cu);
         any_blocks = std::min<uint32_t>(any_blocks, (uint32_t)n_cu * tune.blocks_per_cu);
     }
-    hipLaunchKernelGGL(k_sregen, dim3(regen_blocks), dim3(kPtBlock), 0, s, sp, jobs, qi);
-    hipLaunchKernelGGL((k_sextend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, s, sp, jobs);
-    hipLaunchKernelGGL(k_sshade, dim3(flat), dim3(kPtBlock), 0, s, sp, jobs, qi);
-    hipLaunchKernelGGL((k_sanyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, s, sp, jobs, qi);
-    hipLaunchKernelGGL(k_sresolve, dim3(flat), dim3(kPtBlock), 0, s, sp, jobs, qi);
+    int qi = 0;
+    for (uint32_t b = 0; b < max_bounces; b++) {
+        hipLaunchKernelGGL((k_bextend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, s, a, qi);
+        hipLaunchKernelGGL(k_bzero2, dim3(1), dim3(1), 0, s, a.st.count + 2, a.st.count + (qi ^ 1));
+        hipLaunchKernelGGL(k_bshade, dim3(flat), dim3(kPtBlock), 0, s, a, jobs, qi);
+        hipLaunchKernelGGL((k_banyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, s, a);
+        hipLaunchKernelGGL(k_bgather, dim3(flat), dim3(kPtBlock), 0, s, a, qi, b);
+        qi ^= 1;
+    }
+    hipLaunchKernelGGL(k_bflush, dim3(flat), dim3(kPtBlock), 0, s, a, qi);
 }
 
-int stream_launch_iteration(const StreamPool &sp, const void *d_jobs, int qi, const StreamTune &tune, int n_cu,
-                            void *stream) {
+int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, uint32_t paths, uint32_t max_bounces,
+                          const StreamTune &tune, int n_cu, void *stream) {
     const DevJobSlot *jobs = (const DevJobSlot *)d_jobs;
     hipStream_t s = (hipStream_t)stream;
     const bool w8 = tune.width == 8;
     switch (tune.stack) {
-        case 32: w8 ? iteration_impl<32, 8>(sp, jobs, qi, tune, n_cu, s) : iteration_impl<32, 4>(sp, jobs, qi, tune, n_cu, s); break;
-        case 64: w8 ? iteration_impl<64, 8>(sp, jobs, qi, tune, n_cu, s) : iteration_impl<64, 4>(sp, jobs, qi, tune, n_cu, s); break;
-        case 128: w8 ? iteration_impl<128, 8>(sp, jobs, qi, tune, n_cu, s) : iteration_impl<128, 4>(sp, jobs, qi, tune, n_cu, s); break;
-        case 256: w8 ? iteration_impl<256, 8>(sp, jobs, qi, tune, n_cu, s) : iteration_impl<256, 4>(sp, jobs, qi, tune, n_cu, s); break;
+        case 32: w8 ? bounces_impl<32, 8>(a, jobs, paths, max_bounces, tune, n_cu, s) : bounces_impl<32, 4>(a, jobs, paths, max_bounces, tune, n_cu, s); break;
+        case 64: w8 ? bounces_impl<64, 8>(a, jobs, paths, max_bounces, tune, n_cu, s) : bounces_impl<64, 4>(a, jobs, paths, max_bounces, tune, n_cu, s); break;
+        case 128: w8 ? bounces_impl<128, 8>(a, jobs, paths, max_bounces, tune, n_cu, s) : bounces_impl<128, 4>(a, jobs, paths, max_bounces, tune, n_cu, s); break;
+        case 256: w8 ? bounces_impl<256, 8>(a, jobs, paths, max_bounces, tune, n_cu, s) : bounces_impl<256, 4>(a, jobs, paths, max_bounces, tune, n_cu, s); break;
         default: return set_error(RG_E_INVALID, "stack must be 32, 64, 128 or 256");
     }
-    return chk_s(hipGetLastError(), "stream iteration launch");
+    return chk_s(hipGetLastError(), "batch bounce launch");
 }
 
-int stream_launch_finish(const void *d_jobs, uint32_t slot, uint32_t npix, void *stream) {
-    hipLaunchKernelGGL(k_sfinish, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, (hipStream_t)stream,
+int stream_launch_accumulate(const BatchArgs &a, const GenChunk &c, float *d_sum, float *d_sumsq, void *stream) {
+    hipLaunchKernelGGL(k_baccumulate, dim3((c.npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0,
+                       (hipStream_t)stream, a.st.sample_out + c.first, c.npix, c.S, d_sum, d_sumsq);
+    return chk_s(hipGetLastError(), "batch accumulate launch");
+}
+
+int stream_launch_resolve(const void *d_jobs, uint32_t slot, uint32_t npix, void *stream) {
+    hipLaunchKernelGGL(k_bresolve, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, (hipStream_t)stream,
                        (const DevJobSlot *)d_jobs, slot);
-    return chk_s(hipGetLastError(), "stream finish launch");
+    return chk_s(hipGetLastError(), "batch resolve launch");
 }
 
 }  // namespace rg

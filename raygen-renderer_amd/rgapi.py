@@ -497,7 +497,7 @@ class Server:
     def stats(self):
         out = np.zeros(8, np.uint64)
         check(lib().rg_server_stats(self.h, _ptr(out)))
-        keys = ["requests", "samples", "rays", "shadow_rays", "iterations", "fb_bytes", "budget", "graphs"]
+        keys = ["requests", "samples", "rays", "shadow_rays", "generations", "fb_bytes", "budget", "graphs"]
         return dict(zip(keys, (int(x) for x in out)))
 
     def tune(self):

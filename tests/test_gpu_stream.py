@@ -41,6 +41,9 @@ JOBS = [
     (1, dict(width=128, height=96, spp=16, max_depth=6, samples_in_flight=3)),
     (0, dict(width=200, height=150, spp=1, max_depth=5, job=7)),
     (1, dict(width=17, height=200, spp=7, max_depth=4, firefly_clamp=0.0)),
+    # depth 1 beside depth 8: this job's paths must stop after their own
+    # max_depth + 2 events although the generation runs 10
+    (1, dict(width=80, height=60, spp=4, max_depth=1)),
 ]
 
 
@@ -64,7 +67,7 @@ def assert_same(got, want, what):
 
 
 def test_batch_matches_single_job_path(ctx, resident, single):
-    say("[stream] batch of 6 uneven jobs vs rg_render")
+    say("[stream] batch of 7 uneven jobs vs rg_render")
     outs, stats = rgapi.render_batch(ctx, resident, make_jobs())
     for i, (o, w) in enumerate(zip(outs, single)):
         assert_same(o, w, f"job {i}")
@@ -75,8 +78,11 @@ def test_batch_matches_single_job_path(ctx, resident, single):
 
 
 @pytest.mark.parametrize("tune", [
-    dict(pool_paths=4096, stack=256, tile=1, blocks_per_cu=4, iters_per_graph=2),
-    dict(pool_paths=1 << 20, tile=16, blocks_per_cu=16, iters_per_graph=8),
+    # generations of one chunk each, bounce loops replayed from graphs
+    dict(pool_paths=4096, stack=256, blocks_per_cu=4, iters_per_graph=1),
+    # several jobs' chunks per generation, direct launches
+    dict(pool_paths=1 << 16, blocks_per_cu=16),
+    dict(pool_paths=1 << 20, blocks_per_cu=16, iters_per_graph=1),
 ])
 def test_tuning_knobs_do_not_change_images(ctx, resident, single, tune):
     say(f"[stream] batch with tune {tune}")
@@ -98,7 +104,7 @@ def test_server_alone_equals_mixed(ctx, resident, single):
         info, alone = srv.wait(t, timeout=60.0)
         assert info is not None and info.status == 0
         assert_same(alone, single[3], "alone")
-        say("[stream] server: all six, staggered")
+        say("[stream] server: all seven, staggered")
         tickets = []
         for i, j in enumerate(jobs):
             tickets.append(srv.submit(j, want_sum=True))
@@ -111,7 +117,8 @@ def test_server_alone_equals_mixed(ctx, resident, single):
             assert info.samples == JOBS[i][1]["width"] * JOBS[i][1]["height"] * JOBS[i][1]["spp"]
             assert_same(out, single[i], f"mixed {i}")
         st = srv.stats()
-        assert st["requests"] == 7 and st["fb_bytes"] == 0 and st["budget"] == budget
+        assert st["requests"] == len(JOBS) + 1 and st["fb_bytes"] == 0 and st["budget"] == budget
+        assert st["generations"] >= 2
     finally:
         srv.close()
 
@@ -142,7 +149,8 @@ def test_autotune_returns_a_working_setting(ctx, resident, single):
     t = rgapi.autotune(ctx, resident, probe)
     d = t.as_dict()
     say(f"[stream] tuned: {d}")
-    assert d["pool_paths"] in (1 << 20, 1 << 21, 1 << 22) and d["stack"] in (32, 64, 128, 256)
+    assert d["pool_paths"] in (1 << 23, 1 << 24, 1 << 25) and d["stack"] in (32, 64, 128, 256)
+    assert d["iters_per_graph"] in (0, 1) and d["blocks_per_cu"] in (4, 8, 16)
     outs, _ = rgapi.render_batch(ctx, resident, make_jobs()[:2], tune=t)
     for i in range(2):
         assert_same(outs[i], single[i], f"tuned job {i}")
