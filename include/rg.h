@@ -325,7 +325,7 @@ typedef struct {
                                   (default: smallest that fits every BVH) */
     uint32_t tile;             /* unused, 0 or 1 */
     uint32_t blocks_per_cu;    /* cap on persistent traversal blocks per compute unit
-                                  (default 8) */
+                                  (default 0: as many as the device holds) */
     uint32_t iters_per_graph;  /* nonzero: replay each generation's bounce loop from a
                                   captured hipGraph (default 0: direct launches) */
     uint32_t pad;

@@ -107,7 +107,6 @@ struct Request {
     double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
     int pin = -1;
     float *d_hdr = nullptr, *d_sum = nullptr, *d_sumsq = nullptr;
-    DevBvh bvh{};
 };
 
 struct PinnedBuf {
@@ -161,8 +160,9 @@ class Engine {
                 return set_error(RG_E_UNSUPPORTED, "all scenes of one engine need the same BVH width");
             width = s->width;
             scenes.push_back(s);
+            scene_bvh.push_back(stream_job_bvh(*s));
         }
-        tune = {kPathsInFlight, 0, 8, 0, (uint32_t)width};
+        tune = {kPathsInFlight, 0, 0, 0, (uint32_t)width};
         if (t) {
             if (t->pool_paths) tune.budget = t->pool_paths;
             if (t->stack) tune.stack = t->stack;
@@ -178,8 +178,7 @@ class Engine {
         if (!stream_stack_ok(tune.stack, need))
             return set_error(RG_E_UNSUPPORTED, "traversal stack too small for the deepest scene BVH");
         if (tune.budget < 1024 || tune.budget > (1u << 26)) return set_error(RG_E_INVALID, "pool_paths out of range");
-        if (tune.blocks_per_cu == 0 || tune.blocks_per_cu > 32)
-            return set_error(RG_E_INVALID, "blocks_per_cu out of range");
+        if (tune.blocks_per_cu > 32) return set_error(RG_E_INVALID, "blocks_per_cu out of range");
         n_cu = stream_cu_count(c->device);
         if (n_cu <= 0) return set_error(RG_E_HIP, "cannot query compute units");
 
@@ -199,6 +198,7 @@ class Engine {
             {(void **)&st.scontrib, P * 32},  {(void **)&st.nshadow, P * 4},   {(void **)&st.q[0], P * 4},
             {(void **)&st.q[1], P * 4},       {(void **)&st.count, 64},        {(void **)&st.sample_out, P * 16},
             {(void **)&st.stats, 64},         {(void **)&args.chunk_of, P * 4},
+            {(void **)&args.mixed, P * 4},
             {&d_chunks, (uint64_t)kMaxChunks * sizeof(GenChunk)},
             {&d_jobs, (uint64_t)kMaxJobs * kDevJobBytes},
         };
@@ -430,7 +430,6 @@ class Engine {
             r->d_hdr = d.d_hdr;
             r->d_sum = d.d_sum;
             r->d_sumsq = d.d_sumsq;
-            r->bvh = stream_job_bvh(*d.scene);
             r->state = 1;
             r->t_admit = now_s();
             active.push_back(r);
@@ -438,16 +437,23 @@ class Engine {
         return RG_OK;
     }
 
-    // Packs the next chunks of the admitted jobs (first come, first served;
-    // a job's chunks in sample order) into one generation and launches it.
+    // Packs the next chunks of the admitted jobs of one scene (the scene of
+    // the oldest job with samples left; first come, first served; a job's
+    // chunks in sample order) into one generation and launches it.
     int launch_generation(bool &launched) {
         launched = false;
         std::vector<GenChunk> ch;
         std::vector<Request *> last;  // jobs whose final chunk is in this generation
         uint64_t total = 0;
         uint32_t bounces = 0;
+        uint32_t scene = ~0u;
+        for (Request *r : active)
+            if (r->state == 1) {
+                scene = r->scene;
+                break;
+            }
         for (Request *r : active) {
-            if (r->state != 1) continue;
+            if (r->state != 1 || r->scene != scene) continue;
             while (r->issued < r->st.spp && ch.size() < kMaxChunks) {
                 const uint32_t S = std::min(r->group, r->st.spp - r->issued);
                 const uint64_t paths = (uint64_t)r->npix * S;
@@ -455,7 +461,6 @@ class Engine {
                 if (total + paths > cap) break;
                 GenChunk c;
                 std::memset(&c, 0, sizeof c);
-                c.bv = r->bvh;
                 c.job = r->slot;
                 c.first = (uint32_t)total;
                 c.npix = r->npix;
@@ -492,7 +497,7 @@ class Engine {
         BatchArgs a = args;
         a.nchunks = (uint32_t)ch.size();
         if (int rc = stream_launch_camera(a, d_jobs, (uint32_t)total, stream)) return rc;
-        if (int rc = launch_bounces(a, (uint32_t)total, bounces)) return rc;
+        if (int rc = launch_bounces(a, scene, (uint32_t)total, bounces)) return rc;
         for (size_t k = 0; k < ch.size(); k++)
             if (int rc = stream_launch_accumulate(a, ch[k], chunk_jobs[k]->d_sum, chunk_jobs[k]->d_sumsq, stream))
                 return rc;
@@ -521,20 +526,21 @@ class Engine {
     // The bounce loop, replayed from a captured graph when tune.graphs is
     // set.  Its kernels read the queue lengths from device counters and the
     // chunks through the per-path chunk index, so one capture serves every
-    // generation with the same bounce count and slot count rounded up to a
-    // power of two (the extra blocks find no work).
-    int launch_bounces(const BatchArgs &a, uint32_t paths, uint32_t bounces) {
-        if (!tune.graphs) return stream_launch_bounces(a, d_jobs, paths, bounces, tune, n_cu, stream);
+    // generation of a scene with the same bounce count and slot count
+    // rounded up to a power of two (the extra blocks find no work).
+    int launch_bounces(const BatchArgs &a, uint32_t scene, uint32_t paths, uint32_t bounces) {
+        const DevBvh &bv = scene_bvh[scene];
+        if (!tune.graphs) return stream_launch_bounces(a, d_jobs, bv, paths, bounces, tune, n_cu, stream);
         uint32_t slots = 1024;
         while (slots < paths && slots < cap) slots <<= 1;
         slots = std::min(slots, cap);
-        const uint64_t key = ((uint64_t)bounces << 32) | slots;
+        const uint64_t key = ((uint64_t)scene << 40) | ((uint64_t)bounces << 32) | slots;
         auto it = graphs_by_bounces.find(key);
         if (it == graphs_by_bounces.end()) {
             hipGraph_t g = nullptr;
             if (int rc = hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "begin capture"))
                 return rc;
-            const int lrc = stream_launch_bounces(a, d_jobs, slots, bounces, tune, n_cu, stream);
+            const int lrc = stream_launch_bounces(a, d_jobs, bv, slots, bounces, tune, n_cu, stream);
             const hipError_t ce = hipStreamEndCapture(stream, &g);
             if (lrc) return lrc;
             if (int rc = hip_ok(ce, "end capture")) return rc;
@@ -671,6 +677,7 @@ class Engine {
 
     Ctx *ctx = nullptr;
     std::vector<Scene *> scenes;
+    std::vector<DevBvh> scene_bvh;
     int n_cu = 0;
     uint32_t cap = 0;
     hipStream_t stream = nullptr;
@@ -845,11 +852,11 @@ int rg_autotune(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const r
     };
     // coordinate descent: one knob at a time over its candidates, keeping
     // the best setting found so far for the others
-    rg_tune best_t{kPathsInFlight, stacks[0], 1, 8, 0, 0};
+    rg_tune best_t{kPathsInFlight, stacks[0], 1, 0, 0, 0};
     double best = 0.0;
     if (int rc = measure(best_t, best)) return rc;
     const std::vector<std::vector<uint32_t>> options = {
-        {1u << 23, 1u << 24, 1u << 25}, stacks, {1u}, {4u, 8u, 16u}, {0u, 1u}};
+        {1u << 23, 1u << 24, 1u << 25}, stacks, {1u}, {0u, 4u, 8u, 16u}, {0u, 1u}};
     for (size_t knob = 0; knob < options.size(); knob++) {
         for (uint32_t v : options[knob]) {
             rg_tune t = best_t;

@@ -4,10 +4,10 @@
 // A generation is a list of chunks; a chunk is one job's samples
 // [base, base + S) of every pixel, S being the single-job path's samples in
 // flight for that job (its accumulation group, capped by what is left).
-// Chunks of any number of jobs share the generation's path slots up to the
-// budget, and the bounce loop (extend, shade, shadow, gather) runs over one
-// compacted queue of all their paths, each ray on its own job's tree and
-// each event with its own job's scene record.  After the loop every chunk is
+// Chunks of any number of jobs of one scene share the generation's path
+// slots up to the budget, and the bounce loop (extend, shade, shadow,
+// gather) runs over one compacted queue of all their paths on the scene's
+// tree, each event with its own job's scene record.  After the loop every chunk is
 // summed per pixel into its job's sums exactly as the single-job path sums
 // one of its generations, so a job's image does not depend on what ran
 // beside it.
@@ -29,14 +29,13 @@ constexpr uint32_t kMaxChunks = 4096;  // chunks of one generation
 struct StreamTune {
     uint32_t budget;         // path slots of a generation
     uint32_t stack;          // traversal stack entries: 32, 64, 128 or 256
-    uint32_t blocks_per_cu;  // cap on persistent traversal blocks per compute unit
+    uint32_t blocks_per_cu;  // cap on persistent traversal blocks per compute unit (0: none)
     uint32_t graphs;         // replay each generation's bounce loop as a captured hipGraph
     uint32_t width;          // node width of every resident scene's BVH (4 or 8)
 };
 
 // One chunk of a generation.  Chunks are sorted by `first`.
 struct GenChunk {
-    DevBvh bv;             // the job's tree
     uint32_t job;          // job slot
     uint32_t first;        // first path slot of the chunk in the generation
     uint32_t npix, width;  // the job's image
@@ -50,8 +49,9 @@ struct GenChunk {
 // Device pointers of a generation, passed to every kernel by value: the
 // single-job path's path state, the chunk of each path slot, the chunk table.
 struct BatchArgs {
-    PTState st;
+    PTState st;            // count[3]: length of the mixed queue
     uint32_t *chunk_of;
+    uint32_t *mixed;       // paths shaded by the mixed-wave shade kernel
     const GenChunk *chunks;
     uint32_t nchunks;
     uint32_t cap;  // path slots of the buffers
@@ -66,7 +66,7 @@ struct JobDesc {
 
 // Writes the device job record (kDevJobBytes) for `d` into `out`.
 void stream_job_record(const JobDesc &d, void *out);
-// The job's tree as a generation chunk carries it.
+// A scene's tree as the traversal kernels take it.
 DevBvh stream_job_bvh(const Scene &s);
 // Grid geometry of the persistent kernels.
 int stream_cu_count(int device);
@@ -76,8 +76,8 @@ int stream_launch_camera(const BatchArgs &a, const void *d_jobs, uint32_t total,
 // counts; each path stops after its own chunk's) and the flush of paths
 // still alive after it; the per-path kernels are sized for `paths` slots
 // (at least the generation's paths, at most the buffers' slots).
-int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, uint32_t paths, uint32_t max_bounces,
-                          const StreamTune &tune, int n_cu, void *stream);
+int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, const DevBvh &bv, uint32_t paths,
+                          uint32_t max_bounces, const StreamTune &tune, int n_cu, void *stream);
 // Per-pixel sums of one chunk into its job's sums (the single-job path's
 // accumulation of one generation).
 int stream_launch_accumulate(const BatchArgs &a, const GenChunk &c, float *d_sum, float *d_sumsq, void *stream);

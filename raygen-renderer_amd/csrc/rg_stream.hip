@@ -7,12 +7,16 @@
 //           with the job's scene record (rayrenderer.cpp:860-934); the queue
 //           is the identity over all paths
 //   then max_bounces times, over the compacted queue of live paths:
-//   extend  closest hit, each ray on its job's tree (persistent waves, leaf
+//   extend  closest hit on the generation's tree (every chunk of a
+//           generation renders one scene; persistent waves, leaf
 //           postponement, lane refill)
-//   shade   one path event with the job's scene record, made wave-uniform
-//           (the same step() as the single-job path); the event's 0-2 NEE
-//           rays are queued with block-aggregated appends
-//   anyhit  the queued NEE rays, one per lane, on their job's tree
+//   shade   one path event with the job's scene record, wave-uniform for
+//           the lanes of the wave's first job (the same step() as the
+//           single-job path, at its register budget); paths of other jobs in
+//           the wave are shaded by a second kernel that loops over the
+//           distinct jobs; the event's 0-2 NEE rays are queued with
+//           block-aggregated appends
+//   anyhit  the queued NEE rays, one per lane
 //   gather  unoccluded contributions in event order; finished paths, and
 //           paths at their job's last event, write their radiance clamped
 //           by their job's firefly clamp, live ones are appended to the
@@ -97,9 +101,11 @@ __device__ __forceinline__ uint32_t find_chunk(const GenChunk *__restrict__ ch, 
 }
 
 __global__ void k_bset_u32(uint32_t *p, uint32_t v) { *p = v; }
-__global__ void k_bzero2(uint32_t *a, uint32_t *b) {
-    *a = 0;
-    *b = 0;
+// Clears the shadow-ray and mixed-queue lengths and the next path queue's.
+__global__ void k_bzero3(uint32_t *count, int next) {
+    count[2] = 0;
+    count[3] = 0;
+    count[next] = 0;
 }
 
 __global__ __launch_bounds__(kPtBlock) void k_bcamera(BatchArgs a, const DevJobSlot *__restrict__ jobs,
@@ -127,17 +133,14 @@ __global__ __launch_bounds__(kPtBlock) void k_bcamera(BatchArgs a, const DevJobS
     if (i == 0) atomicAdd(st.stats + 2, (unsigned long long)total);
 }
 
-// Closest hit of every live path (queue qi), each ray on its job's tree.
+// Closest hit of every live path (queue qi) on the generation's tree (every
+// chunk of a generation renders the same scene).
 struct BatchExtendSrc {
     const rg_ray *ray;
     const uint32_t *q;
     rg_hit *hit;
-    const uint32_t *chunk_of;
-    const GenChunk *chunks;
-    __device__ __forceinline__ bool load(uint64_t i, DevBvh &bv, PsRay &r) const {
-        const uint32_t p = q[i];
-        bv = chunks[chunk_of[p]].bv;
-        const rg_ray x = ray[p];
+    __device__ __forceinline__ bool load(uint64_t i, const DevBvh &, PsRay &r) const {
+        const rg_ray x = ray[q[i]];
         r = {x.o[0], x.o[1], x.o[2], x.d[0], x.d[1], x.d[2], x.tmin, x.tmax, 0u};
         return true;
     }
@@ -148,78 +151,126 @@ struct BatchExtendSrc {
 };
 
 template <int kStack, int kW>
-__global__ __launch_bounds__(kPtBlock) void k_bextend(BatchArgs a, int qi) {
+__global__ __launch_bounds__(kPtBlock) void k_bextend(DevBvh bv, BatchArgs a, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const PTState &st = a.st;
     const uint32_t count = st.count[qi];
     if (blockIdx.x == 0 && threadIdx.x == 0 && count) atomicAdd(st.stats + 0, (unsigned long long)count);
-    const DevBvh none{};
-    trace_ps_loop<kStack, kW, false, kPtBlock, BatchExtendSrc, true>(
-        none, BatchExtendSrc{st.ray, st.q[qi], st.hit, a.chunk_of, a.chunks}, count, lc);
+    trace_ps_loop<kStack, kW, false, kPtBlock>(bv, BatchExtendSrc{st.ray, st.q[qi], st.hit}, count, lc);
 }
 
-// One path event per live path with its job's scene record.
-__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bshade(
-    BatchArgs a, const DevJobSlot *__restrict__ jobs, int qi) {
-    __shared__ uint32_t lds[kPtBlock / 64 + 1];
-    const PTState &st = a.st;
-    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    const uint32_t count = st.count[qi];
-    if (blockIdx.x * kPtBlock >= count) return;  // whole block past the end
-    uint32_t p = 0, ns = 0;
-    if (i < count) {
-        p = st.q[qi][i];
-        const GenChunk &C = a.chunks[a.chunk_of[p]];
-        const uint32_t local = p - C.first;
-        const uint32_t pix = local % C.npix;
-        const int sample = (int)(C.sample_base + local / C.npix);
-        const int4 meta = st.meta[p];
-        with_job(jobs, C.job, [&](const DevJob &J) {
-            const PTScene &sc = J.sc;
-            Rng R = rng_for(sc, pix, sample, meta.z, (uint32_t)meta.w);
-            const rg_ray r = st.ray[p];
-            const float4 b = st.beta[p];
-            Path P;
-            P.o = mk(r.o[0], r.o[1], r.o[2]);
-            P.d = mk(r.d[0], r.d[1], r.d[2]);
-            P.beta = f4v(b);
-            P.bsdf_pdf = b.w;
-            P.tp = f4v(st.tp[p]);
-            P.L = f4v(st.L[p]);
-            P.passes = meta.x;
-            P.medium = meta.y;
-            P.alive = true;
-            Shadows S;
-            S.n = 0;
-            const rg_hit h = st.hit[p];
-            step(sc, R, P, S, h);
-            st.ray[p] = {{P.o.x, P.o.y, P.o.z}, 0.0f, {P.d.x, P.d.y, P.d.z}, MAX_RAY_DISTANCE};
-            st.beta[p] = v4(P.beta, P.bsdf_pdf);
-            st.tp[p] = v4(P.tp, 0.0f);
-            st.L[p] = v4(P.L, P.alive ? 1.0f : 0.0f);
-            st.meta[p] = make_int4(P.passes, P.medium, R.dim, (int)R.k);
-            store_shadows(S, p, st.sray, st.scontrib);
-            ns = (uint32_t)S.n;
-        });
-        st.nshadow[p] = ns;
-    }
+// One path event of live path p, one of chunk C's paths, with its job's
+// scene record.
+__device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk &C, const PTScene &sc, uint32_t p) {
+    const uint32_t local = p - C.first;
+    const uint32_t pix = local % C.npix;
+    const int sample = (int)(C.sample_base + local / C.npix);
+    const int4 meta = st.meta[p];
+    Rng R = rng_for(sc, pix, sample, meta.z, (uint32_t)meta.w);
+    const rg_ray r = st.ray[p];
+    const float4 b = st.beta[p];
+    Path P;
+    P.o = mk(r.o[0], r.o[1], r.o[2]);
+    P.d = mk(r.d[0], r.d[1], r.d[2]);
+    P.beta = f4v(b);
+    P.bsdf_pdf = b.w;
+    P.tp = f4v(st.tp[p]);
+    P.L = f4v(st.L[p]);
+    P.passes = meta.x;
+    P.medium = meta.y;
+    P.alive = true;
+    Shadows S;
+    S.n = 0;
+    const rg_hit h = st.hit[p];
+    step(sc, R, P, S, h);
+    st.ray[p] = {{P.o.x, P.o.y, P.o.z}, 0.0f, {P.d.x, P.d.y, P.d.z}, MAX_RAY_DISTANCE};
+    st.beta[p] = v4(P.beta, P.bsdf_pdf);
+    st.tp[p] = v4(P.tp, 0.0f);
+    st.L[p] = v4(P.L, P.alive ? 1.0f : 0.0f);
+    st.meta[p] = make_int4(P.passes, P.medium, R.dim, (int)R.k);
+    store_shadows(S, p, st.sray, st.scontrib);
+    st.nshadow[p] = (uint32_t)S.n;
+    return (uint32_t)S.n;
+}
+
+__device__ __forceinline__ void queue_shadows(const PTState &st, uint32_t p, uint32_t ns, uint32_t *lds) {
     for (uint32_t k = 0; k < 2; k++) {
         const uint32_t pos = block_slot(k < ns, st.count + 2, lds);
         if (k < ns) st.ssrc[pos] = 2 * p + k;
     }
 }
 
-// Any-hit test of every queued NEE ray on its job's tree.
+// Copies a record whose address is the same in every lane into scalar
+// registers.  Called in wave-uniform control flow: loads from a uniform
+// address inside a divergent branch are vector loads, and the copy keeps the
+// shade body's reads of the record out of vector registers.
+template <class T>
+__device__ __forceinline__ void uniform_copy(T &dst, const T *src) {
+    static_assert(sizeof(T) % 4 == 0, "record of whole words");
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(src);
+    uint32_t *d = reinterpret_cast<uint32_t *>(&dst);
+#pragma unroll
+    for (uint32_t k = 0; k < sizeof(T) / 4; k++) d[k] = __builtin_amdgcn_readfirstlane(s[k]);
+}
+
+// One path event per live path (queue qi).  The lanes whose path is in the
+// chunk of the wave's first lane run with that chunk and its job's scene
+// record in scalar registers (the single-job shade kernel's register budget
+// and occupancy); the other paths of the wave (waves that straddle two
+// chunks) go to the mixed queue for k_bshade_mixed.
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_bshade(
+    BatchArgs a, const DevJobSlot *__restrict__ jobs, int qi) {
+    __shared__ uint32_t lds[kPtBlock / 64 + 1];
+    const PTState &st = a.st;
+    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+    const uint32_t count = st.count[qi];
+    if (blockIdx.x * kPtBlock >= count) return;  // whole block past the end
+    uint32_t p = 0, ns = 0, c = 0xFFFFFFFFu;
+    if (i < count) {
+        p = st.q[qi][i];
+        c = a.chunk_of[p];
+    }
+    // the live lanes of a wave are a prefix of it: lane 0 is live when any is
+    const uint32_t c0 = __builtin_amdgcn_readfirstlane(c);
+    const bool mine = i < count && c == c0;
+    wave_append(i < count && c != c0, p, a.mixed, st.count + 3);
+    GenChunk C;
+    PTScene sc;
+    if (c0 != 0xFFFFFFFFu) {  // wave-uniform
+        uniform_copy(C, &a.chunks[c0]);
+        uniform_copy(sc, &jobs[C.job].j.sc);
+    }
+    if (mine) ns = shade_one(st, C, sc, p);
+    queue_shadows(st, p, ns, lds);
+}
+
+// The mixed queue's path events, each with its own job's record (one
+// distinct job at a time across the wave).  A fixed grid walks the queue in
+// block-sized steps.
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bshade_mixed(
+    BatchArgs a, const DevJobSlot *__restrict__ jobs) {
+    __shared__ uint32_t lds[kPtBlock / 64 + 1];
+    const PTState &st = a.st;
+    const uint32_t count = st.count[3];
+    for (uint32_t base = blockIdx.x * kPtBlock; base < count; base += gridDim.x * kPtBlock) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t p = 0, ns = 0;
+        if (i < count) {
+            p = a.mixed[i];
+            const GenChunk &C = a.chunks[a.chunk_of[p]];
+            with_job(jobs, C.job, [&](const DevJob &J) { ns = shade_one(st, C, J.sc, p); });
+        }
+        queue_shadows(st, p, ns, lds);
+    }
+}
+
+// Any-hit test of every queued NEE ray on the generation's tree.
 struct BatchShadowSrc {
     const rg_ray *sray;
     const uint32_t *ssrc;
     uint8_t *socc;
-    const uint32_t *chunk_of;
-    const GenChunk *chunks;
-    __device__ __forceinline__ bool load(uint64_t i, DevBvh &bv, PsRay &r) const {
-        const uint32_t src = ssrc[i];
-        bv = chunks[chunk_of[src >> 1]].bv;
-        const rg_ray x = sray[src];
+    __device__ __forceinline__ bool load(uint64_t i, const DevBvh &, PsRay &r) const {
+        const rg_ray x = sray[ssrc[i]];
         r = {x.o[0], x.o[1], x.o[2], x.d[0], x.d[1], x.d[2], 0.0f, x.tmax, 1u << (int)x.tmin};
         return true;
     }
@@ -228,14 +279,12 @@ struct BatchShadowSrc {
 };
 
 template <int kStack, int kW>
-__global__ __launch_bounds__(kPtBlock) void k_banyhit(BatchArgs a) {
+__global__ __launch_bounds__(kPtBlock) void k_banyhit(DevBvh bv, BatchArgs a) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const PTState &st = a.st;
     const uint32_t n = st.count[2];
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(st.stats + 1, (unsigned long long)n);
-    const DevBvh none{};
-    trace_ps_loop<kStack, kW, true, kPtBlock, BatchShadowSrc, true>(
-        none, BatchShadowSrc{st.sray, st.ssrc, st.socc, a.chunk_of, a.chunks}, n, lc);
+    trace_ps_loop<kStack, kW, true, kPtBlock>(bv, BatchShadowSrc{st.sray, st.ssrc, st.socc}, n, lc);
 }
 
 __device__ __forceinline__ float4 clamped(float4 L, float clamp_max) {
@@ -357,8 +406,8 @@ int stream_launch_camera(const BatchArgs &a, const void *d_jobs, uint32_t total,
 }
 
 template <int kStack, int kW>
-static void bounces_impl(const BatchArgs &a, const DevJobSlot *jobs, uint32_t paths, uint32_t max_bounces,
-                         const StreamTune &tune, int n_cu, hipStream_t s) {
+static void bounces_impl(const BatchArgs &a, const DevJobSlot *jobs, const DevBvh &bv, uint32_t paths,
+                         uint32_t max_bounces, const StreamTune &tune, int n_cu, hipStream_t s) {
     const uint32_t flat = (std::min(paths, a.cap) + kPtBlock - 1) / kPtBlock;
     // persistent traversal grids: what the device holds at once, or fewer
     // when blocks_per_cu caps it
@@ -368,28 +417,31 @@ static void bounces_impl(const BatchArgs &a, const DevJobSlot *jobs, uint32_t pa
         ext_blocks = std::min<uint32_t>(ext_blocks, (uint32_t)n_cu * tune.blocks_per_cu);
         any_blocks = std::min<uint32_t>(any_blocks, (uint32_t)n_cu * tune.blocks_per_cu);
     }
+    // the mixed queue holds a few waves' worth of paths per chunk boundary
+    const uint32_t mixed_blocks = std::min<uint32_t>(flat, 1024);
     int qi = 0;
     for (uint32_t b = 0; b < max_bounces; b++) {
-        hipLaunchKernelGGL((k_bextend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, s, a, qi);
-        hipLaunchKernelGGL(k_bzero2, dim3(1), dim3(1), 0, s, a.st.count + 2, a.st.count + (qi ^ 1));
+        hipLaunchKernelGGL((k_bextend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, s, bv, a, qi);
+        hipLaunchKernelGGL(k_bzero3, dim3(1), dim3(1), 0, s, a.st.count, qi ^ 1);
         hipLaunchKernelGGL(k_bshade, dim3(flat), dim3(kPtBlock), 0, s, a, jobs, qi);
-        hipLaunchKernelGGL((k_banyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, s, a);
+        hipLaunchKernelGGL(k_bshade_mixed, dim3(mixed_blocks), dim3(kPtBlock), 0, s, a, jobs);
+        hipLaunchKernelGGL((k_banyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, s, bv, a);
         hipLaunchKernelGGL(k_bgather, dim3(flat), dim3(kPtBlock), 0, s, a, qi, b);
         qi ^= 1;
     }
     hipLaunchKernelGGL(k_bflush, dim3(flat), dim3(kPtBlock), 0, s, a, qi);
 }
 
-int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, uint32_t paths, uint32_t max_bounces,
-                          const StreamTune &tune, int n_cu, void *stream) {
+int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, const DevBvh &bv, uint32_t paths,
+                          uint32_t max_bounces, const StreamTune &tune, int n_cu, void *stream) {
     const DevJobSlot *jobs = (const DevJobSlot *)d_jobs;
     hipStream_t s = (hipStream_t)stream;
     const bool w8 = tune.width == 8;
     switch (tune.stack) {
-        case 32: w8 ? bounces_impl<32, 8>(a, jobs, paths, max_bounces, tune, n_cu, s) : bounces_impl<32, 4>(a, jobs, paths, max_bounces, tune, n_cu, s); break;
-        case 64: w8 ? bounces_impl<64, 8>(a, jobs, paths, max_bounces, tune, n_cu, s) : bounces_impl<64, 4>(a, jobs, paths, max_bounces, tune, n_cu, s); break;
-        case 128: w8 ? bounces_impl<128, 8>(a, jobs, paths, max_bounces, tune, n_cu, s) : bounces_impl<128, 4>(a, jobs, paths, max_bounces, tune, n_cu, s); break;
-        case 256: w8 ? bounces_impl<256, 8>(a, jobs, paths, max_bounces, tune, n_cu, s) : bounces_impl<256, 4>(a, jobs, paths, max_bounces, tune, n_cu, s); break;
+        case 32: w8 ? bounces_impl<32, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, s) : bounces_impl<32, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, s); break;
+        case 64: w8 ? bounces_impl<64, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, s) : bounces_impl<64, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, s); break;
+        case 128: w8 ? bounces_impl<128, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, s) : bounces_impl<128, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, s); break;
+        case 256: w8 ? bounces_impl<256, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, s) : bounces_impl<256, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, s); break;
         default: return set_error(RG_E_INVALID, "stack must be 32, 64, 128 or 256");
     }
     return chk_s(hipGetLastError(), "batch bounce launch");

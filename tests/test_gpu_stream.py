@@ -150,7 +150,7 @@ def test_autotune_returns_a_working_setting(ctx, resident, single):
     d = t.as_dict()
     say(f"[stream] tuned: {d}")
     assert d["pool_paths"] in (1 << 23, 1 << 24, 1 << 25) and d["stack"] in (32, 64, 128, 256)
-    assert d["iters_per_graph"] in (0, 1) and d["blocks_per_cu"] in (4, 8, 16)
+    assert d["iters_per_graph"] in (0, 1) and d["blocks_per_cu"] in (0, 4, 8, 16)
     outs, _ = rgapi.render_batch(ctx, resident, make_jobs()[:2], tune=t)
     for i in range(2):
         assert_same(outs[i], single[i], f"tuned job {i}")
