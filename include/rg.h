@@ -323,7 +323,8 @@ typedef struct {
                                   generation always takes at least one chunk) */
     uint32_t stack;            /* traversal stack entries 32/64/128/256
                                   (default: smallest that fits every BVH) */
-    uint32_t tile;             /* unused, 0 or 1 */
+    uint32_t lanes;            /* streams with their own path state that consecutive
+                                  generations rotate over, 1 to 4 (default 3) */
     uint32_t blocks_per_cu;    /* cap on persistent traversal blocks per compute unit
                                   (default 0: as many as the device holds) */
     uint32_t iters_per_graph;  /* nonzero: replay each generation's bounce loop from a
@@ -338,8 +339,8 @@ typedef struct {
 int rg_render_batch(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const rg_job *jobs, uint32_t n_jobs,
                     const rg_tune *tune, float *const *hdr_out, float *const *sum_out, uint64_t stats[4]);
 
-/* Times every candidate of a small grid of generation size / stack / blocks
- * per CU / graph settings on the probe jobs (NULL: one 512x384, 8 spp,
+/* Times every candidate of a small grid of generation size / stack / lanes /
+ * blocks per CU / graph settings on the probe jobs (NULL: one 512x384, 8 spp,
  * depth 6 job per scene) and returns the fastest. */
 int rg_autotune(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const rg_job *probe, uint32_t n_probe,
                 rg_tune *out);

@@ -128,12 +128,25 @@ uint32_t plain_group(const rg_settings &st) {
     return std::min(S, spp);
 }
 
-// A generation launched and not yet known to be finished: its event, the
-// jobs whose last samples it carried, and the counter snapshot it copies.
+// A generation launched and not yet known to be finished: its event, its
+// lane and the jobs whose last samples it carried.
 struct Inflight {
     hipEvent_t ev = nullptr;
-    int stage = 0;  // pinned chunk-table / counter staging buffer in use
+    int lane = 0;
     std::vector<Request *> finishing;
+};
+
+// One stream with its own path state: consecutive generations rotate over
+// the lanes, so the tail bounces of one overlap the next one's work.
+struct Lane {
+    hipStream_t stream = nullptr;
+    BatchArgs args{};
+    void *d_chunks = nullptr;
+    GenChunk *h_chunks = nullptr;          // pinned chunk-table staging
+    unsigned long long *h_stats = nullptr;  // pinned counter snapshot
+    hipEvent_t stage_ev = nullptr;          // staging of the last generation consumed
+    hipEvent_t acc_ev = nullptr;            // sums and results of the last generation queued
+    unsigned long long stats[3] = {0, 0, 0};
 };
 
 }  // namespace
@@ -148,7 +161,9 @@ class Engine {
         if (c->device < 0) return set_error(RG_E_NODEVICE, "host-only context has no device");
         if (n == 0) return set_error(RG_E_INVALID, "no scenes");
         if (int rc = dev_set_device(c->device)) return rc;
-        if (int rc = hip_ok(hipStreamCreate(&stream), "stream create")) return rc;
+        for (Lane &ln : lanes)
+            if (int rc = hip_ok(hipStreamCreate(&ln.stream), "stream create")) return rc;
+        hipStream_t stream = lanes[0].stream;
         uint32_t need = 0;
         int width = 0;
         for (uint32_t i = 0; i < n; i++) {
@@ -163,7 +178,9 @@ class Engine {
             scene_bvh.push_back(stream_job_bvh(*s));
         }
         tune = {kPathsInFlight, 0, 0, 0, (uint32_t)width};
+        n_lanes = 3;
         if (t) {
+            if (t->lanes) n_lanes = (int)t->lanes;
             if (t->pool_paths) tune.budget = t->pool_paths;
             if (t->stack) tune.stack = t->stack;
             if (t->blocks_per_cu) tune.blocks_per_cu = t->blocks_per_cu;
@@ -179,6 +196,7 @@ class Engine {
             return set_error(RG_E_UNSUPPORTED, "traversal stack too small for the deepest scene BVH");
         if (tune.budget < 1024 || tune.budget > (1u << 26)) return set_error(RG_E_INVALID, "pool_paths out of range");
         if (tune.blocks_per_cu > 32) return set_error(RG_E_INVALID, "blocks_per_cu out of range");
+        if (n_lanes < 1 || n_lanes > kMaxLanes) return set_error(RG_E_INVALID, "lanes must be 1 to 4");
         n_cu = stream_cu_count(c->device);
         if (n_cu <= 0) return set_error(RG_E_HIP, "cannot query compute units");
 
@@ -186,43 +204,47 @@ class Engine {
         // least one whole chunk of the default grouping
         cap = std::max(tune.budget, kPathsInFlight);
         const uint64_t P = cap;
-        PTState &st = args.st;
-        std::memset(&st, 0, sizeof st);
-        struct A {
-            void **p;
-            uint64_t bytes;
-        } allocs[] = {
-            {(void **)&st.ray, P * 32},       {(void **)&st.hit, P * 16},      {(void **)&st.beta, P * 16},
-            {(void **)&st.tp, P * 16},        {(void **)&st.L, P * 16},        {(void **)&st.meta, P * 16},
-            {(void **)&st.sray, P * 64},      {(void **)&st.ssrc, P * 8},      {(void **)&st.socc, P * 2},
-            {(void **)&st.scontrib, P * 32},  {(void **)&st.nshadow, P * 4},   {(void **)&st.q[0], P * 4},
-            {(void **)&st.q[1], P * 4},       {(void **)&st.count, 64},        {(void **)&st.sample_out, P * 16},
-            {(void **)&st.stats, 64},         {(void **)&args.chunk_of, P * 4},
-            {(void **)&args.mixed, P * 4},
-            {&d_chunks, (uint64_t)kMaxChunks * sizeof(GenChunk)},
-            {&d_jobs, (uint64_t)kMaxJobs * kDevJobBytes},
-        };
-        for (auto &a : allocs) {
-            if (int rc = dev_malloc(a.p, a.bytes)) return rc;
-            dev_allocs.push_back(*a.p);
+        for (int l = 0; l < n_lanes; l++) {
+            Lane &ln = lanes[l];
+            PTState &st = ln.args.st;
+            std::memset(&st, 0, sizeof st);
+            struct A {
+                void **p;
+                uint64_t bytes;
+            } allocs[] = {
+                {(void **)&st.ray, P * 32},      {(void **)&st.hit, P * 16},      {(void **)&st.beta, P * 16},
+                {(void **)&st.tp, P * 16},       {(void **)&st.L, P * 16},        {(void **)&st.meta, P * 16},
+                {(void **)&st.sray, P * 64},     {(void **)&st.ssrc, P * 8},      {(void **)&st.socc, P * 2},
+                {(void **)&st.scontrib, P * 32}, {(void **)&st.nshadow, P * 4},   {(void **)&st.q[0], P * 4},
+                {(void **)&st.q[1], P * 4},      {(void **)&st.count, 64},        {(void **)&st.sample_out, P * 16},
+                {(void **)&st.stats, 64},        {(void **)&ln.args.chunk_of, P * 4},
+                {(void **)&ln.args.mixed, P * 4},
+                {&ln.d_chunks, (uint64_t)kMaxChunks * sizeof(GenChunk)},
+            };
+            for (auto &a : allocs) {
+                if (int rc = dev_malloc(a.p, a.bytes)) return rc;
+                dev_allocs.push_back(*a.p);
+            }
+            ln.args.chunks = (const GenChunk *)ln.d_chunks;
+            ln.args.cap = cap;
+            if (int rc = launch_fill_u32(st.stats, 0u, 16, stream)) return rc;
+            if (int rc = hip_ok(hipHostMalloc((void **)&ln.h_chunks, (size_t)kMaxChunks * sizeof(GenChunk)),
+                                "pinned alloc"))
+                return rc;
+            if (int rc = hip_ok(hipHostMalloc((void **)&ln.h_stats, 64), "pinned alloc")) return rc;
+            for (hipEvent_t *e : {&ln.stage_ev, &ln.acc_ev}) {
+                if (int rc = hip_ok(hipEventCreateWithFlags(e, hipEventDisableTiming), "event")) return rc;
+                if (int rc = hip_ok(hipEventRecord(*e, stream), "event record")) return rc;
+            }
         }
-        args.chunks = (const GenChunk *)d_chunks;
-        args.cap = cap;
-        if (int rc = launch_fill_u32(st.stats, 0u, 16, stream)) return rc;
+        if (int rc = dev_malloc(&d_jobs, (uint64_t)kMaxJobs * kDevJobBytes)) return rc;
+        dev_allocs.push_back(d_jobs);
         if (budget == 0) budget = 16ull << 30;
         if (int rc = dev_malloc((void **)&d_arena, budget)) return rc;
         dev_allocs.push_back(d_arena);
         arena.reset(budget);
-
         if (int rc = hip_ok(hipHostMalloc((void **)&h_jobs, (size_t)kMaxJobs * kDevJobBytes), "pinned alloc")) return rc;
-        for (int k = 0; k < 2; k++) {
-            if (int rc = hip_ok(hipHostMalloc((void **)&h_chunks[k], (size_t)kMaxChunks * sizeof(GenChunk)),
-                                "pinned alloc"))
-                return rc;
-            if (int rc = hip_ok(hipHostMalloc((void **)&h_stats[k], 64), "pinned alloc")) return rc;
-            if (int rc = hip_ok(hipEventCreateWithFlags(&stage_ev[k], hipEventDisableTiming), "event")) return rc;
-            if (int rc = hip_ok(hipEventRecord(stage_ev[k], stream), "event record")) return rc;
-        }
+        if (int rc = hip_ok(hipEventCreateWithFlags(&adm_ev, hipEventDisableTiming), "event")) return rc;
         for (uint32_t s = kMaxJobs; s-- > 0;) free_slots.push_back(s);
         return hip_ok(hipStreamSynchronize(stream), "engine init");
     }
@@ -236,7 +258,7 @@ class Engine {
             cv_work.notify_all();
             worker.join();
         }
-        if (stream) (void)hipStreamSynchronize(stream);
+        sync_lanes();
         {
             std::lock_guard<std::mutex> lk(mu);
             for (auto &kv : requests) {
@@ -248,9 +270,9 @@ class Engine {
             }
         }
         cv_done.notify_all();
-        for (auto &g : graphs_by_bounces)
+        for (auto &g : graphs)
             if (g.second) (void)hipGraphExecDestroy(g.second);
-        graphs_by_bounces.clear();
+        graphs.clear();
         for (Inflight &f : inflight)
             if (f.ev) (void)hipEventDestroy(f.ev);
         inflight.clear();
@@ -258,20 +280,21 @@ class Engine {
         spare_events.clear();
         for (void *p : dev_allocs) dev_free(p);
         dev_allocs.clear();
-        for (char *p : {h_jobs, (char *)h_chunks[0], (char *)h_chunks[1], (char *)h_stats[0], (char *)h_stats[1]})
-            if (p) (void)hipHostFree(p);
+        if (h_jobs) (void)hipHostFree(h_jobs);
         h_jobs = nullptr;
-        h_chunks[0] = h_chunks[1] = nullptr;
-        h_stats[0] = h_stats[1] = nullptr;
         for (PinnedBuf &b : pins)
             if (b.p) (void)hipHostFree(b.p);
         pins.clear();
-        for (int k = 0; k < 2; k++) {
-            if (stage_ev[k]) (void)hipEventDestroy(stage_ev[k]);
-            stage_ev[k] = nullptr;
+        if (adm_ev) (void)hipEventDestroy(adm_ev);
+        adm_ev = nullptr;
+        for (Lane &ln : lanes) {
+            if (ln.h_chunks) (void)hipHostFree(ln.h_chunks);
+            if (ln.h_stats) (void)hipHostFree(ln.h_stats);
+            for (hipEvent_t e : {ln.stage_ev, ln.acc_ev})
+                if (e) (void)hipEventDestroy(e);
+            if (ln.stream) (void)hipStreamDestroy(ln.stream);
+            ln = Lane{};
         }
-        if (stream) (void)hipStreamDestroy(stream);
-        stream = nullptr;
     }
 
     // Validates and queues one job; thread-safe.
@@ -323,9 +346,10 @@ class Engine {
             }
         }
         if (int rc = admit()) return fail_all(rc, nullptr);
-        // at most two generations in flight: later requests join the next
-        // generation instead of queueing behind everything already launched
-        while (inflight.size() >= 2)
+        // one generation running per lane and one queued: later requests
+        // join the next generation instead of queueing behind everything
+        // already launched
+        while (inflight.size() >= (size_t)n_lanes + 1)
             if (int rc = retire(true)) return fail_all(rc, nullptr);
         bool launched = false;
         if (int rc = launch_generation(launched)) return fail_all(rc, nullptr);
@@ -396,11 +420,17 @@ class Engine {
         out[7] = n_replays;
     }
 
+    int lanes_used() const { return n_lanes; }
+
     StreamTune tune{};
     uint64_t dev_stats[4] = {0, 0, 0, 0};
 
   private:
+    // Uploads go to the stream of the lane the next generation runs on;
+    // adm_ev, recorded after them, orders the other lane's next generation.
     int admit() {
+        hipStream_t stream = lanes[next_lane].stream;
+        bool any = false;
         while (!pending.empty() && !free_slots.empty()) {
             Request *r = pending.front();
             uint64_t off = 0;
@@ -433,6 +463,14 @@ class Engine {
             r->state = 1;
             r->t_admit = now_s();
             active.push_back(r);
+            any = true;
+        }
+        if (any) {
+            // earlier admissions this lane has not waited for are folded in,
+            // so the new event covers every upload so far
+            if (int rc = wait_admissions(next_lane)) return rc;
+            if (int rc = hip_ok(hipEventRecord(adm_ev, stream), "event record")) return rc;
+            lane_adm_seq[next_lane] = ++adm_seq;
         }
         return RG_OK;
     }
@@ -485,28 +523,35 @@ class Engine {
             chunk_jobs.clear();
             return RG_OK;
         }
-        // pinned staging of this generation's chunk table and counters
-        const int b = stage_cur;
-        stage_cur ^= 1;
-        if (int rc = hip_ok(hipEventSynchronize(stage_ev[b]), "chunk staging")) return rc;
-        std::memcpy(h_chunks[b], ch.data(), ch.size() * sizeof(GenChunk));
-        if (int rc = hip_ok(hipMemcpyAsync(d_chunks, h_chunks[b], ch.size() * sizeof(GenChunk), hipMemcpyHostToDevice,
-                                           stream),
+        const int L = next_lane;
+        Lane &ln = lanes[L];
+        hipStream_t stream = ln.stream;
+        // the lane's pinned chunk table and counter snapshot are free again
+        if (int rc = hip_ok(hipEventSynchronize(ln.stage_ev), "chunk staging")) return rc;
+        std::memcpy(ln.h_chunks, ch.data(), ch.size() * sizeof(GenChunk));
+        if (int rc = wait_admissions(L)) return rc;
+        if (int rc = hip_ok(hipMemcpyAsync(ln.d_chunks, ln.h_chunks, ch.size() * sizeof(GenChunk),
+                                           hipMemcpyHostToDevice, stream),
                             "chunk upload"))
             return rc;
-        BatchArgs a = args;
+        BatchArgs a = ln.args;
         a.nchunks = (uint32_t)ch.size();
         if (int rc = stream_launch_camera(a, d_jobs, (uint32_t)total, stream)) return rc;
-        if (int rc = launch_bounces(a, scene, (uint32_t)total, bounces)) return rc;
+        if (int rc = launch_bounces(L, a, scene, (uint32_t)total, bounces)) return rc;
+        // a job's chunks are summed in generation order across the lanes
+        if (last_lane >= 0 && last_lane != L)
+            if (int rc = hip_ok(hipStreamWaitEvent(stream, lanes[last_lane].acc_ev, 0), "stream wait")) return rc;
         for (size_t k = 0; k < ch.size(); k++)
             if (int rc = stream_launch_accumulate(a, ch[k], chunk_jobs[k]->d_sum, chunk_jobs[k]->d_sumsq, stream))
                 return rc;
         chunk_jobs.clear();
         for (Request *r : last)
-            if (int rc = finish(r)) return rc;
-        if (int rc = hip_ok(hipMemcpyAsync(h_stats[b], args.st.stats, 32, hipMemcpyDeviceToHost, stream), "stats copy"))
+            if (int rc = finish(r, stream)) return rc;
+        if (int rc = hip_ok(hipEventRecord(ln.acc_ev, stream), "event record")) return rc;
+        if (int rc = hip_ok(hipMemcpyAsync(ln.h_stats, ln.args.st.stats, 32, hipMemcpyDeviceToHost, stream),
+                            "stats copy"))
             return rc;
-        if (int rc = hip_ok(hipEventRecord(stage_ev[b], stream), "event record")) return rc;
+        if (int rc = hip_ok(hipEventRecord(ln.stage_ev, stream), "event record")) return rc;
         Inflight f;
         if (!spare_events.empty()) {
             f.ev = spare_events.back();
@@ -515,10 +560,12 @@ class Engine {
             return rc;
         }
         if (int rc = hip_ok(hipEventRecord(f.ev, stream), "event record")) return rc;
-        f.stage = b;
+        f.lane = L;
         f.finishing = std::move(last);
         inflight.push_back(std::move(f));
         n_generations++;
+        last_lane = L;
+        next_lane = (L + 1) % n_lanes;
         launched = true;
         return RG_OK;
     }
@@ -528,15 +575,16 @@ class Engine {
     // chunks through the per-path chunk index, so one capture serves every
     // generation of a scene with the same bounce count and slot count
     // rounded up to a power of two (the extra blocks find no work).
-    int launch_bounces(const BatchArgs &a, uint32_t scene, uint32_t paths, uint32_t bounces) {
+    int launch_bounces(int L, const BatchArgs &a, uint32_t scene, uint32_t paths, uint32_t bounces) {
+        hipStream_t stream = lanes[L].stream;
         const DevBvh &bv = scene_bvh[scene];
         if (!tune.graphs) return stream_launch_bounces(a, d_jobs, bv, paths, bounces, tune, n_cu, stream);
         uint32_t slots = 1024;
         while (slots < paths && slots < cap) slots <<= 1;
         slots = std::min(slots, cap);
-        const uint64_t key = ((uint64_t)scene << 40) | ((uint64_t)bounces << 32) | slots;
-        auto it = graphs_by_bounces.find(key);
-        if (it == graphs_by_bounces.end()) {
+        const uint64_t key = ((uint64_t)L << 56) | ((uint64_t)scene << 40) | ((uint64_t)bounces << 32) | slots;
+        auto it = graphs.find(key);
+        if (it == graphs.end()) {
             hipGraph_t g = nullptr;
             if (int rc = hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "begin capture"))
                 return rc;
@@ -548,14 +596,14 @@ class Engine {
             const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (int rc = hip_ok(ie, "graph instantiate")) return rc;
-            it = graphs_by_bounces.emplace(key, ex).first;
+            it = graphs.emplace(key, ex).first;
         }
         n_replays++;
         return hip_ok(hipGraphLaunch(it->second, stream), "graph launch");
     }
 
     // Queues a job's HDR resolve and the copies of its results.
-    int finish(Request *r) {
+    int finish(Request *r, hipStream_t stream) {
         if (int rc = stream_launch_resolve(d_jobs, r->slot, r->npix, stream)) return rc;
         const size_t hb = (size_t)r->npix * 16, sb = (size_t)r->npix * 12;
         const size_t need = (r->out_hdr ? hb : 0) + (r->out_sum ? sb : 0) + (r->out_sumsq ? sb : 0);
@@ -603,8 +651,12 @@ class Engine {
             if (q == hipErrorNotReady && block) q = hipEventSynchronize(f.ev);
             if (q == hipErrorNotReady) return RG_OK;
             if (int rc = hip_ok(q, "generation wait")) return rc;
-            const unsigned long long *S = (const unsigned long long *)h_stats[f.stage];
-            for (int k = 0; k < 3; k++) dev_stats[k] = S[k];
+            Lane &ln = lanes[f.lane];
+            for (int k = 0; k < 3; k++) ln.stats[k] = ln.h_stats[k];
+            for (int k = 0; k < 3; k++) {
+                dev_stats[k] = 0;
+                for (const Lane &x : lanes) dev_stats[k] += x.stats[k];
+            }
             dev_stats[3] = n_generations;
             for (Request *r : f.finishing) deliver(r);
             spare_events.push_back(f.ev);
@@ -645,7 +697,7 @@ class Engine {
 
     int fail_all(int rc, const char *msg) {
         std::string text = msg ? std::string(msg) : std::string(rg_last_error());
-        if (stream) (void)hipStreamSynchronize(stream);
+        sync_lanes();
         for (Inflight &f : inflight) spare_events.push_back(f.ev);
         inflight.clear();
         chunk_jobs.clear();
@@ -678,21 +730,32 @@ class Engine {
     Ctx *ctx = nullptr;
     std::vector<Scene *> scenes;
     std::vector<DevBvh> scene_bvh;
+    // Orders lane L after every admission upload queued so far.
+    int wait_admissions(int L) {
+        if (lane_adm_seq[L] == adm_seq) return RG_OK;
+        lane_adm_seq[L] = adm_seq;
+        return hip_ok(hipStreamWaitEvent(lanes[L].stream, adm_ev, 0), "stream wait");
+    }
+
+    void sync_lanes() {
+        for (Lane &ln : lanes)
+            if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+    }
+
     int n_cu = 0;
     uint32_t cap = 0;
-    hipStream_t stream = nullptr;
-    std::map<uint64_t, hipGraphExec_t> graphs_by_bounces;
-    BatchArgs args{};
+    static constexpr int kMaxLanes = 4;
+    Lane lanes[kMaxLanes];
+    int n_lanes = 3, next_lane = 0, last_lane = -1;
+    hipEvent_t adm_ev = nullptr;  // after every admission upload so far
+    uint64_t adm_seq = 0;         // admissions recorded into adm_ev
+    uint64_t lane_adm_seq[kMaxLanes] = {0, 0, 0, 0};  // the last of them each lane is ordered after
+    std::map<uint64_t, hipGraphExec_t> graphs;  // bounce loops by lane, scene, bounces, slots
     void *d_jobs = nullptr;
-    void *d_chunks = nullptr;
     char *d_arena = nullptr;
     std::vector<void *> dev_allocs;
     Arena arena;
     char *h_jobs = nullptr;
-    GenChunk *h_chunks[2] = {nullptr, nullptr};
-    unsigned long long *h_stats[2] = {nullptr, nullptr};
-    hipEvent_t stage_ev[2] = {nullptr, nullptr};
-    int stage_cur = 0;
     std::deque<Inflight> inflight;
     std::vector<hipEvent_t> spare_events;
     std::vector<Request *> chunk_jobs;  // job of each chunk of the generation being built
@@ -796,7 +859,7 @@ int rg_server_stats(rg_server *srv, uint64_t out[8]) {
 int rg_server_get_tune(rg_server *srv, rg_tune *out) {
     if (!srv || !out) return set_error(RG_E_INVALID, "null argument");
     const StreamTune &t = srv->eng->tune;
-    *out = {t.budget, t.stack, 1, t.blocks_per_cu, t.graphs ? 2u : 0u, 0};
+    *out = {t.budget, t.stack, (uint32_t)srv->eng->lanes_used(), t.blocks_per_cu, t.graphs ? 1u : 0u, 0};
     return RG_OK;
 }
 
@@ -852,15 +915,15 @@ int rg_autotune(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const r
     };
     // coordinate descent: one knob at a time over its candidates, keeping
     // the best setting found so far for the others
-    rg_tune best_t{kPathsInFlight, stacks[0], 1, 0, 0, 0};
+    rg_tune best_t{kPathsInFlight, stacks[0], 3, 0, 0, 0};
     double best = 0.0;
     if (int rc = measure(best_t, best)) return rc;
     const std::vector<std::vector<uint32_t>> options = {
-        {1u << 23, 1u << 24, 1u << 25}, stacks, {1u}, {0u, 4u, 8u, 16u}, {0u, 1u}};
+        {1u << 23, 1u << 24, 1u << 25}, stacks, {1u, 2u, 3u, 4u}, {0u, 4u, 8u, 16u}, {0u, 1u}};
     for (size_t knob = 0; knob < options.size(); knob++) {
         for (uint32_t v : options[knob]) {
             rg_tune t = best_t;
-            uint32_t *field[] = {&t.pool_paths, &t.stack, &t.tile, &t.blocks_per_cu, &t.iters_per_graph};
+            uint32_t *field[] = {&t.pool_paths, &t.stack, &t.lanes, &t.blocks_per_cu, &t.iters_per_graph};
             if (*field[knob] == v) continue;
             *field[knob] = v;
             double dt = 0.0;

@@ -399,7 +399,7 @@ class Job(ctypes.Structure):
 
 
 class Tune(ctypes.Structure):
-    _fields_ = [("pool_paths", ctypes.c_uint32), ("stack", ctypes.c_uint32), ("tile", ctypes.c_uint32),
+    _fields_ = [("pool_paths", ctypes.c_uint32), ("stack", ctypes.c_uint32), ("lanes", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("iters_per_graph", ctypes.c_uint32),
                 ("pad", ctypes.c_uint32)]
 

@@ -80,9 +80,10 @@ def test_batch_matches_single_job_path(ctx, resident, single):
 @pytest.mark.parametrize("tune", [
     # generations of one chunk each, bounce loops replayed from graphs
     dict(pool_paths=4096, stack=256, blocks_per_cu=4, iters_per_graph=1),
-    # several jobs' chunks per generation, direct launches
-    dict(pool_paths=1 << 16, blocks_per_cu=16),
-    dict(pool_paths=1 << 20, blocks_per_cu=16, iters_per_graph=1),
+    # several jobs' chunks per generation, direct launches, one lane
+    dict(pool_paths=1 << 16, blocks_per_cu=16, lanes=1),
+    dict(pool_paths=1 << 20, blocks_per_cu=16, iters_per_graph=1, lanes=2),
+    dict(pool_paths=1 << 18, lanes=4),
 ])
 def test_tuning_knobs_do_not_change_images(ctx, resident, single, tune):
     say(f"[stream] batch with tune {tune}")
@@ -150,7 +151,7 @@ def test_autotune_returns_a_working_setting(ctx, resident, single):
     d = t.as_dict()
     say(f"[stream] tuned: {d}")
     assert d["pool_paths"] in (1 << 23, 1 << 24, 1 << 25) and d["stack"] in (32, 64, 128, 256)
-    assert d["iters_per_graph"] in (0, 1) and d["blocks_per_cu"] in (0, 4, 8, 16)
+    assert d["iters_per_graph"] in (0, 1) and d["blocks_per_cu"] in (0, 4, 8, 16) and d["lanes"] in (1, 2, 3, 4)
     outs, _ = rgapi.render_batch(ctx, resident, make_jobs()[:2], tune=t)
     for i in range(2):
         assert_same(outs[i], single[i], f"tuned job {i}")
