@@ -111,13 +111,17 @@ struct WideTest<4> {
         const float4 hy = *reinterpret_cast<const float4 *>(nd->hiy);
         const float4 hz = *reinterpret_cast<const float4 *>(nd->hiz);
         const uint4 ch = *reinterpret_cast<const uint4 *>(nd->child);
-        const uint2 ordw = *reinterpret_cast<const uint2 *>(nd->order);
-        ord = (oct < 4 ? (ordw.x >> (8 * oct)) : (ordw.y >> (8 * (oct - 4)))) & 0xFFu;
+        // the octant's order byte by a computed address: no branch, and the
+        // load issues with the box loads
+        const uint32_t ow = reinterpret_cast<const uint32_t *>(nd->order)[oct >> 2];
+        ord = (ow >> (8 * (oct & 3))) & 0xFFu;
         cref[0] = ch.x; cref[1] = ch.y; cref[2] = ch.z; cref[3] = ch.w;
-        ok[0] = ch.x != kEmpty && slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, tmax, tm[0]);
-        ok[1] = ch.y != kEmpty && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, tmax, tm[1]);
-        ok[2] = ch.z != kEmpty && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, tmax, tm[2]);
-        ok[3] = ch.w != kEmpty && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, tmax, tm[3]);
+        // every slot's slab test runs (an empty slot's zero box is masked
+        // off): straight-line code instead of a branch per child
+        ok[0] = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, tmax, tm[0]) & (ch.x != kEmpty);
+        ok[1] = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, tmax, tm[1]) & (ch.y != kEmpty);
+        ok[2] = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, tmax, tm[2]) & (ch.z != kEmpty);
+        ok[3] = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, tmax, tm[3]) & (ch.w != kEmpty);
     }
 };
 
@@ -140,13 +144,13 @@ struct WideTest<8> {
             const uint4 ch = *reinterpret_cast<const uint4 *>(nd->child + 4 * h);
             const int b = 4 * h;
             cref[b] = ch.x; cref[b + 1] = ch.y; cref[b + 2] = ch.z; cref[b + 3] = ch.w;
-            ok[b] = ch.x != kEmpty && slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, tmax, tm[b]);
+            ok[b] = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ox, oy, oz, ix, iy, iz, tmax, tm[b]) & (ch.x != kEmpty);
             ok[b + 1] =
-                ch.y != kEmpty && slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, tmax, tm[b + 1]);
+                slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ox, oy, oz, ix, iy, iz, tmax, tm[b + 1]) & (ch.y != kEmpty);
             ok[b + 2] =
-                ch.z != kEmpty && slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, tmax, tm[b + 2]);
+                slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ox, oy, oz, ix, iy, iz, tmax, tm[b + 2]) & (ch.z != kEmpty);
             ok[b + 3] =
-                ch.w != kEmpty && slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, tmax, tm[b + 3]);
+                slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ox, oy, oz, ix, iy, iz, tmax, tm[b + 3]) & (ch.w != kEmpty);
         }
     }
 };

@@ -155,7 +155,9 @@ class Engine {
   public:
     ~Engine() { shutdown(); }
 
-    int init(Ctx *c, rg_scene *const *sc, uint32_t n, uint64_t budget, const rg_tune *t) {
+    // paths_hint (0: unknown): the most path slots a generation can use, the
+    // total samples of a known job list.
+    int init(Ctx *c, rg_scene *const *sc, uint32_t n, uint64_t budget, const rg_tune *t, uint64_t paths_hint = 0) {
         ctx = c;
         if (!c || (n && !sc)) return set_error(RG_E_INVALID, "null argument");
         if (c->device < 0) return set_error(RG_E_NODEVICE, "host-only context has no device");
@@ -203,6 +205,7 @@ class Engine {
         // path slots: a generation holds up to `budget` paths, and always at
         // least one whole chunk of the default grouping
         cap = std::max(tune.budget, kPathsInFlight);
+        if (paths_hint) cap = (uint32_t)std::min<uint64_t>(cap, std::max<uint64_t>(paths_hint, 1024));
         const uint64_t P = cap;
         for (int l = 0; l < n_lanes; l++) {
             Lane &ln = lanes[l];
@@ -800,6 +803,13 @@ int batch_on(Engine &e, const rg_job *jobs, uint32_t n, float *const *hdr_out, f
     return RG_OK;
 }
 
+// Samples of a job list: no generation holds more paths.
+uint64_t batch_paths(const rg_job *jobs, uint32_t n) {
+    uint64_t t = 0;
+    for (uint32_t i = 0; i < n; i++) t += (uint64_t)jobs[i].settings.width * jobs[i].settings.height * jobs[i].settings.spp;
+    return std::max<uint64_t>(t, 1);
+}
+
 uint64_t batch_budget(const rg_job *jobs, uint32_t n) {
     uint64_t need = 0, big = 0;
     for (uint32_t i = 0; i < n; i++) {
@@ -821,7 +831,7 @@ int rg_render_batch(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, con
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c || (n_jobs && !jobs)) return set_error(RG_E_INVALID, "null argument");
     Engine e;
-    if (int rc = e.init(c, scenes, n_scenes, batch_budget(jobs, n_jobs), tune)) return rc;
+    if (int rc = e.init(c, scenes, n_scenes, batch_budget(jobs, n_jobs), tune, batch_paths(jobs, n_jobs))) return rc;
     if (int rc = batch_on(e, jobs, n_jobs, hdr_out, sum_out)) return rc;
     if (stats)
         for (int k = 0; k < 4; k++) stats[k] = e.dev_stats[k];
@@ -905,7 +915,8 @@ int rg_autotune(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const r
     // the second timed (graph, pinned buffers and caches warm)
     auto measure = [&](const rg_tune &t, double &dt) -> int {
         Engine e;
-        if (int rc = e.init(c, scenes, n_scenes, budget, &t)) return rc;
+        if (int rc = e.init(c, scenes, n_scenes, budget, &t, batch_paths(jobs.data(), (uint32_t)jobs.size())))
+            return rc;
         for (int rep = 0; rep < 2; rep++) {
             const double t0 = now_s();
             if (int rc = batch_on(e, jobs.data(), (uint32_t)jobs.size(), nullptr, nullptr)) return rc;

@@ -169,3 +169,34 @@ def test_batch_on_eight_wide_trees(ctx, single):
     with pytest.raises(rgapi.RGError) as e:
         rgapi.render_batch(ctx, [r4, r8[1]], make_jobs()[:1])
     assert e.value.code == -6
+
+
+def test_cancel_and_destroy_with_work_in_flight():
+    """rg_cancel on the engine's context fails the batch and the server's
+    requests with RG_E_CANCELLED; a server destroyed with requests still
+    queued or rendering returns and leaves the device usable."""
+    c2 = rgapi.Context(0)
+    room = rgapi.RenderScene(c2, scenes.cube_room())
+    srv = rgapi.Server(c2, [room])
+    try:
+        say("[stream] destroy with requests in flight")
+        for _ in range(4):
+            srv.submit(rgapi.make_job(0, width=512, height=512, spp=64, max_depth=5))
+    finally:
+        srv.close()
+    # the context still renders after the server is gone
+    out = room.render(rgapi.default_settings(width=32, height=32, spp=2, max_depth=3))
+    assert np.isfinite(out["hdr"]).all()
+    say("[stream] cancel, then batch and server")
+    c2.cancel()
+    with pytest.raises(rgapi.RGError) as e:
+        rgapi.render_batch(c2, [room], [rgapi.make_job(0, width=32, height=32, spp=2)])
+    assert e.value.code == -5
+    srv = rgapi.Server(c2, [room])
+    try:
+        t = srv.submit(rgapi.make_job(0, width=32, height=32, spp=2))
+        with pytest.raises(rgapi.RGError) as e:
+            srv.wait(t, timeout=60.0)
+        assert e.value.code == -5
+    finally:
+        srv.close()
