@@ -70,10 +70,14 @@ __global__ __launch_bounds__(kPtBlock) void k_camera(PTScene sc, PTState st, uin
         if (i == 0) atomicAdd(st.stats + 2, (unsigned long long)npix * S);
         return;
     }
-    wave_append(active, i, st.q[0], st.count + 0);
-    const uint64_t m = __ballot(active ? 1 : 0);
-    if (m && __lane_id() == (uint32_t)__ffsll((unsigned long long)m) - 1)
-        atomicAdd(st.stats + 2, (unsigned long long)__popcll(m));
+    // live samples of the active tiles: one queue atomic and one counter
+    // atomic per block (same-address atomics from every wave of a
+    // 1080p pass cost more than the rays)
+    __shared__ uint32_t lds[kPtBlock / 64 + 1];
+    const uint32_t pos = block_slot(active, st.count + 0, lds);
+    if (active) st.q[0][pos] = i;
+    const int live = __syncthreads_count(active ? 1 : 0);
+    if (threadIdx.x == 0 && live) atomicAdd(st.stats + 2, (unsigned long long)live);
 }
 
 // Closest hit of every live path's ray (queue qi), persistent waves.
