@@ -51,6 +51,11 @@ def cpu_baseline(v9, rays_np, seconds, threads):
         pops += int(cnt[:, 0].sum())
         tests += int(cnt[:, 1].sum())
         done += len(r)
+    # the reference's own default is one render thread (rayrenderer.h:44)
+    n1 = min(len(rays_np), 1 << 17)
+    t0 = time.perf_counter()
+    oracle.trace_closest(nodes, perm, v9, rays_np[:n1], threads=1)
+    one = n1 / (time.perf_counter() - t0) / 1e6
     return {
         "value": done / busy / 1e6,
         "unit": "Mrays/s",
@@ -59,6 +64,8 @@ def cpu_baseline(v9, rays_np, seconds, threads):
         "sample": f"first {done} of the {len(rays_np)} rays, closest-hit, oracle binary-tree "
                   f"traversal with {threads} threads (4096-ray chunks, atomic work claiming); "
                   f"BVH build {build_s:.2f}s excluded",
+        "value_1_thread": round(one, 3),
+        "sample_1_thread": f"first {n1} rays, the same traversal on 1 thread",
     }, pops / done, tests / done
 
 
