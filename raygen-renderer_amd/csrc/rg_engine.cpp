@@ -415,18 +415,19 @@ class Engine {
         std::lock_guard<std::mutex> lk(mu);
         out[0] = n_done;
         out[1] = samples_done;
-        out[2] = dev_stats[0];
-        out[3] = dev_stats[1];
-        out[4] = n_generations;
-        out[5] = arena.used();
+        out[2] = dev_stats[0].load();
+        out[3] = dev_stats[1].load();
+        out[4] = n_generations.load();
+        out[5] = fb_used.load();
         out[6] = arena.size();
-        out[7] = n_replays;
+        out[7] = n_replays.load();
     }
 
     int lanes_used() const { return n_lanes; }
 
     StreamTune tune{};
-    uint64_t dev_stats[4] = {0, 0, 0, 0};
+    // written by the engine loop, read by rg_server_stats from any thread
+    std::atomic<uint64_t> dev_stats[4] = {{0}, {0}, {0}, {0}};
 
   private:
     // Uploads go to the stream of the lane the next generation runs on;
@@ -438,6 +439,7 @@ class Engine {
             Request *r = pending.front();
             uint64_t off = 0;
             if (!arena.alloc(r->bytes, off)) break;  // first come, first served
+            fb_used = arena.used();
             pending.pop_front();
             r->off = off;
             r->slot = free_slots.back();
@@ -657,10 +659,11 @@ class Engine {
             Lane &ln = lanes[f.lane];
             for (int k = 0; k < 3; k++) ln.stats[k] = ln.h_stats[k];
             for (int k = 0; k < 3; k++) {
-                dev_stats[k] = 0;
-                for (const Lane &x : lanes) dev_stats[k] += x.stats[k];
+                uint64_t v = 0;
+                for (const Lane &x : lanes) v += x.stats[k];
+                dev_stats[k] = v;
             }
-            dev_stats[3] = n_generations;
+            dev_stats[3] = n_generations.load();
             for (Request *r : f.finishing) deliver(r);
             spare_events.push_back(f.ev);
             inflight.pop_front();
@@ -686,6 +689,7 @@ class Engine {
             r->pin = -1;
         }
         arena.release(r->off, r->bytes);
+        fb_used = arena.used();
         free_slots.push_back(r->slot);
         active.erase(std::find(active.begin(), active.end(), r));
         {
@@ -710,6 +714,7 @@ class Engine {
             arena.release(r->off, r->bytes);
             free_slots.push_back(r->slot);
         }
+        fb_used = arena.used();
         active.clear();
         {
             std::lock_guard<std::mutex> lk(mu);
@@ -775,7 +780,8 @@ class Engine {
     bool stop = false;
     std::thread worker;
     std::string failure;
-    uint64_t n_done = 0, samples_done = 0, n_generations = 0, n_replays = 0;
+    uint64_t n_done = 0, samples_done = 0;
+    std::atomic<uint64_t> n_generations{0}, n_replays{0}, fb_used{0};
 };
 
 }  // namespace rg
