@@ -89,8 +89,21 @@ const char *rg_last_error(void);
 int rg_set_stream(rg_ctx *ctx, void *hip_stream);
 int rg_synchronize(rg_ctx *ctx);
 /* Cooperative cancel (RayRenderer::cancelRequested, rayrenderer.h:427):
- * async-safe; observed between launches. */
+ * async-safe; observed between launches.  Like the reference, which clears
+ * the flag when a render starts (rayrenderer.cpp:537), rg_render and
+ * rg_render_batch clear it on entry, so it cancels the render in flight
+ * (call it from another thread or from the progress callback); a request
+ * server fails its queued and running requests and clears it. */
 void rg_cancel(rg_ctx *ctx);
+
+/* Progress of rg_render (RendererSettings::progressCallback,
+ * rayrenderer.h:317, called at rayrenderer.cpp:804-806 and :1155-1156): fn
+ * receives the fraction of the sample budget whose paths have finished,
+ * non-decreasing, on the thread that called rg_render (a generation behind
+ * the GPU, which keeps running), and 1.0 when the render is done.  NULL
+ * removes it. */
+typedef void (*rg_progress_fn)(float progress, void *user);
+int rg_set_progress(rg_ctx *ctx, rg_progress_fn fn, void *user);
 
 /* ---- scene geometry ----------------------------------------------------- */
 /* Replaces RayRenderer::transformScene's triangle list (rayrenderer.cpp:157-363):

@@ -99,6 +99,14 @@ void rg_cancel(rg_ctx *ctx) {
     if (c) c->cancel.store(true);
 }
 
+int rg_set_progress(rg_ctx *ctx, rg_progress_fn fn, void *user) {
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    if (!c) return set_error(RG_E_INVALID, "null context");
+    c->progress = fn;
+    c->progress_user = user;
+    return RG_OK;
+}
+
 int rg_scene_create(rg_ctx *ctx, const float *v9, uint32_t n_tris, rg_scene **out) {
     if (!ctx || !out) return set_error(RG_E_INVALID, "null argument");
     if (n_tris > 0 && !v9) return set_error(RG_E_INVALID, "v9 is null");
@@ -214,7 +222,6 @@ int rg_trace_closest(rg_ctx *ctx, const rg_scene *scene, const rg_ray *d_rays, r
     if (int rc = need_device(c)) return rc;
     if (!s || (n && (!d_rays || !d_hits))) return set_error(RG_E_INVALID, "null argument");
     if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
-    if (c->cancel.load()) return set_error(RG_E_CANCELLED, "cancelled");
     return launch_closest(*s, d_rays, d_hits, n, c->stream);
 }
 
@@ -226,7 +233,6 @@ int rg_trace_any(rg_ctx *ctx, const rg_scene *scene, const rg_ray *d_rays, uint8
     if (!s || (n && (!d_rays || !d_occluded))) return set_error(RG_E_INVALID, "null argument");
     if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
     if (pred < RG_PRED_ALL || pred > RG_PRED_ENV) return set_error(RG_E_INVALID, "unknown predicate");
-    if (c->cancel.load()) return set_error(RG_E_CANCELLED, "cancelled");
     return launch_any(*s, d_rays, d_occluded, n, pred, c->stream);
 }
 

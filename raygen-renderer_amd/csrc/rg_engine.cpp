@@ -340,7 +340,8 @@ class Engine {
 
     // One round of the loop: admit, launch the next generation, deliver.
     int step() {
-        if (ctx->cancel.load()) return fail_all(RG_E_CANCELLED, "cancelled");
+        // rg_cancel fails what is queued and running; later requests run
+        if (ctx->cancel.exchange(false)) return fail_all(RG_E_CANCELLED, "cancelled");
         {
             std::lock_guard<std::mutex> lk(mu);
             while (!inbox.empty()) {
@@ -836,6 +837,7 @@ int rg_render_batch(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, con
                     const rg_tune *tune, float *const *hdr_out, float *const *sum_out, uint64_t stats[4]) {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     if (!c || (n_jobs && !jobs)) return set_error(RG_E_INVALID, "null argument");
+    c->cancel.store(false);  // as rg_render: a batch clears the flag when it starts
     Engine e;
     if (int rc = e.init(c, scenes, n_scenes, batch_budget(jobs, n_jobs), tune, batch_paths(jobs, n_jobs))) return rc;
     if (int rc = batch_on(e, jobs, n_jobs, hdr_out, sum_out)) return rc;

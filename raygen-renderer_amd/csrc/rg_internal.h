@@ -111,6 +111,9 @@ struct Ctx {
     void *stream = nullptr;        // hipStream_t in use
     void *own_stream = nullptr;    // stream created by rg_create
     std::atomic<bool> cancel{false};
+    rg_progress_fn progress = nullptr;  // rg_render's progress callback
+    void *progress_user = nullptr;
+    void *progress_ev[2] = {nullptr, nullptr};  // hipEvent_t after the last two generations
     PTBuffers pt;
     PostBuffers post;
     uint64_t last_stats[3] = {0, 0, 0};

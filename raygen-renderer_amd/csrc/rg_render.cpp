@@ -286,7 +286,13 @@ int prepare_scene(Scene *s, void *stream) {
     return RG_OK;
 }
 
-void release_pt(Ctx *c) { free_buffers(c->pt); }
+void release_pt(Ctx *c) {
+    free_buffers(c->pt);
+    for (void *&e : c->progress_ev) {
+        if (e) (void)hipEventDestroy((hipEvent_t)e);
+        e = nullptr;
+    }
+}
 void release_pt_scene(Scene *s) { free_pt_scene(s); }
 
 }  // namespace rg
@@ -478,7 +484,8 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     if (st.width == 0 || st.height == 0 || st.width > 16384 || st.height > 16384)
         return set_error(RG_E_INVALID, "resolution must be 1..16384 per axis");
     if (st.max_depth < 1 || st.max_depth > 64) return set_error(RG_E_INVALID, "max_depth must be 1..64");
-    if (c->cancel.load()) return set_error(RG_E_CANCELLED, "cancelled");
+    // a render clears the cancel flag when it starts (rayrenderer.cpp:537)
+    c->cancel.store(false);
     if (int rc = dev_set_device(c->device)) return rc;
     void *stream = c->stream;
     if (int rc = prepare_scene(s, stream)) return rc;
@@ -524,6 +531,23 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     if (int rc = launch_fill_u32(d_nspp, 0u, ntiles, stream)) return rc;
     if (int rc = launch_fill_u32(B.stats, 0u, 6, stream)) return rc;
 
+    // Progress: after queueing a generation the host waits for the one
+    // before it (the GPU keeps the new one) and reports the share of the
+    // sample budget finished so far.
+    float reported = 0.0f;
+    auto report = [&](float f) {
+        if (c->progress && f > reported) {
+            reported = f;
+            c->progress(f, c->progress_user);
+        }
+    };
+    if (c->progress)
+        for (void *&e : c->progress_ev)
+            if (!e)
+                if (int rc = hip_ok(hipEventCreateWithFlags((hipEvent_t *)&e, hipEventDisableTiming), "event"))
+                    return rc;
+    int gen = 0;
+    float queued_frac = 0.0f;  // budget share of the samples queued before the newest generation
     auto run_samples = [&](uint32_t first, uint32_t count, const uint8_t *mask) -> int {
         uint32_t done = 0;
         while (done < count) {
@@ -533,6 +557,18 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
                                            stream))
                 return rc;
             done += n;
+            if (c->progress) {
+                hipEvent_t ev = (hipEvent_t)c->progress_ev[gen & 1];
+                if (int rc = hip_ok(hipEventRecord(ev, (hipStream_t)stream), "event record")) return rc;
+                if (gen > 0) {
+                    if (int rc = hip_ok(hipEventSynchronize((hipEvent_t)c->progress_ev[(gen - 1) & 1]),
+                                        "progress wait"))
+                        return rc;
+                    report(queued_frac);
+                }
+                queued_frac = (float)(first + done) / (float)spp;
+                gen++;
+            }
         }
         return RG_OK;
     };
@@ -575,7 +611,9 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     if (nspp)
         if (int rc = dev_download(nspp, d_nspp, (size_t)ntiles * 4, stream)) return rc;
     if (int rc = dev_download(c->last_stats, B.stats, sizeof c->last_stats, stream)) return rc;
-    return hip_ok(hipStreamSynchronize((hipStream_t)stream), "render sync");
+    if (int rc = hip_ok(hipStreamSynchronize((hipStream_t)stream), "render sync")) return rc;
+    report(1.0f);
+    return RG_OK;
 }
 
 int rg_render_stats(rg_ctx *ctx, uint64_t out[3]) {

@@ -34,6 +34,9 @@ class BvhParams(ctypes.Structure):
 
 _lib = None
 
+# void (*)(float progress, void *user)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_float, ctypes.c_void_p)
+
 
 def lib():
     """Load librg.so once; raise RGError when it is not built."""
@@ -56,6 +59,7 @@ def lib():
             "rg_destroy": (None, [vp]),
             "rg_last_error": (ctypes.c_char_p, []),
             "rg_set_stream": (i32, [vp, vp]),
+            "rg_set_progress": (i32, [vp, vp, vp]),
             "rg_synchronize": (i32, [vp]),
             "rg_cancel": (None, [vp]),
             "rg_scene_create": (i32, [vp, vp, u32, pp]),
@@ -131,6 +135,16 @@ class Context:
 
     def cancel(self):
         lib().rg_cancel(self.h)
+
+    def set_progress(self, fn):
+        """rg_set_progress: fn(progress: float) on the rendering thread, or
+        None to remove it."""
+        if fn is None:
+            self._progress = None
+            check(lib().rg_set_progress(self.h, None, None))
+            return
+        self._progress = PROGRESS_FN(lambda p, user: fn(float(p)))
+        check(lib().rg_set_progress(self.h, ctypes.cast(self._progress, ctypes.c_void_p), None))
 
     def close(self):
         if self.h:

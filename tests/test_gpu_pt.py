@@ -132,10 +132,48 @@ def test_render_errors_and_cancel(ctx):
         rgapi.check(rgapi.lib().rg_render(ctx.h, s.h, rgapi.ctypes.byref(st), None, None, None, None))
     c2 = rgapi.Context(0)
     rs = rgapi.RenderScene(c2, sc)
+    # as the reference (rayrenderer.cpp:537), a render clears an earlier cancel
     c2.cancel()
+    want = rs.render(st)
+    # cancelled from the progress callback, on the rendering thread
+    big = rgapi.default_settings(width=64, height=64, spp=64, max_depth=5, samples_in_flight=4)
+    seen = []
+
+    def cancel_at_first(p):
+        seen.append(p)
+        c2.cancel()
+    c2.set_progress(cancel_at_first)
     with pytest.raises(rgapi.RGError) as e:
-        rs.render(st)
-    assert e.value.code == -5
+        rs.render(big)
+    assert e.value.code == -5 and len(seen) == 1 and 0.0 < seen[0] < 1.0
+    c2.set_progress(None)
+    again = rs.render(st)
+    assert np.array_equal(want["hdr"], again["hdr"])
+
+
+def test_progress_callback(ctx):
+    """Progress reaches 1.0 through non-decreasing fractions of the sample
+    budget, one per finished generation, and does not change the image."""
+    rs = rgapi.RenderScene(ctx, scenes.cube_room())
+    st = rgapi.default_settings(width=48, height=32, spp=20, max_depth=5, samples_in_flight=3)
+    plain = rs.render(st)
+    seen = []
+    ctx.set_progress(seen.append)
+    try:
+        out = rs.render(st)
+        ada = rgapi.default_settings(width=48, height=32, spp=32, max_depth=5, adaptive=1,
+                                     adaptive_base=4, adaptive_threshold=0.05)
+        seen_ada_start = len(seen)
+        rs.render(ada)
+    finally:
+        ctx.set_progress(None)
+    assert np.array_equal(plain["hdr"], out["hdr"])
+    first = seen[:seen_ada_start]
+    # 7 generations of 3, 3, ..., 2 samples: the first six are reported when
+    # the next is queued, then 1.0
+    np.testing.assert_allclose(first, [3 / 20, 6 / 20, 9 / 20, 12 / 20, 15 / 20, 18 / 20, 1.0], rtol=1e-6)
+    second = seen[seen_ada_start:]
+    assert second[-1] == 1.0 and all(a < b for a, b in zip(second, second[1:]))
 
 
 def test_loaded_json_scene_matches_single_lobe_restatement(ctx):

@@ -172,31 +172,51 @@ def test_batch_on_eight_wide_trees(ctx, single):
 
 
 def test_cancel_and_destroy_with_work_in_flight():
-    """rg_cancel on the engine's context fails the batch and the server's
-    requests with RG_E_CANCELLED; a server destroyed with requests still
-    queued or rendering returns and leaves the device usable."""
+    """A server destroyed with requests queued or rendering returns and leaves
+    the device usable; rg_cancel cancels the batch in flight (a batch clears
+    an earlier cancel when it starts, as rg_render does) and fails a
+    server's queued and running requests, after which it serves again."""
+    import threading
+
     c2 = rgapi.Context(0)
     room = rgapi.RenderScene(c2, scenes.cube_room())
+    big = rgapi.make_job(0, width=1024, height=1024, spp=1024, max_depth=5)
+    small = rgapi.make_job(0, width=32, height=32, spp=2)
     srv = rgapi.Server(c2, [room])
     try:
         say("[stream] destroy with requests in flight")
         for _ in range(4):
-            srv.submit(rgapi.make_job(0, width=512, height=512, spp=64, max_depth=5))
+            srv.submit(big)
     finally:
         srv.close()
-    # the context still renders after the server is gone
     out = room.render(rgapi.default_settings(width=32, height=32, spp=2, max_depth=3))
     assert np.isfinite(out["hdr"]).all()
-    say("[stream] cancel, then batch and server")
+
+    say("[stream] cancel before a batch, then during one")
     c2.cancel()
-    with pytest.raises(rgapi.RGError) as e:
-        rgapi.render_batch(c2, [room], [rgapi.make_job(0, width=32, height=32, spp=2)])
-    assert e.value.code == -5
+    outs, _ = rgapi.render_batch(c2, [room], [small])
+    assert np.isfinite(outs[0]["hdr"]).all()
+    timer = threading.Timer(0.3, c2.cancel)
+    timer.start()
+    try:
+        with pytest.raises(rgapi.RGError) as e:
+            rgapi.render_batch(c2, [room], [big] * 3, want_sum=False)
+        assert e.value.code == -5
+    finally:
+        timer.join()
+
+    say("[stream] cancel a server's running requests, then serve again")
     srv = rgapi.Server(c2, [room])
     try:
-        t = srv.submit(rgapi.make_job(0, width=32, height=32, spp=2))
-        with pytest.raises(rgapi.RGError) as e:
-            srv.wait(t, timeout=60.0)
-        assert e.value.code == -5
+        tickets = [srv.submit(big) for _ in range(3)]
+        time.sleep(0.2)
+        c2.cancel()
+        for t in tickets:
+            with pytest.raises(rgapi.RGError) as e:
+                srv.wait(t, timeout=60.0)
+            assert e.value.code == -5
+        t = srv.submit(small)
+        info, out = srv.wait(t, timeout=60.0)
+        assert info.status == 0 and np.isfinite(out["hdr"]).all()
     finally:
         srv.close()
