@@ -167,10 +167,19 @@ int main(int argc, char **argv) {
                 st.height, st.spp, shader, st.antialias ? "yes" : "no", st.denoise ? "yes" : "no",
                 st.adaptive ? "yes" : "no");
     std::vector<float> hdr((size_t)st.width * st.height * 4);
+    // progress line as main.cpp:492-506 prints it
+    rg_set_progress(
+        ctx,
+        [](float p, void *) {
+            std::printf("\rrendering... %d%%   ", (int)(p * 100.0f));
+            std::fflush(stdout);
+        },
+        nullptr);
     if (rg_render(ctx, sc, &st, hdr.data(), nullptr, nullptr, nullptr) != RG_OK) {
-        std::fprintf(stderr, "render: %s\n", rg_last_error());
+        std::fprintf(stderr, "\nrender: %s\n", rg_last_error());
         return 1;
     }
+    std::printf("\n");
     // render()'s output stage: bloom (when enabled) and the tonemap
     const bool is_hdr = out.size() >= 4 && (out.compare(out.size() - 4, 4, ".hdr") == 0);
     std::vector<float> ldr(is_hdr ? 0 : hdr.size());

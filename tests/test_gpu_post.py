@@ -128,6 +128,7 @@ def test_cli_writes_the_postprocessed_image(ctx, tmp_path):
     r = subprocess.run([exe, "render", scene, "-r", "64,48", "-s", "4", "-enpp", "true", "-blth", "0.7",
                         "-o", str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    assert "rendering... 100%" in r.stdout   # the progress line, as main.cpp prints it
     raw = out.read_bytes()
     head = b"P6\n64 48\n255\n"
     assert raw.startswith(head)
