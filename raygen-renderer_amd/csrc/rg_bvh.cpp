@@ -281,6 +281,37 @@ struct Collapse {
     }
 };
 
+// Moves the first `top` nodes in breadth-first order to the front (root
+// first), keeping the others in their depth-first order, and renumbers the
+// child refs.  Siblings stay contiguous either way.
+template <class NodeT, int W>
+void hoist_top(std::vector<NodeT> &nodes, uint32_t top) {
+    const uint32_t n = (uint32_t)nodes.size();
+    if (n <= 1) return;
+    top = std::min(top, n);
+    std::vector<uint32_t> bfs;
+    bfs.reserve(top);
+    bfs.push_back(0);
+    for (size_t h = 0; h < bfs.size() && bfs.size() < top; h++) {
+        const NodeT &nd = nodes[bfs[h]];
+        for (int s = 0; s < W && bfs.size() < top; s++)
+            if (nd.child[s] != kEmpty && !(nd.child[s] & kLeafBit)) bfs.push_back(nd.child[s]);
+    }
+    std::vector<uint32_t> to(n, ~0u);
+    uint32_t next = 0;
+    for (uint32_t k : bfs) to[k] = next++;
+    for (uint32_t k = 0; k < n; k++)
+        if (to[k] == ~0u) to[k] = next++;
+    std::vector<NodeT> out(n);
+    for (uint32_t k = 0; k < n; k++) {
+        NodeT nd = nodes[k];
+        for (int s = 0; s < W; s++)
+            if (nd.child[s] != kEmpty && !(nd.child[s] & kLeafBit)) nd.child[s] = to[nd.child[s]];
+        out[to[k]] = nd;
+    }
+    nodes.swap(out);
+}
+
 template <class NodeT, int W>
 void collapse_wide(const std::vector<Node2> &nodes2, std::vector<NodeT> &out) {
     out.clear();
@@ -303,6 +334,7 @@ void collapse_wide(const std::vector<Node2> &nodes2, std::vector<NodeT> &out) {
     }
     Collapse<NodeT, W> c{nodes2, out};
     c.fill(0, 0);
+    hoist_top<NodeT, W>(out, kTreeletBytes / (uint32_t)sizeof(NodeT));
 }
 
 // Deepest sum of (children - 1) pushed along any root-to-node path, plus one.

@@ -347,14 +347,25 @@ constexpr uint32_t kNoNode = 0xFFFFFFFEu;
 // face rounds below the slab's entry distance (a hit on an edge shared by
 // triangles in two leaves).
 //
-// Src provides: bool load(i, bv, PsRay &) (false: nothing to trace at i; it
-// may replace bv with the ray's own tree), store_closest(i, prim, t, u, v),
-// store_any(i, occluded).  kLaneBvh: the tree differs per ray.
-template <int kStack, int kW, bool kAny, int kStride, class Src, bool kLaneBvh = false>
-__device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src, uint64_t n,
+// The tree's top levels (the first bv.top nodes, breadth-first) are copied
+// into LDS when the block starts: every ray passes through them, and LDS
+// answers in a fraction of an L2 hit's time.
+//
+// Src provides: bool load(i, bv, PsRay &) (false: nothing to trace at i),
+// store_closest(i, prim, t, u, v), store_any(i, occluded).  Every thread of
+// the block must call it (it synchronises the block once).
+template <int kStack, int kW, bool kAny, int kStride, class Src>
+__device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, uint64_t n,
                                               const LdsColumn<kStride> lc) {
     constexpr int kSpill = kStack > kLdsStack ? kStack - kLdsStack : 1;
     constexpr int kOrdBits = WideTest<kW>::kOrdBits;
+    __shared__ uint4 treelet[kTreeletBytes / 16];
+    {
+        const uint32_t words = bv.top * (uint32_t)(kW == 8 ? sizeof(Node8) : sizeof(Node4)) / 16;
+        const uint4 *src_words = reinterpret_cast<const uint4 *>(bv.nodes);
+        for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) treelet[w] = src_words[w];
+        __syncthreads();
+    }
     uint32_t stk_ref[kSpill];
     float stk_t[kSpill];
     const uint32_t lane = __lane_id();
@@ -363,7 +374,6 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src,
     const uint64_t chunk = (n + waves - 1) / waves;
     uint64_t cursor = wave * chunk;
     const uint64_t chunk_end = cursor + chunk < n ? cursor + chunk : n;
-    DevBvh bv = bv0;
     bool active = false;
     uint64_t ri = 0;
     PsRay r{};
@@ -373,15 +383,29 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src,
     float cur_t = 0.0f;  // entry distance of cur (closest hit only)
     bool hit_any = false;
 
+    // The LDS slot is read unconditionally (clamped to a valid one) and the
+    // private one only past the LDS part.  Left alone, the compiler turns
+    // the pair into one flat load from a selected address, and a flat load
+    // waits for every load in flight, node fetches included; the empty asm
+    // statements (no instruction, a register operand only) keep the LDS
+    // reads as they are.
     auto pop = [&]() -> uint32_t {
         while (sp > 0) {
             sp--;
             const bool in_lds = sp < kLdsStack;
-            if (kAny) return in_lds ? lc.ref[sp * kStride] : stk_ref[sp - kLdsStack];
-            const float et = in_lds ? lc.t[sp * kStride] : stk_t[sp - kLdsStack];
+            const int li = in_lds ? sp : kLdsStack - 1;
+            uint32_t ref = lc.ref[li * kStride];
+            float et = kAny ? 0.0f : lc.t[li * kStride];
+            asm volatile("" : "+v"(ref));
+            if (!kAny) asm volatile("" : "+v"(et));
+            if (!in_lds) {
+                ref = stk_ref[sp - kLdsStack];
+                if (!kAny) et = stk_t[sp - kLdsStack];
+            }
+            if (kAny) return ref;
             if (!(et > t)) {
                 cur_t = et;
-                return in_lds ? lc.ref[sp * kStride] : stk_ref[sp - kLdsStack];
+                return ref;
             }
         }
         return kNoNode;
@@ -399,7 +423,6 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src,
             const uint64_t idx = cursor + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
             if (!active && idx < chunk_end) {
                 ri = idx;
-                if (kLaneBvh) bv = bv0;
                 if (src.load(idx, bv, r)) {
                     t = r.tmax;
                     ix = inv_dir(r.dx); iy = inv_dir(r.dy); iz = inv_dir(r.dz);
@@ -436,7 +459,10 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv0, const Src &src,
                 bool ok[kW];
                 float tm[kW];
                 uint32_t cref[kW], ord;
-                WideTest<kW>::run(bv.nodes, cur, oct, r.ox, r.oy, r.oz, ix, iy, iz, t, ok, tm, cref, ord);
+                // the top levels from LDS, the rest from HBM (a flat load
+                // reaches either)
+                const void *base = cur < bv.top ? (const void *)treelet : bv.nodes;
+                WideTest<kW>::run(base, cur, oct, r.ox, r.oy, r.oz, ix, iy, iz, t, ok, tm, cref, ord);
                 bool have = false;
                 uint32_t next = 0;
                 float next_t = 0.0f;

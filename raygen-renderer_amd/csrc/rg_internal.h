@@ -184,6 +184,10 @@ int set_error(int code, const std::string &msg);
 // rg_bvh.cpp
 void build_bvh2(const float *v9, uint32_t n, uint32_t bins, uint32_t leaf,
                 std::vector<Node2> &nodes, std::vector<uint32_t> &perm);
+// The first kTreeletBytes of a wide tree's nodes are its top levels in
+// breadth-first order (21 4-wide or 10 8-wide nodes); the traversal kernels
+// keep them in LDS.
+constexpr uint32_t kTreeletBytes = 2688;
 void collapse_bvh4(const std::vector<Node2> &nodes2, std::vector<Node4> &out);
 void collapse_bvh8(const std::vector<Node2> &nodes2, std::vector<Node8> &out);
 void make_tris(const float *v9, const std::vector<Node2> &nodes2,

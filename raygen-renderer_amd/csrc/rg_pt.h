@@ -21,6 +21,7 @@ struct DevBvh {
     RootBox root;
     int has_tree;
     int width;          // 4 or 8
+    uint32_t top;       // leading nodes (the top levels) the persistent traversal copies to LDS
 };
 
 // One SceneObject's material (material.h:20-70); same layout as rg_object.

@@ -145,6 +145,9 @@ DevBvh dev_bvh(const Scene &s) {
         for (int k = 0; k < 3; k++) { b.root.lo[k] = 0.0f; b.root.hi[k] = 0.0f; }
     }
     b.has_tree = (s.nodes4.empty() && s.nodes8.empty()) ? 0 : 1;
+    const size_t n_nodes = s.width == 8 ? s.nodes8.size() : s.nodes4.size();
+    const size_t node_bytes = s.width == 8 ? sizeof(Node8) : sizeof(Node4);
+    b.top = (uint32_t)std::min<size_t>(n_nodes, kTreeletBytes / node_bytes);
     return b;
 }
 
