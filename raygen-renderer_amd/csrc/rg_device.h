@@ -496,9 +496,14 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, 
         if (active && leaf != kNoNode) {
             uint32_t k = leaf & ~kLeafBit;
             for (;;) {
-                const float4 a = bv.tris[3 * (size_t)k];
+                float4 a = bv.tris[3 * (size_t)k];
                 const float4 b = bv.tris[3 * (size_t)k + 1];
-                const float4 c = bv.tris[3 * (size_t)k + 2];
+                float4 c = bv.tris[3 * (size_t)k + 2];
+                // all 48 bytes as three aligned 16-byte loads: left to
+                // itself the compiler reads only the words each variant
+                // uses, as 12 + 16 + 16 bytes that can straddle cache lines
+                asm volatile("" : "+v"(a.w));
+                asm volatile("" : "+v"(c.w));
                 if (kAny) {
                     if ((__float_as_uint(c.w) & r.pred_bit) &&
                         mt_any(a, b, c, r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, t)) {
