@@ -278,6 +278,36 @@ def bench_batch(ctx, resident, n_jobs=64, seed=0xC3C30000):
             "tune": tune.as_dict(), "autotune_s": round(tune_s, 2)}
 
 
+def bench_batch_small(ctx, resident, n_jobs=256, seed=0xC5C50000):
+    """Many small jobs (preview renders: 128x128, 4 spp, depth 5, alternating
+    the two scenes): batched against one at a time.  Each job alone keeps
+    only 65,536 paths in flight, a fraction of what the GPU holds, so this is
+    where packing jobs into shared generations pays; not one of the
+    configured workloads."""
+    import numpy as np
+
+    import rgapi
+
+    spec = [(i % len(resident), dict(width=128, height=128, spp=4, max_depth=5, seed=(seed + i) & 0xFFFFFFFF, job=i))
+            for i in range(n_jobs)]
+    jobs = [rgapi.make_job(s, **kw) for s, kw in spec]
+    rgapi.render_batch(ctx, resident, jobs[:8], want_sum=False)  # warm
+    t0 = time.perf_counter()
+    outs, _ = rgapi.render_batch(ctx, resident, jobs, want_sum=False)
+    batch_s = time.perf_counter() - t0
+    single_s, same = 0.0, 0
+    for i, (s, kw) in enumerate(spec):
+        t0 = time.perf_counter()
+        o = resident[s].render(rgapi.default_settings(**kw), hdr_only=True)
+        single_s += time.perf_counter() - t0
+        same += int(np.array_equal(o["hdr"], outs[i]["hdr"]))
+    samples = n_jobs * 128 * 128 * 4
+    return {"workload": f"{n_jobs} preview jobs 128x128, 4 spp, depth 5, cube room / BSDF frame alternating",
+            "batched_s": round(batch_s, 4), "one_at_a_time_s": round(single_s, 4),
+            "speedup": round(single_s / batch_s, 3), "batched_samples_per_s": round(samples / batch_s, 1),
+            "images_bit_equal": f"{same}/{n_jobs}"}
+
+
 def bench_serve(ctx, resident, n_req, rates, seed=0xC4C40000, budget=32 << 30, n_plain=24):
     """Poisson request stream over four resident scenes: served by rg_server
     (continuous batching, admission by framebuffer footprint) at several
@@ -478,6 +508,8 @@ def main():
         if "batch" in legs:
             log("batch")
             extra["batch"] = bench_batch(ctx, [room, frame], n_jobs=args.batch_jobs)
+            log("batch of small jobs")
+            extra["batch_small"] = bench_batch_small(ctx, [room, frame])
         if "serve" in legs:
             frame2 = rgapi.RenderScene(ctx, scenes.bsdf_frame(grid=5, stacks=200, slices=200, seed=0xB5DF0003))
             ico = rgapi.RenderScene(ctx, scenes.ico_ggx())

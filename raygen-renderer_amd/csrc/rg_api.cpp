@@ -43,6 +43,7 @@ namespace rg {
 int dev_stream_create(void **s);
 void dev_stream_destroy(void *s);
 void release_pt(Ctx *c);
+void release_engine(Ctx *c);
 void release_pt_scene(Scene *s);
 void release_post(Ctx *c);
 }  // namespace rg
@@ -75,6 +76,7 @@ void rg_destroy(rg_ctx *ctx) {
         dev_set_device(c->device);
         release_pt(c);
         release_post(c);
+        release_engine(c);
     }
     if (c->own_stream) dev_stream_destroy(c->own_stream);
     if (c->device >= 0) dev_clear_error();

@@ -105,14 +105,16 @@ struct Request {
     std::atomic<int> state{0};  // 0 pending, 1 admitted, 2 all samples issued, 3 done
     int status = RG_OK;
     double t_submit = 0.0, t_admit = 0.0, t_done = 0.0;
-    int pin = -1;
+    int pin = -1;           // pinned staging slab of the results (-1: none)
+    uint64_t pin_off = 0, pin_bytes = 0;
     float *d_hdr = nullptr, *d_sum = nullptr, *d_sumsq = nullptr;
 };
 
-struct PinnedBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    bool busy = false;
+// Pinned host memory for result copies, carved out of a few large slabs:
+// one hipHostMalloc per job cost more than a small job's whole render.
+struct PinnedSlab {
+    char *p = nullptr;
+    Arena arena;
 };
 
 // Footprint of one job in the arena: sums, sums of squares, HDR image.
@@ -143,6 +145,8 @@ struct Lane {
     BatchArgs args{};
     void *d_chunks = nullptr;
     GenChunk *h_chunks = nullptr;          // pinned chunk-table staging
+    AccGroup *h_groups = nullptr;          // pinned accumulation-group staging
+    AccGroup *d_groups = nullptr;
     unsigned long long *h_stats = nullptr;  // pinned counter snapshot
     hipEvent_t stage_ev = nullptr;          // staging of the last generation consumed
     hipEvent_t acc_ev = nullptr;            // sums and results of the last generation queued
@@ -157,14 +161,23 @@ class Engine {
 
     // paths_hint (0: unknown): the most path slots a generation can use, the
     // total samples of a known job list.
-    int init(Ctx *c, rg_scene *const *sc, uint32_t n, uint64_t budget, const rg_tune *t, uint64_t paths_hint = 0) {
+    // pinned_hint (0: unknown): bytes of results the jobs will copy out.
+    // lanes_hint (0: unknown): the most generations a known job list needs.
+    int init(Ctx *c, rg_scene *const *sc, uint32_t n, uint64_t budget, const rg_tune *t, uint64_t paths_hint = 0,
+             uint64_t pinned_hint = 0, uint32_t lanes_hint = 0) {
         ctx = c;
+        if (pinned_hint) next_slab = std::min<uint64_t>(std::max<uint64_t>(pinned_hint, 1u << 20), 2ull << 30);
         if (!c || (n && !sc)) return set_error(RG_E_INVALID, "null argument");
         if (c->device < 0) return set_error(RG_E_NODEVICE, "host-only context has no device");
         if (n == 0) return set_error(RG_E_INVALID, "no scenes");
         if (int rc = dev_set_device(c->device)) return rc;
-        for (Lane &ln : lanes)
-            if (int rc = hip_ok(hipStreamCreate(&ln.stream), "stream create")) return rc;
+        // the lanes' streams belong to the context: creating a stream costs
+        // milliseconds, more than a small batch's render
+        for (int l = 0; l < kMaxLanes; l++) {
+            if (!c->lane_streams[l])
+                if (int rc = hip_ok(hipStreamCreate((hipStream_t *)&c->lane_streams[l]), "stream create")) return rc;
+            lanes[l].stream = (hipStream_t)c->lane_streams[l];
+        }
         hipStream_t stream = lanes[0].stream;
         uint32_t need = 0;
         int width = 0;
@@ -199,6 +212,7 @@ class Engine {
         if (tune.budget < 1024 || tune.budget > (1u << 26)) return set_error(RG_E_INVALID, "pool_paths out of range");
         if (tune.blocks_per_cu > 32) return set_error(RG_E_INVALID, "blocks_per_cu out of range");
         if (n_lanes < 1 || n_lanes > kMaxLanes) return set_error(RG_E_INVALID, "lanes must be 1 to 4");
+        if (lanes_hint) n_lanes = std::min<int>(n_lanes, (int)lanes_hint);
         n_cu = stream_cu_count(c->device);
         if (n_cu <= 0) return set_error(RG_E_HIP, "cannot query compute units");
 
@@ -207,46 +221,72 @@ class Engine {
         cap = std::max(tune.budget, kPathsInFlight);
         if (paths_hint) cap = (uint32_t)std::min<uint64_t>(cap, std::max<uint64_t>(paths_hint, 1024));
         const uint64_t P = cap;
+        // one device block for every lane's path state, chunk table and the
+        // job records, and one pinned block for their host staging (each
+        // allocation call costs about as much as a small job's render)
+        static const uint64_t kLaneArrays[] = {32, 16, 16, 16, 16, 16, 64, 8, 2, 32, 4, 4, 4, 16, 4, 4};
+        uint64_t lane_bytes = 2 * align_up(64) + align_up((uint64_t)kMaxChunks * sizeof(GenChunk)) +
+                              align_up((uint64_t)kMaxChunks * sizeof(AccGroup));
+        for (uint64_t b : kLaneArrays) lane_bytes += align_up(P * b);
+        const uint64_t dev_bytes = n_lanes * lane_bytes + align_up((uint64_t)kMaxJobs * kDevJobBytes);
+        char *blk = nullptr;
+        if (int rc = dev_malloc((void **)&blk, dev_bytes)) return rc;
+        dev_allocs.push_back(blk);
+        auto carve = [&](uint64_t bytes) {
+            char *q = blk;
+            blk += align_up(bytes);
+            return q;
+        };
+        const uint64_t host_lane = align_up((uint64_t)kMaxChunks * sizeof(GenChunk)) +
+                                   align_up((uint64_t)kMaxChunks * sizeof(AccGroup)) + align_up(64);
+        char *hblk = nullptr;
+        if (int rc = hip_ok(hipHostMalloc((void **)&hblk, align_up((uint64_t)kMaxJobs * kDevJobBytes) +
+                                                             n_lanes * host_lane),
+                            "pinned alloc"))
+            return rc;
+        h_jobs = hblk;
+        hblk += align_up((uint64_t)kMaxJobs * kDevJobBytes);
         for (int l = 0; l < n_lanes; l++) {
             Lane &ln = lanes[l];
             PTState &st = ln.args.st;
             std::memset(&st, 0, sizeof st);
-            struct A {
-                void **p;
-                uint64_t bytes;
-            } allocs[] = {
-                {(void **)&st.ray, P * 32},      {(void **)&st.hit, P * 16},      {(void **)&st.beta, P * 16},
-                {(void **)&st.tp, P * 16},       {(void **)&st.L, P * 16},        {(void **)&st.meta, P * 16},
-                {(void **)&st.sray, P * 64},     {(void **)&st.ssrc, P * 8},      {(void **)&st.socc, P * 2},
-                {(void **)&st.scontrib, P * 32}, {(void **)&st.nshadow, P * 4},   {(void **)&st.q[0], P * 4},
-                {(void **)&st.q[1], P * 4},      {(void **)&st.count, 64},        {(void **)&st.sample_out, P * 16},
-                {(void **)&st.stats, 64},        {(void **)&ln.args.chunk_of, P * 4},
-                {(void **)&ln.args.mixed, P * 4},
-                {&ln.d_chunks, (uint64_t)kMaxChunks * sizeof(GenChunk)},
-            };
-            for (auto &a : allocs) {
-                if (int rc = dev_malloc(a.p, a.bytes)) return rc;
-                dev_allocs.push_back(*a.p);
-            }
+            st.ray = (rg_ray *)carve(P * 32);
+            st.hit = (rg_hit *)carve(P * 16);
+            st.beta = (float4 *)carve(P * 16);
+            st.tp = (float4 *)carve(P * 16);
+            st.L = (float4 *)carve(P * 16);
+            st.meta = (int4 *)carve(P * 16);
+            st.sray = (rg_ray *)carve(P * 64);
+            st.ssrc = (uint32_t *)carve(P * 8);
+            st.socc = (uint8_t *)carve(P * 2);
+            st.scontrib = (float4 *)carve(P * 32);
+            st.nshadow = (uint32_t *)carve(P * 4);
+            st.q[0] = (uint32_t *)carve(P * 4);
+            st.q[1] = (uint32_t *)carve(P * 4);
+            st.sample_out = (float4 *)carve(P * 16);
+            ln.args.chunk_of = (uint32_t *)carve(P * 4);
+            ln.args.mixed = (uint32_t *)carve(P * 4);
+            st.count = (uint32_t *)carve(64);
+            st.stats = (unsigned long long *)carve(64);
+            ln.d_chunks = carve((uint64_t)kMaxChunks * sizeof(GenChunk));
+            ln.d_groups = (AccGroup *)carve((uint64_t)kMaxChunks * sizeof(AccGroup));
             ln.args.chunks = (const GenChunk *)ln.d_chunks;
             ln.args.cap = cap;
             if (int rc = launch_fill_u32(st.stats, 0u, 16, stream)) return rc;
-            if (int rc = hip_ok(hipHostMalloc((void **)&ln.h_chunks, (size_t)kMaxChunks * sizeof(GenChunk)),
-                                "pinned alloc"))
-                return rc;
-            if (int rc = hip_ok(hipHostMalloc((void **)&ln.h_stats, 64), "pinned alloc")) return rc;
+            ln.h_chunks = (GenChunk *)hblk;
+            ln.h_groups = (AccGroup *)(hblk + align_up((uint64_t)kMaxChunks * sizeof(GenChunk)));
+            ln.h_stats = (unsigned long long *)((char *)ln.h_groups + align_up((uint64_t)kMaxChunks * sizeof(AccGroup)));
+            hblk += host_lane;
             for (hipEvent_t *e : {&ln.stage_ev, &ln.acc_ev}) {
                 if (int rc = hip_ok(hipEventCreateWithFlags(e, hipEventDisableTiming), "event")) return rc;
                 if (int rc = hip_ok(hipEventRecord(*e, stream), "event record")) return rc;
             }
         }
-        if (int rc = dev_malloc(&d_jobs, (uint64_t)kMaxJobs * kDevJobBytes)) return rc;
-        dev_allocs.push_back(d_jobs);
+        d_jobs = carve((uint64_t)kMaxJobs * kDevJobBytes);
         if (budget == 0) budget = 16ull << 30;
         if (int rc = dev_malloc((void **)&d_arena, budget)) return rc;
         dev_allocs.push_back(d_arena);
         arena.reset(budget);
-        if (int rc = hip_ok(hipHostMalloc((void **)&h_jobs, (size_t)kMaxJobs * kDevJobBytes), "pinned alloc")) return rc;
         if (int rc = hip_ok(hipEventCreateWithFlags(&adm_ev, hipEventDisableTiming), "event")) return rc;
         for (uint32_t s = kMaxJobs; s-- > 0;) free_slots.push_back(s);
         return hip_ok(hipStreamSynchronize(stream), "engine init");
@@ -285,18 +325,16 @@ class Engine {
         dev_allocs.clear();
         if (h_jobs) (void)hipHostFree(h_jobs);
         h_jobs = nullptr;
-        for (PinnedBuf &b : pins)
+        for (PinnedSlab &b : slabs)
             if (b.p) (void)hipHostFree(b.p);
-        pins.clear();
+        slabs.clear();
         if (adm_ev) (void)hipEventDestroy(adm_ev);
         adm_ev = nullptr;
         for (Lane &ln : lanes) {
-            if (ln.h_chunks) (void)hipHostFree(ln.h_chunks);
-            if (ln.h_stats) (void)hipHostFree(ln.h_stats);
+            // h_chunks / h_stats are parts of the h_jobs block
             for (hipEvent_t e : {ln.stage_ev, ln.acc_ev})
                 if (e) (void)hipEventDestroy(e);
-            if (ln.stream) (void)hipStreamDestroy(ln.stream);
-            ln = Lane{};
+            ln = Lane{};  // the stream stays with the context
         }
     }
 
@@ -461,8 +499,8 @@ class Engine {
                                                hipMemcpyHostToDevice, stream),
                                 "job upload"))
                 return rc;
-            if (int rc = launch_fill_u32(d.d_sum, 0u, 3 * np, stream)) return rc;
-            if (int rc = launch_fill_u32(d.d_sumsq, 0u, 3 * np, stream)) return rc;
+            // sums and sums of squares are adjacent in the arena: one fill
+            if (int rc = launch_fill_u32(d.d_sum, 0u, (align_up(np * 12) + np * 12) / 4, stream)) return rc;
             r->d_hdr = d.d_hdr;
             r->d_sum = d.d_sum;
             r->d_sumsq = d.d_sumsq;
@@ -535,10 +573,28 @@ class Engine {
         // the lane's pinned chunk table and counter snapshot are free again
         if (int rc = hip_ok(hipEventSynchronize(ln.stage_ev), "chunk staging")) return rc;
         std::memcpy(ln.h_chunks, ch.data(), ch.size() * sizeof(GenChunk));
+        // one accumulation group per job (its chunks are consecutive)
+        uint32_t ngroups = 0, total_pix = 0;
+        for (size_t k = 0; k < ch.size(); k++) {
+            if (k == 0 || chunk_jobs[k] != chunk_jobs[k - 1]) {
+                AccGroup &g = ln.h_groups[ngroups++];
+                std::memset(&g, 0, sizeof g);
+                g.c0 = (uint32_t)k;
+                g.pix_first = total_pix;
+                g.npix = ch[k].npix;
+                g.finish = chunk_jobs[k]->state == 2 ? 1u : 0u;
+                total_pix += ch[k].npix;
+            }
+            ln.h_groups[ngroups - 1].c1 = (uint32_t)k + 1;
+        }
         if (int rc = wait_admissions(L)) return rc;
         if (int rc = hip_ok(hipMemcpyAsync(ln.d_chunks, ln.h_chunks, ch.size() * sizeof(GenChunk),
                                            hipMemcpyHostToDevice, stream),
                             "chunk upload"))
+            return rc;
+        if (int rc = hip_ok(hipMemcpyAsync(ln.d_groups, ln.h_groups, ngroups * sizeof(AccGroup),
+                                           hipMemcpyHostToDevice, stream),
+                            "group upload"))
             return rc;
         BatchArgs a = ln.args;
         a.nchunks = (uint32_t)ch.size();
@@ -547,9 +603,7 @@ class Engine {
         // a job's chunks are summed in generation order across the lanes
         if (last_lane >= 0 && last_lane != L)
             if (int rc = hip_ok(hipStreamWaitEvent(stream, lanes[last_lane].acc_ev, 0), "stream wait")) return rc;
-        for (size_t k = 0; k < ch.size(); k++)
-            if (int rc = stream_launch_accumulate(a, ch[k], chunk_jobs[k]->d_sum, chunk_jobs[k]->d_sumsq, stream))
-                return rc;
+        if (int rc = stream_launch_accumulate(a, d_jobs, ln.d_groups, ngroups, total_pix, stream)) return rc;
         chunk_jobs.clear();
         for (Request *r : last)
             if (int rc = finish(r, stream)) return rc;
@@ -608,15 +662,15 @@ class Engine {
         return hip_ok(hipGraphLaunch(it->second, stream), "graph launch");
     }
 
-    // Queues a job's HDR resolve and the copies of its results.
+    // Queues the copies of a finished job's results (its HDR image is
+    // written by the generation's accumulation).
     int finish(Request *r, hipStream_t stream) {
-        if (int rc = stream_launch_resolve(d_jobs, r->slot, r->npix, stream)) return rc;
         const size_t hb = (size_t)r->npix * 16, sb = (size_t)r->npix * 12;
         const size_t need = (r->out_hdr ? hb : 0) + (r->out_sum ? sb : 0) + (r->out_sumsq ? sb : 0);
         if (!need) return RG_OK;
-        r->pin = get_pinned(need);
-        if (r->pin < 0) return set_error(RG_E_NOMEM, "pinned staging allocation failed");
-        char *p = (char *)pins[r->pin].p;
+        if (!get_pinned(need, r->pin, r->pin_off)) return set_error(RG_E_NOMEM, "pinned staging allocation failed");
+        r->pin_bytes = need;
+        char *p = slabs[r->pin].p + r->pin_off;
         if (r->out_hdr) {
             if (int rc = hip_ok(hipMemcpyAsync(p, r->d_hdr, hb, hipMemcpyDeviceToHost, stream), "result copy"))
                 return rc;
@@ -633,19 +687,22 @@ class Engine {
         return RG_OK;
     }
 
-    int get_pinned(size_t need) {
-        int best = -1;
-        for (size_t i = 0; i < pins.size(); i++)
-            if (!pins[i].busy && pins[i].cap >= need && (best < 0 || pins[i].cap < pins[best].cap)) best = (int)i;
-        if (best < 0) {
-            PinnedBuf b;
-            b.cap = (need + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
-            if (hipHostMalloc(&b.p, b.cap) != hipSuccess) return -1;
-            pins.push_back(b);
-            best = (int)pins.size() - 1;
-        }
-        pins[best].busy = true;
-        return best;
+    // First fit in the existing slabs, else a new slab (the first sized by
+    // the hint, then doubling up to 1 GiB, at least `need`).
+    bool get_pinned(size_t need, int &slab, uint64_t &off) {
+        for (size_t i = 0; i < slabs.size(); i++)
+            if (slabs[i].arena.alloc(need, off)) {
+                slab = (int)i;
+                return true;
+            }
+        const uint64_t size = std::max<uint64_t>(align_up(need), next_slab);
+        next_slab = std::min<uint64_t>(2 * next_slab, 1ull << 30);
+        PinnedSlab b;
+        if (hipHostMalloc((void **)&b.p, size) != hipSuccess) return false;
+        b.arena.reset(size);
+        slabs.push_back(std::move(b));
+        slab = (int)slabs.size() - 1;
+        return slabs[slab].arena.alloc(need, off);
     }
 
     // Retires finished generations in launch order and hands their finished
@@ -675,7 +732,7 @@ class Engine {
 
     void deliver(Request *r) {
         if (r->pin >= 0) {
-            const char *p = (const char *)pins[r->pin].p;
+            const char *p = slabs[r->pin].p + r->pin_off;
             const size_t hb = (size_t)r->npix * 16, sb = (size_t)r->npix * 12;
             if (r->out_hdr) {
                 std::memcpy(r->out_hdr, p, hb);
@@ -686,7 +743,7 @@ class Engine {
                 p += sb;
             }
             if (r->out_sumsq) std::memcpy(r->out_sumsq, p, sb);
-            pins[r->pin].busy = false;
+            slabs[r->pin].arena.release(r->pin_off, r->pin_bytes);
             r->pin = -1;
         }
         arena.release(r->off, r->bytes);
@@ -726,7 +783,7 @@ class Engine {
                 r->state = 3;
                 r->status = rc;
                 r->t_done = now_s();
-                if (r->pin >= 0) pins[r->pin].busy = false;
+                if (r->pin >= 0) slabs[r->pin].arena.release(r->pin_off, r->pin_bytes);
                 r->pin = -1;
             }
             inbox.clear();
@@ -768,7 +825,8 @@ class Engine {
     std::deque<Inflight> inflight;
     std::vector<hipEvent_t> spare_events;
     std::vector<Request *> chunk_jobs;  // job of each chunk of the generation being built
-    std::vector<PinnedBuf> pins;
+    std::vector<PinnedSlab> slabs;
+    uint64_t next_slab = 64ull << 20;
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -784,6 +842,18 @@ class Engine {
     uint64_t n_done = 0, samples_done = 0;
     std::atomic<uint64_t> n_generations{0}, n_replays{0}, fb_used{0};
 };
+
+}  // namespace rg
+
+namespace rg {
+
+// Destroys the lane streams a context's engines used (rg_destroy).
+void release_engine(Ctx *c) {
+    for (void *&s : c->lane_streams) {
+        if (s) (void)hipStreamDestroy((hipStream_t)s);
+        s = nullptr;
+    }
+}
 
 }  // namespace rg
 
@@ -810,11 +880,21 @@ int batch_on(Engine &e, const rg_job *jobs, uint32_t n, float *const *hdr_out, f
     return RG_OK;
 }
 
-// Samples of a job list: no generation holds more paths.
-uint64_t batch_paths(const rg_job *jobs, uint32_t n) {
-    uint64_t t = 0;
-    for (uint32_t i = 0; i < n; i++) t += (uint64_t)jobs[i].settings.width * jobs[i].settings.height * jobs[i].settings.spp;
-    return std::max<uint64_t>(t, 1);
+// A job list's samples per scene: a generation (chunks of one scene) holds
+// at most the largest of them, and the list needs at most the sum over the
+// scenes of ceil(samples / generation budget) generations.
+void batch_shape(const rg_job *jobs, uint32_t n, const rg_tune *tune, uint64_t &paths, uint32_t &gens) {
+    std::map<uint32_t, uint64_t> per;
+    for (uint32_t i = 0; i < n; i++)
+        per[jobs[i].scene] += (uint64_t)jobs[i].settings.width * jobs[i].settings.height * jobs[i].settings.spp;
+    const uint64_t budget = (tune && tune->pool_paths) ? tune->pool_paths : kPathsInFlight;
+    paths = 1;
+    uint64_t g = 0;
+    for (auto &kv : per) {
+        paths = std::max(paths, kv.second);
+        g += (kv.second + budget - 1) / budget;
+    }
+    gens = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 1u << 30));
 }
 
 uint64_t batch_budget(const rg_job *jobs, uint32_t n) {
@@ -839,7 +919,15 @@ int rg_render_batch(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, con
     if (!c || (n_jobs && !jobs)) return set_error(RG_E_INVALID, "null argument");
     c->cancel.store(false);  // as rg_render: a batch clears the flag when it starts
     Engine e;
-    if (int rc = e.init(c, scenes, n_scenes, batch_budget(jobs, n_jobs), tune, batch_paths(jobs, n_jobs))) return rc;
+    uint64_t out_bytes = 0;
+    for (uint32_t i = 0; i < n_jobs; i++) {
+        const uint64_t np = (uint64_t)jobs[i].settings.width * jobs[i].settings.height;
+        out_bytes += align_up((hdr_out && hdr_out[i] ? np * 16 : 0) + (sum_out && sum_out[i] ? np * 12 : 0));
+    }
+    uint64_t paths = 0;
+    uint32_t gens = 0;
+    batch_shape(jobs, n_jobs, tune, paths, gens);
+    if (int rc = e.init(c, scenes, n_scenes, batch_budget(jobs, n_jobs), tune, paths, out_bytes, gens)) return rc;
     if (int rc = batch_on(e, jobs, n_jobs, hdr_out, sum_out)) return rc;
     if (stats)
         for (int k = 0; k < 4; k++) stats[k] = e.dev_stats[k];
@@ -923,8 +1011,10 @@ int rg_autotune(rg_ctx *ctx, rg_scene *const *scenes, uint32_t n_scenes, const r
     // the second timed (graph, pinned buffers and caches warm)
     auto measure = [&](const rg_tune &t, double &dt) -> int {
         Engine e;
-        if (int rc = e.init(c, scenes, n_scenes, budget, &t, batch_paths(jobs.data(), (uint32_t)jobs.size())))
-            return rc;
+        uint64_t paths = 0;
+        uint32_t gens = 0;
+        batch_shape(jobs.data(), (uint32_t)jobs.size(), &t, paths, gens);
+        if (int rc = e.init(c, scenes, n_scenes, budget, &t, paths, 0, gens)) return rc;
         for (int rep = 0; rep < 2; rep++) {
             const double t0 = now_s();
             if (int rc = batch_on(e, jobs.data(), (uint32_t)jobs.size(), nullptr, nullptr)) return rc;

@@ -114,6 +114,7 @@ struct Ctx {
     rg_progress_fn progress = nullptr;  // rg_render's progress callback
     void *progress_user = nullptr;
     void *progress_ev[2] = {nullptr, nullptr};  // hipEvent_t after the last two generations
+    void *lane_streams[4] = {nullptr, nullptr, nullptr, nullptr};  // the batched engines' lanes, kept
     PTBuffers pt;
     PostBuffers post;
     uint64_t last_stats[3] = {0, 0, 0};

@@ -46,6 +46,17 @@ struct GenChunk {
     float clamp;           // the job's firefly clamp (0: none)
 };
 
+// One job's chunks in a generation, [c0, c1) of the chunk table (a job's
+// chunks are consecutive), and its pixels' place in the generation's
+// accumulation grid.  finish: the job's last samples are in this
+// generation, so its HDR image is written too.
+struct AccGroup {
+    uint32_t c0, c1;
+    uint32_t pix_first, npix;
+    uint32_t finish;
+    uint32_t pad[3];
+};
+
 // Device pointers of a generation, passed to every kernel by value: the
 // single-job path's path state, the chunk of each path slot, the chunk table.
 struct BatchArgs {
@@ -78,11 +89,12 @@ int stream_launch_camera(const BatchArgs &a, const void *d_jobs, uint32_t total,
 // (at least the generation's paths, at most the buffers' slots).
 int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, const DevBvh &bv, uint32_t paths,
                           uint32_t max_bounces, const StreamTune &tune, int n_cu, void *stream);
-// Per-pixel sums of one chunk into its job's sums (the single-job path's
-// accumulation of one generation).
-int stream_launch_accumulate(const BatchArgs &a, const GenChunk &c, float *d_sum, float *d_sumsq, void *stream);
-// HDR image of a finished job: mean * exposure, clamped at 0.
-int stream_launch_resolve(const void *d_jobs, uint32_t slot, uint32_t npix, void *stream);
+// One launch for the generation: every chunk summed per pixel into its
+// job's sums (the single-job path's accumulation of one generation, a job's
+// chunks in order), and the HDR image (mean * exposure, clamped at 0) of
+// every job that finishes in it.  total_pix = sum of the groups' npix.
+int stream_launch_accumulate(const BatchArgs &a, const void *d_jobs, const AccGroup *d_groups, uint32_t ngroups,
+                             uint32_t total_pix, void *stream);
 // Largest stack_need the traversal templates cover for a stack knob value.
 bool stream_stack_ok(uint32_t stack, uint32_t need);
 

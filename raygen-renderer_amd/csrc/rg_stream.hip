@@ -21,7 +21,8 @@
 //           paths at their job's last event, write their radiance clamped
 //           by their job's firefly clamp, live ones are appended to the
 //           next queue
-//   then each chunk is summed per pixel into its job's sums.
+//   then one launch sums each chunk per pixel into its job's sums and
+//   writes the HDR image of every job that finishes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -341,41 +342,59 @@ __global__ __launch_bounds__(kPtBlock) void k_bflush(BatchArgs a, int qi) {
     st.sample_out[p] = clamped(st.L[p], a.chunks[a.chunk_of[p]].clamp);
 }
 
-// Per-pixel sums of a chunk's S samples in sample order, added to the job's
-// sums: the single-job path's accumulation of one generation.
-__global__ __launch_bounds__(kPtBlock) void k_baccumulate(const float4 *__restrict__ out, uint32_t npix, uint32_t S,
-                                                           float *__restrict__ sum, float *__restrict__ sumsq) {
-    const uint32_t pix = blockIdx.x * kPtBlock + threadIdx.x;
-    if (pix >= npix) return;
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
-    for (uint32_t s = 0; s < S; s++) {
-        const float4 c = out[(size_t)s * npix + pix];
-        sx = sx + c.x;
-        sy = sy + c.y;
-        sz = sz + c.z;
-        qx = qx + c.x * c.x;
-        qy = qy + c.y * c.y;
-        qz = qz + c.z * c.z;
+// Per-pixel sums of each chunk's S samples in sample order, added to its
+// job's sums chunk after chunk (the single-job path's accumulation of one
+// generation, the same float operations), then, for a job that finishes,
+// its HDR pixel (rayrenderer.cpp:990-1009): mean * exposure, clamped at 0.
+__global__ __launch_bounds__(kPtBlock) void k_baccumulate(BatchArgs a, const DevJobSlot *__restrict__ jobs,
+                                                           const AccGroup *__restrict__ groups, uint32_t ngroups,
+                                                           uint32_t total_pix) {
+    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+    if (i >= total_pix) return;
+    uint32_t lo = 0, hi = ngroups;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (groups[mid].pix_first <= i) lo = mid;
+        else hi = mid;
     }
-    sum[3 * pix] += sx;
-    sum[3 * pix + 1] += sy;
-    sum[3 * pix + 2] += sz;
-    sumsq[3 * pix] += qx;
-    sumsq[3 * pix + 1] += qy;
-    sumsq[3 * pix + 2] += qz;
-}
-
-// HDR output of a finished job (rayrenderer.cpp:990-1009): mean * exposure,
-// clamped at 0.
-__global__ __launch_bounds__(kPtBlock) void k_bresolve(const DevJobSlot *__restrict__ jobs, uint32_t slot) {
-    const DevJob &J = jobs[slot].j;
-    const uint32_t pix = blockIdx.x * kPtBlock + threadIdx.x;
-    if (pix >= J.npix) return;
-    const float inv = J.spp > 0 ? 1.0f / (float)J.spp : 0.0f;
-    J.hdr[4 * pix] = fmaxf(J.sum[3 * pix] * inv * J.exposure, 0.0f);
-    J.hdr[4 * pix + 1] = fmaxf(J.sum[3 * pix + 1] * inv * J.exposure, 0.0f);
-    J.hdr[4 * pix + 2] = fmaxf(J.sum[3 * pix + 2] * inv * J.exposure, 0.0f);
-    J.hdr[4 * pix + 3] = 1.0f;
+    const AccGroup g = groups[lo];
+    const uint32_t pix = i - g.pix_first;
+    const DevJob &J = jobs[a.chunks[g.c0].job].j;
+    float ax = J.sum[3 * pix], ay = J.sum[3 * pix + 1], az = J.sum[3 * pix + 2];
+    float bx = J.sumsq[3 * pix], by = J.sumsq[3 * pix + 1], bz = J.sumsq[3 * pix + 2];
+    for (uint32_t c = g.c0; c < g.c1; c++) {
+        const GenChunk &C = a.chunks[c];
+        const float4 *out = a.st.sample_out + C.first;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
+        for (uint32_t s = 0; s < C.S; s++) {
+            const float4 v = out[(size_t)s * g.npix + pix];
+            sx = sx + v.x;
+            sy = sy + v.y;
+            sz = sz + v.z;
+            qx = qx + v.x * v.x;
+            qy = qy + v.y * v.y;
+            qz = qz + v.z * v.z;
+        }
+        ax = ax + sx;
+        ay = ay + sy;
+        az = az + sz;
+        bx = bx + qx;
+        by = by + qy;
+        bz = bz + qz;
+    }
+    J.sum[3 * pix] = ax;
+    J.sum[3 * pix + 1] = ay;
+    J.sum[3 * pix + 2] = az;
+    J.sumsq[3 * pix] = bx;
+    J.sumsq[3 * pix + 1] = by;
+    J.sumsq[3 * pix + 2] = bz;
+    if (g.finish) {
+        const float inv = J.spp > 0 ? 1.0f / (float)J.spp : 0.0f;
+        J.hdr[4 * pix] = fmaxf(ax * inv * J.exposure, 0.0f);
+        J.hdr[4 * pix + 1] = fmaxf(ay * inv * J.exposure, 0.0f);
+        J.hdr[4 * pix + 2] = fmaxf(az * inv * J.exposure, 0.0f);
+        J.hdr[4 * pix + 3] = 1.0f;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -447,16 +466,12 @@ int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, const DevBvh &
     return chk_s(hipGetLastError(), "batch bounce launch");
 }
 
-int stream_launch_accumulate(const BatchArgs &a, const GenChunk &c, float *d_sum, float *d_sumsq, void *stream) {
-    hipLaunchKernelGGL(k_baccumulate, dim3((c.npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0,
-                       (hipStream_t)stream, a.st.sample_out + c.first, c.npix, c.S, d_sum, d_sumsq);
+int stream_launch_accumulate(const BatchArgs &a, const void *d_jobs, const AccGroup *d_groups, uint32_t ngroups,
+                             uint32_t total_pix, void *stream) {
+    if (!ngroups || !total_pix) return RG_OK;
+    hipLaunchKernelGGL(k_baccumulate, dim3((total_pix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0,
+                       (hipStream_t)stream, a, (const DevJobSlot *)d_jobs, d_groups, ngroups, total_pix);
     return chk_s(hipGetLastError(), "batch accumulate launch");
-}
-
-int stream_launch_resolve(const void *d_jobs, uint32_t slot, uint32_t npix, void *stream) {
-    hipLaunchKernelGGL(k_bresolve, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, (hipStream_t)stream,
-                       (const DevJobSlot *)d_jobs, slot);
-    return chk_s(hipGetLastError(), "batch resolve launch");
 }
 
 }  // namespace rg
