@@ -34,10 +34,6 @@ namespace rg {
 // kernels
 // ---------------------------------------------------------------------------
 __global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
-__global__ void k_zero2(uint32_t *a, uint32_t *b) {
-    *a = 0;
-    *b = 0;
-}
 
 // Camera rays (rayrenderer.cpp:860-934) for the paths of samples
 // [sample_base, sample_base + S) of every pixel in an active tile.
@@ -96,11 +92,18 @@ struct ExtendSrc {
     __device__ __forceinline__ void store_any(uint64_t, bool) const {}
 };
 
+// Block 0 also clears the shadow-ray queue and the next path queue, which
+// nothing reads while this kernel runs and the shade and gather kernels
+// after it fill.
 template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_extend(DevBvh bv, PTState st, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t count = st.count[qi];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && count) atomicAdd(st.stats + 0, (unsigned long long)count);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st.count[2] = 0;
+        st.count[qi ^ 1] = 0;
+        if (count) atomicAdd(st.stats + 0, (unsigned long long)count);
+    }
     trace_ps_loop<kStack, kW, false, kPtBlock>(bv, ExtendSrc{st.ray, st.q[qi], st.hit}, count, lc);
 }
 
@@ -364,7 +367,6 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
     const int bounces = sc.max_depth + 2;
     for (int b = 0; b < bounces; b++) {
         hipLaunchKernelGGL((k_extend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, stream, bv, st, qi);
-        hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, stream, st.count + 2, st.count + (qi ^ 1));
         hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, qi);
         hipLaunchKernelGGL((k_anyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, stream, bv, st);
         hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(kPtBlock), 0, stream, st, qi, clamp_max);

@@ -102,12 +102,6 @@ __device__ __forceinline__ uint32_t find_chunk(const GenChunk *__restrict__ ch, 
 }
 
 __global__ void k_bset_u32(uint32_t *p, uint32_t v) { *p = v; }
-// Clears the shadow-ray and mixed-queue lengths and the next path queue's.
-__global__ void k_bzero3(uint32_t *count, int next) {
-    count[2] = 0;
-    count[3] = 0;
-    count[next] = 0;
-}
 
 __global__ __launch_bounds__(kPtBlock) void k_bcamera(BatchArgs a, const DevJobSlot *__restrict__ jobs,
                                                        uint32_t total) {
@@ -151,12 +145,19 @@ struct BatchExtendSrc {
     __device__ __forceinline__ void store_any(uint64_t, bool) const {}
 };
 
+// Block 0 also clears the shadow-ray queue, the mixed queue and the next
+// path queue, which nothing reads while this kernel runs.
 template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_bextend(DevBvh bv, BatchArgs a, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const PTState &st = a.st;
     const uint32_t count = st.count[qi];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && count) atomicAdd(st.stats + 0, (unsigned long long)count);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st.count[2] = 0;
+        st.count[3] = 0;
+        st.count[qi ^ 1] = 0;
+        if (count) atomicAdd(st.stats + 0, (unsigned long long)count);
+    }
     trace_ps_loop<kStack, kW, false, kPtBlock>(bv, BatchExtendSrc{st.ray, st.q[qi], st.hit}, count, lc);
 }
 
@@ -441,7 +442,6 @@ static void bounces_impl(const BatchArgs &a, const DevJobSlot *jobs, const DevBv
     int qi = 0;
     for (uint32_t b = 0; b < max_bounces; b++) {
         hipLaunchKernelGGL((k_bextend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, s, bv, a, qi);
-        hipLaunchKernelGGL(k_bzero3, dim3(1), dim3(1), 0, s, a.st.count, qi ^ 1);
         hipLaunchKernelGGL(k_bshade, dim3(flat), dim3(kPtBlock), 0, s, a, jobs, qi);
         hipLaunchKernelGGL(k_bshade_mixed, dim3(mixed_blocks), dim3(kPtBlock), 0, s, a, jobs);
         hipLaunchKernelGGL((k_banyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, s, bv, a);
