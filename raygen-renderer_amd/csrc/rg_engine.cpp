@@ -224,7 +224,7 @@ class Engine {
         // one device block for every lane's path state, chunk table and the
         // job records, and one pinned block for their host staging (each
         // allocation call costs about as much as a small job's render)
-        static const uint64_t kLaneArrays[] = {32, 16, 16, 16, 16, 16, 64, 8, 2, 32, 4, 4, 4, 16, 4, 4};
+        static const uint64_t kLaneArrays[] = {32, 16, 16, 16, 16, 16, 64, 8, 2, 32, 4, 4, 4, 4, 4};
         uint64_t lane_bytes = 2 * align_up(64) + align_up((uint64_t)kMaxChunks * sizeof(GenChunk)) +
                               align_up((uint64_t)kMaxChunks * sizeof(AccGroup));
         for (uint64_t b : kLaneArrays) lane_bytes += align_up(P * b);
@@ -263,7 +263,6 @@ class Engine {
             st.nshadow = (uint32_t *)carve(P * 4);
             st.q[0] = (uint32_t *)carve(P * 4);
             st.q[1] = (uint32_t *)carve(P * 4);
-            st.sample_out = (float4 *)carve(P * 16);
             ln.args.chunk_of = (uint32_t *)carve(P * 4);
             ln.args.mixed = (uint32_t *)carve(P * 4);
             st.count = (uint32_t *)carve(64);

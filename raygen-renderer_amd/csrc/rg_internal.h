@@ -87,7 +87,7 @@ struct PTBuffers {
     void *ray = nullptr, *hit = nullptr, *beta = nullptr, *tp = nullptr, *L = nullptr, *meta = nullptr;
     void *sray = nullptr, *ssrc = nullptr, *socc = nullptr, *scontrib = nullptr, *nshadow = nullptr;
     void *q0 = nullptr, *q1 = nullptr;
-    void *count = nullptr, *sample_out = nullptr;
+    void *count = nullptr;
     void *sum = nullptr, *sumsq = nullptr, *hdr = nullptr;
     void *tile_mask = nullptr, *tile_nspp = nullptr, *tile_noise = nullptr, *n_active = nullptr;
     void *aov = nullptr, *dn_a = nullptr, *dn_b = nullptr;

@@ -102,7 +102,7 @@ struct PTState {
     rg_hit *hit;        // its closest hit
     float4 *beta;       // estimator weight, w = bsdfSampledPdf
     float4 *tp;         // BSDFParam::throughput (Russian roulette)
-    float4 *L;          // radiance so far, w = alive flag
+    float4 *L;          // radiance so far (without the last event's NEE), w = alive flag
     int4 *meta;         // passes, medium, next LDS dim, next Philox draw
     rg_ray *sray;       // NEE shadow rays per (path, k), k < 2 (tmin = predicate)
     uint32_t *ssrc;     // queue of the (path, k) pairs with a shadow ray: 2 * path + k
@@ -111,7 +111,6 @@ struct PTState {
     uint32_t *nshadow;  // shadow rays per path this event
     uint32_t *q[2];     // live-path queues
     uint32_t *count;    // [0], [1] path-queue lengths, [2] shadow-ray queue length
-    float4 *sample_out; // clamped sample radiance per path
     unsigned long long *stats; // [0] extension rays, [1] shadow rays, [2] path samples
     uint32_t sample_base;
 };

@@ -1,20 +1,21 @@
 // rg_pt.hip — wavefront BSDF path tracer (shading system 5) for gfx950.
 //
-// One path sample per thread slot.  Each bounce runs five kernels over a
+// One path sample per thread slot.  Each bounce runs three kernels over a
 // compacted queue of live paths:
 //   extend  closest-hit traversal of every live path's ray
 //   shade   one path event: medium free flight, emission + MIS, back-face
 //           rules, depth cap, Russian roulette, lobe choice, NEE shadow-ray
 //           set-up and the next ray (RayBSDFShaderProvider::shade,
 //           rayrenderer.cpp:2776-2896; RayRenderer::tracePath, :1596-1864;
-//           bsdf.cpp:54-610)
+//           bsdf.cpp:54-610); it first adds the previous event's unoccluded
+//           NEE contributions (event order), and appends the live paths to
+//           the next queue with ballot/popcount compaction (one atomic per
+//           block)
 //   anyhit  any-hit tests of the queued NEE rays (area light, envmap), one
 //           ray per lane: shade appends each event's 0-2 rays to a queue
-//   gather  unoccluded contributions are added to the path radiance in
-//           event order; live paths are appended to the next queue with
-//           wave64 ballot/popcount compaction (one atomic per wave)
 // followed, after the last bounce, by a per-pixel accumulate kernel that
-// sums the clamped samples in a fixed order.
+// adds each finished path's last NEE contributions, clamps and sums the
+// samples in a fixed order.
 //
 // The Mix shader's lobes are chosen stochastically (probability proportional
 // to the lobe weights) instead of being evaluated recursively; the estimator
@@ -57,6 +58,7 @@ __global__ __launch_bounds__(kPtBlock) void k_camera(PTScene sc, PTState st, uin
             st.tp[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
             st.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             st.meta[i] = make_int4(-1, sc.global_medium, R.dim, (int)R.k);
+            st.nshadow[i] = 0;
         }
     }
     if (!tile_mask) {
@@ -93,8 +95,7 @@ struct ExtendSrc {
 };
 
 // Block 0 also clears the shadow-ray queue and the next path queue, which
-// nothing reads while this kernel runs and the shade and gather kernels
-// after it fill.
+// nothing reads while this kernel runs and the shade kernel after it fills.
 template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) void k_extend(DevBvh bv, PTState st, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
@@ -114,6 +115,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
     const uint32_t count = st.count[qi];
     if (blockIdx.x * kPtBlock >= count) return;  // whole block past the end
     uint32_t p = 0, ns = 0;
+    bool alive = false;
     if (i < count) {
         p = st.q[qi][i];
         const uint32_t npix = sc.width * sc.height;
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         P.beta = f4v(b);
         P.bsdf_pdf = b.w;
         P.tp = f4v(st.tp[p]);
-        P.L = f4v(st.L[p]);
+        P.L = with_last_nee(st, p);
         P.passes = meta.x;
         P.medium = meta.y;
         P.alive = true;
@@ -145,12 +147,17 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         store_shadows(S, p, st.sray, st.scontrib);
         ns = (uint32_t)S.n;
         st.nshadow[p] = ns;
+        alive = P.alive;
     }
     // the event's NEE rays go to the shadow-ray queue, one ray per lane there
     for (uint32_t k = 0; k < 2; k++) {
         const uint32_t pos = block_slot(k < ns, st.count + 2, lds);
         if (k < ns) st.ssrc[pos] = 2 * p + k;
     }
+    // live paths go to the next queue; a finished path keeps its radiance and
+    // its last event's NEE rays, which the accumulation adds up
+    const uint32_t pos = block_slot(alive, st.count + (qi ^ 1), lds);
+    if (alive) st.q[qi ^ 1][pos] = p;
 }
 
 // Any-hit test of every queued shadow ray (TriangleBVH::intersectAny,
@@ -176,65 +183,12 @@ __global__ __launch_bounds__(kPtBlock) void k_anyhit(DevBvh bv, PTState st) {
     trace_ps_loop<kStack, kW, true, kPtBlock>(bv, ShadowSrc{st.sray, st.ssrc, st.socc}, n, lc);
 }
 
-// Adds the unoccluded NEE contributions in event order, closes finished
-// paths with their clamped radiance and appends the live ones to the next
-// queue.
-__global__ __launch_bounds__(kPtBlock) void k_gather(PTState st, int qi, float clamp_max) {
-    __shared__ uint32_t lds[kPtBlock / 64 + 1];
-    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    const uint32_t count = st.count[qi];
-    if (blockIdx.x * kPtBlock >= count) return;  // whole block past the end
-    bool alive = false;
-    uint32_t p = 0;
-    if (i < count) {
-        p = st.q[qi][i];
-        float4 L = st.L[p];
-        const uint32_t ns = st.nshadow[p];
-        for (uint32_t k = 0; k < ns; k++) {
-            if (!st.socc[2 * p + k]) {
-                const float4 c = st.scontrib[2 * p + k];
-                L.x = L.x + c.x;
-                L.y = L.y + c.y;
-                L.z = L.z + c.z;
-            }
-        }
-        alive = L.w != 0.0f;
-        if (alive) {
-            st.L[p] = L;
-        } else {
-            float4 c = make_float4(fmaxf(L.x, 0.0f), fmaxf(L.y, 0.0f), fmaxf(L.z, 0.0f), 1.0f);
-            if (clamp_max > 0.0f) {
-                c.x = fminf(c.x, clamp_max);
-                c.y = fminf(c.y, clamp_max);
-                c.z = fminf(c.z, clamp_max);
-            }
-            st.sample_out[p] = c;
-        }
-    }
-    const uint32_t pos = block_slot(alive, st.count + (qi ^ 1), lds);
-    if (alive) st.q[qi ^ 1][pos] = p;
-}
-
-// Paths still alive after the last bounce (cannot happen: every event
-// advances the depth counter) are closed with their current radiance.
-__global__ __launch_bounds__(kPtBlock) void k_flush(PTState st, int qi, float clamp_max) {
-    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    if (i >= st.count[qi]) return;
-    const uint32_t p = st.q[qi][i];
-    const float4 L = st.L[p];
-    float4 c = make_float4(fmaxf(L.x, 0.0f), fmaxf(L.y, 0.0f), fmaxf(L.z, 0.0f), 1.0f);
-    if (clamp_max > 0.0f) {
-        c.x = fminf(c.x, clamp_max);
-        c.y = fminf(c.y, clamp_max);
-        c.z = fminf(c.z, clamp_max);
-    }
-    st.sample_out[p] = c;
-}
-
 // Per-pixel sum / sum of squares over the S samples of this generation,
-// in sample order.
+// in sample order.  A sample's value is its path's radiance plus the
+// unoccluded NEE contributions of its last event, clamped (>= 0, and at
+// clamp_max when that is > 0).
 __global__ __launch_bounds__(kPtBlock) void k_accumulate(PTState st, uint32_t width, uint32_t npix, uint32_t S,
-                                                          const uint8_t *__restrict__ tile_mask,
+                                                          const uint8_t *__restrict__ tile_mask, float clamp_max,
                                                           float *__restrict__ sum, float *__restrict__ sumsq) {
     const uint32_t pix = blockIdx.x * kPtBlock + threadIdx.x;
     if (pix >= npix) return;
@@ -244,7 +198,7 @@ __global__ __launch_bounds__(kPtBlock) void k_accumulate(PTState st, uint32_t wi
     }
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
     for (uint32_t s = 0; s < S; s++) {
-        const float4 c = st.sample_out[(size_t)s * npix + pix];
+        const float4 c = sample_value(st, (uint32_t)((size_t)s * npix + pix), clamp_max);
         sx = sx + c.x;
         sy = sy + c.y;
         sz = sz + c.z;
@@ -369,12 +323,10 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
         hipLaunchKernelGGL((k_extend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, stream, bv, st, qi);
         hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, qi);
         hipLaunchKernelGGL((k_anyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, stream, bv, st);
-        hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(kPtBlock), 0, stream, st, qi, clamp_max);
         qi ^= 1;
     }
-    hipLaunchKernelGGL(k_flush, dim3(blocks), dim3(kPtBlock), 0, stream, st, qi, clamp_max);
     hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, stream, st,
-                       sc.width, npix, S, d_tile_mask, d_sum, d_sumsq);
+                       sc.width, npix, S, d_tile_mask, clamp_max, d_sum, d_sumsq);
     return chk(hipGetLastError(), "path-tracer launch");
 }
 

@@ -1025,6 +1025,37 @@ __device__ __forceinline__ uint32_t wave_slot(bool pred, uint32_t *counter) {
     return pred ? base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : 0xFFFFFFFFu;
 }
 
+// A path's radiance with the unoccluded NEE contributions of its last event
+// added in event order (the shadow rays' results arrive after the event that
+// cast them: the next event of a live path adds them first, the accumulation
+// adds them for a finished one).
+__device__ __forceinline__ v3 with_last_nee(const PTState &st, uint32_t p) {
+    float4 L = st.L[p];
+    const uint32_t ns = st.nshadow[p];
+    for (uint32_t k = 0; k < ns; k++) {
+        if (!st.socc[2 * p + k]) {
+            const float4 c = st.scontrib[2 * p + k];
+            L.x = L.x + c.x;
+            L.y = L.y + c.y;
+            L.z = L.z + c.z;
+        }
+    }
+    return mk(L.x, L.y, L.z);
+}
+
+// A finished sample's value: radiance clamped at 0 and, when clamp_max > 0,
+// at clamp_max (the reference's firefly clamp, rayrenderer.cpp:985-988).
+__device__ __forceinline__ float4 sample_value(const PTState &st, uint32_t p, float clamp_max) {
+    const v3 L = with_last_nee(st, p);
+    float4 c = make_float4(fmaxf(L.x, 0.0f), fmaxf(L.y, 0.0f), fmaxf(L.z, 0.0f), 1.0f);
+    if (clamp_max > 0.0f) {
+        c.x = fminf(c.x, clamp_max);
+        c.y = fminf(c.y, clamp_max);
+        c.z = fminf(c.z, clamp_max);
+    }
+    return c;
+}
+
 // Position in a queue for every thread of the block with pred set, in
 // thread order (0xFFFFFFFF for the others): one atomic per block.  Every
 // thread of the block must call it; `lds` holds kPtBlock / 64 + 1 words.

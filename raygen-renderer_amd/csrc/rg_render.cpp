@@ -182,7 +182,7 @@ int reupload_tris(Scene *s, void *stream) {
 void free_buffers(PTBuffers &b) {
     for (void **p : {&b.ray, &b.hit, &b.beta, &b.tp, &b.L, &b.meta, &b.sray, &b.ssrc, &b.socc, &b.scontrib,
                      &b.nshadow, &b.q0,
-                     &b.q1, &b.count, &b.sample_out, &b.sum, &b.sumsq, &b.hdr, &b.tile_mask, &b.tile_nspp,
+                     &b.q1, &b.count, &b.sum, &b.sumsq, &b.hdr, &b.tile_mask, &b.tile_nspp,
                      &b.tile_noise, &b.n_active, &b.aov, &b.dn_a, &b.dn_b, &b.stats}) {
         dev_free(*p);
         *p = nullptr;
@@ -203,7 +203,7 @@ int ensure_buffers(PTBuffers &b, uint64_t cap, uint64_t npix, uint64_t ntiles) {
         {&b.ray, cap * 32}, {&b.hit, cap * 16}, {&b.beta, cap * 16}, {&b.tp, cap * 16}, {&b.L, cap * 16},
         {&b.meta, cap * 16}, {&b.sray, cap * 64}, {&b.ssrc, cap * 8}, {&b.socc, cap * 2},
         {&b.scontrib, cap * 32}, {&b.nshadow, cap * 4},
-        {&b.q0, cap * 4}, {&b.q1, cap * 4}, {&b.count, 64}, {&b.sample_out, cap * 16}, {&b.sum, npix * 12},
+        {&b.q0, cap * 4}, {&b.q1, cap * 4}, {&b.count, 64}, {&b.sum, npix * 12},
         {&b.sumsq, npix * 12}, {&b.hdr, npix * 16}, {&b.tile_mask, ntiles}, {&b.tile_nspp, ntiles * 4},
         {&b.tile_noise, ntiles * 4}, {&b.n_active, 64}, {&b.aov, npix * 36}, {&b.dn_a, npix * 16},
         {&b.dn_b, npix * 16}, {&b.stats, 64},
@@ -519,7 +519,6 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     ps.q[0] = (uint32_t *)B.q0;
     ps.q[1] = (uint32_t *)B.q1;
     ps.count = (uint32_t *)B.count;
-    ps.sample_out = (float4 *)B.sample_out;
     ps.stats = (unsigned long long *)B.stats;
     ps.sample_base = 0;
 

@@ -5,8 +5,8 @@
 // [base, base + S) of every pixel, S being the single-job path's samples in
 // flight for that job (its accumulation group, capped by what is left).
 // Chunks of any number of jobs of one scene share the generation's path
-// slots up to the budget, and the bounce loop (extend, shade, shadow,
-// gather) runs over one compacted queue of all their paths on the scene's
+// slots up to the budget, and the bounce loop (extend, shade, shadow rays)
+// runs over one compacted queue of all their paths on the scene's
 // tree, each event with its own job's scene record.  After the loop every chunk is
 // summed per pixel into its job's sums exactly as the single-job path sums
 // one of its generations, so a job's image does not depend on what ran
@@ -84,8 +84,7 @@ int stream_cu_count(int device);
 // Camera rays of every chunk's samples (queue 0 = all `total` paths).
 int stream_launch_camera(const BatchArgs &a, const void *d_jobs, uint32_t total, void *stream);
 // The bounce loop (max_bounces events, the largest of the chunks' bounce
-// counts; each path stops after its own chunk's) and the flush of paths
-// still alive after it; the per-path kernels are sized for `paths` slots
+// counts; each path stops after its own chunk's); the per-path kernels are sized for `paths` slots
 // (at least the generation's paths, at most the buffers' slots).
 int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, const DevBvh &bv, uint32_t paths,
                           uint32_t max_bounces, const StreamTune &tune, int n_cu, void *stream);

@@ -10,19 +10,18 @@
 //   extend  closest hit on the generation's tree (every chunk of a
 //           generation renders one scene; persistent waves, leaf
 //           postponement, lane refill)
-//   shade   one path event with the job's scene record, wave-uniform for
-//           the lanes of the wave's first job (the same step() as the
+//   shade   the previous event's unoccluded NEE contributions, then one
+//           path event with the job's scene record, wave-uniform for the
+//           lanes of the wave's first job (the same step() as the
 //           single-job path, at its register budget); paths of other jobs in
 //           the wave are shaded by a second kernel that loops over the
-//           distinct jobs; the event's 0-2 NEE rays are queued with
-//           block-aggregated appends
+//           distinct jobs; the event's 0-2 NEE rays and the paths still
+//           live (not finished, not at their job's last event) are queued
+//           with block-aggregated appends
 //   anyhit  the queued NEE rays, one per lane
-//   gather  unoccluded contributions in event order; finished paths, and
-//           paths at their job's last event, write their radiance clamped
-//           by their job's firefly clamp, live ones are appended to the
-//           next queue
-//   then one launch sums each chunk per pixel into its job's sums and
-//   writes the HDR image of every job that finishes.
+//   then one launch adds each finished path's last NEE contributions,
+//   clamps with the job's firefly clamp, sums each chunk per pixel into its
+//   job's sums and writes the HDR image of every job that finishes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -122,6 +121,7 @@ __global__ __launch_bounds__(kPtBlock) void k_bcamera(BatchArgs a, const DevJobS
         st.tp[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         st.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         st.meta[i] = make_int4(-1, J.sc.global_medium, R.dim, (int)R.k);
+        st.nshadow[i] = 0;
     });
     st.q[0][i] = i;
     a.chunk_of[i] = c;
@@ -162,8 +162,14 @@ __global__ __launch_bounds__(kPtBlock) void k_bextend(DevBvh bv, BatchArgs a, in
 }
 
 // One path event of live path p, one of chunk C's paths, with its job's
-// scene record.
-__device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk &C, const PTScene &sc, uint32_t p) {
+// scene record, the bounce-th of the generation.  The previous event's
+// unoccluded NEE contributions are added first (in event order); the path
+// stays live (alive set) unless the event ended it or it was its job's last
+// event (bounce + 1 = max_depth + 2): a closed path keeps its radiance and
+// its last event's NEE rays for the accumulation, as the single-job path
+// closes its paths.
+__device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk &C, const PTScene &sc, uint32_t p,
+                                              uint32_t bounce, bool &alive) {
     const uint32_t local = p - C.first;
     const uint32_t pix = local % C.npix;
     const int sample = (int)(C.sample_base + local / C.npix);
@@ -177,7 +183,7 @@ __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk 
     P.beta = f4v(b);
     P.bsdf_pdf = b.w;
     P.tp = f4v(st.tp[p]);
-    P.L = f4v(st.L[p]);
+    P.L = with_last_nee(st, p);
     P.passes = meta.x;
     P.medium = meta.y;
     P.alive = true;
@@ -192,14 +198,20 @@ __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk 
     st.meta[p] = make_int4(P.passes, P.medium, R.dim, (int)R.k);
     store_shadows(S, p, st.sray, st.scontrib);
     st.nshadow[p] = (uint32_t)S.n;
+    alive = P.alive && bounce + 1 < C.bounces;
     return (uint32_t)S.n;
 }
 
-__device__ __forceinline__ void queue_shadows(const PTState &st, uint32_t p, uint32_t ns, uint32_t *lds) {
+// The event's NEE rays go to the shadow-ray queue (one ray per lane there),
+// a live path to the next path queue.  Every thread of the block calls it.
+__device__ __forceinline__ void queue_results(const PTState &st, uint32_t p, uint32_t ns, bool alive, int qi,
+                                              uint32_t *lds) {
     for (uint32_t k = 0; k < 2; k++) {
         const uint32_t pos = block_slot(k < ns, st.count + 2, lds);
         if (k < ns) st.ssrc[pos] = 2 * p + k;
     }
+    const uint32_t pos = block_slot(alive, st.count + (qi ^ 1), lds);
+    if (alive) st.q[qi ^ 1][pos] = p;
 }
 
 // Copies a record whose address is the same in every lane into scalar
@@ -221,13 +233,14 @@ __device__ __forceinline__ void uniform_copy(T &dst, const T *src) {
 // and occupancy); the other paths of the wave (waves that straddle two
 // chunks) go to the mixed queue for k_bshade_mixed.
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_bshade(
-    BatchArgs a, const DevJobSlot *__restrict__ jobs, int qi) {
+    BatchArgs a, const DevJobSlot *__restrict__ jobs, int qi, uint32_t bounce) {
     __shared__ uint32_t lds[kPtBlock / 64 + 1];
     const PTState &st = a.st;
     const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
     const uint32_t count = st.count[qi];
     if (blockIdx.x * kPtBlock >= count) return;  // whole block past the end
     uint32_t p = 0, ns = 0, c = 0xFFFFFFFFu;
+    bool alive = false;
     if (i < count) {
         p = st.q[qi][i];
         c = a.chunk_of[p];
@@ -242,27 +255,28 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         uniform_copy(C, &a.chunks[c0]);
         uniform_copy(sc, &jobs[C.job].j.sc);
     }
-    if (mine) ns = shade_one(st, C, sc, p);
-    queue_shadows(st, p, ns, lds);
+    if (mine) ns = shade_one(st, C, sc, p, bounce, alive);
+    queue_results(st, p, ns, alive, qi, lds);
 }
 
 // The mixed queue's path events, each with its own job's record (one
 // distinct job at a time across the wave).  A fixed grid walks the queue in
 // block-sized steps.
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bshade_mixed(
-    BatchArgs a, const DevJobSlot *__restrict__ jobs) {
+    BatchArgs a, const DevJobSlot *__restrict__ jobs, int qi, uint32_t bounce) {
     __shared__ uint32_t lds[kPtBlock / 64 + 1];
     const PTState &st = a.st;
     const uint32_t count = st.count[3];
     for (uint32_t base = blockIdx.x * kPtBlock; base < count; base += gridDim.x * kPtBlock) {
         const uint32_t i = base + threadIdx.x;
         uint32_t p = 0, ns = 0;
+        bool alive = false;
         if (i < count) {
             p = a.mixed[i];
             const GenChunk &C = a.chunks[a.chunk_of[p]];
-            with_job(jobs, C.job, [&](const DevJob &J) { ns = shade_one(st, C, J.sc, p); });
+            with_job(jobs, C.job, [&](const DevJob &J) { ns = shade_one(st, C, J.sc, p, bounce, alive); });
         }
-        queue_shadows(st, p, ns, lds);
+        queue_results(st, p, ns, alive, qi, lds);
     }
 }
 
@@ -289,63 +303,11 @@ __global__ __launch_bounds__(kPtBlock) void k_banyhit(DevBvh bv, BatchArgs a) {
     trace_ps_loop<kStack, kW, true, kPtBlock>(bv, BatchShadowSrc{st.sray, st.ssrc, st.socc}, n, lc);
 }
 
-__device__ __forceinline__ float4 clamped(float4 L, float clamp_max) {
-    float4 c = make_float4(fmaxf(L.x, 0.0f), fmaxf(L.y, 0.0f), fmaxf(L.z, 0.0f), 1.0f);
-    if (clamp_max > 0.0f) {
-        c.x = fminf(c.x, clamp_max);
-        c.y = fminf(c.y, clamp_max);
-        c.z = fminf(c.z, clamp_max);
-    }
-    return c;
-}
-
-// Adds the unoccluded NEE contributions in event order, closes finished
-// paths and paths at their job's last event (bounce + 1 = max_depth + 2)
-// with their radiance clamped by the job's firefly clamp and appends the
-// live ones to the next queue.
-__global__ __launch_bounds__(kPtBlock) void k_bgather(BatchArgs a, int qi, uint32_t bounce) {
-    __shared__ uint32_t lds[kPtBlock / 64 + 1];
-    const PTState &st = a.st;
-    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    const uint32_t count = st.count[qi];
-    if (blockIdx.x * kPtBlock >= count) return;  // whole block past the end
-    bool alive = false;
-    uint32_t p = 0;
-    if (i < count) {
-        p = st.q[qi][i];
-        float4 L = st.L[p];
-        const uint32_t ns = st.nshadow[p];
-        for (uint32_t k = 0; k < ns; k++) {
-            if (!st.socc[2 * p + k]) {
-                const float4 c = st.scontrib[2 * p + k];
-                L.x = L.x + c.x;
-                L.y = L.y + c.y;
-                L.z = L.z + c.z;
-            }
-        }
-        const GenChunk &C = a.chunks[a.chunk_of[p]];
-        // a path past its job's last event is closed as the single-job
-        // path's final flush closes it
-        alive = L.w != 0.0f && bounce + 1 < C.bounces;
-        if (alive) st.L[p] = L;
-        else st.sample_out[p] = clamped(L, C.clamp);
-    }
-    const uint32_t pos = block_slot(alive, st.count + (qi ^ 1), lds);
-    if (alive) st.q[qi ^ 1][pos] = p;
-}
-
-// Paths still alive after the last bounce are closed with their radiance.
-__global__ __launch_bounds__(kPtBlock) void k_bflush(BatchArgs a, int qi) {
-    const PTState &st = a.st;
-    const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-    if (i >= st.count[qi]) return;
-    const uint32_t p = st.q[qi][i];
-    st.sample_out[p] = clamped(st.L[p], a.chunks[a.chunk_of[p]].clamp);
-}
-
-// Per-pixel sums of each chunk's S samples in sample order, added to its
-// job's sums chunk after chunk (the single-job path's accumulation of one
-// generation, the same float operations), then, for a job that finishes,
+// Per-pixel sums of each chunk's S samples in sample order (a sample's value:
+// its path's radiance plus its last event's unoccluded NEE contributions,
+// clamped by the job's firefly clamp), added to its job's sums chunk after
+// chunk (the single-job path's accumulation of one generation, the same
+// float operations), then, for a job that finishes,
 // its HDR pixel (rayrenderer.cpp:990-1009): mean * exposure, clamped at 0.
 __global__ __launch_bounds__(kPtBlock) void k_baccumulate(BatchArgs a, const DevJobSlot *__restrict__ jobs,
                                                            const AccGroup *__restrict__ groups, uint32_t ngroups,
@@ -365,10 +327,9 @@ __global__ __launch_bounds__(kPtBlock) void k_baccumulate(BatchArgs a, const Dev
     float bx = J.sumsq[3 * pix], by = J.sumsq[3 * pix + 1], bz = J.sumsq[3 * pix + 2];
     for (uint32_t c = g.c0; c < g.c1; c++) {
         const GenChunk &C = a.chunks[c];
-        const float4 *out = a.st.sample_out + C.first;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
         for (uint32_t s = 0; s < C.S; s++) {
-            const float4 v = out[(size_t)s * g.npix + pix];
+            const float4 v = sample_value(a.st, C.first + s * g.npix + pix, C.clamp);
             sx = sx + v.x;
             sy = sy + v.y;
             sz = sz + v.z;
@@ -442,13 +403,11 @@ static void bounces_impl(const BatchArgs &a, const DevJobSlot *jobs, const DevBv
     int qi = 0;
     for (uint32_t b = 0; b < max_bounces; b++) {
         hipLaunchKernelGGL((k_bextend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, s, bv, a, qi);
-        hipLaunchKernelGGL(k_bshade, dim3(flat), dim3(kPtBlock), 0, s, a, jobs, qi);
-        hipLaunchKernelGGL(k_bshade_mixed, dim3(mixed_blocks), dim3(kPtBlock), 0, s, a, jobs);
+        hipLaunchKernelGGL(k_bshade, dim3(flat), dim3(kPtBlock), 0, s, a, jobs, qi, b);
+        hipLaunchKernelGGL(k_bshade_mixed, dim3(mixed_blocks), dim3(kPtBlock), 0, s, a, jobs, qi, b);
         hipLaunchKernelGGL((k_banyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, s, bv, a);
-        hipLaunchKernelGGL(k_bgather, dim3(flat), dim3(kPtBlock), 0, s, a, qi, b);
         qi ^= 1;
     }
-    hipLaunchKernelGGL(k_bflush, dim3(flat), dim3(kPtBlock), 0, s, a, qi);
 }
 
 int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, const DevBvh &bv, uint32_t paths,
