@@ -160,6 +160,11 @@ typedef struct {
     float metallic, anisotropy, aniso_rotation, emission;
     int32_t interior_medium; /* index into the scene's media, -1 none */
     uint32_t visible;
+    /* Material::chromaDispersion (material.h:36-39): a path's first
+     * dispersive refraction picks one of three bands, index of refraction
+     * refraction_ratio * (1 - s), * 1, * (1 + s) (bsdf.cpp:394-420) */
+    float chroma_dispersion;
+    uint32_t pad;
 } rg_object;
 
 /* HomogeneousMedium authoring fields (medium.h:37-130), constant emission. */
@@ -231,7 +236,7 @@ int rg_scene_set_shading(rg_scene *scene, const float *n9, const uint32_t *obj_o
 int rg_scene_set_textures(rg_scene *scene, const rg_texture *tex, uint32_t n_tex, const int32_t *tex_of_obj,
                           const float *tiling_of_obj, const float *uv6);
 /* Media referenced by rg_object.interior_medium; global_medium = the
- * scene's globalMedium (-1 none). */
+ * scene's globalMedium (-1 none).  At most 32767 media. */
 int rg_scene_set_media(rg_scene *scene, const rg_medium *media, uint32_t n, int32_t global_medium);
 /* env NULL or w == 0 removes the environment. */
 int rg_scene_set_env(rg_scene *scene, const rg_env *env);

@@ -172,7 +172,8 @@ def gen_rays_sphere(box, scale, key, first, n):
 OBJECT_DTYPE = np.dtype([("color", "<f4", (3,)), ("glossy", "<f4"), ("roughness", "<f4"),
                          ("transparency", "<f4"), ("refraction", "<f4"), ("refraction_ratio", "<f4"),
                          ("metallic", "<f4"), ("anisotropy", "<f4"), ("aniso_rotation", "<f4"),
-                         ("emission", "<f4"), ("interior_medium", "<i4"), ("visible", "<u4")])
+                         ("emission", "<f4"), ("interior_medium", "<i4"), ("visible", "<u4"),
+                         ("chroma_dispersion", "<f4"), ("pad", "<u4")])
 MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", (3,)), ("sigma_s", "<f4", (3,)), ("sigma_e", "<f4", (3,)),
                          ("g", "<f4"), ("density", "<f4"), ("pad", "<f4")])
 

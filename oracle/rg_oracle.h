@@ -110,6 +110,8 @@ typedef struct {
     float metallic, anisotropy, aniso_rotation, emission;
     int32_t interior_medium; /* index into media, -1 none */
     uint32_t visible;
+    float chroma_dispersion; /* Material::chromaDispersion (material.h:36-39) */
+    uint32_t pad;
 } or_object;
 
 /* HomogeneousMedium authoring fields (medium.h:37-130, constant emission). */

@@ -621,6 +621,7 @@ typedef struct {
     v3 tp;          /* BSDFParam::throughput */
     float bsdf_pdf; /* BSDFParam::bsdfSampledPdf */
     int medium;     /* BSDFParam::currentMedium */
+    int chroma;     /* BSDFParam::chromaChannel, -1 = none chosen */
 } param_t;
 
 static v3 trace_path(const prep_t *P, rng_t *R, v3 o, v3 d, param_t *sp);
@@ -722,11 +723,35 @@ static int crossed_medium(const prep_t *P, const or_object *m, int entering, int
     return P->d->global_medium;
 }
 
-/* RefractionShader (bsdf.cpp:376-483), no chromatic dispersion */
+/* Chromatic dispersion of RefractionShader (bsdf.cpp:394-420): the first
+ * dispersive interface of a path draws its band (0, 1, 2 with probability
+ * 1/3 each), stored in *chroma for every later interface; the band's index
+ * of refraction is refraction_ratio * (1 - s), * 1, * (1 + s).  Returns the
+ * index of refraction and sets *mask to 3 in the band drawn here (1, 1, 1
+ * when the band was drawn earlier or the material does not disperse). */
+static float dispersion_ior(const or_object *m, rng_t *R, int *chroma, v3 *mask) {
+    float ior = m->refraction_ratio;
+    *mask = mk(1, 1, 1);
+    if (m->chroma_dispersion > 0.0f) {
+        int first = 0;
+        if (*chroma < 0) {
+            *chroma = (int)fminf(2.0f, floorf(prng(R) * 3.0f));
+            first = 1;
+        }
+        const float s = m->chroma_dispersion;
+        if (*chroma == 0) ior = m->refraction_ratio * (1.0f - s);
+        else if (*chroma == 2) ior = m->refraction_ratio * (1.0f + s);
+        if (first) *mask = mk(*chroma == 0 ? 3.0f : 0.0f, *chroma == 1 ? 3.0f : 0.0f, *chroma == 2 ? 3.0f : 0.0f);
+    }
+    return ior;
+}
+
+/* RefractionShader (bsdf.cpp:376-483) */
 static v3 refraction_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_d, v3 n, param_t *pm) {
     const or_object *m = obj_of(P, h->prim);
     const float ct = fabsf(dot(in_d, n));
-    const float ior = m->refraction_ratio;
+    v3 mask;
+    const float ior = dispersion_ior(m, R, &pm->chroma, &mask);
     const float fr = fresnel_schlick(ct, ior);
     const int refl = prng(R) < fr;
     v3 dir = refl ? reflect(in_d, n) : refract_dir(in_d, n, ior);
@@ -748,7 +773,7 @@ static v3 refraction_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_d,
     pm->tp = st;
     pm->bsdf_pdf = sp;
     pm->medium = smed;
-    return mul(c, col);
+    return mul(mul(c, col), mask);
 }
 
 /* TransparencyShader (bsdf.cpp:513-546) */
@@ -795,6 +820,7 @@ static v3 shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3 in_d, v
     pm.tp = mk(1, 1, 1);
     pm.bsdf_pdf = 0.0f;
     pm.medium = sp ? sp->medium : P->d->global_medium;
+    pm.chroma = -1; /* a fresh BSDFParam: the back-face and eye branches keep it */
     if (m->emission > 0.0f) {
         const v3 E = scl(ld3(m->color), m->emission);
         if (sp && sp->bsdf_pdf > 0.0f) {
@@ -829,6 +855,7 @@ static v3 shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3 in_d, v
         }
         pm.passes = sp->passes + 1;
         pm.tp = sp->tp;
+        pm.chroma = sp->chroma; /* rayrenderer.cpp:2863 */
         pm.bsdf_pdf = sp->bsdf_pdf;
         float rrw = 1.0f;
         if (pm.passes >= MIN_RR_DEPTH) {
@@ -893,6 +920,7 @@ static v3 trace_path(const prep_t *P, rng_t *R, v3 o, v3 d, param_t *sp) {
         pm.passes = passes;
         pm.medium = midx;
         pm.tp = sp ? sp->tp : mk(1, 1, 1);
+        pm.chroma = sp ? sp->chroma : -1; /* rayrenderer.cpp:1781-1784 */
         pm.bsdf_pdf = 0.0f;
         float rrw = 1.0f;
         if (passes >= MIN_RR_DEPTH) {
@@ -953,6 +981,7 @@ typedef struct {
     float bsdf_pdf;
     int passes; /* passes of the spawning param; -1 = eye ray */
     int medium;
+    int chroma; /* chromatic dispersion band, -1 = none chosen */
     int alive;
 } path_t;
 
@@ -961,9 +990,10 @@ static inline void add_L(path_t *p, v3 c) { p->L = add(p->L, c); }
 
 static void twin_surface(const prep_t *P, rng_t *R, path_t *p, const isect_t *h, v3 n);
 
-/* TransparencyShader as one path step.  bsdfSampledPdf is carried over from
- * the parent on the front-face branch and starts at 0 on the back-face and
- * eye branches, where the reference builds a fresh BSDFParam. */
+/* TransparencyShader as one path step.  bsdfSampledPdf and the dispersion
+ * band are carried over from the parent on the front-face branch and start
+ * at 0 and none on the back-face and eye branches, where the reference
+ * builds a fresh BSDFParam. */
 static void twin_transparency(const prep_t *P, path_t *p, const isect_t *h, v3 n, float beta_scale,
                               int fresh_param) {
     const or_object *m = obj_of(P, h->prim);
@@ -972,7 +1002,10 @@ static void twin_transparency(const prep_t *P, path_t *p, const isect_t *h, v3 n
     p->medium = crossed_medium(P, m, dot(p->d, n) < 0.0f, p->medium);
     const v3 gN = geom_normal(P, h->prim, n);
     p->o = surface_origin(h->pos, p->d, gN);
-    if (fresh_param) p->bsdf_pdf = 0.0f;
+    if (fresh_param) {
+        p->bsdf_pdf = 0.0f;
+        p->chroma = -1;
+    }
 }
 
 static void twin_step(const prep_t *P, rng_t *R, path_t *p) {
@@ -1151,7 +1184,8 @@ static void twin_mix(const prep_t *P, rng_t *R, path_t *p, const isect_t *h, v3 
     }
     {
         const float ct = fabsf(dot(p->d, n));
-        const float ior = m->refraction_ratio;
+        v3 mask;
+        const float ior = dispersion_ior(m, R, &p->chroma, &mask);
         const float fr = fresnel_schlick(ct, ior);
         const int refl = prng(R) < fr;
         v3 dir = refl ? reflect(p->d, n) : refract_dir(p->d, n, ior);
@@ -1162,7 +1196,7 @@ static void twin_mix(const prep_t *P, rng_t *R, path_t *p, const isect_t *h, v3 
         const v3 gN = geom_normal(P, h->prim, n);
         if (refl && dot(dir, gN) <= 0.0f) dir = reflect(p->d, gN);
         const v3 col = ld3(m->color);
-        p->beta = mul(p->beta, col);
+        p->beta = mul(mul(p->beta, col), mask);
         p->tp = mul(p->tp, col);
         p->bsdf_pdf = 0.0f;
         const float di = dot(p->d, n), dout = dot(dir, n);
@@ -1307,6 +1341,7 @@ static v3 render_sample(const prep_t *P, const or_settings *S, int mode, int x, 
         p.L = mk(0, 0, 0);
         p.passes = -1;
         p.medium = P->d->global_medium;
+        p.chroma = -1;
         p.alive = 1;
         const medium_t *gm = medium_at(P, P->d->global_medium);
         int eye_hit = 0;

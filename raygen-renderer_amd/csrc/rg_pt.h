@@ -31,6 +31,8 @@ struct DevObject {
     float metallic, anisotropy, aniso_rotation, emission;
     int32_t interior_medium;
     uint32_t visible;
+    float chroma_dispersion;
+    uint32_t pad;
 };
 static_assert(sizeof(DevObject) == sizeof(rg_object), "DevObject must mirror rg_object");
 
@@ -103,7 +105,7 @@ struct PTState {
     float4 *beta;       // estimator weight, w = bsdfSampledPdf
     float4 *tp;         // BSDFParam::throughput (Russian roulette)
     float4 *L;          // radiance so far (without the last event's NEE), w = alive flag
-    int4 *meta;         // passes, medium, next LDS dim, next Philox draw
+    int4 *meta;         // passes, medium word (medium, dispersion band), next LDS dim, next Philox draw
     rg_ray *sray;       // NEE shadow rays per (path, k), k < 2 (tmin = predicate)
     uint32_t *ssrc;     // queue of the (path, k) pairs with a shadow ray: 2 * path + k
     uint8_t *socc;      // occluded flag per (path, k)

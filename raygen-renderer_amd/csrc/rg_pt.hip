@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kPtBlock) void k_camera(PTScene sc, PTState st, uin
             st.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
             st.tp[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
             st.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            st.meta[i] = make_int4(-1, sc.global_medium, R.dim, (int)R.k);
+            st.meta[i] = make_int4(-1, pack_medium(sc.global_medium, -1), R.dim, (int)R.k);
             st.nshadow[i] = 0;
         }
     }
@@ -133,7 +133,8 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         P.tp = f4v(st.tp[p]);
         P.L = with_last_nee(st, p);
         P.passes = meta.x;
-        P.medium = meta.y;
+        P.medium = medium_of(meta.y);
+        P.chroma = chroma_of(meta.y);
         P.alive = true;
         Shadows S;
         S.n = 0;
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         st.beta[p] = v4(P.beta, P.bsdf_pdf);
         st.tp[p] = v4(P.tp, 0.0f);
         st.L[p] = v4(P.L, P.alive ? 1.0f : 0.0f);
-        st.meta[p] = make_int4(P.passes, P.medium, R.dim, (int)R.k);
+        st.meta[p] = make_int4(P.passes, pack_medium(P.medium, P.chroma), R.dim, (int)R.k);
         store_shadows(S, p, st.sray, st.scontrib);
         ns = (uint32_t)S.n;
         st.nshadow[p] = ns;

@@ -629,8 +629,17 @@ struct Path {
     v3 o, d, beta, tp, L;
     float bsdf_pdf;
     int passes, medium;
+    int chroma;  // chromatic dispersion band (BSDFParam::chromaChannel), -1 = none chosen
     bool alive;
 };
+
+// The path state's medium word: the medium index (signed, low 16 bits) and
+// the dispersion band + 1 (bits 16-17; 0 = none chosen).
+__device__ __forceinline__ int pack_medium(int medium, int chroma) {
+    return (int)(((uint32_t)medium & 0xFFFFu) | ((uint32_t)(chroma + 1) << 16));
+}
+__device__ __forceinline__ int medium_of(int w) { return (int)(int16_t)(w & 0xFFFF); }
+__device__ __forceinline__ int chroma_of(int w) { return ((uint32_t)w >> 16) - 1; }
 
 __device__ __forceinline__ int crossed_medium(const PTScene &sc, const DevObject &m, bool entering, int cur) {
     if (entering) return m.interior_medium >= 0 ? m.interior_medium : cur;
@@ -646,7 +655,10 @@ __device__ __forceinline__ void step_transparency(const PTScene &sc, Path &p, ui
     p.medium = crossed_medium(sc, m, dot(p.d, n) < 0.0f, p.medium);
     const v3 gN = geom_normal(sc, prim, n);
     p.o = surface_origin(hit, p.d, gN);
-    if (fresh_param) p.bsdf_pdf = 0.0f;
+    if (fresh_param) {  // a fresh BSDFParam: no MIS pdf, no dispersion band
+        p.bsdf_pdf = 0.0f;
+        p.chroma = -1;
+    }
 }
 
 // MixShader (bsdf.cpp:579-610) with one lobe drawn in proportion to its
@@ -761,9 +773,25 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
         p.d = L;
         return;
     }
-    // RefractionShader (bsdf.cpp:376-483), no chromatic dispersion
+    // RefractionShader (bsdf.cpp:376-483).  Chromatic dispersion
+    // (bsdf.cpp:394-420): the path's first dispersive interface draws its
+    // band, kept for every later one; the band's index of refraction is
+    // refraction_ratio * (1 - s), * 1, * (1 + s), and the estimator weight
+    // is masked to 3 in that band.
     const float ct = fabsf(dot(p.d, n));
-    const float ior = m.refraction_ratio;
+    float ior = m.refraction_ratio;
+    v3 mask = mk(1.0f, 1.0f, 1.0f);
+    if (m.chroma_dispersion > 0.0f) {
+        bool first = false;
+        if (p.chroma < 0) {
+            p.chroma = (int)fminf(2.0f, floorf(prng(R) * 3.0f));
+            first = true;
+        }
+        const float s = m.chroma_dispersion;
+        if (p.chroma == 0) ior = m.refraction_ratio * (1.0f - s);
+        else if (p.chroma == 2) ior = m.refraction_ratio * (1.0f + s);
+        if (first) mask = mk(p.chroma == 0 ? 3.0f : 0.0f, p.chroma == 1 ? 3.0f : 0.0f, p.chroma == 2 ? 3.0f : 0.0f);
+    }
     const float fr = fresnel_schlick(ct, ior);
     const bool refl = prng(R) < fr;
     v3 dir = refl ? reflect(p.d, n) : refract_dir(p.d, n, ior);
@@ -774,7 +802,7 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
     const v3 gN = geom_normal(sc, prim, n);
     if (refl && dot(dir, gN) <= 0.0f) dir = reflect(p.d, gN);
     const v3 col = ld3(m.color);
-    p.beta = mul(p.beta, col);
+    p.beta = mul(mul(p.beta, col), mask);
     p.tp = mul(p.tp, col);
     p.bsdf_pdf = 0.0f;
     const float di = dot(p.d, n), dout = dot(dir, n);

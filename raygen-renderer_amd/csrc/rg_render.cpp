@@ -443,6 +443,8 @@ int rg_scene_set_media(rg_scene *scene, const rg_medium *media, uint32_t n, int3
     Scene *s = reinterpret_cast<Scene *>(scene);
     if (!s || (n && !media)) return set_error(RG_E_INVALID, "null argument");
     if (global_medium >= (int32_t)n) return set_error(RG_E_INVALID, "global medium index out of range");
+    // the path state keeps a path's medium index in 16 bits
+    if (n > 32767) return set_error(RG_E_INVALID, "at most 32767 media");
     s->media.assign(media, media + n);
     s->global_medium = global_medium < 0 ? -1 : global_medium;
     s->pt_dirty = true;

@@ -744,6 +744,7 @@ struct Loader {
         num(o, "metallic", m.metallic);
         num(o, "anisotropy", m.anisotropy);
         num(o, "anisoRotation", m.aniso_rotation);
+        num(o, "chromaDispersion", m.chroma_dispersion);
         const json::Value *c = o.get("color");
         if (c && c->is_string()) parse_color(c->str, m.color);
         else if (c && c->is_array() && c->arr.size() >= 3)
@@ -761,9 +762,7 @@ struct Loader {
             }
         }
         // read by the reference but outside the -d 5 path kept here
-        float cd = 0.0f, sr = 0.0f;
-        num(o, "chromaDispersion", cd);
-        if (cd > 0.0f) warnings.push_back("chromatic dispersion ignored");
+        float sr = 0.0f;
         num(o, "spotRange", sr);
         if (sr > 0.0f) warnings.push_back("spot range ignored");
         if (o.get("normalmap")) warnings.push_back("normal map ignored");

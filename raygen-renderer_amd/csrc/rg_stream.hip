@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kPtBlock) void k_bcamera(BatchArgs a, const DevJobS
         st.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         st.tp[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         st.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        st.meta[i] = make_int4(-1, J.sc.global_medium, R.dim, (int)R.k);
+        st.meta[i] = make_int4(-1, pack_medium(J.sc.global_medium, -1), R.dim, (int)R.k);
         st.nshadow[i] = 0;
     });
     st.q[0][i] = i;
@@ -185,7 +185,8 @@ __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk 
     P.tp = f4v(st.tp[p]);
     P.L = with_last_nee(st, p);
     P.passes = meta.x;
-    P.medium = meta.y;
+    P.medium = medium_of(meta.y);
+    P.chroma = chroma_of(meta.y);
     P.alive = true;
     Shadows S;
     S.n = 0;
@@ -195,7 +196,7 @@ __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk 
     st.beta[p] = v4(P.beta, P.bsdf_pdf);
     st.tp[p] = v4(P.tp, 0.0f);
     st.L[p] = v4(P.L, P.alive ? 1.0f : 0.0f);
-    st.meta[p] = make_int4(P.passes, P.medium, R.dim, (int)R.k);
+    st.meta[p] = make_int4(P.passes, pack_medium(P.medium, P.chroma), R.dim, (int)R.k);
     store_shadows(S, p, st.sray, st.scontrib);
     st.nshadow[p] = (uint32_t)S.n;
     alive = P.alive && bounce + 1 < C.bounces;

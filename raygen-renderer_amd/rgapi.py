@@ -324,7 +324,7 @@ class RenderScene(Scene):
         n9 = np.ascontiguousarray(sc["n9"], np.float32)
         obj = np.ascontiguousarray(sc["obj"], np.uint32)
         objs = np.ascontiguousarray(sc["objects"])
-        assert objs.dtype.itemsize == 56
+        assert objs.dtype.itemsize == 64
         media = np.ascontiguousarray(sc.get("media", np.zeros(0, objs.dtype)))
         self._keep += [n9, obj, objs, media]
         check(lib().rg_scene_set_media(self.h, _ptr(media) if len(media) else None, len(media),

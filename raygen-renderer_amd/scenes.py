@@ -12,14 +12,15 @@ import numpy as np
 OBJECT_DTYPE = np.dtype([("color", "<f4", (3,)), ("glossy", "<f4"), ("roughness", "<f4"),
                          ("transparency", "<f4"), ("refraction", "<f4"), ("refraction_ratio", "<f4"),
                          ("metallic", "<f4"), ("anisotropy", "<f4"), ("aniso_rotation", "<f4"),
-                         ("emission", "<f4"), ("interior_medium", "<i4"), ("visible", "<u4")])
+                         ("emission", "<f4"), ("interior_medium", "<i4"), ("visible", "<u4"),
+                         ("chroma_dispersion", "<f4"), ("pad", "<u4")])
 MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", (3,)), ("sigma_s", "<f4", (3,)), ("sigma_e", "<f4", (3,)),
                          ("g", "<f4"), ("density", "<f4"), ("pad", "<f4")])
 
 
 def material(color=(0.8, 0.8, 0.8), glossy=0.0, roughness=0.5, transparency=0.0, refraction=0.0,
              refraction_ratio=1.45, metallic=0.0, anisotropy=0.0, aniso_rotation=0.0, emission=0.0,
-             interior_medium=-1, visible=1):
+             interior_medium=-1, visible=1, chroma_dispersion=0.0):
     """One object's material with the reference defaults (material.h:26-50)."""
     m = np.zeros(1, OBJECT_DTYPE)[0]
     m["color"] = color
@@ -34,6 +35,7 @@ def material(color=(0.8, 0.8, 0.8), glossy=0.0, roughness=0.5, transparency=0.0,
     m["emission"] = emission
     m["interior_medium"] = interior_medium
     m["visible"] = visible
+    m["chroma_dispersion"] = chroma_dispersion
     return m
 
 
@@ -208,12 +210,13 @@ def constant_env(value=0.3, w=64, h=32):
     return {"rgb": np.full((h, w, 3), value, np.float32), "intensity": 1.0, "rotation_deg": 0.0}
 
 
-def bsdf_frame(grid=5, stacks=200, slices=200, seed=0xB5DF0002, env_value=0.3):
+def bsdf_frame(grid=5, stacks=200, slices=200, seed=0xB5DF0002, env_value=0.3, dispersion=0.0):
     """Full BSDF frame: grid x grid tessellated spheres (2*stacks*slices
     triangles each; 5x5 at 200x200 = 2,000,000), seeded GGX materials with
-    roughness U[0.05, 1], glossy U[0, 1] and refraction 0 or 0.9 at IOR 1.5,
-    one area-light quad, a constant environment and one homogeneous medium box
-    (sigma_a 0.1, sigma_s 0.4, in a transparency-1 cube)."""
+    roughness U[0.05, 1], glossy U[0, 1] and refraction 0 or 0.9 at IOR 1.5
+    (chromaDispersion `dispersion` on the refractive ones), one area-light
+    quad, a constant environment and one homogeneous medium box (sigma_a 0.1,
+    sigma_s 0.4, in a transparency-1 cube)."""
     rng = np.random.default_rng(seed)
     b = Builder()
     span = 4.0
@@ -225,6 +228,8 @@ def bsdf_frame(grid=5, stacks=200, slices=200, seed=0xB5DF0002, env_value=0.3):
             mat = material(color=tuple(rng.uniform(0.3, 0.9, 3)), roughness=float(rng.uniform(0.05, 1.0)),
                            glossy=float(rng.uniform(0.0, 1.0)), refraction=float(0.9 if rng.random() < 0.5 else 0.0),
                            refraction_ratio=1.5)
+            if mat["refraction"] > 0:
+                mat["chroma_dispersion"] = dispersion
             o = b.add_object(mat)
             b.sphere(o, (cx, cy, -6.0), r, stacks, slices)
     light = b.add_object(material(color=(1.0, 1.0, 1.0), emission=8.0))

@@ -12,6 +12,10 @@ forms, independent quadrature and each other.  CPU only.
   L times the directional albedo of the separable-Smith GGX lobe, computed
   here by quadrature;
 * absorbing medium: transmittance exp(-sigma_a * d) through a slab;
+* chromatic dispersion: per band, a dispersive plane's Fresnel reflectance
+  at its band's index of refraction (Schlick); a white dispersive glass
+  sphere under a uniform environment returns L in every band, with the
+  second moment of a single 3x band mask (3 L^2);
 * A-Trous: constant images are fixed points, and with the edge-stopping terms
   switched off the filter is the B3-spline a-trous convolution.
 """
@@ -168,6 +172,58 @@ def test_absorbing_medium_transmittance(mode):
     got = s.mean() / n
     want = E * np.exp(-sigma * dist)
     assert abs(got / want - 1) < 0.03, (got, want)
+
+
+def schlick(cos_t, ior):
+    r0 = ((1 - ior) / (1 + ior)) ** 2
+    return r0 + (1 - r0) * (1 - cos_t) ** 5
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_dispersion_fresnel_per_band(mode):
+    """A smooth dispersive refractive plane seen at 45 degrees under a
+    uniform environment L, a black plane behind it: only the Fresnel-reflected
+    branch reaches the environment, so band c returns L * F(cos 45, ior_c)
+    with ior = n (1 - s), n, n (1 + s) for red, green, blue (bsdf.cpp:394-420:
+    the band is drawn with probability 1/3 and masked by 3)."""
+    n_ior, s, L = 1.5, 0.2, 1.0
+    b = scenes.Builder()
+    g = b.add_object(scenes.material(color=(1, 1, 1), refraction=1.0, refraction_ratio=n_ior, roughness=0.0,
+                                     chroma_dispersion=s))
+    c = np.array([0.0, 0.0, -5.0])
+    t1, t2 = np.array([1.0, 0, 0]), np.array([0, 1.0, -1.0]) / np.sqrt(2)
+    b.quad(g, c - 2 * t1 - 2 * t2, c + 2 * t1 - 2 * t2, c + 2 * t1 + 2 * t2, c - 2 * t1 + 2 * t2,
+           normal=np.array([0, 1.0, 1.0]) / np.sqrt(2))
+    k = b.add_object(scenes.material(color=(0, 0, 0)))
+    b.quad(k, (-50, -50, -8), (50, -50, -8), (50, 50, -8), (-50, 50, -8), normal=(0, 0, 1))
+    sc = b.finish(camera={"fov_deg": 1.0, "aperture": 1.8}, env=scenes.constant_env(L, 64, 32))
+    W = H = 16
+    n = 4096
+    sm, _ = oracle.render(sc, W, H, n, 4, mode=mode, threads=THREADS, antialias=False, firefly_clamp=0.0)
+    got = sm.reshape(-1, 3).mean(0) / n
+    ct = np.cos(np.pi / 4)
+    want = L * np.array([schlick(ct, n_ior * (1 - s)), schlick(ct, n_ior), schlick(ct, n_ior * (1 + s))])
+    # about 3,600 red, 15,000 green and 29,000 blue reflections: <= 1.7% noise
+    np.testing.assert_allclose(got, want, rtol=0.08)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_dispersion_white_furnace(mode):
+    """A smooth white dispersive glass sphere under a uniform environment L
+    loses nothing, so every band returns L; the band drawn at the first
+    interface is kept at the second, so a sample is 3 L in one band and 0 in
+    the others and the second moment is 3 L^2 (a band drawn again at each
+    interface would give 9 L^2)."""
+    sc = sphere_under_env(scenes.material(color=(1, 1, 1), refraction=1.0, refraction_ratio=1.5, roughness=0.0,
+                                          chroma_dispersion=0.1))
+    W = H = 16
+    n = 4096
+    s, q = oracle.render(sc, W, H, n, 12, mode=mode, threads=THREADS, antialias=False, firefly_clamp=0.0)
+    m1 = s[5:11, 5:11].reshape(-1, 3).mean(0) / n
+    m2 = q[5:11, 5:11].reshape(-1, 3).mean(0) / n
+    np.testing.assert_allclose(m1, 1.0, rtol=0.03)
+    # Russian roulette reweights the few paths with three or more events
+    np.testing.assert_allclose(m2, 3.0, rtol=0.06)
 
 
 def test_medium_is_not_left_through_a_transparent_exit_face():

@@ -250,7 +250,7 @@ def test_reference_bokeh_and_ibl_scenes():
     """bokehTest.json: an 8-blade iris rotated 10 degrees, depth of field
     4.5, twelve sphere emitters (SphereMesh defaults 16 x 32: 1024 triangles
     each); iblTest.json: a checkerboard floor tiled 12 x 12 (the reference's
-    texture, decoded) and a dispersive glass ball (dispersion reported)."""
+    texture, decoded) and a dispersive glass ball (chromaDispersion 0.03)."""
     ctx = rgapi.Context(-1)
     _, d = rgapi.load_json(ctx, os.path.join(REF, "bokehTest", "bokehTest.json"))
     cam = d["camera"]
@@ -263,7 +263,9 @@ def test_reference_bokeh_and_ibl_scenes():
     floor = int(np.nonzero(d["tex_of_obj"] >= 0)[0][0])
     np.testing.assert_array_equal(d["tiling"][floor], [12, 12])
     assert d["textures"][0]["rgb"].shape == (256, 256, 3)
-    assert any("dispersion" in w for w in d["warnings"])
+    glass = d["objects"][d["objects"]["refraction"] == 1]
+    assert len(glass) == 1 and np.isclose(glass[0]["chroma_dispersion"], 0.03)
+    assert not any("dispersion" in w for w in d["warnings"])
 
 
 def test_reference_suzanne_scene_and_unquoted_keys():
