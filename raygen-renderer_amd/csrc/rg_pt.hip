@@ -97,7 +97,7 @@ struct ExtendSrc {
 // Block 0 also clears the shadow-ray queue and the next path queue, which
 // nothing reads while this kernel runs and the shade kernel after it fills.
 template <int kStack, int kW>
-__global__ __launch_bounds__(kPtBlock) void k_extend(DevBvh bv, PTState st, int qi) {
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_extend(DevBvh bv, PTState st, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t count = st.count[qi];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -177,7 +177,7 @@ struct ShadowSrc {
 };
 
 template <int kStack, int kW>
-__global__ __launch_bounds__(kPtBlock) void k_anyhit(DevBvh bv, PTState st) {
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_anyhit(DevBvh bv, PTState st) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t n = st.count[2];
     if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(st.stats + 1, (unsigned long long)n);

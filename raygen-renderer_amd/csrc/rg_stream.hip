@@ -148,7 +148,7 @@ struct BatchExtendSrc {
 // Block 0 also clears the shadow-ray queue, the mixed queue and the next
 // path queue, which nothing reads while this kernel runs.
 template <int kStack, int kW>
-__global__ __launch_bounds__(kPtBlock) void k_bextend(DevBvh bv, BatchArgs a, int qi) {
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_bextend(DevBvh bv, BatchArgs a, int qi) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const PTState &st = a.st;
     const uint32_t count = st.count[qi];
@@ -296,7 +296,7 @@ struct BatchShadowSrc {
 };
 
 template <int kStack, int kW>
-__global__ __launch_bounds__(kPtBlock) void k_banyhit(DevBvh bv, BatchArgs a) {
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_banyhit(DevBvh bv, BatchArgs a) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const PTState &st = a.st;
     const uint32_t n = st.count[2];
