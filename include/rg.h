@@ -205,6 +205,19 @@ typedef struct {
     uint32_t denoise_levels;
     float sigma_color, sigma_normal, sigma_depth, denoise_intensity;
     uint32_t samples_in_flight; /* paths per launch wave = pixels x this; 0 = auto */
+    /* enableColorSampling (-encs, rayrenderer.h:106; read at bsdf.cpp:69,
+     * 186 and rayrenderer.cpp:2844): 0 = diffuse albedo 1, glossy F0 from the
+     * untextured colour, depth-cap light term 0 */
+    int32_t color_sampling;
+    /* cullBackFace (-cb, rayrenderer.h:110): accepted and kept; the
+     * reference's shading system 5 never reads it (rayrenderer.cpp:82 only
+     * stores it), so it changes nothing here either */
+    int32_t cull_back_face;
+    /* backColor (-bc, rayrenderer.h:152; default (1, 0.95, 0.9, 0) * 0.2):
+     * the radiance of an eye ray that sees neither geometry nor environment
+     * (rayrenderer.cpp:1273, 1296) and the albedo guide of a background pixel
+     * (:841); alpha is kept for the guide only */
+    float back_color[4];
 } rg_settings;
 
 /* One material texture (Texture, texture.h:19-50): w x h texels of RGB

@@ -205,7 +205,22 @@ class _Desc(ctypes.Structure):
 class _Settings(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
                 ("max_depth", ctypes.c_uint32), ("antialias", ctypes.c_int32), ("firefly_clamp", ctypes.c_float),
-                ("seed", ctypes.c_uint32), ("job", ctypes.c_uint32), ("sample_start", ctypes.c_uint32)]
+                ("seed", ctypes.c_uint32), ("job", ctypes.c_uint32), ("sample_start", ctypes.c_uint32),
+                ("color_sampling", ctypes.c_int32), ("back_color", ctypes.c_float * 4)]
+
+
+BACK_COLOR = (1.0 * 0.2, 0.95 * 0.2, 0.9 * 0.2, 0.0)   # RendererSettings::backColor default (rayrenderer.h:152)
+
+
+def _settings(width, height, spp, max_depth, antialias, firefly_clamp, seed, job, sample_start,
+              color_sampling=True, back_color=None):
+    s = _Settings(width, height, spp, max_depth, 1 if antialias else 0, firefly_clamp, seed & 0xFFFFFFFF, job,
+                  sample_start)
+    s.color_sampling = 1 if color_sampling else 0
+    bc = BACK_COLOR if back_color is None else tuple(back_color) + (0.0,) * (4 - len(back_color))
+    for i in range(4):
+        s.back_color[i] = bc[i]
+    return s
 
 
 def _desc(scene, keep):
@@ -253,7 +268,7 @@ def _desc(scene, keep):
 
 
 def render(scene, width, height, spp, max_depth=5, mode=0, threads=8, antialias=True, firefly_clamp=10.0,
-           seed=0xC0DE0000, job=0, sample_start=0, active_tile=None):
+           seed=0xC0DE0000, job=0, sample_start=0, active_tile=None, color_sampling=True, back_color=None):
     """Returns (sum, sumsq), each (H, W, 3) float32, over the given samples."""
     L = lib()
     if not hasattr(L, "_render_sig"):
@@ -266,8 +281,8 @@ def render(scene, width, height, spp, max_depth=5, mode=0, threads=8, antialias=
         L._render_sig = True
     keep = []
     d = _desc(scene, keep)
-    s = _Settings(width, height, spp, max_depth, 1 if antialias else 0, firefly_clamp, seed & 0xFFFFFFFF, job,
-                  sample_start)
+    s = _settings(width, height, spp, max_depth, antialias, firefly_clamp, seed, job, sample_start, color_sampling,
+                  back_color)
     sm = np.zeros((height, width, 3), np.float32)
     sq = np.zeros((height, width, 3), np.float32)
     at = None if active_tile is None else np.ascontiguousarray(active_tile, dtype=np.uint8)
@@ -275,14 +290,14 @@ def render(scene, width, height, spp, max_depth=5, mode=0, threads=8, antialias=
     return sm, sq
 
 
-def aov(scene, width, height):
+def aov(scene, width, height, back_color=None):
     """Primary-hit guide buffers: (normal_rgba, albedo_rgba, depth)."""
     L = lib()
     L.or_aov.restype = None
     L.or_aov.argtypes = [ctypes.c_void_p] * 5
     keep = []
     d = _desc(scene, keep)
-    s = _Settings(width, height, 1, 1, 1, 10.0, 0, 0, 0)
+    s = _settings(width, height, 1, 1, True, 10.0, 0, 0, 0, back_color=back_color)
     nrm = np.zeros((height, width, 4), np.float32)
     alb = np.zeros((height, width, 4), np.float32)
     dep = np.zeros((height, width), np.float32)
@@ -304,7 +319,8 @@ def denoise(noisy_rgba, nrm, alb, dep, levels=5, sigma_c=0.4, sigma_n=128.0, sig
 
 
 def render_adaptive(scene, width, height, spp, max_depth=5, base=4, threshold=0.02, mode=1, threads=8,
-                    antialias=True, firefly_clamp=10.0, seed=0xC0DE0000, job=0):
+                    antialias=True, firefly_clamp=10.0, seed=0xC0DE0000, job=0, color_sampling=True,
+                    back_color=None):
     """renderAdaptive restated: returns (sum, sumsq, nspp per 32x32 tile)."""
     L = lib()
     L.or_render_adaptive.restype = ctypes.c_int
@@ -312,7 +328,7 @@ def render_adaptive(scene, width, height, spp, max_depth=5, base=4, threshold=0.
                                      ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     keep = []
     d = _desc(scene, keep)
-    s = _Settings(width, height, spp, max_depth, 1 if antialias else 0, firefly_clamp, seed & 0xFFFFFFFF, job, 0)
+    s = _settings(width, height, spp, max_depth, antialias, firefly_clamp, seed, job, 0, color_sampling, back_color)
     sm = np.zeros((height, width, 3), np.float32)
     sq = np.zeros((height, width, 3), np.float32)
     nt = np.zeros(((width + 31) // 32) * ((height + 31) // 32), np.uint32)
@@ -342,3 +358,58 @@ def tonemap(hdr):
     out = np.empty_like(src)
     L.or_tonemap(_p(src), w, h, _p(out))
     return out
+
+
+def phase_sample(g, wo, u2):
+    """or_phase_sample: directions sampled about wo for the (u1, u2) rows of u2."""
+    L = lib()
+    L.or_phase_sample.restype = None
+    L.or_phase_sample.argtypes = [ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    wo = np.ascontiguousarray(wo, np.float32)
+    u2 = np.ascontiguousarray(u2, np.float32).reshape(-1, 2)
+    out = np.zeros((len(u2), 3), np.float32)
+    L.or_phase_sample(g, _p(wo), len(u2), _p(u2), _p(out))
+    return out
+
+
+def phase_pdf(g, wo, wi):
+    L = lib()
+    L.or_phase_pdf.restype = None
+    L.or_phase_pdf.argtypes = [ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    wo = np.ascontiguousarray(wo, np.float32)
+    wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    out = np.zeros(len(wi), np.float32)
+    L.or_phase_pdf(g, _p(wo), len(wi), _p(wi), _p(out))
+    return out
+
+
+def env_probe(scene, u2, qdirs):
+    """or_env_probe: (dirs, pdfs, radiance) of sampled directions and
+    (pdf, radiance) of the query directions of scene's environment."""
+    L = lib()
+    L.or_env_probe.restype = None
+    L.or_env_probe.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    keep = []
+    d = _desc(scene, keep)
+    u2 = np.ascontiguousarray(u2, np.float32).reshape(-1, 2)
+    q = np.ascontiguousarray(qdirs, np.float32).reshape(-1, 3)
+    dirs = np.zeros((len(u2), 3), np.float32)
+    pdfs = np.zeros(len(u2), np.float32)
+    rad = np.zeros((len(u2), 3), np.float32)
+    qp = np.zeros(len(q), np.float32)
+    qr = np.zeros((len(q), 3), np.float32)
+    L.or_env_probe(ctypes.byref(d), len(u2), _p(u2), _p(dirs), _p(pdfs), _p(rad), len(q), _p(q), _p(qp), _p(qr))
+    return dirs, pdfs, rad, qp, qr
+
+
+def tile_noise(sum_rgb, sumsq_rgb, tx, ty, n):
+    """or_tile_noise of tile (tx, ty) of (H, W, 3) sums over n samples."""
+    L = lib()
+    L.or_tile_noise.restype = ctypes.c_float
+    L.or_tile_noise.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_int]
+    a = np.ascontiguousarray(sum_rgb, np.float32)
+    b = np.ascontiguousarray(sumsq_rgb, np.float32)
+    h, w = a.shape[:2]
+    return float(L.or_tile_noise(_p(a), _p(b), w, h, tx, ty, n))

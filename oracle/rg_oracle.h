@@ -170,6 +170,8 @@ typedef struct {
     float firefly_clamp;
     uint32_t seed, job;
     uint32_t sample_start;
+    int32_t color_sampling; /* RendererSettings::enableColorSampling (rayrenderer.h:106) */
+    float back_color[4];    /* RendererSettings::backColor (rayrenderer.h:152) */
 } or_settings;
 
 int or_any1_pred(const or_node *nodes, int64_t nnodes, const float *v9, const uint32_t *perm,
@@ -193,6 +195,16 @@ void or_denoise(int w, int h, const float *noisy_rgba, const float *normal_rgba,
 
 /* adaptive sampling (computeTileNoise / renderAdaptive) */
 float or_tile_noise(const float *sum_rgb, const float *sumsq_rgb, int w, int h, int tx, int ty, int n);
+/* Probes of single restated functions, for the tests that pin them:
+ * HomogeneousMedium::samplePhase / phasePdf (medium.cpp:496-525) for n
+ * (u1, u2) pairs about wo; and the environment of a scene description:
+ * sampleEnvmapDirection for n (u0, u1) pairs (directions, pdfs, radiance
+ * seen along them) and envmapPdf + sampleEnvironment for nq query
+ * directions (rayrenderer.cpp:1299-1573). */
+void or_phase_sample(float g, const float wo[3], int n, const float *u2, float *dirs);
+void or_phase_pdf(float g, const float wo[3], int n, const float *wi, float *pdf);
+void or_env_probe(const or_scene_desc *desc, int n, const float *u2, float *dirs, float *pdfs, float *rad, int nq,
+                  const float *qdirs, float *qpdfs, float *qrad);
 int or_render_adaptive(const or_scene_desc *desc, const or_settings *s, int mode, int nthreads, uint32_t base,
                        float threshold, float *sum_rgb, float *sumsq_rgb, uint32_t *nspp);
 

@@ -102,6 +102,9 @@ typedef struct {
     float view_scale, half_w, half_h, aperture, dof, ap_rot;
     int blades;
     int max_depth;
+    /* enableColorSampling, backColor (rgb) */
+    int color_sampling;
+    v3 back;
     /* linear RGB texels of each texture (sRGB decoded once) */
     float **tex_lin;
 } prep_t;
@@ -635,7 +638,8 @@ static v3 diffuse_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3
     const float r1 = lds1(R), r2 = lds1(R);
     const v3 dir = cosine_dir(n, r1, r2);
     const v3 o = surface_origin(h->pos, dir, geom_normal(P, h->prim, n));
-    const v3 alb = mul(ld3(m->color), albedo_tex(P, h));
+    /* albedo: colour x texture, 1 without colour sampling (bsdf.cpp:68-73) */
+    const v3 alb = P->color_sampling ? mul(ld3(m->color), albedo_tex(P, h)) : mk(1.0f, 1.0f, 1.0f);
     const v3 st = pm->tp;
     const float sp = pm->bsdf_pdf;
     pm->tp = mul(pm->tp, alb);
@@ -652,7 +656,9 @@ static v3 diffuse_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3
 /* GlossyShader (bsdf.cpp:173-374) */
 static v3 glossy_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_d, v3 n, param_t *pm) {
     const or_object *m = obj_of(P, h->prim);
-    const v3 alb = mul(ld3(m->color), albedo_tex(P, h));
+    /* albedo: colour x texture, the colour alone without colour sampling
+     * (bsdf.cpp:185-188) */
+    const v3 alb = P->color_sampling ? mul(ld3(m->color), albedo_tex(P, h)) : ld3(m->color);
     const v3 F0 = (m->metallic >= 1.0f) ? alb
                   : add(scl(alb, m->metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m->metallic));
     if (m->roughness < 1e-3f) {
@@ -849,8 +855,11 @@ static v3 shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3 in_d, v
     }
     if (sp) {
         if (sp->passes + 1 >= P->max_depth) {
-            if (1.0f - m->glossy - m->refraction > 0.00001f)
-                return mul(trace_light(P, R, h->pos, n), mul(ld3(m->color), albedo_tex(P, h)));
+            /* colour x texture, zero without colour sampling (rayrenderer.cpp:2839-2856) */
+            if (1.0f - m->glossy - m->refraction > 0.00001f) {
+                const v3 l = trace_light(P, R, h->pos, n);
+                return P->color_sampling ? mul(l, mul(ld3(m->color), albedo_tex(P, h))) : mk(0, 0, 0);
+            }
             return mk(0, 0, 0);
         }
         pm.passes = sp->passes + 1;
@@ -960,7 +969,7 @@ static v3 trace_eye(const prep_t *P, rng_t *R, v3 o, v3 d) {
         isect_t h = nearest(P, o, d);
         const v3 c = trace_path(P, R, o, d, NULL);
         if (h.hit || !is_zero(c)) return mk(fmaxf(c.x, 0), fmaxf(c.y, 0), fmaxf(c.z, 0));
-        return mk(0.2f, 0.19f, 0.18f);
+        return P->back;
     }
     isect_t h = nearest(P, o, d);
     if (h.hit && obj_of(P, h.prim)->visible) {
@@ -969,8 +978,8 @@ static v3 trace_eye(const prep_t *P, rng_t *R, v3 o, v3 d) {
     }
     const v3 env = sample_environment(P, d);
     if (!is_zero(env)) return env;
-    /* settings.backColor = (1, 0.95, 0.9) * 0.2 (rayrenderer.h:152) */
-    return mk(1.0f * 0.2f, 0.95f * 0.2f, 0.9f * 0.2f);
+    /* settings.backColor, default (1, 0.95, 0.9) * 0.2 (rayrenderer.h:152) */
+    return P->back;
 }
 
 /* ======================================================================== */
@@ -1019,7 +1028,7 @@ static void twin_step(const prep_t *P, rng_t *R, path_t *p) {
         }
         v3 env = sample_environment(P, p->d);
         if (eye) {
-            if (is_zero(env)) env = mk(1.0f * 0.2f, 0.95f * 0.2f, 0.9f * 0.2f);
+            if (is_zero(env)) env = P->back;
         } else {
             env = scl(env, env_miss_weight(P, p->d, p->bsdf_pdf));
         }
@@ -1112,7 +1121,7 @@ static void twin_mix(const prep_t *P, rng_t *R, path_t *p, const isect_t *h, v3 
     p->tp = scl(p->tp, wl);
     const v3 hit = h->pos;
     if (lobe == 0) {
-        const v3 alb = mul(ld3(m->color), albedo_tex(P, h));
+        const v3 alb = P->color_sampling ? mul(ld3(m->color), albedo_tex(P, h)) : mk(1.0f, 1.0f, 1.0f);
         const float r1 = lds1(R), r2 = lds1(R);
         const v3 dir = cosine_dir(n, r1, r2);
         const float pdf = fmaxf(0.0f, dot(dir, n)) * (float)(1.0 / M_PI);
@@ -1127,7 +1136,7 @@ static void twin_mix(const prep_t *P, rng_t *R, path_t *p, const isect_t *h, v3 
         return;
     }
     if (lobe == 1) {
-        const v3 alb = mul(ld3(m->color), albedo_tex(P, h));
+        const v3 alb = P->color_sampling ? mul(ld3(m->color), albedo_tex(P, h)) : ld3(m->color);
         if (m->roughness < 1e-3f) {
             const v3 F0 = (m->metallic >= 1.0f) ? alb
                           : add(scl(alb, m->metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m->metallic));
@@ -1243,7 +1252,7 @@ static void twin_surface(const prep_t *P, rng_t *R, path_t *p, const isect_t *h,
     }
     if (!eye) {
         if (p->passes + 1 >= P->max_depth) {
-            if (1.0f - m->glossy - m->refraction > 0.00001f)
+            if (1.0f - m->glossy - m->refraction > 0.00001f && P->color_sampling)
                 add_L(p, mul(mul(p->beta, mul(ld3(m->color), albedo_tex(P, h))), trace_light(P, R, h->pos, n)));
             p->alive = 0;
             return;
@@ -1349,7 +1358,7 @@ static v3 render_sample(const prep_t *P, const or_settings *S, int mode, int x, 
         int guard = 0;
         while (p.alive && guard++ < 4 * P->max_depth + 64) twin_step(P, &R, &p);
         c = mk(fmaxf(p.L.x, 0.0f), fmaxf(p.L.y, 0.0f), fmaxf(p.L.z, 0.0f));
-        if (gm && gm->active && !eye_hit && is_zero(p.L)) c = mk(1.0f * 0.2f, 0.95f * 0.2f, 0.9f * 0.2f);
+        if (gm && gm->active && !eye_hit && is_zero(p.L)) c = P->back;
     }
     if (S->firefly_clamp > 0.0f) {
         c.x = fminf(c.x, S->firefly_clamp);
@@ -1468,6 +1477,8 @@ static int prepare(prep_t *P, const or_scene_desc *d, const or_settings *S) {
     P->blades = d->cam.aperture_blades;
     P->ap_rot = d->cam.aperture_rotation_deg * (float)(M_PI / 180.0);
     P->max_depth = (int)S->max_depth;
+    P->color_sampling = S->color_sampling != 0;
+    P->back = mk(S->back_color[0], S->back_color[1], S->back_color[2]);
     return 0;
 }
 
@@ -1578,10 +1589,10 @@ void or_aov(const or_scene_desc *desc, const or_settings *S, float *normal_rgba,
             } else {
                 normal_rgba[4 * i] = normal_rgba[4 * i + 1] = normal_rgba[4 * i + 2] = 0.5f;
                 normal_rgba[4 * i + 3] = 0.0f;
-                albedo_rgba[4 * i] = 1.0f * 0.2f;
-                albedo_rgba[4 * i + 1] = 0.95f * 0.2f;
-                albedo_rgba[4 * i + 2] = 0.9f * 0.2f;
-                albedo_rgba[4 * i + 3] = 0.0f;
+                albedo_rgba[4 * i] = S->back_color[0];
+                albedo_rgba[4 * i + 1] = S->back_color[1];
+                albedo_rgba[4 * i + 2] = S->back_color[2];
+                albedo_rgba[4 * i + 3] = S->back_color[3];
                 depth[i] = 0.0f;
             }
         }
@@ -1763,4 +1774,60 @@ int or_render_adaptive(const or_scene_desc *desc, const or_settings *S, int mode
     }
     free(active);
     return 0;
+}
+
+/* ======================================================================== */
+/* probes of single functions (tests only)                                  */
+/* ======================================================================== */
+void or_phase_sample(float g, const float wo[3], int n, const float *u2, float *dirs) {
+    medium_t m;
+    memset(&m, 0, sizeof m);
+    m.g = g;
+    const v3 w = mk(wo[0], wo[1], wo[2]);
+    for (int i = 0; i < n; i++) {
+        const v3 d = sample_phase(&m, w, u2[2 * i], u2[2 * i + 1]);
+        dirs[3 * i] = d.x;
+        dirs[3 * i + 1] = d.y;
+        dirs[3 * i + 2] = d.z;
+    }
+}
+
+void or_phase_pdf(float g, const float wo[3], int n, const float *wi, float *pdf) {
+    medium_t m;
+    memset(&m, 0, sizeof m);
+    m.g = g;
+    const v3 w = mk(wo[0], wo[1], wo[2]);
+    for (int i = 0; i < n; i++) pdf[i] = phase_pdf(&m, w, mk(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]));
+}
+
+void or_env_probe(const or_scene_desc *desc, int n, const float *u2, float *dirs, float *pdfs, float *rad, int nq,
+                  const float *qdirs, float *qpdfs, float *qrad) {
+    or_settings S;
+    memset(&S, 0, sizeof S);
+    S.width = S.height = 1;
+    S.spp = S.max_depth = 1;
+    prep_t P;
+    prepare(&P, desc, &S);
+    for (int i = 0; i < n; i++) {
+        v3 d;
+        float pdf;
+        sample_env_dir(&P, u2[2 * i], u2[2 * i + 1], &d, &pdf);
+        const v3 L = sample_environment(&P, d);
+        dirs[3 * i] = d.x;
+        dirs[3 * i + 1] = d.y;
+        dirs[3 * i + 2] = d.z;
+        pdfs[i] = pdf;
+        rad[3 * i] = L.x;
+        rad[3 * i + 1] = L.y;
+        rad[3 * i + 2] = L.z;
+    }
+    for (int i = 0; i < nq; i++) {
+        const v3 d = mk(qdirs[3 * i], qdirs[3 * i + 1], qdirs[3 * i + 2]);
+        qpdfs[i] = env_dir_pdf(&P, d);
+        const v3 L = sample_environment(&P, d);
+        qrad[3 * i] = L.x;
+        qrad[3 * i + 1] = L.y;
+        qrad[3 * i + 2] = L.z;
+    }
+    release(&P);
 }

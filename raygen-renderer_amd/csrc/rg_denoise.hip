@@ -62,7 +62,7 @@ __global__ __launch_bounds__(256) void k_aov(DevBvh bv, PTScene sc, float view_f
         dep[pix] = 1.0f - fminf(fmaxf(d, 0.0f), 1.0f);
     } else {
         nrm[pix] = make_float4(0.5f, 0.5f, 0.5f, 0.0f);
-        alb[pix] = make_float4(1.0f * 0.2f, 0.95f * 0.2f, 0.9f * 0.2f, 0.0f);
+        alb[pix] = make_float4(sc.back[0], sc.back[1], sc.back[2], sc.back[3]);
         dep[pix] = 0.0f;
     }
 }

@@ -96,6 +96,9 @@ struct PTScene {
     int32_t antialias;
     uint32_t width, height;
     uint32_t seed, job;
+    // RendererSettings::enableColorSampling and backColor (rgba)
+    int32_t color_sampling;
+    float back[4];
 };
 
 // Structure-of-arrays path state, capacity = pixels x samples in flight.

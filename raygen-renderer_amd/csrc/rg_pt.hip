@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kPtBlock) void k_camera(PTScene sc, PTState st, uin
             st.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
             st.tp[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
             st.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            st.meta[i] = make_int4(-1, pack_medium(sc.global_medium, -1), R.dim, (int)R.k);
+            st.meta[i] = make_int4(-1, pack_medium(sc.global_medium, -1, false), R.dim, (int)R.k);
             st.nshadow[i] = 0;
         }
     }
@@ -137,6 +137,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         P.passes = meta.x;
         P.medium = medium_of(meta.y);
         P.chroma = chroma_of(meta.y);
+        P.bg = bg_of(meta.y);
         P.alive = true;
         Shadows S;
         S.n = 0;
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         st.beta[p] = v4(P.beta, P.bsdf_pdf);
         st.tp[p] = v4(P.tp, 0.0f);
         st.L[p] = v4(P.L, P.alive ? 1.0f : 0.0f);
-        st.meta[p] = make_int4(P.passes, pack_medium(P.medium, P.chroma), R.dim, (int)R.k);
+        st.meta[p] = make_int4(P.passes, pack_medium(P.medium, P.chroma, P.bg), R.dim, (int)R.k);
         store_shadows(S, p, st.sray, st.scontrib);
         ns = (uint32_t)S.n;
         st.nshadow[p] = ns;
@@ -189,10 +190,15 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(7))) v
 // Per-pixel sum / sum of squares over the S samples of this generation,
 // in sample order.  A sample's value is its path's radiance plus the
 // unoccluded NEE contributions of its last event, clamped (>= 0, and at
-// clamp_max when that is > 0).
+// clamp_max when that is > 0); back = backColor for an eye ray that crossed
+// the global medium without a hit and gathered nothing.
+struct BackColor {
+    float c[3];
+};
 __global__ __launch_bounds__(kPtBlock) void k_accumulate(PTState st, uint32_t width, uint32_t npix, uint32_t S,
                                                           const uint8_t *__restrict__ tile_mask, float clamp_max,
-                                                          float *__restrict__ sum, float *__restrict__ sumsq) {
+                                                          BackColor back, float *__restrict__ sum,
+                                                          float *__restrict__ sumsq) {
     const uint32_t pix = blockIdx.x * kPtBlock + threadIdx.x;
     if (pix >= npix) return;
     if (tile_mask) {
@@ -201,7 +207,7 @@ __global__ __launch_bounds__(kPtBlock) void k_accumulate(PTState st, uint32_t wi
     }
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
     for (uint32_t s = 0; s < S; s++) {
-        const float4 c = sample_value(st, (uint32_t)((size_t)s * npix + pix), clamp_max);
+        const float4 c = sample_value(st, (uint32_t)((size_t)s * npix + pix), clamp_max, back.c);
         sx = sx + c.x;
         sy = sy + c.y;
         sz = sz + c.z;
@@ -329,7 +335,8 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
         qi ^= 1;
     }
     hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, stream, st,
-                       sc.width, npix, S, d_tile_mask, clamp_max, d_sum, d_sumsq);
+                       sc.width, npix, S, d_tile_mask, clamp_max, BackColor{{sc.back[0], sc.back[1], sc.back[2]}},
+                       d_sum, d_sumsq);
     return chk(hipGetLastError(), "path-tracer launch");
 }
 

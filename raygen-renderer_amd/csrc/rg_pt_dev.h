@@ -631,15 +631,19 @@ struct Path {
     int passes, medium;
     int chroma;  // chromatic dispersion band (BSDFParam::chromaChannel), -1 = none chosen
     bool alive;
+    bool bg;     // eye ray through an active global medium that hit nothing
 };
 
-// The path state's medium word: the medium index (signed, low 16 bits) and
-// the dispersion band + 1 (bits 16-17; 0 = none chosen).
-__device__ __forceinline__ int pack_medium(int medium, int chroma) {
-    return (int)(((uint32_t)medium & 0xFFFFu) | ((uint32_t)(chroma + 1) << 16));
+// The path state's medium word: the medium index (signed, low 16 bits), the
+// dispersion band + 1 (bits 16-17; 0 = none chosen) and the background flag
+// (bit 18: the eye ray crossed an active global medium without a hit, so a
+// sample with no radiance at all reads backColor, rayrenderer.cpp:1263-1273).
+__device__ __forceinline__ int pack_medium(int medium, int chroma, bool bg) {
+    return (int)(((uint32_t)medium & 0xFFFFu) | ((uint32_t)(chroma + 1) << 16) | (bg ? (1u << 18) : 0u));
 }
 __device__ __forceinline__ int medium_of(int w) { return (int)(int16_t)(w & 0xFFFF); }
-__device__ __forceinline__ int chroma_of(int w) { return ((uint32_t)w >> 16) - 1; }
+__device__ __forceinline__ int chroma_of(int w) { return (int)(((uint32_t)w >> 16) & 3u) - 1; }
+__device__ __forceinline__ bool bg_of(int w) { return ((uint32_t)w >> 18) & 1u; }
 
 __device__ __forceinline__ int crossed_medium(const PTScene &sc, const DevObject &m, bool entering, int cur) {
     if (entering) return m.interior_medium >= 0 ? m.interior_medium : cur;
@@ -683,8 +687,9 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
     p.beta = scl(p.beta, W);
     p.tp = scl(p.tp, wl);
     if (lobe == 0) {
-        // DiffuseShader (bsdf.cpp:54-108): albedo = colour x texture
-        const v3 alb = mul(ld3(m.color), tex);
+        // DiffuseShader (bsdf.cpp:54-108): albedo = colour x texture, 1
+        // without colour sampling
+        const v3 alb = sc.color_sampling ? mul(ld3(m.color), tex) : mk(1.0f, 1.0f, 1.0f);
         const float r1 = lds1(R), r2 = lds1(R);
         const v3 dir = cosine_dir(n, r1, r2);
         const float pdf = fmaxf(0.0f, dot(dir, n)) * (float)(1.0 / M_PI);
@@ -708,8 +713,9 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
         return;
     }
     if (lobe == 1) {
-        // GlossyShader (bsdf.cpp:173-374): albedo = colour x texture drives F0
-        const v3 alb = mul(ld3(m.color), tex);
+        // GlossyShader (bsdf.cpp:173-374): albedo = colour x texture drives
+        // F0 (the colour alone without colour sampling)
+        const v3 alb = sc.color_sampling ? mul(ld3(m.color), tex) : ld3(m.color);
         if (m.roughness < 1e-3f) {
             const v3 F0 = (m.metallic >= 1.0f) ? alb
                                                : add(scl(alb, m.metallic), scl(mk(0.04f, 0.04f, 0.04f), 1.0f - m.metallic));
@@ -850,7 +856,9 @@ __device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p,
     }
     if (!eye) {
         if (p.passes + 1 >= sc.max_depth) {
-            if (1.0f - m.glossy - m.refraction > 0.00001f) {
+            // the light term times colour x texture, zero without colour
+            // sampling (rayrenderer.cpp:2839-2856)
+            if (1.0f - m.glossy - m.refraction > 0.00001f && sc.color_sampling) {
                 const AreaSample as = sample_area(sc, R, hit, n);
                 if (as.ok) push_shadow(S, as.o, as.lr, 0.99999f, RG_PRED_AREA,
                                        mul(mul(p.beta, mul(ld3(m.color), tex)), area_value(as)));
@@ -899,7 +907,7 @@ __device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows
         }
         v3 env = sample_environment(sc, p.d);
         if (eye) {
-            if (is_zero(env)) env = mk(1.0f * 0.2f, 0.95f * 0.2f, 0.9f * 0.2f);
+            if (is_zero(env)) env = mk(sc.back[0], sc.back[1], sc.back[2]);
         } else {
             env = scl(env, env_miss_weight(sc, p.d, p.bsdf_pdf));
         }
@@ -907,6 +915,7 @@ __device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows
         p.alive = false;
         return;
     }
+    if (eye && !hit) p.bg = true;
     const float maxT = hit ? h.t : RAY_MAX_DISTANCE;
     const float tf = free_flight(*med, prng(R));
     const bool scattered = tf < maxT;
@@ -1072,9 +1081,12 @@ __device__ __forceinline__ v3 with_last_nee(const PTState &st, uint32_t p) {
 }
 
 // A finished sample's value: radiance clamped at 0 and, when clamp_max > 0,
-// at clamp_max (the reference's firefly clamp, rayrenderer.cpp:985-988).
-__device__ __forceinline__ float4 sample_value(const PTState &st, uint32_t p, float clamp_max) {
-    const v3 L = with_last_nee(st, p);
+// at clamp_max (the reference's firefly clamp, rayrenderer.cpp:985-988).  An
+// eye ray that crossed an active global medium without a hit and gathered no
+// radiance reads backColor (traceEyeRay, rayrenderer.cpp:1263-1273).
+__device__ __forceinline__ float4 sample_value(const PTState &st, uint32_t p, float clamp_max, const float *back) {
+    v3 L = with_last_nee(st, p);
+    if (bg_of(st.meta[p].y) && is_zero(L)) L = mk(back[0], back[1], back[2]);
     float4 c = make_float4(fmaxf(L.x, 0.0f), fmaxf(L.y, 0.0f), fmaxf(L.z, 0.0f), 1.0f);
     if (clamp_max > 0.0f) {
         c.x = fminf(c.x, clamp_max);

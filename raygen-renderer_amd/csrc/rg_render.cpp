@@ -270,6 +270,11 @@ PTScene make_ptscene(const Scene &scn, const rg_settings &st) {
     sc.height = H;
     sc.seed = st.seed;
     sc.job = st.job;
+    sc.color_sampling = st.color_sampling ? 1 : 0;
+    sc.back[0] = st.back_color[0];
+    sc.back[1] = st.back_color[1];
+    sc.back[2] = st.back_color[2];
+    sc.back[3] = st.back_color[3];
     return sc;
 }
 
@@ -328,6 +333,11 @@ void rg_settings_default(rg_settings *s) {
     s->sigma_normal = 128.0f;
     s->sigma_depth = 0.1f;
     s->denoise_intensity = 1.0f;
+    s->color_sampling = 1;
+    s->back_color[0] = 1.0f * 0.2f;
+    s->back_color[1] = 0.95f * 0.2f;
+    s->back_color[2] = 0.9f * 0.2f;
+    s->back_color[3] = 0.0f;
 }
 
 int rg_scene_set_shading(rg_scene *scene, const float *n9, const uint32_t *obj_of_tri, const rg_object *objs,

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kPtBlock) void k_bcamera(BatchArgs a, const DevJobS
         st.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         st.tp[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         st.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        st.meta[i] = make_int4(-1, pack_medium(J.sc.global_medium, -1), R.dim, (int)R.k);
+        st.meta[i] = make_int4(-1, pack_medium(J.sc.global_medium, -1, false), R.dim, (int)R.k);
         st.nshadow[i] = 0;
     });
     st.q[0][i] = i;
@@ -187,6 +187,7 @@ __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk 
     P.passes = meta.x;
     P.medium = medium_of(meta.y);
     P.chroma = chroma_of(meta.y);
+    P.bg = bg_of(meta.y);
     P.alive = true;
     Shadows S;
     S.n = 0;
@@ -196,7 +197,7 @@ __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk 
     st.beta[p] = v4(P.beta, P.bsdf_pdf);
     st.tp[p] = v4(P.tp, 0.0f);
     st.L[p] = v4(P.L, P.alive ? 1.0f : 0.0f);
-    st.meta[p] = make_int4(P.passes, pack_medium(P.medium, P.chroma), R.dim, (int)R.k);
+    st.meta[p] = make_int4(P.passes, pack_medium(P.medium, P.chroma, P.bg), R.dim, (int)R.k);
     store_shadows(S, p, st.sray, st.scontrib);
     st.nshadow[p] = (uint32_t)S.n;
     alive = P.alive && bounce + 1 < C.bounces;
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(kPtBlock) void k_baccumulate(BatchArgs a, const Dev
         const GenChunk &C = a.chunks[c];
         float sx = 0.0f, sy = 0.0f, sz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
         for (uint32_t s = 0; s < C.S; s++) {
-            const float4 v = sample_value(a.st, C.first + s * g.npix + pix, C.clamp);
+            const float4 v = sample_value(a.st, C.first + s * g.npix + pix, C.clamp, J.sc.back);
             sx = sx + v.x;
             sy = sy + v.y;
             sz = sz + v.z;

@@ -14,6 +14,9 @@
 //   -d N               shading system; only 5 (BSDF) is on this path
 //   -enaa B  -endn B  -enad B   antialias / denoise / adaptive (yes|true|1|on)
 //   -dni F  --adaptive-base N  --adaptive-threshold F
+//   -encs B            colour sampling (enableColorSampling, default on)
+//   -cb B              cullBackFace (accepted; shading system 5 never reads it)
+//   -bc R,G,B[,A]      backColor (alpha 1 when only R,G,B are given)
 //   --depth N          trace depth (the reference's MAX_TRACE_DEPTH, 32)
 //   --seed N           Philox key of the sample stream
 #include <chrono>
@@ -120,6 +123,14 @@ int main(int argc, char **argv) {
             else if (!strcmp(a, "-blst") || !strcmp(a, "--bloom-strength")) bloom.strength = (float)std::atof(next());
             else if (!strcmp(a, "-blcv") || !strcmp(a, "--bloom-curve")) bloom.curve = (float)std::atof(next());
             else if (!strcmp(a, "-blrd") || !strcmp(a, "--bloom-radius")) bloom.radius = (float)std::atof(next());
+            else if (!strcmp(a, "-encs") || !strcmp(a, "--enable-color-sampling")) st.color_sampling = parse_bool(next());
+            else if (!strcmp(a, "-cb") || !strcmp(a, "--cullback")) st.cull_back_face = parse_bool(next());
+            else if (!strcmp(a, "-bc") || !strcmp(a, "--backcolor")) {
+                float c[4] = {st.back_color[0], st.back_color[1], st.back_color[2], st.back_color[3]};
+                const int n = std::sscanf(next(), "%f,%f,%f,%f", &c[0], &c[1], &c[2], &c[3]);
+                if (n == 3) c[3] = 1.0f;
+                std::memcpy(st.back_color, c, sizeof c);
+            }
             else if (!strcmp(a, "--depth")) st.max_depth = (uint32_t)std::atoi(next());
             else if (!strcmp(a, "--seed")) st.seed = (uint32_t)std::strtoul(next(), nullptr, 0);
             else if (!strcmp(a, "-h") || !strcmp(a, "--help")) {

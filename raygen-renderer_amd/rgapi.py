@@ -288,7 +288,9 @@ class Settings(ctypes.Structure):
                 ("adaptive_base", ctypes.c_uint32), ("adaptive_threshold", ctypes.c_float),
                 ("denoise", ctypes.c_int32), ("denoise_levels", ctypes.c_uint32), ("sigma_color", ctypes.c_float),
                 ("sigma_normal", ctypes.c_float), ("sigma_depth", ctypes.c_float),
-                ("denoise_intensity", ctypes.c_float), ("samples_in_flight", ctypes.c_uint32)]
+                ("denoise_intensity", ctypes.c_float), ("samples_in_flight", ctypes.c_uint32),
+                ("color_sampling", ctypes.c_int32), ("cull_back_face", ctypes.c_int32),
+                ("back_color", ctypes.c_float * 4)]
 
 
 def default_settings(**kw):

@@ -275,3 +275,73 @@ def ico_ggx(level=8, amp=0.05, seed=0x1C05EED, env_value=0.3):
     return b.finish(camera={"fov_deg": 45.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0,
                             "view_far": 50.0},
                     media=med, env=constant_env(env_value))
+
+
+def sky_env(w=64, h=32, sun=(0.62, 0.22), sun_value=60.0, zenith=(0.35, 0.5, 0.9), horizon=(0.9, 0.85, 0.8),
+            ground=(0.12, 0.1, 0.08), rotation_deg=0.0):
+    """A non-constant equirectangular sky: a zenith-to-horizon gradient, a
+    dark ground below the horizon and a small bright sun disc at
+    (u, v) = sun, row 0 at the top."""
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    v = (y + 0.5) / h
+    t = np.clip(v / 0.5, 0.0, 1.0)[..., None]
+    up = (1 - t) * np.asarray(zenith, np.float32) + t * np.asarray(horizon, np.float32)
+    img = np.where((v < 0.5)[..., None], up, np.asarray(ground, np.float32))
+    du = (x + 0.5) / w - sun[0]
+    dv = v - sun[1]
+    img[(du * du + dv * dv) < (1.6 / w) ** 2] = sun_value
+    return {"rgb": np.ascontiguousarray(img, np.float32), "intensity": 1.0, "rotation_deg": rotation_deg}
+
+
+def materials_extra(stacks=16, slices=24, env=None):
+    """The branches the BSDF frame leaves out: a delta mirror (roughness 0,
+    metallic, and a dielectric one mixed with diffuse), anisotropic GGX of
+    both signs with a brush rotation, a Henyey-Greenstein medium (g = 0.6) in
+    a transparency-1 box, a level floor, one area light and a non-constant
+    sky with a sun (rotated by 40 degrees)."""
+    b = Builder()
+    fl = b.add_object(material(color=(0.7, 0.7, 0.65)))
+    b.quad(fl, (-8, -1.0, 2), (8, -1.0, 2), (8, -1.0, -14), (-8, -1.0, -14), normal=(0, 1, 0))
+    mats = [
+        material(color=(0.95, 0.8, 0.55), glossy=1.0, roughness=0.0, metallic=1.0),
+        material(color=(0.3, 0.5, 0.8), glossy=0.6, roughness=0.0),
+        material(color=(0.9, 0.9, 0.9), glossy=1.0, roughness=0.5, metallic=1.0, anisotropy=0.8,
+                 aniso_rotation=30.0),
+        material(color=(0.8, 0.4, 0.3), glossy=0.7, roughness=0.6, metallic=0.5, anisotropy=-0.6),
+    ]
+    for i, m in enumerate(mats):
+        o = b.add_object(m)
+        b.sphere(o, (-2.4 + 1.6 * i, -0.3, -6.0), 0.65, stacks, slices)
+    box = b.add_object(material(color=(1.0, 1.0, 1.0), transparency=1.0, interior_medium=0))
+    b.box(box, (-1.2, -0.99, -4.2), (0.4, 0.4, -3.0))
+    light = b.add_object(material(color=(1.0, 0.95, 0.9), emission=10.0))
+    b.quad(light, (-0.8, 3.0, -6.5), (0.8, 3.0, -6.5), (0.8, 3.0, -5.0), (-0.8, 3.0, -5.0), normal=(0, -1, 0))
+    med = np.zeros(1, MEDIUM_DTYPE)
+    med[0]["sigma_a"] = (0.05, 0.1, 0.2)
+    med[0]["sigma_s"] = (0.6, 0.5, 0.4)
+    med[0]["g"] = 0.6
+    med[0]["density"] = 1.0
+    return b.finish(camera={"fov_deg": 45.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0,
+                            "view_far": 50.0},
+                    media=med, env=env if env is not None else sky_env(rotation_deg=40.0))
+
+
+def fog_room(scatter=0.15, absorb=0.05, g=0.3):
+    """A scene inside a global medium without an environment: a floor, a
+    box, an emissive quad and open space around them, so many eye rays cross
+    the fog without meeting anything (the backColor case of traceEyeRay,
+    rayrenderer.cpp:1263-1273)."""
+    b = Builder()
+    fl = b.add_object(material(color=(0.6, 0.6, 0.6)))
+    b.quad(fl, (-2, -1.0, -2), (2, -1.0, -2), (2, -1.0, -6), (-2, -1.0, -6), normal=(0, 1, 0))
+    bx = b.add_object(material(color=(0.7, 0.3, 0.2), glossy=0.3, roughness=0.4))
+    b.box(bx, (-0.9, -1.0, -4.6), (-0.2, -0.2, -3.8))
+    light = b.add_object(material(color=(1.0, 1.0, 1.0), emission=6.0))
+    b.quad(light, (0.3, 0.2, -4.6), (1.1, 0.2, -4.6), (1.1, 1.0, -4.6), (0.3, 1.0, -4.6), normal=(0, 0, 1))
+    med = np.zeros(1, MEDIUM_DTYPE)
+    med[0]["sigma_a"] = absorb
+    med[0]["sigma_s"] = scatter
+    med[0]["g"] = g
+    med[0]["density"] = 1.0
+    return b.finish(camera={"fov_deg": 60.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0},
+                    media=med, global_medium=0)
