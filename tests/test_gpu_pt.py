@@ -194,21 +194,28 @@ def test_loaded_json_scene_matches_single_lobe_restatement(ctx):
     assert want.mean() > 0.01
 
 
-@pytest.mark.parametrize("name,spp,depth", [("cubeRoom/cubeRoom.json", 32, 6),
-                                            ("sphereArray/sphereArray.json", 16, 6),
-                                            ("mediumTest/waterCube.json", 16, 8),
-                                            ("bokehTest/bokehTest.json", 16, 6),
-                                            ("iblTest/iblTest.json", 16, 6),
-                                            ("suzanne/suzanne.json", 16, 8)])
-def test_reference_scene_files_match_single_lobe_restatement(ctx, name, spp, depth):
+# name, spp, depth, largest fraction of pixels off by more than 1e-3: cubeRoom's
+# big cube stands on the floor with coplanar faces, where a last-ulp
+# difference of a transcendental function picks the other face; every other
+# scene is held to the materials scene's bar
+REF_SCENES = [("cubeRoom/cubeRoom.json", 32, 6, 0.10),
+              ("sphereArray/sphereArray.json", 16, 6, 0.01),
+              ("mediumTest/waterCube.json", 16, 8, 0.01),
+              ("mediumTest/cloudHomo.json", 16, 8, 0.01),
+              ("anisoTest/anisoTest.json", 16, 6, 0.01),
+              ("bokehTest/bokehTest.json", 16, 6, 0.01),
+              ("iblTest/iblTest.json", 16, 6, 0.01),
+              ("suzanne/suzanne.json", 16, 8, 0.01)]
+
+
+@pytest.mark.parametrize("name,spp,depth,bar", REF_SCENES)
+def test_reference_scene_files_match_single_lobe_restatement(ctx, name, spp, depth, bar):
     """The reference's own sample scenes (loaded by rg_scene_load_json, with
     the checkerboard and wood floor textures; environment maps absent, so
     iblTest and suzanne see only the background colour and their own
-    surfaces) rendered on the GPU against the
-    oracle twin on the loader's flattened arrays.  cubeRoom's big cube sits
-    on the floor with coplanar faces, where last-ulp differences of
-    transcendental functions pick the other face; so the bar is per-pixel
-    agreement on most pixels plus the image mean."""
+    surfaces) rendered on the GPU against the oracle twin on the loader's
+    flattened arrays: anisoTest runs the anisotropic GGX lobes, cloudHomo a
+    Henyey-Greenstein medium with g = 0.6."""
     import os
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_scenes", name)
     sc, d = rgapi.load_json(ctx, path)
@@ -219,7 +226,7 @@ def test_reference_scene_files_match_single_lobe_restatement(ctx, name, spp, dep
     want, _ = oracle.render(d, W, H, spp, depth, mode=1, threads=8)
     r = rel_diff(out["sum"], want)
     say(f"[pt]   frac > 1e-3: {np.mean(r > 1e-3):.4f}, mean ratio {out['sum'].mean() / want.mean():.5f}")
-    assert np.mean(r > 1e-3) < 0.10
+    assert np.mean(r > 1e-3) < bar
     assert np.median(r) < 1e-5
     assert abs(out["sum"].mean() / want.mean() - 1) < 5e-3
     assert want.mean() > 1e-3

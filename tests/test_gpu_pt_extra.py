@@ -40,13 +40,34 @@ def rel_diff(a, b, floor=1e-3):
     return np.abs(a - b) / np.maximum(np.abs(b), floor)
 
 
+# Both renderers sum a pixel's samples in binary32, in sample order, as the
+# reference does; over 4096 samples of a constant value (a pixel that sees
+# only sky) that sum is off by up to ~2.4e-4 relative, far above the Monte
+# Carlo error of such a pixel.  The estimates are therefore built from
+# 4096-sample chunks added in float64, and the z-test allows that rounding.
+CHUNK = 4096
+SUM_ROUNDING = 3e-4
+
+
+def chunked(render_chunk, n, seed0):
+    s = q = 0.0
+    for k in range(n // CHUNK):
+        a, b = render_chunk(seed0 + k)
+        s = s + a.astype(np.float64)
+        q = q + b.astype(np.float64)
+    return s, q
+
+
 def converged(g_sum, g_sq, n_g, c_sum, c_sq, n_c):
     mg, mc = g_sum / n_g, c_sum / n_c
     vg = np.maximum(g_sq / n_g - mg * mg, 0) / n_g
     vc = np.maximum(c_sq / n_c - mc * mc, 0) / n_c
-    z = (mg - mc) / np.sqrt(vg + vc + 1e-12)
+    z = (mg - mc) / np.sqrt(vg + vc + (SUM_ROUNDING * mg) ** 2 + (SUM_ROUNDING * mc) ** 2 + 1e-12)
     rmse = np.sqrt(np.mean((mg - mc) ** 2)) / mc.mean()
-    return np.mean(np.abs(z) > 3), rmse
+    # E[(mg - mc)^2] = bias^2 + vg + vc: the systematic part of the RMSE with
+    # the two estimates' own Monte Carlo variance taken out
+    rmse_sys = np.sqrt(max(np.mean((mg - mc) ** 2 - vg - vc), 0.0)) / mc.mean()
+    return np.mean(np.abs(z) > 3), rmse, rmse_sys
 
 
 def test_extra_materials_match_single_lobe_restatement(ctx):
@@ -80,15 +101,21 @@ def test_converges_to_recursive_restatement(ctx, which):
         depth = 8
     rs = rgapi.RenderScene(ctx, sc)
     W, H = 16, 12
-    n_gpu, n_cpu = 65536, 16384
+    n_gpu, n_cpu = 131072, 32768
     say(f"[pt+] convergence {which} {W}x{H}: gpu {n_gpu} spp vs recursive {n_cpu} spp")
-    g = rs.render(rgapi.default_settings(width=W, height=H, spp=n_gpu, max_depth=depth, seed=101,
-                                         firefly_clamp=0.0))
-    c, cq = oracle.render(sc, W, H, n_cpu, depth, mode=0, threads=8, seed=202, firefly_clamp=0.0)
-    frac, rmse = converged(g["sum"], g["sumsq"], n_gpu, c, cq, n_cpu)
-    say(f"[pt+]   |z| > 3 on {frac:.4f} of pixel channels, rel RMSE {rmse:.4f}")
+    def gpu_chunk(seed):
+        o = rs.render(rgapi.default_settings(width=W, height=H, spp=CHUNK, max_depth=depth, seed=seed,
+                                             firefly_clamp=0.0))
+        return o["sum"], o["sumsq"]
+
+    g, gq = chunked(gpu_chunk, n_gpu, 0x100)
+    c, cq = chunked(lambda seed: oracle.render(sc, W, H, CHUNK, depth, mode=0, threads=8, seed=seed,
+                                               firefly_clamp=0.0), n_cpu, 0x200)
+    frac, rmse, rmse_sys = converged(g, gq, n_gpu, c, cq, n_cpu)
+    say(f"[pt+]   |z| > 3 on {frac:.4f} of pixel channels, rel RMSE {rmse:.4f} (noise removed {rmse_sys:.4f})")
     assert frac <= 0.02
-    assert rmse <= 0.01
+    assert rmse_sys <= 0.01
+    assert rmse <= 0.03
 
 
 def test_medium_furnace_closed_form_on_the_gpu(ctx):
