@@ -292,6 +292,15 @@ int rg_postprocess(rg_ctx *ctx, uint32_t width, uint32_t height, float *hdr_rgba
 int rg_postprocess_device(rg_ctx *ctx, uint32_t width, uint32_t height, float *d_hdr_rgba, const rg_bloom *bloom,
                           float *d_ldr_rgba);
 
+/* Image writers of the render CLI's output stage (src/main.cpp:400-410,
+ * 479-485, saveImage): the format follows the extension of path.  ".hdr"
+ * writes Radiance RGBE of the linear HDR image; ".jpg"/".jpeg" (baseline
+ * JPEG, 4:4:4, quality 1-100, <= 0 = 90), ".png" (8-bit RGB), ".ppm" and
+ * ".bmp" write an LDR image (RGBA floats in [0, 1], e.g. rg_postprocess's
+ * ldr_rgba) as bytes round(v * 255).  RG_E_UNSUPPORTED for another
+ * extension, RG_E_IO when the file cannot be written. */
+int rg_image_write(const char *path, uint32_t width, uint32_t height, const float *rgba, int quality);
+
 /* ---- scene ingest ------------------------------------------------------- */
 /* Replaces RendererSceneLoader::load + RayRenderer::transformScene
  * (sceneloader.cpp:847-882, 505-783; rayrenderer.cpp:157-363): reads the

@@ -2,9 +2,10 @@
 // (src/main.cpp:162-490): `rgx render <scene.json> [options]`.
 //
 //   -o FILE            output; .hdr = linear HDR (RGBE, after bloom when
-//                      enabled), otherwise the tonemapped image as a binary
-//                      PPM (rg_postprocess: Reinhard on luminance + 1/2.2
-//                      gamma, RayRenderer::applyTonemapGamma)
+//                      enabled), .jpg/.png/.bmp/.ppm = the tonemapped image
+//                      (rg_postprocess: Reinhard on luminance + 1/2.2 gamma,
+//                      RayRenderer::applyTonemapGamma); default
+//                      <scene dir>/<scene name>.jpg (main.cpp:402-410)
 //   -enpp B            HDR bloom (enableRenderingPostProcess)
 //   -blth F -blst F -blcv F -blrd F   bloom threshold / strength / curve /
 //                      radius (--bloom-threshold, ...), main.cpp:225-262
@@ -33,52 +34,9 @@ static bool parse_bool(const char *s) {
     return !strcasecmp(s, "yes") || !strcasecmp(s, "true") || !strcmp(s, "1") || !strcasecmp(s, "on");
 }
 
-// Radiance RGBE writer (flat scanlines)
-static bool write_hdr(const char *path, const float *rgba, int w, int h) {
-    FILE *f = std::fopen(path, "wb");
-    if (!f) return false;
-    std::fprintf(f, "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n", h, w);
-    std::vector<unsigned char> row((size_t)w * 4);
-    for (int y = 0; y < h; y++) {
-        for (int x = 0; x < w; x++) {
-            const float *c = rgba + 4 * ((size_t)y * w + x);
-            const float v = std::fmax(c[0], std::fmax(c[1], c[2]));
-            unsigned char *o = &row[4 * (size_t)x];
-            if (v < 1e-32f) {
-                o[0] = o[1] = o[2] = o[3] = 0;
-            } else {
-                int e;
-                const float m = std::frexp(v, &e) * 256.0f / v;
-                o[0] = (unsigned char)(c[0] * m);
-                o[1] = (unsigned char)(c[1] * m);
-                o[2] = (unsigned char)(c[2] * m);
-                o[3] = (unsigned char)(e + 128);
-            }
-        }
-        std::fwrite(row.data(), 1, row.size(), f);
-    }
-    return std::fclose(f) == 0;
-}
-
-// LDR image (RGBA floats in [0, 1]) as an 8-bit binary PPM
-static bool write_ppm(const char *path, const float *ldr, int w, int h) {
-    FILE *f = std::fopen(path, "wb");
-    if (!f) return false;
-    std::fprintf(f, "P6\n%d %d\n255\n", w, h);
-    for (size_t i = 0; i < (size_t)w * h; i++) {
-        unsigned char px[3];
-        for (int k = 0; k < 3; k++) {
-            const float v = std::fmin(std::fmax(ldr[4 * i + k], 0.0f), 1.0f);
-            px[k] = (unsigned char)(v * 255.0f + 0.5f);
-        }
-        std::fwrite(px, 1, 3, f);
-    }
-    return std::fclose(f) == 0;
-}
-
 int main(int argc, char **argv) {
     if (argc < 2) {
-        std::printf("usage: rgx render <scene.json> [-o out.hdr|out.ppm] [-r WxH] [-s N] [-d 5] [-enaa B] ...\n");
+        std::printf("usage: rgx render <scene.json> [-o out.jpg|.png|.bmp|.ppm|.hdr] [-r WxH] [-s N] [-d 5] [-enaa B] ...\n");
         return 0;
     }
     rg_settings st;
@@ -87,7 +45,7 @@ int main(int argc, char **argv) {
     rg_bloom_default(&bloom);
     bool post = false;
     int shader = 5;
-    std::string cmd, scene, out = "out.hdr";
+    std::string cmd, scene, out;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
         auto next = [&]() -> const char * {
@@ -153,6 +111,12 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "only the render command is on this path\n");
         return 1;
     }
+    if (out.empty()) {  // <scene dir>/<scene base name>.jpg
+        const size_t slash = scene.find_last_of('/');
+        const size_t dot = scene.find_last_of('.');
+        const size_t base = slash == std::string::npos ? 0 : slash + 1;
+        out = scene.substr(0, (dot != std::string::npos && dot > base) ? dot : scene.size()) + ".jpg";
+    }
     if (shader != 5) {
         std::fprintf(stderr, "shading system %d is outside this path (only -d 5)\n", shader);
         return 1;
@@ -200,10 +164,8 @@ int main(int argc, char **argv) {
         return 1;
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    const bool ok = is_hdr ? write_hdr(out.c_str(), hdr.data(), (int)st.width, (int)st.height)
-                           : write_ppm(out.c_str(), ldr.data(), (int)st.width, (int)st.height);
-    if (!ok) {
-        std::fprintf(stderr, "cannot write %s\n", out.c_str());
+    if (rg_image_write(out.c_str(), st.width, st.height, is_hdr ? hdr.data() : ldr.data(), 0) != RG_OK) {
+        std::fprintf(stderr, "%s\n", rg_last_error());
         return 1;
     }
     std::printf("done. (%.3f s) -> %s\n", secs, out.c_str());

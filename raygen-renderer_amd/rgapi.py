@@ -100,6 +100,7 @@ def lib():
             "rg_scene_export_textures": (i32, [vp, vp, vp, vp, vp, vp, vp]),
             "rg_postprocess": (i32, [vp, u32, u32, vp, vp, vp]),
             "rg_postprocess_device": (i32, [vp, u32, u32, vp, vp, vp]),
+            "rg_image_write": (i32, [ctypes.c_char_p, u32, u32, vp, i32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -564,3 +565,13 @@ def postprocess_device(ctx, width, height, d_hdr, bloom=None, d_ldr=None):
     context's stream."""
     check(lib().rg_postprocess_device(ctx.h, width, height, d_hdr,
                                       ctypes.byref(bloom) if bloom is not None else None, d_ldr))
+
+
+def image_write(path, rgba, quality=0):
+    """rg_image_write: the format follows the extension (.hdr RGBE of a
+    linear image; .jpg/.png/.ppm/.bmp of an LDR image in [0, 1])."""
+    a = np.ascontiguousarray(rgba, np.float32)
+    h, w = a.shape[:2]
+    if a.shape[2] != 4:
+        raise ValueError("rgba image expected")
+    check(lib().rg_image_write(str(path).encode(), w, h, _ptr(a), quality))
