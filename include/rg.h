@@ -301,6 +301,20 @@ int rg_postprocess_device(rg_ctx *ctx, uint32_t width, uint32_t height, float *d
  * extension, RG_E_IO when the file cannot be written. */
 int rg_image_write(const char *path, uint32_t width, uint32_t height, const float *rgba, int quality);
 
+/* rg_postprocess_device without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as rg_postprocess_device (same kernels, the
+ * bloomed HDR and LDR images are bit-identical), but the two glow buffers
+ * live in a fixed per-GPU scratch array of 2 x 1,048,576 glow pixels
+ * (32 MiB, held until the process ends) and the Gaussian taps are passed as
+ * kernel arguments (at most 127 per side).  A glow buffer larger than the
+ * scratch (image size x size_aspect, per axis) or a wider blur fails with
+ * RG_E_BUDGET.  Calls that overlap on one GPU (two contexts or streams at
+ * once) share the scratch and race; calls queued on one stream do not. */
+int rg_postprocess_device_noalloc(rg_ctx *ctx, uint32_t width, uint32_t height, float *d_hdr_rgba,
+                                  const rg_bloom *bloom, float *d_ldr_rgba);
+
 /* ---- scene ingest ------------------------------------------------------- */
 /* Replaces RendererSceneLoader::load + RayRenderer::transformScene
  * (sceneloader.cpp:847-882, 505-783; rayrenderer.cpp:157-363): reads the

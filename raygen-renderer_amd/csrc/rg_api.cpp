@@ -46,6 +46,7 @@ void release_pt(Ctx *c);
 void release_engine(Ctx *c);
 void release_pt_scene(Scene *s);
 void release_post(Ctx *c);
+int post_scratch_init(int device);
 }  // namespace rg
 
 extern "C" {
@@ -64,6 +65,9 @@ int rg_create(int device, rg_ctx **out) {
         rc = dev_stream_create(&c->own_stream);
         if (rc) { delete c; return rc; }
         c->stream = c->own_stream;
+        // the no-allocation entry point's scratch address, once per device;
+        // a failure is reported by that entry point, not here
+        (void)post_scratch_init(device);
     }
     *out = reinterpret_cast<rg_ctx *>(c);
     return RG_OK;

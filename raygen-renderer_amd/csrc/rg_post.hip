@@ -21,8 +21,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -77,11 +79,20 @@ __global__ __launch_bounds__(kBlk) void k_glow_down(const float4 *__restrict__ h
     glow[i] = o;
 }
 
+// Gaussian taps handed to the blur by value (kernel arguments), for the
+// entry point that keeps no device buffer of its own.
+constexpr int kArgTapR = 127;
+struct TapArg {
+    float w[2 * kArgTapR + 1];
+    __device__ __forceinline__ float operator[](int i) const { return w[i]; }
+};
+
 // One pass of separableGaussianBlur (rayrenderer.cpp:425-477): taps
-// -r..r in order, clamp to the edge.
-template <int kAxis>
+// -r..r in order, clamp to the edge.  Wt: a device pointer to the taps or
+// a TapArg.
+template <int kAxis, class Wt>
 __global__ __launch_bounds__(kBlk) void k_blur(const float4 *__restrict__ src, float4 *__restrict__ dst, int w, int h,
-                                                const float *__restrict__ wt, int r) {
+                                                Wt wt, int r) {
     const int i = blockIdx.x * kBlk + threadIdx.x;
     if (i >= w * h) return;
     const int x = i % w, y = i / w;
@@ -182,23 +193,55 @@ int grow(void *&p, uint64_t &cap, uint64_t bytes) {
 
 // Taps of separableGaussianBlur (rayrenderer.cpp:429-441): r = max(1,
 // ceil(3 sigma)), w_i = exp(-x^2 / (2 sigma^2)) normalised by its sum.
-int gauss_taps(float sigma, std::vector<float> &w) {
-    const int r = std::max(1, (int)ceilf(sigma * 3.0f));
-    w.assign((size_t)(2 * r + 1), 0.0f);
+int gauss_radius(float sigma) { return std::max(1, (int)ceilf(sigma * 3.0f)); }
+void gauss_fill(float sigma, int r, float *w) {
     float sum = 0.0f;
     const float two_s2 = 2.0f * sigma * sigma;
     for (int i = 0; i < 2 * r + 1; ++i) {
         const float x = (float)(i - r);
         const float v = expf(-(x * x) / two_s2);
-        w[(size_t)i] = v;
+        w[i] = v;
         sum += v;
     }
     const float inv = 1.0f / sum;
-    for (float &v : w) v *= inv;
+    for (int i = 0; i < 2 * r + 1; ++i) w[i] *= inv;
+}
+int gauss_taps(float sigma, std::vector<float> &w) {
+    const int r = gauss_radius(sigma);
+    w.assign((size_t)(2 * r + 1), 0.0f);
+    gauss_fill(sigma, r, w.data());
     return r;
 }
 
+// Scratch of rg_postprocess_device_noalloc: its two glow buffers, at most
+// kScratchGlowPix glow pixels each (2 x 16 MiB), one copy per GPU, held
+// until the process ends.  Host code uses only the per-device addresses
+// that post_scratch_init takes from hipGetSymbolAddress.
+constexpr uint32_t kScratchGlowPix = 1u << 20;
+constexpr int kMaxScratchDevices = 64;
+__device__ __align__(256) float4 scratch[2 * (size_t)kScratchGlowPix];
+std::mutex g_scratch_mu;
+std::atomic<float4 *> g_scratch_ptr[kMaxScratchDevices];
+std::string g_scratch_err[kMaxScratchDevices];
+
 }  // namespace
+
+// Once per device, with that device current (rg_create): the device
+// address of the no-allocation entry point's scratch array.
+int post_scratch_init(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return RG_OK;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device].load()) return RG_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return set_error(RG_E_HIP, g_scratch_err[device]);
+    }
+    g_scratch_ptr[device].store((float4 *)p);
+    return RG_OK;
+}
 
 void release_post(Ctx *c) {
     PostBuffers &p = c->post;
@@ -254,8 +297,63 @@ int post_device(Ctx *c, uint32_t W, uint32_t H, float4 *d_hdr, const rg_bloom *b
             const int r = P.taps_r;
             const float *wt = (const float *)P.taps;
             for (int pass = 0; pass < 2; pass++) {
-                hipLaunchKernelGGL(k_blur<0>, dim3(gblocks), dim3(kBlk), 0, s, ga, gb, gw, gh, wt, r);
-                hipLaunchKernelGGL(k_blur<1>, dim3(gblocks), dim3(kBlk), 0, s, gb, ga, gw, gh, wt, r);
+                hipLaunchKernelGGL((k_blur<0, const float *>), dim3(gblocks), dim3(kBlk), 0, s, ga, gb, gw, gh, wt, r);
+                hipLaunchKernelGGL((k_blur<1, const float *>), dim3(gblocks), dim3(kBlk), 0, s, gb, ga, gw, gh, wt, r);
+            }
+        }
+        glow = ga;
+        rx = (float)gw / (float)w;
+        ry = (float)gh / (float)h;
+    }
+    if (!bloom && !d_ldr) return RG_OK;
+    hipLaunchKernelGGL(k_compose, dim3((npix + kBlk - 1) / kBlk), dim3(kBlk), 0, s, d_hdr, w, h, glow, gw, gh, rx, ry,
+                       bloom ? bloom->strength : 0.0f, bloom ? 1 : 0, d_ldr);
+    return hip_chk(hipGetLastError(), "post-process launch");
+}
+
+// The output stage with no allocation: the glow buffers live in the
+// per-device scratch array, the Gaussian taps travel as kernel arguments.
+int post_device_noalloc(Ctx *c, uint32_t W, uint32_t H, float4 *d_hdr, const rg_bloom *bloom, float4 *d_ldr) {
+    hipStream_t s = (hipStream_t)c->stream;
+    const int w = (int)W, h = (int)H;
+    const uint32_t npix = W * H;
+    int gw = 1, gh = 1;
+    float rx = 1.0f, ry = 1.0f;
+    const float4 *glow = nullptr;
+    if (bloom) {
+        float4 *base = (c->device >= 0 && c->device < kMaxScratchDevices) ? g_scratch_ptr[c->device].load() : nullptr;
+        if (!base)
+            return set_error(RG_E_HIP, "no-allocation scratch unavailable on this device" +
+                                           (c->device >= 0 && c->device < kMaxScratchDevices
+                                                ? ": " + g_scratch_err[c->device] : std::string()));
+        const float threshold = fmaxf(bloom->threshold, 0.0f);
+        const float curve = fmaxf(bloom->curve, 1e-3f);
+        const float aspect = fmaxf(0.02f, bloom->size_aspect);
+        gw = std::max(1, (int)((float)w * aspect));
+        gh = std::max(1, (int)((float)h * aspect));
+        if ((uint64_t)gw * gh > kScratchGlowPix)
+            return set_error(RG_E_BUDGET, "bloom glow buffer of " + std::to_string(gw) + " x " + std::to_string(gh) +
+                                              " pixels exceeds the no-allocation scratch (1,048,576 pixels)");
+        const float sigma_down = fmaxf(0.0f, bloom->radius) * (float)w * aspect;
+        const float sigma = sigma_down * 0.70710678f;
+        const bool blur = sigma_down >= 0.5f && sigma >= 0.25f;
+        TapArg taps;
+        int r = 0;
+        if (blur) {
+            r = gauss_radius(sigma);
+            if (r > kArgTapR)
+                return set_error(RG_E_BUDGET, "bloom blur radius " + std::to_string(r) +
+                                                  " exceeds the no-allocation limit of 127 taps per side");
+            gauss_fill(sigma, r, taps.w);
+        }
+        float4 *ga = base, *gb = base + kScratchGlowPix;
+        const uint32_t gblocks = (uint32_t)(((uint64_t)gw * gh + kBlk - 1) / kBlk);
+        // every glow pixel is written before it is read: nothing to clear
+        hipLaunchKernelGGL(k_glow_down, dim3(gblocks), dim3(kBlk), 0, s, d_hdr, w, h, gw, gh, threshold, curve, ga);
+        if (blur) {
+            for (int pass = 0; pass < 2; pass++) {
+                hipLaunchKernelGGL((k_blur<0, TapArg>), dim3(gblocks), dim3(kBlk), 0, s, ga, gb, gw, gh, taps, r);
+                hipLaunchKernelGGL((k_blur<1, TapArg>), dim3(gblocks), dim3(kBlk), 0, s, gb, ga, gw, gh, taps, r);
             }
         }
         glow = ga;
@@ -298,6 +396,13 @@ int rg_postprocess_device(rg_ctx *ctx, uint32_t width, uint32_t height, float *d
     if (int rc = post_args(ctx, width, height, d_hdr_rgba)) return rc;
     return post_device(reinterpret_cast<Ctx *>(ctx), width, height, (float4 *)d_hdr_rgba, bloom,
                        (float4 *)d_ldr_rgba);
+}
+
+int rg_postprocess_device_noalloc(rg_ctx *ctx, uint32_t width, uint32_t height, float *d_hdr_rgba,
+                                  const rg_bloom *bloom, float *d_ldr_rgba) {
+    if (int rc = post_args(ctx, width, height, d_hdr_rgba)) return rc;
+    return post_device_noalloc(reinterpret_cast<Ctx *>(ctx), width, height, (float4 *)d_hdr_rgba, bloom,
+                               (float4 *)d_ldr_rgba);
 }
 
 int rg_postprocess(rg_ctx *ctx, uint32_t width, uint32_t height, float *hdr_rgba, const rg_bloom *bloom,

@@ -100,6 +100,7 @@ def lib():
             "rg_scene_export_textures": (i32, [vp, vp, vp, vp, vp, vp, vp]),
             "rg_postprocess": (i32, [vp, u32, u32, vp, vp, vp]),
             "rg_postprocess_device": (i32, [vp, u32, u32, vp, vp, vp]),
+            "rg_postprocess_device_noalloc": (i32, [vp, u32, u32, vp, vp, vp]),
             "rg_image_write": (i32, [ctypes.c_char_p, u32, u32, vp, i32]),
         }
         for name, (res, args) in sig.items():
@@ -565,6 +566,13 @@ def postprocess_device(ctx, width, height, d_hdr, bloom=None, d_ldr=None):
     context's stream."""
     check(lib().rg_postprocess_device(ctx.h, width, height, d_hdr,
                                       ctypes.byref(bloom) if bloom is not None else None, d_ldr))
+
+
+def postprocess_device_noalloc(ctx, width, height, d_hdr, bloom=None, d_ldr=None):
+    """rg_postprocess_device_noalloc: as postprocess_device, with the glow
+    buffers in the library's fixed per-GPU scratch array (no allocation)."""
+    check(lib().rg_postprocess_device_noalloc(ctx.h, width, height, d_hdr,
+                                              ctypes.byref(bloom) if bloom is not None else None, d_ldr))
 
 
 def image_write(path, rgba, quality=0):
