@@ -28,6 +28,32 @@ RAY_KEY = 0x5EED0001
 MESH_SEED = 0x1C05EED
 
 
+def host_threads():
+    """Threads for the CPU baselines: the CPUs this process may run on,
+    capped at 16 (the host share of one GPU on the boxes this runs on)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_desc(threads):
+    return {"threads": threads, "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+            "oracle_build": "gcc -O3 -ffp-contract=off -fno-fast-math"}
+
+
 def cpu_baseline(v9, rays_np, seconds, threads):
     """Oracle traversal on a bounded prefix of the same rays.  Also returns
     the mean binary-tree node pops and triangle tests per ray, which fix the
@@ -66,6 +92,8 @@ def cpu_baseline(v9, rays_np, seconds, threads):
                   f"BVH build {build_s:.2f}s excluded",
         "value_1_thread": round(one, 3),
         "sample_1_thread": f"first {n1} rays, the same traversal on 1 thread",
+        "bvh_build_s": round(build_s, 3),
+        "host": host_desc(threads),
     }, pops / done, tests / done
 
 
@@ -96,7 +124,7 @@ def bench_cube_room(ctx, stream, steps, cpu_seconds, do_cpu):
     if do_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_threads()
         spp, t, done = 1, 0.0, 0
         while t < cpu_seconds and done < 16:
             t0 = time.perf_counter()
@@ -106,7 +134,8 @@ def bench_cube_room(ctx, stream, steps, cpu_seconds, do_cpu):
         res["cpu_baseline"] = {"value": round(256 * 256 * done / t, 1), "unit": "samples/s", "cores": threads,
                                "kind": "port",
                                "sample": f"{done} spp of the 256x256 frame, recursive restatement of the "
-                                         f"reference shader (all Mix lobes), {threads} threads"}
+                                         f"reference shader (all Mix lobes), {threads} threads",
+                               "host": host_desc(threads)}
     return res
 
 
@@ -140,20 +169,31 @@ def bench_bsdf_frame(ctx, do_cpu, cpu_seconds):
            "samples": out["samples"],
            "samples_per_s": round(out["samples"] / wall, 1),
            "mrays_per_s": round((out["rays"] + out["shadow_rays"]) / wall / 1e6, 2),
+           "rays": out["rays"], "shadow_rays": out["shadow_rays"],
            "mean_spp": round(float(out["samples"]) / (1920 * 1080), 2),
            "scene_gen_s": round(t1 - t0, 2), "bvh_build_upload_s": round(t2 - t1, 2)}
     res["postprocess"] = bench_postprocess(ctx, out["hdr"], do_cpu)
     if do_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_threads()
+        # the oracle builds its BVH inside every render call: a 1-pixel render
+        # measures that build, which is subtracted and reported apart
         t0 = time.perf_counter()
-        oracle.render(sc, 192, 108, 2, 8, mode=0, threads=threads)
-        t = time.perf_counter() - t0
-        res["cpu_baseline"] = {"value": round(192 * 108 * 2 / t, 1), "unit": "samples/s", "cores": threads,
+        oracle.render(sc, 1, 1, 1, 8, mode=0, threads=1)
+        build = time.perf_counter() - t0
+        W, H, spp, t, done = 384, 216, 1, 0.0, 0
+        while t < cpu_seconds and done < 64:
+            t0 = time.perf_counter()
+            oracle.render(sc, W, H, spp, 8, mode=0, threads=threads, sample_start=done)
+            t += time.perf_counter() - t0 - build
+            done += spp
+        res["cpu_baseline"] = {"value": round(W * H * done / t, 1), "unit": "samples/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"192x108 at 2 spp of the same scene (includes the oracle's own BVH "
-                                         f"build), recursive restatement, {threads} threads"}
+                               "sample": f"{W}x{H} at {done} spp of the same scene (no adaptive pass, no "
+                                         f"denoise), recursive restatement, {threads} threads; the oracle's "
+                                         f"BVH build ({build:.2f}s per call) subtracted",
+                               "bvh_build_s": round(build, 3), "host": host_desc(threads)}
     return res
 
 
@@ -308,11 +348,18 @@ def bench_batch_small(ctx, resident, n_jobs=256, seed=0xC5C50000):
             "images_bit_equal": f"{same}/{n_jobs}"}
 
 
-def bench_serve(ctx, resident, n_req, rates, seed=0xC4C40000, budget=32 << 30, n_plain=24):
+def bench_serve(ctx, resident, n_req, rates, seed=0xC4C40000, budget=32 << 30, slo_s=1.0):
     """Poisson request stream over four resident scenes: served by rg_server
-    (continuous batching, admission by framebuffer footprint) at several
-    arrival rates, with p50/p99 latency, against the plain path one request
-    at a time."""
+    (continuous batching, admission by framebuffer footprint, overload
+    policy: a request that would take the outstanding work past slo_s of the
+    plain path's throughput is refused with RG_E_BUSY) at several arrival
+    rates, with p50/p99 latency of the accepted requests; against the plain
+    path one request at a time, which also renders every request once so each
+    served image is compared with its plain-path image."""
+    import hashlib
+    import queue
+    import threading
+
     import numpy as np
 
     import rgapi
@@ -320,17 +367,16 @@ def bench_serve(ctx, resident, n_req, rates, seed=0xC4C40000, budget=32 << 30, n
     spec = draw_jobs(n_req, len(resident), seed)
     jobs = [rgapi.make_job(s, **kw) for s, kw in spec]
     sam = [kw["width"] * kw["height"] * kw["spp"] for _, kw in spec]
-    # plain path, one at a time, on the first n_plain requests
-    plain_s, plain_hdr = 0.0, {}
-    for i in range(min(n_plain, n_req)):
-        s, kw = spec[i]
+    # plain path, one at a time, on every request; a digest of each image
+    plain_s, digest = 0.0, []
+    for i, (s, kw) in enumerate(spec):
         t0 = time.perf_counter()
         o = resident[s].render(rgapi.default_settings(**kw), hdr_only=True)
         plain_s += time.perf_counter() - t0
-        plain_hdr[i] = o["hdr"]
-    plain_sps = sum(sam[:len(plain_hdr)]) / plain_s
+        digest.append(hashlib.blake2b(o["hdr"].tobytes(), digest_size=16).digest())
+    plain_sps = sum(sam) / plain_s
     lam_plain = plain_sps / float(np.mean(sam))
-    log(f"plain {plain_sps / 1e6:.1f} Msamples/s -> {lam_plain:.2f} req/s")
+    log(f"plain {plain_sps / 1e6:.1f} Msamples/s over {n_req} requests -> {lam_plain:.2f} req/s")
     probe = []
     for s, kw in draw_jobs(8, len(resident), seed ^ 0x9E3779B9):
         kw["spp"] = min(kw["spp"], 64)
@@ -338,50 +384,95 @@ def bench_serve(ctx, resident, n_req, rates, seed=0xC4C40000, budget=32 << 30, n
     t0 = time.perf_counter()
     tune = rgapi.autotune(ctx, resident, probe)
     log("autotune", tune.as_dict(), f"{time.perf_counter() - t0:.1f}s")
+    limit = int(slo_s * plain_sps)
     res = {"workload": f"{n_req} requests per rate, drawn like the batch jobs over 4 resident scenes "
                        f"(cube room, two BSDF frames, GGX icosphere 1,310,736 tris + medium box), "
                        f"Poisson arrivals, HBM budget {budget >> 30} GiB",
-           "plain_samples_per_s": round(plain_sps, 1), "plain_requests": len(plain_hdr),
+           "plain_samples_per_s": round(plain_sps, 1), "plain_requests": n_req,
+           "slo_p99_s": slo_s, "queue_limit_samples": limit,
+           "overload_policy": "refuse (RG_E_BUSY) a request that would take the samples submitted and not done "
+                              "past slo_p99_s x plain_samples_per_s",
            "tune": tune.as_dict(), "rates": []}
     rng = np.random.default_rng(seed ^ 0x5A5A)
     for f in rates:
         lam = f * lam_plain
         gaps = rng.exponential(1.0 / lam, n_req)
         srv = rgapi.Server(ctx, resident, hbm_budget=budget, tune=tune)
-        tickets = []
-        t_start = time.perf_counter()
-        t_next = t_start
+        srv.set_queue_limit(limit)
+        todo = queue.Queue()
+        lat, spans, checked = [], [None, None], {"same": 0, "differ": [], "failed": []}
+
+        def consume():
+            while True:
+                item = todo.get()
+                if item is None:
+                    return
+                i, t = item
+                info, out = srv.wait(t, timeout=600.0)
+                if info is None or info.status != 0:
+                    checked["failed"].append(i)
+                    continue
+                lat.append(info.done_s - info.submit_s)
+                spans[0] = info.submit_s if spans[0] is None else min(spans[0], info.submit_s)
+                spans[1] = info.done_s if spans[1] is None else max(spans[1], info.done_s)
+                if hashlib.blake2b(out["hdr"].tobytes(), digest_size=16).digest() == digest[i]:
+                    checked["same"] += 1
+                else:
+                    checked["differ"].append(i)
+
+        th = threading.Thread(target=consume)
+        th.start()
+        accepted, refused, acc_samples = 0, 0, 0
+        t_next = time.perf_counter()
         for i, j in enumerate(jobs):
             t_next += gaps[i]
             dt = t_next - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
-            tickets.append(srv.submit(j))
-        lat, first, last, rmse = [], None, None, []
-        for i, t in enumerate(tickets):
-            info, out = srv.wait(t, timeout=600.0)
-            if info is None or info.status != 0:
-                raise RuntimeError(f"request {i} failed")
-            lat.append(info.done_s - info.submit_s)
-            first = info.submit_s if first is None else min(first, info.submit_s)
-            last = info.done_s if last is None else max(last, info.done_s)
-            if i in plain_hdr:
-                a, b = out["hdr"][..., :3], plain_hdr[i][..., :3]
-                rmse.append(float(np.sqrt(np.mean((a - b) ** 2)) / max(float(b.mean()), 1e-12)))
+            try:
+                todo.put((i, srv.submit(j)))
+                accepted += 1
+                acc_samples += sam[i]
+            except rgapi.RGError as e:
+                if e.code != -9:
+                    raise
+                refused += 1
+        todo.put(None)
+        th.join()
         st = srv.stats()
         tn = srv.tune().as_dict()
         srv.close()
-        span = last - first
+        if checked["failed"]:
+            raise RuntimeError(f"requests {checked['failed'][:5]} failed")
+        # a served image that is not bit-identical: its relative RMSE against
+        # the plain path's image, rendered again
+        rmse = 0.0
+        for i in checked["differ"]:
+            s_, kw = spec[i]
+            a = resident[s_].render(rgapi.default_settings(**kw), hdr_only=True)["hdr"][..., :3]
+            srv2 = rgapi.Server(ctx, resident, hbm_budget=budget, tune=tune)
+            info, out = srv2.wait(srv2.submit(jobs[i]), timeout=600.0)
+            srv2.close()
+            b = out["hdr"][..., :3]
+            rmse = max(rmse, float(np.sqrt(np.mean((a - b) ** 2)) / max(float(a.mean()), 1e-12)))
+        span = spans[1] - spans[0]
         lat = np.array(lat)
         r = {"rate_factor": f, "arrival_rate_req_s": round(lam, 3),
-             "served_samples_per_s": round(sum(sam) / span, 1),
-             "vs_plain": round(sum(sam) / span / plain_sps, 3),
+             "accepted": accepted, "refused": refused,
+             "served_samples_per_s": round(acc_samples / span, 1),
+             "vs_plain": round(acc_samples / span / plain_sps, 3),
              "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
              "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-             "makespan_s": round(span, 3), "max_rel_rmse_vs_plain": max(rmse) if rmse else None,
+             "makespan_s": round(span, 3),
+             "images_bit_equal_plain": f"{checked['same']}/{accepted}", "max_rel_rmse_vs_plain": rmse,
              "graph_replays": st["graphs"], "tune": tn}
         log("rate", r)
         res["rates"].append(r)
+    # the highest rate served within the SLO with at most 1% of requests refused
+    ok = [r for r in res["rates"] if r["p99_latency_s"] <= slo_s and r["refused"] <= 0.01 * n_req]
+    res["capacity_at_slo"] = ({"rate_factor": max(r["rate_factor"] for r in ok),
+                               "served_samples_per_s": max(r["served_samples_per_s"] for r in ok)}
+                              if ok else None)
     return res
 
 
@@ -393,7 +484,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-render", action="store_true", help="traversal microbenchmark only")
-    ap.add_argument("--serve-requests", type=int, default=60, help="requests per arrival rate")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="requests per arrival rate")
     ap.add_argument("--legs", default="cube,frame,batch,serve",
                     help="comma-separated workloads run after the traversal headline: cube, frame, batch, serve")
     ap.add_argument("--batch-jobs", type=int, default=64)
@@ -464,24 +555,35 @@ def main():
     pops = tests = None
     if not args.no_cpu_baseline:
         rays_np = d_rays.view(n, 8)[: 1 << 22].cpu().numpy().view(rgapi.RAY_DTYPE).reshape(-1)
-        threads = max(1, min(16, os.cpu_count() or 1))
-        cpu, pops, tests = cpu_baseline(v9, rays_np, args.cpu_seconds, threads)
+        cpu, pops, tests = cpu_baseline(v9, rays_np, args.cpu_seconds, host_threads())
 
+    # algorithmic bytes per ray: the reference traversal's node pops and
+    # triangle tests over all 16M rays (tests/golden/c1_bytes_per_ray.json);
+    # the CPU leg's own counts on its sample when the fixture is absent
+    npr_src = None
+    fx = os.path.join(ROOT, "tests", "golden", "c1_bytes_per_ray.json")
+    if os.path.exists(fx):
+        with open(fx) as f:
+            fj = json.load(f)
+        pops, tests, npr_src = fj["node_pops_per_ray"], fj["tri_tests_per_ray"], "tests/golden/c1_bytes_per_ray.json"
+    elif pops is not None:
+        npr_src = "CPU leg sample"
     roof = None
     if pops is not None:
         b_ray = 32 + 16 + 32 * pops + 36 * tests
         achieved = b_ray * n / (avg_ms / 1e3) / 1e9
-        traffic = l2 = None
+        traffic = l2 = limiter = None
         tf = os.path.join(ROOT, "profiles", "c1_closest_traffic.json")
         if os.path.exists(tf):
             with open(tf) as f:
                 tj = json.load(f)
-            traffic, l2 = tj.get("hbm_bytes_per_launch"), tj.get("l2_hit_rate")
+            traffic, l2, limiter = tj.get("hbm_bytes_per_launch"), tj.get("l2_hit_rate"), tj.get("limiter")
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic, "l2_hit_rate": l2,
+                "limiter": limiter,
                 "bytes_per_ray": round(b_ray, 2), "node_pops_per_ray": round(pops, 3),
-                "tri_tests_per_ray": round(tests, 3), "kernel": "k_trace_ps<64, 4, closest>",
-                "kernel_avg_ms": round(avg_ms, 4)}
+                "tri_tests_per_ray": round(tests, 3), "bytes_per_ray_source": npr_src,
+                "kernel": "k_trace_ps<64, 4, closest>", "kernel_avg_ms": round(avg_ms, 4)}
 
     extra = {}
     if args.copy_calibration:

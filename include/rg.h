@@ -29,6 +29,7 @@ extern "C" {
 #define RG_E_UNSUPPORTED -6 /* option outside the built path */
 #define RG_E_BUDGET -7      /* request exceeds the context's memory budget */
 #define RG_E_IO -8          /* file could not be read or parsed */
+#define RG_E_BUSY -9        /* rg_server_submit: refused by the server's queue limit */
 #define RG_TIMEOUT 1        /* rg_server_wait: not finished within the timeout */
 
 typedef struct rg_ctx rg_ctx;
@@ -424,6 +425,15 @@ int rg_server_wait(rg_server *srv, uint64_t ticket, double timeout_s, rg_request
  * [4] generations, [5] framebuffer bytes in use, [6] budget, [7] graph replays */
 int rg_server_stats(rg_server *srv, uint64_t out[8]);
 int rg_server_get_tune(rg_server *srv, rg_tune *out);
+/* Overload policy of the request server (the reference has no server; this
+ * bounds the latency of what it accepts): with a nonzero limit,
+ * rg_server_submit refuses with RG_E_BUSY a request whose samples (W*H*spp)
+ * would take the samples submitted and not yet done past
+ * max_outstanding_samples; a request is always accepted when nothing is
+ * outstanding.  0 (the default) removes the limit. */
+int rg_server_set_queue_limit(rg_server *srv, uint64_t max_outstanding_samples);
+/* [0] samples outstanding, [1] the limit, [2] requests refused so far. */
+int rg_server_queue(rg_server *srv, uint64_t out[3]);
 int rg_server_destroy(rg_server *srv);
 
 #ifdef __cplusplus

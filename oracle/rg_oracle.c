@@ -2,7 +2,7 @@
  * rg_oracle.c — CPU restatement of RayGen's BVH build, ray/triangle tests and
  * BVH traversal (TEST INFRASTRUCTURE: the checker, never the product).
  *
- * Compiled with -O2 -ffp-contract=off and no fast-math: every float operation
+ * Compiled with -O3 -ffp-contract=off and no fast-math: every float operation
  * is evaluated in the order written, in binary32, so results are bit-exact
  * against the HIP kernels built with the same contract setting.
  *

@@ -7,7 +7,11 @@
 // the same in-place partition, so it yields the same nodes in the same order
 // and the same permutation.  It is written as an explicit work stack (DFS,
 // left subtree first), which allocates sibling pairs in the same order as the
-// reference's recursion.
+// reference's recursion.  Large inputs split their top levels on the calling
+// thread and build the subtrees below on worker threads (each owns its
+// primitive range, which the partition only permutes within), then number
+// the nodes exactly as the sequential recursion would: every decision of a
+// subtree depends only on its own range, so the tree is the same.
 //
 // collapse_bvh4 then folds the binary tree into 128-byte 4-wide nodes for the
 // GPU.  Every leaf and every leaf's primitive order is kept, and each node
@@ -19,9 +23,11 @@
 // parent boxes and the slab arithmetic is monotone), so hits, including the
 // winner of exact-t ties, are identical.
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "rg_internal.h"
 
@@ -53,9 +59,307 @@ struct Box {
     }
 };
 
+struct Bins {
+    std::vector<Box> bbox;
+    std::vector<int> bcnt, lcnt, rcnt;
+    std::vector<float> larea, rarea;
+    explicit Bins(uint32_t n) : bbox(n), bcnt(n), lcnt(n), rcnt(n), larea(n), rarea(n) {}
+};
+
+// Per-primitive arrays in the current permutation order: the partition
+// swaps every one of them with perm, so pbox[i] is the box of prim perm[i]
+// and every pass over a node's range reads memory sequentially.
+struct BuildData {
+    uint32_t bins, leaf;
+    std::vector<uint32_t> &perm;
+    std::vector<Box> pbox;
+    std::vector<float> cent[3];
+    std::vector<uint8_t> bin;  // bin of each primitive in the node being split
+    unsigned threads;
+    uint32_t par_min;          // ranges at least this large bin on all threads
+};
+
+constexpr uint32_t kLeafNode = 0xFFFFFFFFu;
+constexpr uint32_t kParallelRange = 1u << 18;  // smallest range that bins on all threads
+constexpr uint32_t kChunk = 1u << 15;
+
+template <class F>
+void parallel_for(uint32_t n, unsigned threads, F &&f);
+
+// Bounds of the primitive boxes and of the centroids over [first, end).
+void range_bounds(const BuildData &D, uint32_t first, uint32_t end, Box &pb, Box &cb) {
+    const Box *__restrict__ bx = D.pbox.data();
+    const float *__restrict__ x = D.cent[0].data();
+    const float *__restrict__ y = D.cent[1].data();
+    const float *__restrict__ z = D.cent[2].data();
+    pb.clear();
+    cb.clear();
+    for (uint32_t i = first; i < end; i++) {
+        pb.grow(bx[i]);
+        const float c[3] = {x[i], y[i], z[i]};
+        cb.grow(c);
+    }
+}
+
+// Bin boxes and counts over [a, e) along the split axis; keeps each
+// primitive's bin for the partition.
+void bin_range(BuildData &D, uint32_t a, uint32_t e, const float *__restrict__ ca, float cmin, float scale,
+               int last_bin, Box *__restrict__ bb, int *__restrict__ bc) {
+    const Box *__restrict__ bx = D.pbox.data();
+    uint8_t *__restrict__ bn = D.bin.data();
+    for (int b = 0; b <= last_bin; b++) { bb[b].clear(); bc[b] = 0; }
+    for (uint32_t i = a; i < e; i++) {
+        int b = (int)((ca[i] - cmin) * scale);
+        if (b < 0) b = 0;
+        if (b > last_bin) b = last_bin;
+        bn[i] = (uint8_t)b;
+        bb[b].grow(bx[i]);
+        bc[b]++;
+    }
+}
+
+// One node of buildRecursive (bvh.cpp:100-231) over [first, first + count):
+// its bounds go to nd; returns the split point (the left child is
+// [first, mid)) with nd.axis set, or kLeafNode when the node is a leaf
+// (nd then holds the leaf's range and count).  Bounds and bin boxes are
+// min/max reductions and bin counts integer sums, so splitting those passes
+// over threads gives the sequential results exactly; the partition runs on
+// one thread in the reference's swap order.
+uint32_t split_node(BuildData &D, Bins &B, uint32_t first, uint32_t count, Node2 &nd) {
+    const float kTrav = 1.0f, kIsect = 1.5f;
+    const uint32_t end = first + count;
+    const uint32_t bins = D.bins;
+    const bool par = D.threads > 1 && count >= D.par_min;
+    const uint32_t nchunks = par ? (count + kChunk - 1) / kChunk : 1;
+    Box pb, cb;
+    if (par) {
+        std::vector<Box> ppb(nchunks), pcb(nchunks);
+        parallel_for(nchunks, D.threads, [&](uint32_t k, unsigned) {
+            range_bounds(D, first + k * kChunk, std::min(end, first + (k + 1) * kChunk), ppb[k], pcb[k]);
+        });
+        pb.clear();
+        cb.clear();
+        for (uint32_t k = 0; k < nchunks; k++) {
+            pb.grow(ppb[k]);
+            cb.grow(pcb[k]);
+        }
+    } else {
+        range_bounds(D, first, end, pb, cb);
+    }
+    for (int k = 0; k < 3; k++) { nd.bmin[k] = pb.lo[k]; nd.bmax[k] = pb.hi[k]; }
+    auto make_leaf = [&]() {
+        nd.first_or_left = first;
+        nd.count = (uint16_t)std::min<uint32_t>(count, 0xFFFFu);
+        nd.axis = 0;
+        return kLeafNode;
+    };
+    if (count <= D.leaf) return make_leaf();
+
+    const float ext[3] = {cb.hi[0] - cb.lo[0], cb.hi[1] - cb.lo[1], cb.hi[2] - cb.lo[2]};
+    int axis = 0;
+    if (ext[1] > ext[0]) axis = 1;
+    if (ext[2] > ((axis == 0) ? ext[0] : ext[1])) axis = 2;
+    const float cmin = cb.lo[axis];
+    const float cext = ext[axis];
+    if (cext < 1e-12f) return make_leaf();
+    const float scale = (float)bins / cext;
+    const int last_bin = (int)bins - 1;
+    const float *ca = D.cent[axis].data();
+    if (par) {
+        std::vector<Box> cbb((size_t)nchunks * bins);
+        std::vector<int> cbc((size_t)nchunks * bins);
+        parallel_for(nchunks, D.threads, [&](uint32_t k, unsigned) {
+            bin_range(D, first + k * kChunk, std::min(end, first + (k + 1) * kChunk), ca, cmin, scale, last_bin,
+                      &cbb[(size_t)k * bins], &cbc[(size_t)k * bins]);
+        });
+        for (uint32_t b = 0; b < bins; b++) { B.bbox[b].clear(); B.bcnt[b] = 0; }
+        for (uint32_t k = 0; k < nchunks; k++)
+            for (uint32_t b = 0; b < bins; b++) {
+                B.bbox[b].grow(cbb[(size_t)k * bins + b]);
+                B.bcnt[b] += cbc[(size_t)k * bins + b];
+            }
+    } else {
+        bin_range(D, first, end, ca, cmin, scale, last_bin, B.bbox.data(), B.bcnt.data());
+    }
+    {
+        Box acc; acc.clear();
+        int c = 0;
+        for (int i = 0; i < last_bin; i++) {
+            if (B.bcnt[i] > 0) acc.grow(B.bbox[i]);
+            c += B.bcnt[i];
+            B.larea[i] = (c > 0) ? acc.area() : 0.0f;
+            B.lcnt[i] = c;
+        }
+        acc.clear();
+        c = 0;
+        for (int i = last_bin; i > 0; i--) {
+            if (B.bcnt[i] > 0) acc.grow(B.bbox[i]);
+            c += B.bcnt[i];
+            B.rarea[i - 1] = (c > 0) ? acc.area() : 0.0f;
+            B.rcnt[i - 1] = c;
+        }
+    }
+    const float parea = pb.area();
+    const float inv_parea = (parea > 0.0f) ? 1.0f / parea : 0.0f;
+    float best = (float)count * kIsect;
+    int best_bin = -1;
+    for (int i = 0; i < last_bin; i++) {
+        if (B.lcnt[i] == 0 || B.rcnt[i] == 0) continue;
+        const float cost = kTrav + kIsect * inv_parea *
+                                       (B.larea[i] * (float)B.lcnt[i] + B.rarea[i] * (float)B.rcnt[i]);
+        if (cost < best) { best = cost; best_bin = i; }
+    }
+    if (best_bin < 0) return make_leaf();
+
+    // in-place partition in the reference's order (bvh.cpp:190-206); the
+    // bins were kept from the binning pass
+    uint32_t mid = first;
+    {
+        uint32_t *__restrict__ pm = D.perm.data();
+        Box *__restrict__ bx = D.pbox.data();
+        float *__restrict__ x = D.cent[0].data();
+        float *__restrict__ y = D.cent[1].data();
+        float *__restrict__ z = D.cent[2].data();
+        uint8_t *__restrict__ bn = D.bin.data();
+        const uint8_t bb = (uint8_t)best_bin;
+        for (uint32_t i = first; i < end; i++) {
+            if (bn[i] <= bb) {
+                std::swap(pm[i], pm[mid]);
+                std::swap(bx[i], bx[mid]);
+                std::swap(x[i], x[mid]);
+                std::swap(y[i], y[mid]);
+                std::swap(z[i], z[mid]);
+                std::swap(bn[i], bn[mid]);
+                mid++;
+            }
+        }
+    }
+    const uint32_t nl = mid - first, nr = count - nl;
+    if (nl == 0 || nr == 0) return make_leaf();
+    nd.count = 0;
+    nd.axis = (uint16_t)axis;
+    return mid;
+}
+
 struct Task {
     uint32_t node, first, count;
 };
+
+// The subtree of [first, first + count) by the reference's recursion, as an
+// explicit work stack (left subtree first, sibling pairs allocated when
+// their parent splits): root at nodes[0], the rest in the order the
+// recursion allocates them.
+void build_local(BuildData &D, Bins &B, uint32_t first, uint32_t count, std::vector<Node2> &nodes) {
+    nodes.clear();
+    nodes.push_back(Node2{});
+    std::vector<Task> work;
+    work.push_back({0, first, count});
+    while (!work.empty()) {
+        const Task tk = work.back();
+        work.pop_back();
+        Node2 nd{};
+        const uint32_t mid = split_node(D, B, tk.first, tk.count, nd);
+        if (mid == kLeafNode) {
+            nodes[tk.node] = nd;
+            continue;
+        }
+        const uint32_t left = (uint32_t)nodes.size();
+        nd.first_or_left = left;
+        nodes[tk.node] = nd;
+        nodes.push_back(Node2{});
+        nodes.push_back(Node2{});
+        work.push_back({left + 1, mid, tk.first + tk.count - mid});  // right after left, as the recursion does
+        work.push_back({left, tk.first, mid - tk.first});
+    }
+}
+
+// Top of the tree, split on the calling thread down to ranges small enough
+// to build as independent subtrees on worker threads.
+struct TopNode {
+    Node2 nd{};
+    int left = -1, right = -1;  // children (internal node)
+    int sub = -1;               // subtree index (a range built by a worker)
+    uint32_t first = 0, count = 0;
+};
+
+int expand_top(BuildData &D, Bins &B, uint32_t first, uint32_t count, uint32_t cutoff, std::vector<TopNode> &top,
+               std::vector<int> &subs) {
+    const int id = (int)top.size();
+    top.emplace_back();
+    top[id].first = first;
+    top[id].count = count;
+    if (count <= cutoff) {
+        top[id].sub = (int)subs.size();
+        subs.push_back(id);
+        return id;
+    }
+    Node2 nd{};
+    const uint32_t mid = split_node(D, B, first, count, nd);
+    top[id].nd = nd;
+    if (mid == kLeafNode) return id;
+    const int l = expand_top(D, B, first, mid - first, cutoff, top, subs);
+    const int r = expand_top(D, B, mid, first + count - mid, cutoff, top, subs);
+    top[id].left = l;
+    top[id].right = r;
+    return id;
+}
+
+size_t count_top(const std::vector<TopNode> &top, const std::vector<std::vector<Node2>> &sub_nodes, int id) {
+    const TopNode &t = top[id];
+    if (t.sub >= 0) return sub_nodes[t.sub].size();
+    if (t.left < 0) return 1;
+    return 1 + count_top(top, sub_nodes, t.left) + count_top(top, sub_nodes, t.right);
+}
+
+// Writes the top node id at out[at] and its descendants from out[alloc] on,
+// in the order the sequential recursion allocates them.
+void emit_top(const std::vector<TopNode> &top, const std::vector<std::vector<Node2>> &sub_nodes, int id,
+              uint32_t at, uint32_t &alloc, std::vector<Node2> &out) {
+    const TopNode &t = top[id];
+    if (t.sub >= 0) {
+        const std::vector<Node2> &S = sub_nodes[t.sub];
+        const uint32_t base = alloc;  // local node k >= 1 -> base + k - 1
+        auto remap = [&](Node2 nd) {
+            if (nd.count == 0) nd.first_or_left = base + nd.first_or_left - 1;
+            return nd;
+        };
+        out[at] = remap(S[0]);
+        for (size_t k = 1; k < S.size(); k++) out[base + k - 1] = remap(S[k]);
+        alloc += (uint32_t)S.size() - 1;
+        return;
+    }
+    Node2 nd = t.nd;
+    if (t.left < 0) {
+        out[at] = nd;
+        return;
+    }
+    const uint32_t l = alloc;
+    alloc += 2;
+    nd.first_or_left = l;
+    out[at] = nd;
+    emit_top(top, sub_nodes, t.left, l, alloc, out);
+    emit_top(top, sub_nodes, t.right, l + 1, alloc, out);
+}
+
+unsigned build_threads() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(16u, hw ? hw : 1u));
+}
+
+template <class F>
+void parallel_for(uint32_t n, unsigned threads, F &&f) {
+    if (threads <= 1 || n < 2) {
+        for (uint32_t i = 0; i < n; i++) f(i, 0u);
+        return;
+    }
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < threads; t++)
+        pool.emplace_back([&, t]() {
+            for (uint32_t i; (i = next.fetch_add(1)) < n;) f(i, t);
+        });
+    for (std::thread &th : pool) th.join();
+}
 
 }  // namespace
 
@@ -64,131 +368,52 @@ void build_bvh2(const float *v9, uint32_t n, uint32_t bins, uint32_t leaf,
     nodes.clear();
     perm.assign(n, 0);
     if (n == 0) return;
-    const float kTrav = 1.0f, kIsect = 1.5f;
-
-    std::vector<Box> pbox(n);
-    std::vector<float> cx(n), cy(n), cz(n);
-    for (uint32_t i = 0; i < n; i++) {
-        const float *t = v9 + 9 * (size_t)i;
-        pbox[i].clear();
-        pbox[i].grow(t);
-        pbox[i].grow(t + 3);
-        pbox[i].grow(t + 6);
-        cx[i] = (pbox[i].lo[0] + pbox[i].hi[0]) * 0.5f;
-        cy[i] = (pbox[i].lo[1] + pbox[i].hi[1]) * 0.5f;
-        cz[i] = (pbox[i].lo[2] + pbox[i].hi[2]) * 0.5f;
-        perm[i] = i;
-    }
-    float *cent[3] = {cx.data(), cy.data(), cz.data()};
-
-    nodes.reserve((size_t)n * 2);
-    nodes.push_back(Node2{});
-    std::vector<Box> bbox(bins);
-    std::vector<int> bcnt(bins);
-    std::vector<float> larea(bins), rarea(bins);
-    std::vector<int> lcnt(bins), rcnt(bins);
-
-    auto make_leaf = [&](uint32_t idx, uint32_t first, uint32_t count) {
-        nodes[idx].first_or_left = first;
-        nodes[idx].count = (uint16_t)std::min<uint32_t>(count, 0xFFFFu);
-        nodes[idx].axis = 0;
-    };
-
-    std::vector<Task> work;
-    work.push_back({0, 0, n});
-    while (!work.empty()) {
-        const Task tk = work.back();
-        work.pop_back();
-        const uint32_t first = tk.first, count = tk.count, end = tk.first + tk.count;
-
-        Box pb, cb;
-        pb.clear();
-        cb.clear();
-        for (uint32_t i = first; i < end; i++) {
-            pb.grow(pbox[perm[i]]);
-            const float c[3] = {cx[i], cy[i], cz[i]};
-            cb.grow(c);
+    const unsigned threads = n >= (1u << 16) ? build_threads() : 1u;
+    BuildData D{bins, leaf, perm, std::vector<Box>(n), {std::vector<float>(n), std::vector<float>(n),
+                std::vector<float>(n)}, std::vector<uint8_t>(n), threads, kParallelRange};
+    const uint32_t blocks = (n + kChunk - 1) / kChunk;
+    parallel_for(blocks, threads, [&](uint32_t blk, unsigned) {
+        const uint32_t end = std::min(n, (blk + 1) * kChunk);
+        for (uint32_t i = blk * kChunk; i < end; i++) {
+            const float *t = v9 + 9 * (size_t)i;
+            Box &b = D.pbox[i];
+            b.clear();
+            b.grow(t);
+            b.grow(t + 3);
+            b.grow(t + 6);
+            D.cent[0][i] = (b.lo[0] + b.hi[0]) * 0.5f;
+            D.cent[1][i] = (b.lo[1] + b.hi[1]) * 0.5f;
+            D.cent[2][i] = (b.lo[2] + b.hi[2]) * 0.5f;
+            perm[i] = i;
         }
-        Node2 &nd = nodes[tk.node];
-        for (int k = 0; k < 3; k++) { nd.bmin[k] = pb.lo[k]; nd.bmax[k] = pb.hi[k]; }
-
-        if (count <= leaf) { make_leaf(tk.node, first, count); continue; }
-
-        const float ext[3] = {cb.hi[0] - cb.lo[0], cb.hi[1] - cb.lo[1], cb.hi[2] - cb.lo[2]};
-        int axis = 0;
-        if (ext[1] > ext[0]) axis = 1;
-        if (ext[2] > ((axis == 0) ? ext[0] : ext[1])) axis = 2;
-        const float cmin = cb.lo[axis];
-        const float cext = ext[axis];
-        if (cext < 1e-12f) { make_leaf(tk.node, first, count); continue; }
-        const float scale = (float)bins / cext;
-        const int last_bin = (int)bins - 1;
-        const float *ca = cent[axis];
-        auto bin_of = [&](uint32_t i) {
-            int b = (int)((ca[i] - cmin) * scale);
-            if (b < 0) b = 0;
-            if (b > last_bin) b = last_bin;
-            return b;
-        };
-
-        for (uint32_t b = 0; b < bins; b++) { bbox[b].clear(); bcnt[b] = 0; }
-        for (uint32_t i = first; i < end; i++) {
-            const int b = bin_of(i);
-            bbox[b].grow(pbox[perm[i]]);
-            bcnt[b]++;
-        }
-        {
-            Box acc; acc.clear();
-            int c = 0;
-            for (int i = 0; i < last_bin; i++) {
-                if (bcnt[i] > 0) acc.grow(bbox[i]);
-                c += bcnt[i];
-                larea[i] = (c > 0) ? acc.area() : 0.0f;
-                lcnt[i] = c;
-            }
-            acc.clear();
-            c = 0;
-            for (int i = last_bin; i > 0; i--) {
-                if (bcnt[i] > 0) acc.grow(bbox[i]);
-                c += bcnt[i];
-                rarea[i - 1] = (c > 0) ? acc.area() : 0.0f;
-                rcnt[i - 1] = c;
-            }
-        }
-        const float parea = pb.area();
-        const float inv_parea = (parea > 0.0f) ? 1.0f / parea : 0.0f;
-        float best = (float)count * kIsect;
-        int best_bin = -1;
-        for (int i = 0; i < last_bin; i++) {
-            if (lcnt[i] == 0 || rcnt[i] == 0) continue;
-            const float cost = kTrav + kIsect * inv_parea *
-                                           (larea[i] * (float)lcnt[i] + rarea[i] * (float)rcnt[i]);
-            if (cost < best) { best = cost; best_bin = i; }
-        }
-        if (best_bin < 0) { make_leaf(tk.node, first, count); continue; }
-
-        uint32_t mid = first;
-        for (uint32_t i = first; i < end; i++) {
-            if (bin_of(i) <= best_bin) {
-                std::swap(perm[i], perm[mid]);
-                std::swap(cx[i], cx[mid]);
-                std::swap(cy[i], cy[mid]);
-                std::swap(cz[i], cz[mid]);
-                mid++;
-            }
-        }
-        const uint32_t nl = mid - first, nr = count - nl;
-        if (nl == 0 || nr == 0) { make_leaf(tk.node, first, count); continue; }
-
-        const uint32_t left = (uint32_t)nodes.size();
-        nodes.push_back(Node2{});
-        nodes.push_back(Node2{});
-        Node2 &p = nodes[tk.node];
-        p.first_or_left = left;
-        p.count = 0;
-        p.axis = (uint16_t)axis;
-        work.push_back({left + 1, mid, nr});  // right after left, as the recursion does
-        work.push_back({left, first, nl});
+    });
+    Bins B(bins);
+    if (threads <= 1) {
+        build_local(D, B, 0, n, nodes);
+    } else {
+        // the calling thread splits the top (binning on all threads); the
+        // subtrees of at most `cutoff` primitives are built by the workers
+        // (each range is touched by one thread only) and stitched in the
+        // sequential allocation order
+        const uint32_t cutoff = std::max<uint32_t>(4096, n / (8 * threads));
+        D.par_min = std::max(kParallelRange, cutoff + 1);  // the workers' subtrees bin on their own thread
+        std::vector<TopNode> top;
+        std::vector<int> subs;
+        expand_top(D, B, 0, n, cutoff, top, subs);
+        std::vector<std::vector<Node2>> sub_nodes(subs.size());
+        std::vector<uint32_t> order(subs.size());
+        for (uint32_t i = 0; i < subs.size(); i++) order[i] = i;
+        std::sort(order.begin(), order.end(),
+                  [&](uint32_t a, uint32_t b) { return top[subs[a]].count > top[subs[b]].count; });
+        std::vector<Bins> tb(threads, Bins(bins));
+        parallel_for((uint32_t)subs.size(), threads, [&](uint32_t i, unsigned th) {
+            const uint32_t k = order[i];
+            const TopNode &t = top[subs[k]];
+            build_local(D, tb[th], t.first, t.count, sub_nodes[k]);
+        });
+        nodes.resize(count_top(top, sub_nodes, 0));
+        uint32_t alloc = 1;
+        emit_top(top, sub_nodes, 0, 0, alloc, nodes);
     }
 }
 

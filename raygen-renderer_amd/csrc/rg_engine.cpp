@@ -366,6 +366,15 @@ class Engine {
         r->t_submit = now_s();
         {
             std::lock_guard<std::mutex> lk(mu);
+            // overload policy: with a limit set, a request whose samples would
+            // take the work not yet done past it is refused (a request always
+            // runs when nothing else is outstanding)
+            if (queue_limit && outstanding > 0 && outstanding + r->samples > queue_limit) {
+                n_rejected++;
+                return set_error(RG_E_BUSY, "server busy: " + std::to_string(outstanding) +
+                                                " samples outstanding, limit " + std::to_string(queue_limit));
+            }
+            outstanding += r->samples;
             r->ticket = next_ticket++;
             if (ticket) *ticket = r->ticket;
             inbox.push_back(r.get());
@@ -459,6 +468,17 @@ class Engine {
         out[5] = fb_used.load();
         out[6] = arena.size();
         out[7] = n_replays.load();
+    }
+
+    void set_queue_limit(uint64_t limit) {
+        std::lock_guard<std::mutex> lk(mu);
+        queue_limit = limit;
+    }
+    void get_queue(uint64_t out[3]) {
+        std::lock_guard<std::mutex> lk(mu);
+        out[0] = outstanding;
+        out[1] = queue_limit;
+        out[2] = n_rejected;
     }
 
     int lanes_used() const { return n_lanes; }
@@ -755,6 +775,7 @@ class Engine {
             r->state = 3;
             n_done++;
             samples_done += r->samples;
+            outstanding -= r->samples;
         }
         cv_done.notify_all();
     }
@@ -781,6 +802,7 @@ class Engine {
                 if (r->state == 3) continue;
                 r->state = 3;
                 r->status = rc;
+                outstanding -= r->samples;
                 r->t_done = now_s();
                 if (r->pin >= 0) slabs[r->pin].arena.release(r->pin_off, r->pin_bytes);
                 r->pin = -1;
@@ -839,6 +861,9 @@ class Engine {
     std::thread worker;
     std::string failure;
     uint64_t n_done = 0, samples_done = 0;
+    // samples of the requests submitted and not done, the limit on them
+    // (0: none) and the requests refused by it; guarded by mu
+    uint64_t outstanding = 0, queue_limit = 0, n_rejected = 0;
     std::atomic<uint64_t> n_generations{0}, n_replays{0}, fb_used{0};
 };
 
@@ -958,6 +983,18 @@ int rg_server_wait(rg_server *srv, uint64_t ticket, double timeout_s, rg_request
 int rg_server_stats(rg_server *srv, uint64_t out[8]) {
     if (!srv || !out) return set_error(RG_E_INVALID, "null argument");
     srv->eng->get_stats(out);
+    return RG_OK;
+}
+
+int rg_server_set_queue_limit(rg_server *srv, uint64_t max_outstanding_samples) {
+    if (!srv) return set_error(RG_E_INVALID, "null server");
+    srv->eng->set_queue_limit(max_outstanding_samples);
+    return RG_OK;
+}
+
+int rg_server_queue(rg_server *srv, uint64_t out[3]) {
+    if (!srv || !out) return set_error(RG_E_INVALID, "null argument");
+    srv->eng->get_queue(out);
     return RG_OK;
 }
 

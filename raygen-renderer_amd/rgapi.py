@@ -94,6 +94,8 @@ def lib():
             "rg_server_wait": (i32, [vp, u64, ctypes.c_double, vp]),
             "rg_server_stats": (i32, [vp, vp]),
             "rg_server_get_tune": (i32, [vp, vp]),
+            "rg_server_set_queue_limit": (i32, [vp, u64]),
+            "rg_server_queue": (i32, [vp, vp]),
             "rg_server_destroy": (i32, [vp]),
             "rg_bloom_default": (None, [vp]),
             "rg_scene_set_textures": (i32, [vp, vp, u32, vp, vp, vp]),
@@ -517,6 +519,14 @@ class Server:
         check(lib().rg_server_stats(self.h, _ptr(out)))
         keys = ["requests", "samples", "rays", "shadow_rays", "generations", "fb_bytes", "budget", "graphs"]
         return dict(zip(keys, (int(x) for x in out)))
+
+    def set_queue_limit(self, max_outstanding_samples):
+        check(lib().rg_server_set_queue_limit(self.h, int(max_outstanding_samples)))
+
+    def queue(self):
+        out = np.zeros(3, np.uint64)
+        check(lib().rg_server_queue(self.h, _ptr(out)))
+        return {"outstanding": int(out[0]), "limit": int(out[1]), "rejected": int(out[2])}
 
     def tune(self):
         t = Tune()

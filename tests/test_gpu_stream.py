@@ -220,3 +220,38 @@ def test_cancel_and_destroy_with_work_in_flight():
         assert info.status == 0 and np.isfinite(out["hdr"]).all()
     finally:
         srv.close()
+
+
+def test_server_queue_limit_refuses_beyond_the_outstanding_work(ctx, resident, single):
+    """With a limit on the samples submitted and not done, requests beyond it
+    are refused with RG_E_BUSY at submission; the accepted ones render as
+    they do alone, and the outstanding count returns to zero."""
+    jobs = make_jobs()
+    sam = [kw["width"] * kw["height"] * kw["spp"] for _, kw in JOBS]
+    limit = max(sam) + min(sam)
+    srv = rgapi.Server(ctx, resident, hbm_budget=1 << 30)
+    try:
+        srv.set_queue_limit(limit)
+        accepted, refused = [], 0
+        for _ in range(3):
+            for i, j in enumerate(jobs):
+                try:
+                    accepted.append((i, srv.submit(j)))
+                except rgapi.RGError as e:
+                    assert e.code == -9
+                    refused += 1
+        say(f"[stream] queue limit {limit}: {len(accepted)} accepted, {refused} refused")
+        assert refused > 0 and len(accepted) >= 3
+        for i, t in accepted:
+            info, out = srv.wait(t, timeout=120.0)
+            assert info is not None and info.status == 0
+            assert np.array_equal(out["hdr"], single[i]["hdr"]), f"request {i}"
+        q = srv.queue()
+        assert q == {"outstanding": 0, "limit": limit, "rejected": refused}
+        srv.set_queue_limit(0)
+        tickets = [srv.submit(j) for j in jobs]
+        for t in tickets:
+            info, _ = srv.wait(t, timeout=120.0)
+            assert info.status == 0
+    finally:
+        srv.close()
