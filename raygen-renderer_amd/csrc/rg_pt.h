@@ -125,8 +125,16 @@ PTScene make_ptscene(const Scene &scn, const rg_settings &st);
 // Validates shading inputs and uploads what changed since the last upload.
 int prepare_scene(Scene *s, void *stream);
 
+// One generation of S samples per pixel (the active tiles' when d_tile_mask
+// is set) from sample_base on; with accumulate, its samples are added to the
+// sums at the end.
 int launch_generation(const Scene &scn, const PTScene &sc, const PTState &st, uint32_t S, uint32_t sample_base,
-                      const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream);
+                      const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream,
+                      bool accumulate = true);
+// Adds the finished generation's samples [s0, s0 + S) of the pixels of the
+// tiles in d_tile_mask (all pixels when null) to the sums.
+int launch_accumulate(const PTScene &sc, const PTState &st, uint32_t s0, uint32_t S, const uint8_t *d_tile_mask,
+                      float *d_sum, float *d_sumsq, float clamp_max, void *stream);
 int launch_resolve(const float *d_sum, const uint32_t *d_nspp, uint32_t width, uint32_t height, float exposure,
                    float *d_hdr, void *stream);
 // Fills n 32-bit words with value.

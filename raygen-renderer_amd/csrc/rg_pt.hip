@@ -195,9 +195,11 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(7))) v
 struct BackColor {
     float c[3];
 };
-__global__ __launch_bounds__(kPtBlock) void k_accumulate(PTState st, uint32_t width, uint32_t npix, uint32_t S,
-                                                          const uint8_t *__restrict__ tile_mask, float clamp_max,
-                                                          BackColor back, float *__restrict__ sum,
+// Sums the generation's samples [s0, s0 + S) of every pixel (one pass of
+// an adaptive window when s0 > 0).
+__global__ __launch_bounds__(kPtBlock) void k_accumulate(PTState st, uint32_t width, uint32_t npix, uint32_t s0,
+                                                          uint32_t S, const uint8_t *__restrict__ tile_mask,
+                                                          float clamp_max, BackColor back, float *__restrict__ sum,
                                                           float *__restrict__ sumsq) {
     const uint32_t pix = blockIdx.x * kPtBlock + threadIdx.x;
     if (pix >= npix) return;
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(kPtBlock) void k_accumulate(PTState st, uint32_t wi
         if (!tile_mask[(y / 32) * ((width + 31) / 32) + x / 32]) return;
     }
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
-    for (uint32_t s = 0; s < S; s++) {
+    for (uint32_t s = s0; s < s0 + S; s++) {
         const float4 c = sample_value(st, (uint32_t)((size_t)s * npix + pix), clamp_max, back.c);
         sx = sx + c.x;
         sy = sy + c.y;
@@ -314,7 +316,7 @@ DevBvh dev_bvh(const Scene &s);
 template <int kStack, int kW>
 static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint32_t S, uint32_t sample_base,
                            const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max,
-                           hipStream_t stream) {
+                           bool accumulate, hipStream_t stream) {
     const uint32_t npix = sc.width * sc.height;
     const uint32_t cap = npix * S;
     const uint32_t blocks = (cap + kPtBlock - 1) / kPtBlock;
@@ -334,19 +336,30 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
         hipLaunchKernelGGL((k_anyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, stream, bv, st);
         qi ^= 1;
     }
-    hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, stream, st,
-                       sc.width, npix, S, d_tile_mask, clamp_max, BackColor{{sc.back[0], sc.back[1], sc.back[2]}},
-                       d_sum, d_sumsq);
+    if (accumulate)
+        hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, stream, st,
+                           sc.width, npix, 0u, S, d_tile_mask, clamp_max,
+                           BackColor{{sc.back[0], sc.back[1], sc.back[2]}}, d_sum, d_sumsq);
     return chk(hipGetLastError(), "path-tracer launch");
 }
 
 int launch_generation(const Scene &scn, const PTScene &sc, const PTState &st, uint32_t S, uint32_t sample_base,
-                      const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream) {
+                      const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream,
+                      bool accumulate) {
     return with_traversal(scn.stack_need, scn.width, [&](auto tag) {
         using T = decltype(tag);
         return generation_impl<T::kStack, T::kW>(scn, sc, st, S, sample_base, d_tile_mask, d_sum, d_sumsq,
-                                                  clamp_max, (hipStream_t)stream);
+                                                  clamp_max, accumulate, (hipStream_t)stream);
     });
+}
+
+int launch_accumulate(const PTScene &sc, const PTState &st, uint32_t s0, uint32_t S, const uint8_t *d_tile_mask,
+                      float *d_sum, float *d_sumsq, float clamp_max, void *stream) {
+    const uint32_t npix = sc.width * sc.height;
+    hipLaunchKernelGGL(k_accumulate, dim3((npix + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, (hipStream_t)stream,
+                       st, sc.width, npix, s0, S, d_tile_mask, clamp_max,
+                       BackColor{{sc.back[0], sc.back[1], sc.back[2]}}, d_sum, d_sumsq);
+    return chk(hipGetLastError(), "accumulate launch");
 }
 
 int launch_resolve(const float *d_sum, const uint32_t *d_nspp, uint32_t width, uint32_t height, float exposure,

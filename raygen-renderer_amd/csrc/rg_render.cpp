@@ -508,9 +508,22 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     uint32_t S = st.samples_in_flight;
     if (S == 0) S = std::max<uint32_t>(1, kPathsInFlight / npix);
     S = std::min(S, spp);
-    if (st.adaptive) S = std::min(S, std::max<uint32_t>(1, st.adaptive_base));
-    if ((uint64_t)npix * S > (1ull << 31)) return set_error(RG_E_BUDGET, "too many samples in flight");
-    if (int rc = ensure_buffers(c->pt, (uint64_t)npix * S, npix, ntiles)) return rc;
+    // adaptive passes fused into one generation (a window) when the paths fit:
+    // the window's later passes are rendered before the tile decisions that
+    // select them are known; the samples of a tile that stops inside the
+    // window are not accumulated, so the sums and tile counts are those of
+    // one generation per pass
+    uint32_t window_max = 1;
+    if (st.adaptive) {
+        const uint32_t base = std::max<uint32_t>(1, st.adaptive_base);
+        S = std::min(S, base);
+        if (st.samples_in_flight == 0)
+            window_max = std::max<uint32_t>(1, std::min<uint64_t>(8, kPathsInFlight / ((uint64_t)npix * base)));
+    }
+    const uint64_t slots = (uint64_t)npix * std::max<uint32_t>(S, st.adaptive ? window_max *
+                                                                 std::max<uint32_t>(1, st.adaptive_base) : S);
+    if (slots > (1ull << 31)) return set_error(RG_E_BUDGET, "too many samples in flight");
+    if (int rc = ensure_buffers(c->pt, slots, npix, ntiles)) return rc;
     PTBuffers &B = c->pt;
 
     const PTScene sc = make_ptscene(*s, st);
@@ -559,27 +572,31 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
                     return rc;
     int gen = 0;
     float queued_frac = 0.0f;  // budget share of the samples queued before the newest generation
-    auto run_samples = [&](uint32_t first, uint32_t count, const uint8_t *mask) -> int {
-        uint32_t done = 0;
-        while (done < count) {
-            if (c->cancel.load()) return set_error(RG_E_CANCELLED, "cancelled");
-            const uint32_t n = std::min(S, count - done);
-            if (int rc = launch_generation(*s, sc, ps, n, first + done, mask, d_sum, d_sumsq, st.firefly_clamp,
-                                           stream))
-                return rc;
-            done += n;
-            if (c->progress) {
-                hipEvent_t ev = (hipEvent_t)c->progress_ev[gen & 1];
-                if (int rc = hip_ok(hipEventRecord(ev, (hipStream_t)stream), "event record")) return rc;
-                if (gen > 0) {
-                    if (int rc = hip_ok(hipEventSynchronize((hipEvent_t)c->progress_ev[(gen - 1) & 1]),
-                                        "progress wait"))
-                        return rc;
-                    report(queued_frac);
-                }
-                queued_frac = (float)(first + done) / (float)spp;
-                gen++;
+    // Queues one generation of n samples from sample `from` (and its
+    // accumulation when `accumulate`).
+    auto queue_generation = [&](uint32_t n, uint32_t from, const uint8_t *mask, bool accumulate) -> int {
+        if (c->cancel.load()) return set_error(RG_E_CANCELLED, "cancelled");
+        if (int rc = launch_generation(*s, sc, ps, n, from, mask, d_sum, d_sumsq, st.firefly_clamp, stream,
+                                       accumulate))
+            return rc;
+        if (c->progress) {
+            hipEvent_t ev = (hipEvent_t)c->progress_ev[gen & 1];
+            if (int rc = hip_ok(hipEventRecord(ev, (hipStream_t)stream), "event record")) return rc;
+            if (gen > 0) {
+                if (int rc = hip_ok(hipEventSynchronize((hipEvent_t)c->progress_ev[(gen - 1) & 1]), "progress wait"))
+                    return rc;
+                report(queued_frac);
             }
+            queued_frac = (float)(from + n) / (float)spp;
+            gen++;
+        }
+        return RG_OK;
+    };
+    auto run_samples = [&](uint32_t first, uint32_t count, const uint8_t *mask) -> int {
+        for (uint32_t done = 0; done < count;) {
+            const uint32_t n = std::min(S, count - done);
+            if (int rc = queue_generation(n, first + done, mask, true)) return rc;
+            done += n;
         }
         return RG_OK;
     };
@@ -594,6 +611,7 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
         if (int rc = launch_fill_u32(d_nspp, first, ntiles, stream)) return rc;
         uint32_t spent = first;
         uint32_t *d_nactive = (uint32_t *)B.n_active;
+        uint32_t window = 1;  // grows 1, 2, 4, ... up to window_max passes per generation
         while (spent < spp) {
             if (int rc = launch_tile_noise(d_sum, d_sumsq, d_nspp, W, H, spp, st.adaptive_threshold, d_mask,
                                            d_nactive, stream))
@@ -601,10 +619,31 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
             uint32_t nact = 0;
             if (int rc = dev_download(&nact, d_nactive, 4, stream)) return rc;
             if (nact == 0) break;
-            const uint32_t pc = std::min(base, spp - spent);
-            if (int rc = run_samples(spent, pc, d_mask)) return rc;
-            if (int rc = launch_tiles_add(d_nspp, d_mask, ntiles, pc, stream)) return rc;
-            spent += pc;
+            const uint32_t passes_left = (spp - spent + base - 1) / base;
+            const uint32_t w = std::min(window, passes_left);
+            const uint32_t count = std::min(w * base, spp - spent);
+            if (w == 1) {
+                if (int rc = queue_generation(count, spent, d_mask, true)) return rc;
+                if (int rc = launch_tiles_add(d_nspp, d_mask, ntiles, count, stream)) return rc;
+            } else {
+                // the window's paths are those of the tiles active now; pass
+                // j's samples are added for the tiles the decision after pass
+                // j - 1 keeps, on the device, in pass order
+                if (int rc = queue_generation(count, spent, d_mask, false)) return rc;
+                for (uint32_t off = 0; off < count;) {
+                    const uint32_t pc = std::min(base, count - off);
+                    if (off > 0)
+                        if (int rc = launch_tile_noise(d_sum, d_sumsq, d_nspp, W, H, spp, st.adaptive_threshold,
+                                                       d_mask, d_nactive, stream))
+                            return rc;
+                    if (int rc = launch_accumulate(sc, ps, off, pc, d_mask, d_sum, d_sumsq, st.firefly_clamp, stream))
+                        return rc;
+                    if (int rc = launch_tiles_add(d_nspp, d_mask, ntiles, pc, stream)) return rc;
+                    off += pc;
+                }
+            }
+            spent += count;
+            window = std::min(window * 2, window_max);
         }
     }
     float *d_hdr = (float *)B.hdr;
