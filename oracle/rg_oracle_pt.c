@@ -1275,6 +1275,13 @@ static void twin_surface(const prep_t *P, rng_t *R, path_t *p, const isect_t *h,
     if (m->transparency > 0.01f) {
         if (prng(R) < m->transparency) {
             twin_transparency(P, p, h, n, 1.0f, 1);
+            /* the reference evaluates the Mix branch first on the same
+             * BSDFParam (rayrenderer.cpp:2889-2894), and its refraction lobe
+             * leaves the dispersion band it drew there (bsdf.cpp:402-407
+             * set chromaChannel, nothing restores it): the transparency
+             * path carries a uniformly drawn band, without the 3x mask */
+            if (m->refraction > 0.00001f && m->chroma_dispersion > 0.0f)
+                p->chroma = (int)fminf(2.0f, floorf(prng(R) * 3.0f));
             return;
         }
     }

@@ -887,6 +887,12 @@ __device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p,
     if (m.transparency > 0.01f) {
         if (prng(R) < m.transparency) {
             step_transparency(sc, p, prim, hit, n, 1.0f, true);
+            // the reference evaluates the Mix branch first on the same
+            // BSDFParam (rayrenderer.cpp:2889-2894); its refraction lobe
+            // leaves the dispersion band it drew (bsdf.cpp:402-407), which
+            // the transparency path then carries without the 3x mask
+            if (m.refraction > 0.00001f && m.chroma_dispersion > 0.0f)
+                p.chroma = (int)fminf(2.0f, floorf(prng(R) * 3.0f));
             return;
         }
     }

@@ -345,3 +345,27 @@ def fog_room(scatter=0.15, absorb=0.05, g=0.3):
     med[0]["density"] = 1.0
     return b.finish(camera={"fov_deg": 60.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0},
                     media=med, global_medium=0)
+
+
+def banded_env(w=64, h=32, value=2.0):
+    """A strongly coloured environment whose hue changes over short angles,
+    so refraction at the three dispersion bands sees different colours."""
+    y, x = np.mgrid[0:h, 0:w].astype(np.float32)
+    img = np.stack([np.cos(x / w * 12) * 0.5 + 0.5, np.sin(y / h * 9) * 0.5 + 0.5,
+                    np.cos((x + y) / 7) * 0.5 + 0.5], -1) * value
+    return {"rgb": np.ascontiguousarray(img, np.float32), "intensity": 1.0, "rotation_deg": 0.0}
+
+
+def dispersive_window(stacks=24, slices=32):
+    """The case where the reference's Mix branch hands its dispersion band to
+    the transparency branch of an eye hit (rayrenderer.cpp:2889-2894,
+    bsdf.cpp:402-407): a transparent, refractive, dispersive pane seen
+    directly, a dispersive glass sphere behind it and a banded environment."""
+    b = Builder()
+    pane = b.add_object(material(color=(1, 1, 1), transparency=0.9, refraction=0.8, refraction_ratio=1.5,
+                                 roughness=0.0, chroma_dispersion=0.15))
+    b.quad(pane, (-3, -3, -2.5), (3, -3, -2.5), (3, 3, -2.5), (-3, 3, -2.5), normal=(0, 0, 1))
+    glass = b.add_object(material(color=(1, 1, 1), refraction=1.0, refraction_ratio=1.5, roughness=0.0,
+                                  chroma_dispersion=0.15))
+    b.sphere(glass, (0, 0, -5), 1.8, stacks, slices)
+    return b.finish(camera={"fov_deg": 40.0, "aperture": 1.8}, env=banded_env())

@@ -89,15 +89,35 @@ def test_extra_materials_match_single_lobe_restatement(ctx):
     assert np.array_equal(outs[0]["hdr"], single["hdr"])
 
 
-@pytest.mark.parametrize("which", ["extra", "dispersion"])
+def test_dispersive_window_matches_single_lobe_restatement(ctx):
+    """The eye hit's Mix branch hands its dispersion band to the
+    transparency path (see test_pt_oracle): per pixel against the twin, and
+    the batched engine equals the single-job path."""
+    sc = scenes.dispersive_window()
+    rs = rgapi.RenderScene(ctx, sc)
+    W, H, spp = 48, 48, 32
+    out = rs.render(rgapi.default_settings(width=W, height=H, spp=spp, max_depth=8))
+    want, _ = oracle.render(sc, W, H, spp, 8, mode=1, threads=8)
+    r = rel_diff(out["sum"], want)
+    assert np.mean(r > 1e-3) < 0.01
+    assert abs(out["sum"].mean() / want.mean() - 1) < 2e-3
+    st = dict(width=W, height=H, spp=4, max_depth=8)
+    outs, _ = rgapi.render_batch(ctx, [rs], [rgapi.make_job(0, **st)])
+    assert np.array_equal(outs[0]["hdr"], rs.render(rgapi.default_settings(**st))["hdr"])
+
+
+@pytest.mark.parametrize("which", ["extra", "dispersion", "window"])
 def test_converges_to_recursive_restatement(ctx, which):
     """GPU single-lobe estimator vs the recursive restatement (every Mix lobe
     evaluated, the reference's call structure) at high sample counts."""
     if which == "extra":
         sc = scenes.materials_extra()
         depth = 8
-    else:
+    elif which == "dispersion":
         sc = scenes.bsdf_frame(grid=3, stacks=16, slices=24, dispersion=0.05)
+        depth = 8
+    else:
+        sc = scenes.dispersive_window()
         depth = 8
     rs = rgapi.RenderScene(ctx, sc)
     W, H = 16, 12

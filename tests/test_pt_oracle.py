@@ -358,3 +358,25 @@ def test_texture_tiling_repeats_and_wraps():
     got = np.argmax(row, axis=1)
     assert away.sum() > 30 and np.array_equal(got[away], want[away])
     assert set(want[away]) == {0, 1} and (np.diff(want[away]) != 0).sum() >= 3   # four bands
+
+
+def test_dispersion_band_left_by_the_eye_hit_mix_branch():
+    """A transparent, refractive, dispersive pane seen directly: the
+    reference evaluates its Mix branch, whose refraction lobe draws a
+    dispersion band and leaves it in the BSDFParam, before the transparency
+    branch traces on with that band (rayrenderer.cpp:2889-2894; bsdf.cpp:
+    402-407).  The single-lobe estimator must carry such a band on its
+    transparency path to have the recursive restatement's expectation: with a
+    banded environment behind a dispersive sphere the difference is large
+    (over half the pixel channels beyond 3.5 sigma without it)."""
+    sc = scenes.dispersive_window()
+    W = H = 16
+    n = 8192
+    a, aq = oracle.render(sc, W, H, n, 8, mode=0, threads=THREADS, seed=1, firefly_clamp=0.0)
+    b, bq = oracle.render(sc, W, H, n, 8, mode=1, threads=THREADS, seed=2, firefly_clamp=0.0)
+    ma, mb = a / n, b / n
+    va = np.maximum(aq / n - ma * ma, 0) / n
+    vb = np.maximum(bq / n - mb * mb, 0) / n
+    z = (ma - mb) / np.sqrt(va + vb + 1e-12)
+    assert np.mean(np.abs(z) > 3.5) <= 0.02
+    assert abs(ma.mean() / mb.mean() - 1) < 0.005
