@@ -65,6 +65,8 @@ int rg_create(int device, rg_ctx **out) {
         rc = dev_stream_create(&c->own_stream);
         if (rc) { delete c; return rc; }
         c->stream = c->own_stream;
+        rc = dev_malloc((void **)&c->trace_work, Ctx::kTraceSlots * sizeof(uint64_t));
+        if (rc) { dev_stream_destroy(c->own_stream); delete c; return rc; }
         // the no-allocation entry point's scratch address, once per device;
         // a failure is reported by that entry point, not here
         (void)post_scratch_init(device);
@@ -81,6 +83,7 @@ void rg_destroy(rg_ctx *ctx) {
         release_pt(c);
         release_post(c);
         release_engine(c);
+        dev_free(c->trace_work);
     }
     if (c->own_stream) dev_stream_destroy(c->own_stream);
     if (c->device >= 0) dev_clear_error();
@@ -228,7 +231,8 @@ int rg_trace_closest(rg_ctx *ctx, const rg_scene *scene, const rg_ray *d_rays, r
     if (int rc = need_device(c)) return rc;
     if (!s || (n && (!d_rays || !d_hits))) return set_error(RG_E_INVALID, "null argument");
     if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
-    return launch_closest(*s, d_rays, d_hits, n, c->stream);
+    uint64_t *work = c->trace_work + c->trace_slot++ % Ctx::kTraceSlots;
+    return launch_closest(*s, d_rays, d_hits, n, work, c->stream);
 }
 
 int rg_trace_any(rg_ctx *ctx, const rg_scene *scene, const rg_ray *d_rays, uint8_t *d_occluded,
@@ -239,7 +243,8 @@ int rg_trace_any(rg_ctx *ctx, const rg_scene *scene, const rg_ray *d_rays, uint8
     if (!s || (n && (!d_rays || !d_occluded))) return set_error(RG_E_INVALID, "null argument");
     if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
     if (pred < RG_PRED_ALL || pred > RG_PRED_ENV) return set_error(RG_E_INVALID, "unknown predicate");
-    return launch_any(*s, d_rays, d_occluded, n, pred, c->stream);
+    uint64_t *work = c->trace_work + c->trace_slot++ % Ctx::kTraceSlots;
+    return launch_any(*s, d_rays, d_occluded, n, pred, work, c->stream);
 }
 
 int rg_trace_closest_host(rg_ctx *ctx, const rg_scene *scene, const rg_ray *rays, rg_hit *hits,

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "rg_internal.h"
 #include "rg_pt.h"
@@ -327,8 +328,9 @@ constexpr uint32_t kNoNode = 0xFFFFFFFEu;
 
 // Persistent traversal with leaf postponement and lane refill (after Aila
 // & Laine, HPG 2009), shared by every kernel that traces a batch of rays.
-// Each wave owns a contiguous chunk of the n ray indices; a lane whose ray is
-// done takes the next index of the chunk, so a wave does not idle behind its
+// Each wave owns a contiguous chunk of the n ray indices (and, with work
+// sharing, grabs more); a lane whose ray is done takes the next index of the
+// chunk, so a wave does not idle behind its
 // longest ray.  Inner nodes are traversed until every lane of the wave holds
 // a leaf to intersect (the first leaf a lane reaches is postponed and the
 // lane keeps descending with its current t), then all lanes intersect their
@@ -354,6 +356,32 @@ constexpr uint32_t kNoNode = 0xFFFFFFFEu;
 // Src provides: bool load(i, bv, PsRay &) (false: nothing to trace at i),
 // store_closest(i, prim, t, u, v), store_any(i, occluded).  Every thread of
 // the block must call it (it synchronises the block once).
+//
+// Work sharing: a Src with a member `work` (pointer to a counter that is zero
+// when the kernel starts, 32- or 64-bit) and the constants kStaticQuarters
+// and kGrab hands each wave an equal chunk of the first kStaticQuarters / 4
+// of the rays, then the rest in grabs of kGrab indices taken from the
+// counter, so waves whose rays run short take over the long tail of waves
+// whose rays run long.  The counter ends above n by at most kGrab per wave.
+// Without `work`, or with fewer than kShareMinPerWave rays per wave, the n
+// rays are split into equal chunks only: the counter is one address for the
+// whole device, and a burst of read-modify-writes on it from every wave
+// costs more than a short launch's imbalance.
+constexpr uint64_t kShareMinPerWave = 256;
+template <class S, class = void>
+struct WorkShare {
+    using T = unsigned long long;
+    static constexpr uint32_t kStaticQuarters = 4, kGrab = 1;
+    __device__ static T *get(const S &) { return nullptr; }
+};
+template <class S>
+struct WorkShare<S, std::void_t<decltype(&S::work)>> {
+    using T = std::remove_pointer_t<decltype(S::work)>;
+    static constexpr uint32_t kStaticQuarters = S::kStaticQuarters, kGrab = S::kGrab;
+    static_assert(kStaticQuarters <= 4 && kGrab >= 64, "static share at most all rays, grabs of a wave or more");
+    __device__ static T *get(const S &s) { return s.work; }
+};
+
 template <int kStack, int kW, bool kAny, int kStride, class Src>
 __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, uint64_t n,
                                               const LdsColumn<kStride> lc) {
@@ -371,9 +399,13 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, 
     const uint32_t lane = __lane_id();
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
     const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint64_t chunk = (n + waves - 1) / waves;
+    using Share = WorkShare<Src>;
+    typename Share::T *const work = n >= kShareMinPerWave * waves ? Share::get(src) : nullptr;
+    const uint64_t chunk = work ? n / 4 * Share::kStaticQuarters / waves : (n + waves - 1) / waves;
+    const uint64_t static_end = work ? chunk * waves : n;  // <= n: grabs start here
     uint64_t cursor = wave * chunk;
-    const uint64_t chunk_end = cursor + chunk < n ? cursor + chunk : n;
+    uint64_t chunk_end = cursor + chunk < n ? cursor + chunk : n;
+    bool exhausted = work == nullptr;
     bool active = false;
     uint64_t ri = 0;
     PsRay r{};
@@ -417,8 +449,20 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, 
     };
 
     for (;;) {
-        // refill idle lanes from the wave's chunk
+        // refill idle lanes from the wave's chunk or grab
         const uint64_t idle = __ballot(!active);
+        if (idle && cursor >= chunk_end && !exhausted) {
+            // the wave's chunk is used up: grab the next kGrab indices
+            typename Share::T base = 0;
+            if (lane == 0) base = atomicAdd(work, (typename Share::T)Share::kGrab);
+            const uint64_t b = static_end + (uint64_t)__shfl(base, 0);
+            if (b < n) {
+                cursor = b;
+                chunk_end = b + Share::kGrab < n ? b + Share::kGrab : n;
+            } else {
+                exhausted = true;
+            }
+        }
         if (idle && cursor < chunk_end) {
             const uint64_t idx = cursor + (uint64_t)__popcll(idle & ((1ull << lane) - 1ull));
             if (!active && idx < chunk_end) {
@@ -443,7 +487,7 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, 
             cursor += (uint64_t)__popcll(idle);
         }
         if (__ballot(active) == 0) {
-            if (cursor >= chunk_end) break;
+            if (cursor >= chunk_end && exhausted) break;
             continue;
         }
         // inner phase

@@ -117,6 +117,12 @@ struct Ctx {
     void *lane_streams[4] = {nullptr, nullptr, nullptr, nullptr};  // the batched engines' lanes, kept
     PTBuffers pt;
     PostBuffers post;
+    // work-sharing counters of rg_trace_closest / rg_trace_any, one per
+    // launch in turn, so launches on different streams do not share one
+    // unless kTraceSlots launches apart
+    static constexpr uint32_t kTraceSlots = 64;
+    uint64_t *trace_work = nullptr;
+    uint32_t trace_slot = 0;
     uint64_t last_stats[3] = {0, 0, 0};
 };
 
@@ -205,8 +211,12 @@ int dev_download(void *dst, const void *src, size_t bytes, void *stream);
 int dev_sync(void *stream);
 int dev_set_device(int device);
 void dev_clear_error();
-int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n, void *stream);
-int launch_any(const Scene &s, const rg_ray *rays, uint8_t *occ, uint64_t n, int pred, void *stream);
+// work: a device counter the launch may clear and use to share the rays
+// among its waves (null: even split); launches that share one must not
+// overlap.
+int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n, uint64_t *work, void *stream);
+int launch_any(const Scene &s, const rg_ray *rays, uint8_t *occ, uint64_t n, int pred, uint64_t *work,
+               void *stream);
 int launch_gen_rays(const float box[6], float scale, uint32_t key, uint64_t first,
                     rg_ray *rays, uint64_t n, void *stream);
 

@@ -102,6 +102,11 @@ struct PTScene {
 };
 
 // Structure-of-arrays path state, capacity = pixels x samples in flight.
+// PTState::count slots of the extension and shadow-ray traversals' work-
+// sharing counters: each is zero when its kernel starts (the generation's
+// first kernel clears both, and each traversal kernel clears the other's).
+constexpr int kExtendWork = 4, kShadowWork = 5;
+
 struct PTState {
     rg_ray *ray;        // current ray
     rg_hit *hit;        // its closest hit
@@ -115,7 +120,8 @@ struct PTState {
     float4 *scontrib;   // unoccluded contribution per (path, k)
     uint32_t *nshadow;  // shadow rays per path this event
     uint32_t *q[2];     // live-path queues
-    uint32_t *count;    // [0], [1] path-queue lengths, [2] shadow-ray queue length
+    uint32_t *count;    // [0], [1] path-queue lengths, [2] shadow-ray queue length,
+                        // [kExtendWork], [kShadowWork] the single-job traversals' work-sharing counters
     unsigned long long *stats; // [0] extension rays, [1] shadow rays, [2] path samples
     uint32_t sample_base;
 };

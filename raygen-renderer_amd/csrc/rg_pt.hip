@@ -35,6 +35,12 @@ namespace rg {
 // kernels
 // ---------------------------------------------------------------------------
 __global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
+// A generation's first kernel: the first queue's length, zero work counters.
+__global__ void k_begin_counts(uint32_t *count, uint32_t n0) {
+    count[0] = n0;
+    count[kExtendWork] = 0;
+    count[kShadowWork] = 0;
+}
 
 // Camera rays (rayrenderer.cpp:860-934) for the paths of samples
 // [sample_base, sample_base + S) of every pixel in an active tile.
@@ -83,6 +89,8 @@ struct ExtendSrc {
     const rg_ray *ray;
     const uint32_t *q;
     rg_hit *hit;
+    uint32_t *work;
+    static constexpr uint32_t kStaticQuarters = 1, kGrab = 128;
     __device__ __forceinline__ bool load(uint64_t i, const DevBvh &, PsRay &r) const {
         const rg_ray x = ray[q[i]];
         r = {x.o[0], x.o[1], x.o[2], x.d[0], x.d[1], x.d[2], x.tmin, x.tmax, 0u};
@@ -95,7 +103,8 @@ struct ExtendSrc {
 };
 
 // Block 0 also clears the shadow-ray queue and the next path queue, which
-// nothing reads while this kernel runs and the shade kernel after it fills.
+// nothing reads while this kernel runs and the shade kernel after it fills,
+// and the shadow-ray traversal's work counter.
 // Both traversal kernels run at 7 waves per SIMD (72 VGPRs), as the
 // closest-hit kernel: measured faster than the compiler's 8-wave allocation.
 template <int kStack, int kW>
@@ -105,9 +114,11 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(7))) v
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st.count[2] = 0;
         st.count[qi ^ 1] = 0;
+        st.count[kShadowWork] = 0;
         if (count) atomicAdd(st.stats + 0, (unsigned long long)count);
     }
-    trace_ps_loop<kStack, kW, false, kPtBlock>(bv, ExtendSrc{st.ray, st.q[qi], st.hit}, count, lc);
+    trace_ps_loop<kStack, kW, false, kPtBlock>(bv, ExtendSrc{st.ray, st.q[qi], st.hit, st.count + kExtendWork},
+                                               count, lc);
 }
 
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(PTScene sc, PTState st,
@@ -170,6 +181,8 @@ struct ShadowSrc {
     const rg_ray *sray;
     const uint32_t *ssrc;
     uint8_t *socc;
+    uint32_t *work;
+    static constexpr uint32_t kStaticQuarters = 1, kGrab = 128;
     __device__ __forceinline__ bool load(uint64_t i, const DevBvh &, PsRay &r) const {
         const rg_ray x = sray[ssrc[i]];
         r = {x.o[0], x.o[1], x.o[2], x.d[0], x.d[1], x.d[2], 0.0f, x.tmax, 1u << (int)x.tmin};
@@ -183,8 +196,12 @@ template <int kStack, int kW>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_anyhit(DevBvh bv, PTState st) {
     RG_LDS_COLUMN(lc, kPtBlock);
     const uint32_t n = st.count[2];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(st.stats + 1, (unsigned long long)n);
-    trace_ps_loop<kStack, kW, true, kPtBlock>(bv, ShadowSrc{st.sray, st.ssrc, st.socc}, n, lc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st.count[kExtendWork] = 0;  // for the next extension
+        if (n) atomicAdd(st.stats + 1, (unsigned long long)n);
+    }
+    trace_ps_loop<kStack, kW, true, kPtBlock>(bv, ShadowSrc{st.sray, st.ssrc, st.socc, st.count + kShadowWork}, n,
+                                              lc);
 }
 
 // Per-pixel sum / sum of squares over the S samples of this generation,
@@ -326,7 +343,7 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
     const uint32_t ext_blocks = std::min<uint32_t>(blocks, resident_blocks<k_extend<kStack, kW>>(kPtBlock));
     const uint32_t any_blocks = std::min<uint32_t>(2 * blocks, resident_blocks<k_anyhit<kStack, kW>>(kPtBlock));
     st.sample_base = sample_base;
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, stream, st.count + 0, d_tile_mask ? 0u : cap);
+    hipLaunchKernelGGL(k_begin_counts, dim3(1), dim3(1), 0, stream, st.count, d_tile_mask ? 0u : cap);
     hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, S, sample_base, d_tile_mask);
     int qi = 0;
     const int bounces = sc.max_depth + 2;

@@ -85,6 +85,8 @@ struct RayArraySrc {
     rg_hit *hits;
     uint8_t *occ;
     uint32_t pred_bit;
+    unsigned long long *work;  // zeroed work-sharing counter, or null
+    static constexpr uint32_t kStaticQuarters = 1, kGrab = 128;
     __device__ __forceinline__ bool load(uint64_t i, const DevBvh &, PsRay &r) const {
         const float4 r0 = reinterpret_cast<const float4 *>(rays)[2 * i];
         const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
@@ -151,36 +153,44 @@ DevBvh dev_bvh(const Scene &s) {
     return b;
 }
 
-// Persistent launch: as many blocks as are resident at once, each wave
-// taking an equal chunk of the rays.
+// Persistent launch: as many blocks as are resident at once.  With enough
+// rays per wave they are shared through the caller's counter (a quarter in
+// equal chunks, the rest in grabs of 128), which the launch clears first on
+// the same stream; otherwise they are split evenly.
+__global__ void k_clear_u64(unsigned long long *w) { *w = 0ull; }
+
 template <int kStack, int kW, bool kAny>
 static int launch_ps(const DevBvh &bv, const rg_ray *rays, rg_hit *hits, uint8_t *occ, uint64_t n, uint32_t bit,
-                     hipStream_t stream) {
+                     unsigned long long *work, hipStream_t stream) {
     const uint32_t resident = resident_blocks<k_trace_ps<kStack, kW, kAny>>(kBlock);
     const uint64_t blocks = std::min<uint64_t>((uint64_t)resident, (n + kBlock - 1) / kBlock);
-    const RayArraySrc src{rays, hits, occ, bit};
+    if (n < kShareMinPerWave * (blocks * kBlock / 64)) work = nullptr;
+    if (work) hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(1), 0, stream, work);
+    const RayArraySrc src{rays, hits, occ, bit, work};
     hipLaunchKernelGGL((k_trace_ps<kStack, kW, kAny>), dim3((uint32_t)blocks), dim3(kBlock), 0, stream, bv, src, n);
     return hip_check(hipGetLastError(), "traversal launch");
 }
 
 static int trace_ps(const Scene &s, const rg_ray *rays, rg_hit *hits, uint8_t *occ, uint64_t n, uint32_t bit,
-                    void *stream) {
+                    uint64_t *work, void *stream) {
     const DevBvh bv = dev_bvh(s);
     return with_traversal(s.stack_need, s.width, [&](auto tag) {
         using T = decltype(tag);
-        if (occ) return launch_ps<T::kStack, T::kW, true>(bv, rays, hits, occ, n, bit, (hipStream_t)stream);
-        return launch_ps<T::kStack, T::kW, false>(bv, rays, hits, occ, n, bit, (hipStream_t)stream);
+        unsigned long long *w = reinterpret_cast<unsigned long long *>(work);
+        if (occ) return launch_ps<T::kStack, T::kW, true>(bv, rays, hits, occ, n, bit, w, (hipStream_t)stream);
+        return launch_ps<T::kStack, T::kW, false>(bv, rays, hits, occ, n, bit, w, (hipStream_t)stream);
     });
 }
 
-int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n, void *stream) {
+int launch_closest(const Scene &s, const rg_ray *rays, rg_hit *hits, uint64_t n, uint64_t *work, void *stream) {
     if (n == 0) return RG_OK;
-    return trace_ps(s, rays, hits, nullptr, n, 0, stream);
+    return trace_ps(s, rays, hits, nullptr, n, 0, work, stream);
 }
 
-int launch_any(const Scene &s, const rg_ray *rays, uint8_t *occ, uint64_t n, int pred, void *stream) {
+int launch_any(const Scene &s, const rg_ray *rays, uint8_t *occ, uint64_t n, int pred, uint64_t *work,
+               void *stream) {
     if (n == 0) return RG_OK;
-    return trace_ps(s, rays, nullptr, occ, n, 1u << pred, stream);
+    return trace_ps(s, rays, nullptr, occ, n, 1u << pred, work, stream);
 }
 
 int launch_gen_rays(const float box[6], float scale, uint32_t key, uint64_t first, rg_ray *rays,
