@@ -398,7 +398,10 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, 
     float stk_t[kSpill];
     const uint32_t lane = __lane_id();
     const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
-    const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // the wave's index as a wave-uniform value, so that the chunk bounds
+    // and the cursor stay in scalar registers
+    const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) +
+                          (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     using Share = WorkShare<Src>;
     typename Share::T *const work = n >= kShareMinPerWave * waves ? Share::get(src) : nullptr;
     const uint64_t chunk = work ? n / 4 * Share::kStaticQuarters / waves : (n + waves - 1) / waves;
@@ -455,7 +458,9 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, 
             // the wave's chunk is used up: grab the next kGrab indices
             typename Share::T base = 0;
             if (lane == 0) base = atomicAdd(work, (typename Share::T)Share::kGrab);
-            const uint64_t b = static_end + (uint64_t)__shfl(base, 0);
+            uint64_t got = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, 0);
+            if (sizeof(base) == 8) got |= (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)base >> 32), 0) << 32;
+            const uint64_t b = static_end + got;
             if (b < n) {
                 cursor = b;
                 chunk_end = b + Share::kGrab < n ? b + Share::kGrab : n;
