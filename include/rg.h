@@ -77,7 +77,7 @@ typedef struct {
 /* Occlusion predicates of the reference's intersectAny call sites. */
 #define RG_PRED_ALL 0  /* every triangle occludes (microbenchmark)          */
 #define RG_PRED_AREA 1 /* transparency<0.01 && emission<=0 (rayrenderer.cpp:1956-1959) */
-#define RG_PRED_ENV 2  /* !(emission>0) && (transparency<0.01 || refraction>0.1) (:1499-1503) */
+#define RG_PRED_ENV 2  /* !(emission>0) && (transparency<0.01 || refraction>0.1) (:1499-1503; point lights :2211-2215) */
 
 /* ---- context ------------------------------------------------------------ */
 /* device >= 0: HIP device ordinal.  device == -1: host-only context (scene
@@ -221,6 +221,20 @@ typedef struct {
     float back_color[4];
 } rg_settings;
 
+/* Point light: an emissive scene object without meshes, which
+ * transformScene lists as a point light source (rayrenderer.cpp:333-353).
+ * pos = the object's origin and normal = its direction
+ * normalize(cos(ay) sin(ax), cos(ay) cos(ax), sin(ay)) after the
+ * normal matrix, both in render space; color, emission and spot_range are
+ * its material's color, emission and spotRange (material.h:50-52; 0 = omni,
+ * else the cone angle read through RADIAN_TO_DEGREE as the reference does,
+ * rayrenderer.cpp:2228-2232). */
+typedef struct {
+    float pos[3], normal[3], color[3];
+    float emission, spot_range;
+    uint32_t pad;
+} rg_point_light;
+
 /* One material texture (Texture, texture.h:19-50): w x h texels of RGB
  * floats as stored in the image (LDR: byte / 255, HDR: linear radiance), row
  * 0 = top.  srgb != 0 marks an LDR image authored in sRGB: its texels are
@@ -254,6 +268,12 @@ int rg_scene_set_textures(rg_scene *scene, const rg_texture *tex, uint32_t n_tex
 int rg_scene_set_media(rg_scene *scene, const rg_medium *media, uint32_t n, int32_t global_medium);
 /* env NULL or w == 0 removes the environment. */
 int rg_scene_set_env(rg_scene *scene, const rg_env *env);
+/* Point light sources of RayRenderer::traceLight (rayrenderer.cpp:2205-2290;
+ * the pointLightSources list built at :333-353): the diffuse lobe's direct
+ * term and the depth-cap light term add one light's tracePointLight (the
+ * only one, or one picked uniformly), occluded by the triangles that block
+ * environment rays (RG_PRED_ENV).  n == 0 removes them. */
+int rg_scene_set_point_lights(rg_scene *scene, const rg_point_light *lights, uint32_t n);
 int rg_scene_set_camera(rg_scene *scene, const rg_camera *cam);
 
 /* Replaces RayRenderer::render() (rayrenderer.cpp:478-596) for shader 5:
@@ -324,10 +344,12 @@ int rg_postprocess_device_noalloc(rg_ctx *ctx, uint32_t width, uint32_t height, 
  * binary .mesh files, Wavefront .obj files with their .mtl libraries (one
  * child object per OBJ object, ObjFileReader, objreader.cpp:93-590;
  * sceneloader.cpp:405-478), inline and `_materials` materials, object and
- * global `medium`, `mainCamera` and an .hdr `envmap`, and returns a scene in
- * render space with shading, media, environment and camera set (BVH not
- * built).  Parts outside this path (FBX, .toba, textures, cubemaps,
- * non-homogeneous media) are skipped with a warning (rg_scene_warnings). */
+ * global `medium`, `mainCamera`, an .hdr `envmap`, albedo textures and
+ * point lights (emissive objects without meshes), and returns a scene in
+ * render space with shading, media, environment, textures, point lights and
+ * camera set (BVH not built).  Parts outside this path (FBX, .toba,
+ * cubemaps, non-homogeneous media) are skipped with a warning
+ * (rg_scene_warnings). */
 int rg_scene_load_json(rg_ctx *ctx, const char *path, rg_scene **out);
 /* Newline-separated warnings of the load into buf (capacity cap); returns the
  * full length. */
@@ -342,6 +364,10 @@ int rg_scene_export(const rg_scene *scene, float *v9, float *n9, uint32_t *obj_o
  * (env->width * env->height * 3 floats) where non-NULL. */
 int rg_scene_export_media(const rg_scene *scene, rg_medium *media, uint32_t *n_media, int32_t *global_medium,
                           rg_env *env, float *env_rgb);
+
+/* Copies the point lights out: n first, then lights (n records) where
+ * non-NULL. */
+int rg_scene_export_point_lights(const rg_scene *scene, rg_point_light *lights, uint32_t *n);
 
 /* Copies the textures out: n_tex first, then, where non-NULL, tex (w, h,
  * srgb of each; rgb left NULL), texels (the textures' RGB floats as handed

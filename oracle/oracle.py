@@ -174,6 +174,8 @@ OBJECT_DTYPE = np.dtype([("color", "<f4", (3,)), ("glossy", "<f4"), ("roughness"
                          ("metallic", "<f4"), ("anisotropy", "<f4"), ("aniso_rotation", "<f4"),
                          ("emission", "<f4"), ("interior_medium", "<i4"), ("visible", "<u4"),
                          ("chroma_dispersion", "<f4"), ("pad", "<u4")])
+POINT_LIGHT_DTYPE = np.dtype([("pos", "<f4", (3,)), ("normal", "<f4", (3,)), ("color", "<f4", (3,)),
+                              ("emission", "<f4"), ("spot_range", "<f4"), ("pad", "<u4")])
 MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", (3,)), ("sigma_s", "<f4", (3,)), ("sigma_e", "<f4", (3,)),
                          ("g", "<f4"), ("density", "<f4"), ("pad", "<f4")])
 
@@ -199,7 +201,8 @@ class _Desc(ctypes.Structure):
                 ("ntris", ctypes.c_uint32), ("objs", ctypes.c_void_p), ("nobj", ctypes.c_uint32),
                 ("media", ctypes.c_void_p), ("nmedia", ctypes.c_uint32), ("global_medium", ctypes.c_int32),
                 ("env", _Env), ("cam", _Cam), ("tex", ctypes.c_void_p), ("ntex", ctypes.c_uint32),
-                ("tex_of_obj", ctypes.c_void_p), ("tiling", ctypes.c_void_p), ("uv6", ctypes.c_void_p)]
+                ("tex_of_obj", ctypes.c_void_p), ("tiling", ctypes.c_void_p), ("uv6", ctypes.c_void_p),
+                ("plights", ctypes.c_void_p), ("nplights", ctypes.c_uint32)]
 
 
 class _Settings(ctypes.Structure):
@@ -260,6 +263,12 @@ def _desc(scene, keep):
         d.tex_of_obj = too.ctypes.data
         d.tiling = til.ctypes.data
         d.uv6 = uv6.ctypes.data
+    pl = scene.get("point_lights")
+    if pl is not None and len(pl):
+        pla = np.ascontiguousarray(pl, dtype=POINT_LIGHT_DTYPE)
+        keep.append(pla)
+        d.plights = pla.ctypes.data
+        d.nplights = len(pla)
     cam = scene["camera"]
     d.cam = _Cam(cam.get("fov_deg", 75.0), cam.get("exposure", 1.0), cam.get("depth_of_field", 0.0),
                  cam.get("aperture", 1.8), cam.get("aperture_blades", 0), cam.get("aperture_rotation_deg", 0.0),

@@ -143,6 +143,15 @@ typedef struct {
     uint32_t pad;
 } or_texture;
 
+/* Point light source (transformScene, rayrenderer.cpp:333-353): render-
+ * space position and direction, the object's colour, emission and
+ * spotRange (0 = omni). */
+typedef struct {
+    float pos[3], normal[3], color[3];
+    float emission, spot_range;
+    uint32_t pad;
+} or_point_light;
+
 typedef struct {
     const float *v9;            /* render-space vertices */
     const float *n9;            /* render-space vertex normals */
@@ -162,6 +171,9 @@ typedef struct {
     const int32_t *tex_of_obj;
     const float *tiling;
     const float *uv6;
+    /* point light sources (nullable) */
+    const or_point_light *plights;
+    uint32_t nplights;
 } or_scene_desc;
 
 typedef struct {

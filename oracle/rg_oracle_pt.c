@@ -28,8 +28,10 @@
  * ugm routines that are not vendored with the reference and are therefore
  * parity unpinned: vec3::normalize (taken as v * (1/|v|)),
  * randomRayInHemisphere (taken as a uniform hemisphere direction),
- * DEGREE_TO_RADIAN (binary32).  Point lights, textures, cubemaps,
- * heterogeneous / cone / path media and heat haze are outside this path.
+ * DEGREE_TO_RADIAN (binary32), and for point lights RADIAN_TO_DEGREE (x *
+ * 180 / pi), smoothstep (GLSL) and color3 clamp (per channel to [0, 1]).
+ * Cubemaps, heterogeneous / cone / path media and heat haze are outside
+ * this path.
  */
 #include "rg_oracle.h"
 
@@ -562,7 +564,7 @@ static nee_t sample_area(const prep_t *P, rng_t *R, v3 hit, v3 n, float *cos_obj
 }
 
 /* traceAreaLight's returned value (Lambert NEE incl. 1/pi and MIS) */
-static v3 trace_light(const prep_t *P, rng_t *R, v3 hit, v3 n) {
+static v3 trace_area_light(const prep_t *P, rng_t *R, v3 hit, v3 n) {
     float co, cl, r2, sa;
     nee_t s = sample_area(P, R, hit, n, &co, &cl, &r2, &sa);
     if (!s.ok) return mk(0, 0, 0);
@@ -571,6 +573,56 @@ static v3 trace_light(const prep_t *P, rng_t *R, v3 hit, v3 n) {
     const float pl2 = pl * pl, pb2 = pb * pb;
     const float w = pl2 / (pl2 + pb2);
     return scl(scl(s.Le, co * cl * sa / (PI_F * r2)), w);
+}
+
+/* ugm smoothstep (not vendored; the GLSL definition) */
+static float smoothstep_f(float e0, float e1, float x) {
+    float t = (x - e0) / (e1 - e0);
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+static float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+/* tracePointLight (rayrenderer.cpp:2205-2269): shadow ray toward the light
+ * (unnormalised, maxt 0.99999, blocked by non-emissive triangles with
+ * transparency < 0.01 or refraction > 0.1: the environment predicate); the
+ * unblocked value is color x emission x dotToLight / |lightray|^2 with
+ * dotToLight = |dotToObject| for an omni light, the spot falloff
+ * dot(l, n_light) x smoothstep(cos - 0.1, cos + 0.1, dot(l, n_light)) with
+ * cos = cosf(RADIAN_TO_DEGREE(spotRange / 2)) for a spot, each channel
+ * clamped to [0, 1] (ugm clamp).  RADIAN_TO_DEGREE(x) is taken as
+ * x * 180 / pi, its name's meaning (ugm is not vendored). */
+static v3 trace_point_light(const prep_t *P, const or_point_light *L, v3 hit, v3 n) {
+    const v3 lr = sub(ld3(L->pos), hit);
+    if (occluded(P, surface_origin(hit, lr, n), lr, 0.99999f, 2)) return mk(0, 0, 0);
+    const v3 dir = nrm(lr);
+    const float dto = dot(dir, n);
+    float dtl = dot(dir, ld3(L->normal));
+    if (!(dto > 0.0f)) return mk(0, 0, 0);
+    if (L->spot_range > 0.0f) {
+        const float sc = cosf(L->spot_range * 0.5f * (float)(180.0 / M_PI));
+        dtl = dtl * smoothstep_f(fmaxf(sc - 0.1f, 0.0f), fminf(sc + 0.1f, 1.0f), dtl);
+    } else {
+        dtl = fabsf(dto);
+    }
+    if (!(dtl > 0.0f)) return mk(0, 0, 0);
+    const float da = powf(len(lr), -2.0f);
+    const float lum = L->emission * dtl * da;
+    return mk(clamp01(L->color[0] * lum), clamp01(L->color[1] * lum), clamp01(L->color[2] * lum));
+}
+
+/* traceLight (rayrenderer.cpp:2271-2293): one area light's traceAreaLight
+ * plus one point light's tracePointLight (the only one, or one picked
+ * uniformly by a rand() draw) */
+static v3 trace_light(const prep_t *P, rng_t *R, v3 hit, v3 n) {
+    v3 c = trace_area_light(P, R, hit, n);
+    const uint32_t np = P->d->plights ? P->d->nplights : 0;
+    if (np > 0) {
+        const uint32_t li = np == 1 ? 0u : pick(prng(R), np);
+        c = add(c, trace_point_light(P, &P->d->plights[li], hit, n));
+    }
+    return c;
 }
 
 /* sampleEnvmapForNEE (rayrenderer.cpp:2038-2065) */

@@ -224,7 +224,8 @@ class Engine {
         // one device block for every lane's path state, chunk table and the
         // job records, and one pinned block for their host staging (each
         // allocation call costs about as much as a small job's render)
-        static const uint64_t kLaneArrays[] = {32, 16, 16, 16, 16, 16, 64, 8, 2, 32, 4, 4, 4, 4, 4};
+        static const uint64_t kLaneArrays[] = {32, 16, 16, 16, 16, 16, 32 * kNeeSlots, 4 * kNeeSlots, kNeeSlots,
+                                               16 * kNeeSlots, 4, 4, 4, 4, 4};
         uint64_t lane_bytes = 2 * align_up(64) + align_up((uint64_t)kMaxChunks * sizeof(GenChunk)) +
                               align_up((uint64_t)kMaxChunks * sizeof(AccGroup));
         for (uint64_t b : kLaneArrays) lane_bytes += align_up(P * b);
@@ -256,10 +257,11 @@ class Engine {
             st.tp = (float4 *)carve(P * 16);
             st.L = (float4 *)carve(P * 16);
             st.meta = (int4 *)carve(P * 16);
-            st.sray = (rg_ray *)carve(P * 64);
-            st.ssrc = (uint32_t *)carve(P * 8);
-            st.socc = (uint8_t *)carve(P * 2);
-            st.scontrib = (float4 *)carve(P * 32);
+            st.sray = (rg_ray *)carve(P * 32 * kNeeSlots);
+            st.slot_stride = (uint32_t)P;
+            st.ssrc = (uint32_t *)carve(P * 4 * kNeeSlots);
+            st.socc = (uint8_t *)carve(P * kNeeSlots);
+            st.scontrib = (float4 *)carve(P * 16 * kNeeSlots);
             st.nshadow = (uint32_t *)carve(P * 4);
             st.q[0] = (uint32_t *)carve(P * 4);
             st.q[1] = (uint32_t *)carve(P * 4);

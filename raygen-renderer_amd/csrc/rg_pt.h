@@ -63,6 +63,18 @@ struct DevLight {
     uint32_t obj, first, count, pad;
 };
 
+// Point light (tracePointLight, rayrenderer.cpp:2205-2269); same layout as
+// rg_point_light, spot_range replaced by the cone's cosine (0 with no cone).
+struct DevPointLight {
+    float pos[3], normal[3], color[3];
+    float emission, spot_cos;
+    int32_t spot;
+};
+static_assert(sizeof(DevPointLight) == sizeof(rg_point_light), "DevPointLight must mirror rg_point_light");
+
+// NEE shadow-ray slots per path event: area light, point light, environment.
+constexpr uint32_t kNeeSlots = 3;
+
 // Everything a shade kernel reads, passed by value.
 struct PTScene {
     const float *v9;
@@ -71,6 +83,8 @@ struct PTScene {
     const DevLight *lights;
     const uint32_t *light_tris;
     uint32_t nlights;
+    const DevPointLight *plights;
+    uint32_t nplights;
     const DevMedium *media;
     int32_t global_medium;
     // equirect environment
@@ -114,8 +128,10 @@ struct PTState {
     float4 *tp;         // BSDFParam::throughput (Russian roulette)
     float4 *L;          // radiance so far (without the last event's NEE), w = alive flag
     int4 *meta;         // passes, medium word (medium, dispersion band), next LDS dim, next Philox draw
-    rg_ray *sray;       // NEE shadow rays per (path, k), k < 2 (tmin = predicate)
-    uint32_t *ssrc;     // queue of the (path, k) pairs with a shadow ray: 2 * path + k
+    // NEE slots are slot-major: (path p, slot k) at k * slot_stride + p, so
+    // a wave's paths write and read one slot's records contiguously
+    rg_ray *sray;       // NEE shadow rays per (path, k), k < kNeeSlots (tmin = predicate)
+    uint32_t *ssrc;     // queue of the (path, k) pairs with a shadow ray: k * slot_stride + path
     uint8_t *socc;      // occluded flag per (path, k)
     float4 *scontrib;   // unoccluded contribution per (path, k)
     uint32_t *nshadow;  // shadow rays per path this event
@@ -124,6 +140,7 @@ struct PTState {
                         // [kExtendWork], [kShadowWork] the single-job traversals' work-sharing counters
     unsigned long long *stats; // [0] extension rays, [1] shadow rays, [2] path samples
     uint32_t sample_base;
+    uint32_t slot_stride;  // path capacity of the slot arrays
 };
 
 // Kernel-side scene view of one job: scene buffers, camera and settings.

@@ -150,8 +150,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         P.chroma = chroma_of(meta.y);
         P.bg = bg_of(meta.y);
         P.alive = true;
-        Shadows S;
-        S.n = 0;
+        Shadows S = shadows_of(st, p);
         const rg_hit h = st.hit[p];
         step(sc, R, P, S, h);
         st.ray[p] = {{P.o.x, P.o.y, P.o.z}, 0.0f, {P.d.x, P.d.y, P.d.z}, MAX_RAY_DISTANCE};
@@ -159,15 +158,15 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         st.tp[p] = v4(P.tp, 0.0f);
         st.L[p] = v4(P.L, P.alive ? 1.0f : 0.0f);
         st.meta[p] = make_int4(P.passes, pack_medium(P.medium, P.chroma, P.bg), R.dim, (int)R.k);
-        store_shadows(S, p, st.sray, st.scontrib);
         ns = (uint32_t)S.n;
         st.nshadow[p] = ns;
         alive = P.alive;
     }
     // the event's NEE rays go to the shadow-ray queue, one ray per lane there
-    for (uint32_t k = 0; k < 2; k++) {
+    const uint32_t slots = sc.nplights ? kNeeSlots : kNeeSlots - 1;  // the point-light slot only with point lights
+    for (uint32_t k = 0; k < slots; k++) {
         const uint32_t pos = block_slot(k < ns, st.count + 2, lds);
-        if (k < ns) st.ssrc[pos] = 2 * p + k;
+        if (k < ns) st.ssrc[pos] = k * st.slot_stride + p;
     }
     // live paths go to the next queue; a finished path keeps its radiance and
     // its last event's NEE rays, which the accumulation adds up
@@ -341,7 +340,7 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
     // persistent traversal grids: what the device holds at once, at most one
     // thread per possible ray
     const uint32_t ext_blocks = std::min<uint32_t>(blocks, resident_blocks<k_extend<kStack, kW>>(kPtBlock));
-    const uint32_t any_blocks = std::min<uint32_t>(2 * blocks, resident_blocks<k_anyhit<kStack, kW>>(kPtBlock));
+    const uint32_t any_blocks = std::min<uint32_t>(kNeeSlots * blocks, resident_blocks<k_anyhit<kStack, kW>>(kPtBlock));
     st.sample_base = sample_base;
     hipLaunchKernelGGL(k_begin_counts, dim3(1), dim3(1), 0, stream, st.count, d_tile_mask ? 0u : cap);
     hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, S, sample_base, d_tile_mask);

@@ -513,38 +513,30 @@ __device__ __forceinline__ bool ggx_setup(const DevObject &m, v3 alb, v3 n, v3 i
 // shadow-ray emission: NEE samples are drawn in the shade kernel and tested
 // in the shadow kernel; a contribution is added only when unoccluded
 // ---------------------------------------------------------------------------
-// Up to two NEE rays per event, in two named slots (no dynamically indexed
-// private array, which would live in scratch memory).
+// Up to kNeeSlots NEE rays per event, written to the path's slots
+// k * slot_stride + p as they are drawn (ray: tmin = predicate; contribution).
 struct Shadows {
     int n;
-    float4 o0, d0, c0;   // d.w = tmax, o.w = predicate
-    float4 o1, d1, c1;
+    uint32_t p, stride;
+    rg_ray *sray;
+    float4 *scontrib;
 };
 
-__device__ __forceinline__ void push_shadow(Shadows &S, v3 o, v3 d, float tmax, int pred, v3 contrib) {
-    const float4 o4 = v4(o, (float)pred), d4 = v4(d, tmax), c4 = v4(contrib, 0.0f);
-    if (S.n == 0) {
-        S.o0 = o4;
-        S.d0 = d4;
-        S.c0 = c4;
-    } else {
-        S.o1 = o4;
-        S.d1 = d4;
-        S.c1 = c4;
-    }
-    S.n++;
+__device__ __forceinline__ Shadows shadows_of(const PTState &st, uint32_t p) {
+    Shadows S;
+    S.n = 0;
+    S.p = p;
+    S.stride = st.slot_stride;
+    S.sray = st.sray;
+    S.scontrib = st.scontrib;
+    return S;
 }
 
-// Stores the event's NEE rays and contributions at slots 2·p and 2·p + 1.
-__device__ __forceinline__ void store_shadows(const Shadows &S, uint32_t p, rg_ray *sray, float4 *scontrib) {
-    if (S.n > 0) {
-        sray[2 * p] = {{S.o0.x, S.o0.y, S.o0.z}, S.o0.w, {S.d0.x, S.d0.y, S.d0.z}, S.d0.w};
-        scontrib[2 * p] = S.c0;
-    }
-    if (S.n > 1) {
-        sray[2 * p + 1] = {{S.o1.x, S.o1.y, S.o1.z}, S.o1.w, {S.d1.x, S.d1.y, S.d1.z}, S.d1.w};
-        scontrib[2 * p + 1] = S.c1;
-    }
+__device__ __forceinline__ void push_shadow(Shadows &S, v3 o, v3 d, float tmax, int pred, v3 contrib) {
+    const uint32_t k = (uint32_t)S.n * S.stride + S.p;
+    S.sray[k] = {{o.x, o.y, o.z}, (float)pred, {d.x, d.y, d.z}, tmax};
+    S.scontrib[k] = v4(contrib, 0.0f);
+    S.n++;
 }
 
 struct AreaSample {
@@ -593,6 +585,50 @@ __device__ __forceinline__ v3 area_value(const AreaSample &s) {
     const float pl2 = pl * pl, pb2 = pb * pb;
     const float w = pl2 / (pl2 + pb2);
     return scl(scl(s.Le, s.co * s.cl * s.sa / (PI_F * s.r2)), w);
+}
+
+// tracePointLight (rayrenderer.cpp:2205-2269) up to the shadow test: the
+// shadow ray toward the light (unnormalised, tmax 0.99999, the environment
+// rays' occluders) and the light's value when unoccluded, color x emission x
+// dotToLight / |lightray|^2 clamped to [0, 1] per channel; ok = false when
+// that value is zero whatever the occlusion.
+struct PointSample {
+    bool ok;
+    v3 o, lr, val;
+};
+
+__device__ __forceinline__ float smoothstep01(float e0, float e1, float x) {
+    float t = (x - e0) / (e1 - e0);
+    t = fminf(fmaxf(t, 0.0f), 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+// traceLight's point-light part (rayrenderer.cpp:2281-2290): the only light,
+// or one picked uniformly (a rand() draw)
+__device__ __forceinline__ PointSample sample_point(const PTScene &sc, Rng &R, v3 hit, v3 n) {
+    PointSample s;
+    s.ok = false;
+    if (sc.nplights == 0) return s;
+    const uint32_t li = sc.nplights == 1 ? 0u : pick(prng(R), sc.nplights);
+    const DevPointLight L = sc.plights[li];
+    const v3 lr = sub(mk(L.pos[0], L.pos[1], L.pos[2]), hit);
+    const v3 dir = nrm(lr);
+    const float dto = dot(dir, n);
+    if (!(dto > 0.0f)) return s;
+    float dtl = dot(dir, mk(L.normal[0], L.normal[1], L.normal[2]));
+    if (L.spot) dtl = dtl * smoothstep01(fmaxf(L.spot_cos - 0.1f, 0.0f), fminf(L.spot_cos + 0.1f, 1.0f), dtl);
+    else dtl = fabsf(dto);
+    if (!(dtl > 0.0f)) return s;
+    const float da = powf(len(lr), -2.0f);
+    const float lum = L.emission * dtl * da;
+    const v3 c = mk(fminf(fmaxf(L.color[0] * lum, 0.0f), 1.0f), fminf(fmaxf(L.color[1] * lum, 0.0f), 1.0f),
+                    fminf(fmaxf(L.color[2] * lum, 0.0f), 1.0f));
+    if (is_zero(c)) return s;
+    s.ok = true;
+    s.o = surface_origin(hit, lr, n);
+    s.lr = lr;
+    s.val = c;
+    return s;
 }
 
 struct EnvSample {
@@ -694,8 +730,11 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
         const v3 dir = cosine_dir(n, r1, r2);
         const float pdf = fmaxf(0.0f, dot(dir, n)) * (float)(1.0 / M_PI);
         const v3 ba = mul(p.beta, alb);
+        // traceLight (area, then point lights), then traceEnvmapLight
         const AreaSample as = sample_area(sc, R, hit, n);
         if (as.ok) push_shadow(S, as.o, as.lr, 0.99999f, RG_PRED_AREA, mul(ba, area_value(as)));
+        const PointSample ps = sample_point(sc, R, hit, n);
+        if (ps.ok) push_shadow(S, ps.o, ps.lr, 0.99999f, RG_PRED_ENV, mul(ba, ps.val));
         const EnvSample es = sample_env_nee(sc, R, hit, n);
         if (es.ok) {
             float w = 1.0f;
@@ -859,9 +898,11 @@ __device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p,
             // the light term times colour x texture, zero without colour
             // sampling (rayrenderer.cpp:2839-2856)
             if (1.0f - m.glossy - m.refraction > 0.00001f && sc.color_sampling) {
+                const v3 bc = mul(p.beta, mul(ld3(m.color), tex));
                 const AreaSample as = sample_area(sc, R, hit, n);
-                if (as.ok) push_shadow(S, as.o, as.lr, 0.99999f, RG_PRED_AREA,
-                                       mul(mul(p.beta, mul(ld3(m.color), tex)), area_value(as)));
+                if (as.ok) push_shadow(S, as.o, as.lr, 0.99999f, RG_PRED_AREA, mul(bc, area_value(as)));
+                const PointSample ps = sample_point(sc, R, hit, n);
+                if (ps.ok) push_shadow(S, ps.o, ps.lr, 0.99999f, RG_PRED_ENV, mul(bc, ps.val));
             }
             p.alive = false;
             return;
@@ -1076,8 +1117,8 @@ __device__ __forceinline__ v3 with_last_nee(const PTState &st, uint32_t p) {
     float4 L = st.L[p];
     const uint32_t ns = st.nshadow[p];
     for (uint32_t k = 0; k < ns; k++) {
-        if (!st.socc[2 * p + k]) {
-            const float4 c = st.scontrib[2 * p + k];
+        if (!st.socc[k * st.slot_stride + p]) {
+            const float4 c = st.scontrib[k * st.slot_stride + p];
             L.x = L.x + c.x;
             L.y = L.y + c.y;
             L.z = L.z + c.z;

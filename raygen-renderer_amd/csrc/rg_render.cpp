@@ -40,7 +40,7 @@ inline float vlen(V a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
 void free_pt_scene(Scene *s) {
     for (void **p : {&s->d_v9, &s->d_shade, &s->d_objs, &s->d_lights, &s->d_light_tris, &s->d_media,
                      &s->d_env_rgb, &s->d_env_marg, &s->d_env_cond, &s->d_texels, &s->d_tex_desc, &s->d_otex,
-                     &s->d_uv6}) {
+                     &s->d_uv6, &s->d_plights}) {
         dev_free(*p);
         *p = nullptr;
     }
@@ -133,6 +133,20 @@ int upload_pt_scene(Scene *s, void *stream) {
         }
     }
     s->nlights = (uint32_t)lights.size();
+    // point lights: the spot cone's cosine, cosf(RADIAN_TO_DEGREE(spotRange
+    // / 2)) as the reference computes it (rayrenderer.cpp:2230), with
+    // RADIAN_TO_DEGREE(x) = x * 180 / pi
+    std::vector<DevPointLight> plights(s->point_lights.size());
+    for (size_t i = 0; i < plights.size(); i++) {
+        const rg_point_light &l = s->point_lights[i];
+        DevPointLight &q = plights[i];
+        std::memcpy(q.pos, l.pos, sizeof q.pos);
+        std::memcpy(q.normal, l.normal, sizeof q.normal);
+        std::memcpy(q.color, l.color, sizeof q.color);
+        q.emission = l.emission;
+        q.spot = l.spot_range > 0.0f ? 1 : 0;
+        q.spot_cos = q.spot ? cosf(l.spot_range * 0.5f * (float)(180.0 / M_PI)) : 0.0f;
+    }
     // texture tables (kept empty when no object has a texture)
     std::vector<int32_t> otex;
     std::vector<uint32_t> tex_desc;
@@ -162,6 +176,7 @@ int upload_pt_scene(Scene *s, void *stream) {
     if (!rc) rc = upload_vec(&s->d_objs, s->objects, stream);
     if (!rc) rc = upload_vec(&s->d_lights, lights, stream);
     if (!rc) rc = upload_vec(&s->d_light_tris, light_tris, stream);
+    if (!rc) rc = upload_vec(&s->d_plights, plights, stream);
     if (!rc) rc = upload_vec(&s->d_media, media, stream);
     if (!rc) rc = upload_vec(&s->d_env_rgb, s->env_rgb, stream);
     if (!rc) rc = upload_vec(&s->d_env_marg, marg, stream);
@@ -201,8 +216,8 @@ int ensure_buffers(PTBuffers &b, uint64_t cap, uint64_t npix, uint64_t ntiles) {
         uint64_t bytes;
     } allocs[] = {
         {&b.ray, cap * 32}, {&b.hit, cap * 16}, {&b.beta, cap * 16}, {&b.tp, cap * 16}, {&b.L, cap * 16},
-        {&b.meta, cap * 16}, {&b.sray, cap * 64}, {&b.ssrc, cap * 8}, {&b.socc, cap * 2},
-        {&b.scontrib, cap * 32}, {&b.nshadow, cap * 4},
+        {&b.meta, cap * 16}, {&b.sray, cap * 32 * kNeeSlots}, {&b.ssrc, cap * 4 * kNeeSlots},
+        {&b.socc, cap * kNeeSlots}, {&b.scontrib, cap * 16 * kNeeSlots}, {&b.nshadow, cap * 4},
         {&b.q0, cap * 4}, {&b.q1, cap * 4}, {&b.count, 64}, {&b.sum, npix * 12},
         {&b.sumsq, npix * 12}, {&b.hdr, npix * 16}, {&b.tile_mask, ntiles}, {&b.tile_nspp, ntiles * 4},
         {&b.tile_noise, ntiles * 4}, {&b.n_active, 64}, {&b.aov, npix * 36}, {&b.dn_a, npix * 16},
@@ -232,6 +247,8 @@ PTScene make_ptscene(const Scene &scn, const rg_settings &st) {
     sc.lights = (const DevLight *)s->d_lights;
     sc.light_tris = (const uint32_t *)s->d_light_tris;
     sc.nlights = s->nlights;
+    sc.plights = (const DevPointLight *)s->d_plights;
+    sc.nplights = (uint32_t)s->point_lights.size();
     sc.media = (const DevMedium *)s->d_media;
     sc.global_medium = s->global_medium;
     sc.env_rgb = (const float *)s->d_env_rgb;
@@ -479,6 +496,20 @@ int rg_scene_set_env(rg_scene *scene, const rg_env *env) {
     return RG_OK;
 }
 
+int rg_scene_set_point_lights(rg_scene *scene, const rg_point_light *lights, uint32_t n) {
+    Scene *s = reinterpret_cast<Scene *>(scene);
+    if (!s) return set_error(RG_E_INVALID, "null scene");
+    if (n && !lights) return set_error(RG_E_INVALID, "lights is null");
+    for (uint32_t i = 0; i < n; i++)
+        for (float v : {lights[i].pos[0], lights[i].pos[1], lights[i].pos[2], lights[i].normal[0],
+                        lights[i].normal[1], lights[i].normal[2], lights[i].color[0], lights[i].color[1],
+                        lights[i].color[2], lights[i].emission, lights[i].spot_range})
+            if (!std::isfinite(v)) return set_error(RG_E_INVALID, "point light " + std::to_string(i) + " not finite");
+    s->point_lights.assign(lights, lights + n);
+    s->pt_dirty = true;
+    return RG_OK;
+}
+
 int rg_scene_set_camera(rg_scene *scene, const rg_camera *cam) {
     Scene *s = reinterpret_cast<Scene *>(scene);
     if (!s || !cam) return set_error(RG_E_INVALID, "null argument");
@@ -522,7 +553,8 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     }
     const uint64_t slots = (uint64_t)npix * std::max<uint32_t>(S, st.adaptive ? window_max *
                                                                  std::max<uint32_t>(1, st.adaptive_base) : S);
-    if (slots > (1ull << 31)) return set_error(RG_E_BUDGET, "too many samples in flight");
+    // the NEE slot index k * slots + p is 32-bit
+    if (slots * kNeeSlots > 0xFFFFFFFFull) return set_error(RG_E_BUDGET, "too many samples in flight");
     if (int rc = ensure_buffers(c->pt, slots, npix, ntiles)) return rc;
     PTBuffers &B = c->pt;
 
@@ -537,6 +569,7 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     ps.L = (float4 *)B.L;
     ps.meta = (int4 *)B.meta;
     ps.sray = (rg_ray *)B.sray;
+    ps.slot_stride = (uint32_t)B.cap;
     ps.ssrc = (uint32_t *)B.ssrc;
     ps.socc = (uint8_t *)B.socc;
     ps.scontrib = (float4 *)B.scontrib;

@@ -672,6 +672,7 @@ struct Mat {
     rg_object o;
     std::string tex;
     float tiling[2] = {1.0f, 1.0f};
+    float spot_range = 0.0f;  // Material::spotRange (material.h:52), read by point lights
     Mat() { rg_object_default(&o); }
 };
 
@@ -761,10 +762,8 @@ struct Loader {
                 num(*tt, "y", mt.tiling[1]);
             }
         }
+        num(o, "spotRange", mt.spot_range);
         // read by the reference but outside the -d 5 path kept here
-        float sr = 0.0f;
-        num(o, "spotRange", sr);
-        if (sr > 0.0f) warnings.push_back("spot range ignored");
         if (o.get("normalmap")) warnings.push_back("normal map ignored");
     }
 
@@ -979,6 +978,10 @@ struct Loader {
         if (!meshes.empty()) {
             const uint32_t oid = add_object(mat);
             pending.push_back({model, oid, std::move(meshes)});
+        } else if (mat.o.emission > 0.0f) {
+            // an emissive object without meshes is a point light source
+            // (rayrenderer.cpp:333-353)
+            pending_lights.push_back({model, mat, {ang[0], ang[1], ang[2]}});
         }
         for (Child &c : children) {
             if (c.json) {
@@ -1008,10 +1011,46 @@ struct Loader {
         std::vector<MeshData> meshes;
     };
     std::vector<Pending> pending;
+    struct PendingLight {
+        Affine model;
+        Mat mat;
+        double angle[3];
+    };
+    std::vector<PendingLight> pending_lights;
+    std::vector<rg_point_light> point_lights;
 
     // transformObject (rayrenderer.cpp:237-363): render-space vertices and
     // normalised normals, once the camera is known
     void flatten() {
+        // point lights (rayrenderer.cpp:339-352): the origin through the
+        // view-model matrix; the direction from the object's angle.x / .y
+        // read through RADIAN_TO_DEGREE (x * 180 / pi) in binary32, then the
+        // normal matrix (not normalised again, as there)
+        for (PendingLight &pl : pending_lights) {
+            const Affine vm = view.then_after(pl.model);
+            rg_point_light L;
+            std::memset(&L, 0, sizeof L);
+            const float zero[3] = {0.0f, 0.0f, 0.0f};
+            double q[3];
+            vm.apply(zero, q);
+            const float r2d = (float)(180.0 / M_PI);
+            const float ax = (float)pl.angle[0] * r2d, ay = (float)pl.angle[1] * r2d;
+            const float s1 = sinf(ax), c1 = cosf(ax), s2 = sinf(ay), c2 = cosf(ay);
+            float d[3] = {c2 * s1, c2 * c1, s2};
+            const float dl = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            const float inv = dl > 0.0f ? 1.0f / dl : 0.0f;
+            for (float &x : d) x *= inv;
+            double nm[3][3];
+            const bool ok = vm.normal_matrix(nm);
+            for (int c = 0; c < 3; c++) {
+                L.pos[c] = (float)q[c];
+                L.normal[c] = ok ? (float)(nm[c][0] * d[0] + nm[c][1] * d[1] + nm[c][2] * d[2]) : d[c];
+                L.color[c] = pl.mat.o.color[c];
+            }
+            L.emission = pl.mat.o.emission;
+            L.spot_range = pl.mat.spot_range;
+            point_lights.push_back(L);
+        }
         for (Pending &p : pending) {
             const Affine vm = view.then_after(p.model);
             double nm[3][3];
@@ -1069,6 +1108,8 @@ extern "C" int rg_scene_load_json(rg_ctx *ctx, const char *path, rg_scene **out)
     int rc = rg_scene_set_media(sc, L.media.data(), (uint32_t)L.media.size(), L.global_medium);
     if (!rc) rc = rg_scene_set_shading(sc, L.n9.data(), L.obj_of_tri.data(), L.objects.data(), (uint32_t)L.objects.size());
     if (!rc) rc = rg_scene_set_camera(sc, &L.camera);
+    if (!rc && !L.point_lights.empty())
+        rc = rg_scene_set_point_lights(sc, L.point_lights.data(), (uint32_t)L.point_lights.size());
     if (!rc && !L.env_path.empty()) {
         const size_t dot = L.env_path.rfind('.');
         const std::string ext = dot == std::string::npos ? "" : L.env_path.substr(dot);
@@ -1170,6 +1211,15 @@ extern "C" int rg_scene_export(const rg_scene *scene, float *v9, float *n9, uint
     if (obj_of_tri && !s->obj_of_tri.empty())
         std::memcpy(obj_of_tri, s->obj_of_tri.data(), s->obj_of_tri.size() * sizeof(uint32_t));
     if (objs && !s->objects.empty()) std::memcpy(objs, s->objects.data(), s->objects.size() * sizeof(rg_object));
+    return RG_OK;
+}
+
+extern "C" int rg_scene_export_point_lights(const rg_scene *scene, rg_point_light *lights, uint32_t *n) {
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (!s || !n) return set_error(RG_E_INVALID, "null argument");
+    *n = (uint32_t)s->point_lights.size();
+    if (lights && !s->point_lights.empty())
+        std::memcpy(lights, s->point_lights.data(), s->point_lights.size() * sizeof(rg_point_light));
     return RG_OK;
 }
 

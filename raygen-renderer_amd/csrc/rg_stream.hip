@@ -189,8 +189,7 @@ __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk 
     P.chroma = chroma_of(meta.y);
     P.bg = bg_of(meta.y);
     P.alive = true;
-    Shadows S;
-    S.n = 0;
+    Shadows S = shadows_of(st, p);
     const rg_hit h = st.hit[p];
     step(sc, R, P, S, h);
     st.ray[p] = {{P.o.x, P.o.y, P.o.z}, 0.0f, {P.d.x, P.d.y, P.d.z}, MAX_RAY_DISTANCE};
@@ -198,7 +197,6 @@ __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk 
     st.tp[p] = v4(P.tp, 0.0f);
     st.L[p] = v4(P.L, P.alive ? 1.0f : 0.0f);
     st.meta[p] = make_int4(P.passes, pack_medium(P.medium, P.chroma, P.bg), R.dim, (int)R.k);
-    store_shadows(S, p, st.sray, st.scontrib);
     st.nshadow[p] = (uint32_t)S.n;
     alive = P.alive && bounce + 1 < C.bounces;
     return (uint32_t)S.n;
@@ -208,9 +206,9 @@ __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk 
 // a live path to the next path queue.  Every thread of the block calls it.
 __device__ __forceinline__ void queue_results(const PTState &st, uint32_t p, uint32_t ns, bool alive, int qi,
                                               uint32_t *lds) {
-    for (uint32_t k = 0; k < 2; k++) {
+    for (uint32_t k = 0; k < kNeeSlots; k++) {
         const uint32_t pos = block_slot(k < ns, st.count + 2, lds);
-        if (k < ns) st.ssrc[pos] = 2 * p + k;
+        if (k < ns) st.ssrc[pos] = k * st.slot_stride + p;
     }
     const uint32_t pos = block_slot(alive, st.count + (qi ^ 1), lds);
     if (alive) st.q[qi ^ 1][pos] = p;

@@ -79,6 +79,8 @@ def lib():
             "rg_scene_set_shading": (i32, [vp, vp, vp, vp, u32]),
             "rg_scene_set_media": (i32, [vp, vp, u32, i32]),
             "rg_scene_set_env": (i32, [vp, vp]),
+            "rg_scene_set_point_lights": (i32, [vp, vp, u32]),
+            "rg_scene_export_point_lights": (i32, [vp, vp, vp]),
             "rg_scene_set_camera": (i32, [vp, vp]),
             "rg_render": (i32, [vp, vp, vp, vp, vp, vp, vp]),
             "rg_render_stats": (i32, [vp, vp]),
@@ -353,6 +355,12 @@ class RenderScene(Scene):
             uv6 = np.ascontiguousarray(sc["uv6"], np.float32)
             self._keep += [rgbs, tab, too, til, uv6]
             check(lib().rg_scene_set_textures(self.h, tab, len(texs), _ptr(too), _ptr(til), _ptr(uv6)))
+        pl = sc.get("point_lights")
+        if pl is not None and len(pl):
+            from scenes import POINT_LIGHT_DTYPE
+            pla = np.ascontiguousarray(pl, dtype=POINT_LIGHT_DTYPE)
+            self._keep.append(pla)
+            check(lib().rg_scene_set_point_lights(self.h, _ptr(pla), len(pla)))
         cam = make_camera(sc["camera"])
         check(lib().rg_scene_set_camera(self.h, ctypes.byref(cam)))
         self.build(width=width)
@@ -388,6 +396,13 @@ def load_json(ctx, path):
          "global_medium": gm.value, "warnings": [w for w in buf.value.decode().split("\n") if w]}
     if env.w and env.h:
         d["env"] = {"rgb": env_rgb, "intensity": env.intensity, "rotation_deg": env.rotation_deg}
+    npl = ctypes.c_uint32()
+    check(lib().rg_scene_export_point_lights(h, None, ctypes.byref(npl)))
+    if npl.value:
+        from scenes import POINT_LIGHT_DTYPE
+        pla = np.zeros(npl.value, POINT_LIGHT_DTYPE)
+        check(lib().rg_scene_export_point_lights(h, _ptr(pla), ctypes.byref(npl)))
+        d["point_lights"] = pla
     ntex = ctypes.c_uint32()
     check(lib().rg_scene_export_textures(h, ctypes.byref(ntex), None, None, None, None, None))
     if ntex.value:

@@ -16,6 +16,21 @@ OBJECT_DTYPE = np.dtype([("color", "<f4", (3,)), ("glossy", "<f4"), ("roughness"
                          ("chroma_dispersion", "<f4"), ("pad", "<u4")])
 MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", (3,)), ("sigma_s", "<f4", (3,)), ("sigma_e", "<f4", (3,)),
                          ("g", "<f4"), ("density", "<f4"), ("pad", "<f4")])
+# rg_point_light: a point light source in render space (an emissive object
+# without meshes, rayrenderer.cpp:333-353)
+POINT_LIGHT_DTYPE = np.dtype([("pos", "<f4", (3,)), ("normal", "<f4", (3,)), ("color", "<f4", (3,)),
+                              ("emission", "<f4"), ("spot_range", "<f4"), ("pad", "<u4")])
+
+
+def point_light(pos, color=(1.0, 1.0, 1.0), emission=1.0, normal=(0.0, -1.0, 0.0), spot_range=0.0):
+    """One point light record (spot_range 0 = omni)."""
+    L = np.zeros(1, POINT_LIGHT_DTYPE)[0]
+    L["pos"] = pos
+    L["normal"] = normal
+    L["color"] = color
+    L["emission"] = emission
+    L["spot_range"] = spot_range
+    return L
 
 
 def material(color=(0.8, 0.8, 0.8), glossy=0.0, roughness=0.5, transparency=0.0, refraction=0.0,
@@ -108,7 +123,7 @@ class Builder:
         self.o.append(np.full(len(idx), obj, np.uint32))
         self.uv.append(np.zeros((len(idx), 6), np.float32))  # uvCount 0: no UVs (mesh.h:92)
 
-    def finish(self, camera, media=None, global_medium=-1, env=None):
+    def finish(self, camera, media=None, global_medium=-1, env=None, point_lights=None):
         v = [np.atleast_2d(x) for x in self.v]
         n = [np.atleast_2d(x) for x in self.n]
         o = [np.atleast_1d(np.asarray(x, np.uint32)) for x in self.o]
@@ -123,6 +138,8 @@ class Builder:
         }
         if env is not None:
             scene["env"] = env
+        if point_lights is not None and len(point_lights):
+            scene["point_lights"] = np.array(point_lights, dtype=POINT_LIGHT_DTYPE)
         if self.textures:
             scene["textures"] = self.textures
             scene["tex_of_obj"] = np.asarray(self.tex_of_obj, np.int32)
@@ -369,3 +386,34 @@ def dispersive_window(stacks=24, slices=32):
                                   chroma_dispersion=0.15))
     b.sphere(glass, (0, 0, -5), 1.8, stacks, slices)
     return b.finish(camera={"fov_deg": 40.0, "aperture": 1.8}, env=banded_env())
+
+
+def point_lit_room(spot_range=14.0, env_value=0.0):
+    """The cube room lit by its ceiling quad and two point lights of
+    traceLight (rayrenderer.cpp:2205-2290): an omni light near the back wall
+    and a spot above the short box aimed down (spot cosine read through
+    RADIAN_TO_DEGREE, as the reference does); env_value > 0 adds a constant
+    environment, so a diffuse event casts all three NEE rays (area, point,
+    environment)."""
+    sc = cube_room()
+    sc["point_lights"] = np.array([
+        point_light((-0.5, 0.5, -4.3), color=(1.0, 0.8, 0.6), emission=0.8),
+        point_light((0.35, 0.6, -3.2), color=(0.6, 0.7, 1.0), emission=0.6, normal=(0.0, 1.0, 0.0),
+                    spot_range=spot_range),
+    ], dtype=POINT_LIGHT_DTYPE)
+    if env_value > 0.0:
+        sc["env"] = constant_env(env_value)
+    return sc
+
+
+def point_light_plane(height=1.0, emission=2.0, color=(1.0, 0.5, 0.25), albedo=(0.8, 0.6, 0.4)):
+    """A diffuse plane y = -1 under one omni point light at (0, -1 + height,
+    -3), nothing else: with no environment and no other surface, a pixel
+    that sees the plane at p with normal n = +Y has, at depth 1 or more,
+    the radiance albedo x clamp(color x emission x cos / r^2) (the point
+    light's term of traceLight multiplied by the diffuse albedo)."""
+    b = Builder()
+    pl = b.add_object(material(color=albedo))
+    b.quad(pl, (-4, -1.0, -1), (4, -1.0, -1), (4, -1.0, -7), (-4, -1.0, -7), normal=(0, 1, 0))
+    return b.finish(camera={"fov_deg": 60.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0},
+                    point_lights=[point_light((0.0, -1.0 + height, -3.0), color=color, emission=emission)])
