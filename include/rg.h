@@ -78,6 +78,7 @@ typedef struct {
 #define RG_PRED_ALL 0  /* every triangle occludes (microbenchmark)          */
 #define RG_PRED_AREA 1 /* transparency<0.01 && emission<=0 (rayrenderer.cpp:1956-1959) */
 #define RG_PRED_ENV 2  /* !(emission>0) && (transparency<0.01 || refraction>0.1) (:1499-1503; point lights :2211-2215) */
+#define RG_PRED_VOLUME 3 /* !(emission>0) && transparency<0.01 && refraction<=0.1 (volume lights, :2172-2176) */
 
 /* ---- context ------------------------------------------------------------ */
 /* device >= 0: HIP device ordinal.  device == -1: host-only context (scene
@@ -168,11 +169,57 @@ typedef struct {
     uint32_t pad;
 } rg_object;
 
-/* HomogeneousMedium authoring fields (medium.h:37-130), constant emission. */
+/* One control point of a path-emission tube (HomogeneousMedium::PathSample,
+ * medium.h:54-62): position (render space), tube radius, axial parameter. */
+typedef struct {
+    float p[3];
+    float radius, t;
+} rg_path_point;
+#define RG_MAX_PATH_POINTS 16
+
+/* HomogeneousMedium authoring fields (medium.h:37-214).  Positions and axes
+ * (cone_origin, cone_axis, path points) are in render space: the loader
+ * bakes them as HomogeneousMedium::bake does (medium.cpp:59-104).
+ * rg_medium_default gives the reference's defaults. */
 typedef struct {
     float sigma_a[3], sigma_s[3], sigma_e[3];
-    float g, density, pad;
+    float g, density;
+    /* emission profile (EmissionMode): 0 constant sigma_e, 1 cone, 2 path */
+    int32_t emission_mode;
+    float cone_origin[3], cone_axis[3];
+    float cone_length, cone_radius;
+    float cone_inner[3], cone_outer[3];
+    float cone_intensity, cone_peak_axial, cone_peak_sharpness;
+    int32_t cone_samples;                 /* coneEmissionSamples */
+    rg_path_point path[RG_MAX_PATH_POINTS];
+    uint32_t n_path;
+    float path_inner[3], path_outer[3];
+    float path_intensity, path_falloff_power;
+    int32_t path_samples;                 /* pathEmissionSamples */
+    /* density field (DensityFieldMode): 0 none, 1 fBm value noise */
+    int32_t density_field;
+    float noise_frequency;
+    int32_t noise_octaves;
+    float noise_gain, noise_lacunarity, noise_amplitude, noise_bias;
+    float noise_offset[3];
+    /* heat haze: the ray bends by the gradient of n = 1 + amplitude * fbm */
+    int32_t heat_haze;
+    float ior_amplitude, ior_frequency;
+    int32_t ior_octaves;
+    float ior_gain, ior_lacunarity;
+    int32_t ior_march_steps;
+    float ior_offset[3], ior_falloff;
 } rg_medium;
+
+/* An emissive volume that NEE samples (RayRenderer's
+ * emissiveVolumeSources, rayrenderer.cpp:180-206): a medium with a cone
+ * profile of positive intensity, or constant non-zero emission, inside an
+ * object; pos = that object's own location through the view matrix (the
+ * point a constant-emission volume is sampled at, :2089-2096). */
+typedef struct {
+    uint32_t medium;
+    float pos[3];
+} rg_volume_light;
 
 /* Camera (scene.h Camera): render space has the eye at the origin looking
  * down -Z, as after RayRenderer's view transform (rayrenderer.cpp:247). */
@@ -247,6 +294,7 @@ typedef struct {
 } rg_texture;
 
 void rg_object_default(rg_object *obj);
+void rg_medium_default(rg_medium *m);
 void rg_settings_default(rg_settings *s);
 
 /* Per-triangle shading inputs of RayRenderer::transformObject
@@ -264,8 +312,17 @@ int rg_scene_set_shading(rg_scene *scene, const float *n9, const uint32_t *obj_o
 int rg_scene_set_textures(rg_scene *scene, const rg_texture *tex, uint32_t n_tex, const int32_t *tex_of_obj,
                           const float *tiling_of_obj, const float *uv6);
 /* Media referenced by rg_object.interior_medium; global_medium = the
- * scene's globalMedium (-1 none).  At most 32767 media. */
+ * scene's globalMedium (-1 none).  At most 32767 media.  A heat-haze global
+ * medium is refused (RG_E_UNSUPPORTED): there the reference's tracePath
+ * recurses without end (rayrenderer.cpp:1645-1663 with no BSDFParam). */
 int rg_scene_set_media(rg_scene *scene, const rg_medium *media, uint32_t n, int32_t global_medium);
+/* The scene's emissive volumes for NEE (the diffuse and glossy lobes and
+ * in-medium scattering, bsdf.cpp:92-103, 318-329; rayrenderer.cpp:1756-1768):
+ * one picked uniformly per call, its shadow ray blocked by RG_PRED_VOLUME
+ * triangles.  n == 0 removes them. */
+int rg_scene_set_volume_lights(rg_scene *scene, const rg_volume_light *v, uint32_t n);
+/* Copies them out: n first, then v (n records) where non-NULL. */
+int rg_scene_export_volume_lights(const rg_scene *scene, rg_volume_light *v, uint32_t *n);
 /* env NULL or w == 0 removes the environment. */
 int rg_scene_set_env(rg_scene *scene, const rg_env *env);
 /* Point light sources of RayRenderer::traceLight (rayrenderer.cpp:2205-2290;

@@ -177,7 +177,21 @@ OBJECT_DTYPE = np.dtype([("color", "<f4", (3,)), ("glossy", "<f4"), ("roughness"
 POINT_LIGHT_DTYPE = np.dtype([("pos", "<f4", (3,)), ("normal", "<f4", (3,)), ("color", "<f4", (3,)),
                               ("emission", "<f4"), ("spot_range", "<f4"), ("pad", "<u4")])
 MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", (3,)), ("sigma_s", "<f4", (3,)), ("sigma_e", "<f4", (3,)),
-                         ("g", "<f4"), ("density", "<f4"), ("pad", "<f4")])
+                         ("g", "<f4"), ("density", "<f4"),
+                         ("emission_mode", "<i4"), ("cone_origin", "<f4", (3,)), ("cone_axis", "<f4", (3,)),
+                         ("cone_length", "<f4"), ("cone_radius", "<f4"), ("cone_inner", "<f4", (3,)),
+                         ("cone_outer", "<f4", (3,)), ("cone_intensity", "<f4"), ("cone_peak_axial", "<f4"),
+                         ("cone_peak_sharpness", "<f4"), ("cone_samples", "<i4"),
+                         ("path", [("p", "<f4", (3,)), ("radius", "<f4"), ("t", "<f4")], (16,)),
+                         ("n_path", "<u4"), ("path_inner", "<f4", (3,)), ("path_outer", "<f4", (3,)),
+                         ("path_intensity", "<f4"), ("path_falloff_power", "<f4"), ("path_samples", "<i4"),
+                         ("density_field", "<i4"), ("noise_frequency", "<f4"), ("noise_octaves", "<i4"),
+                         ("noise_gain", "<f4"), ("noise_lacunarity", "<f4"), ("noise_amplitude", "<f4"),
+                         ("noise_bias", "<f4"), ("noise_offset", "<f4", (3,)),
+                         ("heat_haze", "<i4"), ("ior_amplitude", "<f4"), ("ior_frequency", "<f4"),
+                         ("ior_octaves", "<i4"), ("ior_gain", "<f4"), ("ior_lacunarity", "<f4"),
+                         ("ior_march_steps", "<i4"), ("ior_offset", "<f4", (3,)), ("ior_falloff", "<f4")])
+VOLUME_LIGHT_DTYPE = np.dtype([("medium", "<u4"), ("pos", "<f4", (3,))])
 
 
 class _Cam(ctypes.Structure):
@@ -202,7 +216,8 @@ class _Desc(ctypes.Structure):
                 ("media", ctypes.c_void_p), ("nmedia", ctypes.c_uint32), ("global_medium", ctypes.c_int32),
                 ("env", _Env), ("cam", _Cam), ("tex", ctypes.c_void_p), ("ntex", ctypes.c_uint32),
                 ("tex_of_obj", ctypes.c_void_p), ("tiling", ctypes.c_void_p), ("uv6", ctypes.c_void_p),
-                ("plights", ctypes.c_void_p), ("nplights", ctypes.c_uint32)]
+                ("plights", ctypes.c_void_p), ("nplights", ctypes.c_uint32),
+                ("vlights", ctypes.c_void_p), ("nvlights", ctypes.c_uint32)]
 
 
 class _Settings(ctypes.Structure):
@@ -263,6 +278,12 @@ def _desc(scene, keep):
         d.tex_of_obj = too.ctypes.data
         d.tiling = til.ctypes.data
         d.uv6 = uv6.ctypes.data
+    vl = scene.get("volume_lights")
+    if vl is not None and len(vl):
+        vla = np.ascontiguousarray(vl, dtype=VOLUME_LIGHT_DTYPE)
+        keep.append(vla)
+        d.vlights = vla.ctypes.data
+        d.nvlights = len(vla)
     pl = scene.get("point_lights")
     if pl is not None and len(pl):
         pla = np.ascontiguousarray(pl, dtype=POINT_LIGHT_DTYPE)
@@ -422,3 +443,65 @@ def tile_noise(sum_rgb, sumsq_rgb, tx, ty, n):
     b = np.ascontiguousarray(sumsq_rgb, np.float32)
     h, w = a.shape[:2]
     return float(L.or_tile_noise(_p(a), _p(b), w, h, tx, ty, n))
+
+
+def _medium_arg(med):
+    m = np.ascontiguousarray(np.atleast_1d(med)[:1], dtype=MEDIUM_DTYPE)
+    return m, _p(m)
+
+
+def medium_eval(med, op, pts):
+    """or_medium_eval: op "density" (densityAt), "emission" (emissionAt,
+    RGB) or "ior" (iorAt) of one medium record at render-space points."""
+    L = lib()
+    L.or_medium_eval.restype = None
+    L.or_medium_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    m, mp = _medium_arg(med)
+    pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    code = {"density": 0, "emission": 1, "ior": 2}[op]
+    out = np.zeros((len(pts), 3) if code == 1 else len(pts), np.float32)
+    L.or_medium_eval(mp, code, len(pts), _p(pts), _p(out))
+    return out
+
+
+def delta_track(med, o, d, max_t, n, seed=1):
+    """or_delta_track_probe: the collision distance of n independent
+    delta-tracking walks along o + t d (-1 when none before max_t)."""
+    L = lib()
+    L.or_delta_track_probe.restype = None
+    L.or_delta_track_probe.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                       ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p]
+    m, mp = _medium_arg(med)
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.zeros(n, np.float32)
+    L.or_delta_track_probe(mp, _p(o), _p(d), max_t, n, seed, _p(out))
+    return out
+
+
+def emission_integral(med, o, d, max_t, n, seed=1):
+    """or_emission_integral_probe: n independent estimates (RGB) of the
+    emission integral over [0, max_t] of o + t d."""
+    L = lib()
+    L.or_emission_integral_probe.restype = None
+    L.or_emission_integral_probe.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                             ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p]
+    m, mp = _medium_arg(med)
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.zeros((n, 3), np.float32)
+    L.or_emission_integral_probe(mp, _p(o), _p(d), max_t, n, seed, _p(out))
+    return out
+
+
+def bend(med, o, d, length):
+    """or_bend_probe: (origin, direction) of the ray bent over length."""
+    L = lib()
+    L.or_bend_probe.restype = None
+    L.or_bend_probe.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p]
+    m, mp = _medium_arg(med)
+    o = np.ascontiguousarray(o, np.float32)
+    d = np.ascontiguousarray(d, np.float32)
+    out = np.zeros(6, np.float32)
+    L.or_bend_probe(mp, _p(o), _p(d), length, _p(out))
+    return out[:3], out[3:]

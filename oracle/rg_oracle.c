@@ -512,6 +512,7 @@ void or_closest1(const or_node *nodes, int64_t nnodes, const float *v9, const ui
 static int occludes(const or_object *m, uint8_t pred) {
     if (pred == 1) return m->transparency < 0.01f && m->emission <= 0.0f;
     if (pred == 2) return !(m->emission > 0.0f) && (m->transparency < 0.01f || m->refraction > 0.1f);
+    if (pred == 3) return !(m->emission > 0.0f) && m->transparency < 0.01f && m->refraction <= 0.1f;
     return 1;
 }
 

@@ -114,11 +114,46 @@ typedef struct {
     uint32_t pad;
 } or_object;
 
-/* HomogeneousMedium authoring fields (medium.h:37-130, constant emission). */
+/* HomogeneousMedium authoring fields (medium.h:37-214), positions in render
+ * space (after HomogeneousMedium::bake, medium.cpp:59-104). */
+typedef struct {
+    float p[3];
+    float radius, t;
+} or_path_point;
+#define OR_MAX_PATH_POINTS 16
 typedef struct {
     float sigma_a[3], sigma_s[3], sigma_e[3];
-    float g, density, pad;
+    float g, density;
+    int32_t emission_mode;              /* 0 constant, 1 cone, 2 path */
+    float cone_origin[3], cone_axis[3];
+    float cone_length, cone_radius;
+    float cone_inner[3], cone_outer[3];
+    float cone_intensity, cone_peak_axial, cone_peak_sharpness;
+    int32_t cone_samples;
+    or_path_point path[OR_MAX_PATH_POINTS];
+    uint32_t n_path;
+    float path_inner[3], path_outer[3];
+    float path_intensity, path_falloff_power;
+    int32_t path_samples;
+    int32_t density_field;              /* 0 none, 1 fBm */
+    float noise_frequency;
+    int32_t noise_octaves;
+    float noise_gain, noise_lacunarity, noise_amplitude, noise_bias;
+    float noise_offset[3];
+    int32_t heat_haze;
+    float ior_amplitude, ior_frequency;
+    int32_t ior_octaves;
+    float ior_gain, ior_lacunarity;
+    int32_t ior_march_steps;
+    float ior_offset[3], ior_falloff;
 } or_medium;
+
+/* An emissive volume for NEE: medium index and the owning object's location
+ * in render space (rayrenderer.cpp:180-206, 2083-2096). */
+typedef struct {
+    uint32_t medium;
+    float pos[3];
+} or_volume_light;
 
 /* Camera at the render-space origin looking down -Z (scene.h:115-135). */
 typedef struct {
@@ -174,6 +209,9 @@ typedef struct {
     /* point light sources (nullable) */
     const or_point_light *plights;
     uint32_t nplights;
+    /* emissive volumes for NEE (nullable) */
+    const or_volume_light *vlights;
+    uint32_t nvlights;
 } or_scene_desc;
 
 typedef struct {
@@ -214,6 +252,15 @@ float or_tile_noise(const float *sum_rgb, const float *sumsq_rgb, int w, int h, 
  * seen along them) and envmapPdf + sampleEnvironment for nq query
  * directions (rayrenderer.cpp:1299-1573). */
 void or_phase_sample(float g, const float wo[3], int n, const float *u2, float *dirs);
+/* medium probes: op 0 densityAt, 1 emissionAt (3 floats), 2 iorAt at n
+ * points; delta-tracking collision distances (-1: none before maxT) and
+ * emission integrals for n independent generators; one bent ray */
+void or_medium_eval(const or_medium *m, int op, int n, const float *pts, float *out);
+void or_delta_track_probe(const or_medium *m, const float o[3], const float d[3], float maxT, int n,
+                          uint32_t seed, float *t_out);
+void or_emission_integral_probe(const or_medium *m, const float o[3], const float d[3], float maxT, int n,
+                                uint32_t seed, float *out);
+void or_bend_probe(const or_medium *m, const float o[3], const float d[3], float length, float *out6);
 void or_phase_pdf(float g, const float wo[3], int n, const float *wi, float *pdf);
 void or_env_probe(const or_scene_desc *desc, int n, const float *u2, float *dirs, float *pdfs, float *rad, int nq,
                   const float *qdirs, float *qpdfs, float *qrad);

@@ -77,6 +77,12 @@ typedef struct {
     v3 sigma_t, sigma_s_eff, sigma_e_eff;
     float hero, g;
     int active;
+    const or_medium *src;   /* authored fields (noise, cone, path, haze) */
+    int het;                /* fBm density field */
+    float t_max_hero, peak; /* delta-tracking bound sigma_t_hero * peak */
+    v3 cone_axis;           /* normalised (render space) */
+    float density;          /* authored density (scales cone / path emission) */
+    int haze;               /* isHeatHaze */
 } medium_t;
 
 typedef struct {
@@ -356,6 +362,213 @@ static float free_flight(const medium_t *m, float u) {
     const float xi = fminf(0.9999999f, fmaxf(0.0f, u));
     return -logf(1.0f - xi) / m->hero;
 }
+/* value-noise basis of the density and refractive-index fields
+ * (medium.cpp:125-186): integer lattice hash to [0, 1), smoothstep-weighted
+ * trilinear interpolation, fBm normalised by the summed amplitudes */
+static float lattice(int x, int y, int z) {
+    uint32_t h = (uint32_t)x * 374761393u + (uint32_t)y * 668265263u + (uint32_t)z * 2147483647u;
+    h = (h ^ (h >> 13)) * 1274126177u;
+    h ^= h >> 16;
+    return (float)(h & 0x00FFFFFFu) / (float)0x01000000u;
+}
+static float fade(float x) { return x * x * (3.0f - 2.0f * x); }
+static float value_noise(float x, float y, float z) {
+    const int xi = (int)floorf(x), yi = (int)floorf(y), zi = (int)floorf(z);
+    const float u = fade(x - (float)xi), v = fade(y - (float)yi), w = fade(z - (float)zi);
+    float c[2][2][2];
+    for (int k = 0; k < 2; k++)
+        for (int j = 0; j < 2; j++)
+            for (int i = 0; i < 2; i++) c[k][j][i] = lattice(xi + i, yi + j, zi + k);
+    const float x00 = c[0][0][0] * (1.0f - u) + c[0][0][1] * u;
+    const float x10 = c[0][1][0] * (1.0f - u) + c[0][1][1] * u;
+    const float x01 = c[1][0][0] * (1.0f - u) + c[1][0][1] * u;
+    const float x11 = c[1][1][0] * (1.0f - u) + c[1][1][1] * u;
+    const float y0 = x00 * (1.0f - v) + x10 * v;
+    const float y1 = x01 * (1.0f - v) + x11 * v;
+    return y0 * (1.0f - w) + y1 * w;
+}
+static float fbm01(float x, float y, float z, float freq, int octaves, float gain, float lac) {
+    float fx = x * freq, fy = y * freq, fz = z * freq;
+    float amp = 1.0f, total = 0.0f, sum = 0.0f;
+    const int n = octaves > 0 ? octaves : 1;
+    for (int i = 0; i < n; i++) {
+        total += amp * value_noise(fx, fy, fz);
+        sum += amp;
+        amp *= gain;
+        fx *= lac;
+        fy *= lac;
+        fz *= lac;
+    }
+    return sum > 0.0f ? total / sum : 0.0f;
+}
+
+/* densityAt (medium.cpp:271-290): 1 without a density field */
+static float density_at(const medium_t *m, v3 p) {
+    if (!m->het) return 1.0f;
+    const or_medium *s = m->src;
+    const float n = fbm01(p.x - s->noise_offset[0], p.y - s->noise_offset[1], p.z - s->noise_offset[2],
+                          s->noise_frequency, s->noise_octaves, s->noise_gain, s->noise_lacunarity);
+    return fmaxf(0.0f, s->noise_amplitude * n + s->noise_bias);
+}
+
+/* emissionAt (medium.cpp:292-404): constant sigma_e (x density), a path
+ * tube's radial falloff and inner-to-outer blend, or the cone's axial
+ * gaussian and (1 - v^2)^2 radial profile, all times the density field */
+static v3 emission_at(const medium_t *m, v3 p) {
+    const or_medium *s = m->src;
+    if (s->emission_mode == 0) return m->het ? scl(m->sigma_e_eff, density_at(m, p)) : m->sigma_e_eff;
+    if (s->emission_mode == 2) {
+        const uint32_t n = s->n_path;
+        if (n < 2) return mk(0, 0, 0);
+        float best = INFINITY, bt = 0.0f, br = 0.0f;
+        for (uint32_t i = 0; i + 1 < n; i++) {
+            const or_path_point *a = &s->path[i], *b = &s->path[i + 1];
+            const v3 ab = mk(b->p[0] - a->p[0], b->p[1] - a->p[1], b->p[2] - a->p[2]);
+            const float ab2 = ab.x * ab.x + ab.y * ab.y + ab.z * ab.z;
+            if (ab2 < 1e-12f) continue;
+            const v3 ap = mk(p.x - a->p[0], p.y - a->p[1], p.z - a->p[2]);
+            float u = (ap.x * ab.x + ap.y * ab.y + ap.z * ab.z) / ab2;
+            u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+            const v3 q = mk(a->p[0] + ab.x * u, a->p[1] + ab.y * u, a->p[2] + ab.z * u);
+            const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < best) {
+                best = d2;
+                bt = a->t + (b->t - a->t) * u;
+                br = a->radius + (b->radius - a->radius) * u;
+            }
+        }
+        if (br <= 0.0f) return mk(0, 0, 0);
+        if (best >= br * br) return mk(0, 0, 0);
+        const float r = sqrtf(best) / br;
+        const float radial = powf(fmaxf(0.0f, 1.0f - r), s->path_falloff_power);
+        if (radial <= 0.0f) return mk(0, 0, 0);
+        const float tb = bt < 0.0f ? 0.0f : (bt > 1.0f ? 1.0f : bt);
+        const v3 col = add(scl(ld3(s->path_inner), 1.0f - tb), scl(ld3(s->path_outer), tb));
+        float k = radial * s->path_intensity * fmaxf(0.0f, m->density);
+        if (m->het) k *= density_at(m, p);
+        return scl(col, k);
+    }
+    const v3 rel = mk(p.x - s->cone_origin[0], p.y - s->cone_origin[1], p.z - s->cone_origin[2]);
+    const v3 ax = m->cone_axis;
+    const float at = rel.x * ax.x + rel.y * ax.y + rel.z * ax.z;
+    if (s->cone_length <= 0.0f) return mk(0, 0, 0);
+    const float u = at / s->cone_length;
+    if (u < 0.0f || u > 1.0f) return mk(0, 0, 0);
+    const v3 pp = mk(rel.x - ax.x * at, rel.y - ax.y * at, rel.z - ax.z * at);
+    const float r2 = pp.x * pp.x + pp.y * pp.y + pp.z * pp.z;
+    if (s->cone_radius <= 0.0f) return mk(0, 0, 0);
+    const float v2 = r2 / (s->cone_radius * s->cone_radius);
+    if (v2 > 1.0f) return mk(0, 0, 0);
+    const float dx = (u - s->cone_peak_axial) * s->cone_peak_sharpness;
+    const float t = expf(-dx * dx) * ((1.0f - v2) * (1.0f - v2));
+    if (t <= 0.0f) return mk(0, 0, 0);
+    const v3 col = add(scl(ld3(s->cone_inner), t), scl(ld3(s->cone_outer), 1.0f - t));
+    float k = t * s->cone_intensity * fmaxf(0.0f, m->density);
+    if (m->het) k *= density_at(m, p);
+    return scl(col, k);
+}
+
+/* sampleDeltaTracking (medium.cpp:406-440): exponential steps at the
+ * bound's rate, each accepted with probability density / peak; at most 64 */
+static int delta_track(const medium_t *m, rng_t *R, v3 o, v3 d, float maxT, float *out_t, float *out_den) {
+    if (m->t_max_hero <= 0.0f || maxT <= 0.0f) return 0;
+    const float peak = m->hero > 0.0f ? m->t_max_hero / m->hero : 1.0f;
+    if (peak <= 0.0f) return 0;
+    float t = 0.0f;
+    for (int i = 0; i < 64; i++) {
+        const float xi = fminf(0.9999999f, fmaxf(0.0f, prng(R)));
+        t += -logf(1.0f - xi) / m->t_max_hero;
+        if (t >= maxT) return 0;
+        const float den = density_at(m, add(o, scl(d, t)));
+        if (prng(R) < den / peak) {
+            *out_t = t;
+            *out_den = den;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* emissionIntegralAlongRay (medium.cpp:442-472): the closed form for
+ * constant emission, else one jittered sample per stratum of Tr x sigma_e */
+static v3 emission_integral(const medium_t *m, rng_t *R, v3 o, v3 d, float maxT) {
+    const or_medium *s = m->src;
+    if (s->emission_mode == 0) return emission_along(m, maxT);
+    if (maxT <= 0.0f) return mk(0, 0, 0);
+    const int cnt = s->emission_mode == 2 ? s->path_samples : s->cone_samples;
+    const int N = cnt > 0 ? cnt : 1;
+    const float dt = maxT / (float)N;
+    v3 acc = mk(0, 0, 0);
+    for (int i = 0; i < N; i++) {
+        const float t = ((float)i + prng(R)) * dt;
+        const v3 se = emission_at(m, add(o, scl(d, t)));
+        if (is_zero(se)) continue;
+        const v3 Tr = transmittance(m, t);
+        acc.x += se.x * Tr.x * dt;
+        acc.y += se.y * Tr.y * dt;
+        acc.z += se.z * Tr.z * dt;
+    }
+    return acc;
+}
+
+/* iorAt / iorGradientAt / bendRay (medium.cpp:188-269): n = 1 + amplitude x
+ * fade x 2 (fbm - 1/2), central differences at half a noise cell, and an
+ * eikonal march of iorMarchSteps midpoint steps */
+static float ior_at(const medium_t *m, v3 p) {
+    const or_medium *s = m->src;
+    const float n = fbm01(p.x - s->ior_offset[0], p.y - s->ior_offset[1], p.z - s->ior_offset[2], s->ior_frequency,
+                          s->ior_octaves, s->ior_gain, s->ior_lacunarity);
+    float am = 1.0f;
+    if (s->ior_falloff != 0.0f && s->cone_length > 0.0f) {
+        const float rx = p.x - s->cone_origin[0], ry = p.y - s->cone_origin[1], rz = p.z - s->cone_origin[2];
+        const float at = rx * m->cone_axis.x + ry * m->cone_axis.y + rz * m->cone_axis.z;
+        float u = at / s->cone_length;
+        u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+        const float mag = s->ior_falloff < 0.0f ? -s->ior_falloff : s->ior_falloff;
+        const float t = s->ior_falloff >= 0.0f ? u : 1.0f - u;
+        am = 1.0f - mag * t;
+        if (am < 0.0f) am = 0.0f;
+    }
+    return 1.0f + s->ior_amplitude * am * (n - 0.5f) * 2.0f;
+}
+static v3 ior_grad(const medium_t *m, v3 p) {
+    const float f = m->src->ior_frequency;
+    const float h = f > 0.0f ? 0.5f / f : 0.05f;
+    const float xp = ior_at(m, mk(p.x + h, p.y, p.z)), xm = ior_at(m, mk(p.x - h, p.y, p.z));
+    const float yp = ior_at(m, mk(p.x, p.y + h, p.z)), ym = ior_at(m, mk(p.x, p.y - h, p.z));
+    const float zp = ior_at(m, mk(p.x, p.y, p.z + h)), zm = ior_at(m, mk(p.x, p.y, p.z - h));
+    const float i2h = 0.5f / h;
+    return mk((xp - xm) * i2h, (yp - ym) * i2h, (zp - zm) * i2h);
+}
+static void bend_ray(const medium_t *m, v3 o, v3 d, float length, v3 *out_o, v3 *out_d) {
+    if (length <= 0.0f || !m->haze) {
+        *out_o = add(o, scl(d, length));
+        *out_d = d;
+        return;
+    }
+    const int N = m->src->ior_march_steps;
+    const float ds = length / (float)N;
+    v3 pos = o, dir = d;
+    for (int i = 0; i < N; i++) {
+        const v3 mid = mk(pos.x + dir.x * (ds * 0.5f), pos.y + dir.y * (ds * 0.5f), pos.z + dir.z * (ds * 0.5f));
+        const float n = ior_at(m, mid);
+        const v3 g = ior_grad(m, mid);
+        const float gd = g.x * dir.x + g.y * dir.y + g.z * dir.z;
+        const v3 pp = mk(g.x - dir.x * gd, g.y - dir.y * gd, g.z - dir.z * gd);
+        const float in = n > 1e-6f ? 1.0f / n : 1.0f;
+        const v3 nd = mk(dir.x + pp.x * (ds * in), dir.y + pp.y * (ds * in), dir.z + pp.z * (ds * in));
+        const float l2 = nd.x * nd.x + nd.y * nd.y + nd.z * nd.z;
+        if (l2 > 1e-12f) {
+            const float il = 1.0f / sqrtf(l2);
+            dir = mk(nd.x * il, nd.y * il, nd.z * il);
+        }
+        pos = mk(pos.x + dir.x * ds, pos.y + dir.y * ds, pos.z + dir.z * ds);
+    }
+    *out_o = pos;
+    *out_d = dir;
+}
+
 static float phase_pdf(const medium_t *m, v3 wo, v3 wi) {
     const float c = dot(wo, wi);
     const float g = m->g;
@@ -612,6 +825,78 @@ static v3 trace_point_light(const prep_t *P, const or_point_light *L, v3 hit, v3
     return mk(clamp01(L->color[0] * lum), clamp01(L->color[1] * lum), clamp01(L->color[2] * lum));
 }
 
+/* HomogeneousMedium::prepare (medium.cpp:18-57) and isActive (medium.h:214-223) */
+static void prep_medium(medium_t *q, const or_medium *m) {
+    memset(q, 0, sizeof *q);
+    const float den = fmaxf(0.0f, m->density);
+    q->sigma_s_eff = scl(ld3(m->sigma_s), den);
+    q->sigma_e_eff = scl(ld3(m->sigma_e), den);
+    q->sigma_t = add(scl(ld3(m->sigma_a), den), q->sigma_s_eff);
+    q->hero = (q->sigma_t.x + q->sigma_t.y + q->sigma_t.z) * (1.0f / 3.0f);
+    q->g = m->g;
+    q->src = m;
+    q->het = m->density_field == 1;
+    q->peak = q->het ? fmaxf(0.0f, m->noise_amplitude * 1.0f + m->noise_bias) : 1.0f;
+    q->t_max_hero = q->het ? q->hero * q->peak : q->hero;
+    const float al2 = m->cone_axis[0] * m->cone_axis[0] + m->cone_axis[1] * m->cone_axis[1] +
+                      m->cone_axis[2] * m->cone_axis[2];
+    q->cone_axis = al2 > 1e-12f ? scl(ld3(m->cone_axis), 1.0f / sqrtf(al2)) : mk(0.0f, 0.0f, -1.0f);
+    q->density = m->density;
+    q->haze = m->heat_haze && m->ior_amplitude > 0.0f && m->ior_march_steps > 0;
+    q->active = q->hero > 0.0f || (m->emission_mode == 1 && m->cone_intensity > 0.0f) ||
+                (m->emission_mode == 2 && m->path_intensity > 0.0f && m->n_path >= 2) ||
+                (m->heat_haze && m->ior_amplitude > 0.0f) || !is_zero(q->sigma_e_eff);
+}
+
+/* sampleVolumeLightForNEE (rayrenderer.cpp:2066-2203): one emissive volume
+ * picked uniformly (a rand() draw); equiangular sampling (Kulla & Fajardo
+ * 2012) of its cone axis, or of the owning object's location for constant
+ * emission; shadow ray with the volume lights' occluders; sigma_e at the
+ * point, times the medium's homogeneous transmittance over the distance;
+ * solid-angle pdf times the number of volumes. */
+static int sample_volume(const prep_t *P, rng_t *R, v3 hit, v3 n, v3 *out_dir, float *out_pdf, v3 *out_le) {
+    const uint32_t N = P->d->vlights ? P->d->nvlights : 0;
+    if (N == 0) return 0;
+    const or_volume_light *ev = &P->d->vlights[pick(prng(R), N)];
+    const medium_t *m = medium_at(P, (int)ev->medium);
+    if (!m) return 0;
+    v3 a, b;
+    if (m->src->emission_mode == 1) {
+        a = ld3(m->src->cone_origin);
+        b = add(a, scl(m->cone_axis, m->src->cone_length));
+    } else {
+        a = b = ld3(ev->pos);
+    }
+    const v3 sd = sub(b, a);
+    const float sl = len(sd);
+    const v3 axis = sl > 1e-6f ? scl(sd, 1.0f / sl) : mk(0.0f, 0.0f, -1.0f);
+    const v3 toA = sub(a, hit);
+    const float along = dot(toA, axis);
+    const v3 perp = sub(toA, scl(axis, along));
+    const float D = len(perp);
+    if (!(D > 1e-5f) && sl <= 1e-6f) return 0;
+    const float thA = atanf(along / fmaxf(D, 1e-5f));
+    const float thB = atanf((along + sl) / fmaxf(D, 1e-5f));
+    if (!(fabsf(thB - thA) > 1e-6f) && sl > 1e-6f) return 0;
+    const float u = prng(R);
+    const float th = thA + u * (thB - thA);
+    const float ts = D * tanf(th);
+    const v3 q = sl > 1e-6f ? add(a, scl(axis, ts - along)) : a;
+    const v3 tq = sub(q, hit);
+    const float r2 = tq.x * tq.x + tq.y * tq.y + tq.z * tq.z;
+    const float r = sqrtf(r2);
+    if (!(r > 1e-5f)) return 0;
+    if (occluded(P, surface_origin(hit, tq, n), tq, 0.99999f, 3)) return 0;
+    v3 se = emission_at(m, q);
+    if (is_zero(se)) return 0;
+    if (m->hero > 0.0f) se = mul(se, transmittance(m, r));
+    const float pdf_t = D / (fmaxf(1e-6f, fabsf(thB - thA)) * (D * D + ts * ts));
+    *out_dir = scl(tq, 1.0f / r);
+    *out_pdf = pdf_t * r2 * (float)N;
+    *out_le = se;
+    return 1;
+}
+
 /* traceLight (rayrenderer.cpp:2271-2293): one area light's traceAreaLight
  * plus one point light's tracePointLight (the only one, or one picked
  * uniformly by a rand() draw) */
@@ -700,6 +985,14 @@ static v3 diffuse_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3
     v3 c = trace_path(P, R, o, dir, pm);
     c = add(c, trace_light(P, R, h->pos, n));
     c = add(c, trace_env_light(P, R, h->pos, n, pm->bsdf_pdf));
+    {
+        v3 vd, vle;
+        float vp = 0.0f;
+        if (sample_volume(P, R, h->pos, n, &vd, &vp, &vle) && vp > 0.0f) {
+            const float cl = fmaxf(0.0f, dot(vd, n));
+            c = add(c, scl(vle, cl * (float)(1.0 / M_PI) / vp));
+        }
+    }
     pm->tp = st;
     pm->bsdf_pdf = sp;
     return mul(c, alb);
@@ -752,6 +1045,12 @@ static v3 glossy_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_d, v3 
             const float w = pe2 / (pe2 + pb2);
             direct = add(direct, scl(mul(s.Le, fc), w / s.pdf));
         }
+    }
+    {
+        v3 vd, vle, fc;
+        float vp = 0.0f, pb;
+        if (sample_volume(P, R, h->pos, n, &vd, &vp, &vle) && vp > 0.0f && eval_glossy(&G, vd, &fc, &pb))
+            direct = add(direct, scl(mul(vle, fc), 1.0f / vp));
     }
     const float u1 = prng(R), u2 = prng(R);
     const v3 H = sample_vndf(G.V, G.ax, G.ay, u1, u2);
@@ -935,7 +1234,10 @@ static v3 shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3 in_d, v
     return mix_shade(P, R, h, in_o, in_d, n, &pm);
 }
 
-/* RayRenderer::tracePath (rayrenderer.cpp:1596-1864), homogeneous media */
+/* RayRenderer::tracePath (rayrenderer.cpp:1596-1864): surfaces, heat haze
+ * (a bent ray continuing in the outer medium), and homogeneous (free
+ * flight) or fBm (delta tracking) media with constant, cone or path
+ * emission */
 static v3 trace_path(const prep_t *P, rng_t *R, v3 o, v3 d, param_t *sp) {
     isect_t h = nearest(P, o, d);
     const int midx = sp ? sp->medium : P->d->global_medium;
@@ -946,13 +1248,34 @@ static v3 trace_path(const prep_t *P, rng_t *R, v3 o, v3 d, param_t *sp) {
         if (sp) env = scl(env, env_miss_weight(P, d, sp->bsdf_pdf));
         return env;
     }
+    if (med->haze) {
+        /* the global medium is never heat haze (refused when preparing) */
+        v3 bo, bd;
+        bend_ray(med, o, d, fminf(h.hit ? h.t : RAY_MAX_DISTANCE, 1000.0f), &bo, &bd);
+        const int saved = sp ? sp->medium : -1;
+        if (sp) sp->medium = P->d->global_medium;
+        const v3 r = trace_path(P, R, bo, bd, sp);
+        if (sp) sp->medium = saved;
+        return r;
+    }
     const float maxT = h.hit ? h.t : RAY_MAX_DISTANCE;
-    const float tf = free_flight(med, prng(R));
-    const int scattered = tf < maxT;
-    const v3 memi = emission_along(med, scattered ? tf : maxT);
+    int scattered;
+    float tf = 0.0f, den = 1.0f;
+    if (med->het) {
+        scattered = delta_track(med, R, o, d, maxT, &tf, &den);
+        if (!scattered) tf = maxT;
+    } else {
+        tf = free_flight(med, prng(R));
+        scattered = tf < maxT;
+    }
+    const v3 memi = emission_integral(med, R, o, d, scattered ? tf : maxT);
     if (scattered) {
         v3 sw = mk(0, 0, 0);
-        {
+        if (med->het) {
+            const float inv = med->t_max_hero > 0.0f ? 1.0f / med->t_max_hero : 0.0f;
+            const v3 ss = scl(med->sigma_s_eff, den);
+            sw = mk(ss.x * inv, ss.y * inv, ss.z * inv);
+        } else {
             const float ph = med->hero * expf(-med->hero * tf);
             const v3 Tr = transmittance(med, tf);
             if (ph > 0.0f)
@@ -975,6 +1298,12 @@ static v3 trace_path(const prep_t *P, rng_t *R, v3 o, v3 d, param_t *sp) {
             nee_t s = sample_env_nee(P, R, pos, proxy, &co);
             if (s.ok && s.pdf > 0.0f) direct = add(direct, scl(s.Le, phase_pdf(med, neg(d), s.dir) * (1.0f / s.pdf)));
         }
+        {
+            v3 vd, vle;
+            float vp = 0.0f;
+            if (sample_volume(P, R, pos, proxy, &vd, &vp, &vle) && vp > 0.0f)
+                direct = add(direct, scl(vle, phase_pdf(med, neg(d), vd) * (1.0f / vp)));
+        }
         const float u1 = prng(R), u2 = prng(R);
         const v3 nd = sample_phase(med, d, u1, u2);
         param_t pm;
@@ -995,7 +1324,7 @@ static v3 trace_path(const prep_t *P, rng_t *R, v3 o, v3 d, param_t *sp) {
         return add(memi, mul(sw, add(direct, ind)));
     }
     v3 sw = mk(1, 1, 1);
-    {
+    if (!med->het) {
         const float surv = expf(-med->hero * maxT);
         if (surv > 0.0f) {
             const v3 Tr = transmittance(med, maxT);
@@ -1088,13 +1417,34 @@ static void twin_step(const prep_t *P, rng_t *R, path_t *p) {
         p->alive = 0;
         return;
     }
+    if (med->haze) {
+        /* the bent ray continues in the outer medium as the path's next
+         * event, with the same passes, throughput and BSDF pdf */
+        v3 bo, bd;
+        bend_ray(med, p->o, p->d, fminf(h.hit ? h.t : RAY_MAX_DISTANCE, 1000.0f), &bo, &bd);
+        p->o = bo;
+        p->d = bd;
+        p->medium = P->d->global_medium;
+        return;
+    }
     const float maxT = h.hit ? h.t : RAY_MAX_DISTANCE;
-    const float tf = free_flight(med, prng(R));
-    const int scattered = tf < maxT;
-    add_L(p, mul(p->beta, emission_along(med, scattered ? tf : maxT)));
+    int scattered;
+    float tf = 0.0f, den = 1.0f;
+    if (med->het) {
+        scattered = delta_track(med, R, p->o, p->d, maxT, &tf, &den);
+        if (!scattered) tf = maxT;
+    } else {
+        tf = free_flight(med, prng(R));
+        scattered = tf < maxT;
+    }
+    add_L(p, mul(p->beta, emission_integral(med, R, p->o, p->d, scattered ? tf : maxT)));
     if (scattered) {
         v3 sw = mk(0, 0, 0);
-        {
+        if (med->het) {
+            const float inv = med->t_max_hero > 0.0f ? 1.0f / med->t_max_hero : 0.0f;
+            const v3 ss = scl(med->sigma_s_eff, den);
+            sw = mk(ss.x * inv, ss.y * inv, ss.z * inv);
+        } else {
             const float ph = med->hero * expf(-med->hero * tf);
             const v3 Tr = transmittance(med, tf);
             if (ph > 0.0f)
@@ -1117,6 +1467,12 @@ static void twin_step(const prep_t *P, rng_t *R, path_t *p) {
             nee_t s = sample_env_nee(P, R, pos, proxy, &co);
             if (s.ok && s.pdf > 0.0f) add_L(p, mul(bsw, scl(s.Le, phase_pdf(med, neg(p->d), s.dir) * (1.0f / s.pdf))));
         }
+        {
+            v3 vd, vle;
+            float vp = 0.0f;
+            if (sample_volume(P, R, pos, proxy, &vd, &vp, &vle) && vp > 0.0f)
+                add_L(p, mul(bsw, scl(vle, phase_pdf(med, neg(p->d), vd) * (1.0f / vp))));
+        }
         const float u1 = prng(R), u2 = prng(R);
         const v3 nd = sample_phase(med, p->d, u1, u2);
         float rrw = 1.0f;
@@ -1134,7 +1490,7 @@ static void twin_step(const prep_t *P, rng_t *R, path_t *p) {
         return;
     }
     v3 sw = mk(1, 1, 1);
-    {
+    if (!med->het) {
         const float surv = expf(-med->hero * maxT);
         if (surv > 0.0f) {
             const v3 Tr = transmittance(med, maxT);
@@ -1180,6 +1536,14 @@ static void twin_mix(const prep_t *P, rng_t *R, path_t *p, const isect_t *h, v3 
         const v3 ba = mul(p->beta, alb);
         add_L(p, mul(ba, trace_light(P, R, hit, n)));
         add_L(p, mul(ba, trace_env_light(P, R, hit, n, pdf)));
+        {
+            v3 vd, vle;
+            float vp = 0.0f;
+            if (sample_volume(P, R, hit, n, &vd, &vp, &vle) && vp > 0.0f) {
+                const float cl = fmaxf(0.0f, dot(vd, n));
+                add_L(p, mul(ba, scl(vle, cl * (float)(1.0 / M_PI) / vp)));
+            }
+        }
         p->beta = ba;
         p->tp = mul(p->tp, alb);
         p->bsdf_pdf = pdf;
@@ -1225,6 +1589,12 @@ static void twin_mix(const prep_t *P, rng_t *R, path_t *p, const isect_t *h, v3 
                 const float pe2 = s.pdf * s.pdf, pb2 = pb * pb;
                 add_L(p, mul(p->beta, scl(mul(s.Le, fc), (pe2 / (pe2 + pb2)) / s.pdf)));
             }
+        }
+        {
+            v3 vd, vle, fc;
+            float vp = 0.0f, pb;
+            if (sample_volume(P, R, hit, n, &vd, &vp, &vle) && vp > 0.0f && eval_glossy(&G, vd, &fc, &pb))
+                add_L(p, mul(p->beta, scl(mul(vle, fc), 1.0f / vp)));
         }
         const float u1 = prng(R), u2 = prng(R);
         const v3 H = sample_vndf(G.V, G.ax, G.ay, u1, u2);
@@ -1481,16 +1851,12 @@ static int prepare(prep_t *P, const or_scene_desc *d, const or_settings *S) {
     free(cnt);
     /* media (HomogeneousMedium::prepare, medium.cpp:18-57) */
     P->media = (medium_t *)calloc(d->nmedia ? d->nmedia : 1, sizeof(medium_t));
-    for (uint32_t i = 0; i < d->nmedia; i++) {
-        const or_medium *m = &d->media[i];
-        const float den = fmaxf(0.0f, m->density);
-        medium_t *q = &P->media[i];
-        q->sigma_s_eff = scl(ld3(m->sigma_s), den);
-        q->sigma_e_eff = scl(ld3(m->sigma_e), den);
-        q->sigma_t = add(scl(ld3(m->sigma_a), den), q->sigma_s_eff);
-        q->hero = (q->sigma_t.x + q->sigma_t.y + q->sigma_t.z) * (1.0f / 3.0f);
-        q->g = m->g;
-        q->active = q->hero > 0.0f || !is_zero(q->sigma_e_eff);
+    for (uint32_t i = 0; i < d->nmedia; i++) prep_medium(&P->media[i], &d->media[i]);
+    /* a heat-haze global medium (where tracePath recurses without end, and
+     * which the library refuses) renders as no medium */
+    if (d->global_medium >= 0 && P->media[d->global_medium].haze) {
+        P->media[d->global_medium].haze = 0;
+        P->media[d->global_medium].active = 0;
     }
     /* envmap CDF (scene.cpp:400-463) */
     P->has_env = d->env.rgb != NULL && d->env.w > 0 && d->env.h > 0;
@@ -1889,4 +2255,66 @@ void or_env_probe(const or_scene_desc *desc, int n, const float *u2, float *dirs
         qrad[3 * i + 2] = L.z;
     }
     release(&P);
+}
+
+/* ---- medium probes (tests only) ------------------------------------------ */
+static rng_t probe_rng(uint32_t seed, uint32_t i) {
+    rng_t R;
+    memset(&R, 0, sizeof R);
+    R.pix = i;
+    R.sample = 0;
+    R.key0 = seed;
+    R.key1 = 0x9E3779B9u;
+    R.blk_id = 0xFFFFFFFFu;
+    return R;
+}
+
+void or_medium_eval(const or_medium *src, int op, int n, const float *pts, float *out) {
+    medium_t m;
+    prep_medium(&m, src);
+    for (int i = 0; i < n; i++) {
+        const v3 p = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+        if (op == 0) out[i] = density_at(&m, p);
+        else if (op == 1) {
+            const v3 e = emission_at(&m, p);
+            out[3 * i] = e.x;
+            out[3 * i + 1] = e.y;
+            out[3 * i + 2] = e.z;
+        } else {
+            out[i] = ior_at(&m, p);
+        }
+    }
+}
+
+void or_delta_track_probe(const or_medium *src, const float o[3], const float d[3], float maxT, int n,
+                          uint32_t seed, float *t_out) {
+    medium_t m;
+    prep_medium(&m, src);
+    for (int i = 0; i < n; i++) {
+        rng_t R = probe_rng(seed, (uint32_t)i);
+        float t = 0.0f, den = 0.0f;
+        t_out[i] = delta_track(&m, &R, ld3(o), ld3(d), maxT, &t, &den) ? t : -1.0f;
+    }
+}
+
+void or_emission_integral_probe(const or_medium *src, const float o[3], const float d[3], float maxT, int n,
+                                uint32_t seed, float *out) {
+    medium_t m;
+    prep_medium(&m, src);
+    for (int i = 0; i < n; i++) {
+        rng_t R = probe_rng(seed, (uint32_t)i);
+        const v3 e = emission_integral(&m, &R, ld3(o), ld3(d), maxT);
+        out[3 * i] = e.x;
+        out[3 * i + 1] = e.y;
+        out[3 * i + 2] = e.z;
+    }
+}
+
+void or_bend_probe(const or_medium *src, const float o[3], const float d[3], float length, float *out6) {
+    medium_t m;
+    prep_medium(&m, src);
+    v3 bo, bd;
+    bend_ray(&m, ld3(o), ld3(d), length, &bo, &bd);
+    out6[0] = bo.x; out6[1] = bo.y; out6[2] = bo.z;
+    out6[3] = bd.x; out6[4] = bd.y; out6[5] = bd.z;
 }

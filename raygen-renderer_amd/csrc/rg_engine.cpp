@@ -570,12 +570,12 @@ class Engine {
                 c.width = r->st.width;
                 c.sample_base = r->issued;
                 c.S = S;
-                c.bounces = r->st.max_depth + 2;
+                c.bounces = (uint32_t)path_events(*scenes[r->scene], r->st.max_depth);
                 c.clamp = r->st.firefly_clamp;
                 ch.push_back(c);
                 total += paths;
                 r->issued += S;
-                bounces = std::max(bounces, r->st.max_depth + 2);
+                bounces = std::max(bounces, c.bounces);
                 chunk_jobs.push_back(r);
             }
             if (r->issued >= r->st.spp) {
@@ -659,7 +659,8 @@ class Engine {
     int launch_bounces(int L, const BatchArgs &a, uint32_t scene, uint32_t paths, uint32_t bounces) {
         hipStream_t stream = lanes[L].stream;
         const DevBvh &bv = scene_bvh[scene];
-        if (!tune.graphs) return stream_launch_bounces(a, d_jobs, bv, paths, bounces, tune, n_cu, stream);
+        const bool ext = scene_uses_ext(*scenes[scene]);
+        if (!tune.graphs) return stream_launch_bounces(a, d_jobs, bv, paths, bounces, tune, n_cu, ext, stream);
         uint32_t slots = 1024;
         while (slots < paths && slots < cap) slots <<= 1;
         slots = std::min(slots, cap);
@@ -669,7 +670,7 @@ class Engine {
             hipGraph_t g = nullptr;
             if (int rc = hip_ok(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "begin capture"))
                 return rc;
-            const int lrc = stream_launch_bounces(a, d_jobs, bv, slots, bounces, tune, n_cu, stream);
+            const int lrc = stream_launch_bounces(a, d_jobs, bv, slots, bounces, tune, n_cu, ext, stream);
             const hipError_t ce = hipStreamEndCapture(stream, &g);
             if (lrc) return lrc;
             if (int rc = hip_ok(ce, "end capture")) return rc;

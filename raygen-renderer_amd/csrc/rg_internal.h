@@ -163,12 +163,13 @@ struct Scene {
     float env_intensity = 1.0f, env_rotation_deg = 0.0f;
     rg_camera camera{75.0f, 1.0f, 0.0f, 1.8f, 0, 0.0f, 50.0f, 0.0f};
     std::vector<rg_point_light> point_lights;
+    std::vector<rg_volume_light> volume_lights;
     // device copies for the path tracer (uploaded lazily)
     bool pt_dirty = true;
     bool tris_dirty = false;
     void *d_v9 = nullptr, *d_shade = nullptr, *d_objs = nullptr, *d_lights = nullptr;
     void *d_light_tris = nullptr, *d_media = nullptr, *d_env_rgb = nullptr, *d_env_marg = nullptr;
-    void *d_env_cond = nullptr, *d_plights = nullptr;
+    void *d_env_cond = nullptr, *d_plights = nullptr, *d_vlights = nullptr;
     uint32_t nlights = 0;
     float env_total = 0.0f;
     // albedo textures (rg_scene_set_textures): the texels as handed in, their

@@ -50,12 +50,26 @@ struct DevTriShade {
 };
 static_assert(sizeof(DevTriShade) == 64, "DevTriShade must be 64 bytes");
 
-// HomogeneousMedium after prepare() (medium.cpp:18-57).
+// HomogeneousMedium after prepare() (medium.cpp:18-57): the homogeneous
+// coefficients, the fBm bound for delta tracking, the normalised cone axis,
+// the heat-haze test, and the authored fields (positions in render space).
 struct DevMedium {
     float sigma_t[3], sigma_s[3], sigma_e[3];
     float hero, g;
     int32_t active;
+    int32_t het;          // fBm density field
+    int32_t haze;         // isHeatHaze
+    float t_max_hero;     // sigma_t_hero x the fBm peak (delta-tracking rate)
+    float cone_axis[3];
+    rg_medium src;
 };
+
+// An emissive volume for NEE (same layout as rg_volume_light).
+struct DevVolumeLight {
+    uint32_t medium;
+    float pos[3];
+};
+static_assert(sizeof(DevVolumeLight) == sizeof(rg_volume_light), "DevVolumeLight must mirror rg_volume_light");
 
 // Area light = an emissive object's triangles (transformScene,
 // rayrenderer.cpp:333-354).
@@ -72,8 +86,9 @@ struct DevPointLight {
 };
 static_assert(sizeof(DevPointLight) == sizeof(rg_point_light), "DevPointLight must mirror rg_point_light");
 
-// NEE shadow-ray slots per path event: area light, point light, environment.
-constexpr uint32_t kNeeSlots = 3;
+// NEE shadow-ray slots per path event: area light, point light,
+// environment, emissive volume.
+constexpr uint32_t kNeeSlots = 4;
 
 // Everything a shade kernel reads, passed by value.
 struct PTScene {
@@ -85,6 +100,8 @@ struct PTScene {
     uint32_t nlights;
     const DevPointLight *plights;
     uint32_t nplights;
+    const DevVolumeLight *vlights;
+    uint32_t nvlights;
     const DevMedium *media;
     int32_t global_medium;
     // equirect environment
@@ -113,6 +130,12 @@ struct PTScene {
     // RendererSettings::enableColorSampling and backColor (rgba)
     int32_t color_sampling;
     float back[4];
+    // path events a generation runs (max_depth + 2, twice that when a
+    // heat-haze medium can add an event without a pass)
+    int32_t events;
+    // the scene needs the shade kernels built with the extended media
+    // (fBm density, cone / path emission, heat haze, volume-light NEE)
+    int32_t ext;
 };
 
 // Structure-of-arrays path state, capacity = pixels x samples in flight.
@@ -145,6 +168,15 @@ struct PTState {
 
 // Kernel-side scene view of one job: scene buffers, camera and settings.
 PTScene make_ptscene(const Scene &scn, const rg_settings &st);
+// Path events a generation runs: max_depth + 2 (every event but the eye
+// hit's adds a pass, and the last pass ends the path), twice that when the
+// scene has a heat-haze medium (a bend adds an event without a pass, and at
+// most one bend follows each other event, its outer medium not being heat
+// haze).
+int path_events(const Scene &s, uint32_t max_depth);
+// The scene has a medium with a density field, a cone or path emission
+// profile or heat haze, or emissive-volume lights.
+bool scene_uses_ext(const Scene &s);
 // Validates shading inputs and uploads what changed since the last upload.
 int prepare_scene(Scene *s, void *stream);
 

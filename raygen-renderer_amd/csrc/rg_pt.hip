@@ -121,6 +121,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(7))) v
                                                count, lc);
 }
 
+template <bool kExt>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(PTScene sc, PTState st,
                                                                                            int qi) {
     __shared__ uint32_t lds[kPtBlock / 64 + 1];
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         P.alive = true;
         Shadows S = shadows_of(st, p);
         const rg_hit h = st.hit[p];
-        step(sc, R, P, S, h);
+        step<kExt>(sc, R, P, S, h);
         st.ray[p] = {{P.o.x, P.o.y, P.o.z}, 0.0f, {P.d.x, P.d.y, P.d.z}, MAX_RAY_DISTANCE};
         st.beta[p] = v4(P.beta, P.bsdf_pdf);
         st.tp[p] = v4(P.tp, 0.0f);
@@ -163,7 +164,8 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         alive = P.alive;
     }
     // the event's NEE rays go to the shadow-ray queue, one ray per lane there
-    const uint32_t slots = sc.nplights ? kNeeSlots : kNeeSlots - 1;  // the point-light slot only with point lights
+    // the point-light and volume slots only when the scene has such lights
+    const uint32_t slots = 2 + (sc.nplights ? 1u : 0u) + (sc.nvlights ? 1u : 0u);
     for (uint32_t k = 0; k < slots; k++) {
         const uint32_t pos = block_slot(k < ns, st.count + 2, lds);
         if (k < ns) st.ssrc[pos] = k * st.slot_stride + p;
@@ -345,10 +347,11 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
     hipLaunchKernelGGL(k_begin_counts, dim3(1), dim3(1), 0, stream, st.count, d_tile_mask ? 0u : cap);
     hipLaunchKernelGGL(k_camera, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, S, sample_base, d_tile_mask);
     int qi = 0;
-    const int bounces = sc.max_depth + 2;
+    const int bounces = sc.events;
     for (int b = 0; b < bounces; b++) {
         hipLaunchKernelGGL((k_extend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, stream, bv, st, qi);
-        hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, qi);
+        if (sc.ext) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, qi);
+        else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(kPtBlock), 0, stream, sc, st, qi);
         hipLaunchKernelGGL((k_anyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, stream, bv, st);
         qi ^= 1;
     }

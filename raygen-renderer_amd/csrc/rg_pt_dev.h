@@ -370,6 +370,211 @@ __device__ __forceinline__ v3 sample_phase(const DevMedium &m, v3 wo, float u1, 
 }
 
 // ---------------------------------------------------------------------------
+// heterogeneous and emissive media, heat haze (medium.cpp:125-472)
+// ---------------------------------------------------------------------------
+// value-noise basis: integer lattice hash to [0, 1), smoothstep-weighted
+// trilinear interpolation, fBm normalised by the summed amplitudes
+__device__ __forceinline__ float lattice(int x, int y, int z) {
+    uint32_t h = (uint32_t)x * 374761393u + (uint32_t)y * 668265263u + (uint32_t)z * 2147483647u;
+    h = (h ^ (h >> 13)) * 1274126177u;
+    h ^= h >> 16;
+    return (float)(h & 0x00FFFFFFu) / (float)0x01000000u;
+}
+__device__ __forceinline__ float fade(float x) { return x * x * (3.0f - 2.0f * x); }
+__device__ __forceinline__ float value_noise(float x, float y, float z) {
+    const int xi = (int)floorf(x), yi = (int)floorf(y), zi = (int)floorf(z);
+    const float u = fade(x - (float)xi), v = fade(y - (float)yi), w = fade(z - (float)zi);
+    const float x00 = lattice(xi, yi, zi) * (1.0f - u) + lattice(xi + 1, yi, zi) * u;
+    const float x10 = lattice(xi, yi + 1, zi) * (1.0f - u) + lattice(xi + 1, yi + 1, zi) * u;
+    const float x01 = lattice(xi, yi, zi + 1) * (1.0f - u) + lattice(xi + 1, yi, zi + 1) * u;
+    const float x11 = lattice(xi, yi + 1, zi + 1) * (1.0f - u) + lattice(xi + 1, yi + 1, zi + 1) * u;
+    const float y0 = x00 * (1.0f - v) + x10 * v;
+    const float y1 = x01 * (1.0f - v) + x11 * v;
+    return y0 * (1.0f - w) + y1 * w;
+}
+__device__ __forceinline__ float fbm01(float x, float y, float z, float freq, int octaves, float gain, float lac) {
+    float fx = x * freq, fy = y * freq, fz = z * freq;
+    float amp = 1.0f, total = 0.0f, sum = 0.0f;
+    const int n = octaves > 0 ? octaves : 1;
+    for (int i = 0; i < n; i++) {
+        total += amp * value_noise(fx, fy, fz);
+        sum += amp;
+        amp *= gain;
+        fx *= lac;
+        fy *= lac;
+        fz *= lac;
+    }
+    return sum > 0.0f ? total / sum : 0.0f;
+}
+
+// densityAt (medium.cpp:271-290): 1 without a density field
+__device__ __forceinline__ float density_at(const DevMedium &m, v3 p) {
+    if (!m.het) return 1.0f;
+    const rg_medium &s = m.src;
+    const float n = fbm01(p.x - s.noise_offset[0], p.y - s.noise_offset[1], p.z - s.noise_offset[2],
+                          s.noise_frequency, s.noise_octaves, s.noise_gain, s.noise_lacunarity);
+    return fmaxf(0.0f, s.noise_amplitude * n + s.noise_bias);
+}
+
+// emissionAt (medium.cpp:292-404): constant sigma_e (x density), a path
+// tube's radial falloff and inner-to-outer blend, or the cone's axial
+// gaussian and (1 - v^2)^2 radial profile, all times the density field
+__device__ __forceinline__ v3 emission_at(const DevMedium &m, v3 p) {
+    const rg_medium &s = m.src;
+    if (s.emission_mode == 0)
+        return m.het ? scl(mk(m.sigma_e[0], m.sigma_e[1], m.sigma_e[2]), density_at(m, p))
+                     : mk(m.sigma_e[0], m.sigma_e[1], m.sigma_e[2]);
+    if (s.emission_mode == 2) {
+        const uint32_t n = s.n_path;
+        if (n < 2) return mk(0, 0, 0);
+        float best = __builtin_inff(), bt = 0.0f, br = 0.0f;
+        for (uint32_t i = 0; i + 1 < n; i++) {
+            const rg_path_point &a = s.path[i], &b = s.path[i + 1];
+            const v3 ab = mk(b.p[0] - a.p[0], b.p[1] - a.p[1], b.p[2] - a.p[2]);
+            const float ab2 = ab.x * ab.x + ab.y * ab.y + ab.z * ab.z;
+            if (ab2 < 1e-12f) continue;
+            const v3 ap = mk(p.x - a.p[0], p.y - a.p[1], p.z - a.p[2]);
+            float u = (ap.x * ab.x + ap.y * ab.y + ap.z * ab.z) / ab2;
+            u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+            const v3 q = mk(a.p[0] + ab.x * u, a.p[1] + ab.y * u, a.p[2] + ab.z * u);
+            const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < best) {
+                best = d2;
+                bt = a.t + (b.t - a.t) * u;
+                br = a.radius + (b.radius - a.radius) * u;
+            }
+        }
+        if (br <= 0.0f) return mk(0, 0, 0);
+        if (best >= br * br) return mk(0, 0, 0);
+        const float r = sqrtf(best) / br;
+        const float radial = powf(fmaxf(0.0f, 1.0f - r), s.path_falloff_power);
+        if (radial <= 0.0f) return mk(0, 0, 0);
+        const float tb = bt < 0.0f ? 0.0f : (bt > 1.0f ? 1.0f : bt);
+        const v3 col = add(scl(ld3(s.path_inner), 1.0f - tb), scl(ld3(s.path_outer), tb));
+        float k = radial * s.path_intensity * fmaxf(0.0f, s.density);
+        if (m.het) k *= density_at(m, p);
+        return scl(col, k);
+    }
+    const v3 rel = mk(p.x - s.cone_origin[0], p.y - s.cone_origin[1], p.z - s.cone_origin[2]);
+    const v3 ax = mk(m.cone_axis[0], m.cone_axis[1], m.cone_axis[2]);
+    const float at = rel.x * ax.x + rel.y * ax.y + rel.z * ax.z;
+    if (s.cone_length <= 0.0f) return mk(0, 0, 0);
+    const float u = at / s.cone_length;
+    if (u < 0.0f || u > 1.0f) return mk(0, 0, 0);
+    const v3 pp = mk(rel.x - ax.x * at, rel.y - ax.y * at, rel.z - ax.z * at);
+    const float r2 = pp.x * pp.x + pp.y * pp.y + pp.z * pp.z;
+    if (s.cone_radius <= 0.0f) return mk(0, 0, 0);
+    const float v2 = r2 / (s.cone_radius * s.cone_radius);
+    if (v2 > 1.0f) return mk(0, 0, 0);
+    const float dx = (u - s.cone_peak_axial) * s.cone_peak_sharpness;
+    const float t = expf(-dx * dx) * ((1.0f - v2) * (1.0f - v2));
+    if (t <= 0.0f) return mk(0, 0, 0);
+    const v3 col = add(scl(ld3(s.cone_inner), t), scl(ld3(s.cone_outer), 1.0f - t));
+    float k = t * s.cone_intensity * fmaxf(0.0f, s.density);
+    if (m.het) k *= density_at(m, p);
+    return scl(col, k);
+}
+
+// sampleDeltaTracking (medium.cpp:406-440): exponential steps at the
+// bound's rate, each accepted with probability density / peak; at most 64
+// (inlined: a call taking the generator by reference would move it to
+// scratch memory for the whole kernel)
+__device__ __forceinline__ bool delta_track(const DevMedium &m, Rng &R, v3 o, v3 d, float maxT, float &out_t,
+                                         float &out_den) {
+    if (m.t_max_hero <= 0.0f || maxT <= 0.0f) return false;
+    const float peak = m.hero > 0.0f ? m.t_max_hero / m.hero : 1.0f;
+    if (peak <= 0.0f) return false;
+    float t = 0.0f;
+    for (int i = 0; i < 64; i++) {
+        const float xi = fminf(0.9999999f, fmaxf(0.0f, prng(R)));
+        t += -logf(1.0f - xi) / m.t_max_hero;
+        if (t >= maxT) return false;
+        const float den = density_at(m, add(o, scl(d, t)));
+        if (prng(R) < den / peak) {
+            out_t = t;
+            out_den = den;
+            return true;
+        }
+    }
+    return false;
+}
+
+// emissionIntegralAlongRay (medium.cpp:442-472): the closed form for
+// constant emission, else one jittered sample per stratum of Tr x sigma_e
+__device__ __forceinline__ v3 emission_integral(const DevMedium &m, Rng &R, v3 o, v3 d, float maxT) {
+    const rg_medium &s = m.src;
+    if (s.emission_mode == 0) return emission_along(m, maxT);
+    if (maxT <= 0.0f) return mk(0, 0, 0);
+    const int cnt = s.emission_mode == 2 ? s.path_samples : s.cone_samples;
+    const int N = cnt > 0 ? cnt : 1;
+    const float dt = maxT / (float)N;
+    v3 acc = mk(0, 0, 0);
+    for (int i = 0; i < N; i++) {
+        const float t = ((float)i + prng(R)) * dt;
+        const v3 se = emission_at(m, add(o, scl(d, t)));
+        if (is_zero(se)) continue;
+        const v3 Tr = transmittance(m, t);
+        acc.x += se.x * Tr.x * dt;
+        acc.y += se.y * Tr.y * dt;
+        acc.z += se.z * Tr.z * dt;
+    }
+    return acc;
+}
+
+// iorAt / iorGradientAt / bendRay (medium.cpp:188-269): n = 1 + amplitude x
+// fade x 2 (fbm - 1/2), central differences at half a noise cell, and an
+// eikonal march of iorMarchSteps midpoint steps
+__device__ __forceinline__ float ior_at(const DevMedium &m, v3 p) {
+    const rg_medium &s = m.src;
+    const float n = fbm01(p.x - s.ior_offset[0], p.y - s.ior_offset[1], p.z - s.ior_offset[2], s.ior_frequency,
+                          s.ior_octaves, s.ior_gain, s.ior_lacunarity);
+    float am = 1.0f;
+    if (s.ior_falloff != 0.0f && s.cone_length > 0.0f) {
+        const float rx = p.x - s.cone_origin[0], ry = p.y - s.cone_origin[1], rz = p.z - s.cone_origin[2];
+        const float at = rx * m.cone_axis[0] + ry * m.cone_axis[1] + rz * m.cone_axis[2];
+        float u = at / s.cone_length;
+        u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+        const float mag = s.ior_falloff < 0.0f ? -s.ior_falloff : s.ior_falloff;
+        const float t = s.ior_falloff >= 0.0f ? u : 1.0f - u;
+        am = 1.0f - mag * t;
+        if (am < 0.0f) am = 0.0f;
+    }
+    return 1.0f + s.ior_amplitude * am * (n - 0.5f) * 2.0f;
+}
+struct Ray3 {
+    v3 o, d;
+};
+__device__ __forceinline__ Ray3 bend_ray(const DevMedium &m, v3 o, v3 d, float length) {
+    if (length <= 0.0f || !m.haze) return {add(o, scl(d, length)), d};
+    const int N = m.src.ior_march_steps;
+    const float ds = length / (float)N;
+    const float f = m.src.ior_frequency;
+    const float h = f > 0.0f ? 0.5f / f : 0.05f;
+    const float i2h = 0.5f / h;
+    v3 pos = o, dir = d;
+    for (int i = 0; i < N; i++) {
+        const v3 mid = mk(pos.x + dir.x * (ds * 0.5f), pos.y + dir.y * (ds * 0.5f), pos.z + dir.z * (ds * 0.5f));
+        const float n = ior_at(m, mid);
+        const float xp = ior_at(m, mk(mid.x + h, mid.y, mid.z)), xm = ior_at(m, mk(mid.x - h, mid.y, mid.z));
+        const float yp = ior_at(m, mk(mid.x, mid.y + h, mid.z)), ym = ior_at(m, mk(mid.x, mid.y - h, mid.z));
+        const float zp = ior_at(m, mk(mid.x, mid.y, mid.z + h)), zm = ior_at(m, mk(mid.x, mid.y, mid.z - h));
+        const v3 g = mk((xp - xm) * i2h, (yp - ym) * i2h, (zp - zm) * i2h);
+        const float gd = g.x * dir.x + g.y * dir.y + g.z * dir.z;
+        const v3 pp = mk(g.x - dir.x * gd, g.y - dir.y * gd, g.z - dir.z * gd);
+        const float in = n > 1e-6f ? 1.0f / n : 1.0f;
+        const v3 nd = mk(dir.x + pp.x * (ds * in), dir.y + pp.y * (ds * in), dir.z + pp.z * (ds * in));
+        const float l2 = nd.x * nd.x + nd.y * nd.y + nd.z * nd.z;
+        if (l2 > 1e-12f) {
+            const float il = 1.0f / sqrtf(l2);
+            dir = mk(nd.x * il, nd.y * il, nd.z * il);
+        }
+        pos = mk(pos.x + dir.x * ds, pos.y + dir.y * ds, pos.z + dir.z * ds);
+    }
+    return {pos, dir};
+}
+
+// ---------------------------------------------------------------------------
 // sampling helpers
 // ---------------------------------------------------------------------------
 // cosineWeightedDirection (raycommon.cpp:94-115)
@@ -631,6 +836,64 @@ __device__ __forceinline__ PointSample sample_point(const PTScene &sc, Rng &R, v
     return s;
 }
 
+// sampleVolumeLightForNEE (rayrenderer.cpp:2066-2203) up to the shadow
+// test: one emissive volume picked uniformly (a rand() draw), equiangular
+// sampling (Kulla & Fajardo 2012) of its cone axis, or of the owning
+// object's location for constant emission; Le = sigma_e at the point times
+// the medium's homogeneous transmittance over the distance; pdf = the
+// solid-angle pdf times the number of volumes.  The shadow ray (unnormalised,
+// tmax 0.99999) is tested against the volume lights' occluders.
+struct VolumeSample {
+    bool ok;
+    v3 o, lr, dir, Le;
+    float pdf;
+};
+
+__device__ __forceinline__ VolumeSample sample_volume(const PTScene &sc, Rng &R, v3 hit, v3 n) {
+    VolumeSample s;
+    s.ok = false;
+    if (sc.nvlights == 0) return s;
+    const DevVolumeLight ev = sc.vlights[pick(prng(R), sc.nvlights)];
+    const DevMedium &m = sc.media[ev.medium];
+    v3 a, b;
+    if (m.src.emission_mode == 1) {
+        a = ld3(m.src.cone_origin);
+        b = add(a, scl(mk(m.cone_axis[0], m.cone_axis[1], m.cone_axis[2]), m.src.cone_length));
+    } else {
+        a = b = mk(ev.pos[0], ev.pos[1], ev.pos[2]);
+    }
+    const v3 sd = sub(b, a);
+    const float sl = len(sd);
+    const v3 axis = sl > 1e-6f ? scl(sd, 1.0f / sl) : mk(0.0f, 0.0f, -1.0f);
+    const v3 toA = sub(a, hit);
+    const float along = dot(toA, axis);
+    const v3 perp = sub(toA, scl(axis, along));
+    const float D = len(perp);
+    if (!(D > 1e-5f) && sl <= 1e-6f) return s;
+    const float thA = atanf(along / fmaxf(D, 1e-5f));
+    const float thB = atanf((along + sl) / fmaxf(D, 1e-5f));
+    if (!(fabsf(thB - thA) > 1e-6f) && sl > 1e-6f) return s;
+    const float u = prng(R);
+    const float th = thA + u * (thB - thA);
+    const float ts = D * tanf(th);
+    const v3 q = sl > 1e-6f ? add(a, scl(axis, ts - along)) : a;
+    const v3 tq = sub(q, hit);
+    const float r2 = tq.x * tq.x + tq.y * tq.y + tq.z * tq.z;
+    const float r = sqrtf(r2);
+    if (!(r > 1e-5f)) return s;
+    v3 se = emission_at(m, q);
+    if (is_zero(se)) return s;
+    if (m.hero > 0.0f) se = mul(se, transmittance(m, r));
+    const float pdf_t = D / (fmaxf(1e-6f, fabsf(thB - thA)) * (D * D + ts * ts));
+    s.ok = true;
+    s.o = surface_origin(hit, tq, n);
+    s.lr = tq;
+    s.dir = scl(tq, 1.0f / r);
+    s.Le = se;
+    s.pdf = pdf_t * r2 * (float)sc.nvlights;
+    return s;
+}
+
 struct EnvSample {
     bool ok;
     v3 dir, Le, o;
@@ -703,6 +966,10 @@ __device__ __forceinline__ void step_transparency(const PTScene &sc, Path &p, ui
 
 // MixShader (bsdf.cpp:579-610) with one lobe drawn in proportion to its
 // weight; tex = the albedo texture sample at the hit (1 when untextured)
+// kExt: the scene uses a heterogeneous density, cone or path emission, heat
+// haze or emissive-volume NEE; the kernels of other scenes are compiled
+// without that code (it costs the common path registers).
+template <bool kExt>
 __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n,
                                          v3 tex) {
     const DevObject &m = obj_of(sc, prim);
@@ -743,6 +1010,15 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
                 w = e2 / (e2 + b2);
             }
             push_shadow(S, es.o, es.dir, 1e30f, RG_PRED_ENV, mul(ba, scl(es.Le, es.co * w / (PI_F * es.pdf))));
+        }
+        if constexpr (kExt) {
+            // emissive-volume NEE (bsdf.cpp:92-103), no MIS
+            const VolumeSample vs = sample_volume(sc, R, hit, n);
+            if (vs.ok && vs.pdf > 0.0f) {
+                const float cl = fmaxf(0.0f, dot(vs.dir, n));
+                push_shadow(S, vs.o, vs.lr, 0.99999f, RG_PRED_VOLUME,
+                            mul(ba, scl(vs.Le, cl * (float)(1.0 / M_PI) / vs.pdf)));
+            }
         }
         p.beta = ba;
         p.tp = mul(p.tp, alb);
@@ -794,6 +1070,14 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
                 push_shadow(S, es.o, es.dir, 1e30f, RG_PRED_ENV,
                             mul(p.beta, scl(mul(es.Le, fc), (pe2 / (pe2 + pb2)) / es.pdf)));
             }
+        }
+        if constexpr (kExt) {
+            // emissive-volume NEE (bsdf.cpp:318-329), no MIS
+            const VolumeSample vs = sample_volume(sc, R, hit, n);
+            v3 fc;
+            float pb;
+            if (vs.ok && vs.pdf > 0.0f && eval_glossy(G, vs.dir, fc, pb))
+                push_shadow(S, vs.o, vs.lr, 0.99999f, RG_PRED_VOLUME, mul(p.beta, scl(mul(vs.Le, fc), 1.0f / vs.pdf)));
         }
         const float u1 = prng(R), u2 = prng(R);
         const v3 H = sample_vndf(G.V, G.ax, G.ay, u1, u2);
@@ -857,6 +1141,7 @@ __device__ __forceinline__ void step_mix(const PTScene &sc, Rng &R, Path &p, Sha
 }
 
 // RayBSDFShaderProvider::shade (rayrenderer.cpp:2776-2896) as one path event
+template <bool kExt>
 __device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p, Shadows &S, uint32_t prim, v3 hit, v3 n,
                                              v3 tex) {
     const DevObject &m = obj_of(sc, prim);
@@ -921,7 +1206,7 @@ __device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p,
             step_transparency(sc, p, prim, hit, n, m.transparency, false);
             return;
         }
-        step_mix(sc, R, p, S, prim, hit, n, tex);
+        step_mix<kExt>(sc, R, p, S, prim, hit, n, tex);
         return;
     }
     p.passes = 0;
@@ -937,10 +1222,11 @@ __device__ __forceinline__ void step_surface(const PTScene &sc, Rng &R, Path &p,
             return;
         }
     }
-    step_mix(sc, R, p, S, prim, hit, n, tex);
+    step_mix<kExt>(sc, R, p, S, prim, hit, n, tex);
 }
 
 // RayRenderer::tracePath (rayrenderer.cpp:1596-1864) as one path event
+template <bool kExt>
 __device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows &S, const rg_hit &h) {
     const bool eye = p.passes < 0;
     const bool hit = h.prim != 0xFFFFFFFFu;
@@ -948,7 +1234,7 @@ __device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows
     const DevMedium *med = (p.medium >= 0 && sc.media[p.medium].active) ? &sc.media[p.medium] : nullptr;
     if (!med) {
         if (hit && (!eye || obj_of(sc, h.prim).visible)) {
-            step_surface(sc, R, p, S, h.prim, pos, interp_normal(sc, h.prim, h.u, h.v),
+            step_surface<kExt>(sc, R, p, S, h.prim, pos, interp_normal(sc, h.prim, h.u, h.v),
                          albedo_tex(sc, h.prim, h.u, h.v));
             return;
         }
@@ -962,14 +1248,37 @@ __device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows
         p.alive = false;
         return;
     }
+    if (kExt && med->haze) {
+        // heat haze (rayrenderer.cpp:1645-1663): the ray bends through the
+        // volume and continues in the outer medium as the path's next event,
+        // with the same passes, throughput and BSDF pdf (a heat-haze global
+        // medium is refused, so the outer one is never heat haze)
+        const Ray3 b = bend_ray(*med, p.o, p.d, fminf(hit ? h.t : RAY_MAX_DISTANCE, 1000.0f));
+        p.o = b.o;
+        p.d = b.d;
+        p.medium = sc.global_medium;
+        return;
+    }
     if (eye && !hit) p.bg = true;
     const float maxT = hit ? h.t : RAY_MAX_DISTANCE;
-    const float tf = free_flight(*med, prng(R));
-    const bool scattered = tf < maxT;
-    p.L = add(p.L, mul(p.beta, emission_along(*med, scattered ? tf : maxT)));
+    bool scattered;
+    float tf = 0.0f, den = 1.0f;
+    if (kExt && med->het) {
+        // delta tracking through the fBm density (rayrenderer.cpp:1677-1686)
+        scattered = delta_track(*med, R, p.o, p.d, maxT, tf, den);
+        if (!scattered) tf = maxT;
+    } else {
+        tf = free_flight(*med, prng(R));
+        scattered = tf < maxT;
+    }
+    const float elen = scattered ? tf : maxT;
+    p.L = add(p.L, mul(p.beta, kExt ? emission_integral(*med, R, p.o, p.d, elen) : emission_along(*med, elen)));
     if (scattered) {
         v3 sw = mk(0, 0, 0);
-        {
+        if (kExt && med->het) {
+            const float inv = med->t_max_hero > 0.0f ? 1.0f / med->t_max_hero : 0.0f;
+            sw = mk(med->sigma_s[0] * den * inv, med->sigma_s[1] * den * inv, med->sigma_s[2] * den * inv);
+        } else {
             const float ph = med->hero * expf(-med->hero * tf);
             const v3 Tr = transmittance(*med, tf);
             if (ph > 0.0f)
@@ -999,6 +1308,12 @@ __device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows
                 push_shadow(S, es.o, es.dir, 1e30f, RG_PRED_ENV,
                             mul(bsw, scl(es.Le, phase_pdf(*med, neg(p.d), es.dir) * (1.0f / es.pdf))));
         }
+        if constexpr (kExt) {
+            const VolumeSample vs = sample_volume(sc, R, spos, proxy);
+            if (vs.ok && vs.pdf > 0.0f)
+                push_shadow(S, vs.o, vs.lr, 0.99999f, RG_PRED_VOLUME,
+                            mul(bsw, scl(vs.Le, phase_pdf(*med, neg(p.d), vs.dir) * (1.0f / vs.pdf))));
+        }
         const float u1 = prng(R), u2 = prng(R);
         const v3 nd = sample_phase(*med, p.d, u1, u2);
         float rrw = 1.0f;
@@ -1019,7 +1334,7 @@ __device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows
         return;
     }
     v3 sw = mk(1, 1, 1);
-    {
+    if (!(kExt && med->het)) {
         const float surv = expf(-med->hero * maxT);
         if (surv > 0.0f) {
             const v3 Tr = transmittance(*med, maxT);
@@ -1032,7 +1347,7 @@ __device__ __forceinline__ void step(const PTScene &sc, Rng &R, Path &p, Shadows
         const v3 g = ld3(sc.shade[h.prim].gn);
         const v3 gN = dot(g, p.d) <= 0.0f ? g : neg(g);
         if (dot(p.d, n) > 0.0f && dot(p.d, gN) < 0.0f) n = gN;
-        step_surface(sc, R, p, S, h.prim, pos, n, albedo_tex(sc, h.prim, h.u, h.v));
+        step_surface<kExt>(sc, R, p, S, h.prim, pos, n, albedo_tex(sc, h.prim, h.u, h.v));
         return;
     }
     v3 env = sample_environment(sc, p.d);

@@ -40,7 +40,7 @@ inline float vlen(V a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
 void free_pt_scene(Scene *s) {
     for (void **p : {&s->d_v9, &s->d_shade, &s->d_objs, &s->d_lights, &s->d_light_tris, &s->d_media,
                      &s->d_env_rgb, &s->d_env_marg, &s->d_env_cond, &s->d_texels, &s->d_tex_desc, &s->d_otex,
-                     &s->d_uv6, &s->d_plights}) {
+                     &s->d_uv6, &s->d_plights, &s->d_vlights}) {
         dev_free(*p);
         *p = nullptr;
     }
@@ -99,7 +99,25 @@ int upload_pt_scene(Scene *s, void *stream) {
         }
         q.hero = (q.sigma_t[0] + q.sigma_t[1] + q.sigma_t[2]) * (1.0f / 3.0f);
         q.g = m.g;
-        q.active = (q.hero > 0.0f || q.sigma_e[0] != 0.0f || q.sigma_e[1] != 0.0f || q.sigma_e[2] != 0.0f) ? 1 : 0;
+        q.src = m;
+        q.het = m.density_field == 1 ? 1 : 0;
+        const float peak = q.het ? fmaxf(0.0f, m.noise_amplitude * 1.0f + m.noise_bias) : 1.0f;
+        q.t_max_hero = q.het ? q.hero * peak : q.hero;
+        const float al2 = m.cone_axis[0] * m.cone_axis[0] + m.cone_axis[1] * m.cone_axis[1] +
+                          m.cone_axis[2] * m.cone_axis[2];
+        if (al2 > 1e-12f) {
+            const float inv = 1.0f / sqrtf(al2);
+            for (int k = 0; k < 3; k++) q.cone_axis[k] = m.cone_axis[k] * inv;
+        } else {
+            q.cone_axis[0] = q.cone_axis[1] = 0.0f;
+            q.cone_axis[2] = -1.0f;
+        }
+        q.haze = (m.heat_haze && m.ior_amplitude > 0.0f && m.ior_march_steps > 0) ? 1 : 0;
+        // isActive (medium.h:214-223)
+        const bool emits = q.sigma_e[0] != 0.0f || q.sigma_e[1] != 0.0f || q.sigma_e[2] != 0.0f;
+        q.active = (q.hero > 0.0f || (m.emission_mode == 1 && m.cone_intensity > 0.0f) ||
+                    (m.emission_mode == 2 && m.path_intensity > 0.0f && m.n_path >= 2) ||
+                    (m.heat_haze && m.ior_amplitude > 0.0f) || emits) ? 1 : 0;
     }
     std::vector<float> marg, cond;
     s->env_total = 0.0f;
@@ -177,6 +195,7 @@ int upload_pt_scene(Scene *s, void *stream) {
     if (!rc) rc = upload_vec(&s->d_lights, lights, stream);
     if (!rc) rc = upload_vec(&s->d_light_tris, light_tris, stream);
     if (!rc) rc = upload_vec(&s->d_plights, plights, stream);
+    if (!rc) rc = upload_vec(&s->d_vlights, s->volume_lights, stream);
     if (!rc) rc = upload_vec(&s->d_media, media, stream);
     if (!rc) rc = upload_vec(&s->d_env_rgb, s->env_rgb, stream);
     if (!rc) rc = upload_vec(&s->d_env_marg, marg, stream);
@@ -236,6 +255,19 @@ int ensure_buffers(PTBuffers &b, uint64_t cap, uint64_t npix, uint64_t ntiles) {
 
 }  // namespace
 
+int path_events(const Scene &s, uint32_t max_depth) {
+    bool haze = false;
+    for (const rg_medium &m : s.media) haze = haze || (m.heat_haze && m.ior_amplitude > 0.0f && m.ior_march_steps > 0);
+    return (int)(max_depth + 2) * (haze ? 2 : 1);
+}
+
+bool scene_uses_ext(const Scene &s) {
+    if (!s.volume_lights.empty()) return true;
+    for (const rg_medium &m : s.media)
+        if (m.density_field != 0 || m.emission_mode != 0 || m.heat_haze) return true;
+    return false;
+}
+
 PTScene make_ptscene(const Scene &scn, const rg_settings &st) {
     const Scene *s = &scn;
     const uint32_t W = st.width, H = st.height;
@@ -249,6 +281,10 @@ PTScene make_ptscene(const Scene &scn, const rg_settings &st) {
     sc.nlights = s->nlights;
     sc.plights = (const DevPointLight *)s->d_plights;
     sc.nplights = (uint32_t)s->point_lights.size();
+    sc.vlights = (const DevVolumeLight *)s->d_vlights;
+    sc.nvlights = (uint32_t)s->volume_lights.size();
+    sc.events = path_events(*s, st.max_depth);
+    sc.ext = scene_uses_ext(*s) ? 1 : 0;
     sc.media = (const DevMedium *)s->d_media;
     sc.global_medium = s->global_medium;
     sc.env_rgb = (const float *)s->d_env_rgb;
@@ -333,6 +369,40 @@ void rg_object_default(rg_object *o) {
     o->visible = 1;
 }
 
+void rg_medium_default(rg_medium *m) {
+    if (!m) return;
+    std::memset(m, 0, sizeof *m);
+    // HomogeneousMedium's member initialisers (medium.h:66-205)
+    m->density = 1.0f;
+    m->cone_axis[2] = -1.0f;
+    m->cone_length = 1.0f;
+    m->cone_radius = 0.3f;
+    const float inner[3] = {0.6f, 0.7f, 1.0f}, outer[3] = {1.0f, 0.4f, 0.05f}, pouter[3] = {0.3f, 0.4f, 0.6f};
+    std::memcpy(m->cone_inner, inner, sizeof inner);
+    std::memcpy(m->cone_outer, outer, sizeof outer);
+    m->cone_intensity = 50.0f;
+    m->cone_peak_axial = 0.15f;
+    m->cone_peak_sharpness = 5.0f;
+    m->cone_samples = 4;
+    std::memcpy(m->path_inner, inner, sizeof inner);
+    std::memcpy(m->path_outer, pouter, sizeof pouter);
+    m->path_intensity = 1.0f;
+    m->path_falloff_power = 2.0f;
+    m->path_samples = 6;
+    for (rg_path_point &q : m->path) q.radius = 0.1f;
+    m->noise_frequency = 1.0f;
+    m->noise_octaves = 4;
+    m->noise_gain = 0.5f;
+    m->noise_lacunarity = 2.0f;
+    m->noise_amplitude = 1.0f;
+    m->ior_amplitude = 0.005f;
+    m->ior_frequency = 4.0f;
+    m->ior_octaves = 3;
+    m->ior_gain = 0.5f;
+    m->ior_lacunarity = 2.0f;
+    m->ior_march_steps = 16;
+}
+
 void rg_settings_default(rg_settings *s) {
     if (!s) return;
     std::memset(s, 0, sizeof *s);
@@ -377,6 +447,7 @@ int rg_scene_set_shading(rg_scene *scene, const float *n9, const uint32_t *obj_o
         uint8_t mask = 1u << RG_PRED_ALL;
         if (m.transparency < 0.01f && m.emission <= 0.0f) mask |= 1u << RG_PRED_AREA;
         if (!(m.emission > 0.0f) && (m.transparency < 0.01f || m.refraction > 0.1f)) mask |= 1u << RG_PRED_ENV;
+        if (!(m.emission > 0.0f) && m.transparency < 0.01f && m.refraction <= 0.1f) mask |= 1u << RG_PRED_VOLUME;
         s->pred_mask[t] = mask;
     }
     s->has_shading = true;
@@ -472,6 +543,16 @@ int rg_scene_set_media(rg_scene *scene, const rg_medium *media, uint32_t n, int3
     if (global_medium >= (int32_t)n) return set_error(RG_E_INVALID, "global medium index out of range");
     // the path state keeps a path's medium index in 16 bits
     if (n > 32767) return set_error(RG_E_INVALID, "at most 32767 media");
+    for (uint32_t i = 0; i < n; i++) {
+        if (media[i].n_path > RG_MAX_PATH_POINTS)
+            return set_error(RG_E_INVALID, "medium " + std::to_string(i) + ": more than 16 path points");
+        if (media[i].emission_mode < 0 || media[i].emission_mode > 2 || media[i].density_field < 0 ||
+            media[i].density_field > 1)
+            return set_error(RG_E_INVALID, "medium " + std::to_string(i) + ": unknown emission or density mode");
+    }
+    if (global_medium >= 0 && media[global_medium].heat_haze && media[global_medium].ior_amplitude > 0.0f &&
+        media[global_medium].ior_march_steps > 0)
+        return set_error(RG_E_UNSUPPORTED, "a heat-haze global medium (tracePath recurses without end there)");
     s->media.assign(media, media + n);
     s->global_medium = global_medium < 0 ? -1 : global_medium;
     s->pt_dirty = true;
@@ -493,6 +574,27 @@ int rg_scene_set_env(rg_scene *scene, const rg_env *env) {
         s->env_rotation_deg = env->rotation_deg;
     }
     s->pt_dirty = true;
+    return RG_OK;
+}
+
+int rg_scene_set_volume_lights(rg_scene *scene, const rg_volume_light *v, uint32_t n) {
+    Scene *s = reinterpret_cast<Scene *>(scene);
+    if (!s) return set_error(RG_E_INVALID, "null scene");
+    if (n && !v) return set_error(RG_E_INVALID, "volume lights is null");
+    for (uint32_t i = 0; i < n; i++)
+        if (v[i].medium >= s->media.size())
+            return set_error(RG_E_INVALID, "volume light " + std::to_string(i) + ": medium index out of range");
+    s->volume_lights.assign(v, v + n);
+    s->pt_dirty = true;
+    return RG_OK;
+}
+
+int rg_scene_export_volume_lights(const rg_scene *scene, rg_volume_light *v, uint32_t *n) {
+    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    if (!s || !n) return set_error(RG_E_INVALID, "null argument");
+    *n = (uint32_t)s->volume_lights.size();
+    if (v && !s->volume_lights.empty())
+        std::memcpy(v, s->volume_lights.data(), s->volume_lights.size() * sizeof(rg_volume_light));
     return RG_OK;
 }
 

@@ -684,6 +684,8 @@ struct Loader {
     std::vector<std::string> obj_tex;      // texture path per object ("" none)
     std::vector<float> obj_tiling;         // 2 per object
     std::vector<rg_medium> media;
+    std::vector<Affine> media_model;       // owning object's model matrix (identity: global)
+    std::vector<rg_volume_light> volume_lights;  // pos: object location, world space until flatten
     int32_t global_medium = -1;
     std::vector<std::vector<std::pair<std::string, Mat>>> scopes;  // _materials per level
     bool has_camera = false;
@@ -717,6 +719,19 @@ struct Loader {
     static void num(const json::Value &o, const char *key, float &dst) {
         const json::Value *v = o.get(key);
         if (v && v->is_number()) dst = (float)v->num;
+    }
+    static void inum(const json::Value &o, const char *key, int32_t &dst) {
+        const json::Value *v = o.get(key);
+        if (v && v->is_number()) dst = (int32_t)v->num;
+    }
+    static bool is_true(const json::Value &o, const char *key) {
+        const json::Value *v = o.get(key);
+        return v && v->kind == json::Value::Bool && v->b;
+    }
+    static void vec3f(const json::Value &o, const char *key, float dst[3]) {
+        double v[3];
+        if (vec3_of(o.get(key), v))
+            for (int k = 0; k < 3; k++) dst[k] = (float)v[k];
     }
 
     // colour string: "#rrggbb", "#rrggbbaa" (and the reference's 3/4-digit forms)
@@ -775,25 +790,101 @@ struct Loader {
     }
 
     // SceneJsonLoader::readMedium (sceneloader.cpp:202-334), homogeneous part
-    int read_medium(const json::Value &o) {
+    // SceneJsonLoader::readMedium (sceneloader.cpp:202-334); `model` is the
+    // owning object's model matrix (identity for the global medium), used
+    // when the cone or path follows its object (medium.cpp:59-104)
+    int read_medium(const json::Value &o, const Affine &model) {
         rg_medium m;
-        std::memset(&m, 0, sizeof m);
-        m.density = 1.0f;
-        double v[3];
-        if (vec3_of(o.get("sigma_a"), v)) for (int k = 0; k < 3; k++) m.sigma_a[k] = (float)v[k];
-        if (vec3_of(o.get("sigma_s"), v)) for (int k = 0; k < 3; k++) m.sigma_s[k] = (float)v[k];
-        if (vec3_of(o.get("emission"), v)) for (int k = 0; k < 3; k++) m.sigma_e[k] = (float)v[k];
+        rg_medium_default(&m);
+        vec3f(o, "sigma_a", m.sigma_a);
+        vec3f(o, "sigma_s", m.sigma_s);
+        vec3f(o, "emission", m.sigma_e);
         num(o, "g", m.g);
         num(o, "density", m.density);
-        for (const char *k : {"emissionMode", "densityField", "heatHaze"})
-            if (o.get(k)) warnings.push_back(std::string("medium field outside the homogeneous path ignored: ") + k);
-        // an inactive medium is dropped (sceneloader.cpp:326-332)
+        const json::Value *mode = o.get("emissionMode");
+        if (mode && mode->is_string() && mode->str == "cone") m.emission_mode = 1;
+        else if (mode && mode->is_string() && mode->str == "path") m.emission_mode = 2;
+        vec3f(o, "coneAxis", m.cone_axis);
+        vec3f(o, "coneOrigin", m.cone_origin);
+        num(o, "coneLength", m.cone_length);
+        num(o, "coneRadius", m.cone_radius);
+        vec3f(o, "coneInner", m.cone_inner);
+        vec3f(o, "coneOuter", m.cone_outer);
+        num(o, "coneIntensity", m.cone_intensity);
+        num(o, "conePeakAxial", m.cone_peak_axial);
+        num(o, "conePeakSharpness", m.cone_peak_sharpness);
+        inum(o, "coneEmissionSamples", m.cone_samples);
+        const bool cone_follow = is_true(o, "coneFollowObject");
+        vec3f(o, "pathInner", m.path_inner);
+        vec3f(o, "pathOuter", m.path_outer);
+        num(o, "pathIntensity", m.path_intensity);
+        num(o, "pathFalloffPower", m.path_falloff_power);
+        inum(o, "pathEmissionSamples", m.path_samples);
+        const bool path_follow = is_true(o, "pathFollowObject");
+        if (const json::Value *pts = o.get("pathPoints")) {
+            if (pts->is_array()) {
+                for (const json::Value &e : pts->arr) {
+                    if (!e.is_object()) continue;
+                    if (m.n_path == RG_MAX_PATH_POINTS) {
+                        warnings.push_back("medium pathPoints beyond 16 ignored");
+                        break;
+                    }
+                    rg_path_point &q = m.path[m.n_path++];
+                    q.p[0] = q.p[1] = q.p[2] = 0.0f;
+                    q.radius = 0.1f;
+                    q.t = 0.0f;
+                    vec3f(e, "p", q.p);
+                    num(e, "radius", q.radius);
+                    num(e, "t", q.t);
+                }
+            }
+        }
+        const json::Value *df = o.get("densityField");
+        if (df && df->is_string() && df->str == "fbm") m.density_field = 1;
+        num(o, "noiseFrequency", m.noise_frequency);
+        inum(o, "noiseOctaves", m.noise_octaves);
+        num(o, "noiseGain", m.noise_gain);
+        num(o, "noiseLacunarity", m.noise_lacunarity);
+        num(o, "noiseAmplitude", m.noise_amplitude);
+        num(o, "noiseBias", m.noise_bias);
+        vec3f(o, "noiseOffset", m.noise_offset);
+        if (o.get("heatHaze")) m.heat_haze = is_true(o, "heatHaze") ? 1 : 0;
+        num(o, "iorAmplitude", m.ior_amplitude);
+        num(o, "iorFrequency", m.ior_frequency);
+        inum(o, "iorOctaves", m.ior_octaves);
+        num(o, "iorGain", m.ior_gain);
+        num(o, "iorLacunarity", m.ior_lacunarity);
+        inum(o, "iorMarchSteps", m.ior_march_steps);
+        vec3f(o, "iorOffset", m.ior_offset);
+        num(o, "iorFalloff", m.ior_falloff);
+        // prepare (medium.cpp:18-57): the cone axis normalised, -Z when degenerate
+        const float al2 = m.cone_axis[0] * m.cone_axis[0] + m.cone_axis[1] * m.cone_axis[1] +
+                          m.cone_axis[2] * m.cone_axis[2];
+        if (al2 > 1e-12f) {
+            const float inv = 1.0f / sqrtf(al2);
+            for (float &x : m.cone_axis) x *= inv;
+        } else {
+            m.cone_axis[0] = m.cone_axis[1] = 0.0f;
+            m.cone_axis[2] = -1.0f;
+        }
+        // an inactive medium is dropped (isActive, medium.h:214-223;
+        // sceneloader.cpp:326-332)
         const float d = fmaxf(0.0f, m.density);
-        const float hero = ((m.sigma_a[0] + m.sigma_s[0]) * d + (m.sigma_a[1] + m.sigma_s[1]) * d +
-                            (m.sigma_a[2] + m.sigma_s[2]) * d) * (1.0f / 3.0f);
-        const bool emits = m.sigma_e[0] * d != 0.0f || m.sigma_e[1] * d != 0.0f || m.sigma_e[2] * d != 0.0f;
-        if (!(hero > 0.0f) && !emits) return -1;
+        float st[3];
+        bool emits = false;
+        for (int k = 0; k < 3; k++) {
+            st[k] = m.sigma_a[k] * d + m.sigma_s[k] * d;
+            emits = emits || m.sigma_e[k] * d != 0.0f;
+        }
+        const float hero = (st[0] + st[1] + st[2]) * (1.0f / 3.0f);
+        const bool active = hero > 0.0f || (m.emission_mode == 1 && m.cone_intensity > 0.0f) ||
+                            (m.emission_mode == 2 && m.path_intensity > 0.0f && m.n_path >= 2) ||
+                            (m.heat_haze && m.ior_amplitude > 0.0f) || emits;
+        if (!active) return -1;
         media.push_back(m);
+        // the cone's / path's own frame: the object's, or world space
+        media_model.push_back((m.emission_mode == 1 && cone_follow) || (m.emission_mode == 2 && path_follow)
+                                  ? model : Affine());
         return (int)media.size() - 1;
     }
 
@@ -950,9 +1041,19 @@ struct Loader {
             } else if (key == "visible" && val.kind == json::Value::Bool) {
                 visible = val.b;
             } else if (key == "medium" && val.is_object()) {
-                const int m = read_medium(val);
-                if (is_root) global_medium = m;
-                else interior = m;
+                const int m = read_medium(val, model);
+                if (is_root) {
+                    global_medium = m;
+                    const rg_medium *g = m >= 0 ? &media[(size_t)m] : nullptr;
+                    if (g && g->heat_haze && g->ior_amplitude > 0.0f && g->ior_march_steps > 0) {
+                        // tracePath recurses without end in a heat-haze global medium
+                        warnings.push_back("heat-haze global medium dropped");
+                        global_medium = -1;
+                    }
+                } else {
+                    interior = m;
+                }
+                if (!is_root && m >= 0) register_volume(m, loc);
             } else if (key == "envmap") {
                 if (val.is_string()) env_path = resolve(val.str);
                 else if (val.is_object()) {
@@ -971,6 +1072,7 @@ struct Loader {
             }
         }
         if (!visible) {
+            hidden_volumes(o, model);
             scopes.pop_back();
             return true;
         }
@@ -998,6 +1100,42 @@ struct Loader {
         return true;
     }
 
+    // emissive-volume registration (rayrenderer.cpp:180-206): a cone of
+    // positive intensity or constant non-zero emission, at the object's own
+    // location (baked through the view matrix in flatten)
+    void register_volume(int m, const double loc[3]) {
+        const rg_medium &q = media[(size_t)m];
+        const float d = fmaxf(0.0f, q.density);
+        const bool cone = q.emission_mode == 1 && q.cone_intensity > 0.0f;
+        const bool cst = q.emission_mode == 0 &&
+                         (q.sigma_e[0] * d != 0.0f || q.sigma_e[1] * d != 0.0f || q.sigma_e[2] * d != 0.0f);
+        if (!cone && !cst) return;
+        volume_lights.push_back({(uint32_t)m, {(float)loc[0], (float)loc[1], (float)loc[2]}});
+    }
+
+    // objects under an invisible one are not rendered, but their media are
+    // still registered as emissive volumes, as the reference's registration
+    // walks every object (rayrenderer.cpp:185-204)
+    void hidden_volumes(const json::Value &o, const Affine &parent) {
+        for (auto &p : o.obj) {
+            if (!p.second.is_object() || p.first == "_materials" || p.first == "_bundles" || p.first == "mesh" ||
+                p.first == "mat" || p.first == "medium" || p.first == "envmap" || is_camera_like(p.first))
+                continue;
+            const json::Value &c = p.second;
+            double loc[3] = {0, 0, 0}, ang[3] = {0, 0, 0}, sc[3] = {1, 1, 1};
+            vec3_of(c.get("location"), loc);
+            vec3_of(c.get("angle"), ang);
+            vec3_of(c.get("scale"), sc);
+            const Affine model = parent.then_after(translation(loc).then_after(euler(ang).then_after(scaling(sc))));
+            if (const json::Value *md = c.get("medium"))
+                if (md->is_object()) {
+                    const int m = read_medium(*md, model);
+                    if (m >= 0) register_volume(m, loc);
+                }
+            hidden_volumes(c, model);
+        }
+    }
+
     uint32_t add_object(const Mat &m) {
         objects.push_back(m.o);
         obj_tex.push_back(m.tex);
@@ -1022,6 +1160,36 @@ struct Loader {
     // transformObject (rayrenderer.cpp:237-363): render-space vertices and
     // normalised normals, once the camera is known
     void flatten() {
+        // HomogeneousMedium::bake (medium.cpp:59-104): a cone's origin and
+        // axis, a path's points, into render space through the view matrix
+        // (and the object's model matrix when they follow it); media of
+        // other modes keep their authored cone fields
+        for (size_t i = 0; i < media.size(); i++) {
+            rg_medium &m = media[i];
+            if (m.emission_mode != 1 && m.emission_mode != 2) continue;
+            const Affine full = view.then_after(media_model[i]);
+            double q[3];
+            if (m.emission_mode == 1) {
+                full.apply(m.cone_origin, q);
+                double a[3];
+                for (int c = 0; c < 3; c++)
+                    a[c] = full.m[c][0] * m.cone_axis[0] + full.m[c][1] * m.cone_axis[1] + full.m[c][2] * m.cone_axis[2];
+                for (int c = 0; c < 3; c++) m.cone_origin[c] = (float)q[c];
+                const double l2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+                if (l2 > 1e-12)
+                    for (int c = 0; c < 3; c++) m.cone_axis[c] = (float)(a[c] / std::sqrt(l2));
+            } else {
+                for (uint32_t k = 0; k < m.n_path; k++) {
+                    full.apply(m.path[k].p, q);
+                    for (int c = 0; c < 3; c++) m.path[k].p[c] = (float)q[c];
+                }
+            }
+        }
+        for (rg_volume_light &v : volume_lights) {
+            double q[3];
+            view.apply(v.pos, q);
+            for (int c = 0; c < 3; c++) v.pos[c] = (float)q[c];
+        }
         // point lights (rayrenderer.cpp:339-352): the origin through the
         // view-model matrix; the direction from the object's angle.x / .y
         // read through RADIAN_TO_DEGREE (x * 180 / pi) in binary32, then the
@@ -1110,6 +1278,8 @@ extern "C" int rg_scene_load_json(rg_ctx *ctx, const char *path, rg_scene **out)
     if (!rc) rc = rg_scene_set_camera(sc, &L.camera);
     if (!rc && !L.point_lights.empty())
         rc = rg_scene_set_point_lights(sc, L.point_lights.data(), (uint32_t)L.point_lights.size());
+    if (!rc && !L.volume_lights.empty())
+        rc = rg_scene_set_volume_lights(sc, L.volume_lights.data(), (uint32_t)L.volume_lights.size());
     if (!rc && !L.env_path.empty()) {
         const size_t dot = L.env_path.rfind('.');
         const std::string ext = dot == std::string::npos ? "" : L.env_path.substr(dot);

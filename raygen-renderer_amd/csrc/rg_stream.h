@@ -86,8 +86,9 @@ int stream_launch_camera(const BatchArgs &a, const void *d_jobs, uint32_t total,
 // The bounce loop (max_bounces events, the largest of the chunks' bounce
 // counts; each path stops after its own chunk's); the per-path kernels are sized for `paths` slots
 // (at least the generation's paths, at most the buffers' slots).
+// ext: the scene needs the extended-media shade kernels (scene_uses_ext)
 int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, const DevBvh &bv, uint32_t paths,
-                          uint32_t max_bounces, const StreamTune &tune, int n_cu, void *stream);
+                          uint32_t max_bounces, const StreamTune &tune, int n_cu, bool ext, void *stream);
 // One launch for the generation: every chunk summed per pixel into its
 // job's sums (the single-job path's accumulation of one generation, a job's
 // chunks in order), and the HDR image (mean * exposure, clamped at 0) of

@@ -168,6 +168,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(7))) v
 // event (bounce + 1 = max_depth + 2): a closed path keeps its radiance and
 // its last event's NEE rays for the accumulation, as the single-job path
 // closes its paths.
+template <bool kExt>
 __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk &C, const PTScene &sc, uint32_t p,
                                               uint32_t bounce, bool &alive) {
     const uint32_t local = p - C.first;
@@ -191,7 +192,7 @@ __device__ __forceinline__ uint32_t shade_one(const PTState &st, const GenChunk 
     P.alive = true;
     Shadows S = shadows_of(st, p);
     const rg_hit h = st.hit[p];
-    step(sc, R, P, S, h);
+    step<kExt>(sc, R, P, S, h);
     st.ray[p] = {{P.o.x, P.o.y, P.o.z}, 0.0f, {P.d.x, P.d.y, P.d.z}, MAX_RAY_DISTANCE};
     st.beta[p] = v4(P.beta, P.bsdf_pdf);
     st.tp[p] = v4(P.tp, 0.0f);
@@ -232,6 +233,7 @@ __device__ __forceinline__ void uniform_copy(T &dst, const T *src) {
 // record in scalar registers (the single-job shade kernel's register budget
 // and occupancy); the other paths of the wave (waves that straddle two
 // chunks) go to the mixed queue for k_bshade_mixed.
+template <bool kExt>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_bshade(
     BatchArgs a, const DevJobSlot *__restrict__ jobs, int qi, uint32_t bounce) {
     __shared__ uint32_t lds[kPtBlock / 64 + 1];
@@ -255,13 +257,14 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4))) v
         uniform_copy(C, &a.chunks[c0]);
         uniform_copy(sc, &jobs[C.job].j.sc);
     }
-    if (mine) ns = shade_one(st, C, sc, p, bounce, alive);
+    if (mine) ns = shade_one<kExt>(st, C, sc, p, bounce, alive);
     queue_results(st, p, ns, alive, qi, lds);
 }
 
 // The mixed queue's path events, each with its own job's record (one
 // distinct job at a time across the wave).  A fixed grid walks the queue in
 // block-sized steps.
+template <bool kExt>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bshade_mixed(
     BatchArgs a, const DevJobSlot *__restrict__ jobs, int qi, uint32_t bounce) {
     __shared__ uint32_t lds[kPtBlock / 64 + 1];
@@ -274,7 +277,7 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3))) v
         if (i < count) {
             p = a.mixed[i];
             const GenChunk &C = a.chunks[a.chunk_of[p]];
-            with_job(jobs, C.job, [&](const DevJob &J) { ns = shade_one(st, C, J.sc, p, bounce, alive); });
+            with_job(jobs, C.job, [&](const DevJob &J) { ns = shade_one<kExt>(st, C, J.sc, p, bounce, alive); });
         }
         queue_results(st, p, ns, alive, qi, lds);
     }
@@ -388,7 +391,7 @@ int stream_launch_camera(const BatchArgs &a, const void *d_jobs, uint32_t total,
 
 template <int kStack, int kW>
 static void bounces_impl(const BatchArgs &a, const DevJobSlot *jobs, const DevBvh &bv, uint32_t paths,
-                         uint32_t max_bounces, const StreamTune &tune, int n_cu, hipStream_t s) {
+                         uint32_t max_bounces, const StreamTune &tune, int n_cu, bool ext, hipStream_t s) {
     const uint32_t flat = (std::min(paths, a.cap) + kPtBlock - 1) / kPtBlock;
     // persistent traversal grids: what the device holds at once, or fewer
     // when blocks_per_cu caps it
@@ -403,23 +406,28 @@ static void bounces_impl(const BatchArgs &a, const DevJobSlot *jobs, const DevBv
     int qi = 0;
     for (uint32_t b = 0; b < max_bounces; b++) {
         hipLaunchKernelGGL((k_bextend<kStack, kW>), dim3(ext_blocks), dim3(kPtBlock), 0, s, bv, a, qi);
-        hipLaunchKernelGGL(k_bshade, dim3(flat), dim3(kPtBlock), 0, s, a, jobs, qi, b);
-        hipLaunchKernelGGL(k_bshade_mixed, dim3(mixed_blocks), dim3(kPtBlock), 0, s, a, jobs, qi, b);
+        if (ext) {
+            hipLaunchKernelGGL(k_bshade<true>, dim3(flat), dim3(kPtBlock), 0, s, a, jobs, qi, b);
+            hipLaunchKernelGGL(k_bshade_mixed<true>, dim3(mixed_blocks), dim3(kPtBlock), 0, s, a, jobs, qi, b);
+        } else {
+            hipLaunchKernelGGL(k_bshade<false>, dim3(flat), dim3(kPtBlock), 0, s, a, jobs, qi, b);
+            hipLaunchKernelGGL(k_bshade_mixed<false>, dim3(mixed_blocks), dim3(kPtBlock), 0, s, a, jobs, qi, b);
+        }
         hipLaunchKernelGGL((k_banyhit<kStack, kW>), dim3(any_blocks), dim3(kPtBlock), 0, s, bv, a);
         qi ^= 1;
     }
 }
 
 int stream_launch_bounces(const BatchArgs &a, const void *d_jobs, const DevBvh &bv, uint32_t paths,
-                          uint32_t max_bounces, const StreamTune &tune, int n_cu, void *stream) {
+                          uint32_t max_bounces, const StreamTune &tune, int n_cu, bool ext, void *stream) {
     const DevJobSlot *jobs = (const DevJobSlot *)d_jobs;
     hipStream_t s = (hipStream_t)stream;
     const bool w8 = tune.width == 8;
     switch (tune.stack) {
-        case 32: w8 ? bounces_impl<32, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, s) : bounces_impl<32, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, s); break;
-        case 64: w8 ? bounces_impl<64, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, s) : bounces_impl<64, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, s); break;
-        case 128: w8 ? bounces_impl<128, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, s) : bounces_impl<128, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, s); break;
-        case 256: w8 ? bounces_impl<256, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, s) : bounces_impl<256, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, s); break;
+        case 32: w8 ? bounces_impl<32, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, ext, s) : bounces_impl<32, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, ext, s); break;
+        case 64: w8 ? bounces_impl<64, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, ext, s) : bounces_impl<64, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, ext, s); break;
+        case 128: w8 ? bounces_impl<128, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, ext, s) : bounces_impl<128, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, ext, s); break;
+        case 256: w8 ? bounces_impl<256, 8>(a, jobs, bv, paths, max_bounces, tune, n_cu, ext, s) : bounces_impl<256, 4>(a, jobs, bv, paths, max_bounces, tune, n_cu, ext, s); break;
         default: return set_error(RG_E_INVALID, "stack must be 32, 64, 128 or 256");
     }
     return chk_s(hipGetLastError(), "batch bounce launch");

@@ -80,6 +80,9 @@ def lib():
             "rg_scene_set_media": (i32, [vp, vp, u32, i32]),
             "rg_scene_set_env": (i32, [vp, vp]),
             "rg_scene_set_point_lights": (i32, [vp, vp, u32]),
+            "rg_scene_set_volume_lights": (i32, [vp, vp, u32]),
+            "rg_scene_export_volume_lights": (i32, [vp, vp, vp]),
+            "rg_medium_default": (None, [vp]),
             "rg_scene_export_point_lights": (i32, [vp, vp, vp]),
             "rg_scene_set_camera": (i32, [vp, vp]),
             "rg_render": (i32, [vp, vp, vp, vp, vp, vp, vp]),
@@ -338,6 +341,12 @@ class RenderScene(Scene):
         check(lib().rg_scene_set_media(self.h, _ptr(media) if len(media) else None, len(media),
                                        int(sc.get("global_medium", -1))))
         check(lib().rg_scene_set_shading(self.h, _ptr(n9), _ptr(obj), _ptr(objs), len(objs)))
+        vl = sc.get("volume_lights")
+        if vl is not None and len(vl):
+            from scenes import VOLUME_LIGHT_DTYPE
+            vla = np.ascontiguousarray(vl, dtype=VOLUME_LIGHT_DTYPE)
+            self._keep.append(vla)
+            check(lib().rg_scene_set_volume_lights(self.h, _ptr(vla), len(vla)))
         env = sc.get("env")
         if env is not None:
             rgb = np.ascontiguousarray(env["rgb"], np.float32)
@@ -396,6 +405,13 @@ def load_json(ctx, path):
          "global_medium": gm.value, "warnings": [w for w in buf.value.decode().split("\n") if w]}
     if env.w and env.h:
         d["env"] = {"rgb": env_rgb, "intensity": env.intensity, "rotation_deg": env.rotation_deg}
+    nvl = ctypes.c_uint32()
+    check(lib().rg_scene_export_volume_lights(h, None, ctypes.byref(nvl)))
+    if nvl.value:
+        from scenes import VOLUME_LIGHT_DTYPE
+        vla = np.zeros(nvl.value, VOLUME_LIGHT_DTYPE)
+        check(lib().rg_scene_export_volume_lights(h, _ptr(vla), ctypes.byref(nvl)))
+        d["volume_lights"] = vla
     npl = ctypes.c_uint32()
     check(lib().rg_scene_export_point_lights(h, None, ctypes.byref(npl)))
     if npl.value:

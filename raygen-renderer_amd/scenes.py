@@ -14,8 +14,48 @@ OBJECT_DTYPE = np.dtype([("color", "<f4", (3,)), ("glossy", "<f4"), ("roughness"
                          ("metallic", "<f4"), ("anisotropy", "<f4"), ("aniso_rotation", "<f4"),
                          ("emission", "<f4"), ("interior_medium", "<i4"), ("visible", "<u4"),
                          ("chroma_dispersion", "<f4"), ("pad", "<u4")])
+# rg_medium (HomogeneousMedium's authoring fields, medium.h:37-214)
 MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", (3,)), ("sigma_s", "<f4", (3,)), ("sigma_e", "<f4", (3,)),
-                         ("g", "<f4"), ("density", "<f4"), ("pad", "<f4")])
+                         ("g", "<f4"), ("density", "<f4"),
+                         ("emission_mode", "<i4"), ("cone_origin", "<f4", (3,)), ("cone_axis", "<f4", (3,)),
+                         ("cone_length", "<f4"), ("cone_radius", "<f4"), ("cone_inner", "<f4", (3,)),
+                         ("cone_outer", "<f4", (3,)), ("cone_intensity", "<f4"), ("cone_peak_axial", "<f4"),
+                         ("cone_peak_sharpness", "<f4"), ("cone_samples", "<i4"),
+                         ("path", [("p", "<f4", (3,)), ("radius", "<f4"), ("t", "<f4")], (16,)),
+                         ("n_path", "<u4"), ("path_inner", "<f4", (3,)), ("path_outer", "<f4", (3,)),
+                         ("path_intensity", "<f4"), ("path_falloff_power", "<f4"), ("path_samples", "<i4"),
+                         ("density_field", "<i4"), ("noise_frequency", "<f4"), ("noise_octaves", "<i4"),
+                         ("noise_gain", "<f4"), ("noise_lacunarity", "<f4"), ("noise_amplitude", "<f4"),
+                         ("noise_bias", "<f4"), ("noise_offset", "<f4", (3,)),
+                         ("heat_haze", "<i4"), ("ior_amplitude", "<f4"), ("ior_frequency", "<f4"),
+                         ("ior_octaves", "<i4"), ("ior_gain", "<f4"), ("ior_lacunarity", "<f4"),
+                         ("ior_march_steps", "<i4"), ("ior_offset", "<f4", (3,)), ("ior_falloff", "<f4")])
+VOLUME_LIGHT_DTYPE = np.dtype([("medium", "<u4"), ("pos", "<f4", (3,))])
+
+
+def medium(**kw):
+    """One rg_medium with the reference's defaults (medium.h:37-214), fields
+    overridden by keyword (path = a list of (p, radius, t))."""
+    m = np.zeros(1, MEDIUM_DTYPE)[0]
+    defaults = dict(density=1.0, cone_axis=(0.0, 0.0, -1.0), cone_length=1.0, cone_radius=0.3,
+                    cone_inner=(0.6, 0.7, 1.0), cone_outer=(1.0, 0.4, 0.05), cone_intensity=50.0,
+                    cone_peak_axial=0.15, cone_peak_sharpness=5.0, cone_samples=4, path_inner=(0.6, 0.7, 1.0),
+                    path_outer=(0.3, 0.4, 0.6), path_intensity=1.0, path_falloff_power=2.0, path_samples=6,
+                    noise_frequency=1.0, noise_octaves=4, noise_gain=0.5, noise_lacunarity=2.0,
+                    noise_amplitude=1.0, ior_amplitude=0.005, ior_frequency=4.0, ior_octaves=3, ior_gain=0.5,
+                    ior_lacunarity=2.0, ior_march_steps=16)
+    defaults.update(kw)
+    path = defaults.pop("path", None)
+    m["path"]["radius"] = 0.1       # PathSample's default radius
+    for k, v in defaults.items():
+        m[k] = v
+    if path is not None:
+        m["n_path"] = len(path)
+        for i, (p, r, t) in enumerate(path):
+            m["path"][i]["p"] = p
+            m["path"][i]["radius"] = r
+            m["path"][i]["t"] = t
+    return m
 # rg_point_light: a point light source in render space (an emissive object
 # without meshes, rayrenderer.cpp:333-353)
 POINT_LIGHT_DTYPE = np.dtype([("pos", "<f4", (3,)), ("normal", "<f4", (3,)), ("color", "<f4", (3,)),
@@ -417,3 +457,49 @@ def point_light_plane(height=1.0, emission=2.0, color=(1.0, 0.5, 0.25), albedo=(
     b.quad(pl, (-4, -1.0, -1), (4, -1.0, -1), (4, -1.0, -7), (-4, -1.0, -7), normal=(0, 1, 0))
     return b.finish(camera={"fov_deg": 60.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0},
                     point_lights=[point_light((0.0, -1.0 + height, -3.0), color=color, emission=emission)])
+
+
+def media_room(env_value=0.15):
+    """The media of rank-4 "extended media": a floor and a back wall with an
+    area light and a dim constant environment; inside transparent boxes an
+    fBm cloud (scattering, delta tracking), a cone flame (emissive, also an
+    NEE volume light), a path-tube glow and a heat-haze volume in front of a
+    striped wall (medium.cpp:125-472; rayrenderer.cpp:1596-1864, 2066-2203)."""
+    b = Builder()
+    fl = b.add_object(material(color=(0.6, 0.6, 0.6)))
+    b.quad(fl, (-4, -1.0, 0), (4, -1.0, 0), (4, -1.0, -8), (-4, -1.0, -8), normal=(0, 1, 0))
+    back = b.add_object(material(color=(0.7, 0.7, 0.7)))
+    b.quad(back, (-4, -1, -6), (4, -1, -6), (4, 3, -6), (-4, 3, -6), normal=(0, 0, 1))
+    for i in range(6):     # stripes behind the haze volume
+        st = b.add_object(material(color=(0.9, 0.2, 0.1) if i % 2 else (0.1, 0.3, 0.9)))
+        x0 = 0.4 + 0.25 * i
+        b.quad(st, (x0, -1, -5.98), (x0 + 0.25, -1, -5.98), (x0 + 0.25, 1.5, -5.98), (x0, 1.5, -5.98),
+               normal=(0, 0, 1))
+    light = b.add_object(material(color=(1.0, 1.0, 1.0), emission=6.0))
+    b.quad(light, (-0.5, 2.5, -4.0), (0.5, 2.5, -4.0), (0.5, 2.5, -3.0), (-0.5, 2.5, -3.0), normal=(0, -1, 0))
+    media = [
+        medium(sigma_a=(0.05, 0.05, 0.05), sigma_s=(2.0, 2.0, 2.0), g=0.6, density_field=1, noise_frequency=1.6,
+               noise_octaves=4, noise_amplitude=1.4, noise_bias=-0.25),
+        medium(emission_mode=1, cone_origin=(-0.6, -0.2, -4.0), cone_axis=(-1.0, 0.3, 0.0), cone_length=1.2,
+               cone_radius=0.3, cone_inner=(4.0, 6.0, 12.0), cone_outer=(10.0, 3.0, 0.4), cone_intensity=2.5,
+               cone_peak_axial=0.18, cone_peak_sharpness=4.0, cone_samples=8, density_field=1,
+               noise_frequency=2.0, noise_octaves=3, noise_amplitude=1.2, noise_bias=-0.1),
+        medium(emission_mode=2, path=[((-0.4, 0.6, -3.0), 0.15, 0.0), ((0.0, 0.9, -3.2), 0.2, 0.5),
+                                      ((0.4, 0.7, -3.1), 0.1, 1.0)], path_intensity=3.0, path_samples=6),
+        medium(heat_haze=1, ior_amplitude=0.02, ior_frequency=4.0, ior_octaves=3, ior_march_steps=16),
+    ]
+    clear = dict(color=(1.0, 1.0, 1.0), transparency=1.0)
+    cloud = b.add_object(material(interior_medium=0, **clear))
+    b.box(cloud, (0.3, -0.6, -4.4), (1.7, 0.4, -3.4))
+    flame = b.add_object(material(interior_medium=1, **clear))
+    b.box(flame, (-2.0, -0.6, -4.4), (-0.5, 0.4, -3.6))
+    tube = b.add_object(material(interior_medium=2, **clear))
+    b.box(tube, (-0.7, 0.3, -3.5), (0.7, 1.2, -2.8))
+    haze = b.add_object(material(interior_medium=3, **clear))
+    b.box(haze, (0.3, -0.9, -2.8), (2.0, 1.2, -2.0))
+    sc = b.finish(camera={"fov_deg": 60.0, "exposure": 1.0, "aperture": 1.8, "depth_of_field": 0.0},
+                  media=np.array(media, MEDIUM_DTYPE), env=constant_env(env_value) if env_value > 0 else None)
+    # the flame is registered as an NEE volume light (a cone of positive
+    # intensity); the path tube is not (rayrenderer.cpp:193-203)
+    sc["volume_lights"] = np.array([(1, (-1.25, -0.1, -4.0))], VOLUME_LIGHT_DTYPE)
+    return sc
