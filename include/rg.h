@@ -79,6 +79,7 @@ typedef struct {
 #define RG_PRED_AREA 1 /* transparency<0.01 && emission<=0 (rayrenderer.cpp:1956-1959) */
 #define RG_PRED_ENV 2  /* !(emission>0) && (transparency<0.01 || refraction>0.1) (:1499-1503; point lights :2211-2215) */
 #define RG_PRED_VOLUME 3 /* !(emission>0) && transparency<0.01 && refraction<=0.1 (volume lights, :2172-2176) */
+#define RG_PRED_OPAQUE 4 /* transparency<0.01 (calcAO, :2391-2393) */
 
 /* ---- context ------------------------------------------------------------ */
 /* device >= 0: HIP device ordinal.  device == -1: host-only context (scene
@@ -266,6 +267,16 @@ typedef struct {
      * (rayrenderer.cpp:1273, 1296) and the albedo guide of a background pixel
      * (:841); alpha is kept for the guide only */
     float back_color[4];
+    /* shaderProvider (-d, rayrenderer.cpp:59-77): 5 = the BSDF path tracer
+     * (default); 0 = RaySimpleShaderProvider, 1 = RayAmbientOcclusion-
+     * ShaderProvider, 2 = LambertShaderProvider, 3 = LambertWithAO-
+     * ShaderProvider (rayrenderer.h:446-475, lambert.cpp:14-67).  Systems
+     * 0-3 take one eye ray per sample and no bounces; their calcAO casts
+     * spp rays per sample (settings.samples, rayrenderer.cpp:2385). */
+    int32_t shader;
+    /* worldColor (-wc, rayrenderer.h:151; default (1, 0.95, 0.9) * 0.1):
+     * the ambient term lambertTraceLights adds (rayrenderer.cpp:2334) */
+    float world_color[3];
 } rg_settings;
 
 /* Point light: an emissive scene object without meshes, which

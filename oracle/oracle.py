@@ -224,20 +224,28 @@ class _Settings(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
                 ("max_depth", ctypes.c_uint32), ("antialias", ctypes.c_int32), ("firefly_clamp", ctypes.c_float),
                 ("seed", ctypes.c_uint32), ("job", ctypes.c_uint32), ("sample_start", ctypes.c_uint32),
-                ("color_sampling", ctypes.c_int32), ("back_color", ctypes.c_float * 4)]
+                ("color_sampling", ctypes.c_int32), ("back_color", ctypes.c_float * 4),
+                ("shader", ctypes.c_int32), ("world_color", ctypes.c_float * 3), ("ao_samples", ctypes.c_uint32)]
 
 
 BACK_COLOR = (1.0 * 0.2, 0.95 * 0.2, 0.9 * 0.2, 0.0)   # RendererSettings::backColor default (rayrenderer.h:152)
+# RendererSettings::worldColor default (1, 0.95, 0.9) * 0.1 (rayrenderer.h:151), binary32 products
+WORLD_COLOR = tuple(float(np.float32(c) * np.float32(0.1)) for c in (1.0, 0.95, 0.9))
 
 
 def _settings(width, height, spp, max_depth, antialias, firefly_clamp, seed, job, sample_start,
-              color_sampling=True, back_color=None):
+              color_sampling=True, back_color=None, shader=5, world_color=None, ao_samples=0):
     s = _Settings(width, height, spp, max_depth, 1 if antialias else 0, firefly_clamp, seed & 0xFFFFFFFF, job,
                   sample_start)
     s.color_sampling = 1 if color_sampling else 0
     bc = BACK_COLOR if back_color is None else tuple(back_color) + (0.0,) * (4 - len(back_color))
     for i in range(4):
         s.back_color[i] = bc[i]
+    s.shader = shader
+    wc = WORLD_COLOR if world_color is None else tuple(world_color)
+    for i in range(3):
+        s.world_color[i] = wc[i]
+    s.ao_samples = ao_samples
     return s
 
 
@@ -298,8 +306,11 @@ def _desc(scene, keep):
 
 
 def render(scene, width, height, spp, max_depth=5, mode=0, threads=8, antialias=True, firefly_clamp=10.0,
-           seed=0xC0DE0000, job=0, sample_start=0, active_tile=None, color_sampling=True, back_color=None):
-    """Returns (sum, sumsq), each (H, W, 3) float32, over the given samples."""
+           seed=0xC0DE0000, job=0, sample_start=0, active_tile=None, color_sampling=True, back_color=None,
+           shader=5, world_color=None, ao_samples=0):
+    """Returns (sum, sumsq), each (H, W, 3) float32, over the given samples.
+    shader selects the shading system (0-3, 5); ao_samples is calcAO's
+    sample count (settings.samples, the whole render's spp; 0 = spp)."""
     L = lib()
     if not hasattr(L, "_render_sig"):
         L.or_render.restype = ctypes.c_int
@@ -312,7 +323,7 @@ def render(scene, width, height, spp, max_depth=5, mode=0, threads=8, antialias=
     keep = []
     d = _desc(scene, keep)
     s = _settings(width, height, spp, max_depth, antialias, firefly_clamp, seed, job, sample_start, color_sampling,
-                  back_color)
+                  back_color, shader, world_color, ao_samples)
     sm = np.zeros((height, width, 3), np.float32)
     sq = np.zeros((height, width, 3), np.float32)
     at = None if active_tile is None else np.ascontiguousarray(active_tile, dtype=np.uint8)
@@ -350,7 +361,7 @@ def denoise(noisy_rgba, nrm, alb, dep, levels=5, sigma_c=0.4, sigma_n=128.0, sig
 
 def render_adaptive(scene, width, height, spp, max_depth=5, base=4, threshold=0.02, mode=1, threads=8,
                     antialias=True, firefly_clamp=10.0, seed=0xC0DE0000, job=0, color_sampling=True,
-                    back_color=None):
+                    back_color=None, shader=5, world_color=None):
     """renderAdaptive restated: returns (sum, sumsq, nspp per 32x32 tile)."""
     L = lib()
     L.or_render_adaptive.restype = ctypes.c_int
@@ -358,7 +369,8 @@ def render_adaptive(scene, width, height, spp, max_depth=5, base=4, threshold=0.
                                      ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     keep = []
     d = _desc(scene, keep)
-    s = _settings(width, height, spp, max_depth, antialias, firefly_clamp, seed, job, 0, color_sampling, back_color)
+    s = _settings(width, height, spp, max_depth, antialias, firefly_clamp, seed, job, 0, color_sampling, back_color,
+                  shader, world_color, spp)
     sm = np.zeros((height, width, 3), np.float32)
     sq = np.zeros((height, width, 3), np.float32)
     nt = np.zeros(((width + 31) // 32) * ((height + 31) // 32), np.uint32)

@@ -508,11 +508,13 @@ void or_closest1(const or_node *nodes, int64_t nnodes, const float *v9, const ui
 }
 
 /* intersectAny with the reference's shadow predicates (rayrenderer.cpp:1956-1959,
- * 1499-1503): pred 0 = every triangle, 1 = area-light NEE, 2 = envmap NEE. */
+ * 1499-1503, 2172-2176, 2391-2393): pred 0 = every triangle, 1 = area-light
+ * NEE, 2 = envmap NEE and point lights, 3 = volume lights, 4 = calcAO. */
 static int occludes(const or_object *m, uint8_t pred) {
     if (pred == 1) return m->transparency < 0.01f && m->emission <= 0.0f;
     if (pred == 2) return !(m->emission > 0.0f) && (m->transparency < 0.01f || m->refraction > 0.1f);
     if (pred == 3) return !(m->emission > 0.0f) && m->transparency < 0.01f && m->refraction <= 0.1f;
+    if (pred == 4) return m->transparency < 0.01f;
     return 1;
 }
 

@@ -222,6 +222,9 @@ typedef struct {
     uint32_t sample_start;
     int32_t color_sampling; /* RendererSettings::enableColorSampling (rayrenderer.h:106) */
     float back_color[4];    /* RendererSettings::backColor (rayrenderer.h:152) */
+    int32_t shader;         /* RendererSettings::shaderProvider (-d; rayrenderer.cpp:59-77): 0-3 or 5 */
+    float world_color[3];   /* RendererSettings::worldColor (rayrenderer.h:151) */
+    uint32_t ao_samples;    /* RendererSettings::samples as calcAO reads it (0 = spp) */
 } or_settings;
 
 int or_any1_pred(const or_node *nodes, int64_t nnodes, const float *v9, const uint32_t *perm,

@@ -113,6 +113,10 @@ typedef struct {
     /* enableColorSampling, backColor (rgb) */
     int color_sampling;
     v3 back;
+    /* shaderProvider, worldColor, calcAO's sample count */
+    int shader;
+    v3 world;
+    int ao_samples;
     /* linear RGB texels of each texture (sRGB decoded once) */
     float **tex_lin;
 } prep_t;
@@ -1169,8 +1173,12 @@ static v3 mix_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3 in_
     return c;
 }
 
-/* RayBSDFShaderProvider::shade (rayrenderer.cpp:2776-2896) */
+static v3 preview_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_d, v3 n, param_t *sp);
+
+/* RayBSDFShaderProvider::shade (rayrenderer.cpp:2776-2896); the provider
+ * of shading systems 0-3 instead (rayrenderer.cpp:59-77) */
 static v3 shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_o, v3 in_d, v3 n, param_t *sp) {
+    if (P->shader != 5) return preview_shade(P, R, h, in_d, n, sp);
     const or_object *m = obj_of(P, h->prim);
     param_t pm;
     pm.passes = 0;
@@ -1361,6 +1369,70 @@ static v3 trace_eye(const prep_t *P, rng_t *R, v3 o, v3 d) {
     if (!is_zero(env)) return env;
     /* settings.backColor, default (1, 0.95, 0.9) * 0.2 (rayrenderer.h:152) */
     return P->back;
+}
+
+/* ======================================================================== */
+/* shading systems 0-3 (-d 0..3, rayrenderer.cpp:59-77)                     */
+/* ======================================================================== */
+/* ThicknessRay (raycommon.h:69-71): the origin pushed SURFACE_THICKNESS
+ * along the normalised direction */
+static inline v3 thickness_origin(v3 o, v3 d) { return add(o, scl(nrm(d), SURFACE_THICKNESS)); }
+
+/* calcAO (rayrenderer.cpp:2381-2407): settings.samples cosine-weighted
+ * directions about the normal (cosineWeightedDirection, an ldsNext2D pair
+ * each); a direction whose ThicknessRay meets no triangle with transparency
+ * < 0.01 within traceDistance adds its cosine; the sum over the samples */
+static float calc_ao(const prep_t *P, rng_t *R, v3 hit, v3 n, float dist) {
+    float sum = 0.0f;
+    for (int i = 0; i < P->ao_samples; i++) {
+        const float r1 = lds1(R), r2 = lds1(R);
+        const v3 dir = cosine_dir(n, r1, r2);
+        if (!occluded(P, thickness_origin(hit, dir), dir, dist, 4)) sum += fmaxf(dot(dir, n), 0.0f);
+    }
+    return sum / (float)P->ao_samples;
+}
+
+/* lambertTraceLights (rayrenderer.cpp:2319-2335): one area light's
+ * traceAreaLight (the light picked by a rand() draw inside
+ * trace_area_light), the sum of every point light's tracePointLight, plus
+ * settings.worldColor */
+static v3 lambert_lights(const prep_t *P, rng_t *R, v3 hit, v3 n) {
+    v3 a = mk(0, 0, 0), p = mk(0, 0, 0);
+    if (P->nlights > 0) a = add(a, trace_area_light(P, R, hit, n));
+    const uint32_t np = P->d->plights ? P->d->nplights : 0;
+    for (uint32_t i = 0; i < np; i++) p = add(p, trace_point_light(P, &P->d->plights[i], hit, n));
+    return add(add(a, p), P->world);
+}
+
+/* RaySimpleShaderProvider and RayAmbientOcclusionShaderProvider
+ * (rayrenderer.h:446-475; traceDistance RAY_MAX_DISTANCE),
+ * LambertShaderProvider and LambertWithAOShaderProvider (lambert.cpp:14-67):
+ * a transparent surface continues with tracePath of the ThicknessRay along
+ * the incoming direction, with the caller's shaderParam */
+static v3 preview_shade(const prep_t *P, rng_t *R, const isect_t *h, v3 in_d, v3 n, param_t *sp) {
+    const or_object *m = obj_of(P, h->prim);
+    if (P->shader == 0) {
+        const float c = 0.1f + fmaxf(dot(nrm(mk(1.0f, 1.0f, 1.0f)), n), 0.0f);
+        return mk(c, c, c);
+    }
+    if (P->shader == 1) {
+        const float c = clamp01(calc_ao(P, R, h->pos, n, RAY_MAX_DISTANCE));
+        return mk(c, c, c);
+    }
+    if (m->emission > 0.0f) return scl(ld3(m->color), m->emission);
+    const v3 light = scl(lambert_lights(P, R, h->pos, n), 0.3f);
+    v3 c;
+    if (P->shader == 2) c = light;
+    else c = add(scl(light, 0.2f), scl(scl(light, 0.8f), calc_ao(P, R, h->pos, n, 2.0f)));
+    if (P->color_sampling) c = mul(mul(c, ld3(m->color)), albedo_tex(P, h));
+    if (m->transparency > 0.0f) {
+        const v3 next = scl(trace_path(P, R, thickness_origin(h->pos, in_d), in_d, sp), m->transparency);
+        if (P->shader == 2)
+            c = add(scl(mul(ld3(m->color), add(light, mk(0.5f, 0.5f, 0.5f))), 1.0f - m->transparency), next);
+        else
+            c = add(c, next);
+    }
+    return c;
 }
 
 /* ======================================================================== */
@@ -1767,7 +1839,8 @@ static v3 render_sample(const prep_t *P, const or_settings *S, int mode, int x, 
     v3 o, d;
     camera_ray(P, &R, x, y, S->antialias, &o, &d);
     v3 c;
-    if (mode == 0) {
+    if (mode == 0 || P->shader != 5) {
+        /* shading systems 0-3 have no lobe choice: one restatement */
         c = trace_eye(P, &R, o, d);
     } else {
         path_t p;
@@ -1904,6 +1977,9 @@ static int prepare(prep_t *P, const or_scene_desc *d, const or_settings *S) {
     P->max_depth = (int)S->max_depth;
     P->color_sampling = S->color_sampling != 0;
     P->back = mk(S->back_color[0], S->back_color[1], S->back_color[2]);
+    P->shader = S->shader;
+    P->world = mk(S->world_color[0], S->world_color[1], S->world_color[2]);
+    P->ao_samples = (int)(S->ao_samples ? S->ao_samples : (S->spp ? S->spp : 1));
     return 0;
 }
 

@@ -242,7 +242,7 @@ int rg_trace_any(rg_ctx *ctx, const rg_scene *scene, const rg_ray *d_rays, uint8
     if (int rc = need_device(c)) return rc;
     if (!s || (n && (!d_rays || !d_occluded))) return set_error(RG_E_INVALID, "null argument");
     if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
-    if (pred < RG_PRED_ALL || pred > RG_PRED_VOLUME) return set_error(RG_E_INVALID, "unknown predicate");
+    if (pred < RG_PRED_ALL || pred > RG_PRED_OPAQUE) return set_error(RG_E_INVALID, "unknown predicate");
     uint64_t *work = c->trace_work + c->trace_slot++ % Ctx::kTraceSlots;
     return launch_any(*s, d_rays, d_occluded, n, pred, work, c->stream);
 }

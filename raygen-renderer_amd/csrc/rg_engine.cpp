@@ -349,6 +349,9 @@ class Engine {
         if (st.spp == 0) return set_error(RG_E_INVALID, "spp must be >= 1");
         if (st.adaptive || st.denoise)
             return set_error(RG_E_UNSUPPORTED, "batched jobs render a fixed sample count without denoise");
+        if (int rc = check_shader(st)) return rc;
+        if (st.shader != 5)
+            return set_error(RG_E_UNSUPPORTED, "batched jobs use shading system 5; render systems 0-3 with rg_render");
         const uint64_t npix = (uint64_t)st.width * st.height;
         if (npix * st.spp >= (1ull << 32)) return set_error(RG_E_BUDGET, "job has 2^32 or more samples");
         const uint32_t group = plain_group(st);

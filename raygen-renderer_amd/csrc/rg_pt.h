@@ -136,6 +136,11 @@ struct PTScene {
     // the scene needs the shade kernels built with the extended media
     // (fBm density, cone / path emission, heat haze, volume-light NEE)
     int32_t ext;
+    // shading system (5 = BSDF path tracer, 0-3 = the eye-ray providers),
+    // calcAO's sample count (the render's spp) and worldColor
+    int32_t shader;
+    uint32_t ao_samples;
+    float world[3];
 };
 
 // Structure-of-arrays path state, capacity = pixels x samples in flight.
@@ -168,6 +173,8 @@ struct PTState {
 
 // Kernel-side scene view of one job: scene buffers, camera and settings.
 PTScene make_ptscene(const Scene &scn, const rg_settings &st);
+// RG_E_INVALID unless the shading system is 0-3 or 5 and worldColor finite.
+int check_shader(const rg_settings &st);
 // Path events a generation runs: max_depth + 2 (every event but the eye
 // hit's adds a pass, and the last pass ends the path), twice that when the
 // scene has a heat-haze medium (a bend adds an event without a pass, and at
@@ -186,6 +193,13 @@ int prepare_scene(Scene *s, void *stream);
 int launch_generation(const Scene &scn, const PTScene &sc, const PTState &st, uint32_t S, uint32_t sample_base,
                       const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream,
                       bool accumulate = true);
+// Shading systems 0-3: one kernel computes every sample's value of the
+// generation (eye ray, provider, transparency continuation, the global
+// medium when one is set) into the path state's radiance, as the wavefront
+// leaves it; the same accumulation follows.
+int launch_preview(const Scene &scn, const PTScene &sc, const PTState &st, uint32_t S, uint32_t sample_base,
+                   const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream,
+                   bool accumulate);
 // Adds the finished generation's samples [s0, s0 + S) of the pixels of the
 // tiles in d_tile_mask (all pixels when null) to the sums.
 int launch_accumulate(const PTScene &sc, const PTState &st, uint32_t s0, uint32_t S, const uint8_t *d_tile_mask,

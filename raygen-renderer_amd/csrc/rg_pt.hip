@@ -365,6 +365,8 @@ static int generation_impl(const Scene &scn, const PTScene &sc, PTState st, uint
 int launch_generation(const Scene &scn, const PTScene &sc, const PTState &st, uint32_t S, uint32_t sample_base,
                       const uint8_t *d_tile_mask, float *d_sum, float *d_sumsq, float clamp_max, void *stream,
                       bool accumulate) {
+    if (sc.shader != 5)
+        return launch_preview(scn, sc, st, S, sample_base, d_tile_mask, d_sum, d_sumsq, clamp_max, stream, accumulate);
     return with_traversal(scn.stack_need, scn.width, [&](auto tag) {
         using T = decltype(tag);
         return generation_impl<T::kStack, T::kW>(scn, sc, st, S, sample_base, d_tile_mask, d_sum, d_sumsq,

@@ -808,14 +808,10 @@ __device__ __forceinline__ float smoothstep01(float e0, float e1, float x) {
     return t * t * (3.0f - 2.0f * t);
 }
 
-// traceLight's point-light part (rayrenderer.cpp:2281-2290): the only light,
-// or one picked uniformly (a rand() draw)
-__device__ __forceinline__ PointSample sample_point(const PTScene &sc, Rng &R, v3 hit, v3 n) {
+// One point light seen from hit (tracePointLight up to the shadow test).
+__device__ __forceinline__ PointSample point_light_at(const DevPointLight &L, v3 hit, v3 n) {
     PointSample s;
     s.ok = false;
-    if (sc.nplights == 0) return s;
-    const uint32_t li = sc.nplights == 1 ? 0u : pick(prng(R), sc.nplights);
-    const DevPointLight L = sc.plights[li];
     const v3 lr = sub(mk(L.pos[0], L.pos[1], L.pos[2]), hit);
     const v3 dir = nrm(lr);
     const float dto = dot(dir, n);
@@ -834,6 +830,18 @@ __device__ __forceinline__ PointSample sample_point(const PTScene &sc, Rng &R, v
     s.lr = lr;
     s.val = c;
     return s;
+}
+
+// traceLight's point-light part (rayrenderer.cpp:2281-2290): the only light,
+// or one picked uniformly (a rand() draw)
+__device__ __forceinline__ PointSample sample_point(const PTScene &sc, Rng &R, v3 hit, v3 n) {
+    if (sc.nplights == 0) {
+        PointSample s;
+        s.ok = false;
+        return s;
+    }
+    const uint32_t li = sc.nplights == 1 ? 0u : pick(prng(R), sc.nplights);
+    return point_light_at(sc.plights[li], hit, n);
 }
 
 // sampleVolumeLightForNEE (rayrenderer.cpp:2066-2203) up to the shadow

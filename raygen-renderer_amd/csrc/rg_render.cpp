@@ -268,6 +268,14 @@ bool scene_uses_ext(const Scene &s) {
     return false;
 }
 
+int check_shader(const rg_settings &st) {
+    if (st.shader != 0 && st.shader != 1 && st.shader != 2 && st.shader != 3 && st.shader != 5)
+        return set_error(RG_E_INVALID, "shader must be 0, 1, 2, 3 or 5 (the reference's providers)");
+    for (float v : st.world_color)
+        if (!std::isfinite(v)) return set_error(RG_E_INVALID, "world_color not finite");
+    return RG_OK;
+}
+
 PTScene make_ptscene(const Scene &scn, const rg_settings &st) {
     const Scene *s = &scn;
     const uint32_t W = st.width, H = st.height;
@@ -328,6 +336,11 @@ PTScene make_ptscene(const Scene &scn, const rg_settings &st) {
     sc.back[1] = st.back_color[1];
     sc.back[2] = st.back_color[2];
     sc.back[3] = st.back_color[3];
+    sc.shader = st.shader;
+    sc.ao_samples = std::max<uint32_t>(1, st.spp);
+    sc.world[0] = st.world_color[0];
+    sc.world[1] = st.world_color[1];
+    sc.world[2] = st.world_color[2];
     return sc;
 }
 
@@ -425,6 +438,10 @@ void rg_settings_default(rg_settings *s) {
     s->back_color[1] = 0.95f * 0.2f;
     s->back_color[2] = 0.9f * 0.2f;
     s->back_color[3] = 0.0f;
+    s->shader = 5;
+    s->world_color[0] = 1.0f * 0.1f;
+    s->world_color[1] = 0.95f * 0.1f;
+    s->world_color[2] = 0.9f * 0.1f;
 }
 
 int rg_scene_set_shading(rg_scene *scene, const float *n9, const uint32_t *obj_of_tri, const rg_object *objs,
@@ -448,6 +465,7 @@ int rg_scene_set_shading(rg_scene *scene, const float *n9, const uint32_t *obj_o
         if (m.transparency < 0.01f && m.emission <= 0.0f) mask |= 1u << RG_PRED_AREA;
         if (!(m.emission > 0.0f) && (m.transparency < 0.01f || m.refraction > 0.1f)) mask |= 1u << RG_PRED_ENV;
         if (!(m.emission > 0.0f) && m.transparency < 0.01f && m.refraction <= 0.1f) mask |= 1u << RG_PRED_VOLUME;
+        if (m.transparency < 0.01f) mask |= 1u << RG_PRED_OPAQUE;
         s->pred_mask[t] = mask;
     }
     s->has_shading = true;
@@ -624,11 +642,12 @@ int rg_render(rg_ctx *ctx, rg_scene *scene, const rg_settings *settings, float *
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     Scene *s = reinterpret_cast<Scene *>(scene);
     if (!c || !s || !settings) return set_error(RG_E_INVALID, "null argument");
-    if (c->device < 0) return set_error(RG_E_NODEVICE, "host-only context has no device");
     const rg_settings &st = *settings;
     if (st.width == 0 || st.height == 0 || st.width > 16384 || st.height > 16384)
         return set_error(RG_E_INVALID, "resolution must be 1..16384 per axis");
     if (st.max_depth < 1 || st.max_depth > 64) return set_error(RG_E_INVALID, "max_depth must be 1..64");
+    if (int rc = check_shader(st)) return rc;
+    if (c->device < 0) return set_error(RG_E_NODEVICE, "host-only context has no device");
     // a render clears the cancel flag when it starts (rayrenderer.cpp:537)
     c->cancel.store(false);
     if (int rc = dev_set_device(c->device)) return rc;

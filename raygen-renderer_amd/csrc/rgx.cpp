@@ -36,7 +36,7 @@ static bool parse_bool(const char *s) {
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        std::printf("usage: rgx render <scene.json> [-o out.jpg|.png|.bmp|.ppm|.hdr] [-r WxH] [-s N] [-d 5] [-enaa B] ...\n");
+        std::printf("usage: rgx render <scene.json> [-o out.jpg|.png|.bmp|.ppm|.hdr] [-r WxH] [-s N] [-d 0|1|2|3|5] [-enaa B] ...\n");
         return 0;
     }
     rg_settings st;
@@ -44,7 +44,6 @@ int main(int argc, char **argv) {
     rg_bloom bloom;
     rg_bloom_default(&bloom);
     bool post = false;
-    int shader = 5;
     std::string cmd, scene, out;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
@@ -69,7 +68,7 @@ int main(int argc, char **argv) {
                 }
             } else if (!strcmp(a, "-s") || !strcmp(a, "--samples")) st.spp = (uint32_t)std::atoi(next());
             else if (!strcmp(a, "-c") || !strcmp(a, "--threads") || !strcmp(a, "--cores")) (void)next();
-            else if (!strcmp(a, "-d") || !strcmp(a, "--shader")) shader = std::atoi(next());
+            else if (!strcmp(a, "-d") || !strcmp(a, "--shader")) st.shader = std::atoi(next());
             else if (!strcmp(a, "-enaa") || !strcmp(a, "--enable-antialias")) st.antialias = parse_bool(next());
             else if (!strcmp(a, "-endn") || !strcmp(a, "--enable-denoise")) st.denoise = parse_bool(next());
             else if (!strcmp(a, "-dni") || !strcmp(a, "--denoise-intensity")) st.denoise_intensity = (float)std::atof(next());
@@ -88,6 +87,11 @@ int main(int argc, char **argv) {
                 const int n = std::sscanf(next(), "%f,%f,%f,%f", &c[0], &c[1], &c[2], &c[3]);
                 if (n == 3) c[3] = 1.0f;
                 std::memcpy(st.back_color, c, sizeof c);
+            } else if (!strcmp(a, "-wc") || !strcmp(a, "--worldcolor")) {
+                // alpha is read and dropped: worldColor is a colour3 (rayrenderer.h:151)
+                float c[4] = {st.world_color[0], st.world_color[1], st.world_color[2], 1.0f};
+                std::sscanf(next(), "%f,%f,%f,%f", &c[0], &c[1], &c[2], &c[3]);
+                std::memcpy(st.world_color, c, sizeof st.world_color);
             }
             else if (!strcmp(a, "--depth")) st.max_depth = (uint32_t)std::atoi(next());
             else if (!strcmp(a, "--seed")) st.seed = (uint32_t)std::strtoul(next(), nullptr, 0);
@@ -107,6 +111,10 @@ int main(int argc, char **argv) {
             return 1;
         }
     }
+    if (st.shader != 0 && st.shader != 1 && st.shader != 2 && st.shader != 3 && st.shader != 5) {
+        std::fprintf(stderr, "unknown shading system %d (0, 1, 2, 3 or 5)\n", st.shader);
+        return 1;
+    }
     if (cmd != "render") {
         std::fprintf(stderr, "only the render command is on this path\n");
         return 1;
@@ -116,10 +124,6 @@ int main(int argc, char **argv) {
         const size_t dot = scene.find_last_of('.');
         const size_t base = slash == std::string::npos ? 0 : slash + 1;
         out = scene.substr(0, (dot != std::string::npos && dot > base) ? dot : scene.size()) + ".jpg";
-    }
-    if (shader != 5) {
-        std::fprintf(stderr, "shading system %d is outside this path (only -d 5)\n", shader);
-        return 1;
     }
     rg_ctx *ctx = nullptr;
     if (rg_create(0, &ctx) != RG_OK) {
@@ -139,7 +143,7 @@ int main(int argc, char **argv) {
         return 1;
     }
     std::printf("resolution: %u x %u, samples: %u, shader: %d, antialias: %s, denoise: %s, adaptive: %s\n", st.width,
-                st.height, st.spp, shader, st.antialias ? "yes" : "no", st.denoise ? "yes" : "no",
+                st.height, st.spp, st.shader, st.antialias ? "yes" : "no", st.denoise ? "yes" : "no",
                 st.adaptive ? "yes" : "no");
     std::vector<float> hdr((size_t)st.width * st.height * 4);
     // progress line as main.cpp:492-506 prints it

@@ -299,7 +299,7 @@ class Settings(ctypes.Structure):
                 ("sigma_normal", ctypes.c_float), ("sigma_depth", ctypes.c_float),
                 ("denoise_intensity", ctypes.c_float), ("samples_in_flight", ctypes.c_uint32),
                 ("color_sampling", ctypes.c_int32), ("cull_back_face", ctypes.c_int32),
-                ("back_color", ctypes.c_float * 4)]
+                ("back_color", ctypes.c_float * 4), ("shader", ctypes.c_int32), ("world_color", ctypes.c_float * 3)]
 
 
 def default_settings(**kw):

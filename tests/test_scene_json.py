@@ -181,9 +181,9 @@ def test_rgx_cli_argument_errors():
     exe = os.path.join(rgapi.HERE, "rgx")
     if not os.path.exists(exe):
         pytest.skip("rgx not built")
-    r = subprocess.run([exe, "render", os.path.join(GOLD, "room_scene.json"), "-d", "3"], capture_output=True,
+    r = subprocess.run([exe, "render", os.path.join(GOLD, "room_scene.json"), "-d", "4"], capture_output=True,
                        text=True, timeout=60)
-    assert r.returncode == 1 and "only -d 5" in r.stderr
+    assert r.returncode == 1 and "unknown shading system 4" in r.stderr
     r = subprocess.run([exe, "render", "x.json", "--bogus"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "unknown argument" in r.stdout
 
