@@ -34,6 +34,13 @@
 namespace rg {
 namespace {
 
+// fminf(acc, x) / fmaxf(acc, x) as the reference's expandBBox calls them
+// (bvh.cpp:36-51) with glibc's tie rule: of two equal operands (+0 and -0)
+// the second is returned, and a NaN operand yields the other one.  Written
+// out because clang's inline fminf keeps the first operand on ties.
+inline float min_acc(float acc, float x) { return (acc < x || x != x) ? acc : x; }
+inline float max_acc(float acc, float x) { return (acc > x || x != x) ? acc : x; }
+
 struct Box {
     float lo[3], hi[3];
     void clear() {
@@ -41,14 +48,14 @@ struct Box {
     }
     void grow(const Box &b) {
         for (int k = 0; k < 3; k++) {
-            lo[k] = fminf(lo[k], b.lo[k]);
-            hi[k] = fmaxf(hi[k], b.hi[k]);
+            lo[k] = min_acc(lo[k], b.lo[k]);
+            hi[k] = max_acc(hi[k], b.hi[k]);
         }
     }
     void grow(const float p[3]) {
         for (int k = 0; k < 3; k++) {
-            lo[k] = fminf(lo[k], p[k]);
-            hi[k] = fmaxf(hi[k], p[k]);
+            lo[k] = min_acc(lo[k], p[k]);
+            hi[k] = max_acc(hi[k], p[k]);
         }
     }
     float area() const {

@@ -56,6 +56,14 @@ def soup(n, seed):
     return (c + rng.normal(0, 0.05, (n, 3, 3))).astype(np.float32).reshape(n, 9)
 
 
+def signed_zero(n, seed):
+    rng = np.random.default_rng(seed)
+    v = soup(n, seed).reshape(n, 3, 3)
+    v[:, :, 0] = np.where(rng.random((n, 3)) < 0.5, np.float32(0.0), np.float32(-0.0))
+    v[:, :, 2] = np.where(rng.random((n, 3)) < 0.3, np.float32(-0.0), v[:, :, 2])
+    return v.reshape(n, 9)
+
+
 CASES = {
     "ico4": lambda: rgapi.gen_icosphere(4),
     "soup1": lambda: soup(1, 1),
@@ -63,6 +71,9 @@ CASES = {
     "soup5": lambda: soup(5, 3),
     "soup3000": lambda: soup(3000, 4),
     "coincident": lambda: np.repeat(np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32), 7, axis=0),
+    # coordinates of +0.0 and -0.0 mixed: box min/max ties keep the later
+    # operand's sign, as fminf/fmaxf do (bvh.cpp:36-51)
+    "signed_zero": lambda: signed_zero(400, 8),
     "flat_z": lambda: np.concatenate([soup(500, 5)[:, :2], np.zeros((500, 1), np.float32),
                                       soup(500, 6)[:, 3:5], np.zeros((500, 1), np.float32),
                                       soup(500, 7)[:, 6:8], np.zeros((500, 1), np.float32)], axis=1),
