@@ -30,6 +30,7 @@ extern "C" {
 #define RG_E_BUDGET -7      /* request exceeds the context's memory budget */
 #define RG_E_IO -8          /* file could not be read or parsed */
 #define RG_E_BUSY -9        /* rg_server_submit: refused by the server's queue limit */
+#define RG_E_INTERNAL -10   /* an internal consistency check failed */
 #define RG_TIMEOUT 1        /* rg_server_wait: not finished within the timeout */
 
 typedef struct rg_ctx rg_ctx;
@@ -57,11 +58,17 @@ typedef struct {
 
 /* Build parameters of TriangleBVH::build (bvh.cpp:22-25): bins = 16,
  * leaf = 4; width = 2 keeps the binary layout, 4 collapses to 4-wide nodes
- * (the traversal order and results are identical either way). */
+ * (the traversal order and results are identical either way).  builder
+ * picks where the tree is built; every builder gives the same nodes and
+ * permutation. */
+#define RG_BUILDER_AUTO 0   /* the GPU for device contexts from 2^14 triangles, else the host */
+#define RG_BUILDER_HOST 1   /* the host's cores */
+#define RG_BUILDER_DEVICE 2 /* the context's GPU (bins <= 32) */
 typedef struct {
     uint32_t bins;
     uint32_t leaf;
     uint32_t width;
+    uint32_t builder;
 } rg_bvh_params;
 
 /* 32-byte binary node, layout of BVHNode (bvh.h:22-30); used by
@@ -113,9 +120,11 @@ int rg_set_progress(rg_ctx *ctx, rg_progress_fn fn, void *user);
  * v9 = n_tris x (v1, v2, v3) in render (camera) space.  The data is copied. */
 int rg_scene_create(rg_ctx *ctx, const float *v9, uint32_t n_tris, rg_scene **out);
 void rg_scene_destroy(rg_scene *scene);
-/* Replaces TriangleBVH::build (bvh.cpp:76-231): host binned-SAH build, then
- * (device contexts) the 4-wide collapse and upload to HBM.  params may be
- * NULL (16 bins, leaf 4, width 4). */
+/* Replaces TriangleBVH::build (bvh.cpp:76-231): the binned-SAH binary tree,
+ * its 4-wide (or 8-wide) collapse and, for device contexts, the triangle
+ * records in HBM.  Built on the GPU (device contexts, RG_BUILDER_AUTO from
+ * 2^14 triangles, or RG_BUILDER_DEVICE) or on the host's cores.  params may
+ * be NULL (16 bins, leaf 4, width 4, RG_BUILDER_AUTO). */
 int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params);
 /* Copies the binary tree out: nodes (capacity cap) and the primitive
  * permutation (n_tris entries, nullable).  *n_nodes receives the count. */

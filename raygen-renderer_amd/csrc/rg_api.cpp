@@ -30,8 +30,9 @@ int need_device(const Ctx *c) {
 void free_device(Scene *s) {
     dev_free(s->d_nodes4);
     dev_free(s->d_nodes2);
+    dev_free(s->d_perm);
     dev_free(s->d_tris);
-    s->d_nodes4 = s->d_nodes2 = s->d_tris = nullptr;
+    s->d_nodes4 = s->d_nodes2 = s->d_perm = s->d_tris = nullptr;
     s->device_bytes = 0;
 }
 
@@ -160,28 +161,56 @@ int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params) {
     Ctx *c = reinterpret_cast<Ctx *>(ctx);
     Scene *s = reinterpret_cast<Scene *>(scene);
     if (!c || !s) return set_error(RG_E_INVALID, "null argument");
-    rg_bvh_params p = {16, 4, 4};
+    rg_bvh_params p = {16, 4, 4, RG_BUILDER_AUTO};
     if (params) p = *params;
     if (p.bins < 2 || p.bins > 256) return set_error(RG_E_INVALID, "bins must be in [2, 256]");
     if (p.leaf < 1 || p.leaf > 0xFFFF) return set_error(RG_E_INVALID, "leaf must be in [1, 65535]");
     if (p.width != 2 && p.width != 4 && p.width != 8) return set_error(RG_E_UNSUPPORTED, "width must be 2, 4 or 8");
+    if (p.builder > RG_BUILDER_DEVICE) return set_error(RG_E_INVALID, "unknown builder");
+    if (p.builder == RG_BUILDER_DEVICE) {
+        if (c->device < 0) return set_error(RG_E_NODEVICE, "the device builder needs a device context");
+        if (!gpu_build_supported(p.bins, p.leaf)) return set_error(RG_E_UNSUPPORTED, "the device builder takes bins <= 32");
+    }
+    const bool on_gpu = c->device >= 0 && s->n_tris > 0 && gpu_build_supported(p.bins, p.leaf) &&
+                        (p.builder == RG_BUILDER_DEVICE || (p.builder == RG_BUILDER_AUTO && s->n_tris >= kGpuBuildMin));
+    s->width = p.width == 8 ? 8 : 4;
+    s->built = false;
+    s->nodes2.clear();
+    s->perm.clear();
+    s->nodes4.clear();
+    s->nodes8.clear();
+    if (c->device >= 0) {
+        if (int rc = dev_set_device(c->device)) return rc;
+        free_device(s);
+        s->device = c->device;
+    }
+    if (on_gpu) {
+        if (int rc = gpu_build_bvh(*s, p.bins, p.leaf, s->width, c->stream)) return rc;
+        s->host_tree = false;
+        s->built = true;
+        s->tris_dirty = false;
+        s->device_bytes = s->n_wide * (s->width == 8 ? sizeof(Node8) : sizeof(Node4)) +
+                          (uint64_t)s->n_tris * sizeof(Tri48);
+        return RG_OK;
+    }
     try {
         build_bvh2(s->v9.data(), s->n_tris, p.bins, p.leaf, s->nodes2, s->perm);
-        s->nodes4.clear();
-        s->nodes8.clear();
         if (p.width == 8) collapse_bvh8(s->nodes2, s->nodes8);
         else collapse_bvh4(s->nodes2, s->nodes4);
     } catch (const std::bad_alloc &) {
         return set_error(RG_E_NOMEM, "BVH build ran out of host memory");
     }
-    s->width = p.width == 8 ? 8 : 4;
+    s->host_tree = true;
+    s->n_nodes2 = (uint32_t)s->nodes2.size();
+    s->n_wide = p.width == 8 ? s->nodes8.size() : s->nodes4.size();
+    for (int k = 0; k < 3; k++) {
+        s->root_box[k] = s->nodes2.empty() ? 0.0f : s->nodes2[0].bmin[k];
+        s->root_box[3 + k] = s->nodes2.empty() ? 0.0f : s->nodes2[0].bmax[k];
+    }
     s->stack_need = p.width == 8 ? stack_need8(s->nodes8) : stack_need4(s->nodes4);
     s->built = true;
     if (c->device < 0) return RG_OK;
 
-    if (int rc = dev_set_device(c->device)) return rc;
-    free_device(s);
-    s->device = c->device;
     std::vector<Tri48> tris;
     make_tris(s->v9.data(), s->nodes2, s->perm, s->pred_mask, tris);
     const void *hn = s->width == 8 ? (const void *)s->nodes8.data() : (const void *)s->nodes4.data();
@@ -198,14 +227,17 @@ int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params) {
 
 int rg_bvh_export2(const rg_scene *scene, rg_bvh2_node *nodes, uint64_t cap, uint32_t *perm,
                    uint64_t *n_nodes) {
-    const Scene *s = reinterpret_cast<const Scene *>(scene);
+    Scene *s = const_cast<Scene *>(reinterpret_cast<const Scene *>(scene));
     if (!s || !n_nodes) return set_error(RG_E_INVALID, "null argument");
     if (!s->built) return set_error(RG_E_INVALID, "BVH not built");
-    *n_nodes = s->nodes2.size();
-    if (nodes) {
-        if (cap < s->nodes2.size()) return set_error(RG_E_INVALID, "node capacity too small");
-        std::memcpy(nodes, s->nodes2.data(), s->nodes2.size() * sizeof(Node2));
+    *n_nodes = s->n_nodes2;
+    if (!nodes && !perm) return RG_OK;
+    if (nodes && cap < s->n_nodes2) return set_error(RG_E_INVALID, "node capacity too small");
+    if (!s->host_tree) {
+        if (int rc = dev_set_device(s->device)) return rc;
+        if (int rc = download_host_tree(*s, nullptr)) return rc;
     }
+    if (nodes) std::memcpy(nodes, s->nodes2.data(), s->nodes2.size() * sizeof(Node2));
     if (perm) std::memcpy(perm, s->perm.data(), s->perm.size() * sizeof(uint32_t));
     return RG_OK;
 }

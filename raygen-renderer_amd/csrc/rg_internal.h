@@ -131,9 +131,14 @@ struct Scene {
     uint32_t n_tris = 0;
     std::vector<float> v9;            // as handed in
     float box[6] = {0, 0, 0, 0, 0, 0};
-    // binary tree (reference layout) + permutation
+    // binary tree (reference layout) + permutation; after a GPU build they
+    // stay on the device (d_nodes2, d_perm) until a host reader asks
     std::vector<Node2> nodes2;
     std::vector<uint32_t> perm;
+    bool host_tree = true;            // nodes2 / perm hold the current tree
+    uint32_t n_nodes2 = 0;            // binary nodes
+    uint64_t n_wide = 0;              // wide nodes
+    float root_box[6] = {0, 0, 0, 0, 0, 0};  // the binary root's bounds
     // wide tree
     std::vector<Node4> nodes4;      // width 2 or 4
     std::vector<Node8> nodes8;      // width 8
@@ -144,7 +149,8 @@ struct Scene {
     bool built = false;
     // device copies
     void *d_nodes4 = nullptr;        // the wide nodes on the device (Node4 or Node8 by width)
-    void *d_nodes2 = nullptr;
+    void *d_nodes2 = nullptr;        // binary nodes of a GPU build
+    void *d_perm = nullptr;          // and its permutation
     void *d_tris = nullptr;
     void *d_tri_flags = nullptr;  // per-triangle occluder mask bits (predicates)
     uint64_t device_bytes = 0;
@@ -204,6 +210,18 @@ void make_tris(const float *v9, const std::vector<Node2> &nodes2,
                std::vector<Tri48> &out);
 uint32_t stack_need4(const std::vector<Node4> &nodes);
 uint32_t stack_need8(const std::vector<Node8> &nodes);
+
+// rg_build.hip: the same binary tree, wide tree and triangle records built
+// on the device (d_nodes2, d_perm, d_nodes4, d_tris, n_nodes2, n_wide,
+// stack_need and root_box of the scene); the host copies are left empty.
+bool gpu_build_supported(uint32_t bins, uint32_t leaf);
+int gpu_build_bvh(Scene &s, uint32_t bins, uint32_t leaf, int width, void *stream);
+// Rewrites the triangle records of a GPU-built scene (occluder bits changed).
+int gpu_make_tris(Scene &s, void *stream);
+// Fills nodes2 / perm from the device copies of a GPU build.
+int download_host_tree(Scene &s, void *stream);
+// Smallest scene RG_BUILDER_AUTO builds on the GPU.
+constexpr uint32_t kGpuBuildMin = 1u << 14;
 
 // rg_trace.hip (device side)
 int dev_malloc(void **p, size_t bytes);

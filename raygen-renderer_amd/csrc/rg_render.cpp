@@ -206,6 +206,10 @@ int upload_pt_scene(Scene *s, void *stream) {
 }
 
 int reupload_tris(Scene *s, void *stream) {
+    if (!s->host_tree) {
+        s->tris_dirty = false;
+        return gpu_make_tris(*s, stream);
+    }
     std::vector<Tri48> tris;
     make_tris(s->v9.data(), s->nodes2, s->perm, s->pred_mask, tris);
     if (tris.empty()) return RG_OK;

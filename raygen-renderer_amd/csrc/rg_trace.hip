@@ -141,13 +141,9 @@ DevBvh dev_bvh(const Scene &s) {
     b.nodes = s.d_nodes4;
     b.width = s.width;
     b.tris = (const float4 *)s.d_tris;
-    if (!s.nodes2.empty()) {
-        for (int k = 0; k < 3; k++) { b.root.lo[k] = s.nodes2[0].bmin[k]; b.root.hi[k] = s.nodes2[0].bmax[k]; }
-    } else {
-        for (int k = 0; k < 3; k++) { b.root.lo[k] = 0.0f; b.root.hi[k] = 0.0f; }
-    }
-    b.has_tree = (s.nodes4.empty() && s.nodes8.empty()) ? 0 : 1;
-    const size_t n_nodes = s.width == 8 ? s.nodes8.size() : s.nodes4.size();
+    for (int k = 0; k < 3; k++) { b.root.lo[k] = s.root_box[k]; b.root.hi[k] = s.root_box[3 + k]; }
+    b.has_tree = s.n_wide > 0 ? 1 : 0;
+    const size_t n_nodes = s.n_wide;
     const size_t node_bytes = s.width == 8 ? sizeof(Node8) : sizeof(Node4);
     b.top = (uint32_t)std::min<size_t>(n_nodes, kTreeletBytes / node_bytes);
     return b;

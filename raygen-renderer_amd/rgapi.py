@@ -29,7 +29,11 @@ class RGError(RuntimeError):
 
 
 class BvhParams(ctypes.Structure):
-    _fields_ = [("bins", ctypes.c_uint32), ("leaf", ctypes.c_uint32), ("width", ctypes.c_uint32)]
+    _fields_ = [("bins", ctypes.c_uint32), ("leaf", ctypes.c_uint32), ("width", ctypes.c_uint32),
+                ("builder", ctypes.c_uint32)]
+
+
+BUILDER_AUTO, BUILDER_HOST, BUILDER_DEVICE = 0, 1, 2
 
 
 _lib = None
@@ -185,8 +189,8 @@ class Scene:
     def n_tris(self):
         return len(self.v9)
 
-    def build(self, bins=16, leaf=4, width=4):
-        p = BvhParams(bins, leaf, width)
+    def build(self, bins=16, leaf=4, width=4, builder=BUILDER_AUTO):
+        p = BvhParams(bins, leaf, width, builder)
         check(lib().rg_bvh_build(self.ctx.h, self.h, ctypes.byref(p)))
 
     def export2(self):

@@ -92,6 +92,20 @@ def test_product_bvh_equals_oracle(case):
     assert (p_p == p_o).all()
 
 
+def test_builder_choice_checked():
+    ctx = rgapi.Context(-1)
+    sc = rgapi.Scene(ctx, soup(10, 1))
+    with pytest.raises(rgapi.RGError) as e:
+        sc.build(builder=rgapi.BUILDER_DEVICE)  # a host-only context has no device
+    assert e.value.code == -2
+    with pytest.raises(rgapi.RGError):
+        sc.build(builder=3)
+    sc.build(builder=rgapi.BUILDER_HOST)
+    n_p, p_p = sc.export2()
+    n_o, p_o = oracle.bvh_build(sc.v9)
+    assert n_p.tobytes() == n_o.tobytes()
+
+
 def test_empty_scene_builds():
     ctx = rgapi.Context(-1)
     sc = rgapi.Scene(ctx, np.zeros((0, 9), np.float32))
