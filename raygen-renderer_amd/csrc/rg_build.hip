@@ -55,9 +55,10 @@ typedef unsigned long long u64;
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kMaxBins = 32;    // bins the device builder supports
-constexpr uint32_t kSmall = 512;     // ranges up to this size are built by one wave
+constexpr uint32_t kSmall = 256;     // ranges up to this size are built by one wave
 constexpr uint32_t kBlk = 256;       // threads per block of the level kernels
-constexpr uint32_t kPerBlk = 512;    // elements per block (<= kSmall: a block meets at most two ranges)
+constexpr uint32_t kPerBlk = 256;    // elements per block (<= kSmall: a block meets at most two ranges)
+constexpr uint32_t kPerThread = kPerBlk / kBlk;
 constexpr u64 kMinId = ~0ull;        // identity of the min keys
 constexpr u64 kMaxId = 0ull;         // identity of the max keys
 
@@ -214,8 +215,8 @@ __global__ __launch_bounds__(kBlk) void k_bounds(Prims p, uint32_t n, uint32_t n
     if (tid == 0) { s_lo = kNone; s_hi = 0; }
     __syncthreads();
     const uint32_t base = blockIdx.x * kPerBlk;
-    uint32_t sg[2];
-    for (int e = 0; e < 2; e++) {
+    uint32_t sg[kPerThread];
+    for (uint32_t e = 0; e < kPerThread; e++) {
         const uint32_t i = base + e * kBlk + tid;
         sg[e] = i < n ? p.seg[i] : kNone;
         if (sg[e] >= nseg) sg[e] = kNone;
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(kBlk) void k_bounds(Prims p, uint32_t n, uint32_t n
         }
     }
     __syncthreads();
-    for (int e = 0; e < 2; e++) {
+    for (uint32_t e = 0; e < kPerThread; e++) {
         if (sg[e] == kNone) continue;
         const uint32_t i = base + e * kBlk + tid;
         const int slot = sg[e] == s_lo ? 0 : 1;
@@ -307,8 +308,8 @@ __global__ __launch_bounds__(kBlk) void k_bin(Prims p, uint32_t n, uint32_t nseg
     if (tid == 0) { s_lo = kNone; s_hi = 0; }
     __syncthreads();
     const uint32_t base = blockIdx.x * kPerBlk;
-    uint32_t sg[2];
-    for (int e = 0; e < 2; e++) {
+    uint32_t sg[kPerThread];
+    for (uint32_t e = 0; e < kPerThread; e++) {
         const uint32_t i = base + e * kBlk + tid;
         sg[e] = i < n ? p.seg[i] : kNone;
         if (sg[e] >= nseg || segs[sg[e]].axis < 0) sg[e] = kNone;
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(kBlk) void k_bin(Prims p, uint32_t n, uint32_t nseg
         }
     }
     __syncthreads();
-    for (int e = 0; e < 2; e++) {
+    for (uint32_t e = 0; e < kPerThread; e++) {
         if (sg[e] == kNone) continue;
         const uint32_t i = base + e * kBlk + tid;
         const SegInfo &S = segs[sg[e]];
@@ -565,15 +566,13 @@ __global__ void k_scatter(Build B, const SegInfo *__restrict__ segs) {
 // Binary build: one wave per range of at most kSmall primitives
 // ---------------------------------------------------------------------------
 struct WaveLds {
-    float v[9][kSmall];
+    float v[6][kSmall];  // box lo xyz, hi xyz (the centroid is (lo + hi) * 0.5 again)
     uint32_t perm[kSmall];
     uint16_t ord[kSmall], ord2[kSmall], dst[kSmall], jump[kSmall];
     uint8_t binv[kSmall];
     u64 bk[kMaxBins][6];
     uint32_t bc[kMaxBins];
     uint16_t stack[kSmall][3];
-    int32_t best;
-    uint32_t nl;
 };
 
 __device__ __forceinline__ u64 wave_min(u64 v) {
@@ -591,6 +590,75 @@ __device__ __forceinline__ u64 wave_max(u64 v) {
     return v;
 }
 
+// The SAH decision of sah_split with one lane per bin: the left boxes and
+// counts are prefix min/max/sums over the lanes, the right ones suffix
+// ones, and the first minimum cost below the leaf cost wins, as the
+// reference's ascending strict-< scan picks it.  The prefix boxes equal the
+// sequential fold's up to the sign of a zero bound, which changes no area
+// comparison.
+__device__ int wave_sah(const u64 (*bk)[6], const uint32_t *bc, uint32_t bins, uint32_t count, const float plo[3],
+                        const float phi[3], uint32_t lane, uint32_t &nl) {
+    const float kTrav = 1.0f, kIsect = 1.5f;
+    uint32_t c = 0;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    if (lane < bins && bc[lane] > 0) {
+        c = bc[lane];
+        for (int k = 0; k < 3; k++) {
+            lo[k] = key_val(bk[lane][k]);
+            hi[k] = key_val(bk[lane][3 + k]);
+        }
+    }
+    uint32_t lc = c, rc = c;
+    float llo[3], lhi[3], rlo[3], rhi[3];
+    for (int k = 0; k < 3; k++) { llo[k] = rlo[k] = lo[k]; lhi[k] = rhi[k] = hi[k]; }
+    for (int o = 1; o < (int)kMaxBins; o <<= 1) {
+        const uint32_t xl = __shfl_up(lc, o, 64), xr = __shfl_down(rc, o, 64);
+        float al[3], ah[3], bl[3], bh[3];
+        for (int k = 0; k < 3; k++) {
+            al[k] = __shfl_up(llo[k], o, 64);
+            ah[k] = __shfl_up(lhi[k], o, 64);
+            bl[k] = __shfl_down(rlo[k], o, 64);
+            bh[k] = __shfl_down(rhi[k], o, 64);
+        }
+        if (lane >= (uint32_t)o) {
+            lc += xl;
+            for (int k = 0; k < 3; k++) { llo[k] = fminf(llo[k], al[k]); lhi[k] = fmaxf(lhi[k], ah[k]); }
+        }
+        if (lane + o < bins) {
+            rc += xr;
+            for (int k = 0; k < 3; k++) { rlo[k] = fminf(rlo[k], bl[k]); rhi[k] = fmaxf(rhi[k], bh[k]); }
+        }
+    }
+    // split i: left = prefix through bin i (this lane), right = suffix from bin i + 1
+    const uint32_t rcn = __shfl_down(rc, 1, 64);
+    float nlo[3], nhi[3];
+    for (int k = 0; k < 3; k++) {
+        nlo[k] = __shfl_down(rlo[k], 1, 64);
+        nhi[k] = __shfl_down(rhi[k], 1, 64);
+    }
+    const float parea = box_area(plo, phi);
+    const float inv_parea = (parea > 0.0f) ? 1.0f / parea : 0.0f;
+    const float leaf_cost = (float)count * kIsect;
+    float cost = FLT_MAX;
+    bool valid = false;
+    if (lane + 1 < bins && lc > 0 && rcn > 0) {
+        const float la = box_area(llo, lhi), ra = box_area(nlo, nhi);
+        cost = kTrav + kIsect * inv_parea * (la * (float)lc + ra * (float)rcn);
+        valid = cost < leaf_cost;
+    }
+    // first lane holding the smallest valid cost
+    float m = valid ? cost : FLT_MAX;
+    for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o, 64));
+    const u64 at = __ballot(valid && cost == m);
+    if (!at) {
+        nl = 0;
+        return -1;
+    }
+    const int best = __ffsll((long long)at) - 1;
+    nl = (uint32_t)__shfl(lc, best, 64);
+    return best;
+}
+
 __global__ __launch_bounds__(64) void k_wave_build(Build B, uint32_t ntask) {
     __shared__ WaveLds L;
     const uint32_t t = blockIdx.x;
@@ -604,7 +672,7 @@ __global__ __launch_bounds__(64) void k_wave_build(Build B, uint32_t ntask) {
     }
     Node2 *out = B.task_nodes + 2 * (size_t)first;
     for (uint32_t p = lane; p < m; p += 64) {
-        for (int k = 0; k < 9; k++) L.v[k][p] = B.a.v[k][first + p];
+        for (int k = 0; k < 6; k++) L.v[k][p] = B.a.v[k][first + p];
         L.perm[p] = B.a.perm[first + p];
         L.ord[p] = (uint16_t)p;
     }
@@ -623,8 +691,10 @@ __global__ __launch_bounds__(64) void k_wave_build(Build B, uint32_t ntask) {
         for (uint32_t p = a + lane; p < a + c; p += 64) {
             const uint32_t e = L.ord[p];
             for (int j = 0; j < 3; j++) {
-                const u64 a0 = key_min(L.v[j][e], p), a1 = key_max(L.v[3 + j][e], p);
-                const u64 a2 = key_min(L.v[6 + j][e], p), a3 = key_max(L.v[6 + j][e], p);
+                const float lo = L.v[j][e], hi = L.v[3 + j][e];
+                const float ce = (lo + hi) * 0.5f;
+                const u64 a0 = key_min(lo, p), a1 = key_max(hi, p);
+                const u64 a2 = key_min(ce, p), a3 = key_max(ce, p);
                 km[j] = a0 < km[j] ? a0 : km[j];
                 km[3 + j] = a1 > km[3 + j] ? a1 : km[3 + j];
                 km[6 + j] = a2 < km[6 + j] ? a2 : km[6 + j];
@@ -644,42 +714,37 @@ __global__ __launch_bounds__(64) void k_wave_build(Build B, uint32_t ntask) {
         nd.count = (uint16_t)min(c, 0xFFFFu);
         nd.axis = 0;
         bool leaf = c <= B.leaf;
-        int axis = 0;
-        float cmin = 0.0f, scale = 0.0f;
+        int axis = 0, best = -1;
+        uint32_t nl = 0;
         if (!leaf) {
             const float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
             if (ext[1] > ext[0]) axis = 1;
             if (ext[2] > ((axis == 0) ? ext[0] : ext[1])) axis = 2;
             if (ext[axis] < 1e-12f) leaf = true;
-            cmin = clo[axis];
-            scale = (float)B.bins / ext[axis];
-        }
-        if (!leaf) {
-            for (uint32_t j = lane; j < B.bins; j++) {
-                for (int q = 0; q < 3; q++) { L.bk[j][q] = kMinId; L.bk[j][3 + q] = kMaxId; }
-                L.bc[j] = 0;
-            }
-            __syncthreads();
-            for (uint32_t p = a + lane; p < a + c; p += 64) {
-                const uint32_t e = L.ord[p];
-                const int b = bin_of(L.v[6 + axis][e], cmin, scale, last_bin);
-                L.binv[p] = (uint8_t)b;
-                for (int q = 0; q < 3; q++) {
-                    atomicMin(&L.bk[b][q], key_min(L.v[q][e], p));
-                    atomicMax(&L.bk[b][3 + q], key_max(L.v[3 + q][e], p));
+            const float cmin = clo[axis];
+            const float scale = (float)B.bins / ext[axis];
+            if (!leaf) {
+                for (uint32_t j = lane; j < B.bins; j += 64) {
+                    for (int q = 0; q < 3; q++) { L.bk[j][q] = kMinId; L.bk[j][3 + q] = kMaxId; }
+                    L.bc[j] = 0;
                 }
-                atomicAdd(&L.bc[b], 1u);
-            }
-            __syncthreads();
-            if (lane == 0) {
-                uint32_t nl = 0;
-                int best = sah_split(L.bk, L.bc, B.bins, c, nd.bmin, nd.bmax, nl);
+                __syncthreads();
+                for (uint32_t p = a + lane; p < a + c; p += 64) {
+                    const uint32_t e = L.ord[p];
+                    const float ce = (L.v[axis][e] + L.v[3 + axis][e]) * 0.5f;
+                    const int b = bin_of(ce, cmin, scale, last_bin);
+                    L.binv[p] = (uint8_t)b;
+                    for (int q = 0; q < 3; q++) {
+                        atomicMin(&L.bk[b][q], key_min(L.v[q][e], p));
+                        atomicMax(&L.bk[b][3 + q], key_max(L.v[3 + q][e], p));
+                    }
+                    atomicAdd(&L.bc[b], 1u);
+                }
+                __syncthreads();
+                best = wave_sah(L.bk, L.bc, B.bins, c, nd.bmin, nd.bmax, lane, nl);
                 if (best >= 0 && (nl == 0 || nl >= c)) best = -1;
-                L.best = best;
-                L.nl = nl;
+                if (best < 0) leaf = true;
             }
-            __syncthreads();
-            if (L.best < 0) leaf = true;
         }
         if (!leaf && sp + 2 > kSmall) {
             if (lane == 0) B.out->error = 6;
@@ -690,8 +755,6 @@ __global__ __launch_bounds__(64) void k_wave_build(Build B, uint32_t ntask) {
             __syncthreads();
             continue;
         }
-        const int best = L.best;
-        const uint32_t nl = L.nl;
         // partition: left ranks by ballot, right elements along their chains
         uint32_t run = 0;
         for (uint32_t p0 = a; p0 < a + c; p0 += 64) {

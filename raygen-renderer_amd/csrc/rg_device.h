@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cstdint>
 #include <type_traits>
 
@@ -577,19 +579,23 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, 
 
 // Blocks of `kernel` the whole device holds at once (occupancy x compute
 // units), looked up once per kernel: the grid of a persistent launch.
+// Threads of the engine and of callers may ask at once: the cache is an
+// atomic (two first callers both compute the same value).
 template <auto kernel>
 uint32_t resident_blocks(int block) {
-    static int cached = 0;  // one per kernel instantiation
-    if (!cached) {
+    static std::atomic<int> cached{0};  // one per kernel instantiation
+    int v = cached.load(std::memory_order_relaxed);
+    if (!v) {
         int dev = 0, n_cu = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu <= 0)
             n_cu = 256;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu <= 0)
             per_cu = 1;
-        cached = n_cu * per_cu;
+        v = n_cu * per_cu;
+        cached.store(v, std::memory_order_relaxed);
     }
-    return (uint32_t)cached;
+    return (uint32_t)v;
 }
 
 // Host-side dispatch to the traversal kernels instantiated for a stack size

@@ -73,6 +73,9 @@ CASES = {
     "soup1": lambda: soup(1, 1),
     "soup4": lambda: soup(4, 2),
     "soup5": lambda: soup(5, 3),
+    "soup255": lambda: soup(255, 13),
+    "soup256": lambda: soup(256, 14),
+    "soup257": lambda: soup(257, 15),
     "soup511": lambda: soup(511, 4),
     "soup512": lambda: soup(512, 5),
     "soup513": lambda: soup(513, 6),
