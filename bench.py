@@ -109,11 +109,13 @@ def bench_cube_room(ctx, stream, steps, cpu_seconds, do_cpu):
     sc = scenes.cube_room()
     rs = rgapi.RenderScene(ctx, sc)
     st = rgapi.default_settings(width=256, height=256, spp=16, max_depth=5, antialias=1, seed=0xC0DE0000)
-    out = rs.render(st)  # warm-up (buffers, upload)
+    # the HDR image goes to a page-locked host buffer, as render() leaves it
+    hdr = rgapi.host_zeros((256, 256, 4))
+    out = rs.render(st, hdr_only=True, hdr_out=hdr)  # warm-up (buffers, upload)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        out = rs.render(st)
+        out = rs.render(st, hdr_only=True, hdr_out=hdr)
     wall = (time.perf_counter() - t0) / steps
     res = {"workload": "cube room 36 tris, 256x256, 16 spp, depth 5, NEE+MIS, AA, no denoise",
            "ms_per_frame": round(wall * 1e3, 3),
@@ -159,10 +161,11 @@ def bench_bsdf_frame(ctx, do_cpu, cpu_seconds):
     st = rgapi.default_settings(width=1920, height=1080, spp=64, max_depth=8, adaptive=1, adaptive_base=4,
                                 adaptive_threshold=1e-3, denoise=1, denoise_levels=5, seed=0xB5DF0002)
     warm = rgapi.default_settings(width=1920, height=1080, spp=1, max_depth=8)
-    rs.render(warm)
+    hdr = rgapi.host_zeros((1080, 1920, 4))  # page-locked: the image arrives by DMA
+    rs.render(warm, sums=False, hdr_out=hdr)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
-    out = rs.render(st)
+    out = rs.render(st, sums=False, hdr_out=hdr)  # the HDR image and the tile sample counts to the host
     wall = time.perf_counter() - t3
     # samples taken: each 32x32 tile's sample count times its pixels (the
     # path tracer also traces the later passes of an adaptive window for

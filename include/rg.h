@@ -98,6 +98,12 @@ const char *rg_last_error(void);
  * context's own stream). */
 int rg_set_stream(rg_ctx *ctx, void *hip_stream);
 int rg_synchronize(rg_ctx *ctx);
+/* Page-locked host memory (no reference counterpart; for callers that want
+ * rg_render's images delivered by DMA straight into their buffers instead of
+ * through the HIP runtime's staging copies of pageable memory).  Free with
+ * rg_host_free. */
+int rg_host_alloc(uint64_t bytes, void **out);
+void rg_host_free(void *p);
 /* Cooperative cancel (RayRenderer::cancelRequested, rayrenderer.h:427):
  * async-safe; observed between launches.  Like the reference, which clears
  * the flag when a render starts (rayrenderer.cpp:537), rg_render and

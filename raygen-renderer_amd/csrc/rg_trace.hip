@@ -25,6 +25,31 @@ static int hip_check(hipError_t e, const char *what) {
     return set_error(RG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+}  // namespace rg
+
+extern "C" {
+
+int rg_host_alloc(uint64_t bytes, void **out) {
+    if (!out) return rg::set_error(RG_E_INVALID, "null argument");
+    *out = nullptr;
+    if (bytes == 0) return RG_OK;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return rg::set_error(RG_E_NOMEM, "hipHostMalloc failed");
+    }
+    *out = p;
+    return RG_OK;
+}
+
+void rg_host_free(void *p) {
+    if (p) (void)hipHostFree(p);
+}
+
+}  // extern "C"
+
+namespace rg {
+
 int dev_set_device(int device) { return hip_check(hipSetDevice(device), "hipSetDevice"); }
 
 // Drops a pending error of a call whose failure was already handled, so a

@@ -91,6 +91,8 @@ def lib():
             "rg_scene_set_camera": (i32, [vp, vp]),
             "rg_render": (i32, [vp, vp, vp, vp, vp, vp, vp]),
             "rg_render_stats": (i32, [vp, vp]),
+            "rg_host_alloc": (i32, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+            "rg_host_free": (None, [vp]),
             "rg_scene_load_json": (i32, [vp, ctypes.c_char_p, pp]),
             "rg_scene_warnings": (i32, [vp, ctypes.c_char_p, u64]),
             "rg_scene_get_camera": (i32, [vp, vp]),
@@ -229,23 +231,31 @@ class Scene:
         check(lib().rg_trace_any_host(self.ctx.h, self.h, _ptr(rays), _ptr(occ), len(rays), pred))
         return occ
 
-    def render(self, settings, hdr_only=False):
+    def render(self, settings, hdr_only=False, sums=True, hdr_out=None):
         """rg_render of this scene (shading set and BVH built) with host
-        outputs; hdr_only skips the sums and tile sample counts."""
+        outputs; hdr_only skips the sums and tile sample counts, sums=False
+        the sums only; hdr_out (H, W, 4) float32, e.g. from host_zeros,
+        receives the HDR image instead of a new array."""
         W, H = settings.width, settings.height
-        hdr = np.zeros((H, W, 4), np.float32)
-        if hdr_only:
-            sm = sq = nt = None
+        if hdr_out is not None:
+            assert hdr_out.shape == (H, W, 4) and hdr_out.dtype == np.float32 and hdr_out.flags.c_contiguous
+            hdr = hdr_out
         else:
-            sm = np.zeros((H, W, 3), np.float32)
-            sq = np.zeros((H, W, 3), np.float32)
+            hdr = np.zeros((H, W, 4), np.float32)
+        sm = sq = nt = None
+        if not hdr_only:
+            if sums:
+                sm = np.zeros((H, W, 3), np.float32)
+                sq = np.zeros((H, W, 3), np.float32)
             nt = np.zeros(((W + 31) // 32) * ((H + 31) // 32), np.uint32)
         check(lib().rg_render(self.ctx.h, self.h, ctypes.byref(settings), _ptr(hdr), _ptr(sm), _ptr(sq), _ptr(nt)))
         stats = np.zeros(3, np.uint64)
         check(lib().rg_render_stats(self.ctx.h, _ptr(stats)))
         out = {"hdr": hdr, "rays": int(stats[0]), "shadow_rays": int(stats[1]), "samples": int(stats[2])}
         if not hdr_only:
-            out.update({"sum": sm, "sumsq": sq, "nspp": nt})
+            out["nspp"] = nt
+            if sums:
+                out.update({"sum": sm, "sumsq": sq})
         return out
 
     def close(self):
@@ -326,6 +336,35 @@ class Texture(ctypes.Structure):
     """rg_texture: RGB floats as stored (LDR byte / 255), srgb flag."""
     _fields_ = [("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("rgb", ctypes.c_void_p), ("srgb", ctypes.c_int32),
                 ("pad", ctypes.c_uint32)]
+
+
+class _HostBlock:
+    """Owner of one rg_host_alloc block; freed when the arrays over it are."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        check(lib().rg_host_alloc(nbytes, ctypes.byref(p)))
+        self.p = p.value
+
+    def __del__(self):
+        if self.p:
+            lib().rg_host_free(ctypes.c_void_p(self.p))
+            self.p = None
+
+
+def host_zeros(shape, dtype=np.float32):
+    """A zeroed numpy array in page-locked host memory (rg_host_alloc):
+    rg_render's copies into it go by DMA, without staging."""
+    dtype = np.dtype(dtype)
+    nbytes = int(np.prod(shape)) * dtype.itemsize
+    if nbytes == 0:
+        return np.zeros(shape, dtype)
+    blk = _HostBlock(nbytes)
+    buf = (ctypes.c_byte * nbytes).from_address(blk.p)
+    buf._owner = blk
+    a = np.frombuffer(buf, dtype=dtype).reshape(shape)
+    a[...] = 0
+    return a
 
 
 class RenderScene(Scene):

@@ -313,3 +313,16 @@ def test_dispersive_glass_matches_single_lobe_restatement(ctx):
     single = rs.render(rgapi.default_settings(**st))
     outs, _ = rgapi.render_batch(ctx, [rs], [rgapi.make_job(0, **st)])
     assert np.array_equal(outs[0]["hdr"], single["hdr"])
+
+
+def test_render_into_page_locked_buffer(ctx):
+    """rg_host_alloc'd output (rgapi.host_zeros) receives the same image as
+    pageable memory, over repeated renders into the same buffer."""
+    rs = rgapi.RenderScene(ctx, scenes.cube_room())
+    st = rgapi.default_settings(width=40, height=24, spp=4, max_depth=5)
+    want = rs.render(st, hdr_only=True)["hdr"]
+    buf = rgapi.host_zeros((24, 40, 4))
+    for _ in range(2):
+        got = rs.render(st, hdr_only=True, hdr_out=buf)["hdr"]
+        assert got is buf
+        assert np.array_equal(buf, want)
