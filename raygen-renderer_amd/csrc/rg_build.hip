@@ -81,13 +81,20 @@ __device__ __forceinline__ uint32_t ord_bits(float v) {
     if (b == 0x80000000u) b = 0u;
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
+// A NaN takes no part (the identity key), as fminf/fmaxf skip a NaN operand.
 __device__ __forceinline__ u64 key_min(float v, uint32_t pos) {
+    if (v != v) return ~0ull;
     return ((u64)ord_bits(v) << 32) | ((u64)(0x7FFFFFFFu - pos) << 1) | (u64)(__float_as_uint(v) >> 31);
 }
 __device__ __forceinline__ u64 key_max(float v, uint32_t pos) {
+    if (v != v) return 0ull;
     return ((u64)ord_bits(v) << 32) | ((u64)pos << 1) | (u64)(__float_as_uint(v) >> 31);
 }
+// A key's value; the identities are the empty box's bounds (no real key
+// equals either).
 __device__ __forceinline__ float key_val(u64 k) {
+    if (k == ~0ull) return FLT_MAX;
+    if (k == 0ull) return -FLT_MAX;
     const uint32_t o = (uint32_t)(k >> 32);
     const uint32_t b = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
     return (b == 0u && (k & 1ull)) ? -0.0f : __uint_as_float(b);

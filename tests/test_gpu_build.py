@@ -4,7 +4,7 @@
   TriangleBVH::build (bvh.cpp:76-231) node for node and bit for bit, on
   meshes around the size where ranges move from the level kernels to the
   per-wave builder, degenerate inputs (coincident centroids, flat boxes,
-  signed zeros, a far triangle that makes one partition chain run through a
+  signed zeros, NaN coordinates, a far triangle that makes one partition chain run through a
   whole range, more than 65535 coincident triangles) and a tessellated
   sphere;
 * the wide trees (4- and 8-wide) and triangle records equal the host path's:
@@ -59,6 +59,15 @@ def far_first(n, seed):
     return np.concatenate([far, soup(n - 1, seed)], axis=0)
 
 
+def nan_coords(n, seed):
+    v = soup(n, seed)
+    v[7, 1] = np.nan
+    v[1000:1004, 3:6] = np.nan
+    v[2000, :] = np.nan
+    v[2500:2900, :] = np.nan
+    return v
+
+
 def flat(n, seed):
     v = soup(n, seed).reshape(n, 3, 3)
     v[:, :, 2] = 0.25
@@ -84,6 +93,7 @@ CASES = {
     "soup40000": lambda: soup(40000, 9, spread=0.01),
     "ico5": lambda: rgapi.gen_icosphere(5),
     "signed_zero": lambda: signed_zero(3000, 10),
+    "nan_coords": lambda: nan_coords(3000, 16),
     "far_first": lambda: far_first(6000, 11),
     "flat_z": lambda: flat(2500, 12),
     "coincident9": lambda: stacked(9),

@@ -64,6 +64,14 @@ def signed_zero(n, seed):
     return v.reshape(n, 9)
 
 
+def nan_coords(n, seed):
+    v = soup(n, seed)
+    v[7, 1] = np.nan
+    v[100:104, 3:6] = np.nan
+    v[200, :] = np.nan
+    return v
+
+
 CASES = {
     "ico4": lambda: rgapi.gen_icosphere(4),
     "soup1": lambda: soup(1, 1),
@@ -74,6 +82,8 @@ CASES = {
     # coordinates of +0.0 and -0.0 mixed: box min/max ties keep the later
     # operand's sign, as fminf/fmaxf do (bvh.cpp:36-51)
     "signed_zero": lambda: signed_zero(400, 8),
+    # NaN coordinates take no part in a box (fminf/fmaxf skip a NaN operand)
+    "nan_coords": lambda: nan_coords(300, 9),
     "flat_z": lambda: np.concatenate([soup(500, 5)[:, :2], np.zeros((500, 1), np.float32),
                                       soup(500, 6)[:, 3:5], np.zeros((500, 1), np.float32),
                                       soup(500, 7)[:, 6:8], np.zeros((500, 1), np.float32)], axis=1),
