@@ -136,7 +136,8 @@ int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params);
  * permutation (n_tris entries, nullable).  *n_nodes receives the count. */
 int rg_bvh_export2(const rg_scene *scene, rg_bvh2_node *nodes, uint64_t cap,
                    uint32_t *perm, uint64_t *n_nodes);
-/* Device-side footprint of the scene (triangles + nodes), in bytes. */
+/* Device-side footprint of the scene's tree (triangle records + wide nodes,
+ * and after a GPU build also its binary nodes and permutation), in bytes. */
 int rg_scene_device_bytes(const rg_scene *scene, uint64_t *bytes);
 /* Bounding box of the geometry: lo xyz, hi xyz. */
 int rg_scene_bounds(const rg_scene *scene, float box[6]);

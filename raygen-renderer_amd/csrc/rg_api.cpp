@@ -189,8 +189,11 @@ int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params) {
         s->host_tree = false;
         s->built = true;
         s->tris_dirty = false;
+        // the traversal's wide nodes and triangle records, plus the binary
+        // tree and permutation kept for exports and triangle rewrites
         s->device_bytes = s->n_wide * (s->width == 8 ? sizeof(Node8) : sizeof(Node4)) +
-                          (uint64_t)s->n_tris * sizeof(Tri48);
+                          (uint64_t)s->n_tris * sizeof(Tri48) + (uint64_t)s->n_nodes2 * sizeof(Node2) +
+                          (uint64_t)s->n_tris * 4;
         return RG_OK;
     }
     try {
