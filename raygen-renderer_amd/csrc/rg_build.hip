@@ -1079,7 +1079,7 @@ struct HostTop {
 static_assert(sizeof(HostTop) == sizeof(TopNode), "top node layouts differ");
 
 template <class NodeT, int W>
-int build_wide(Scene &s, const Node2 *d_n2, uint32_t nn2, hipStream_t st, void **d_wide_out, uint32_t &n_wide,
+int build_wide(const Node2 *d_n2, uint32_t nn2, hipStream_t st, void **d_wide_out, uint32_t &n_wide,
                uint32_t &stack_need) {
     // upper bound: one wide node per internal binary node
     const uint32_t cap = std::max<uint32_t>(1, nn2);
@@ -1159,7 +1159,6 @@ int build_wide(Scene &s, const Node2 *d_n2, uint32_t nn2, hipStream_t st, void *
     *d_wide_out = wide;
     n_wide = nw;
     stack_need = best + 1;
-    (void)s;
     return RG_OK;
 }
 
@@ -1417,9 +1416,9 @@ int gpu_build_bvh(Scene &s, uint32_t bins, uint32_t leaf, int width, void *strea
             n_wide = 1;
             stack_need = 1;
         } else if (width == 8) {
-            rc = build_wide<Node8, 8>(s, d_n2, total, st, &d_wide, n_wide, stack_need);
+            rc = build_wide<Node8, 8>(d_n2, total, st, &d_wide, n_wide, stack_need);
         } else {
-            rc = build_wide<Node4, 4>(s, d_n2, total, st, &d_wide, n_wide, stack_need);
+            rc = build_wide<Node4, 4>(d_n2, total, st, &d_wide, n_wide, stack_need);
         }
     }
     dev_free(mem);
