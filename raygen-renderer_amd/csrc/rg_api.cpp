@@ -132,13 +132,19 @@ int rg_scene_create(rg_ctx *ctx, const float *v9, uint32_t n_tris, rg_scene **ou
         delete s;
         return set_error(RG_E_NOMEM, "geometry copy failed");
     }
-    for (int k = 0; k < 3; k++) { s->box[k] = INFINITY; s->box[3 + k] = -INFINITY; }
-    for (size_t i = 0; i < (size_t)n_tris * 3; i++)
-        for (int k = 0; k < 3; k++) {
-            const float x = s->v9[3 * i + k];
-            s->box[k] = fminf(s->box[k], x);
-            s->box[3 + k] = fmaxf(s->box[3 + k], x);
-        }
+    // bounds in registers (a loop through s->box kept every step in memory)
+    float lo0 = INFINITY, lo1 = INFINITY, lo2 = INFINITY, hi0 = -INFINITY, hi1 = -INFINITY, hi2 = -INFINITY;
+    const float *p = s->v9.data();
+    for (size_t i = 0; i < (size_t)n_tris * 3; i++, p += 3) {
+        lo0 = p[0] < lo0 ? p[0] : lo0;
+        lo1 = p[1] < lo1 ? p[1] : lo1;
+        lo2 = p[2] < lo2 ? p[2] : lo2;
+        hi0 = p[0] > hi0 ? p[0] : hi0;
+        hi1 = p[1] > hi1 ? p[1] : hi1;
+        hi2 = p[2] > hi2 ? p[2] : hi2;
+    }
+    s->box[0] = lo0; s->box[1] = lo1; s->box[2] = lo2;
+    s->box[3] = hi0; s->box[4] = hi1; s->box[5] = hi2;
     *out = reinterpret_cast<rg_scene *>(s);
     return RG_OK;
 }
