@@ -99,6 +99,22 @@ __device__ __forceinline__ float key_val(u64 k) {
     const uint32_t b = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
     return (b == 0u && (k & 1ull)) ? -0.0f : __uint_as_float(b);
 }
+// Key reductions over a wave (every lane gets the result).
+__device__ __forceinline__ u64 wave_min(u64 v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const u64 w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ u64 wave_max(u64 v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const u64 w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
 // the reference's fold steps, glibc's tie rule (the later operand)
 __device__ __forceinline__ float min_acc(float acc, float x) { return (acc < x || x != x) ? acc : x; }
 __device__ __forceinline__ float max_acc(float acc, float x) { return (acc > x || x != x) ? acc : x; }
@@ -234,9 +250,34 @@ __global__ __launch_bounds__(kBlk) void k_bounds(Prims p, uint32_t n, uint32_t n
     }
     __syncthreads();
     for (uint32_t e = 0; e < kPerThread; e++) {
-        if (sg[e] == kNone) continue;
         const uint32_t i = base + e * kBlk + tid;
-        const int slot = sg[e] == s_lo ? 0 : 1;
+        const uint32_t my = sg[e];
+        const u64 act = __ballot(my != kNone);
+        if (!act) continue;
+        // a wave whose elements all belong to one range (the usual case)
+        // reduces its keys first and updates LDS from one lane
+        const uint32_t ws = __shfl(my, __ffsll((long long)act) - 1, 64);
+        if (__ballot(my != kNone && my != ws) == 0) {
+            u64 km[12];
+            for (int k = 0; k < 3; k++) {
+                const bool a = my != kNone;
+                km[k] = a ? key_min(p.v[k][i], i) : kMinId;
+                km[3 + k] = a ? key_max(p.v[3 + k][i], i) : kMaxId;
+                km[6 + k] = a ? key_min(p.v[6 + k][i], i) : kMinId;
+                km[9 + k] = a ? key_max(p.v[6 + k][i], i) : kMaxId;
+            }
+            for (int k = 0; k < 12; k++) km[k] = ((k / 3) & 1) ? wave_max(km[k]) : wave_min(km[k]);
+            if ((threadIdx.x & 63u) == 0) {
+                const int slot = ws == s_lo ? 0 : 1;
+                for (int k = 0; k < 12; k++) {
+                    if (((k / 3) & 1) == 0) atomicMin(&sk[slot][k], km[k]);
+                    else atomicMax(&sk[slot][k], km[k]);
+                }
+            }
+            continue;
+        }
+        if (my == kNone) continue;
+        const int slot = my == s_lo ? 0 : 1;
         for (int k = 0; k < 3; k++) {
             atomicMin(&sk[slot][k], key_min(p.v[k][i], i));
             atomicMax(&sk[slot][3 + k], key_max(p.v[3 + k][i], i));
@@ -355,63 +396,111 @@ __global__ __launch_bounds__(kBlk) void k_bin(Prims p, uint32_t n, uint32_t nseg
     }
 }
 
+
 // The SAH sweep and split decision of one range (bvh.cpp:145-186) from its
-// bin keys; returns the best bin (-1: leaf) and the left count.
-__device__ int sah_split(const u64 (*bk)[6], const uint32_t *bc, uint32_t bins, uint32_t count,
-                         const float plo[3], const float phi[3], uint32_t &nl) {
+// bin keys, one lane per bin: the left boxes and counts are prefix
+// min/max/sums over the lanes, the right ones suffix ones, the cost is the
+// reference's 1 + 1.5 / A_p (A_L N_L + A_R N_R) in its operation order, and
+// the first minimum cost below the leaf cost 1.5 N wins, as the reference's
+// ascending strict-< scan picks it.  Returns the best bin (-1: leaf) and
+// the left count.  The prefix boxes equal the
+// sequential fold's up to the sign of a zero bound, which changes no area
+// comparison.
+__device__ int wave_sah(const u64 (*bk)[6], const uint32_t *bc, uint32_t bins, uint32_t count, const float plo[3],
+                        const float phi[3], uint32_t lane, uint32_t &nl) {
     const float kTrav = 1.0f, kIsect = 1.5f;
-    float larea[kMaxBins], rarea[kMaxBins];
-    uint32_t lcnt[kMaxBins], rcnt[kMaxBins];
-    const int last = (int)bins - 1;
-    {
-        float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-        uint32_t c = 0;
-        for (int i = 0; i < last; i++) {
-            if (bc[i] > 0)
-                for (int k = 0; k < 3; k++) {
-                    lo[k] = min_acc(lo[k], key_val(bk[i][k]));
-                    hi[k] = max_acc(hi[k], key_val(bk[i][3 + k]));
-                }
-            c += bc[i];
-            larea[i] = (c > 0) ? box_area(lo, hi) : 0.0f;
-            lcnt[i] = c;
+    uint32_t c = 0;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    if (lane < bins && bc[lane] > 0) {
+        c = bc[lane];
+        for (int k = 0; k < 3; k++) {
+            lo[k] = key_val(bk[lane][k]);
+            hi[k] = key_val(bk[lane][3 + k]);
         }
     }
-    {
-        float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-        uint32_t c = 0;
-        for (int i = last; i > 0; i--) {
-            if (bc[i] > 0)
-                for (int k = 0; k < 3; k++) {
-                    lo[k] = min_acc(lo[k], key_val(bk[i][k]));
-                    hi[k] = max_acc(hi[k], key_val(bk[i][3 + k]));
-                }
-            c += bc[i];
-            rarea[i - 1] = (c > 0) ? box_area(lo, hi) : 0.0f;
-            rcnt[i - 1] = c;
+    uint32_t lc = c, rc = c;
+    float llo[3], lhi[3], rlo[3], rhi[3];
+    for (int k = 0; k < 3; k++) { llo[k] = rlo[k] = lo[k]; lhi[k] = rhi[k] = hi[k]; }
+    for (int o = 1; o < (int)kMaxBins; o <<= 1) {
+        const uint32_t xl = __shfl_up(lc, o, 64), xr = __shfl_down(rc, o, 64);
+        float al[3], ah[3], bl[3], bh[3];
+        for (int k = 0; k < 3; k++) {
+            al[k] = __shfl_up(llo[k], o, 64);
+            ah[k] = __shfl_up(lhi[k], o, 64);
+            bl[k] = __shfl_down(rlo[k], o, 64);
+            bh[k] = __shfl_down(rhi[k], o, 64);
         }
+        if (lane >= (uint32_t)o) {
+            lc += xl;
+            for (int k = 0; k < 3; k++) { llo[k] = fminf(llo[k], al[k]); lhi[k] = fmaxf(lhi[k], ah[k]); }
+        }
+        if (lane + o < bins) {
+            rc += xr;
+            for (int k = 0; k < 3; k++) { rlo[k] = fminf(rlo[k], bl[k]); rhi[k] = fmaxf(rhi[k], bh[k]); }
+        }
+    }
+    // split i: left = prefix through bin i (this lane), right = suffix from bin i + 1
+    const uint32_t rcn = __shfl_down(rc, 1, 64);
+    float nlo[3], nhi[3];
+    for (int k = 0; k < 3; k++) {
+        nlo[k] = __shfl_down(rlo[k], 1, 64);
+        nhi[k] = __shfl_down(rhi[k], 1, 64);
     }
     const float parea = box_area(plo, phi);
     const float inv_parea = (parea > 0.0f) ? 1.0f / parea : 0.0f;
-    float best = (float)count * kIsect;
-    int best_bin = -1;
-    for (int i = 0; i < last; i++) {
-        if (lcnt[i] == 0 || rcnt[i] == 0) continue;
-        const float cost = kTrav + kIsect * inv_parea * (larea[i] * (float)lcnt[i] + rarea[i] * (float)rcnt[i]);
-        if (cost < best) { best = cost; best_bin = i; }
+    const float leaf_cost = (float)count * kIsect;
+    float cost = FLT_MAX;
+    bool valid = false;
+    if (lane + 1 < bins && lc > 0 && rcn > 0) {
+        const float la = box_area(llo, lhi), ra = box_area(nlo, nhi);
+        cost = kTrav + kIsect * inv_parea * (la * (float)lc + ra * (float)rcn);
+        valid = cost < leaf_cost;
     }
-    nl = best_bin >= 0 ? lcnt[best_bin] : 0;
-    return best_bin;
+    // first lane holding the smallest valid cost
+    float m = valid ? cost : FLT_MAX;
+    for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o, 64));
+    const u64 at = __ballot(valid && cost == m);
+    if (!at) {
+        nl = 0;
+        return -1;
+    }
+    const int best = __ffsll((long long)at) - 1;
+    nl = (uint32_t)__shfl(lc, best, 64);
+    return best;
 }
 
-// One block: decides every range of the level, allocates the children (top
+// The split decision of every range of the level, one wave per range (the
+// bins' keys read by the lanes together): best bin and left count.
+__global__ __launch_bounds__(256) void k_decide(Build B, uint32_t nseg, SegInfo *segs,
+                                                const SegAcc *__restrict__ acc) {
+    const uint32_t s = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (s >= nseg) return;
+    SegInfo &S = segs[s];
+    int best = -1;
+    uint32_t nl = 0;
+    if (S.axis >= 0) {
+        float plo[3], phi[3];
+        for (int k = 0; k < 3; k++) {
+            plo[k] = key_val(acc[s].box[k]);
+            phi[k] = key_val(acc[s].box[3 + k]);
+        }
+        best = wave_sah(acc[s].bin, acc[s].cnt, B.bins, S.count, plo, phi, lane, nl);
+        if (best >= 0 && (nl == 0 || nl >= S.count)) best = -1;
+    }
+    if (lane == 0) {
+        S.best = best;
+        S.nl = best >= 0 ? nl : 0;
+    }
+}
+
+// One block: allocates the children of every range k_decide split (top
 // nodes in range order, left then right; next-level ranges and wave tasks in
-// the same order) and writes the counters.
+// the same order), marks the others leaves and writes the counters.
 constexpr uint32_t kSplitBlk = 1024;
 
 __global__ __launch_bounds__(kSplitBlk) void k_split(Build B, uint32_t nseg, uint32_t ntop0, uint32_t ntask0,
-                                                     SegInfo *segs, const SegAcc *__restrict__ acc, SegInfo *next,
-                                                     SegAcc *next_acc) {
+                                                     SegInfo *segs, SegInfo *next, SegAcc *next_acc) {
     typedef hipcub::BlockScan<uint32_t, kSplitBlk> Scan;
     __shared__ typename Scan::TempStorage tmp;
     __shared__ uint32_t run_top, run_seg, run_task, max_nl;
@@ -424,18 +513,9 @@ __global__ __launch_bounds__(kSplitBlk) void k_split(Build B, uint32_t nseg, uin
         uint32_t nl = 0;
         int best = -1;
         if (s < nseg) {
-            SegInfo &S = segs[s];
-            if (S.axis >= 0) {
-                float plo[3], phi[3];
-                for (int k = 0; k < 3; k++) {
-                    plo[k] = key_val(acc[s].box[k]);
-                    phi[k] = key_val(acc[s].box[3 + k]);
-                }
-                best = sah_split(acc[s].bin, acc[s].cnt, B.bins, S.count, plo, phi, nl);
-                if (best >= 0 && (nl == 0 || nl >= S.count)) best = -1;
-            }
-            S.best = best;
-            S.nl = nl;
+            const SegInfo &S = segs[s];
+            best = S.best;
+            nl = S.nl;
             if (best >= 0) {
                 nsplit = 1;
                 nbig = (nl > kSmall) + (S.count - nl > kSmall);
@@ -458,6 +538,7 @@ __global__ __launch_bounds__(kSplitBlk) void k_split(Build B, uint32_t nseg, uin
                               run_task + r_small + nsmall > B.task_cap)) {
                 B.out->error = 4;
                 best = -1;
+                S.best = -1;
             }
             if (best < 0) {
                 T.kind = 1;
@@ -581,90 +662,6 @@ struct WaveLds {
     uint32_t bc[kMaxBins];
     uint16_t stack[kSmall][3];
 };
-
-__device__ __forceinline__ u64 wave_min(u64 v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const u64 w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-__device__ __forceinline__ u64 wave_max(u64 v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const u64 w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-
-// The SAH decision of sah_split with one lane per bin: the left boxes and
-// counts are prefix min/max/sums over the lanes, the right ones suffix
-// ones, and the first minimum cost below the leaf cost wins, as the
-// reference's ascending strict-< scan picks it.  The prefix boxes equal the
-// sequential fold's up to the sign of a zero bound, which changes no area
-// comparison.
-__device__ int wave_sah(const u64 (*bk)[6], const uint32_t *bc, uint32_t bins, uint32_t count, const float plo[3],
-                        const float phi[3], uint32_t lane, uint32_t &nl) {
-    const float kTrav = 1.0f, kIsect = 1.5f;
-    uint32_t c = 0;
-    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    if (lane < bins && bc[lane] > 0) {
-        c = bc[lane];
-        for (int k = 0; k < 3; k++) {
-            lo[k] = key_val(bk[lane][k]);
-            hi[k] = key_val(bk[lane][3 + k]);
-        }
-    }
-    uint32_t lc = c, rc = c;
-    float llo[3], lhi[3], rlo[3], rhi[3];
-    for (int k = 0; k < 3; k++) { llo[k] = rlo[k] = lo[k]; lhi[k] = rhi[k] = hi[k]; }
-    for (int o = 1; o < (int)kMaxBins; o <<= 1) {
-        const uint32_t xl = __shfl_up(lc, o, 64), xr = __shfl_down(rc, o, 64);
-        float al[3], ah[3], bl[3], bh[3];
-        for (int k = 0; k < 3; k++) {
-            al[k] = __shfl_up(llo[k], o, 64);
-            ah[k] = __shfl_up(lhi[k], o, 64);
-            bl[k] = __shfl_down(rlo[k], o, 64);
-            bh[k] = __shfl_down(rhi[k], o, 64);
-        }
-        if (lane >= (uint32_t)o) {
-            lc += xl;
-            for (int k = 0; k < 3; k++) { llo[k] = fminf(llo[k], al[k]); lhi[k] = fmaxf(lhi[k], ah[k]); }
-        }
-        if (lane + o < bins) {
-            rc += xr;
-            for (int k = 0; k < 3; k++) { rlo[k] = fminf(rlo[k], bl[k]); rhi[k] = fmaxf(rhi[k], bh[k]); }
-        }
-    }
-    // split i: left = prefix through bin i (this lane), right = suffix from bin i + 1
-    const uint32_t rcn = __shfl_down(rc, 1, 64);
-    float nlo[3], nhi[3];
-    for (int k = 0; k < 3; k++) {
-        nlo[k] = __shfl_down(rlo[k], 1, 64);
-        nhi[k] = __shfl_down(rhi[k], 1, 64);
-    }
-    const float parea = box_area(plo, phi);
-    const float inv_parea = (parea > 0.0f) ? 1.0f / parea : 0.0f;
-    const float leaf_cost = (float)count * kIsect;
-    float cost = FLT_MAX;
-    bool valid = false;
-    if (lane + 1 < bins && lc > 0 && rcn > 0) {
-        const float la = box_area(llo, lhi), ra = box_area(nlo, nhi);
-        cost = kTrav + kIsect * inv_parea * (la * (float)lc + ra * (float)rcn);
-        valid = cost < leaf_cost;
-    }
-    // first lane holding the smallest valid cost
-    float m = valid ? cost : FLT_MAX;
-    for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o, 64));
-    const u64 at = __ballot(valid && cost == m);
-    if (!at) {
-        nl = 0;
-        return -1;
-    }
-    const int best = __ffsll((long long)at) - 1;
-    nl = (uint32_t)__shfl(lc, best, 64);
-    return best;
-}
 
 __global__ __launch_bounds__(64) void k_wave_build(Build B, uint32_t ntask) {
     __shared__ WaveLds L;
@@ -1268,7 +1265,8 @@ int gpu_build_bvh(Scene &s, uint32_t bins, uint32_t leaf, int width, void *strea
         hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(kBlk), 0, st, B.a, n, nseg, acc);
         hipLaunchKernelGGL(k_axis, dim3(blocks_for(nseg, 128)), dim3(128), 0, st, B, nseg, acc, segs, acc);
         hipLaunchKernelGGL(k_bin, dim3(nb), dim3(kBlk), 0, st, B.a, n, nseg, bins, segs, acc, B.binv);
-        hipLaunchKernelGGL(k_split, dim3(1), dim3(kSplitBlk), 0, st, B, nseg, ntop, ntask, segs, acc, next, next_acc);
+        hipLaunchKernelGGL(k_decide, dim3(blocks_for((uint64_t)nseg * 64, 256)), dim3(256), 0, st, B, nseg, segs, acc);
+        hipLaunchKernelGGL(k_split, dim3(1), dim3(kSplitBlk), 0, st, B, nseg, ntop, ntask, segs, next, next_acc);
         rc = hip_ok(hipGetLastError(), "level kernels");
         if (!rc) rc = hip_ok(hipMemcpyAsync(host_out, B.out, sizeof(LevelOut), hipMemcpyDeviceToHost, st), "level");
         if (!rc) rc = hip_ok(hipStreamSynchronize(st), "level");
