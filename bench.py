@@ -156,7 +156,7 @@ def bench_bsdf_frame(ctx, do_cpu, cpu_seconds):
     t1 = time.perf_counter()
     rs = rgapi.RenderScene(ctx, sc)
     t2 = time.perf_counter()
-    rs.build()  # again, warm: the GPU builder alone
+    rs.build()  # again: the GPU builder alone, the triangles already on the device
     build_s = time.perf_counter() - t2
     st = rgapi.default_settings(width=1920, height=1080, spp=64, max_depth=8, adaptive=1, adaptive_base=4,
                                 adaptive_threshold=1e-3, denoise=1, denoise_levels=5, seed=0xB5DF0002)
@@ -186,7 +186,7 @@ def bench_bsdf_frame(ctx, do_cpu, cpu_seconds):
            "rays": out["rays"], "shadow_rays": out["shadow_rays"],
            "mean_spp": round(float(taken) / (1920 * 1080), 2),
            "scene_gen_s": round(t1 - t0, 2), "scene_setup_s": round(t2 - t1, 3),
-           "bvh_build_s": round(build_s, 4)}
+           "bvh_rebuild_s": round(build_s, 4)}
     res["postprocess"] = bench_postprocess(ctx, out["hdr"], do_cpu)
     if do_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -535,9 +535,12 @@ def main():
     build_host_s = time.perf_counter() - t0
     builds = []
     for _ in range(3):
+        fresh = rgapi.Scene(ctx, v9)  # a new scene: its first build uploads the triangles
         t0 = time.perf_counter()
-        sc.build(bins=16, leaf=4, width=4, builder=rgapi.BUILDER_DEVICE)
+        fresh.build(bins=16, leaf=4, width=4, builder=rgapi.BUILDER_DEVICE)
         builds.append(time.perf_counter() - t0)
+        fresh.close()
+    sc.build(bins=16, leaf=4, width=4, builder=rgapi.BUILDER_DEVICE)
     build_s = min(builds)
 
     n = N_RAYS

@@ -84,6 +84,7 @@ void rg_destroy(rg_ctx *ctx) {
         release_pt(c);
         release_post(c);
         release_engine(c);
+        release_build(*c);
         dev_free(c->trace_work);
     }
     if (c->own_stream) dev_stream_destroy(c->own_stream);
@@ -191,7 +192,7 @@ int rg_bvh_build(rg_ctx *ctx, rg_scene *scene, const rg_bvh_params *params) {
         s->device = c->device;
     }
     if (on_gpu) {
-        if (int rc = gpu_build_bvh(*s, p.bins, p.leaf, s->width, c->stream)) return rc;
+        if (int rc = gpu_build_bvh(*c, *s, p.bins, p.leaf, s->width)) return rc;
         s->host_tree = false;
         s->built = true;
         s->tris_dirty = false;

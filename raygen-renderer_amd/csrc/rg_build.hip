@@ -1069,6 +1069,20 @@ struct Arena {
 template <class T>
 size_t piece(size_t count) { return ((count * sizeof(T)) + 255) & ~(size_t)255; }
 
+// The context's workspace of at least `bytes` (kept between builds).
+int workspace(Ctx &c, size_t bytes, char **out) {
+    if (c.build_ws_cap < bytes) {
+        dev_free(c.build_ws);
+        c.build_ws = nullptr;
+        c.build_ws_cap = 0;
+        const size_t cap = bytes + bytes / 8;
+        if (int rc = dev_malloc(&c.build_ws, cap)) return rc;
+        c.build_ws_cap = cap;
+    }
+    *out = (char *)c.build_ws;
+    return RG_OK;
+}
+
 struct HostTop {
     Node2 nd;
     uint32_t kind, child[2], task;
@@ -1076,7 +1090,7 @@ struct HostTop {
 static_assert(sizeof(HostTop) == sizeof(TopNode), "top node layouts differ");
 
 template <class NodeT, int W>
-int build_wide(const Node2 *d_n2, uint32_t nn2, hipStream_t st, void **d_wide_out, uint32_t &n_wide,
+int build_wide(Ctx &c, const Node2 *d_n2, uint32_t nn2, hipStream_t st, void **d_wide_out, uint32_t &n_wide,
                uint32_t &stack_need) {
     // upper bound: one wide node per internal binary node
     const uint32_t cap = std::max<uint32_t>(1, nn2);
@@ -1085,7 +1099,7 @@ int build_wide(const Node2 *d_n2, uint32_t nn2, hipStream_t st, void **d_wide_ou
     const size_t bytes = piece<WideRec<W>>(cap) + 6 * piece<uint32_t>(cap) + piece<uint32_t>(64) +
                          piece<char>(scan_bytes) + 1024;
     char *mem = nullptr;
-    if (int rc = dev_malloc((void **)&mem, bytes)) return rc;
+    if (int rc = workspace(c, bytes, &mem)) return rc;
     Arena ar{mem, 0, bytes};
     WideRec<W> *rec = ar.take<WideRec<W>>(cap);
     uint32_t *frontier = ar.take<uint32_t>(cap);
@@ -1148,7 +1162,6 @@ int build_wide(const Node2 *d_n2, uint32_t nn2, hipStream_t st, void **d_wide_ou
     if (!rc) rc = hip_ok(hipStreamSynchronize(st), "wide tree");
     if (!rc && flags[1]) rc = set_error(RG_E_INTERNAL, "GPU BVH build: wide tree inconsistent");
     const uint32_t best = flags[0];
-    dev_free(mem);
     if (rc) {
         dev_free(wide);
         return rc;
@@ -1163,15 +1176,22 @@ int build_wide(const Node2 *d_n2, uint32_t nn2, hipStream_t st, void **d_wide_ou
 
 bool gpu_build_supported(uint32_t bins, uint32_t leaf) { return bins >= 2 && bins <= kMaxBins && leaf >= 1; }
 
+// The scene's triangles on the device: the copy kept from an earlier upload
+// (they never change after rg_scene_create), else uploaded now and kept.
+static int device_v9(Scene &s, hipStream_t st) {
+    if (s.d_v9) return RG_OK;
+    if (int rc = dev_malloc(&s.d_v9, (size_t)s.n_tris * 36)) return rc;
+    return hip_ok(hipMemcpyAsync(s.d_v9, s.v9.data(), (size_t)s.n_tris * 36, hipMemcpyHostToDevice, st), "upload");
+}
+
 int gpu_make_tris(Scene &s, void *stream) {
     hipStream_t st = (hipStream_t)stream;
     const uint32_t n = s.n_tris;
     if (n == 0 || !s.d_tris || !s.d_perm || !s.d_nodes2) return RG_OK;
-    float *v9 = nullptr;
     uint8_t *pred = nullptr;
-    int rc = dev_malloc((void **)&v9, (size_t)n * 36);
+    int rc = device_v9(s, st);
+    const float *v9 = (const float *)s.d_v9;
     if (!rc && !s.pred_mask.empty()) rc = dev_malloc((void **)&pred, n);
-    if (!rc) rc = hip_ok(hipMemcpyAsync(v9, s.v9.data(), (size_t)n * 36, hipMemcpyHostToDevice, st), "upload");
     if (!rc && pred) rc = hip_ok(hipMemcpyAsync(pred, s.pred_mask.data(), n, hipMemcpyHostToDevice, st), "upload");
     if (!rc) {
         hipLaunchKernelGGL(k_tris, dim3(blocks_for(n, 256)), dim3(256), 0, st, v9, (const uint32_t *)s.d_perm, pred,
@@ -1181,13 +1201,20 @@ int gpu_make_tris(Scene &s, void *stream) {
         rc = hip_ok(hipGetLastError(), "triangle records");
     }
     if (!rc) rc = hip_ok(hipStreamSynchronize(st), "triangle records");
-    dev_free(v9);
     dev_free(pred);
     return rc;
 }
 
-int gpu_build_bvh(Scene &s, uint32_t bins, uint32_t leaf, int width, void *stream) {
-    hipStream_t st = (hipStream_t)stream;
+void release_build(Ctx &c) {
+    dev_free(c.build_ws);
+    c.build_ws = nullptr;
+    c.build_ws_cap = 0;
+    if (c.build_host) (void)hipHostFree(c.build_host);
+    c.build_host = nullptr;
+}
+
+int gpu_build_bvh(Ctx &c, Scene &s, uint32_t bins, uint32_t leaf, int width) {
+    hipStream_t st = (hipStream_t)c.stream;
     const uint32_t n = s.n_tris;
     if (n == 0 || !gpu_build_supported(bins, leaf))
         return set_error(RG_E_INTERNAL, "GPU BVH build: unsupported input");
@@ -1200,9 +1227,9 @@ int gpu_build_bvh(Scene &s, uint32_t bins, uint32_t leaf, int width, void *strea
     const size_t bytes = 20 * piece<float>(n) + 4 * piece<uint32_t>(n) + piece<uint8_t>(n) + 3 * piece<uint32_t>(n) +
                          2 * piece<SegInfo>(seg_cap) + 2 * piece<SegAcc>(seg_cap) + piece<TopNode>(top_cap) +
                          piece<Task>(task_cap) + piece<uint32_t>(task_cap) + piece<Node2>(2 * (size_t)n) +
-                         piece<LevelOut>(1) + piece<char>(scan_bytes) + piece<float>(9 * (size_t)n) + 4096;
+                         piece<LevelOut>(1) + piece<char>(scan_bytes) + 4096;
     char *mem = nullptr;
-    if (int rc = dev_malloc((void **)&mem, bytes)) return rc;
+    if (int rc = workspace(c, bytes, &mem)) return rc;
     Arena ar{mem, 0, bytes};
     Build B;
     B.n = n;
@@ -1231,7 +1258,6 @@ int gpu_build_bvh(Scene &s, uint32_t bins, uint32_t leaf, int width, void *strea
     B.task_nodes = ar.take<Node2>(2 * (size_t)n);
     B.out = ar.take<LevelOut>(1);
     void *scan_tmp = ar.take<char>(scan_bytes);
-    float *v9 = ar.take<float>(9 * (size_t)n);
 
     // results kept by the scene
     Node2 *d_n2 = nullptr;
@@ -1241,9 +1267,10 @@ int gpu_build_bvh(Scene &s, uint32_t bins, uint32_t leaf, int width, void *strea
     if (!rc) rc = dev_malloc((void **)&d_tris, (size_t)n * sizeof(Tri48));
     B.final_perm = d_perm;
 
-    LevelOut *host_out = nullptr;
-    if (!rc) rc = hip_ok(hipHostMalloc((void **)&host_out, sizeof(LevelOut)), "hipHostMalloc");
-    if (!rc) rc = hip_ok(hipMemcpyAsync(v9, s.v9.data(), (size_t)n * 36, hipMemcpyHostToDevice, st), "upload");
+    if (!rc && !c.build_host) rc = hip_ok(hipHostMalloc(&c.build_host, 64), "hipHostMalloc");
+    LevelOut *host_out = (LevelOut *)c.build_host;
+    if (!rc) rc = device_v9(s, st);
+    const float *v9 = (const float *)s.d_v9;
     if (!rc) rc = hip_ok(hipMemsetAsync(B.out, 0, sizeof(LevelOut), st), "clear");
     if (!rc) rc = hip_ok(hipMemsetAsync(B.jump, 0, (size_t)n * 4, st), "clear");
     if (!rc) {
@@ -1374,7 +1401,6 @@ int gpu_build_bvh(Scene &s, uint32_t bins, uint32_t leaf, int width, void *strea
     dev_free(d_top);
     dev_free(d_idx);
     dev_free(d_where);
-    if (host_out) (void)hipHostFree(host_out);
 
     // the wide tree
     void *d_wide = nullptr;
@@ -1414,12 +1440,11 @@ int gpu_build_bvh(Scene &s, uint32_t bins, uint32_t leaf, int width, void *strea
             n_wide = 1;
             stack_need = 1;
         } else if (width == 8) {
-            rc = build_wide<Node8, 8>(d_n2, total, st, &d_wide, n_wide, stack_need);
+            rc = build_wide<Node8, 8>(c, d_n2, total, st, &d_wide, n_wide, stack_need);
         } else {
-            rc = build_wide<Node4, 4>(d_n2, total, st, &d_wide, n_wide, stack_need);
+            rc = build_wide<Node4, 4>(c, d_n2, total, st, &d_wide, n_wide, stack_need);
         }
     }
-    dev_free(mem);
     if (rc) {
         dev_free(d_n2);
         dev_free(d_perm);

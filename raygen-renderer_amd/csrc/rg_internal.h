@@ -123,6 +123,11 @@ struct Ctx {
     static constexpr uint32_t kTraceSlots = 64;
     uint64_t *trace_work = nullptr;
     uint32_t trace_slot = 0;
+    // the GPU BVH builder's scratch, kept between builds (grown on demand)
+    // and its page-locked word for the per-level counters
+    void *build_ws = nullptr;
+    uint64_t build_ws_cap = 0;
+    void *build_host = nullptr;
     uint64_t last_stats[3] = {0, 0, 0};
 };
 
@@ -215,7 +220,10 @@ uint32_t stack_need8(const std::vector<Node8> &nodes);
 // on the device (d_nodes2, d_perm, d_nodes4, d_tris, n_nodes2, n_wide,
 // stack_need and root_box of the scene); the host copies are left empty.
 bool gpu_build_supported(uint32_t bins, uint32_t leaf);
-int gpu_build_bvh(Scene &s, uint32_t bins, uint32_t leaf, int width, void *stream);
+// The context provides the stream and the builder's scratch workspace.
+int gpu_build_bvh(Ctx &c, Scene &s, uint32_t bins, uint32_t leaf, int width);
+// Frees the builder's workspace of a context.
+void release_build(Ctx &c);
 // Rewrites the triangle records of a GPU-built scene (occluder bits changed).
 int gpu_make_tris(Scene &s, void *stream);
 // Fills nodes2 / perm from the device copies of a GPU build.

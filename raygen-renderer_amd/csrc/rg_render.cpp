@@ -185,7 +185,9 @@ int upload_pt_scene(Scene *s, void *stream) {
         for (size_t k = 0; k < s->tex.size(); k++)
             tex_desc.insert(tex_desc.end(), {s->tex[k].w, s->tex[k].h, s->tex_offset[k], 0u});
     }
-    int rc = upload_vec(&s->d_v9, s->v9, stream);
+    // the triangles never change after rg_scene_create: a copy already on the
+    // device (the GPU builder's, or an earlier upload) is current
+    int rc = s->d_v9 ? RG_OK : upload_vec(&s->d_v9, s->v9, stream);
     if (!rc) rc = upload_vec(&s->d_otex, otex, stream);
     if (!rc) rc = upload_vec(&s->d_tex_desc, tex_desc, stream);
     if (!rc) rc = upload_vec(&s->d_texels, s->tex.empty() ? std::vector<float>() : s->tex_linear, stream);
