@@ -208,5 +208,11 @@ int launch_resolve(const float *d_sum, const uint32_t *d_nspp, uint32_t width, u
                    float *d_hdr, void *stream);
 // Fills n 32-bit words with value.
 int launch_fill_u32(void *p, uint32_t value, uint64_t n, void *stream);
+// The per-triangle shading records (precalc, raycommon.cpp:128-149) from
+// the device triangles, vertex normals and object ids; scale[o] = the
+// object's triangle count when it emits, else 0 (sampled_area = scale x
+// area).
+int launch_tri_shade(const float *d_v9, const float *d_n9, const uint32_t *d_obj, const float *d_scale, uint32_t n,
+                     DevTriShade *d_out, void *stream);
 
 }  // namespace rg

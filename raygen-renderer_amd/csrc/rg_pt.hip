@@ -391,6 +391,41 @@ int launch_resolve(const float *d_sum, const uint32_t *d_nspp, uint32_t width, u
     return chk(hipGetLastError(), "resolve launch");
 }
 
+// One thread per triangle: the records upload_pt_scene used to build on the
+// host, with the same binary32 operations in the same order (strict FP,
+// correctly rounded square root and division), so the values are the same.
+__global__ __launch_bounds__(kPtBlock) void k_tri_shade(const float *__restrict__ v9, const float *__restrict__ n9,
+                                                         const uint32_t *__restrict__ obj,
+                                                         const float *__restrict__ scale, uint32_t n,
+                                                         DevTriShade *__restrict__ out) {
+    const uint32_t t = blockIdx.x * kPtBlock + threadIdx.x;
+    if (t >= n) return;
+    const float *v = v9 + 9 * (size_t)t;
+    const v3 a = mk(v[0], v[1], v[2]), b = mk(v[3], v[4], v[5]), c = mk(v[6], v[7], v[8]);
+    const v3 pd = cross(sub(b, a), sub(c, b));
+    DevTriShade r;
+    for (int k = 0; k < 9; k++) r.n[k] = n9[9 * (size_t)t + k];
+    const float inv = 1.0f / len(pd);
+    r.gn[0] = pd.x * inv;
+    r.gn[1] = pd.y * inv;
+    r.gn[2] = pd.z * inv;
+    r.obj = obj[t];
+    const v3 q = cross(sub(a, b), sub(a, c));
+    r.ti_a = 1.0f / len(q);
+    const v3 w = cross(sub(b, a), sub(c, a));
+    r.area = 0.5f * len(w);
+    r.sampled_area = scale[r.obj] > 0.0f ? scale[r.obj] * r.area : 0.0f;
+    out[t] = r;
+}
+
+int launch_tri_shade(const float *d_v9, const float *d_n9, const uint32_t *d_obj, const float *d_scale, uint32_t n,
+                     DevTriShade *d_out, void *stream) {
+    if (n == 0) return RG_OK;
+    hipLaunchKernelGGL(k_tri_shade, dim3((n + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, (hipStream_t)stream,
+                       d_v9, d_n9, d_obj, d_scale, n, d_out);
+    return chk(hipGetLastError(), "shading records launch");
+}
+
 int launch_fill_u32(void *p, uint32_t value, uint64_t n, void *stream) {
     if (n == 0) return RG_OK;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((n + kPtBlock - 1) / kPtBlock)), dim3(kPtBlock), 0,

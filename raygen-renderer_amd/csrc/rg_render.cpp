@@ -30,13 +30,6 @@ int hip_ok(hipError_t e, const char *what) {
     return set_error(RG_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-struct V {
-    float x, y, z;
-};
-inline V vsub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-inline V vcross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-inline float vlen(V a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
-
 void free_pt_scene(Scene *s) {
     for (void **p : {&s->d_v9, &s->d_shade, &s->d_objs, &s->d_lights, &s->d_light_tris, &s->d_media,
                      &s->d_env_rgb, &s->d_env_marg, &s->d_env_cond, &s->d_texels, &s->d_tex_desc, &s->d_otex,
@@ -60,32 +53,29 @@ int upload_vec(void **dst, const std::vector<T> &v, void *stream) {
 // buildEnvmapCDF scene.cpp:400-463).
 int upload_pt_scene(Scene *s, void *stream) {
     const uint32_t n = s->n_tris;
-    std::vector<DevTriShade> sh(n);
-    std::vector<uint32_t> cnt(s->objects.size(), 0);
+    const size_t n_obj = s->objects.size();
+    std::vector<uint32_t> cnt(n_obj, 0);
     for (uint32_t t = 0; t < n; t++) cnt[s->obj_of_tri[t]]++;
-    for (uint32_t t = 0; t < n; t++) {
-        const float *v = s->v9.data() + 9 * (size_t)t;
-        const V a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
-        const V pd = vcross(vsub(b, a), vsub(c, b));
-        DevTriShade &r = sh[t];
-        std::memcpy(r.n, s->n9.data() + 9 * (size_t)t, sizeof r.n);
-        const float inv = 1.0f / vlen(pd);
-        r.gn[0] = pd.x * inv;
-        r.gn[1] = pd.y * inv;
-        r.gn[2] = pd.z * inv;
-        r.obj = s->obj_of_tri[t];
-        r.ti_a = 1.0f / vlen(vcross(vsub(a, b), vsub(a, c)));
-        r.area = 0.5f * vlen(vcross(vsub(b, a), vsub(c, a)));
-        r.sampled_area = (s->objects[r.obj].emission > 0.0f) ? (float)cnt[r.obj] * r.area : 0.0f;
-    }
+    // per object: its triangle count when it emits (sampledArea = count x
+    // area), else 0; emissive objects' triangles in object then index order
+    std::vector<float> scale(n_obj, 0.0f);
+    std::vector<uint32_t> first(n_obj, 0);
     std::vector<DevLight> lights;
-    std::vector<uint32_t> light_tris;
-    for (uint32_t o = 0; o < s->objects.size(); o++) {
+    uint32_t n_light_tris = 0;
+    for (uint32_t o = 0; o < n_obj; o++) {
         if (!(s->objects[o].emission > 0.0f) || cnt[o] == 0) continue;
-        DevLight L{o, (uint32_t)light_tris.size(), cnt[o], 0};
-        for (uint32_t t = 0; t < n; t++)
-            if (s->obj_of_tri[t] == o) light_tris.push_back(t);
-        lights.push_back(L);
+        scale[o] = (float)cnt[o];
+        first[o] = n_light_tris;
+        lights.push_back(DevLight{o, n_light_tris, cnt[o], 0});
+        n_light_tris += cnt[o];
+    }
+    std::vector<uint32_t> light_tris(n_light_tris);
+    if (n_light_tris) {
+        std::vector<uint32_t> at(first);
+        for (uint32_t t = 0; t < n; t++) {
+            const uint32_t o = s->obj_of_tri[t];
+            if (scale[o] > 0.0f) light_tris[at[o]++] = t;
+        }
     }
     std::vector<DevMedium> media(s->media.size());
     for (size_t i = 0; i < s->media.size(); i++) {
@@ -188,11 +178,26 @@ int upload_pt_scene(Scene *s, void *stream) {
     // the triangles never change after rg_scene_create: a copy already on the
     // device (the GPU builder's, or an earlier upload) is current
     int rc = s->d_v9 ? RG_OK : upload_vec(&s->d_v9, s->v9, stream);
+    // the shading records are made on the device from the triangles, the
+    // vertex normals and the object ids
+    void *d_n9 = nullptr, *d_obj = nullptr, *d_scale = nullptr;
+    if (!rc) rc = upload_vec(&d_n9, s->n9, stream);
+    if (!rc) rc = upload_vec(&d_obj, s->obj_of_tri, stream);
+    if (!rc) rc = upload_vec(&d_scale, scale, stream);
+    dev_free(s->d_shade);
+    s->d_shade = nullptr;
+    if (!rc && n) rc = dev_malloc(&s->d_shade, (size_t)n * sizeof(DevTriShade));
+    if (!rc)
+        rc = launch_tri_shade((const float *)s->d_v9, (const float *)d_n9, (const uint32_t *)d_obj,
+                              (const float *)d_scale, n, (DevTriShade *)s->d_shade, stream);
+    if (!rc) rc = dev_sync(stream);
+    dev_free(d_n9);
+    dev_free(d_obj);
+    dev_free(d_scale);
     if (!rc) rc = upload_vec(&s->d_otex, otex, stream);
     if (!rc) rc = upload_vec(&s->d_tex_desc, tex_desc, stream);
     if (!rc) rc = upload_vec(&s->d_texels, s->tex.empty() ? std::vector<float>() : s->tex_linear, stream);
     if (!rc) rc = upload_vec(&s->d_uv6, s->tex.empty() ? std::vector<float>() : s->uv6, stream);
-    if (!rc) rc = upload_vec(&s->d_shade, sh, stream);
     if (!rc) rc = upload_vec(&s->d_objs, s->objects, stream);
     if (!rc) rc = upload_vec(&s->d_lights, lights, stream);
     if (!rc) rc = upload_vec(&s->d_light_tris, light_tris, stream);
