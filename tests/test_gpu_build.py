@@ -187,3 +187,53 @@ def test_icosphere_1_3m_device_equals_host(ctx):
     assert dev.device_bytes() == host.device_bytes() + 32 * len(n_d) + 4 * len(v9)
     rays = rays_for(v9, 1 << 18, 0xB1D)
     assert dev.trace_closest_host(rays).tobytes() == host.trace_closest_host(rays).tobytes()
+
+
+def clusters(n, seed):
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-5, 5, (12, 1, 3))
+    which = rng.integers(0, 12, n)
+    c = centers[which] + rng.normal(0, rng.uniform(0.01, 0.5, (n, 1, 1)), (n, 1, 3))
+    return (c + rng.normal(0, 0.02, (n, 3, 3))).astype(np.float32).reshape(n, 9)
+
+
+def sorted_line(n, seed):
+    """Triangles along x in ascending order with a few far ones interleaved:
+    every level's partition moves long runs of elements."""
+    rng = np.random.default_rng(seed)
+    x = np.sort(rng.uniform(0, 100, n))
+    c = np.stack([x, rng.normal(0, 0.1, n), rng.normal(0, 0.1, n)], 1)[:, None, :]
+    v = c + rng.normal(0, 0.05, (n, 3, 3))
+    far = rng.random(n) < 0.01
+    v[far] += np.array([0.0, 40.0, 0.0])
+    return v.astype(np.float32).reshape(n, 9)
+
+
+def mixed_scales(n, seed):
+    """Small triangles on a sphere, a few huge ones spanning it, duplicates."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    v = d[:, None, :] + rng.normal(0, 0.01, (n, 3, 3))
+    big = rng.random(n) < 0.002
+    v[big] = rng.uniform(-3, 3, (int(big.sum()), 3, 3))
+    dup = rng.random(n) < 0.05
+    v[dup] = v[0]
+    return v.astype(np.float32).reshape(n, 9)
+
+
+@pytest.mark.parametrize("maker,n,seed", [
+    (soup, 50000, 21), (soup, 150000, 22), (clusters, 80000, 23), (clusters, 30001, 24),
+    (sorted_line, 60000, 25), (sorted_line, 9000, 26), (mixed_scales, 70000, 27), (mixed_scales, 4097, 28),
+])
+def test_device_tree_equals_host_builder_on_random_meshes(ctx, maker, n, seed):
+    v9 = np.ascontiguousarray(maker(n, seed), dtype=np.float32)
+    say(f"[build] {maker.__name__} n={n} seed={seed}")
+    dev = rgapi.Scene(ctx, v9)
+    dev.build(builder=rgapi.BUILDER_DEVICE)
+    host = rgapi.Scene(ctx, v9)
+    host.build(builder=rgapi.BUILDER_HOST)
+    n_d, p_d = dev.export2()
+    n_h, p_h = host.export2()
+    assert n_d.tobytes() == n_h.tobytes()
+    assert (p_d == p_h).all()
