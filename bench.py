@@ -51,7 +51,18 @@ def cpu_model():
 
 def host_desc(threads):
     return {"threads": threads, "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
-            "oracle_build": "gcc -O3 -ffp-contract=off -fno-fast-math"}
+            "oracle_build": "gcc -O3 -ffp-contract=off -fno-fast-math (value); "
+                            "gcc -O3 -ffast-math -funroll-loops -march=x86-64-v3 (value_fast_math)"}
+
+
+def host_has_avx2_fma():
+    """The fast-math oracle build targets x86-64-v3 (AVX2, FMA)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = next((ln.split(":", 1)[1].split() for ln in f if ln.startswith("flags")), [])
+    except OSError:
+        return False
+    return "avx2" in flags and "fma" in flags
 
 
 def cpu_baseline(v9, rays_np, seconds, threads):
@@ -77,6 +88,15 @@ def cpu_baseline(v9, rays_np, seconds, threads):
         pops += int(cnt[:, 0].sum())
         tests += int(cnt[:, 1].sum())
         done += len(r)
+    # the same traversal built with the reference's compiler flags
+    # (fast-math, build/linux/makefile:3), on the same rays and threads
+    fdone, fbusy = 0, 0.0
+    while fdone < done and fbusy < seconds / 2 and host_has_avx2_fma():
+        r = rays_np[fdone: fdone + chunk]
+        t0 = time.perf_counter()
+        oracle.trace_closest(nodes, perm, v9, r, threads=threads, fast_math=True)
+        fbusy += time.perf_counter() - t0
+        fdone += len(r)
     # the reference's own default is one render thread (rayrenderer.h:44)
     n1 = min(len(rays_np), 1 << 17)
     t0 = time.perf_counter()
@@ -92,6 +112,9 @@ def cpu_baseline(v9, rays_np, seconds, threads):
                   f"BVH build {build_s:.2f}s excluded",
         "value_1_thread": round(one, 3),
         "sample_1_thread": f"first {n1} rays, the same traversal on 1 thread",
+        "value_fast_math": round(fdone / fbusy / 1e6, 3) if fbusy else None,
+        "sample_fast_math": f"first {fdone} rays, the same traversal built with the reference's flags "
+                            f"(gcc -O3 -ffast-math -funroll-loops -march=x86-64-v3), {threads} threads",
         "bvh_build_s": round(build_s, 3),
         "host": host_desc(threads),
     }, pops / done, tests / done

@@ -10,6 +10,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# the same sources under the reference's compiler flags (fast-math): for
+# timing the CPU baseline only, never for checking results
+FAST_LIB_PATH = os.path.join(HERE, "liboracle_fast.so")
 
 RAY_DTYPE = np.dtype([("o", "<f4", (3,)), ("tmin", "<f4"), ("d", "<f4", (3,)), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("prim", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
@@ -18,34 +21,47 @@ NODE2_DTYPE = np.dtype([("bmin", "<f4", (3,)), ("first_or_left", "<u4"), ("bmax"
 MISS = 0xFFFFFFFF
 
 _lib = None
+_fast = None
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} not built (make -C oracle)")
+    L = ctypes.CDLL(path)
+    vp, u32, u64, i32, i64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_int64
+    sig = {
+        "or_philox4x32_10": (None, [vp, vp, vp]),
+        "or_hash_pixel": (u32, [i32, i32, u32]),
+        "or_radical_inverse": (ctypes.c_float, [u32, i32]),
+        "or_lds_sample": (ctypes.c_float, [i32, i32, i32, i32]),
+        "or_mt_closest": (i32, [vp, vp, vp, vp, vp, ctypes.c_float, vp, vp, vp]),
+        "or_mt_any": (i32, [vp, vp, vp, vp, vp, ctypes.c_float, vp]),
+        "or_bvh_build": (i64, [vp, u32, vp, vp]),
+        "or_trace_closest": (None, [vp, i64, vp, vp, vp, u64, vp, vp, vp, i32]),
+        "or_trace_any": (None, [vp, i64, vp, vp, vp, u64, vp, i32]),
+        "or_brute_closest": (None, [vp, u32, vp, u64, vp, vp]),
+        "or_brute_any": (None, [vp, u32, vp, u64, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
-        L = ctypes.CDLL(LIB_PATH)
-        vp, u32, u64, i32, i64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_int64
-        sig = {
-            "or_philox4x32_10": (None, [vp, vp, vp]),
-            "or_hash_pixel": (u32, [i32, i32, u32]),
-            "or_radical_inverse": (ctypes.c_float, [u32, i32]),
-            "or_lds_sample": (ctypes.c_float, [i32, i32, i32, i32]),
-            "or_mt_closest": (i32, [vp, vp, vp, vp, vp, ctypes.c_float, vp, vp, vp]),
-            "or_mt_any": (i32, [vp, vp, vp, vp, vp, ctypes.c_float, vp]),
-            "or_bvh_build": (i64, [vp, u32, vp, vp]),
-            "or_trace_closest": (None, [vp, i64, vp, vp, vp, u64, vp, vp, vp, i32]),
-            "or_trace_any": (None, [vp, i64, vp, vp, vp, u64, vp, i32]),
-            "or_brute_closest": (None, [vp, u32, vp, u64, vp, vp]),
-            "or_brute_any": (None, [vp, u32, vp, u64, vp]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def fast_lib():
+    """The fast-math build: CPU-baseline timing only."""
+    global _fast
+    if _fast is None:
+        _fast = _load(FAST_LIB_PATH)
+    return _fast
 
 
 def _p(a):
@@ -97,13 +113,14 @@ def bvh_build(v9):
     return nodes[:nn].copy(), perm[:n].copy()
 
 
-def trace_closest(nodes, perm, v9, rays, counts=False, ties=False, threads=1):
+def trace_closest(nodes, perm, v9, rays, counts=False, ties=False, threads=1, fast_math=False):
+    """fast_math: the fast-math build, for timing only."""
     v9 = np.ascontiguousarray(v9, dtype=np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
     hits = np.zeros(len(rays), dtype=HIT_DTYPE)
     cnt = np.zeros((len(rays), 2), dtype=np.uint32) if counts else None
     tie = np.zeros(len(rays), dtype=np.uint8) if ties else None
-    lib().or_trace_closest(_p(nodes), len(nodes), _p(v9), _p(perm), _p(rays), len(rays),
+    (fast_lib() if fast_math else lib()).or_trace_closest(_p(nodes), len(nodes), _p(v9), _p(perm), _p(rays), len(rays),
                            _p(hits), _p(cnt), _p(tie), threads)
     out = [hits]
     if counts:
