@@ -965,6 +965,7 @@ __device__ __forceinline__ uint32_t final_index(uint32_t b, const uint32_t *__re
 
 __device__ __forceinline__ void set_order_dev(Node4 &nd, int oct, const int *ord) {
     nd.order[oct] = (uint8_t)(ord[0] | (ord[1] << 2) | (ord[2] << 4) | (ord[3] << 6));
+    nd.rank[oct] = (uint8_t)(0 << (2 * ord[0]) | 1 << (2 * ord[1]) | 2 << (2 * ord[2]) | 3 << (2 * ord[3]));
 }
 __device__ __forceinline__ void set_order_dev(Node8 &nd, int oct, const int *ord) {
     uint32_t o = 0;
@@ -1432,7 +1433,10 @@ int gpu_build_bvh(Ctx &c, Scene &s, uint32_t bins, uint32_t leaf, int width) {
                 nd.hix[0] = root.bmax[0]; nd.hiy[0] = root.bmax[1]; nd.hiz[0] = root.bmax[2];
                 nd.child[0] = kLeafBit | root.first_or_left;
                 for (int q = 1; q < 4; q++) nd.child[q] = kEmpty;
-                for (int oct = 0; oct < 8; oct++) nd.order[oct] = (uint8_t)(0 | (1 << 2) | (2 << 4) | (3 << 6));
+                for (int oct = 0; oct < 8; oct++) {
+                    nd.order[oct] = (uint8_t)(0 | (1 << 2) | (2 << 4) | (3 << 6));
+                    nd.rank[oct] = nd.order[oct];  // the identity is its own inverse
+                }
                 std::memcpy(raw.data(), &nd, sizeof nd);
             }
             rc = dev_malloc(&d_wide, raw.size());

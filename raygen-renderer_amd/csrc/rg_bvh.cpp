@@ -429,6 +429,7 @@ namespace {
 // Packs a visit order (slots ord[0..W-1], first visited first) into a node.
 inline void set_order(Node4 &nd, int oct, const int *ord) {
     nd.order[oct] = (uint8_t)(ord[0] | (ord[1] << 2) | (ord[2] << 4) | (ord[3] << 6));
+    nd.rank[oct] = (uint8_t)(0 << (2 * ord[0]) | 1 << (2 * ord[1]) | 2 << (2 * ord[2]) | 3 << (2 * ord[3]));
 }
 inline void set_order(Node8 &nd, int oct, const int *ord) {
     uint32_t o = 0;

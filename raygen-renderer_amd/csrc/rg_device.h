@@ -105,7 +105,7 @@ struct WideTest<4> {
     static constexpr int kOrdBits = 2;
     __device__ __forceinline__ static void run(const void *nodes, uint32_t cur, int oct, float ox, float oy,
                                                float oz, float ix, float iy, float iz, float tmax, bool *ok,
-                                               float *tm, uint32_t *cref, uint32_t &ord) {
+                                               float *tm, uint32_t *cref, uint32_t &ord, uint32_t &rk) {
         const Node4 *nd = reinterpret_cast<const Node4 *>(nodes) + cur;
         const float4 lx = *reinterpret_cast<const float4 *>(nd->lox);
         const float4 ly = *reinterpret_cast<const float4 *>(nd->loy);
@@ -118,6 +118,8 @@ struct WideTest<4> {
         // load issues with the box loads
         const uint32_t ow = reinterpret_cast<const uint32_t *>(nd->order)[oct >> 2];
         ord = (ow >> (8 * (oct & 3))) & 0xFFu;
+        const uint32_t rw = reinterpret_cast<const uint32_t *>(nd->rank)[oct >> 2];
+        rk = (rw >> (8 * (oct & 3))) & 0xFFu;
         cref[0] = ch.x; cref[1] = ch.y; cref[2] = ch.z; cref[3] = ch.w;
         // every slot's slab test runs (an empty slot's zero box is masked
         // off): straight-line code instead of a branch per child
@@ -133,9 +135,12 @@ struct WideTest<8> {
     static constexpr int kOrdBits = 3;
     __device__ __forceinline__ static void run(const void *nodes, uint32_t cur, int oct, float ox, float oy,
                                                float oz, float ix, float iy, float iz, float tmax, bool *ok,
-                                               float *tm, uint32_t *cref, uint32_t &ord) {
+                                               float *tm, uint32_t *cref, uint32_t &ord, uint32_t &rk) {
         const Node8 *nd = reinterpret_cast<const Node8 *>(nodes) + cur;
         ord = nd->order[oct];
+        rk = 0;  // each slot's visit position
+#pragma unroll
+        for (int j = 0; j < 8; j++) rk |= (uint32_t)j << (3 * ((ord >> (3 * j)) & 7));
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const float4 lx = *reinterpret_cast<const float4 *>(nd->lox + 4 * h);
@@ -208,8 +213,8 @@ __device__ __forceinline__ void trace_closest_dev(const DevBvh &bv, float ox, fl
         if (!(cur & kLeafBit)) {
             bool ok[W];
             float tm[W];
-            uint32_t cref[W], ord;
-            WideTest<W>::run(bv.nodes, cur, oct, ox, oy, oz, ix, iy, iz, t, ok, tm, cref, ord);
+            uint32_t cref[W], ord, rk;
+            WideTest<W>::run(bv.nodes, cur, oct, ox, oy, oz, ix, iy, iz, t, ok, tm, cref, ord, rk);
             bool have = false;
             uint32_t next = 0;
             float next_t = 0.0f;
@@ -283,8 +288,8 @@ __device__ __forceinline__ bool trace_any_dev(const DevBvh &bv, float ox, float 
         if (!(cur & kLeafBit)) {
             bool ok[W];
             float tm[W];
-            uint32_t cref[W], ord;
-            WideTest<W>::run(bv.nodes, cur, oct, ox, oy, oz, ix, iy, iz, maxt, ok, tm, cref, ord);
+            uint32_t cref[W], ord, rk;
+            WideTest<W>::run(bv.nodes, cur, oct, ox, oy, oz, ix, iy, iz, maxt, ok, tm, cref, ord, rk);
             bool have = false;
             uint32_t next = 0;
 #pragma unroll
@@ -509,37 +514,62 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, 
             if (want) {
                 bool ok[kW];
                 float tm[kW];
-                uint32_t cref[kW], ord;
+                uint32_t cref[kW], ord, rk;
                 // the top levels from LDS, the rest from HBM (a flat load
                 // reaches either)
                 const void *base = cur < bv.top ? (const void *)treelet : bv.nodes;
-                WideTest<kW>::run(base, cur, oct, r.ox, r.oy, r.oz, ix, iy, iz, t, ok, tm, cref, ord);
-                bool have = false;
-                uint32_t next = 0;
-                float next_t = 0.0f;
+                WideTest<kW>::run(base, cur, oct, r.ox, r.oy, r.oz, ix, iy, iz, t, ok, tm, cref, ord, rk);
+                // Each passing child's rank in the octant's visit order
+                // fixes its stack position: the first is entered, the k-th
+                // (k >= 1) of c goes to sp + c - 1 - k, so the second ends on
+                // top, as pushing them one by one in reverse order leaves
+                // them.  Slots that push nothing write to position sp + c - 1
+                // (the new top, free), which keeps the writes unconditional.
+                uint32_t pos[kW], m = 0;
 #pragma unroll
-                for (int j = kW - 1; j >= 0; j--) {
-                    const int s = (ord >> (kOrdBits * j)) & (kW - 1);
-                    if (!ok[s]) continue;
-                    if (have) {
-                        if (sp < kLdsStack) {
-                            lc.ref[sp * kStride] = next;
-                            if (!kAny) lc.t[sp * kStride] = next_t;
-                        } else {
-                            stk_ref[sp - kLdsStack] = next;
-                            if (!kAny) stk_t[sp - kLdsStack] = next_t;
-                        }
-                        sp++;
-                    }
-                    next = cref[s];
-                    next_t = tm[s];
-                    have = true;
+                for (int i = 0; i < kW; i++) {
+                    pos[i] = (rk >> (kOrdBits * i)) & (kW - 1);
+                    m |= (uint32_t)ok[i] << pos[i];
                 }
-                if (have) {
+                if (m == 0) {
+                    cur = pop();
+                } else {
+                    const int top = sp + __popc(m) - 1;  // stack size after the pushes
+                    int e[kW];
+                    uint32_t next = 0;
+                    float next_t = 0.0f;
+#pragma unroll
+                    for (int i = 0; i < kW; i++) {
+                        const int rank = __popc(m & ((1u << pos[i]) - 1u));
+                        if (ok[i] && rank == 0) {
+                            next = cref[i];
+                            next_t = tm[i];
+                        }
+                        e[i] = ok[i] && rank > 0 ? top - rank : top;
+                    }
+                    // positions up to top stay within the stack: the tree's
+                    // deepest push sequence sized it (stack_need)
+                    if (top < kLdsStack) {
+#pragma unroll
+                        for (int i = 0; i < kW; i++) {
+                            lc.ref[e[i] * kStride] = cref[i];
+                            if (!kAny) lc.t[e[i] * kStride] = tm[i];
+                        }
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < kW; i++) {
+                            if (e[i] < kLdsStack) {
+                                lc.ref[e[i] * kStride] = cref[i];
+                                if (!kAny) lc.t[e[i] * kStride] = tm[i];
+                            } else {
+                                stk_ref[e[i] - kLdsStack] = cref[i];
+                                if (!kAny) stk_t[e[i] - kLdsStack] = tm[i];
+                            }
+                        }
+                    }
+                    sp = top;
                     cur = next;
                     cur_t = next_t;
-                } else {
-                    cur = pop();
                 }
             }
         }

@@ -22,13 +22,14 @@ namespace rg {
 //   0xFFFFFFFF   : empty slot
 // order[oct] lists the 4 slots (2 bits each, first visited in bits 0-1) in the
 // depth-first order the binary tree would visit them for a ray whose
-// direction-sign octant is oct (bit k set when 1/d_k < 0).
+// direction-sign octant is oct (bit k set when 1/d_k < 0); rank[oct] is its
+// inverse, each slot's visit position (2 bits per slot, slot 0 in bits 0-1).
 struct alignas(128) Node4 {
     float lox[4], loy[4], loz[4];
     float hix[4], hiy[4], hiz[4];
     uint32_t child[4];
     uint8_t order[8];
-    uint32_t pad[2];
+    uint8_t rank[8];
 };
 static_assert(sizeof(Node4) == 128, "Node4 must be 128 bytes");
 
