@@ -49,10 +49,11 @@ def cpu_model():
     return "unknown"
 
 
-def host_desc(threads):
-    return {"threads": threads, "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
-            "oracle_build": "gcc -O3 -ffp-contract=off -fno-fast-math (value); "
-                            "gcc -O3 -ffast-math -funroll-loops -march=x86-64-v3 (value_fast_math)"}
+def host_desc(threads, fast_math=False):
+    build = "gcc -O3 -ffp-contract=off -fno-fast-math"
+    if fast_math:
+        build += " (value); gcc -O3 -ffast-math -funroll-loops -march=x86-64-v3 (value_fast_math)"
+    return {"threads": threads, "host_cpus": os.cpu_count(), "cpu_model": cpu_model(), "oracle_build": build}
 
 
 def host_has_avx2_fma():
@@ -116,7 +117,7 @@ def cpu_baseline(v9, rays_np, seconds, threads):
         "sample_fast_math": f"first {fdone} rays, the same traversal built with the reference's flags "
                             f"(gcc -O3 -ffast-math -funroll-loops -march=x86-64-v3), {threads} threads",
         "bvh_build_s": round(build_s, 3),
-        "host": host_desc(threads),
+        "host": host_desc(threads, fast_math=fbusy > 0),
     }, pops / done, tests / done
 
 
