@@ -333,15 +333,23 @@ struct PsRay {
 
 constexpr uint32_t kNoNode = 0xFFFFFFFEu;
 
+// Lanes of a wave that must hold a postponed leaf before the wave intersects
+// leaves (unless every lane is ready sooner).  Waiting for all 64 left the
+// lanes that had found a leaf idle behind the slowest one: measured on the
+// 16M-ray closest-hit benchmark 3.37 ms at 64, 3.43 at 48, 3.18 at 40, 2.62 at
+// 32, 2.41 at 24, 2.39 at 16, 2.52 at 8 (any hit 3.00 -> 2.11 ms at 16).
+constexpr int kLeafBatch = 16;
+
 // Persistent traversal with leaf postponement and lane refill (after Aila
 // & Laine, HPG 2009), shared by every kernel that traces a batch of rays.
 // Each wave owns a contiguous chunk of the n ray indices (and, with work
 // sharing, grabs more); a lane whose ray is done takes the next index of the
 // chunk, so a wave does not idle behind its
 // longest ray.  Inner nodes are traversed until every lane of the wave holds
-// a leaf to intersect (the first leaf a lane reaches is postponed and the
-// lane keeps descending with its current t), then all lanes intersect their
-// leaves together.
+// a leaf to intersect, or kLeafBatch lanes do (the first leaf a lane reaches
+// is postponed and the lane keeps descending with its current t), then the
+// lanes holding one intersect their leaves together; the others carry on
+// descending after that.
 //
 // The hits are those of trace_closest_dev, exact-t ties included, whatever
 // rays share the wave: descending with the stale, larger t tests boxes
@@ -510,7 +518,11 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, 
             }
             const bool want = active && cur != kNoNode && !(cur & kLeafBit);
             const bool ready = !active || leaf != kNoNode || cur == kNoNode;
-            if (!__any(want) || __all(ready)) break;
+            // to the leaf phase when no lane can descend, every lane is
+            // ready, or kLeafBatch lanes hold a leaf (at least one, so every
+            // break leaves the leaf phase work to do)
+            const int holding = __popcll(__ballot(active && leaf != kNoNode));
+            if (!__any(want) || __all(ready) || holding >= kLeafBatch) break;
             if (want) {
                 bool ok[kW];
                 float tm[kW];
