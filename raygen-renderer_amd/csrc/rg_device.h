@@ -519,10 +519,11 @@ __device__ __forceinline__ void trace_ps_loop(const DevBvh &bv, const Src &src, 
             const bool want = active && cur != kNoNode && !(cur & kLeafBit);
             const bool ready = !active || leaf != kNoNode || cur == kNoNode;
             // to the leaf phase when no lane can descend, every lane is
-            // ready, or kLeafBatch lanes hold a leaf (at least one, so every
-            // break leaves the leaf phase work to do)
+            // ready, or bv.leaf_batch lanes (kLeafBatch, or 64 for a tree
+            // held whole in LDS) hold a leaf (at least one, so every break
+            // leaves the leaf phase work to do)
             const int holding = __popcll(__ballot(active && leaf != kNoNode));
-            if (!__any(want) || __all(ready) || holding >= kLeafBatch) break;
+            if (!__any(want) || __all(ready) || holding >= bv.leaf_batch) break;
             if (want) {
                 bool ok[kW];
                 float tm[kW];

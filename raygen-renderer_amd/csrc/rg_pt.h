@@ -22,6 +22,7 @@ struct DevBvh {
     int has_tree;
     int width;          // 4 or 8
     uint32_t top;       // leading nodes (the top levels) the persistent traversal copies to LDS
+    int leaf_batch = 16;  // lanes holding a leaf that send a wave to its leaf phase
 };
 
 // One SceneObject's material (material.h:20-70); same layout as rg_object.
